@@ -1,0 +1,226 @@
+/*
+ * opengraph_amd.h -- drop-in C ABI of the OpenGraph PageRank-over-CSR path,
+ * implemented on MI355X (gfx950) by libopengraph_amd.so.
+ *
+ * Every type below has the exact field order and x86-64 layout of the
+ * reference type it replaces (compiled with WEIGHTED=0, DIRECTED=1), so a
+ * caller built against the reference headers can link this library instead
+ * of the reference objects for this path.  Every function keeps the name,
+ * signature, argument meaning and ownership rules of the reference function
+ * it replaces; the comment on each one cites it (paths relative to the
+ * reference's 00_graph_bench/ tree).
+ *
+ * Errors: the reference has no error returns (it exits through err()/exit()
+ * on failure, e.g. edgeList.c:354-374).  The drop-in entry points keep that
+ * convention: on a HIP failure or an unsupported option they print the cause
+ * to stderr and exit(EXIT_FAILURE).  They never compute on the CPU instead.
+ * The ogpu_* entry points in opengraph_amd_ext.h return status codes.
+ */
+#ifndef OPENGRAPH_AMD_H
+#define OPENGRAPH_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* graphConfig.h:7-8 -- the configuration this ABI is laid out for. */
+#define WEIGHTED 0
+#define DIRECTED 1
+
+/* mt19937.h:59-63 -- embedded in struct Arguments; must keep its size. */
+typedef struct mt19937_stateStruct
+{
+    unsigned long mt[624];
+    int mti;
+} mt19937state;
+
+/* graphConfig.h:11-44 */
+struct Arguments
+{
+    int wflag;
+    int xflag;
+    int sflag;
+    int Sflag;
+    int dflag;
+    uint32_t binSize;
+    uint32_t verbosity;
+    uint32_t iterations;
+    uint32_t trials;
+    double epsilon;
+    int source;
+    uint32_t algorithm;
+    uint32_t datastructure;
+    uint32_t pushpull;
+    uint32_t sort;
+    uint32_t lmode;
+    uint32_t lmode_l2;
+    uint32_t lmode_l3;
+    uint32_t mmode;
+    uint32_t symmetric;
+    uint32_t weighted;
+    uint32_t delta;
+    int pre_numThreads;
+    int algo_numThreads;
+    int ker_numThreads;
+    char *fnameb;
+    char *fnamel;
+    uint32_t fnameb_format;
+    uint32_t convert_format;
+    mt19937state mt19937var;
+    uint32_t cache_size;
+};
+
+/* vertex.h:13-18 */
+struct Vertex
+{
+    uint32_t *out_degree;
+    uint32_t *in_degree;
+    uint32_t *edges_idx;
+};
+
+/* edgeList.h:8-23 */
+struct EdgeList
+{
+    uint32_t num_edges;
+    uint32_t num_vertices;
+    uint32_t avg_degree;
+    uint32_t *edges_array_src;
+    uint32_t *edges_array_dest;
+    uint32_t *mask_array;
+    uint32_t *label_array;
+    uint32_t *inverse_label_array;
+};
+
+/* graphCSR.h:12-29 */
+struct GraphCSR
+{
+    uint32_t num_edges;
+    uint32_t num_vertices;
+    uint32_t avg_degree;
+    struct Vertex *vertices;
+    struct EdgeList *sorted_edges_array;
+    struct Vertex *inverse_vertices;
+    struct EdgeList *inverse_sorted_edges_array;
+};
+
+/* pageRank.h:19-30 */
+struct PageRankStats
+{
+    float damp;
+    float base_pr;
+    uint32_t iterations;
+    uint32_t num_vertices;
+    uint32_t activeVertices;
+    uint32_t *realRanks;
+    float *pageRanks;
+    double time_total;
+    double error_total;
+};
+
+/* ------------------------------------------------------------------------
+ * PageRank on CSR.  All variants run on the GPU.  The graph is borrowed and
+ * never modified; its device copy is cached by pointer and array identity
+ * and rebuilt when any of them changes.  The returned stats and both of its
+ * arrays are malloc-family allocations owned by the caller (free() them, or
+ * call freePageRankStats).
+ * ---------------------------------------------------------------------- */
+
+/* pageRank.c:871-936 (decl pageRank.h:69): dispatch on arguments->pushpull. */
+struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:939-1062: pull, fp32 ranks, fp64 error sum (-p 0). */
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1064-1186: push with fp32 atomics (-p 1). */
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1190-1324: pull, uint64 fixed point with 32 fraction bits (-p 2). */
+struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1326-1459: pull, 16 fraction bits in uint32 contributions (-p 9). */
+struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1461-1591: pull, 14 fraction bits in uint16 contributions (-p 10). */
+struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1593-1724: pull, 6 fraction bits in uint8 contributions (-p 11). */
+struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1726-1849: push, uint64 fixed point atomics (-p 3). */
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1853-1988: pull, per-iteration 32-bit quantized contributions (-p 4). */
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:1990-2128: pull, 16-bit quantized contributions (-p 12). */
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2130-2270: pull, 8-bit quantized contributions (-p 13). */
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2273-2401: push, 32-bit quantized contributions with zero point (-p 5). */
+struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2405-2564: data-driven pull over byte worklists (-p 6). */
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2566-2720: data-driven residual push (-p 7). */
+struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2723-2896: data-driven pull-push hybrid (-p 8). */
+struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+
+/* pageRank.c:47-75: damp 0.85f, base_pr 0.15f, pageRanks[v]=0.15f, realRanks[v]=v. */
+struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph);
+/* pageRank.c:167-177 */
+void freePageRankStats(struct PageRankStats *stats);
+
+/* ------------------------------------------------------------------------
+ * Edge list -> CSR builder.  Sorting and degree mapping run on the GPU; the
+ * host structures produced are laid out and allocated (128-byte aligned
+ * posix_memalign, myMalloc.c:24-67) exactly as the reference's, so the
+ * reference's own free functions also release them.
+ * ---------------------------------------------------------------------- */
+
+/* graphCSR.c:164-248: read arguments->fnameb (.bin), sort with
+ * arguments->sort, map forward and inverse CSR.  Reordering (-l/-L/-O),
+ * masking (-M) and duplicate removal (-k) are not supported and exit. */
+struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments);
+/* graphCSR.c:107-134 */
+struct GraphCSR *graphCSRNew(uint32_t V, uint32_t E, uint8_t inverse);
+/* graphCSR.c:137-161: attach a sorted list and map its vertices. */
+struct GraphCSR *graphCSRAssignEdgeList(struct GraphCSR *graphCSR, struct EdgeList *edgeList, uint8_t inverse);
+/* graphCSR.c:42-64 */
+void graphCSRFree(struct GraphCSR *graphCSR);
+/* graphCSR.c:66-104 */
+void graphCSRPrint(struct GraphCSR *graphCSR);
+/* vertex.c:204-325: out_degree / edges_idx of the (inverse) list on the GPU;
+ * edges_idx[v] is 0 for a vertex with no edges, as in the reference. */
+struct GraphCSR *mapVerticesWithInOutDegree(struct GraphCSR *graph, uint8_t inverse);
+/* vertex.c:26-49 */
+struct Vertex *newVertexArray(uint32_t num_vertices);
+/* vertex.c:406-420 */
+void freeVertexArray(struct Vertex *vertices);
+
+/* edgeList.c:336-594: mmap a .bin/.wbin file; num_edges = size / record;
+ * num_vertices = max id + 1.  Weights (-w) are skipped as the reference
+ * does with WEIGHTED=0. */
+struct EdgeList *readEdgeListsbin(const char *fname, uint8_t inverse, uint32_t symmetric, uint32_t weighted);
+/* edgeList.c:597-745: copy (inverse=1: swap src/dst) of an in-memory list. */
+struct EdgeList *readEdgeListsMem(struct EdgeList *edgeListmem, uint8_t inverse, uint32_t symmetric, uint32_t weighted);
+/* edgeList.c:98-135 */
+struct EdgeList *newEdgeList(uint32_t num_edges);
+/* edgeList.c:213-239 */
+void freeEdgeList(struct EdgeList *edgeList);
+
+/* sortRun.c:26-69: 0 radix by src, 1 radix by src then dst, 2/3 the same
+ * orders by counting sort.  All four are stable, so 0 == 2 and 1 == 3. */
+struct EdgeList *sortRunAlgorithms(struct EdgeList *edgeList, uint32_t sort);
+/* radixsort.c:231-278 */
+struct EdgeList *radixSortEdgesBySource(struct EdgeList *edgeList);
+/* radixsort.c:282-335 */
+struct EdgeList *radixSortEdgesBySourceAndDestination(struct EdgeList *edgeList);
+/* countsort.c:28-141 */
+struct EdgeList *countSortEdgesBySource(struct EdgeList *edgeList);
+/* countsort.c:262-290 */
+struct EdgeList *countSortEdgesBySourceAndDestination(struct EdgeList *edgeList);
+
+/* reorder.c:165-221: caller-side post-sort of vertex labels by
+ * FloatToFixed32SORT(rank) (graphRun.c:1076).  Returns labels, reordered
+ * in place, ascending by fixed-point rank, stable. */
+uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t num_vertices);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENGRAPH_AMD_H */

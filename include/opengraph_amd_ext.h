@@ -1,0 +1,124 @@
+/*
+ * opengraph_amd_ext.h -- device-resident entry points of libopengraph_amd.
+ *
+ * None of these has a counterpart in the reference: they serve callers that
+ * keep graphs in HBM across calls (benchmarks, batched and learned-start
+ * PageRank, services).  They take plain pointers and sizes, return 0 on
+ * success and -1 on failure with the cause in ogpu_last_error(), and never
+ * compute on the CPU.  "d_" arguments are device pointers (from ogpu_malloc
+ * or any HIP allocation on the same device).
+ */
+#ifndef OPENGRAPH_AMD_EXT_H
+#define OPENGRAPH_AMD_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "opengraph_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ogpu_graph ogpu_graph;   /* device forward + inverse CSR */
+typedef struct ogpu_pr ogpu_pr;         /* a prepared PageRank run on a graph */
+
+/* ---- device, memory ---- */
+int ogpu_init(int device);
+const char *ogpu_last_error(void);
+int ogpu_device_info(char *name, size_t name_len, int *num_cus, size_t *total_mem);
+int ogpu_malloc(void **d_ptr, size_t bytes);
+int ogpu_free(void *d_ptr);
+int ogpu_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
+int ogpu_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
+int ogpu_synchronize(void);
+
+/* ---- synthetic inputs (the reference reads .bin files only) ----
+ * R-MAT: edge e draws one quadrant per level from counter e*64+l of the
+ * SplitMix64 stream (seed, 0) with probabilities a, b, c, 1-a-b-c; vertex
+ * ids are then permuted by a keyed Feistel bijection of [0, 2^scale) when
+ * permute != 0.  Duplicates and self-loops are kept.  Erdos-Renyi: src and
+ * dst uniform in [0, n).  The host and device versions produce identical
+ * arrays. */
+int ogpu_rmat_device(uint32_t scale, uint64_t num_edges, double a, double b, double c,
+                     uint64_t seed, int permute, uint32_t *d_src, uint32_t *d_dst);
+int ogpu_rmat_host(uint32_t scale, uint64_t num_edges, double a, double b, double c, uint64_t seed,
+                   int permute, uint32_t *src, uint32_t *dst);
+int ogpu_er_device(uint32_t num_vertices, uint64_t num_edges, uint64_t seed, uint32_t *d_src,
+                   uint32_t *d_dst);
+int ogpu_er_host(uint32_t num_vertices, uint64_t num_edges, uint64_t seed, uint32_t *src,
+                 uint32_t *dst);
+/* Write (src,dst) u32 pairs in the reference's raw .bin layout (edgeList.c:383-407). */
+int ogpu_write_bin(const char *path, const uint32_t *src, const uint32_t *dst, uint64_t num_edges);
+
+/* ---- device CSR ---- */
+/* Build forward + inverse CSR from device edge arrays (graphCSR.c:164-248
+ * semantics; sort_mode as -o).  Inputs are not modified. */
+int ogpu_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
+                     uint32_t sort_mode, ogpu_graph **out);
+/* Upload host arrays in the order f_src, f_dst, f_out_degree, f_in_degree,
+ * f_edges_idx, i_src, i_dst, i_out_degree, i_in_degree, i_edges_idx. */
+int ogpu_graph_from_arrays(uint32_t num_vertices, uint32_t num_edges,
+                           const uint32_t *const host_arrays[10], ogpu_graph **out);
+int ogpu_graph_info(const ogpu_graph *g, uint32_t *num_vertices, uint32_t *num_edges);
+enum
+{
+    OGPU_F_SRC = 0, OGPU_F_DST, OGPU_F_OUT_DEGREE, OGPU_F_IN_DEGREE, OGPU_F_EDGES_IDX,
+    OGPU_I_SRC, OGPU_I_DST, OGPU_I_OUT_DEGREE, OGPU_I_IN_DEGREE, OGPU_I_EDGES_IDX
+};
+/* Copy one CSR array to host memory (E or N entries). */
+int ogpu_graph_array(const ogpu_graph *g, uint32_t which, uint32_t *host_out);
+void ogpu_graph_free(ogpu_graph *g);
+
+/* ---- PageRank on a device graph ---- */
+typedef struct
+{
+    uint32_t variant;       /* the reference's -p code, 0..13 */
+    uint32_t max_iter;      /* arguments->iterations */
+    double epsilon;         /* arguments->epsilon */
+    uint32_t stop_rule;     /* 0: activeVertices == 0 (reference); 1: error_total < epsilon */
+    uint32_t init_mode;     /* 0: 0.15 (reference); 1: uniform 1.0 (= 1/N normalized);
+                               2: caller's vector (unnormalized, old ids) */
+    uint32_t relabel;       /* 1: degree-ordered rows and columns (default); 0: hubs first only */
+    uint32_t hub_threshold; /* rows with more in-edges are split into chunks; 0: default */
+    uint32_t reserved;
+} ogpu_pr_options;
+
+typedef struct
+{
+    uint32_t iterations;    /* stats->iterations (the reference's off-by-one rule) */
+    uint32_t rows;          /* iterations executed */
+    uint32_t active;        /* last iteration's activeVertices */
+    uint32_t pad;
+    double error_total;     /* last iteration's error_total */
+    double pr_sum;          /* sum of the normalized ranks */
+    double seconds_total;   /* wall time from init to normalized ranks on device */
+    double seconds_plan;    /* one-time plan build */
+} ogpu_pr_result;
+
+void ogpu_pr_default_options(ogpu_pr_options *opt);
+int ogpu_pr_create(ogpu_graph *g, const ogpu_pr_options *opt, ogpu_pr **out);
+/* Set the start vector for init_mode 2 (host pointer, N floats, old ids). */
+int ogpu_pr_set_init(ogpu_pr *pr, const float *init);
+/* Run to the stop rule or max_iter and normalize; ranks stay on device. */
+int ogpu_pr_run(ogpu_pr *pr, ogpu_pr_result *res);
+/* Benchmark helpers: reset the state, then enqueue n iterations with the
+ * stop rule disabled (every iteration does full work); optionally time the
+ * dominant kernel of each iteration with HIP events on the library stream. */
+int ogpu_pr_reset(ogpu_pr *pr);
+int ogpu_pr_enqueue(ogpu_pr *pr, uint32_t n, int time_kernels);
+int ogpu_pr_kernel_ms(ogpu_pr *pr, double *total_ms, uint32_t *count);
+/* Normalized ranks (old ids) to host memory. */
+int ogpu_pr_ranks(ogpu_pr *pr, float *host_out);
+/* Per-iteration activeVertices / error_total rows of the last run. */
+int ogpu_pr_rows(ogpu_pr *pr, uint32_t *active, double *error_total, uint32_t max_rows);
+/* Algorithmic HBM bytes of one iteration of this variant (pull fp32:
+ * 8E + 20N). */
+double ogpu_pr_bytes_per_iteration(const ogpu_pr *pr);
+void ogpu_pr_free(ogpu_pr *pr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

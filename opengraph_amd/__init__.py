@@ -1,0 +1,454 @@
+"""Python host mirror of libopengraph_amd's C ABI (ctypes, no torch).
+
+The library is the product: PageRank over CSR and the edge-list -> CSR
+builder of OpenGraph, on MI355X (gfx950).  This module only binds its
+``extern "C"`` entry points (include/opengraph_amd.h, opengraph_amd_ext.h)
+so tests, the benchmark and Python callers can reach them:
+
+* ``Arguments``, ``Vertex``, ``EdgeList``, ``GraphCSR``, ``PageRankStats`` are
+  the reference's structs (graphConfig.h:11-44, vertex.h:13-18,
+  edgeList.h:8-23, graphCSR.h:12-29, pageRank.h:19-30) as ctypes types, so
+  ``pagerank_graph_csr`` calls the drop-in ``pageRankGraphCSR`` exactly as
+  the reference's driver does (graphRun.c:1035-1040).
+* ``DeviceArray``, ``Graph`` and ``PageRank`` wrap the device-resident
+  ``ogpu_*`` API.
+
+Loading fails loudly when the shared library has not been built
+(``python -c "import __graft_entry__ as g; g.build()"`` or
+``make -C opengraph_amd``); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopengraph_amd.so")
+
+u32 = C.c_uint32
+u32p = C.POINTER(C.c_uint32)
+
+
+class MT19937State(C.Structure):
+    """mt19937.h:59-63."""
+    _fields_ = [("mt", C.c_ulong * 624), ("mti", C.c_int)]
+
+
+class Arguments(C.Structure):
+    """graphConfig.h:11-44."""
+    _fields_ = [
+        ("wflag", C.c_int), ("xflag", C.c_int), ("sflag", C.c_int), ("Sflag", C.c_int),
+        ("dflag", C.c_int), ("binSize", u32), ("verbosity", u32), ("iterations", u32),
+        ("trials", u32), ("epsilon", C.c_double), ("source", C.c_int), ("algorithm", u32),
+        ("datastructure", u32), ("pushpull", u32), ("sort", u32), ("lmode", u32),
+        ("lmode_l2", u32), ("lmode_l3", u32), ("mmode", u32), ("symmetric", u32),
+        ("weighted", u32), ("delta", u32), ("pre_numThreads", C.c_int),
+        ("algo_numThreads", C.c_int), ("ker_numThreads", C.c_int), ("fnameb", C.c_char_p),
+        ("fnamel", C.c_char_p), ("fnameb_format", u32), ("convert_format", u32),
+        ("mt19937var", MT19937State), ("cache_size", u32),
+    ]
+
+
+class Vertex(C.Structure):
+    """vertex.h:13-18."""
+    _fields_ = [("out_degree", u32p), ("in_degree", u32p), ("edges_idx", u32p)]
+
+
+class EdgeList(C.Structure):
+    """edgeList.h:8-23."""
+    _fields_ = [
+        ("num_edges", u32), ("num_vertices", u32), ("avg_degree", u32),
+        ("edges_array_src", u32p), ("edges_array_dest", u32p), ("mask_array", u32p),
+        ("label_array", u32p), ("inverse_label_array", u32p),
+    ]
+
+
+class GraphCSR(C.Structure):
+    """graphCSR.h:12-29 (WEIGHTED 0, DIRECTED 1)."""
+    _fields_ = [
+        ("num_edges", u32), ("num_vertices", u32), ("avg_degree", u32),
+        ("vertices", C.POINTER(Vertex)), ("sorted_edges_array", C.POINTER(EdgeList)),
+        ("inverse_vertices", C.POINTER(Vertex)),
+        ("inverse_sorted_edges_array", C.POINTER(EdgeList)),
+    ]
+
+
+class PageRankStats(C.Structure):
+    """pageRank.h:19-30."""
+    _fields_ = [
+        ("damp", C.c_float), ("base_pr", C.c_float), ("iterations", u32), ("num_vertices", u32),
+        ("activeVertices", u32), ("realRanks", u32p), ("pageRanks", C.POINTER(C.c_float)),
+        ("time_total", C.c_double), ("error_total", C.c_double),
+    ]
+
+
+class PrOptions(C.Structure):
+    """ogpu_pr_options (opengraph_amd_ext.h)."""
+    _fields_ = [
+        ("variant", u32), ("max_iter", u32), ("epsilon", C.c_double), ("stop_rule", u32),
+        ("init_mode", u32), ("relabel", u32), ("hub_threshold", u32), ("reserved", u32),
+    ]
+
+
+class PrResult(C.Structure):
+    """ogpu_pr_result (opengraph_amd_ext.h)."""
+    _fields_ = [
+        ("iterations", u32), ("rows", u32), ("active", u32), ("pad", u32),
+        ("error_total", C.c_double), ("pr_sum", C.c_double), ("seconds_total", C.c_double),
+        ("seconds_plan", C.c_double),
+    ]
+
+
+# Every entry point of include/opengraph_amd.h and include/opengraph_amd_ext.h:
+# name -> (restype, argtypes).
+_VP = C.c_void_p
+_PR_FN = (C.POINTER(PageRankStats), [C.POINTER(Arguments), C.POINTER(GraphCSR)])
+SIGNATURES = {
+    # opengraph_amd.h
+    "pageRankGraphCSR": _PR_FN, "pageRankPullGraphCSR": _PR_FN, "pageRankPushGraphCSR": _PR_FN,
+    "pageRankPullFixedPoint64BitGraphCSR": _PR_FN, "pageRankPullFixedPoint32BitGraphCSR": _PR_FN,
+    "pageRankPullFixedPoint16BitGraphCSR": _PR_FN, "pageRankPullFixedPoint8BitGraphCSR": _PR_FN,
+    "pageRankPushFixedPointGraphCSR": _PR_FN, "pageRankPullQuant32BitGraphCSR": _PR_FN,
+    "pageRankPullQuant16BitGraphCSR": _PR_FN, "pageRankPullQuant8BitGraphCSR": _PR_FN,
+    "pageRankPushQuantGraphCSR": _PR_FN, "pageRankDataDrivenPullGraphCSR": _PR_FN,
+    "pageRankDataDrivenPushGraphCSR": _PR_FN, "pageRankDataDrivenPullPushGraphCSR": _PR_FN,
+    "newPageRankStatsGraphCSR": (C.POINTER(PageRankStats), [C.POINTER(GraphCSR)]),
+    "freePageRankStats": (None, [C.POINTER(PageRankStats)]),
+    "graphCSRPreProcessingStep": (C.POINTER(GraphCSR), [C.POINTER(Arguments)]),
+    "graphCSRNew": (C.POINTER(GraphCSR), [u32, u32, C.c_uint8]),
+    "graphCSRAssignEdgeList": (C.POINTER(GraphCSR), [C.POINTER(GraphCSR), C.POINTER(EdgeList), C.c_uint8]),
+    "graphCSRFree": (None, [C.POINTER(GraphCSR)]),
+    "graphCSRPrint": (None, [C.POINTER(GraphCSR)]),
+    "mapVerticesWithInOutDegree": (C.POINTER(GraphCSR), [C.POINTER(GraphCSR), C.c_uint8]),
+    "newVertexArray": (C.POINTER(Vertex), [u32]),
+    "freeVertexArray": (None, [C.POINTER(Vertex)]),
+    "readEdgeListsbin": (C.POINTER(EdgeList), [C.c_char_p, C.c_uint8, u32, u32]),
+    "readEdgeListsMem": (C.POINTER(EdgeList), [C.POINTER(EdgeList), C.c_uint8, u32, u32]),
+    "newEdgeList": (C.POINTER(EdgeList), [u32]),
+    "freeEdgeList": (None, [C.POINTER(EdgeList)]),
+    "sortRunAlgorithms": (C.POINTER(EdgeList), [C.POINTER(EdgeList), u32]),
+    "radixSortEdgesBySource": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
+    "radixSortEdgesBySourceAndDestination": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
+    "countSortEdgesBySource": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
+    "countSortEdgesBySourceAndDestination": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
+    "radixSortEdgesByPageRank": (u32p, [C.POINTER(C.c_float), u32p, u32]),
+    # opengraph_amd_ext.h
+    "ogpu_init": (C.c_int, [C.c_int]),
+    "ogpu_last_error": (C.c_char_p, []),
+    "ogpu_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "ogpu_malloc": (C.c_int, [C.POINTER(_VP), C.c_size_t]),
+    "ogpu_free": (C.c_int, [_VP]),
+    "ogpu_memcpy_h2d": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "ogpu_memcpy_d2h": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "ogpu_synchronize": (C.c_int, []),
+    "ogpu_rmat_device": (C.c_int, [u32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, _VP, _VP]),
+    "ogpu_rmat_host": (C.c_int, [u32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, _VP, _VP]),
+    "ogpu_er_device": (C.c_int, [u32, C.c_uint64, C.c_uint64, _VP, _VP]),
+    "ogpu_er_host": (C.c_int, [u32, C.c_uint64, C.c_uint64, _VP, _VP]),
+    "ogpu_write_bin": (C.c_int, [C.c_char_p, _VP, _VP, C.c_uint64]),
+    "ogpu_graph_build": (C.c_int, [_VP, _VP, u32, u32, C.POINTER(_VP)]),
+    "ogpu_graph_from_arrays": (C.c_int, [u32, u32, C.POINTER(_VP), C.POINTER(_VP)]),
+    "ogpu_graph_info": (C.c_int, [_VP, u32p, u32p]),
+    "ogpu_graph_array": (C.c_int, [_VP, u32, _VP]),
+    "ogpu_graph_free": (None, [_VP]),
+    "ogpu_pr_default_options": (None, [C.POINTER(PrOptions)]),
+    "ogpu_pr_create": (C.c_int, [_VP, C.POINTER(PrOptions), C.POINTER(_VP)]),
+    "ogpu_pr_set_init": (C.c_int, [_VP, _VP]),
+    "ogpu_pr_run": (C.c_int, [_VP, C.POINTER(PrResult)]),
+    "ogpu_pr_reset": (C.c_int, [_VP]),
+    "ogpu_pr_enqueue": (C.c_int, [_VP, u32, C.c_int]),
+    "ogpu_pr_kernel_ms": (C.c_int, [_VP, C.POINTER(C.c_double), u32p]),
+    "ogpu_pr_ranks": (C.c_int, [_VP, _VP]),
+    "ogpu_pr_rows": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
+    "ogpu_pr_bytes_per_iteration": (C.c_double, [_VP]),
+    "ogpu_pr_free": (None, [_VP]),
+}
+
+GRAPH_ARRAYS = ["f_src", "f_dst", "f_out_degree", "f_in_degree", "f_edges_idx",
+                "i_src", "i_dst", "i_out_degree", "i_in_degree", "i_edges_idx"]
+
+_lib = None
+
+
+class OgpuError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libopengraph_amd.so and declare every entry point (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OgpuError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C opengraph_amd")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise OgpuError(load().ogpu_last_error().decode())
+
+
+def init(device: int = 0):
+    check(load().ogpu_init(device))
+
+
+def device_info():
+    lib = load()
+    name = C.create_string_buffer(256)
+    cus = C.c_int()
+    mem = C.c_size_t()
+    check(lib.ogpu_device_info(name, 256, C.byref(cus), C.byref(mem)))
+    return name.value.decode(), cus.value, mem.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A device allocation owned by Python (ogpu_malloc / ogpu_free)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(load().ogpu_malloc(C.byref(self.ptr), max(self.nbytes, 1)))
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            check(load().ogpu_memcpy_h2d(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def to_numpy(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(load().ogpu_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().ogpu_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rmat_host(scale, num_edges, a=0.57, b=0.19, c=0.19, seed=1, permute=True):
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    check(load().ogpu_rmat_host(scale, num_edges, a, b, c, seed, int(permute), _ptr(src), _ptr(dst)))
+    return src, dst
+
+
+def er_host(num_vertices, num_edges, seed):
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    check(load().ogpu_er_host(num_vertices, num_edges, seed, _ptr(src), _ptr(dst)))
+    return src, dst
+
+
+def rmat_device(scale, num_edges, a=0.57, b=0.19, c=0.19, seed=1, permute=True):
+    src = DeviceArray(4 * num_edges)
+    dst = DeviceArray(4 * num_edges)
+    check(load().ogpu_rmat_device(scale, num_edges, a, b, c, seed, int(permute), src.ptr, dst.ptr))
+    return src, dst
+
+
+def er_device(num_vertices, num_edges, seed):
+    src = DeviceArray(4 * num_edges)
+    dst = DeviceArray(4 * num_edges)
+    check(load().ogpu_er_device(num_vertices, num_edges, seed, src.ptr, dst.ptr))
+    return src, dst
+
+
+def write_bin(path, src, dst):
+    check(load().ogpu_write_bin(path.encode(), _ptr(src), _ptr(dst), len(src)))
+
+
+class Graph:
+    """Device forward + inverse CSR (ogpu_graph)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        n, e = u32(), u32()
+        check(load().ogpu_graph_info(self.h, C.byref(n), C.byref(e)))
+        self.num_vertices, self.num_edges = n.value, e.value
+
+    @classmethod
+    def build(cls, d_src: DeviceArray, d_dst: DeviceArray, num_edges: int, sort_mode: int = 0):
+        h = C.c_void_p()
+        check(load().ogpu_graph_build(d_src.ptr, d_dst.ptr, num_edges, sort_mode, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0):
+        ds, dd = DeviceArray.from_numpy(src.astype(np.uint32)), DeviceArray.from_numpy(dst.astype(np.uint32))
+        g = cls.build(ds, dd, len(src), sort_mode)
+        ds.free()
+        dd.free()
+        return g
+
+    @classmethod
+    def from_arrays(cls, arrays: dict):
+        """arrays: the 10 CSR arrays named as GRAPH_ARRAYS (numpy uint32)."""
+        keep = [np.ascontiguousarray(arrays[k], dtype=np.uint32) for k in GRAPH_ARRAYS]
+        ptrs = (C.c_void_p * 10)(*[k.ctypes.data for k in keep])
+        h = C.c_void_p()
+        n = len(keep[2])
+        e = len(keep[0])
+        check(load().ogpu_graph_from_arrays(n, e, ptrs, C.byref(h)))
+        return cls(h)
+
+    def array(self, name: str) -> np.ndarray:
+        i = GRAPH_ARRAYS.index(name)
+        n = self.num_edges if name in ("f_src", "f_dst", "i_src", "i_dst") else self.num_vertices
+        out = np.empty(n, np.uint32)
+        check(load().ogpu_graph_array(self.h, i, _ptr(out)))
+        return out
+
+    def arrays(self) -> dict:
+        return {k: self.array(k) for k in GRAPH_ARRAYS}
+
+    def free(self):
+        if self.h:
+            load().ogpu_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PageRank:
+    """A prepared PageRank run on a device graph (ogpu_pr)."""
+
+    def __init__(self, graph: Graph, variant=0, max_iter=20, epsilon=1e-4, stop_rule=0,
+                 init_mode=0, relabel=1, hub_threshold=0, init=None):
+        lib = load()
+        o = PrOptions()
+        lib.ogpu_pr_default_options(C.byref(o))
+        o.variant, o.max_iter, o.epsilon = variant, max_iter, epsilon
+        o.stop_rule, o.init_mode, o.relabel, o.hub_threshold = stop_rule, init_mode, relabel, hub_threshold
+        self.graph = graph
+        self.h = C.c_void_p()
+        check(lib.ogpu_pr_create(graph.h, C.byref(o), C.byref(self.h)))
+        if init is not None:
+            self.set_init(init)
+
+    def set_init(self, init: np.ndarray):
+        a = np.ascontiguousarray(init, dtype=np.float32)
+        check(load().ogpu_pr_set_init(self.h, _ptr(a)))
+
+    def run(self) -> PrResult:
+        r = PrResult()
+        check(load().ogpu_pr_run(self.h, C.byref(r)))
+        return r
+
+    def ranks(self) -> np.ndarray:
+        out = np.empty(self.graph.num_vertices, np.float32)
+        check(load().ogpu_pr_ranks(self.h, _ptr(out)))
+        return out
+
+    def rows(self, max_rows=100000):
+        act = np.zeros(max_rows, np.uint32)
+        err = np.zeros(max_rows, np.float64)
+        n = load().ogpu_pr_rows(self.h, act.ctypes.data_as(u32p),
+                                err.ctypes.data_as(C.POINTER(C.c_double)), max_rows)
+        return act[:n], err[:n]
+
+    def reset(self):
+        check(load().ogpu_pr_reset(self.h))
+
+    def enqueue(self, n, time_kernels=False):
+        check(load().ogpu_pr_enqueue(self.h, n, int(time_kernels)))
+
+    def kernel_ms(self):
+        t, c = C.c_double(), u32()
+        check(load().ogpu_pr_kernel_ms(self.h, C.byref(t), C.byref(c)))
+        return t.value, c.value
+
+    def bytes_per_iteration(self):
+        return load().ogpu_pr_bytes_per_iteration(self.h)
+
+    def free(self):
+        if self.h:
+            load().ogpu_pr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(load().ogpu_synchronize())
+
+
+# --------------------------------------------------------------------------
+# Drop-in calls through the reference's structs.
+# --------------------------------------------------------------------------
+
+class HostGraphCSR:
+    """A reference-layout GraphCSR built from numpy arrays (kept alive here)."""
+
+    def __init__(self, arrays: dict):
+        self._keep = {k: np.ascontiguousarray(arrays[k], dtype=np.uint32) for k in GRAPH_ARRAYS}
+        k = self._keep
+        n = len(k["f_out_degree"])
+        e = len(k["f_src"])
+
+        def p(a):
+            return a.ctypes.data_as(u32p)
+
+        self.fv = Vertex(p(k["f_out_degree"]), p(k["f_in_degree"]), p(k["f_edges_idx"]))
+        self.iv = Vertex(p(k["i_out_degree"]), p(k["i_in_degree"]), p(k["i_edges_idx"]))
+        self.fe = EdgeList(e, n, e // max(n, 1), p(k["f_src"]), p(k["f_dst"]), None, None, None)
+        self.ie = EdgeList(e, n, e // max(n, 1), p(k["i_src"]), p(k["i_dst"]), None, None, None)
+        self.g = GraphCSR(e, n, e // max(n, 1), C.pointer(self.fv), C.pointer(self.fe),
+                          C.pointer(self.iv), C.pointer(self.ie))
+
+
+def default_arguments(**kw) -> Arguments:
+    """open-graph.c:270-305 defaults, overridden by keyword."""
+    a = Arguments()
+    a.binSize, a.iterations, a.trials, a.epsilon = 512, 20, 1, 0.0001
+    a.algorithm, a.cache_size, a.delta = 1, 32768, 1
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = os.cpu_count() or 1
+    a.fnameb_format = a.convert_format = 1
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+def pagerank_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20, epsilon=1e-4):
+    """Call the drop-in pageRankGraphCSR; returns (ranks, iterations, error_total, time)."""
+    lib = load()
+    args = default_arguments(pushpull=pushpull, iterations=iterations, epsilon=epsilon)
+    st = lib.pageRankGraphCSR(C.byref(args), C.byref(graph.g))
+    if not st:
+        raise OgpuError("pageRankGraphCSR returned NULL")
+    s = st.contents
+    n = s.num_vertices
+    ranks = np.ctypeslib.as_array(s.pageRanks, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    out = (ranks, s.iterations, s.error_total, s.time_total)
+    lib.freePageRankStats(st)
+    return out

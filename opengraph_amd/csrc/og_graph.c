@@ -1,0 +1,351 @@
+/*
+ * og_graph.c -- device-resident CSR (forward + inverse, the reference's
+ * GraphCSR content) and the PageRank plan built from it.
+ *
+ * The CSR build follows graphCSRPreProcessingStep (graphCSR.c:164-248):
+ * N = max id + 1, stable sort of the forward list (sortRun.c:26-69), vertex
+ * mapping (vertex.c:204-325), inverse list = swapped sorted forward list
+ * sorted again, mapping.  Because the forward list is already sorted by
+ * source, the inverse list's first (by-destination) pass of sort modes 1/3
+ * is the identity and only the by-source sort of (dst, src) runs; the
+ * result is the same stable order.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "og_internal.h"
+#include "og_plan.h"
+#include "og_pr.h"
+#include "opengraph_amd_ext.h"
+
+static uint32_t og_bits_for(uint32_t maxval)
+{
+    uint32_t b = 0;
+    while (b < 32 && (maxval >> b)) b++;
+    return b ? b : 1;
+}
+
+void og_graph_destroy(og_graph *g)
+{
+    if (!g) return;
+    og_dfree(g->f_src); og_dfree(g->f_dst); og_dfree(g->f_out_degree);
+    og_dfree(g->f_in_degree); og_dfree(g->f_edges_idx);
+    og_dfree(g->i_src); og_dfree(g->i_dst); og_dfree(g->i_out_degree);
+    og_dfree(g->i_in_degree); og_dfree(g->i_edges_idx);
+    free(g);
+}
+
+static og_graph *og_graph_alloc(uint32_t N, uint32_t E)
+{
+    og_graph *g = (og_graph *)calloc(1, sizeof(og_graph));
+    if (!g) { og_set_error("out of host memory"); return NULL; }
+    g->N = N;
+    g->E = E;
+    size_t eb = (size_t)E * 4, nb = (size_t)N * 4;
+    g->f_src = og_dalloc(eb); g->f_dst = og_dalloc(eb);
+    g->i_src = og_dalloc(eb); g->i_dst = og_dalloc(eb);
+    g->f_out_degree = og_dalloc(nb); g->f_in_degree = og_dalloc(nb); g->f_edges_idx = og_dalloc(nb);
+    g->i_out_degree = og_dalloc(nb); g->i_in_degree = og_dalloc(nb); g->i_edges_idx = og_dalloc(nb);
+    if (!g->f_src || !g->f_dst || !g->i_src || !g->i_dst || !g->f_out_degree || !g->f_in_degree ||
+        !g->f_edges_idx || !g->i_out_degree || !g->i_in_degree || !g->i_edges_idx)
+    {
+        og_graph_destroy(g);
+        return NULL;
+    }
+    return g;
+}
+
+/* Sort (k_in, v_in) stably by key into the pool pair (*k_res, *v_res): the
+ * two pairs (ka, va), (kb, vb) are the ping-pong buffers and must not alias
+ * the inputs; the pair not holding the result is returned as spare. */
+static int og_sort_into(const uint32_t *k_in, const uint32_t *v_in, uint32_t **ka, uint32_t **va,
+                        uint32_t **kb, uint32_t **vb, uint32_t n, uint32_t key_bits, void *scratch)
+{
+    int in_b = 0;
+    if (og_radix_sort_pairs(k_in, v_in, *ka, *va, *kb, *vb, n, key_bits, scratch, og_stream(), &in_b))
+        return -1;
+    if (in_b)
+    {
+        uint32_t *t = *ka; *ka = *kb; *kb = t;
+        t = *va; *va = *vb; *vb = t;
+    }
+    return 0;   /* result now in (*ka, *va) */
+}
+
+int og_graph_build_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E,
+                          uint32_t sort_mode, int keep_inputs, og_graph **out)
+{
+    (void)keep_inputs;
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    if (sort_mode > 3) sort_mode = 0;     /* sortRun.c:60-66: default is radix by source */
+    hipStream_t s = og_stream();
+    uint32_t *d_max = og_dalloc(4);
+    if (!d_max) return -1;
+    uint32_t maxid = 0;
+    if (og_launch_max_id(d_src, d_dst, E, d_max, s) ||
+        og_hip_check(hipMemcpyAsync(&maxid, d_max, 4, hipMemcpyDeviceToHost, s), "copy max id") ||
+        og_hip_check(hipStreamSynchronize(s), "sync max id"))
+    {
+        og_dfree(d_max);
+        return -1;
+    }
+    og_dfree(d_max);
+    if (E && maxid > OG_ID_MASK)
+    {
+        og_set_error("vertex id %u does not fit the 30-bit id space (reorder.h:18-27)", maxid);
+        return -1;
+    }
+    uint32_t N = E ? maxid + 1 : 1;
+    og_graph *g = og_graph_alloc(N, E);
+    if (!g) return -1;
+    g->sort_mode = sort_mode;
+    uint32_t kb_bits = og_bits_for(N - 1);
+    size_t eb = (size_t)E * 4;
+    uint32_t *t0 = og_dalloc(eb), *t1 = og_dalloc(eb);
+    void *scratch = og_dalloc(og_radix_scratch_bytes(E));
+    int rc = -1;
+    if (!t0 || !t1 || !scratch) goto done;
+
+    if (E)
+    {
+        /* forward list */
+        uint32_t *ka = g->f_src, *va = g->f_dst, *kb = t0, *vb = t1;
+        if (sort_mode == 1 || sort_mode == 3)
+        {
+            /* by destination, then by source (radixsort.c:282-335) */
+            if (og_sort_into(d_dst, d_src, &ka, &va, &kb, &vb, E, kb_bits, scratch)) goto done;
+            /* result (dst, src) in (ka, va); sort (src, dst) by src into the other pair */
+            uint32_t *k2 = kb, *v2 = vb, *k3 = ka, *v3 = va;
+            if (og_sort_into(va, ka, &k2, &v2, &k3, &v3, E, kb_bits, scratch)) goto done;
+            ka = k2; va = v2; kb = k3; vb = v3;
+        }
+        else
+        {
+            if (og_sort_into(d_src, d_dst, &ka, &va, &kb, &vb, E, kb_bits, scratch)) goto done;
+        }
+        /* make the graph own the result pair, keep the other as temp */
+        g->f_src = ka; g->f_dst = va; t0 = kb; t1 = vb;
+
+        /* inverse list: (dst, src) of the sorted forward list sorted by dst */
+        uint32_t *ia = g->i_src, *iva = g->i_dst, *ib = t0, *ivb = t1;
+        if (og_sort_into(g->f_dst, g->f_src, &ia, &iva, &ib, &ivb, E, kb_bits, scratch)) goto done;
+        g->i_src = ia; g->i_dst = iva; t0 = ib; t1 = ivb;
+    }
+    if (og_launch_map_vertices(g->f_src, E, N, g->f_out_degree, g->f_edges_idx, s)) goto done;
+    if (og_launch_map_vertices(g->i_src, E, N, g->i_out_degree, g->i_edges_idx, s)) goto done;
+    /* vertex.c:291-310 */
+    if (og_hip_check(hipMemcpyAsync(g->f_in_degree, g->i_out_degree, (size_t)N * 4,
+                                    hipMemcpyDeviceToDevice, s), "copy in_degree") ||
+        og_hip_check(hipMemcpyAsync(g->i_in_degree, g->f_out_degree, (size_t)N * 4,
+                                    hipMemcpyDeviceToDevice, s), "copy in_degree") ||
+        og_hip_check(hipStreamSynchronize(s), "sync csr build"))
+        goto done;
+    rc = 0;
+done:
+    og_dfree(t0);
+    og_dfree(t1);
+    og_dfree(scratch);
+    if (rc) og_graph_destroy(g);
+    else *out = g;
+    return rc;
+}
+
+/* ----------------------------- plan ----------------------------- */
+
+void og_plan_destroy(og_plan *p)
+{
+    if (!p) return;
+    og_dfree(p->new2old); og_dfree(p->old2new); og_dfree(p->row_ptr); og_dfree(p->cols);
+    og_dfree(p->outdeg); og_dfree(p->chunk_base); og_dfree(p->win_row);
+    og_dfree(p->push_src); og_dfree(p->push_dst); og_dfree(p->f_row_ptr); og_dfree(p->f_cols);
+    free(p);
+}
+
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_forward,
+                  og_plan **out)
+{
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    hipStream_t s = og_stream();
+    const uint32_t N = g->N, E = g->E;
+    og_plan *p = (og_plan *)calloc(1, sizeof(og_plan));
+    if (!p) { og_set_error("out of host memory"); return -1; }
+    p->N = N;
+    p->E = E;
+    p->relabel = relabel;
+    p->hub_t = hub_t;
+    size_t nb = (size_t)N * 4;
+    uint32_t *key = og_dalloc(nb), *key2 = og_dalloc(nb), *val2 = og_dalloc(nb);
+    uint32_t *indeg_new = og_dalloc(nb + 4), *d_cnt = og_dalloc(16);
+    size_t scr = og_radix_scratch_bytes(N);
+    size_t scan_scr = og_scan_scratch_bytes(N + 1);
+    void *scratch = og_dalloc(scr > scan_scr ? scr : scan_scr);
+    p->new2old = og_dalloc(nb);
+    p->old2new = og_dalloc(nb);
+    p->outdeg = og_dalloc(nb);
+    p->row_ptr = og_dalloc(nb + 4);
+    p->cols = og_dalloc((size_t)E * 4);
+    int rc = -1;
+    if (!key || !key2 || !val2 || !indeg_new || !d_cnt || !scratch || !p->new2old || !p->old2new ||
+        !p->outdeg || !p->row_ptr || !p->cols)
+        goto done;
+
+    /* 1. new order: hubs first, then by out-degree descending (stable) */
+    if (og_launch_plan_keys(N, g->f_out_degree, g->i_out_degree, hub_t, relabel, key, p->new2old, s))
+        goto done;
+    {
+        uint32_t *ka = key2, *va = val2, *kb = key, *vb = p->new2old;
+        int in_b = 0;
+        if (og_radix_sort_pairs(key, p->new2old, ka, va, kb, vb, N, relabel ? 32 : 1, scratch, s, &in_b))
+            goto done;
+        if (!in_b)
+        {
+            /* result in (key2, val2): swap so new2old holds the permutation */
+            uint32_t *t = p->new2old; p->new2old = val2; val2 = t;
+        }
+    }
+    /* 2. inverse permutation, degrees in new order, hub count */
+    if (og_launch_fill_u32(d_cnt, 0, 4, s)) goto done;
+    if (og_launch_plan_perm(N, p->new2old, g->f_out_degree, g->i_out_degree, hub_t, p->old2new,
+                            p->outdeg, indeg_new, d_cnt, s))
+        goto done;
+    /* 3. row pointers of the relabelled inverse CSR */
+    if (og_scan_exclusive(indeg_new, p->row_ptr, N, scratch, s)) goto done;
+    uint32_t H = 0;
+    if (og_hip_check(hipMemcpyAsync(&H, d_cnt, 4, hipMemcpyDeviceToHost, s), "copy hub count") ||
+        og_hip_check(hipStreamSynchronize(s), "sync plan"))
+        goto done;
+    p->H = H;
+    /* 4. relabelled in-neighbour lists, reference order kept */
+    if (E && og_launch_plan_copy(H, N, p->new2old, p->old2new, g->i_edges_idx, g->i_dst, p->row_ptr,
+                                 p->cols, s))
+        goto done;
+    /* 5. hub chunks */
+    p->chunk_base = og_dalloc((size_t)(H + 1) * 4);
+    if (!p->chunk_base) goto done;
+    if (og_launch_plan_chunks(H, p->row_ptr, OG_TE, key, s)) goto done;
+    if (og_scan_exclusive(key, p->chunk_base, H, scratch, s)) goto done;
+    uint32_t base_edge = 0, n_chunks = 0;
+    if (og_hip_check(hipMemcpyAsync(&base_edge, p->row_ptr + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        og_hip_check(hipMemcpyAsync(&n_chunks, p->chunk_base + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        og_hip_check(hipStreamSynchronize(s), "sync plan"))
+        goto done;
+    p->n_chunks = n_chunks;
+    /* 6. LDS windows over the non-hub rows */
+    {
+        uint32_t e_norm = E - base_edge;
+        uint32_t nw = (e_norm + OG_TE - 1) / OG_TE;
+        if (nw == 0 && N > H) nw = 1;
+        p->n_windows = nw;
+        p->n_tile_blocks = (nw + OG_WIN_PER_BLOCK - 1) / OG_WIN_PER_BLOCK;
+        p->win_row = og_dalloc((size_t)(nw + 1) * 4);
+        if (!p->win_row) goto done;
+        if (og_launch_plan_windows(H, N, p->row_ptr, OG_TE, nw, p->win_row, s)) goto done;
+    }
+    /* 7. forward edges in new ids (push) and the forward CSR in new ids
+     *    (data-driven activation of out-neighbours) */
+    if (need_forward)
+    {
+        p->push_src = og_dalloc((size_t)E * 4);
+        p->push_dst = og_dalloc((size_t)E * 4);
+        p->f_row_ptr = og_dalloc(nb + 4);
+        p->f_cols = og_dalloc((size_t)E * 4);
+        if (!p->push_src || !p->push_dst || !p->f_row_ptr || !p->f_cols) goto done;
+        if (og_launch_plan_map_pairs(E, g->f_src, g->f_dst, p->old2new, p->push_src, p->push_dst, s))
+            goto done;
+        if (og_scan_exclusive(p->outdeg, p->f_row_ptr, N, scratch, s)) goto done;
+        if (E && og_launch_plan_copy(0, N, p->new2old, p->old2new, g->f_edges_idx, g->f_dst,
+                                     p->f_row_ptr, p->f_cols, s))
+            goto done;
+    }
+    if (og_hip_check(hipStreamSynchronize(s), "sync plan")) goto done;
+    rc = 0;
+done:
+    og_dfree(key); og_dfree(key2); og_dfree(val2); og_dfree(indeg_new); og_dfree(d_cnt);
+    og_dfree(scratch);
+    if (rc) og_plan_destroy(p);
+    else *out = p;
+    return rc;
+}
+
+/* ------------------------- public graph API ------------------------- */
+
+int ogpu_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
+                     uint32_t sort_mode, ogpu_graph **out)
+{
+    return og_graph_build_device(d_src, d_dst, num_edges, sort_mode, 1, (og_graph **)out);
+}
+
+int ogpu_graph_info(const ogpu_graph *gg, uint32_t *num_vertices, uint32_t *num_edges)
+{
+    const og_graph *g = (const og_graph *)gg;
+    if (!g) { og_set_error("null graph"); return -1; }
+    if (num_vertices) *num_vertices = g->N;
+    if (num_edges) *num_edges = g->E;
+    return 0;
+}
+
+int ogpu_graph_array(const ogpu_graph *gg, uint32_t which, uint32_t *host_out)
+{
+    const og_graph *g = (const og_graph *)gg;
+    if (!g) { og_set_error("null graph"); return -1; }
+    const uint32_t *src;
+    size_t n;
+    switch (which)
+    {
+    case OGPU_F_SRC: src = g->f_src; n = g->E; break;
+    case OGPU_F_DST: src = g->f_dst; n = g->E; break;
+    case OGPU_F_OUT_DEGREE: src = g->f_out_degree; n = g->N; break;
+    case OGPU_F_IN_DEGREE: src = g->f_in_degree; n = g->N; break;
+    case OGPU_F_EDGES_IDX: src = g->f_edges_idx; n = g->N; break;
+    case OGPU_I_SRC: src = g->i_src; n = g->E; break;
+    case OGPU_I_DST: src = g->i_dst; n = g->E; break;
+    case OGPU_I_OUT_DEGREE: src = g->i_out_degree; n = g->N; break;
+    case OGPU_I_IN_DEGREE: src = g->i_in_degree; n = g->N; break;
+    case OGPU_I_EDGES_IDX: src = g->i_edges_idx; n = g->N; break;
+    default: og_set_error("unknown graph array %u", which); return -1;
+    }
+    if (n == 0) return 0;
+    if (og_hip_check(hipMemcpyAsync(host_out, src, n * 4, hipMemcpyDeviceToHost, og_stream()), "D2H"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(og_stream()), "sync");
+}
+
+int ogpu_graph_from_arrays(uint32_t N, uint32_t E, const uint32_t *const host_arrays[10],
+                           ogpu_graph **out)
+{
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    og_graph *g = og_graph_alloc(N, E);
+    if (!g) return -1;
+    uint32_t *dst[10] = {g->f_src, g->f_dst, g->f_out_degree, g->f_in_degree, g->f_edges_idx,
+                         g->i_src, g->i_dst, g->i_out_degree, g->i_in_degree, g->i_edges_idx};
+    for (int i = 0; i < 10; i++)
+    {
+        size_t n = (i == 0 || i == 1 || i == 5 || i == 6) ? E : N;
+        if (!n) continue;
+        if (!host_arrays[i])
+        {
+            og_set_error("graph array %d missing", i);
+            og_graph_destroy(g);
+            return -1;
+        }
+        if (og_hip_check(hipMemcpyAsync(dst[i], host_arrays[i], n * 4, hipMemcpyHostToDevice, og_stream()),
+                         "H2D graph array"))
+        {
+            og_graph_destroy(g);
+            return -1;
+        }
+    }
+    if (og_hip_check(hipStreamSynchronize(og_stream()), "sync upload"))
+    {
+        og_graph_destroy(g);
+        return -1;
+    }
+    *out = (ogpu_graph *)g;
+    return 0;
+}
+
+void ogpu_graph_free(ogpu_graph *g) { og_graph_destroy((og_graph *)g); }

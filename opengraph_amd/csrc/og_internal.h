@@ -1,0 +1,125 @@
+/*
+ * og_internal.h -- private declarations of libopengraph_amd shared by the
+ * host C files and the HIP kernel files (whose launchers are extern "C").
+ */
+#ifndef OG_INTERNAL_H
+#define OG_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "og_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- errors and device context (og_runtime.c) ---------------- */
+
+void og_set_error(const char *fmt, ...);
+const char *og_get_error(void);
+/* Makes sure the device and stream exist; returns 0 or -1 with the error set. */
+int og_ctx_ensure(void);
+hipStream_t og_stream(void);
+int og_num_cus(void);
+/* hipMalloc / hipFree with error text; og_dalloc returns NULL on failure. */
+void *og_dalloc(size_t bytes);
+void og_dfree(void *p);
+/* Checks a HIP status; on failure records "<what>: <hip error>" and returns -1. */
+int og_hip_check(hipError_t e, const char *what);
+#define OG_HIP(call) do { if (og_hip_check((call), #call)) return -1; } while (0)
+/* 128-byte aligned host allocation, as the reference's my_malloc
+ * (myMalloc.c:24-67); exits on failure like the reference. */
+void *og_host_alloc(size_t bytes);
+
+/* ---------------- generator / CSR kernels (og_csr.hip) ---------------- */
+
+int og_launch_rmat(uint32_t *d_src, uint32_t *d_dst, uint64_t num_edges, const og_rmat_t *g,
+                   hipStream_t s);
+int og_launch_er(uint32_t *d_src, uint32_t *d_dst, uint64_t num_edges, uint32_t n, uint64_t key,
+                 hipStream_t s);
+/* d_out[0] = max over src and dst (d_out zeroed here). */
+int og_launch_max_id(const uint32_t *d_src, const uint32_t *d_dst, uint32_t n, uint32_t *d_out,
+                     hipStream_t s);
+/* Split interleaved (src,dst) u32 pairs into two arrays. */
+int og_launch_deinterleave(const uint32_t *d_pairs, uint32_t n, uint32_t stride_words,
+                           uint32_t *d_src, uint32_t *d_dst, hipStream_t s);
+
+/* Stable LSD radix sort of (key, val) u32 pairs over key bits [0, key_bits),
+ * 8 bits per pass.  Sorts from (k_in, v_in) using (k_tmp, v_tmp) as the
+ * other buffer; *result_in_tmp tells which pair holds the output.  k_in may
+ * equal a different array than the output lists (e.g. the swapped
+ * forward arrays for the inverse list): the input arrays are only read in
+ * the first pass when first_out_is_tmp... see og_csr.hip. */
+size_t og_radix_scratch_bytes(uint32_t n);
+int og_radix_sort_pairs(const uint32_t *k_in, const uint32_t *v_in, uint32_t *k_a, uint32_t *v_a,
+                        uint32_t *k_b, uint32_t *v_b, uint32_t n, uint32_t key_bits,
+                        void *d_scratch, hipStream_t s, int *result_in_b);
+
+/* out_degree / edges_idx of a sorted key list (vertex.c:262-289). */
+int og_launch_map_vertices(const uint32_t *d_sorted_src, uint32_t n_edges, uint32_t n_vertices,
+                           uint32_t *d_out_degree, uint32_t *d_edges_idx, hipStream_t s);
+
+/* ---------------- device graph (og_graph.c) ---------------- */
+
+typedef struct og_graph
+{
+    uint32_t N, E;
+    /* forward (sorted by src) and inverse (sorted by dst) CSR on device */
+    uint32_t *f_src, *f_dst, *f_out_degree, *f_in_degree, *f_edges_idx;
+    uint32_t *i_src, *i_dst, *i_out_degree, *i_in_degree, *i_edges_idx;
+    /* host identity of the GraphCSR this copy was uploaded from */
+    const void *host_key[8];
+    uint32_t sort_mode;
+} og_graph;
+
+int og_graph_build_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E,
+                          uint32_t sort_mode, int keep_inputs, og_graph **out);
+void og_graph_destroy(og_graph *g);
+
+/* ---------------- PageRank plan / run (og_pagerank.c, og_pr.hip) ---------------- */
+
+#define OG_TE 2048u          /* edges per LDS window of the pull tiles */
+#define OG_WIN_PER_BLOCK 4u  /* windows processed by one tile block */
+
+typedef struct og_plan
+{
+    uint32_t N, E;
+    uint32_t relabel;
+    uint32_t hub_t;              /* rows with more in-edges are split into chunks */
+    uint32_t *new2old, *old2new; /* N each */
+    uint32_t *row_ptr;           /* N+1, rows in new order (inverse CSR) */
+    uint32_t *cols;              /* E, in-neighbours as new ids, reference order kept */
+    uint32_t *outdeg;            /* N, forward out-degree in new order */
+    uint32_t H;                  /* hub rows are new ids [0, H) */
+    uint32_t n_chunks;           /* total hub chunks */
+    uint32_t *chunk_base;        /* H+1 prefix of chunk counts */
+    uint32_t n_windows;          /* windows over rows [H, N) */
+    uint32_t *win_row;           /* n_windows+1 first row of each window */
+    uint32_t n_tile_blocks;
+    /* forward side for push: source and destination of each forward edge, new ids */
+    uint32_t *push_src, *push_dst;
+    /* forward CSR in new ids for the data-driven activation */
+    uint32_t *f_row_ptr;         /* N+1 */
+    uint32_t *f_cols;            /* E */
+} og_plan;
+
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_push,
+                  og_plan **out);
+void og_plan_destroy(og_plan *p);
+
+/* Per-iteration statistics written by the device. */
+typedef struct
+{
+    double error_total;
+    uint32_t active;
+    uint32_t pad;
+} og_iter_stat;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OG_INTERNAL_H */

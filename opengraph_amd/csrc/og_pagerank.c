@@ -1,0 +1,742 @@
+/*
+ * og_pagerank.c -- host orchestration of PageRank on the GPU and the
+ * drop-in pageRank*GraphCSR entry points (pageRank.c:47-2896).
+ *
+ * A run is: init kernel (ranks 0.15f or the caller's start, first
+ * contributions), then per iteration the variant's kernels (og_pr.hip),
+ * enqueued back to back without host round trips; every 8 iterations the
+ * host reads the device stop flag.  Kernels of iterations after the stop
+ * return at once, so the result is that of the reference's loop
+ * (pageRank.c:984-1035).  Finally ranks are divided by N and mapped back to
+ * the caller's vertex ids (pageRank.c:1038-1044).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "og_internal.h"
+#include "og_pr.h"
+#include "opengraph_amd_ext.h"
+
+#define OG_CHECK_EVERY 8u
+#define OG_DEFAULT_HUB_T OG_TE
+
+/* Implemented in og_dd.c. */
+int og_dd_run(ogpu_pr *pr, ogpu_pr_result *res);
+
+struct ogpu_pr
+{
+    og_graph *g;
+    og_plan *plan;
+    int own_plan;
+    ogpu_pr_options opt;
+    int mode, push, quant, dd;
+    og_pr_dev dev;
+    uint32_t gv;              /* grid of the per-vertex kernels */
+    uint32_t n_final_pull, n_final_vert;
+    float *r;
+    void *c[2];
+    void *cq;
+    void *next;
+    void *chunk_sum;
+    float *init_d;
+    float *out_old;
+    double *d_sum;
+    double *part_err, *fin_err;
+    uint32_t *part_act, *fin_act;
+    float *part_min, *part_max, *fin_min, *fin_max;
+    uint32_t *ticket, *done;
+    og_iter_stat *stats;
+    uint32_t stats_cap;
+    og_qparams *qp;
+    uint32_t *done_host;
+    uint32_t it;
+    /* per-iteration end events of a run, kernel timing pairs of enqueue */
+    hipEvent_t *iter_ev;
+    uint32_t n_iter_ev;
+    hipEvent_t *kev;
+    uint32_t n_kev, kev_used;
+    double seconds_plan;
+    uint32_t rows;
+    og_iter_stat *rows_host;
+    /* data-driven state (og_dd.c) */
+    void *dd_state;
+};
+
+/* Accessors for og_dd.c. */
+og_graph *og_pr_graph(ogpu_pr *p) { return p->g; }
+og_plan *og_pr_plan(ogpu_pr *p) { return p->plan; }
+const ogpu_pr_options *og_pr_opts(ogpu_pr *p) { return &p->opt; }
+void **og_pr_dd_state(ogpu_pr *p) { return &p->dd_state; }
+float *og_pr_out(ogpu_pr *p) { return p->out_old; }
+float *og_pr_init_d(ogpu_pr *p) { return p->init_d; }
+void og_pr_set_rows(ogpu_pr *p, const uint32_t *act, const double *err, uint32_t n);
+
+static double og_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+void ogpu_pr_default_options(ogpu_pr_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->variant = 0;
+    o->max_iter = 20;          /* open-graph.c:278 */
+    o->epsilon = 0.0001;       /* open-graph.c:280 */
+    o->relabel = 1;
+    o->hub_threshold = 0;
+}
+
+static int og_variant_mode(uint32_t v, int *mode, int *push, int *quant, int *dd)
+{
+    *push = *quant = *dd = 0;
+    switch (v)
+    {
+    case 0: *mode = OG_MODE_F32; break;
+    case 1: *mode = OG_MODE_F32; *push = 1; break;
+    case 2: *mode = OG_MODE_FX64; break;
+    case 3: *mode = OG_MODE_FX64; *push = 1; break;
+    case 4: *mode = OG_MODE_Q32; *quant = 1; break;
+    case 5: *mode = OG_MODE_QPUSH_EPI; *push = 1; *quant = 1; break;
+    case 6: case 7: case 8: *mode = OG_MODE_F32; *dd = 1; break;
+    case 9: *mode = OG_MODE_FX32; break;
+    case 10: *mode = OG_MODE_FX16; break;
+    case 11: *mode = OG_MODE_FX8; break;
+    case 12: *mode = OG_MODE_Q16; *quant = 1; break;
+    case 13: *mode = OG_MODE_Q8; *quant = 1; break;
+    default: *mode = OG_MODE_F32; break;   /* pageRank.c:929-931: default is pull */
+    }
+    return 0;
+}
+
+static size_t og_elem_bytes(int mode)
+{
+    switch (mode)
+    {
+    case OG_MODE_FX64: return 8;
+    case OG_MODE_FX16: case OG_MODE_Q16: return 2;
+    case OG_MODE_FX8: case OG_MODE_Q8: return 1;
+    default: return 4;
+    }
+}
+
+void ogpu_pr_free(ogpu_pr *p)
+{
+    if (!p) return;
+    if (p->own_plan) og_plan_destroy(p->plan);
+    og_dfree(p->r); og_dfree(p->c[0]); og_dfree(p->c[1]); og_dfree(p->cq); og_dfree(p->next);
+    og_dfree(p->chunk_sum); og_dfree(p->init_d); og_dfree(p->out_old); og_dfree(p->d_sum);
+    og_dfree(p->part_err); og_dfree(p->fin_err); og_dfree(p->part_act); og_dfree(p->fin_act);
+    og_dfree(p->part_min); og_dfree(p->part_max); og_dfree(p->fin_min); og_dfree(p->fin_max);
+    og_dfree(p->ticket); og_dfree(p->done); og_dfree(p->stats); og_dfree(p->qp);
+    if (p->done_host) (void)hipHostFree(p->done_host);
+    for (uint32_t i = 0; i < p->n_iter_ev; i++) (void)hipEventDestroy(p->iter_ev[i]);
+    for (uint32_t i = 0; i < p->n_kev; i++) (void)hipEventDestroy(p->kev[i]);
+    free(p->iter_ev);
+    free(p->kev);
+    free(p->rows_host);
+    if (p->dd_state) free(p->dd_state);   /* og_dd.c frees its device buffers first */
+    free(p);
+}
+
+void og_dd_free_state(ogpu_pr *p);
+
+/* Create a run on graph g.  plan may be supplied (borrowed) or built here. */
+static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, ogpu_pr **out)
+{
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    if (opt->variant > 13)
+    {
+        og_set_error("unknown PageRank variant %u", opt->variant);
+        return -1;
+    }
+    ogpu_pr *p = (ogpu_pr *)calloc(1, sizeof(ogpu_pr));
+    if (!p) { og_set_error("out of host memory"); return -1; }
+    p->g = g;
+    p->opt = *opt;
+    if (p->opt.hub_threshold == 0) p->opt.hub_threshold = OG_DEFAULT_HUB_T;
+    og_variant_mode(opt->variant, &p->mode, &p->push, &p->quant, &p->dd);
+    const uint32_t N = g->N, E = g->E;
+    double t0 = og_now();
+    if (plan)
+        p->plan = plan;
+    else
+    {
+        if (og_plan_build(g, p->opt.relabel, p->opt.hub_threshold, p->push || p->dd, &p->plan))
+        {
+            free(p);
+            return -1;
+        }
+        p->own_plan = 1;
+    }
+    p->seconds_plan = og_now() - t0;
+    og_plan *pl = p->plan;
+    if ((p->push || p->dd) && !pl->push_src)
+    {
+        og_set_error("plan has no forward edges for variant %u", opt->variant);
+        ogpu_pr_free(p);
+        return -1;
+    }
+    p->gv = (N + 255) / 256;
+    if (p->gv > 2048) p->gv = 2048;
+    if (p->gv == 0) p->gv = 1;
+    uint32_t f_tiles = (pl->n_tile_blocks + OG_PARTS_PER_FINAL_BLOCK - 1) / OG_PARTS_PER_FINAL_BLOCK;
+    uint32_t f_hubs = (pl->H + 255) / 256;
+    p->n_final_pull = f_tiles > f_hubs ? f_tiles : f_hubs;
+    if (p->n_final_pull == 0) p->n_final_pull = 1;
+    p->n_final_vert = (p->gv + OG_PARTS_PER_FINAL_BLOCK - 1) / OG_PARTS_PER_FINAL_BLOCK;
+    uint32_t parts = pl->n_tile_blocks > p->gv ? pl->n_tile_blocks : p->gv;
+    uint32_t fins = p->n_final_pull > p->n_final_vert ? p->n_final_pull : p->n_final_vert;
+    p->stats_cap = (opt->max_iter ? opt->max_iter : 1) + 1;
+
+    size_t eb = og_elem_bytes(p->mode);
+    size_t cbytes = p->quant ? 4 : eb;
+    p->r = og_dalloc((size_t)N * 4);
+    p->c[0] = og_dalloc((size_t)N * cbytes);
+    p->c[1] = og_dalloc((size_t)N * cbytes);
+    if (p->quant) p->cq = og_dalloc((size_t)N * eb);
+    if (p->push || p->dd) p->next = og_dalloc((size_t)N * 8);
+    p->chunk_sum = og_dalloc((size_t)(pl->n_chunks + 1) * 8);
+    p->out_old = og_dalloc((size_t)N * 4);
+    p->d_sum = og_dalloc(8);
+    p->part_err = og_dalloc((size_t)parts * 8);
+    p->part_act = og_dalloc((size_t)parts * 4);
+    p->part_min = og_dalloc((size_t)parts * 4);
+    p->part_max = og_dalloc((size_t)parts * 4);
+    p->fin_err = og_dalloc((size_t)fins * 8);
+    p->fin_act = og_dalloc((size_t)fins * 4);
+    p->fin_min = og_dalloc((size_t)fins * 4);
+    p->fin_max = og_dalloc((size_t)fins * 4);
+    p->ticket = og_dalloc(4);
+    p->done = og_dalloc(4);
+    p->stats = og_dalloc((size_t)p->stats_cap * sizeof(og_iter_stat));
+    p->qp = og_dalloc(2 * sizeof(og_qparams));
+    if (!p->r || !p->c[0] || !p->c[1] || (p->quant && !p->cq) || ((p->push || p->dd) && !p->next) ||
+        !p->chunk_sum || !p->out_old || !p->d_sum || !p->part_err || !p->part_act || !p->part_min ||
+        !p->part_max || !p->fin_err || !p->fin_act || !p->fin_min || !p->fin_max || !p->ticket ||
+        !p->done || !p->stats || !p->qp)
+    {
+        ogpu_pr_free(p);
+        return -1;
+    }
+    if (og_hip_check(hipHostMalloc((void **)&p->done_host, 4, hipHostMallocDefault), "hipHostMalloc"))
+    {
+        ogpu_pr_free(p);
+        return -1;
+    }
+
+    og_pr_dev *a = &p->dev;
+    memset(a, 0, sizeof(*a));
+    a->N = N;
+    a->E = E;
+    a->H = pl->H;
+    a->n_chunks = pl->n_chunks;
+    a->n_windows = pl->n_windows;
+    a->n_tile_blocks = pl->n_tile_blocks;
+    a->stop_l1 = opt->stop_rule == 1;
+    a->qmode = p->quant ? (uint32_t)p->mode : 0;
+    a->qzero = p->mode == OG_MODE_QPUSH_EPI;
+    a->allow_stop = 1;
+    a->epsilon = opt->epsilon;
+    a->row_ptr = pl->row_ptr;
+    a->cols = pl->cols;
+    a->outdeg = pl->outdeg;
+    a->chunk_base = pl->chunk_base;
+    a->win_row = pl->win_row;
+    a->new2old = pl->new2old;
+    a->old2new = pl->old2new;
+    a->push_src = pl->push_src;
+    a->push_dst = pl->push_dst;
+    a->r = p->r;
+    a->next = p->next;
+    a->chunk_sum = p->chunk_sum;
+    a->part_err = p->part_err;
+    a->part_act = p->part_act;
+    a->part_min = p->part_min;
+    a->part_max = p->part_max;
+    a->fin_err = p->fin_err;
+    a->fin_act = p->fin_act;
+    a->fin_min = p->fin_min;
+    a->fin_max = p->fin_max;
+    a->ticket = p->ticket;
+    a->done = p->done;
+    a->stats = p->stats;
+    a->qp = p->qp;
+    a->qp_next = p->qp + 1;
+    if (og_launch_fill_u32(p->ticket, 0, 1, og_stream())) { ogpu_pr_free(p); return -1; }
+    *out = p;
+    return 0;
+}
+
+int ogpu_pr_create(ogpu_graph *g, const ogpu_pr_options *opt, ogpu_pr **out)
+{
+    ogpu_pr_options o;
+    if (opt) o = *opt; else ogpu_pr_default_options(&o);
+    return og_pr_create_with((og_graph *)g, NULL, &o, out);
+}
+
+int ogpu_pr_set_init(ogpu_pr *p, const float *init)
+{
+    if (!p->init_d)
+    {
+        p->init_d = og_dalloc((size_t)p->g->N * 4);
+        if (!p->init_d) return -1;
+    }
+    if (og_hip_check(hipMemcpyAsync(p->init_d, init, (size_t)p->g->N * 4, hipMemcpyHostToDevice,
+                                    og_stream()), "H2D init"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(og_stream()), "sync init");
+}
+
+/* Initial ranks and contributions (and, for the quantized variants, the
+ * first quantization parameters and quantized contributions). */
+static int og_pr_init_state(ogpu_pr *p)
+{
+    hipStream_t s = og_stream();
+    p->it = 0;
+    p->kev_used = 0;
+    if (og_launch_fill_u32(p->done, 0, 1, s) || og_launch_fill_u32(p->ticket, 0, 1, s)) return -1;
+    if (p->next && og_launch_fill_u32((uint32_t *)p->next, 0, (uint64_t)p->g->N * 2, s)) return -1;
+    og_pr_dev a = p->dev;
+    a.c_out = p->c[0];
+    a.n_parts = p->gv;
+    const float *init_old = NULL;
+    float init_value = OG_BASE_PR;
+    if (p->opt.init_mode == 1) init_value = 1.0f;
+    else if (p->opt.init_mode == 2)
+    {
+        if (!p->init_d) { og_set_error("init_mode 2 needs ogpu_pr_set_init"); return -1; }
+        init_old = p->init_d;
+    }
+    if (og_launch_init(&a, p->mode, init_old, init_value, s)) return -1;
+    if (p->quant)
+    {
+        a.n_final_blocks = p->n_final_vert;
+        a.allow_stop = 0;
+        a.qp_next = p->qp;
+        if (og_launch_final(&a, OG_NO_STATS, s)) return -1;
+        a.cq_src = p->c[0];
+        a.cq_dst = p->cq;
+        a.qp = p->qp;
+        if (og_launch_quantize(&a, p->mode, s)) return -1;
+    }
+    return 0;
+}
+
+static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t e0, hipEvent_t e1)
+{
+    hipStream_t s = og_stream();
+    og_pr_dev a = p->dev;
+    a.allow_stop = allow_stop;
+    a.qp = p->qp + (it & 1);
+    a.qp_next = p->qp + ((it + 1) & 1);
+    a.g_in = p->quant ? p->cq : p->c[it & 1];
+    a.c_out = p->quant ? p->c[0] : p->c[(it + 1) & 1];
+    uint32_t row = it % p->stats_cap;
+    int rc;
+    if (!p->push)
+    {
+        a.n_parts = p->plan->n_tile_blocks;
+        a.n_final_blocks = p->n_final_pull;
+        rc = og_launch_pull(&a, p->mode, row, s, e0, e1);
+    }
+    else
+    {
+        a.n_parts = p->gv;
+        a.n_final_blocks = p->n_final_vert;
+        rc = og_launch_push(&a, p->mode, row, s, e0, e1);
+    }
+    if (rc) return -1;
+    if (p->quant)
+    {
+        a.cq_src = p->c[0];
+        a.cq_dst = p->cq;
+        a.qp = p->qp + ((it + 1) & 1);
+        if (og_launch_quantize(&a, p->mode, s)) return -1;
+    }
+    return 0;
+}
+
+static int og_pr_events(hipEvent_t **arr, uint32_t *have, uint32_t need)
+{
+    if (*have >= need) return 0;
+    hipEvent_t *n = (hipEvent_t *)realloc(*arr, need * sizeof(hipEvent_t));
+    if (!n) { og_set_error("out of host memory"); return -1; }
+    *arr = n;
+    for (uint32_t i = *have; i < need; i++)
+    {
+        if (og_hip_check(hipEventCreate(&n[i]), "hipEventCreate")) return -1;
+        *have = i + 1;
+    }
+    return 0;
+}
+
+static int og_pr_finish(ogpu_pr *p, double *pr_sum)
+{
+    og_pr_dev a = p->dev;
+    a.n_parts = p->gv;
+    if (og_launch_normalize(&a, p->out_old, p->d_sum, og_stream())) return -1;
+    if (og_hip_check(hipMemcpyAsync(pr_sum, p->d_sum, 8, hipMemcpyDeviceToHost, og_stream()), "D2H sum"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(og_stream()), "sync finish");
+}
+
+void og_pr_set_rows(ogpu_pr *p, const uint32_t *act, const double *err, uint32_t n)
+{
+    free(p->rows_host);
+    p->rows_host = (og_iter_stat *)calloc(n ? n : 1, sizeof(og_iter_stat));
+    for (uint32_t i = 0; i < n; i++)
+    {
+        p->rows_host[i].active = act[i];
+        p->rows_host[i].error_total = err[i];
+    }
+    p->rows = n;
+}
+
+int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
+{
+    memset(res, 0, sizeof(*res));
+    res->seconds_plan = p->seconds_plan;
+    if (p->dd) return og_dd_run(p, res);
+    hipStream_t s = og_stream();
+    const uint32_t max_iter = p->opt.max_iter;
+    if (og_pr_events(&p->iter_ev, &p->n_iter_ev, max_iter + 1)) return -1;
+    double t0 = og_now();
+    if (og_hip_check(hipEventRecord(p->iter_ev[0], s), "hipEventRecord")) return -1;
+    if (og_pr_init_state(p)) return -1;
+    uint32_t launched = 0;
+    while (launched < max_iter)
+    {
+        uint32_t n = max_iter - launched < OG_CHECK_EVERY ? max_iter - launched : OG_CHECK_EVERY;
+        for (uint32_t i = 0; i < n; i++)
+        {
+            if (og_pr_launch_iter(p, launched + i, 1, NULL, NULL)) return -1;
+            if (og_hip_check(hipEventRecord(p->iter_ev[launched + i + 1], s), "hipEventRecord")) return -1;
+        }
+        launched += n;
+        if (og_hip_check(hipMemcpyAsync(p->done_host, p->done, 4, hipMemcpyDeviceToHost, s), "D2H done") ||
+            og_hip_check(hipStreamSynchronize(s), "sync iterations"))
+            return -1;
+        if (*p->done_host) break;
+    }
+    double pr_sum = 0.0;
+    if (og_pr_finish(p, &pr_sum)) return -1;
+    double t1 = og_now();
+    /* rows */
+    free(p->rows_host);
+    p->rows_host = (og_iter_stat *)calloc(launched ? launched : 1, sizeof(og_iter_stat));
+    if (launched &&
+        (og_hip_check(hipMemcpyAsync(p->rows_host, p->stats, (size_t)launched * sizeof(og_iter_stat),
+                                     hipMemcpyDeviceToHost, s), "D2H stats") ||
+         og_hip_check(hipStreamSynchronize(s), "sync stats")))
+        return -1;
+    uint32_t rows = launched, iterations = max_iter;
+    for (uint32_t i = 0; i < launched; i++)
+    {
+        int stop = p->opt.stop_rule == 1 ? (p->rows_host[i].error_total < p->opt.epsilon)
+                                         : (p->rows_host[i].active == 0);
+        if (stop)
+        {
+            rows = i + 1;
+            iterations = i;
+            break;
+        }
+    }
+    p->rows = rows;
+    res->iterations = iterations;
+    res->rows = rows;
+    res->active = rows ? p->rows_host[rows - 1].active : 0;
+    res->error_total = rows ? p->rows_host[rows - 1].error_total : 0.0;
+    res->pr_sum = pr_sum;
+    res->seconds_total = t1 - t0;
+    return 0;
+}
+
+int ogpu_pr_rows(ogpu_pr *p, uint32_t *active, double *error_total, uint32_t max_rows)
+{
+    uint32_t n = p->rows < max_rows ? p->rows : max_rows;
+    for (uint32_t i = 0; i < n; i++)
+    {
+        if (active) active[i] = p->rows_host[i].active;
+        if (error_total) error_total[i] = p->rows_host[i].error_total;
+    }
+    return (int)n;
+}
+
+/* Seconds of iteration i (for the reference's per-iteration table). */
+static double og_pr_row_seconds(ogpu_pr *p, uint32_t i)
+{
+    float ms = 0.0f;
+    if (p->dd || i + 1 >= p->n_iter_ev) return 0.0;
+    if (hipEventElapsedTime(&ms, p->iter_ev[i], p->iter_ev[i + 1]) != hipSuccess) return 0.0;
+    return ms * 1e-3;
+}
+
+int ogpu_pr_reset(ogpu_pr *p)
+{
+    if (og_pr_init_state(p)) return -1;
+    return og_hip_check(hipStreamSynchronize(og_stream()), "sync reset");
+}
+
+int ogpu_pr_enqueue(ogpu_pr *p, uint32_t n, int time_kernels)
+{
+    if (p->dd) { og_set_error("ogpu_pr_enqueue: not available for data-driven variants"); return -1; }
+    if (time_kernels && og_pr_events(&p->kev, &p->n_kev, p->kev_used + 2 * n)) return -1;
+    for (uint32_t i = 0; i < n; i++)
+    {
+        hipEvent_t e0 = NULL, e1 = NULL;
+        if (time_kernels)
+        {
+            e0 = p->kev[p->kev_used++];
+            e1 = p->kev[p->kev_used++];
+        }
+        if (og_pr_launch_iter(p, p->it++, 0, e0, e1)) return -1;
+    }
+    return 0;
+}
+
+int ogpu_pr_kernel_ms(ogpu_pr *p, double *total_ms, uint32_t *count)
+{
+    if (og_hip_check(hipStreamSynchronize(og_stream()), "sync")) return -1;
+    double t = 0.0;
+    for (uint32_t i = 0; i + 1 < p->kev_used; i += 2)
+    {
+        float ms = 0.0f;
+        if (og_hip_check(hipEventElapsedTime(&ms, p->kev[i], p->kev[i + 1]), "hipEventElapsedTime"))
+            return -1;
+        t += ms;
+    }
+    *total_ms = t;
+    *count = p->kev_used / 2;
+    p->kev_used = 0;
+    return 0;
+}
+
+int ogpu_pr_ranks(ogpu_pr *p, float *host_out)
+{
+    if (og_hip_check(hipMemcpyAsync(host_out, p->out_old, (size_t)p->g->N * 4, hipMemcpyDeviceToHost,
+                                    og_stream()), "D2H ranks"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(og_stream()), "sync ranks");
+}
+
+double ogpu_pr_bytes_per_iteration(const ogpu_pr *p)
+{
+    double N = p->g->N, E = p->g->E;
+    switch (p->opt.variant)
+    {
+    case 1: return 12.0 * E + 20.0 * N;           /* index + atomic read-modify-write */
+    case 2: return 12.0 * E + 24.0 * N;           /* 8-byte contributions */
+    case 3: return 20.0 * E + 24.0 * N;
+    case 9: case 4: return 8.0 * E + 20.0 * N;
+    case 10: case 12: return 6.0 * E + 18.0 * N;
+    case 11: case 13: return 5.0 * E + 17.0 * N;
+    default: return 8.0 * E + 20.0 * N;           /* index + fp32 contribution per edge */
+    }
+}
+
+/* ===================================================================== */
+/* drop-in entry points                                                  */
+/* ===================================================================== */
+
+/* One cached device copy of the caller's GraphCSR (keyed by the struct and
+ * array pointers, sizes and the sort mode) and its plan. */
+static struct
+{
+    const void *key[12];
+    uint32_t N, E;
+    og_graph *g;
+    og_plan *plan;
+} g_cache;
+
+static void og_fatal(const char *where)
+{
+    fprintf(stderr, "opengraph_amd: %s: %s\n", where, og_get_error());
+    exit(EXIT_FAILURE);
+}
+
+static int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_plan **plan)
+{
+    const void *key[12] = {graph,
+                           graph->vertices ? graph->vertices->out_degree : NULL,
+                           graph->vertices ? graph->vertices->in_degree : NULL,
+                           graph->vertices ? graph->vertices->edges_idx : NULL,
+                           graph->sorted_edges_array ? graph->sorted_edges_array->edges_array_src : NULL,
+                           graph->sorted_edges_array ? graph->sorted_edges_array->edges_array_dest : NULL,
+                           graph->inverse_vertices ? graph->inverse_vertices->out_degree : NULL,
+                           graph->inverse_vertices ? graph->inverse_vertices->in_degree : NULL,
+                           graph->inverse_vertices ? graph->inverse_vertices->edges_idx : NULL,
+                           graph->inverse_sorted_edges_array ? graph->inverse_sorted_edges_array->edges_array_src : NULL,
+                           graph->inverse_sorted_edges_array ? graph->inverse_sorted_edges_array->edges_array_dest : NULL,
+                           NULL};
+    if (g_cache.g && memcmp(g_cache.key, key, sizeof(key)) == 0 && g_cache.N == graph->num_vertices &&
+        g_cache.E == graph->num_edges)
+    {
+        if (need_forward && !g_cache.plan->push_src)
+        {
+            og_plan_destroy(g_cache.plan);
+            g_cache.plan = NULL;
+            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_HUB_T, 1, &g_cache.plan)) return -1;
+        }
+        *g = g_cache.g;
+        *plan = g_cache.plan;
+        return 0;
+    }
+    og_plan_destroy(g_cache.plan);
+    og_graph_destroy(g_cache.g);
+    memset(&g_cache, 0, sizeof(g_cache));
+    if (!graph->vertices || !graph->inverse_vertices || !graph->sorted_edges_array ||
+        !graph->inverse_sorted_edges_array)
+    {
+        og_set_error("GraphCSR without forward and inverse arrays (DIRECTED build expected)");
+        return -1;
+    }
+    const uint32_t *arrays[10] = {
+        graph->sorted_edges_array->edges_array_src, graph->sorted_edges_array->edges_array_dest,
+        graph->vertices->out_degree, graph->vertices->in_degree, graph->vertices->edges_idx,
+        graph->inverse_sorted_edges_array->edges_array_src,
+        graph->inverse_sorted_edges_array->edges_array_dest, graph->inverse_vertices->out_degree,
+        graph->inverse_vertices->in_degree, graph->inverse_vertices->edges_idx};
+    ogpu_graph *gg = NULL;
+    if (ogpu_graph_from_arrays(graph->num_vertices, graph->num_edges, arrays, &gg)) return -1;
+    og_plan *pl = NULL;
+    if (og_plan_build((og_graph *)gg, 1, OG_DEFAULT_HUB_T, need_forward, &pl))
+    {
+        ogpu_graph_free(gg);
+        return -1;
+    }
+    memcpy(g_cache.key, key, sizeof(key));
+    g_cache.N = graph->num_vertices;
+    g_cache.E = graph->num_edges;
+    g_cache.g = (og_graph *)gg;
+    g_cache.plan = pl;
+    *g = g_cache.g;
+    *plan = pl;
+    return 0;
+}
+
+struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph)
+{
+    struct PageRankStats *stats = (struct PageRankStats *)og_host_alloc(sizeof(struct PageRankStats));
+    stats->damp = OG_DAMP;
+    stats->base_pr = 1.0f - stats->damp;
+    stats->iterations = 0;
+    stats->num_vertices = graph->num_vertices;
+    stats->time_total = 0.0;
+    stats->error_total = 0.0;
+    stats->activeVertices = 0;
+    stats->realRanks = (uint32_t *)og_host_alloc((size_t)graph->num_vertices * sizeof(uint32_t));
+    stats->pageRanks = (float *)og_host_alloc((size_t)graph->num_vertices * sizeof(float));
+    for (uint32_t v = 0; v < stats->num_vertices; v++)
+    {
+        stats->pageRanks[v] = stats->base_pr;
+        stats->realRanks[v] = v;
+    }
+    return stats;
+}
+
+void freePageRankStats(struct PageRankStats *stats)
+{
+    if (stats)
+    {
+        free(stats->realRanks);
+        free(stats->pageRanks);
+        free(stats);
+    }
+}
+
+static const char *og_variant_title(uint32_t v)
+{
+    /* the titles each reference variant prints (pageRank.c:968-974, ...) */
+    switch (v)
+    {
+    case 1: return "Starting Page Rank Push (tolerance/epsilon)";
+    case 2: return "Starting Page Rank Pull FP_64 (tolerance/epsilon)";
+    case 3: return "Starting Page Rank Push FP (tolerance/epsilon)";
+    case 4: return "Starting Page Rank Pull Quant_32";
+    case 5: return "Starting Page Rank Push Quant_32";
+    case 6: return "Starting Page Rank Pull DD (tolerance/epsilon)";
+    case 7: return "Starting Page Rank Push DD (tolerance/epsilon)";
+    case 8: return "Starting Page Rank Pull-Push DD (tolerance/epsilon)";
+    case 9: return "Starting Page Rank Pull FP_32 (tolerance/epsilon)";
+    case 10: return "Starting Page Rank Pull FP_16 (tolerance/epsilon)";
+    case 11: return "Starting Page Rank Pull FP_8 (tolerance/epsilon)";
+    case 12: return "Starting Page Rank Pull Quant_16";
+    case 13: return "Starting Page Rank Pull Quant_8";
+    default: return "Starting Page Rank Pull (tolerance/epsilon)";
+    }
+}
+
+static struct PageRankStats *og_dropin(struct Arguments *arguments, struct GraphCSR *graph, uint32_t variant)
+{
+    if (og_ctx_ensure()) og_fatal("device");
+    ogpu_pr_options o;
+    ogpu_pr_default_options(&o);
+    o.variant = variant;
+    o.max_iter = arguments->iterations;
+    o.epsilon = arguments->epsilon;
+    int mode, push, quant, dd;
+    og_variant_mode(variant, &mode, &push, &quant, &dd);
+    og_graph *g = NULL;
+    og_plan *plan = NULL;
+    if (og_cache_graph(graph, push || dd, &g, &plan)) og_fatal("graph upload");
+    ogpu_pr *pr = NULL;
+    if (og_pr_create_with(g, plan, &o, &pr)) og_fatal("pagerank setup");
+
+    struct PageRankStats *stats = newPageRankStatsGraphCSR(graph);
+    const char *title = og_variant_title(variant);
+    printf(" -----------------------------------------------------\n");
+    if (quant)
+        printf("| %-30s %-19s| \n", title, "(tolerance/epsilon)");
+    else
+        printf("| %-51s | \n", title);
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51.13lf | \n", arguments->epsilon);
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iteration", "Active", "Error", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+
+    ogpu_pr_result res;
+    if (ogpu_pr_run(pr, &res)) og_fatal("pagerank run");
+    if (ogpu_pr_ranks(pr, stats->pageRanks)) og_fatal("pagerank ranks");
+    for (uint32_t i = 0; i < pr->rows; i++)
+        printf("| %-10u | %-8u | %-15.13lf | %-9f | \n", i, pr->rows_host[i].active,
+               pr->rows_host[i].error_total, og_pr_row_seconds(pr, i));
+    stats->iterations = res.iterations;
+    stats->activeVertices = res.active;
+    stats->error_total = res.error_total;
+    stats->time_total = res.seconds_total;
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iterations", "PR Sum", "Error", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10u | %-8lf | %-15.13lf | %-9f | \n", stats->iterations, res.pr_sum, stats->error_total,
+           stats->time_total);
+    printf(" -----------------------------------------------------\n");
+    ogpu_pr_free(pr);
+    return stats;
+}
+
+struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    uint32_t v = arguments->pushpull;
+    return og_dropin(arguments, graph, v <= 13 ? v : 0);
+}
+
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 0); }
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 1); }
+struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 2); }
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 3); }
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 4); }
+struct PageRankStats *pageRankPushQuantGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 5); }
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 6); }
+struct PageRankStats *pageRankDataDrivenPushGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 7); }
+struct PageRankStats *pageRankDataDrivenPullPushGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 8); }
+struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 9); }
+struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 10); }
+struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 11); }
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 12); }
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 13); }

@@ -1,0 +1,301 @@
+/*
+ * og_plan.hip -- gfx950 kernels that build the per-graph PageRank plan from
+ * the device CSR: the vertex relabelling, the relabelled inverse CSR (rows
+ * keep the reference's in-neighbour order), hub chunks, LDS windows, the
+ * relabelled forward edge list for push and the forward CSR for the
+ * data-driven variants.  Also a general exclusive scan.
+ */
+#include <hip/hip_runtime.h>
+
+#include "og_internal.h"
+#include "og_plan.h"
+
+#define OG_BLOCK 256
+#define SCAN_ITEMS 16
+#define SCAN_TILE (OG_BLOCK * SCAN_ITEMS)
+
+/* ------------------------------------------------------------------ */
+/* exclusive scan of u32 (out[n] = total)                              */
+/* ------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(OG_BLOCK) void og_scan_tile_kernel(const uint32_t *__restrict__ in,
+                                                               uint32_t *__restrict__ out, uint32_t n,
+                                                               uint32_t *__restrict__ bsum)
+{
+    __shared__ uint32_t s[OG_BLOCK];
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)t * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+    {
+        v[i] = base + i < n ? in[base + i] : 0;
+        tot += v[i];
+    }
+    s[t] = tot;
+    __syncthreads();
+    for (uint32_t o = 1; o < OG_BLOCK; o <<= 1)
+    {
+        uint32_t x = t >= o ? s[t - o] : 0;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = s[t] - tot;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+    {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (t == OG_BLOCK - 1) bsum[blockIdx.x] = s[t];
+}
+
+__global__ __launch_bounds__(1024) void og_scan_single_kernel(uint32_t *__restrict__ c, uint32_t total,
+                                                             uint32_t *__restrict__ grand)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (total + 1023) / 1024;
+    uint32_t b = t * per, e = min(b + per, total);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += c[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1)
+    {
+        uint32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint32_t i = b; i < e; i++)
+    {
+        uint32_t x = c[i];
+        c[i] = run;
+        run += x;
+    }
+    if (t == 1023 && grand) *grand = part[1023];
+}
+
+__global__ void og_scan_add_kernel(uint32_t *__restrict__ out, uint32_t n,
+                                   const uint32_t *__restrict__ bsum)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] += bsum[i / SCAN_TILE];
+}
+
+extern "C" size_t og_scan_scratch_bytes(uint32_t n)
+{
+    return ((size_t)(n / SCAN_TILE) + 2) * sizeof(uint32_t);
+}
+
+extern "C" int og_scan_exclusive(const uint32_t *d_in, uint32_t *d_out, uint32_t n, void *d_scratch,
+                                 hipStream_t s)
+{
+    uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    uint32_t *bsum = (uint32_t *)d_scratch;
+    if (n == 0)
+    {
+        hipLaunchKernelGGL(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, 0u, d_out);
+        return og_hip_check(hipGetLastError(), "og_scan_exclusive");
+    }
+    hipLaunchKernelGGL(og_scan_tile_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, d_in, d_out, n, bsum);
+    hipLaunchKernelGGL(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, nb, d_out + n);
+    unsigned g = nb < 65535 ? nb : 65535;
+    hipLaunchKernelGGL(og_scan_add_kernel, dim3(g), dim3(OG_BLOCK), 0, s, d_out, n, bsum);
+    return og_hip_check(hipGetLastError(), "og_scan_exclusive");
+}
+
+/* ------------------------------------------------------------------ */
+/* relabelling                                                         */
+/* ------------------------------------------------------------------ */
+
+/* Sort key of old vertex v: hubs (in-degree > hub_t) first; with relabel,
+ * then by forward out-degree descending; ties keep id order (stable sort). */
+__global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ outdeg,
+                                    const uint32_t *__restrict__ indeg, uint32_t hub_t,
+                                    uint32_t relabel, uint32_t *__restrict__ key,
+                                    uint32_t *__restrict__ val)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+    {
+        uint32_t hub = indeg[v] > hub_t;
+        uint32_t k;
+        if (relabel)
+        {
+            uint32_t d = min(outdeg[v], 0x7FFFFFFFu);
+            k = (hub ? 0u : 0x80000000u) | (0x7FFFFFFFu - d);
+        }
+        else
+            k = hub ? 0u : 1u;
+        key[v] = k;
+        val[v] = v;
+    }
+}
+
+__global__ void og_plan_perm_kernel(uint32_t N, const uint32_t *__restrict__ new2old,
+                                    const uint32_t *__restrict__ outdeg_old,
+                                    const uint32_t *__restrict__ indeg_old, uint32_t hub_t,
+                                    uint32_t *__restrict__ old2new, uint32_t *__restrict__ outdeg_new,
+                                    uint32_t *__restrict__ indeg_new, uint32_t *__restrict__ n_hubs)
+{
+    uint32_t hubs = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    {
+        uint32_t o = new2old[i];
+        old2new[o] = i;
+        outdeg_new[i] = outdeg_old[o];
+        uint32_t d = indeg_old[o];
+        indeg_new[i] = d;
+        hubs += d > hub_t;
+    }
+    if (hubs) atomicAdd(n_hubs, hubs);
+}
+
+/* cols[row_ptr[i] + k] = old2new[src_list[idx_old[o] + k]], o = new2old[i]:
+ * one wavefront per row for rows [r0, r1). */
+__global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_rows_kernel(
+    uint32_t r0, uint32_t r1, const uint32_t *__restrict__ new2old,
+    const uint32_t *__restrict__ old2new, const uint32_t *__restrict__ idx_old,
+    const uint32_t *__restrict__ lists, const uint32_t *__restrict__ row_ptr,
+    uint32_t *__restrict__ cols)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = r0 + wave; i < r1; i += nwaves)
+    {
+        uint32_t o = new2old[i];
+        uint32_t src = idx_old[o];
+        uint32_t dst = row_ptr[i], n = row_ptr[i + 1] - dst;
+        for (uint32_t k = lane; k < n; k += 64) cols[dst + k] = old2new[lists[src + k] & OG_ID_MASK];
+    }
+}
+
+/* The same for the hub rows [0, H), one block per row. */
+__global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_hubs_kernel(
+    const uint32_t *__restrict__ new2old, const uint32_t *__restrict__ old2new,
+    const uint32_t *__restrict__ idx_old, const uint32_t *__restrict__ lists,
+    const uint32_t *__restrict__ row_ptr, uint32_t *__restrict__ cols)
+{
+    uint32_t i = blockIdx.x;
+    uint32_t o = new2old[i];
+    uint32_t src = idx_old[o];
+    uint32_t dst = row_ptr[i], n = row_ptr[i + 1] - dst;
+    for (uint32_t k = threadIdx.x; k < n; k += OG_BLOCK) cols[dst + k] = old2new[lists[src + k] & OG_ID_MASK];
+}
+
+__global__ void og_plan_chunk_count_kernel(uint32_t H, const uint32_t *__restrict__ row_ptr,
+                                           uint32_t te, uint32_t *__restrict__ cnt)
+{
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += gridDim.x * blockDim.x)
+        cnt[h] = (row_ptr[h + 1] - row_ptr[h] + te - 1) / te;
+}
+
+/* win_row[k] = first row r in [H, N) whose row_ptr[r] >= base + k*te
+ * (k < n_windows); win_row[n_windows] = N. */
+__global__ void og_plan_windows_kernel(uint32_t H, uint32_t N, const uint32_t *__restrict__ row_ptr,
+                                       uint32_t te, uint32_t n_windows, uint32_t *__restrict__ win_row)
+{
+    const uint32_t base = row_ptr[H];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n_windows; k += gridDim.x * blockDim.x)
+    {
+        if (k == n_windows) { win_row[k] = N; continue; }
+        if (k == 0) { win_row[k] = H; continue; }
+        uint64_t target = (uint64_t)base + (uint64_t)k * te;
+        uint32_t lo = H, hi = N;      /* first r in [lo, hi) with row_ptr[r] >= target */
+        while (lo < hi)
+        {
+            uint32_t mid = lo + ((hi - lo) >> 1);
+            if ((uint64_t)row_ptr[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+        win_row[k] = lo;
+    }
+}
+
+__global__ void og_plan_map_pairs_kernel(uint32_t E, const uint32_t *__restrict__ a,
+                                         const uint32_t *__restrict__ b,
+                                         const uint32_t *__restrict__ old2new,
+                                         uint32_t *__restrict__ oa, uint32_t *__restrict__ ob)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
+    {
+        oa[e] = old2new[a[e] & OG_ID_MASK];
+        ob[e] = old2new[b[e] & OG_ID_MASK];
+    }
+}
+
+static unsigned og_grid(uint64_t n, unsigned cap)
+{
+    uint64_t g = (n + OG_BLOCK - 1) / OG_BLOCK;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+extern "C" int og_launch_plan_keys(uint32_t N, const uint32_t *outdeg, const uint32_t *indeg,
+                                   uint32_t hub_t, uint32_t relabel, uint32_t *key, uint32_t *val,
+                                   hipStream_t s)
+{
+    hipLaunchKernelGGL(og_plan_keys_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, outdeg,
+                       indeg, hub_t, relabel, key, val);
+    return og_hip_check(hipGetLastError(), "og_plan_keys_kernel");
+}
+
+extern "C" int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *outdeg_old,
+                                   const uint32_t *indeg_old, uint32_t hub_t, uint32_t *old2new,
+                                   uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *n_hubs,
+                                   hipStream_t s)
+{
+    hipLaunchKernelGGL(og_plan_perm_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, new2old,
+                       outdeg_old, indeg_old, hub_t, old2new, outdeg_new, indeg_new, n_hubs);
+    return og_hip_check(hipGetLastError(), "og_plan_perm_kernel");
+}
+
+extern "C" int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2old,
+                                   const uint32_t *old2new, const uint32_t *idx_old,
+                                   const uint32_t *lists, const uint32_t *row_ptr, uint32_t *cols,
+                                   hipStream_t s)
+{
+    if (H) hipLaunchKernelGGL(og_plan_copy_hubs_kernel, dim3(H), dim3(OG_BLOCK), 0, s, new2old, old2new,
+                              idx_old, lists, row_ptr, cols);
+    if (N > H)
+    {
+        uint64_t waves = N - H;
+        unsigned grid = (unsigned)((waves + 3) / 4 < 65536 ? (waves + 3) / 4 : 65536);
+        hipLaunchKernelGGL(og_plan_copy_rows_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, H, N, new2old,
+                           old2new, idx_old, lists, row_ptr, cols);
+    }
+    return og_hip_check(hipGetLastError(), "og_plan_copy");
+}
+
+extern "C" int og_launch_plan_chunks(uint32_t H, const uint32_t *row_ptr, uint32_t te, uint32_t *cnt,
+                                     hipStream_t s)
+{
+    if (!H) return 0;
+    hipLaunchKernelGGL(og_plan_chunk_count_kernel, dim3(og_grid(H, 4096)), dim3(OG_BLOCK), 0, s, H,
+                       row_ptr, te, cnt);
+    return og_hip_check(hipGetLastError(), "og_plan_chunk_count_kernel");
+}
+
+extern "C" int og_launch_plan_windows(uint32_t H, uint32_t N, const uint32_t *row_ptr, uint32_t te,
+                                      uint32_t n_windows, uint32_t *win_row, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_plan_windows_kernel, dim3(og_grid((uint64_t)n_windows + 1, 16384)),
+                       dim3(OG_BLOCK), 0, s, H, N, row_ptr, te, n_windows, win_row);
+    return og_hip_check(hipGetLastError(), "og_plan_windows_kernel");
+}
+
+extern "C" int og_launch_plan_map_pairs(uint32_t E, const uint32_t *a, const uint32_t *b,
+                                        const uint32_t *old2new, uint32_t *oa, uint32_t *ob,
+                                        hipStream_t s)
+{
+    if (!E) return 0;
+    hipLaunchKernelGGL(og_plan_map_pairs_kernel, dim3(og_grid(E, 65536)), dim3(OG_BLOCK), 0, s, E, a, b,
+                       old2new, oa, ob);
+    return og_hip_check(hipGetLastError(), "og_plan_map_pairs_kernel");
+}

@@ -1,0 +1,93 @@
+/*
+ * og_pr.h -- argument block and launchers of the PageRank kernels (og_pr.hip),
+ * shared with the host orchestration in og_pagerank.c.
+ */
+#ifndef OG_PR_H
+#define OG_PR_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "og_internal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of the gathered contributions. */
+#define OG_MODE_F32 0        /* -p 0, 1 */
+#define OG_MODE_FX64 1       /* -p 2, 3 */
+#define OG_MODE_FX32 2       /* -p 9 */
+#define OG_MODE_FX16 3       /* -p 10 */
+#define OG_MODE_FX8 4        /* -p 11 */
+#define OG_MODE_Q32 5        /* -p 4 */
+#define OG_MODE_Q16 6        /* -p 12 */
+#define OG_MODE_Q8 7         /* -p 13 */
+#define OG_MODE_QPUSH_EPI 8  /* -p 5 */
+
+/* Iteration index that writes no statistics row (the init-time reduction). */
+#define OG_NO_STATS 0xFFFFFFFFu
+
+/* First-level partials folded by one block of the final reduction. */
+#define OG_PARTS_PER_FINAL_BLOCK 1024u
+
+typedef struct
+{
+    double scale_d;
+    float scale_f;
+    uint32_t zero;
+} og_qparams;
+
+/* Everything a PageRank kernel needs, passed by value at each launch. */
+typedef struct og_pr_dev
+{
+    uint32_t N, E, H;
+    uint32_t n_chunks, n_windows, n_tile_blocks;
+    uint32_t n_parts;        /* valid first-level partials for the next final reduction */
+    uint32_t n_final_blocks; /* blocks of the final reduction */
+    uint32_t stop_l1;        /* 0: stop when active == 0; 1: when error_total < epsilon */
+    uint32_t qmode;          /* 0 or the quantized mode whose parameters the final step computes */
+    uint32_t qzero;          /* compute a zero point (quantized push) */
+    uint32_t allow_stop;     /* final reduction may raise *done */
+    double epsilon;
+    /* plan */
+    const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *win_row;
+    const uint32_t *new2old, *old2new;
+    const uint32_t *push_src, *push_dst;
+    /* state */
+    float *r;
+    const void *g_in;        /* gathered contributions of this iteration */
+    void *c_out;             /* contributions written by this iteration's epilogue */
+    void *next;              /* push accumulators */
+    const void *cq_src;      /* quantize: fp32 contributions */
+    void *cq_dst;            /* quantize: quantized contributions */
+    void *chunk_sum;         /* hub chunk partial sums (acc type) */
+    double *part_err;
+    uint32_t *part_act;
+    float *part_min, *part_max;
+    double *fin_err;
+    uint32_t *fin_act;
+    float *fin_min, *fin_max;
+    uint32_t *ticket;
+    uint32_t *done;
+    og_iter_stat *stats;
+    og_qparams *qp, *qp_next;
+} og_pr_dev;
+
+int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hipEvent_t ev0,
+                   hipEvent_t ev1);
+int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s, hipEvent_t ev0,
+                   hipEvent_t ev1);
+int og_launch_quantize(const og_pr_dev *a, int mode, hipStream_t s);
+int og_launch_init(const og_pr_dev *a, int mode, const float *init_old, float init_value,
+                   hipStream_t s);
+int og_launch_final(const og_pr_dev *a, uint32_t it, hipStream_t s);
+int og_launch_normalize(const og_pr_dev *a, float *d_out_old, double *d_sum, hipStream_t s);
+int og_launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,154 @@
+/*
+ * og_runtime.c -- device context, error text and allocation helpers of
+ * libopengraph_amd (host C over the HIP runtime C API).
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "og_internal.h"
+#include "opengraph_amd_ext.h"
+
+static __thread char g_err[512];
+static int g_ready = 0;
+static int g_device = 0;
+static int g_cus = 0;
+static hipStream_t g_stream = NULL;
+
+void og_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *og_get_error(void) { return g_err; }
+
+int og_hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    og_set_error("%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return -1;
+}
+
+int og_ctx_ensure(void)
+{
+    if (g_ready) return 0;
+    int n = 0;
+    if (og_hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return -1;
+    if (n <= 0)
+    {
+        og_set_error("no HIP device is visible");
+        return -1;
+    }
+    if (g_device >= n) g_device = 0;
+    if (og_hip_check(hipSetDevice(g_device), "hipSetDevice")) return -1;
+    hipDeviceProp_t prop;
+    if (og_hip_check(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties")) return -1;
+    g_cus = prop.multiProcessorCount;
+    /* Default-priority, non-blocking stream for all of the library's work. */
+    if (og_hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate"))
+        return -1;
+    g_ready = 1;
+    return 0;
+}
+
+hipStream_t og_stream(void) { return g_stream; }
+int og_num_cus(void) { return g_cus; }
+
+void *og_dalloc(size_t bytes)
+{
+    void *p = NULL;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+    {
+        og_set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return NULL;
+    }
+    return p;
+}
+
+void og_dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+void *og_host_alloc(size_t bytes)
+{
+    void *p = NULL;
+    if (bytes == 0) bytes = 1;
+    if (posix_memalign(&p, 128, bytes) != 0)
+    {
+        fprintf(stderr, "Error, aligned_alloc of %zu bytes failed\n", bytes);
+        exit(EXIT_FAILURE);
+    }
+    return p;
+}
+
+/* ------------------------- public helpers ------------------------- */
+
+int ogpu_init(int device)
+{
+    if (g_ready && device != g_device)
+    {
+        og_set_error("ogpu_init: already initialised on device %d", g_device);
+        return -1;
+    }
+    g_device = device < 0 ? 0 : device;
+    return og_ctx_ensure();
+}
+
+const char *ogpu_last_error(void) { return g_err; }
+
+int ogpu_device_info(char *name, size_t name_len, int *num_cus, size_t *total_mem)
+{
+    if (og_ctx_ensure()) return -1;
+    hipDeviceProp_t prop;
+    if (og_hip_check(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties")) return -1;
+    if (name && name_len)
+    {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (total_mem) *total_mem = prop.totalGlobalMem;
+    return 0;
+}
+
+int ogpu_malloc(void **ptr, size_t bytes)
+{
+    if (og_ctx_ensure()) return -1;
+    *ptr = og_dalloc(bytes);
+    return *ptr ? 0 : -1;
+}
+
+int ogpu_free(void *ptr)
+{
+    if (!ptr) return 0;
+    return og_hip_check(hipFree(ptr), "hipFree");
+}
+
+int ogpu_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (og_ctx_ensure()) return -1;
+    if (og_hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream), "hipMemcpy H2D"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+}
+
+int ogpu_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (og_ctx_ensure()) return -1;
+    if (og_hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream), "hipMemcpy D2H"))
+        return -1;
+    return og_hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+}
+
+int ogpu_synchronize(void)
+{
+    if (og_ctx_ensure()) return -1;
+    if (og_hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize")) return -1;
+    return og_hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}

@@ -1,0 +1,108 @@
+/*
+ * og_oracle.h -- CPU oracle for the OpenGraph PageRank-over-CSR path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C/OpenMP restatement of the
+ * reference algorithms, used as the checker of the GPU library by tests/,
+ * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The
+ * product (libopengraph_amd.so) never links, loads or calls it.
+ *
+ * Each function cites the reference lines it restates (paths relative to the
+ * reference's 00_graph_bench/ tree).  Arithmetic follows the reference's x86-64
+ * gcc -O3 build exactly: fp32 division, no FMA contraction (this file set is
+ * compiled with -ffp-contract=off), and the x86 results of out-of-range
+ * float->integer conversions made explicit by the og_x86_* helpers.
+ *
+ * Pinning (what checks each part, see tests/test_oracle_*.py):
+ *   CSR build   -- golden CSR digests from the compiled reference, numpy
+ *                  stable argsort / lexsort / bincount closed forms, 0==2, 1==3.
+ *   pull / fixed-point 64/32/16/8 / quant 32/16/8 / data-driven pull
+ *               -- bit-exact against golden rank vectors and per-iteration
+ *                  active counts written by the compiled reference
+ *                  (tests/golden); closed forms (cycle, star, dense solve).
+ *   push (fp32 atomics), quant push, data-driven push / pull-push
+ *               -- nondeterministic in the reference as well; pinned against
+ *                  the FP64 oracle and the golden vectors within tolerances.
+ *   FP64 oracle -- dense linear solve on small graphs, mass balance,
+ *                  isolated-vertex value.
+ */
+#ifndef OG_ORACLE_H
+#define OG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Forward and inverse CSR exactly as the reference lays them out:
+ * sorted src/dst lists plus the three Vertex arrays of each direction. */
+typedef struct
+{
+    uint32_t num_vertices;
+    uint32_t num_edges;
+    uint32_t *f_src, *f_dst;              /* graph->sorted_edges_array */
+    uint32_t *f_out_degree, *f_in_degree, *f_edges_idx;   /* graph->vertices */
+    uint32_t *i_src, *i_dst;              /* graph->inverse_sorted_edges_array */
+    uint32_t *i_out_degree, *i_in_degree, *i_edges_idx;   /* graph->inverse_vertices */
+} orc_csr;
+
+/* Per-run outputs of a PageRank restatement. */
+typedef struct
+{
+    uint32_t iterations;        /* stats->iterations (off-by-one rule kept) */
+    uint32_t rows;              /* number of per-iteration rows written */
+    double error_total;         /* last iteration's error_total */
+    double pr_sum;              /* "PR Sum" of the normalized ranks */
+    double seconds_total;       /* wall time of the timed region */
+} orc_pr_result;
+
+/* edgeList.c:559-567: number of vertices = max id + 1. */
+uint32_t orc_num_vertices(const uint32_t *src, const uint32_t *dst, uint32_t num_edges);
+
+/* radixsort.c:46-135 / :231-335 and countsort.c: stable sort of (key, val)
+ * pairs by key; returns 0. */
+int orc_stable_sort_pairs(uint32_t *key, uint32_t *val, uint32_t n, int nthreads);
+
+/* graphCSR.c:164-248 with sortRun.c:26-69, edgeList.c:597-745 and
+ * vertex.c:204-325.  sort_mode: 0/2 by src, 1/3 by src then dst.
+ * Allocates every array of *out.  Returns 0, or -1 on allocation failure. */
+int orc_build_csr(const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
+                  uint32_t sort_mode, int nthreads, orc_csr *out);
+void orc_free_csr(orc_csr *g);
+
+/* pageRank.c:871-2896: the -p code variants 0..13 on CSR.
+ *   ranks_out        N normalized ranks (pageRanks after r/N)
+ *   active_per_iter  optional, max_iter entries: activeVertices per iteration
+ *   error_per_iter   optional, max_iter entries: error_total per iteration
+ *   init             optional unnormalized initial ranks (NULL: 0.15f,
+ *                    pageRank.c:65-70); used by the uniform / learned starts
+ *   stop_l1          0: reference rule (stop when activeVertices == 0);
+ *                    1: stop when error_total < epsilon (codes 0, 2 only)
+ * Returns 0, -1 on allocation failure, -2 on an unknown code. */
+int orc_pagerank(const orc_csr *g, uint32_t code, double epsilon, uint32_t max_iter,
+                 int nthreads, const float *init, int stop_l1,
+                 float *ranks_out, uint32_t *active_per_iter, double *error_per_iter,
+                 orc_pr_result *res);
+
+/* FP64 oracle of the same map r' = 0.15 + 0.85 * sum(r_u / outdeg_u) with no
+ * dangling redistribution, in double throughout with sequential sums.
+ * init: unnormalized start value for every vertex.  Iterates until the
+ * normalized L1 change sum|dr|/N < tol or max_iter.  ranks_out is normalized
+ * (divided by N).  Returns the number of iterations run. */
+uint32_t orc_pagerank_fp64(const orc_csr *g, double init, double tol, uint32_t max_iter,
+                           int nthreads, double *ranks_out, double *last_l1);
+
+/* FP64 oracle of personalized PageRank with a one-hot restart at source s:
+ * x' = 0.15 * e_s + 0.85 * P^T x (no dangling redistribution), start x = e_s,
+ * until sum|dx| < tol.  Returns iterations. */
+uint32_t orc_ppr_fp64(const orc_csr *g, uint32_t source, double tol, uint32_t max_iter,
+                      double *x_out, double *last_l1);
+
+/* reorder.c:64-221: stable ascending sort of labels by FloatToFixed32SORT(rank). */
+void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,184 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/libogoracle.so (the C/OpenMP restatement of the reference,
+og_oracle.c) and, when present, oracle/_ref/libogref.so (the reference's own
+PageRank objects compiled from its sources, ref_lib.c).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, and
+only as the checker / the timed CPU baseline; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(_HERE, "libogoracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libogref.so")
+REF_HARNESS = os.path.join(_HERE, "_ref", "ref_harness")
+
+u32p = C.POINTER(C.c_uint32)
+f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
+
+CSR_FIELDS = ["f_src", "f_dst", "f_out_degree", "f_in_degree", "f_edges_idx",
+              "i_src", "i_dst", "i_out_degree", "i_in_degree", "i_edges_idx"]
+
+
+class OrcCSR(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_edges", C.c_uint32),
+                ("f_src", u32p), ("f_dst", u32p),
+                ("f_out_degree", u32p), ("f_in_degree", u32p), ("f_edges_idx", u32p),
+                ("i_src", u32p), ("i_dst", u32p),
+                ("i_out_degree", u32p), ("i_in_degree", u32p), ("i_edges_idx", u32p)]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("rows", C.c_uint32), ("error_total", C.c_double),
+                ("pr_sum", C.c_double), ("seconds_total", C.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: make -C oracle")
+        L = C.CDLL(ORACLE_LIB)
+        L.orc_build_csr.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(OrcCSR)]
+        L.orc_build_csr.restype = C.c_int
+        L.orc_free_csr.argtypes = [C.POINTER(OrcCSR)]
+        L.orc_pagerank.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, C.c_int,
+                                   f32p, C.c_int, f32p, u32p, f64p, C.POINTER(OrcResult)]
+        L.orc_pagerank.restype = C.c_int
+        L.orc_pagerank_fp64.argtypes = [C.POINTER(OrcCSR), C.c_double, C.c_double, C.c_uint32,
+                                        C.c_int, f64p, f64p]
+        L.orc_pagerank_fp64.restype = C.c_uint32
+        L.orc_ppr_fp64.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, f64p, f64p]
+        L.orc_ppr_fp64.restype = C.c_uint32
+        L.orc_sort_labels_by_rank.argtypes = [f32p, u32p, C.c_uint32]
+        L.orc_stable_sort_pairs.argtypes = [u32p, u32p, C.c_uint32, C.c_int]
+        L.orc_stable_sort_pairs.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_LIB)
+        R.ogref_pagerank.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
+            C.c_uint32, C.c_double, C.c_uint32, C.c_int, f32p, u32p, f64p, f64p]
+        R.ogref_pagerank.restype = C.c_int
+        _ref = R
+    return _ref
+
+
+def _p(a, t=u32p):
+    return a.ctypes.data_as(t)
+
+
+class CSR:
+    """Oracle CSR: numpy copies of the ten reference arrays."""
+
+    def __init__(self, arrays: dict, num_vertices: int, num_edges: int):
+        self.a = {k: np.ascontiguousarray(arrays[k], dtype=np.uint32) for k in CSR_FIELDS}
+        self.num_vertices = num_vertices
+        self.num_edges = num_edges
+        self.s = OrcCSR(num_vertices, num_edges, *[_p(self.a[k]) for k in CSR_FIELDS])
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+
+def build_csr(src, dst, sort_mode=0, nthreads=0) -> CSR:
+    L = lib()
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    g = OrcCSR()
+    rc = L.orc_build_csr(_p(src), _p(dst), len(src), sort_mode, nthreads, C.byref(g))
+    if rc:
+        raise RuntimeError("orc_build_csr failed")
+    N, E = g.num_vertices, g.num_edges
+    out = {}
+    for k in CSR_FIELDS:
+        n = E if k in ("f_src", "f_dst", "i_src", "i_dst") else N
+        ptr = getattr(g, k)
+        out[k] = np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+    L.orc_free_csr(C.byref(g))
+    return CSR(out, N, E)
+
+
+def pagerank(csr: CSR, code=0, epsilon=1e-4, max_iter=20, nthreads=0, init=None, stop_l1=False):
+    """Returns dict(ranks, iterations, rows, error_total, pr_sum, seconds, active, errors)."""
+    L = lib()
+    N = csr.num_vertices
+    ranks = np.zeros(N, np.float32)
+    act = np.zeros(max(max_iter, 1), np.uint32)
+    err = np.zeros(max(max_iter, 1), np.float64)
+    res = OrcResult()
+    ip = None
+    if init is not None:
+        init = np.ascontiguousarray(init, dtype=np.float32)
+        ip = _p(init, f32p)
+    rc = L.orc_pagerank(C.byref(csr.s), code, epsilon, max_iter, nthreads, ip, int(stop_l1),
+                        _p(ranks, f32p), _p(act), _p(err, f64p), C.byref(res))
+    if rc:
+        raise RuntimeError(f"orc_pagerank failed ({rc})")
+    return dict(ranks=ranks, iterations=res.iterations, rows=res.rows,
+                error_total=res.error_total, pr_sum=res.pr_sum, seconds=res.seconds_total,
+                active=act[:res.rows].copy(), errors=err[:res.rows].copy())
+
+
+def pagerank_fp64(csr: CSR, init=0.15, tol=1e-13, max_iter=1000, nthreads=0):
+    L = lib()
+    out = np.zeros(csr.num_vertices, np.float64)
+    l1 = C.c_double()
+    it = L.orc_pagerank_fp64(C.byref(csr.s), init, tol, max_iter, nthreads, _p(out, f64p), C.byref(l1))
+    return out, it, l1.value
+
+
+def ppr_fp64(csr: CSR, source, tol=1e-12, max_iter=1000):
+    L = lib()
+    out = np.zeros(csr.num_vertices, np.float64)
+    l1 = C.c_double()
+    it = L.orc_ppr_fp64(C.byref(csr.s), source, tol, max_iter, _p(out, f64p), C.byref(l1))
+    return out, it, l1.value
+
+
+def sort_labels_by_rank(ranks, labels):
+    r = np.ascontiguousarray(ranks, dtype=np.float32)
+    lab = np.ascontiguousarray(labels, dtype=np.uint32).copy()
+    lib().orc_sort_labels_by_rank(_p(r, f32p), _p(lab), len(lab))
+    return lab
+
+
+def ref_pagerank(csr: CSR, code=0, epsilon=1e-4, iterations=20, threads=1):
+    """Run the reference's own compiled pageRankGraphCSR on these arrays."""
+    R = ref_lib()
+    N, E = csr.num_vertices, csr.num_edges
+    ranks = np.zeros(N, np.float32)
+    it = C.c_uint32()
+    err = C.c_double()
+    sec = C.c_double()
+    arrays = [csr.a[k] for k in ["f_out_degree", "f_in_degree", "f_edges_idx", "f_src", "f_dst",
+                                  "i_out_degree", "i_in_degree", "i_edges_idx", "i_src", "i_dst"]]
+    rc = R.ogref_pagerank(N, E, *[_p(x) for x in arrays], code, epsilon, iterations, threads,
+                          _p(ranks, f32p), C.byref(it), C.byref(err), C.byref(sec))
+    if rc:
+        raise RuntimeError("ogref_pagerank failed")
+    return dict(ranks=ranks, iterations=it.value, error_total=err.value, seconds=sec.value)
+
+
+def read_bin(path):
+    raw = np.fromfile(path, dtype=np.uint32)
+    raw = raw[: (len(raw) // 2) * 2]
+    return raw[0::2].copy(), raw[1::2].copy()

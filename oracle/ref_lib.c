@@ -1,0 +1,76 @@
+/*
+ * ref_lib.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Shared-library wrapper around the reference's own compiled PageRank
+ * (pageRank.c:871 pageRankGraphCSR), built by oracle/Makefile target "ref"
+ * into oracle/_ref/libogref.so.  It wraps caller-owned CSR arrays in the
+ * reference's GraphCSR / Vertex / EdgeList structs (graphCSR.h:12-29) and
+ * runs one variant with the reference's stdout table silenced, so the
+ * reference itself can serve as a timed CPU baseline and as a live parity
+ * check.  Nothing here computes PageRank: the reference objects do.
+ */
+#include <fcntl.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "graphConfig.h"
+#include "graphCSR.h"
+#include "mt19937.h"
+#include "pageRank.h"
+
+int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
+                   uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst,
+                   uint32_t *i_out_degree, uint32_t *i_in_degree, uint32_t *i_edges_idx,
+                   uint32_t *i_src, uint32_t *i_dst, uint32_t pushpull, double epsilon,
+                   uint32_t iterations, int threads, float *ranks_out, uint32_t *iters_out,
+                   double *error_out, double *seconds_out)
+{
+    struct Vertex fv = {f_out_degree, f_in_degree, f_edges_idx};
+    struct Vertex iv = {i_out_degree, i_in_degree, i_edges_idx};
+    struct EdgeList fe, ie;
+    memset(&fe, 0, sizeof(fe));
+    memset(&ie, 0, sizeof(ie));
+    fe.num_edges = ie.num_edges = E;
+    fe.num_vertices = ie.num_vertices = N;
+    fe.avg_degree = ie.avg_degree = N ? E / N : 0;
+    fe.edges_array_src = f_src; fe.edges_array_dest = f_dst;
+    ie.edges_array_src = i_src; ie.edges_array_dest = i_dst;
+    struct GraphCSR g;
+    memset(&g, 0, sizeof(g));
+    g.num_edges = E;
+    g.num_vertices = N;
+    g.avg_degree = N ? E / N : 0;
+    g.vertices = &fv;
+    g.sorted_edges_array = &fe;
+    g.inverse_vertices = &iv;
+    g.inverse_sorted_edges_array = &ie;
+
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.iterations = iterations;
+    a.epsilon = epsilon;
+    a.pushpull = pushpull;
+    a.algorithm = 1;
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = threads;
+    initializeMersenneState(&a.mt19937var, 27491095);
+    omp_set_num_threads(threads);
+
+    fflush(stdout);
+    int saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+    struct PageRankStats *s = pageRankGraphCSR(&a, &g);
+    fflush(stdout);
+    if (saved >= 0) { dup2(saved, 1); close(saved); }
+    if (devnull >= 0) close(devnull);
+    if (!s) return -1;
+    memcpy(ranks_out, s->pageRanks, (size_t)N * sizeof(float));
+    *iters_out = s->iterations;
+    *error_out = s->error_total;
+    *seconds_out = s->time_total;
+    freePageRankStats(s);
+    return 0;
+}
