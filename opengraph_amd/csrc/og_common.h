@@ -18,7 +18,8 @@
 
 /* pageRank.h:13 and pageRank.c:53-54. */
 #define OG_DAMP 0.85f
-#define OG_BASE_PR 0.15f
+/* base_pr = 1.0f - damp = 0.149999976f in fp32 (pageRank.c:54), not 0.15f. */
+#define OG_BASE_PR (1.0f - OG_DAMP)
 /* reorder.h:19: vertex ids carry two mask bits above bit 30. */
 #define OG_ID_MASK 0x3FFFFFFFu
 

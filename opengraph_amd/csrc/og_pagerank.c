@@ -22,8 +22,7 @@
 #define OG_CHECK_EVERY 8u
 #define OG_DEFAULT_HUB_T OG_TE
 
-/* Implemented in og_dd.c. */
-int og_dd_run(ogpu_pr *pr, ogpu_pr_result *res);
+static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res);
 
 struct ogpu_pr
 {
@@ -60,18 +59,11 @@ struct ogpu_pr
     double seconds_plan;
     uint32_t rows;
     og_iter_stat *rows_host;
-    /* data-driven state (og_dd.c) */
-    void *dd_state;
+    /* data-driven state: two byte worklists, changed flags, residuals */
+    uint8_t *wl[2];
+    uint8_t *changed;
+    float *resid;
 };
-
-/* Accessors for og_dd.c. */
-og_graph *og_pr_graph(ogpu_pr *p) { return p->g; }
-og_plan *og_pr_plan(ogpu_pr *p) { return p->plan; }
-const ogpu_pr_options *og_pr_opts(ogpu_pr *p) { return &p->opt; }
-void **og_pr_dd_state(ogpu_pr *p) { return &p->dd_state; }
-float *og_pr_out(ogpu_pr *p) { return p->out_old; }
-float *og_pr_init_d(ogpu_pr *p) { return p->init_d; }
-void og_pr_set_rows(ogpu_pr *p, const uint32_t *act, const double *err, uint32_t n);
 
 static double og_now(void)
 {
@@ -138,11 +130,9 @@ void ogpu_pr_free(ogpu_pr *p)
     free(p->iter_ev);
     free(p->kev);
     free(p->rows_host);
-    if (p->dd_state) free(p->dd_state);   /* og_dd.c frees its device buffers first */
+    og_dfree(p->wl[0]); og_dfree(p->wl[1]); og_dfree(p->changed); og_dfree(p->resid);
     free(p);
 }
-
-void og_dd_free_state(ogpu_pr *p);
 
 /* Create a run on graph g.  plan may be supplied (borrowed) or built here. */
 static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, ogpu_pr **out)
@@ -223,6 +213,19 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
         ogpu_pr_free(p);
         return -1;
     }
+    if (p->dd)
+    {
+        size_t wb = ((size_t)N + 3) & ~(size_t)3;
+        p->wl[0] = og_dalloc(wb);
+        p->wl[1] = og_dalloc(wb);
+        p->changed = og_dalloc(wb);
+        p->resid = og_dalloc((size_t)N * 4);
+        if (!p->wl[0] || !p->wl[1] || !p->changed || !p->resid)
+        {
+            ogpu_pr_free(p);
+            return -1;
+        }
+    }
     if (og_hip_check(hipHostMalloc((void **)&p->done_host, 4, hipHostMallocDefault), "hipHostMalloc"))
     {
         ogpu_pr_free(p);
@@ -267,6 +270,11 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->stats = p->stats;
     a->qp = p->qp;
     a->qp_next = p->qp + 1;
+    a->dd = p->dd ? opt->variant : 0;
+    a->f_row_ptr = pl->f_row_ptr;
+    a->f_cols = pl->f_cols;
+    a->changed = p->changed;
+    a->resid = p->resid;
     if (og_launch_fill_u32(p->ticket, 0, 1, og_stream())) { ogpu_pr_free(p); return -1; }
     *out = p;
     return 0;
@@ -385,18 +393,6 @@ static int og_pr_finish(ogpu_pr *p, double *pr_sum)
     return og_hip_check(hipStreamSynchronize(og_stream()), "sync finish");
 }
 
-void og_pr_set_rows(ogpu_pr *p, const uint32_t *act, const double *err, uint32_t n)
-{
-    free(p->rows_host);
-    p->rows_host = (og_iter_stat *)calloc(n ? n : 1, sizeof(og_iter_stat));
-    for (uint32_t i = 0; i < n; i++)
-    {
-        p->rows_host[i].active = act[i];
-        p->rows_host[i].error_total = err[i];
-    }
-    p->rows = n;
-}
-
 int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
 {
     memset(res, 0, sizeof(*res));
@@ -471,8 +467,9 @@ int ogpu_pr_rows(ogpu_pr *p, uint32_t *active, double *error_total, uint32_t max
 static double og_pr_row_seconds(ogpu_pr *p, uint32_t i)
 {
     float ms = 0.0f;
-    if (p->dd || i + 1 >= p->n_iter_ev) return 0.0;
-    if (hipEventElapsedTime(&ms, p->iter_ev[i], p->iter_ev[i + 1]) != hipSuccess) return 0.0;
+    uint32_t k = p->dd ? i + 1 : i;    /* data-driven runs record an init event first */
+    if (k + 1 >= p->n_iter_ev) return 0.0;
+    if (hipEventElapsedTime(&ms, p->iter_ev[k], p->iter_ev[k + 1]) != hipSuccess) return 0.0;
     return ms * 1e-3;
 }
 
@@ -537,6 +534,102 @@ double ogpu_pr_bytes_per_iteration(const ogpu_pr *p)
     case 11: case 13: return 5.0 * E + 17.0 * N;
     default: return 8.0 * E + 20.0 * N;           /* index + fp32 contribution per edge */
     }
+}
+
+/* Data-driven variants (pageRank.c:2405-2896).  Per iteration:
+ *   6: contributions -> pull over the worklist (hub chunks + LDS windows,
+ *      fused update) -> hub finalize + reduction -> activation of the
+ *      out-neighbours of moved vertices;
+ *   7/8: one residual-push kernel over the worklist + reduction;
+ * then the worklists swap and the old one is cleared
+ * (pageRank.c:2520-2522). */
+static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res)
+{
+    hipStream_t s = og_stream();
+    const uint32_t max_iter = p->opt.max_iter;
+    const uint32_t code = p->opt.variant;
+    const uint32_t N = p->g->N;
+    const uint64_t wwords = ((uint64_t)N + 3) / 4;
+    if (og_pr_events(&p->iter_ev, &p->n_iter_ev, max_iter + 2)) return -1;
+    double t0 = og_now();
+    if (og_hip_check(hipEventRecord(p->iter_ev[0], s), "hipEventRecord")) return -1;
+    p->it = 0;
+    if (og_launch_fill_u32(p->done, 0, 1, s) || og_launch_fill_u32(p->ticket, 0, 1, s)) return -1;
+    /* pageRank.c:2455-2466 / :2612-2633: every vertex active, ranks 0.15 */
+    if (og_launch_fill_u32((uint32_t *)p->wl[0], 0x01010101u, wwords, s) ||
+        og_launch_fill_u32((uint32_t *)p->wl[1], 0, wwords, s))
+        return -1;
+    og_pr_dev a = p->dev;
+    a.c_out = p->c[0];
+    a.n_parts = p->gv;
+    float init_value = p->opt.init_mode == 1 ? 1.0f : OG_BASE_PR;
+    const float *init_old = p->opt.init_mode == 2 ? p->init_d : NULL;
+    if (p->opt.init_mode == 2 && !init_old) { og_set_error("init_mode 2 needs ogpu_pr_set_init"); return -1; }
+    if (og_launch_init(&a, OG_MODE_F32, init_old, init_value, s)) return -1;
+    if (code != 6 && og_launch_dd_resid_init(&a, s)) return -1;
+    if (og_hip_check(hipEventRecord(p->iter_ev[1], s), "hipEventRecord")) return -1;
+    uint32_t launched = 0;
+    while (launched < max_iter)
+    {
+        uint32_t n = max_iter - launched < OG_CHECK_EVERY ? max_iter - launched : OG_CHECK_EVERY;
+        for (uint32_t i = 0; i < n; i++)
+        {
+            uint32_t it = launched + i;
+            og_pr_dev b = p->dev;
+            b.wl_cur = p->wl[it & 1];
+            b.wl_next = p->wl[(it + 1) & 1];
+            b.allow_stop = 1;
+            if (code == 6)
+            {
+                b.c_out = p->c[0];
+                b.g_in = p->c[0];
+                if (og_launch_dd_contrib(&b, s)) return -1;
+                b.n_parts = p->plan->n_tile_blocks;
+                b.n_final_blocks = p->n_final_pull;
+                if (og_launch_pull(&b, OG_MODE_F32, it, s, NULL, NULL)) return -1;
+                if (og_launch_dd_activate(&b, s)) return -1;
+            }
+            else
+            {
+                b.n_parts = p->gv;
+                b.n_final_blocks = p->n_final_vert;
+                if (og_launch_dd_push(&b, code, it, s)) return -1;
+            }
+            if (og_launch_fill_u32((uint32_t *)p->wl[it & 1], 0, wwords, s)) return -1;
+            if (og_hip_check(hipEventRecord(p->iter_ev[it + 2], s), "hipEventRecord")) return -1;
+        }
+        launched += n;
+        if (og_hip_check(hipMemcpyAsync(p->done_host, p->done, 4, hipMemcpyDeviceToHost, s), "D2H done") ||
+            og_hip_check(hipStreamSynchronize(s), "sync iterations"))
+            return -1;
+        if (*p->done_host) break;
+    }
+    double pr_sum = 0.0;
+    if (og_pr_finish(p, &pr_sum)) return -1;
+    double t1 = og_now();
+    free(p->rows_host);
+    p->rows_host = (og_iter_stat *)calloc(launched ? launched : 1, sizeof(og_iter_stat));
+    if (launched &&
+        (og_hip_check(hipMemcpyAsync(p->rows_host, p->stats, (size_t)launched * sizeof(og_iter_stat),
+                                     hipMemcpyDeviceToHost, s), "D2H stats") ||
+         og_hip_check(hipStreamSynchronize(s), "sync stats")))
+        return -1;
+    uint32_t rows = launched, iterations = max_iter;
+    for (uint32_t i = 0; i < launched; i++)
+        if (p->rows_host[i].active == 0)
+        {
+            rows = i + 1;
+            iterations = i;
+            break;
+        }
+    p->rows = rows;
+    res->iterations = iterations;
+    res->rows = rows;
+    res->active = rows ? p->rows_host[rows - 1].active : 0;
+    res->error_total = rows ? p->rows_host[rows - 1].error_total : 0.0;
+    res->pr_sum = pr_sum;
+    res->seconds_total = t1 - t0;
+    return 0;
 }
 
 /* ===================================================================== */
@@ -703,6 +796,13 @@ static struct PageRankStats *og_dropin(struct Arguments *arguments, struct Graph
     ogpu_pr_result res;
     if (ogpu_pr_run(pr, &res)) og_fatal("pagerank run");
     if (ogpu_pr_ranks(pr, stats->pageRanks)) og_fatal("pagerank ranks");
+    if (dd)
+    {
+        /* pageRank.c:2468: the worklist initialisation row */
+        float ms = 0.0f;
+        if (pr->n_iter_ev > 1) (void)hipEventElapsedTime(&ms, pr->iter_ev[0], pr->iter_ev[1]);
+        printf("| %-10s | %-8u | %-15.13lf | %-9f | \n", "Init", graph->num_vertices, 0.0, ms * 1e-3);
+    }
     for (uint32_t i = 0; i < pr->rows; i++)
         printf("| %-10u | %-8u | %-15.13lf | %-9f | \n", i, pr->rows_host[i].active,
                pr->rows_host[i].error_total, og_pr_row_seconds(pr, i));

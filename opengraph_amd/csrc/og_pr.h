@@ -73,6 +73,11 @@ typedef struct og_pr_dev
     uint32_t *done;
     og_iter_stat *stats;
     og_qparams *qp, *qp_next;
+    /* data-driven variants (pageRank.c:2405-2896) */
+    uint32_t dd;             /* 6: pull over worklists; 7: residual push; 8: pull-push */
+    const uint32_t *f_row_ptr, *f_cols;
+    uint8_t *wl_cur, *wl_next, *changed;
+    float *resid;
 } og_pr_dev;
 
 int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hipEvent_t ev0,
@@ -85,6 +90,11 @@ int og_launch_init(const og_pr_dev *a, int mode, const float *init_old, float in
 int og_launch_final(const og_pr_dev *a, uint32_t it, hipStream_t s);
 int og_launch_normalize(const og_pr_dev *a, float *d_out_old, double *d_sum, hipStream_t s);
 int og_launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+/* data-driven variants */
+int og_launch_dd_contrib(const og_pr_dev *a, hipStream_t s);
+int og_launch_dd_activate(const og_pr_dev *a, hipStream_t s);
+int og_launch_dd_resid_init(const og_pr_dev *a, hipStream_t s);
+int og_launch_dd_push(const og_pr_dev *a, uint32_t code, uint32_t it, hipStream_t s);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,27 @@ template <int MODE>
 __device__ __forceinline__ void og_epilogue(const og_pr_dev &a, uint32_t v,
                                             typename og_mode<MODE>::acc sum, og_stat &st)
 {
+    if constexpr (MODE == OG_MODE_F32)
+    {
+        if (a.dd)
+        {
+            /* data-driven pull, pageRank.c:2483-2516: only worklist vertices;
+             * the rank moves only when it changed by at least epsilon, and
+             * then its out-neighbours are activated (og_dd_activate_kernel). */
+            if (!a.wl_cur[v]) return;
+            float old = a.r[v];
+            float nr = og_next_rank<MODE>(sum, a.qp);
+            double err = fabs((double)(nr - old));
+            st.err += err / (double)a.N;
+            if (err >= a.epsilon)
+            {
+                a.r[v] = nr;
+                a.changed[v] = 1;
+                st.active++;
+            }
+            return;
+        }
+    }
     float prev = a.r[v];
     float nr = og_next_rank<MODE>(sum, a.qp);
     a.r[v] = nr;
@@ -222,6 +243,11 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
             if (a.chunk_base[mid] <= c) lo = mid; else hi = mid;
         }
         uint32_t h = lo;
+        if (a.dd && !a.wl_cur[h])
+        {
+            if (t == 0) ((A *)a.chunk_sum)[c] = 0;
+            return;
+        }
         uint32_t eb = a.row_ptr[h] + (c - a.chunk_base[h]) * OG_TE;
         uint32_t ee = min(eb + OG_TE, a.row_ptr[h + 1]);
         A s = 0;
@@ -661,4 +687,135 @@ extern "C" int og_launch_normalize(const og_pr_dev *a, float *d_out_old, double 
     hipLaunchKernelGGL(og_normalize_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, d_out_old);
     hipLaunchKernelGGL(og_sum_parts_kernel, dim3(1), dim3(OG_BLOCK), 0, s, a->part_err, a->n_parts, d_sum);
     return og_hip_check(hipGetLastError(), "og_normalize_kernel");
+}
+
+/* ------------------------------------------------------------------ */
+/* data-driven variants (pageRank.c:2405-2896)                         */
+/* ------------------------------------------------------------------ */
+
+/* pageRank.c:2469-2476: contributions of the current ranks; clears the
+ * changed flags of the previous iteration. */
+__global__ void og_dd_contrib_kernel(og_pr_dev a)
+{
+    if (*a.done) return;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.N; v += gridDim.x * blockDim.x)
+    {
+        uint32_t deg = a.outdeg[v];
+        ((float *)a.c_out)[v] = deg ? a.r[v] / (float)deg : 0.0f;
+        a.changed[v] = 0;
+    }
+}
+
+/* pageRank.c:2504-2513: workListNext[u] = 1 for every out-neighbour u of a
+ * vertex whose rank moved (edge-parallel over the forward edges). */
+__global__ void og_dd_activate_kernel(og_pr_dev a)
+{
+    if (*a.done) return;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x)
+        if (a.changed[a.push_src[e]]) a.wl_next[a.push_dst[e]] = 1;
+}
+
+/* pageRank.c:2616-2633: residual = (1-d) * d * sum over in-neighbours of
+ * 1/outdeg, in CSR order. */
+__global__ void og_dd_resid_init_kernel(og_pr_dev a)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.N; v += gridDim.x * blockDim.x)
+    {
+        float acc = 0.0f;
+        for (uint32_t j = a.row_ptr[v]; j < a.row_ptr[v + 1]; j++)
+        {
+            uint32_t d = a.outdeg[a.cols[j]];
+            if (d) acc += 1.0f / (float)d;
+        }
+        a.resid[v] = (1.0f - OG_DAMP) * OG_DAMP * acc;
+    }
+}
+
+/* pageRank.c:2645-2685 (push) and :2796-2852 (pull-push): one thread per
+ * worklist vertex, residuals moved to out-neighbours with atomic adds.  The
+ * reference's reads of neighbours' residuals and ranks race with their
+ * updates; here they race the same way, so results are nondeterministic. */
+__global__ __launch_bounds__(OG_BLOCK) void og_dd_push_kernel(og_pr_dev a, uint32_t code)
+{
+    if (*a.done) return;
+    og_stat st;
+    og_stat_init(st);
+    const float fn = (float)a.N;
+    for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
+    {
+        if (!a.wl_cur[v]) continue;
+        float old = a.r[v];
+        float nr;
+        if (code == 7)
+            nr = a.resid[v] + old;
+        else
+        {
+            float s = 0.0f;
+            for (uint32_t j = a.row_ptr[v]; j < a.row_ptr[v + 1]; j++)
+            {
+                uint32_t u = a.cols[j];
+                s += __hip_atomic_load(&a.r[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / (float)a.outdeg[u];
+            }
+            nr = OG_BASE_PR + (OG_DAMP * s);
+        }
+        st.err += fabs((double)(nr / fn - old / fn));
+        __hip_atomic_store(&a.r[v], nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t deg = a.outdeg[v];
+        float delta = OG_DAMP * (a.resid[v] / (float)deg);
+        for (uint32_t j = a.f_row_ptr[v]; j < a.f_row_ptr[v + 1]; j++)
+        {
+            uint32_t u = a.f_cols[j];
+            float prev = atomicAdd(&a.resid[u], delta);
+            if ((fabs((double)(prev + delta)) >= a.epsilon) && ((double)prev <= a.epsilon))
+            {
+                st.active++;
+                if (code == 8) atomicAdd(&a.resid[u], delta);
+                a.wl_next[u] = 1;
+            }
+        }
+        a.resid[v] = 0.0f;
+    }
+    og_stat r = og_block_stat(st);
+    if (threadIdx.x == 0)
+    {
+        a.part_err[blockIdx.x] = r.err;
+        a.part_act[blockIdx.x] = r.active;
+        a.part_min[blockIdx.x] = r.cmin;
+        a.part_max[blockIdx.x] = r.cmax;
+    }
+}
+
+static unsigned og_vgrid(uint64_t n)
+{
+    uint64_t g = (n + OG_BLOCK - 1) / OG_BLOCK;
+    if (g > 4096) g = 4096;
+    return g ? (unsigned)g : 1u;
+}
+
+extern "C" int og_launch_dd_contrib(const og_pr_dev *a, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_dd_contrib_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(hipGetLastError(), "og_dd_contrib_kernel");
+}
+
+extern "C" int og_launch_dd_activate(const og_pr_dev *a, hipStream_t s)
+{
+    if (!a->E) return 0;
+    unsigned g = (a->E + OG_BLOCK - 1) / OG_BLOCK;
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(og_dd_activate_kernel, dim3(g), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(hipGetLastError(), "og_dd_activate_kernel");
+}
+
+extern "C" int og_launch_dd_resid_init(const og_pr_dev *a, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_dd_resid_init_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(hipGetLastError(), "og_dd_resid_init_kernel");
+}
+
+extern "C" int og_launch_dd_push(const og_pr_dev *a, uint32_t code, uint32_t it, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_dd_push_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, code);
+    hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
+    return og_hip_check(hipGetLastError(), "og_dd_push_kernel");
 }

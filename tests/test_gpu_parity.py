@@ -1,0 +1,291 @@
+"""GPU parity of libopengraph_amd against the oracle and the reference's
+golden outputs, through the C ABI.
+
+Bars: integer / index work (generators, CSR arrays, fixed-point and
+quantized PageRank) is bit-exact; fp32 pull is bit-exact when every row is
+summed in CSR order (hub_threshold above the largest in-degree) and within
+1e-5 relative of the reference otherwise (hub rows are summed in a fixed
+tree order); fp32 atomic push is nondeterministic in the reference too and
+is held to 1e-5 relative.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import opengraph_amd as og
+from conftest import golden_input
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ALL_EXACT = 1 << 31          # hub threshold above any in-degree: every row sequential
+DETERMINISTIC = (0, 2, 3, 4, 6, 9, 10, 11, 12, 13)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    og.init(0)
+    name, cus, mem = og.device_info()
+    print(f"device: {name}, {cus} CUs, {mem / 2**30:.0f} GiB", flush=True)
+
+
+@pytest.fixture(scope="module")
+def graphs(golden):
+    cache = {}
+
+    def get(name, sort):
+        if (name, sort) not in cache:
+            s, d = golden_input(name, golden)
+            cache[(name, sort)] = og.Graph.from_edges(s, d, sort)
+        return cache[(name, sort)]
+
+    return get
+
+
+# ---------------------------------------------------------------------------
+# generators and CSR build
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scale,edges,permute", [(10, 5000, False), (16, 1 << 20, True), (21, 1 << 22, True)])
+def test_device_rmat_matches_host(scale, edges, permute):
+    hs, hd = og.rmat_host(scale, edges, 0.57, 0.19, 0.19, 1, permute)
+    ds, dd = og.rmat_device(scale, edges, 0.57, 0.19, 0.19, 1, permute)
+    assert np.array_equal(ds.to_numpy(np.uint32, edges), hs)
+    assert np.array_equal(dd.to_numpy(np.uint32, edges), hd)
+
+
+def test_device_er_matches_host():
+    hs, hd = og.er_host(123457, 1 << 20, 2)
+    ds, dd = og.er_device(123457, 1 << 20, 2)
+    assert np.array_equal(ds.to_numpy(np.uint32, 1 << 20), hs)
+    assert np.array_equal(dd.to_numpy(np.uint32, 1 << 20), hd)
+
+
+@pytest.mark.parametrize("name", ["graphbrew", "enron", "rmat16"])
+@pytest.mark.parametrize("sort", [0, 1, 2, 3])
+def test_csr_build_bitexact_vs_reference(golden, graphs, name, sort):
+    g = graphs(name, sort)
+    ref = golden["csr"][f"{name}/sort{sort % 2}"]
+    assert (g.num_vertices, g.num_edges) == (ref["num_vertices"], ref["num_edges"])
+    for k, a in g.arrays().items():
+        assert sha(a) == ref[k], k
+
+
+@pytest.mark.parametrize("case", ["tiny", "selfloops", "sparse_ids", "single", "dups"])
+@pytest.mark.parametrize("sort", [0, 1])
+def test_csr_build_edge_cases_vs_oracle(case, sort):
+    rng = np.random.default_rng(5)
+    if case == "tiny":
+        src, dst = np.array([3, 1, 3, 0], np.uint32), np.array([1, 3, 0, 3], np.uint32)
+    elif case == "selfloops":
+        src = rng.integers(0, 50, 500).astype(np.uint32)
+        dst = src.copy()
+    elif case == "sparse_ids":
+        src = rng.integers(0, 1 << 22, 3000).astype(np.uint32)
+        dst = rng.integers(0, 1 << 22, 3000).astype(np.uint32)
+    elif case == "single":
+        src, dst = np.array([7], np.uint32), np.array([7], np.uint32)
+    else:
+        src = np.repeat(rng.integers(0, 30, 40).astype(np.uint32), 25)
+        dst = np.repeat(rng.integers(0, 30, 40).astype(np.uint32), 25)
+    g = og.Graph.from_edges(src, dst, sort)
+    o = po.build_csr(src, dst, sort, 2)
+    assert g.num_vertices == o.num_vertices
+    for k, a in g.arrays().items():
+        assert np.array_equal(a, o[k]), k
+
+
+def test_csr_build_scale22_properties():
+    """Size-independent checks at R-MAT scale 22 (E = 2^26)."""
+    scale, E = 22, 1 << 26
+    ds, dd = og.rmat_device(scale, E, seed=3)
+    g = og.Graph.build(ds, dd, E, 0)
+    src, dst = ds.to_numpy(np.uint32, E), dd.to_numpy(np.uint32, E)
+    a = g.arrays()
+    N = g.num_vertices
+    assert N == int(max(src.max(), dst.max())) + 1
+    # forward list: sorted by src, stable (dst in input order within a src run)
+    assert np.all(np.diff(a["f_src"].astype(np.int64)) >= 0)
+    order = np.argsort(src, kind="stable")
+    assert np.array_equal(a["f_dst"], dst[order]) and np.array_equal(a["f_src"], src[order])
+    # inverse list: lexicographic by (dst, src)
+    io = np.lexsort((a["f_src"], a["f_dst"]))
+    assert np.array_equal(a["i_src"], a["f_dst"][io]) and np.array_equal(a["i_dst"], a["f_src"][io])
+    deg = np.bincount(src, minlength=N)
+    assert np.array_equal(a["f_out_degree"], deg) and np.array_equal(a["i_in_degree"], deg)
+    ideg = np.bincount(dst, minlength=N)
+    assert np.array_equal(a["i_out_degree"], ideg) and np.array_equal(a["f_in_degree"], ideg)
+    nz = deg > 0
+    assert np.array_equal(a["f_edges_idx"][nz], (np.cumsum(deg) - deg)[nz])
+    assert np.all(a["f_edges_idx"][~nz] == 0)
+
+
+# ---------------------------------------------------------------------------
+# PageRank against the reference's golden outputs
+# ---------------------------------------------------------------------------
+
+def _run(g, code, eps, iters, **kw):
+    pr = og.PageRank(g, variant=code, max_iter=iters, epsilon=eps, **kw)
+    res = pr.run()
+    ranks = pr.ranks()
+    act, err = pr.rows()
+    pr.free()
+    return res, ranks, act, err
+
+
+def test_pagerank_bitexact_vs_reference(golden, graphs):
+    """Every deterministic variant, every golden case: identical rank bits,
+    iteration count and per-iteration activeVertices."""
+    n = 0
+    for c in golden["cases"]:
+        if c["code"] not in DETERMINISTIC:
+            continue
+        g = graphs(c["graph"], c["sort"])
+        res, ranks, act, err = _run(g, c["code"], c["epsilon"], c["max_iter"], hub_threshold=ALL_EXACT)
+        print(c["graph"], c["sort"], c["code"], c["epsilon"], res.iterations, flush=True)
+        assert sha(ranks) == c["ranks_sha256"], c
+        assert res.iterations == c["iterations"]
+        assert list(act) == c["active"]
+        assert res.error_total == pytest.approx(c["error_total"], rel=1e-9, abs=1e-300)
+        n += 1
+    assert n > 50
+
+
+@pytest.mark.parametrize("code", [2, 3, 4, 9, 10, 11, 12, 13])
+def test_integer_sum_variants_bitexact_with_hub_split(golden, graphs, code):
+    """Integer sums are order-free: the default plan (hub rows split into
+    chunks) stays bit-exact."""
+    for c in golden["cases"]:
+        if c["code"] != code or c["graph"] != "rmat16":
+            continue
+        res, ranks, act, _ = _run(graphs("rmat16", c["sort"]), code, c["epsilon"], c["max_iter"])
+        assert sha(ranks) == c["ranks_sha256"]
+        assert list(act) == c["active"]
+
+
+def test_pull_default_plan_within_tolerance(golden, golden_ranks, graphs):
+    """fp32 pull with hub rows summed as chunk trees: max relative error
+    <= 1e-5 against the reference's ranks."""
+    for key, ref in golden_ranks.items():
+        if not key.startswith("ranks/") or "/p0/" not in key:
+            continue
+        _, gname, sort, p, eps, it = key.split("/")
+        res, ranks, act, _ = _run(graphs(gname, int(sort[-1])), 0, float(eps[3:]), int(it[2:]),
+                                  hub_threshold=64)
+        rel = np.abs(ranks - ref) / np.maximum(ref, 1e-30)
+        assert rel.max() <= 1e-5, (key, rel.max())
+
+
+@pytest.mark.parametrize("code", [1, 5])
+def test_push_within_tolerance(golden_ranks, graphs, code):
+    for key, ref in golden_ranks.items():
+        if not key.startswith("ranks/") or f"/p{code}/" not in key:
+            continue
+        _, gname, sort, p, eps, it = key.split("/")
+        res, ranks, _, _ = _run(graphs(gname, int(sort[-1])), code, float(eps[3:]), int(it[2:]))
+        rel = np.abs(ranks - ref) / np.maximum(ref, 1e-30)
+        assert rel.max() <= 1e-5, (key, rel.max())
+
+
+def test_dropin_entry_point_matches_reference(golden, graphs):
+    """pageRankGraphCSR called with the reference's structs (the path the
+    reference's driver takes, graphRun.c:1035-1040)."""
+    for c in golden["cases"]:
+        if c["graph"] != "enron" or c["sort"] != 0 or c["code"] not in (0, 2, 9):
+            continue
+        s, d = golden_input("enron", golden)
+        o = po.build_csr(s, d, 0, 4)
+        host = og.HostGraphCSR(o.a)
+        ranks, iters, err, t = og.pagerank_graph_csr(host, c["code"], c["max_iter"], c["epsilon"])
+        assert iters == c["iterations"]
+        if c["code"] != 0:
+            assert sha(ranks) == c["ranks_sha256"]
+        else:
+            ref = po.pagerank(o, 0, c["epsilon"], c["max_iter"], 4)["ranks"]
+            assert np.max(np.abs(ranks - ref) / ref) <= 1e-5
+
+
+# ---------------------------------------------------------------------------
+# larger graphs: oracle comparisons and properties
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def rmat18():
+    s, d = og.rmat_host(18, 16 << 18, seed=6)
+    return og.Graph.from_edges(s, d, 0), po.build_csr(s, d, 0, 8)
+
+
+def test_scale18_pull_vs_oracle(rmat18):
+    g, o = rmat18
+    res, ranks, act, _ = _run(g, 0, 0.0, 15)
+    ref = po.pagerank(o, 0, 0.0, 15, 8)
+    rel = np.abs(ranks - ref["ranks"]) / ref["ranks"]
+    assert rel.max() <= 1e-5
+    assert res.iterations == 15
+
+
+def test_scale18_fixed_point_bitexact_and_pull_equals_push(rmat18):
+    g, o = rmat18
+    _, a, act_a, _ = _run(g, 2, 1e-7, 60)
+    _, b, act_b, _ = _run(g, 3, 1e-7, 60)
+    ref = po.pagerank(o, 2, 1e-7, 60, 8)
+    assert np.array_equal(a.view(np.uint32), ref["ranks"].view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert list(act_a) == list(act_b) == list(ref["active"])
+
+
+def test_scale18_uniform_start_l1_vs_fp64(rmat18):
+    g, o = rmat18
+    x, _, _ = po.pagerank_fp64(o, 0.15, 1e-13, 2000, 8)
+    res, ranks, act, err = _run(g, 0, 1e-6, 200, stop_rule=1, init_mode=1)
+    assert err[-1] < 1e-6 and (len(err) < 2 or err[-2] >= 1e-6)
+    l1 = np.abs(ranks.astype(np.float64) - x).sum()
+    print("L1 vs FP64 oracle:", l1)
+    assert l1 <= 1e-5
+
+
+def test_empty_and_degenerate_graphs():
+    for src, dst in [(np.zeros(0, np.uint32), np.zeros(0, np.uint32)),
+                     (np.array([0], np.uint32), np.array([0], np.uint32)),
+                     (np.array([5, 5, 5], np.uint32), np.array([2, 2, 2], np.uint32))]:
+        g = og.Graph.from_edges(src, dst, 0)
+        o = po.build_csr(src, dst, 0, 1)
+        for code in (0, 1, 2, 3, 9, 4):
+            res, ranks, act, _ = _run(g, code, 1e-6, 50, hub_threshold=1)
+            ref = po.pagerank(o, code, 1e-6, 50, 1)
+            assert np.allclose(ranks, ref["ranks"], rtol=1e-6), (code, ranks, ref["ranks"])
+            assert res.iterations == ref["iterations"]
+
+
+def test_all_rows_as_hubs_matches_oracle(rmat18):
+    """hub_threshold 1 sends every row with two or more in-edges through the chunk path."""
+    g, o = rmat18
+    _, ranks, _, _ = _run(g, 2, 0.0, 10, hub_threshold=1)
+    ref = po.pagerank(o, 2, 0.0, 10, 8)
+    assert np.array_equal(ranks.view(np.uint32), ref["ranks"].view(np.uint32))
+
+
+def test_relabel_off_same_result(rmat18):
+    g, _ = rmat18
+    _, a, _, _ = _run(g, 2, 0.0, 10, relabel=0)
+    _, b, _, _ = _run(g, 2, 0.0, 10, relabel=1)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_rank_label_post_sort_matches_reference(golden, golden_ranks):
+    lib = og.load()
+    import ctypes as C
+    for c in golden["cases"]:
+        if c["graph"] != "graphbrew" or c["sort"] != 0:
+            continue
+        key = f"ranks/graphbrew/sort0/p{c['code']}/eps{c['epsilon']}/it{c['max_iter']}"
+        r = np.ascontiguousarray(golden_ranks[key], np.float32)
+        lab = np.arange(len(r), dtype=np.uint32)
+        lib.radixSortEdgesByPageRank(r.ctypes.data_as(C.POINTER(C.c_float)),
+                                     lab.ctypes.data_as(C.POINTER(C.c_uint32)), len(r))
+        assert sha(lab) == c["labels_sha256"]
