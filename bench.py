@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-iterations", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-reference", action="store_true",
+                    help="after timing, run one 1 GiB device copy (og_copy_kernel): the reference "
+                         "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path does not "
@@ -87,8 +90,9 @@ def main():
     g = og.Graph.build(d_src, d_dst, E, 0)
     og.synchronize()
     t_csr = time.perf_counter() - t0
-    d_src.free()
-    d_dst.free()
+    if not args.copy_reference:
+        d_src.free()
+        d_dst.free()
     N = g.num_vertices
     log(f"R-MAT-{scale}: N={N} E={E}; generate {t_gen:.3f}s, CSR build {t_csr:.3f}s")
 
@@ -114,6 +118,11 @@ def main():
     achieved = bytes_iter / kern_s / 1e9
     log(f"pull: {step_s * 1e3:.3f} ms/iter wall, dominant kernel {kern_s * 1e3:.3f} ms, "
         f"{gteps:.1f} GTEPS, {achieved:.0f} GB/s algorithmic")
+
+    if args.copy_reference:
+        og.device_copy(d_dst, d_src, 1 << 30)
+        d_src.free()
+        d_dst.free()
 
     # convergence run (reference stop rule) for the record
     res = pr.run()

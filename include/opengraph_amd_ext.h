@@ -32,6 +32,9 @@ int ogpu_free(void *d_ptr);
 int ogpu_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
 int ogpu_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
 int ogpu_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0); the
+ * reference point for memory-traffic counters. */
+int ogpu_device_copy(void *d_dst, const void *d_src, size_t bytes);
 
 /* ---- synthetic inputs (the reference reads .bin files only) ----
  * R-MAT: edge e draws one quadrant per level from counter e*64+l of the

@@ -143,6 +143,7 @@ SIGNATURES = {
     "ogpu_memcpy_h2d": (C.c_int, [_VP, _VP, C.c_size_t]),
     "ogpu_memcpy_d2h": (C.c_int, [_VP, _VP, C.c_size_t]),
     "ogpu_synchronize": (C.c_int, []),
+    "ogpu_device_copy": (C.c_int, [_VP, _VP, C.c_size_t]),
     "ogpu_rmat_device": (C.c_int, [u32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, _VP, _VP]),
     "ogpu_rmat_host": (C.c_int, [u32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int, _VP, _VP]),
     "ogpu_er_device": (C.c_int, [u32, C.c_uint64, C.c_uint64, _VP, _VP]),
@@ -401,6 +402,10 @@ class PageRank:
 
 def synchronize():
     check(load().ogpu_synchronize())
+
+
+def device_copy(dst: DeviceArray, src: DeviceArray, nbytes: int):
+    check(load().ogpu_device_copy(dst.ptr, src.ptr, nbytes))
 
 
 # --------------------------------------------------------------------------

@@ -436,3 +436,23 @@ extern "C" int og_launch_rank_keys(const float *d_ranks, uint32_t n, uint32_t *d
                        d_ranks, n, d_keys);
     return og_hip_check(hipGetLastError(), "og_rank_keys_kernel");
 }
+
+/* ------------------------------------------------------------------ */
+/* plain copy (reference point for memory counters)                    */
+/* ------------------------------------------------------------------ */
+
+__global__ void og_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int og_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s)
+{
+    uint64_t n16 = bytes / 16;
+    if (n16 == 0) return 0;
+    hipLaunchKernelGGL(og_copy_kernel, dim3(og_grid_for(n16, OG_BLOCK * 4, 65536)), dim3(OG_BLOCK), 0, s,
+                       (const uint4 *)src, (uint4 *)dst, n16);
+    return og_hip_check(hipGetLastError(), "og_copy_kernel");
+}

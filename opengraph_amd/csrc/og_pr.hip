@@ -79,8 +79,10 @@ __device__ __forceinline__ float og_next_rank(typename og_mode<MODE>::acc s, con
 template <int MODE>
 __device__ __forceinline__ void og_store_contrib(void *c_out, uint32_t v, float r, uint32_t deg)
 {
+    /* Stores are non-temporal: the next iteration reads them back through
+     * L2, this one must not let them evict the lines being gathered. */
     if constexpr (MODE == OG_MODE_F32 || MODE >= OG_MODE_Q32)
-        ((float *)c_out)[v] = deg ? r / (float)deg : 0.0f;
+        __builtin_nontemporal_store(deg ? r / (float)deg : 0.0f, &((float *)c_out)[v]);
     else
     {
         typedef typename og_mode<MODE>::elem E;
@@ -96,7 +98,7 @@ __device__ __forceinline__ void og_store_contrib(void *c_out, uint32_t v, float 
             else
                 x = (uint8_t)og_x86_f2i32(q * 64.0f + (q >= 0 ? 0.5f : -0.5f));
         }
-        ((E *)c_out)[v] = x;
+        __builtin_nontemporal_store(x, &((E *)c_out)[v]);
     }
 }
 
@@ -143,13 +145,14 @@ __device__ __forceinline__ void og_epilogue(const og_pr_dev &a, uint32_t v,
             return;
         }
     }
-    float prev = a.r[v];
+    /* per-vertex arrays stream through once: non-temporal accesses */
+    float prev = __builtin_nontemporal_load(&a.r[v]);
+    uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
     float nr = og_next_rank<MODE>(sum, a.qp);
-    a.r[v] = nr;
+    __builtin_nontemporal_store(nr, &a.r[v]);
     double err = fabs((double)(nr - prev));
     st.err += err / (double)a.N;
     st.active += err >= a.epsilon;
-    uint32_t deg = a.outdeg[v];
     og_store_contrib<MODE>(a.c_out, v, nr, deg);
     if constexpr (MODE >= OG_MODE_Q32)
     {
@@ -250,17 +253,20 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
         }
         uint32_t eb = a.row_ptr[h] + (c - a.chunk_base[h]) * OG_TE;
         uint32_t ee = min(eb + OG_TE, a.row_ptr[h + 1]);
+        /* Branch-free: out-of-range lanes re-read the chunk's last edge and
+         * drop it, so all index loads and then all gathers are in flight
+         * together. */
         A s = 0;
         uint32_t idx[OG_TE / OG_BLOCK];
 #pragma unroll
         for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++)
-        {
-            uint32_t k = eb + u * OG_BLOCK + t;
-            idx[u] = k < ee ? __builtin_nontemporal_load(&a.cols[k]) : 0xFFFFFFFFu;
-        }
+            idx[u] = __builtin_nontemporal_load(&a.cols[min(eb + u * OG_BLOCK + t, ee - 1)]);
+        E val[OG_TE / OG_BLOCK];
+#pragma unroll
+        for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++) val[u] = g[idx[u]];
 #pragma unroll
         for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++)
-            if (idx[u] != 0xFFFFFFFFu) s += (A)g[idx[u]];
+            if (eb + u * OG_BLOCK + t < ee) s += (A)val[u];
         s = og_block_sum<A>(s);
         if (t == 0) ((A *)a.chunk_sum)[c] = s;
         return;
@@ -278,25 +284,39 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
         const uint32_t eb = a.row_ptr[rb], ne = a.row_ptr[re] - eb;
         if (ne <= OG_LDS_EDGES)
         {
+            /* Gather the window into LDS: 8 coalesced index loads, then 8
+             * independent gathers per thread in flight (branch-free; lanes
+             * past the end re-read the last edge and do not store). */
             for (uint32_t k0 = 0; k0 < ne; k0 += 8 * OG_BLOCK)
             {
                 uint32_t idx[8];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; u++)
-                {
-                    uint32_t k = k0 + u * OG_BLOCK + t;
-                    idx[u] = k < ne ? __builtin_nontemporal_load(&a.cols[eb + k]) : 0xFFFFFFFFu;
-                }
+                    idx[u] = __builtin_nontemporal_load(&a.cols[eb + min(k0 + u * OG_BLOCK + t, ne - 1)]);
+                E val[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; u++) val[u] = g[idx[u]];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; u++)
-                    if (idx[u] != 0xFFFFFFFFu) lds[k0 + u * OG_BLOCK + t] = g[idx[u]];
+                    if (k0 + u * OG_BLOCK + t < ne) lds[k0 + u * OG_BLOCK + t] = val[u];
             }
             __syncthreads();
+            /* Each thread sums whole rows in CSR order; LDS reads are issued
+             * eight at a time ahead of the (sequential) adds. */
             for (uint32_t v = rb + t; v < re; v += OG_BLOCK)
             {
-                uint32_t b = a.row_ptr[v] - eb, e = a.row_ptr[v + 1] - eb;
+                uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]) - eb;
+                uint32_t e = __builtin_nontemporal_load(&a.row_ptr[v + 1]) - eb;
                 A s = 0;
-                for (uint32_t j = b; j < e; j++) s += (A)lds[j];
+                for (; j + 8 <= e; j += 8)
+                {
+                    E x[8];
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; u++) x[u] = lds[j + u];
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; u++) s += (A)x[u];
+                }
+                for (; j < e; j++) s += (A)lds[j];
                 og_epilogue<MODE>(a, v, s, st);
             }
             __syncthreads();

@@ -146,6 +146,20 @@ int ogpu_memcpy_d2h(void *dst, const void *src, size_t bytes)
     return og_hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
 }
 
+int og_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+
+int ogpu_device_copy(void *d_dst, const void *d_src, size_t bytes)
+{
+    if (og_ctx_ensure()) return -1;
+    if (bytes % 16)
+    {
+        og_set_error("ogpu_device_copy: size %zu is not a multiple of 16", bytes);
+        return -1;
+    }
+    if (og_launch_copy(d_src, d_dst, bytes, g_stream)) return -1;
+    return og_hip_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+}
+
 int ogpu_synchronize(void)
 {
     if (og_ctx_ensure()) return -1;
