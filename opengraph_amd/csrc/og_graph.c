@@ -158,7 +158,7 @@ void og_plan_destroy(og_plan *p)
 {
     if (!p) return;
     og_dfree(p->new2old); og_dfree(p->old2new); og_dfree(p->row_ptr); og_dfree(p->cols);
-    og_dfree(p->outdeg); og_dfree(p->chunk_base); og_dfree(p->win_row);
+    og_dfree(p->outdeg); og_dfree(p->chunk_base); og_dfree(p->win_row); og_dfree(p->win_edge);
     og_dfree(p->push_src); og_dfree(p->push_dst); og_dfree(p->f_row_ptr); og_dfree(p->f_cols);
     free(p);
 }
@@ -235,14 +235,18 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_
     p->n_chunks = n_chunks;
     /* 6. LDS windows over the non-hub rows */
     {
-        uint32_t e_norm = E - base_edge;
-        uint32_t nw = (e_norm + OG_TE - 1) / OG_TE;
-        if (nw == 0 && N > H) nw = 1;
+        /* measure: edges + OG_ROW_W per row, so a window holds < OG_TE/OG_ROW_W rows */
+        uint64_t m_norm = (uint64_t)(E - base_edge) + (uint64_t)OG_ROW_W * (N - H);
+        uint64_t nw64 = (m_norm + OG_TE - 1) / OG_TE;
+        if (nw64 == 0 && N > H) nw64 = 1;
+        uint32_t nw = (uint32_t)nw64;
         p->n_windows = nw;
         p->n_tile_blocks = (nw + OG_WIN_PER_BLOCK - 1) / OG_WIN_PER_BLOCK;
         p->win_row = og_dalloc((size_t)(nw + 1) * 4);
-        if (!p->win_row) goto done;
-        if (og_launch_plan_windows(H, N, p->row_ptr, OG_TE, nw, p->win_row, s)) goto done;
+        p->win_edge = og_dalloc((size_t)(nw + 1) * 4);
+        if (!p->win_row || !p->win_edge) goto done;
+        if (og_launch_plan_windows(H, N, p->row_ptr, OG_TE, OG_ROW_W, nw, p->win_row, p->win_edge, s))
+            goto done;
     }
     /* 7. forward edges in new ids (push) and the forward CSR in new ids
      *    (data-driven activation of out-neighbours) */

@@ -81,8 +81,9 @@ void og_graph_destroy(og_graph *g);
 
 /* ---------------- PageRank plan / run (og_pagerank.c, og_pr.hip) ---------------- */
 
-#define OG_TE 2048u          /* edges per LDS window of the pull tiles */
-#define OG_WIN_PER_BLOCK 4u  /* windows processed by one tile block */
+#define OG_TE 2048u          /* window size in edges + OG_ROW_W per row */
+#define OG_ROW_W 4u          /* so a window holds < OG_TE / OG_ROW_W = 512 rows */
+#define OG_WIN_PER_BLOCK 8u  /* windows processed by one tile block */
 
 typedef struct og_plan
 {
@@ -98,6 +99,7 @@ typedef struct og_plan
     uint32_t *chunk_base;        /* H+1 prefix of chunk counts */
     uint32_t n_windows;          /* windows over rows [H, N) */
     uint32_t *win_row;           /* n_windows+1 first row of each window */
+    uint32_t *win_edge;          /* n_windows+1 row_ptr[win_row[k]] */
     uint32_t n_tile_blocks;
     /* forward side for push: source and destination of each forward edge, new ids */
     uint32_t *push_src, *push_dst;

@@ -250,6 +250,7 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->outdeg = pl->outdeg;
     a->chunk_base = pl->chunk_base;
     a->win_row = pl->win_row;
+    a->win_edge = pl->win_edge;
     a->new2old = pl->new2old;
     a->old2new = pl->old2new;
     a->push_src = pl->push_src;

@@ -28,7 +28,8 @@ int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2old, const u
 int og_launch_plan_chunks(uint32_t H, const uint32_t *row_ptr, uint32_t te, uint32_t *cnt,
                           hipStream_t s);
 int og_launch_plan_windows(uint32_t H, uint32_t N, const uint32_t *row_ptr, uint32_t te,
-                           uint32_t n_windows, uint32_t *win_row, hipStream_t s);
+                           uint32_t rw, uint32_t n_windows, uint32_t *win_row, uint32_t *win_edge,
+                           hipStream_t s);
 int og_launch_plan_map_pairs(uint32_t E, const uint32_t *a, const uint32_t *b,
                              const uint32_t *old2new, uint32_t *oa, uint32_t *ob, hipStream_t s);
 

@@ -196,24 +196,33 @@ __global__ void og_plan_chunk_count_kernel(uint32_t H, const uint32_t *__restric
         cnt[h] = (row_ptr[h + 1] - row_ptr[h] + te - 1) / te;
 }
 
-/* win_row[k] = first row r in [H, N) whose row_ptr[r] >= base + k*te
- * (k < n_windows); win_row[n_windows] = N. */
+/* Windows cut the rows [H, N) by the measure m(r) = row_ptr[r] + rw * r
+ * (edges plus rw per row): win_row[k] = first row r with
+ * m(r) >= m(H) + k*te (k < n_windows), win_row[n_windows] = N.  A window
+ * then holds fewer than te/rw rows and fewer than te + (largest row) edges. */
 __global__ void og_plan_windows_kernel(uint32_t H, uint32_t N, const uint32_t *__restrict__ row_ptr,
-                                       uint32_t te, uint32_t n_windows, uint32_t *__restrict__ win_row)
+                                       uint32_t te, uint32_t rw, uint32_t n_windows,
+                                       uint32_t *__restrict__ win_row, uint32_t *__restrict__ win_edge)
 {
-    const uint32_t base = row_ptr[H];
+    const uint64_t base = (uint64_t)row_ptr[H] + (uint64_t)rw * H;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n_windows; k += gridDim.x * blockDim.x)
     {
-        if (k == n_windows) { win_row[k] = N; continue; }
-        if (k == 0) { win_row[k] = H; continue; }
-        uint64_t target = (uint64_t)base + (uint64_t)k * te;
-        uint32_t lo = H, hi = N;      /* first r in [lo, hi) with row_ptr[r] >= target */
+        if (k == n_windows || k == 0)
+        {
+            uint32_t r = k == 0 ? H : N;
+            win_row[k] = r;
+            win_edge[k] = row_ptr[r];
+            continue;
+        }
+        uint64_t target = base + (uint64_t)k * te;
+        uint32_t lo = H, hi = N;      /* first r in [lo, hi) with m(r) >= target */
         while (lo < hi)
         {
             uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)row_ptr[mid] >= target) hi = mid; else lo = mid + 1;
+            if ((uint64_t)row_ptr[mid] + (uint64_t)rw * mid >= target) hi = mid; else lo = mid + 1;
         }
         win_row[k] = lo;
+        win_edge[k] = row_ptr[lo];
     }
 }
 
@@ -283,10 +292,11 @@ extern "C" int og_launch_plan_chunks(uint32_t H, const uint32_t *row_ptr, uint32
 }
 
 extern "C" int og_launch_plan_windows(uint32_t H, uint32_t N, const uint32_t *row_ptr, uint32_t te,
-                                      uint32_t n_windows, uint32_t *win_row, hipStream_t s)
+                                      uint32_t rw, uint32_t n_windows, uint32_t *win_row,
+                                      uint32_t *win_edge, hipStream_t s)
 {
     hipLaunchKernelGGL(og_plan_windows_kernel, dim3(og_grid((uint64_t)n_windows + 1, 16384)),
-                       dim3(OG_BLOCK), 0, s, H, N, row_ptr, te, n_windows, win_row);
+                       dim3(OG_BLOCK), 0, s, H, N, row_ptr, te, rw, n_windows, win_row, win_edge);
     return og_hip_check(hipGetLastError(), "og_plan_windows_kernel");
 }
 

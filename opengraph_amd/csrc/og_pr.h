@@ -52,7 +52,7 @@ typedef struct og_pr_dev
     uint32_t allow_stop;     /* final reduction may raise *done */
     double epsilon;
     /* plan */
-    const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *win_row;
+    const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *win_row, *win_edge;
     const uint32_t *new2old, *old2new;
     const uint32_t *push_src, *push_dst;
     /* state */

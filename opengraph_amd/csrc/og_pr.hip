@@ -118,6 +118,11 @@ __device__ __forceinline__ void og_stat_init(og_stat &s)
     s.cmax = -3.402823466e38f;
 }
 
+template <int MODE>
+__device__ __forceinline__ void og_epilogue_pf(const og_pr_dev &a, uint32_t v,
+                                               typename og_mode<MODE>::acc sum, float prev,
+                                               uint32_t deg, og_stat &st);
+
 /* pageRank.c:1016-1027: update, |delta| in double, error_total += err/N,
  * activeVertices += err >= epsilon; then the next contribution. */
 template <int MODE>
@@ -148,6 +153,16 @@ __device__ __forceinline__ void og_epilogue(const og_pr_dev &a, uint32_t v,
     /* per-vertex arrays stream through once: non-temporal accesses */
     float prev = __builtin_nontemporal_load(&a.r[v]);
     uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
+    og_epilogue_pf<MODE>(a, v, sum, prev, deg, st);
+}
+
+/* The same with the old rank and out-degree already loaded (the windowed
+ * pull prefetches them with the window's gathers). */
+template <int MODE>
+__device__ __forceinline__ void og_epilogue_pf(const og_pr_dev &a, uint32_t v,
+                                               typename og_mode<MODE>::acc sum, float prev,
+                                               uint32_t deg, og_stat &st)
+{
     float nr = og_next_rank<MODE>(sum, a.qp);
     __builtin_nontemporal_store(nr, &a.r[v]);
     double err = fabs((double)(nr - prev));
@@ -225,6 +240,25 @@ template <typename T> __device__ T og_block_sum(T v)
 /* pull iteration                                                      */
 /* ------------------------------------------------------------------ */
 
+/* Issue the index loads, then the gathers, of a window of ne <= LDS_EDGES
+ * edges starting at eb: slot u of thread t holds edge u*BLOCK + t.  The
+ * slot guards are block-uniform, lanes past the end re-read the last edge,
+ * so all loads of a thread are in flight together. */
+template <typename E>
+__device__ __forceinline__ void og_window_issue(const uint32_t *__restrict__ cols,
+                                                const E *__restrict__ g, uint32_t eb, uint32_t ne,
+                                                uint32_t t, E (&val)[OG_LDS_EDGES / OG_BLOCK])
+{
+    constexpr uint32_t SLOTS = OG_LDS_EDGES / OG_BLOCK;
+    uint32_t idx[SLOTS];
+#pragma unroll
+    for (uint32_t u = 0; u < SLOTS; u++)
+        if (u * OG_BLOCK < ne) idx[u] = __builtin_nontemporal_load(&cols[eb + min(u * OG_BLOCK + t, ne - 1)]);
+#pragma unroll
+    for (uint32_t u = 0; u < SLOTS; u++)
+        if (u * OG_BLOCK < ne) val[u] = g[idx[u]];
+}
+
 template <int MODE>
 __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t it)
 {
@@ -272,54 +306,95 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
         return;
     }
 
-    __shared__ E lds[OG_LDS_EDGES];
+    /* Windows are software-pipelined through two LDS buffers: the index
+     * loads and gathers of window w+1 -- and the row offsets, old ranks and
+     * out-degrees of its (at most 2 per thread) rows -- are issued before
+     * window w is summed, so the sums and epilogues of w run on LDS and
+     * registers while w+1's traffic is in flight. */
+    constexpr uint32_t SLOTS = OG_LDS_EDGES / OG_BLOCK;
+    constexpr uint32_t RSLOTS = OG_TE / OG_ROW_W / OG_BLOCK;
+    __shared__ E lds[2][OG_LDS_EDGES];
+    __shared__ uint32_t s_wrow[OG_WIN_PER_BLOCK + 1], s_wedge[OG_WIN_PER_BLOCK + 1];
     og_stat st;
     og_stat_init(st);
     const uint32_t tb = blockIdx.x - a.n_chunks;
     const uint32_t w0 = tb * OG_WIN_PER_BLOCK;
-    const uint32_t w1 = min(w0 + OG_WIN_PER_BLOCK, a.n_windows);
-    for (uint32_t w = w0; w < w1; w++)
+    const uint32_t nw = min(OG_WIN_PER_BLOCK, a.n_windows - w0);
+    if (t <= nw)
     {
-        const uint32_t rb = a.win_row[w], re = a.win_row[w + 1];
-        const uint32_t eb = a.row_ptr[rb], ne = a.row_ptr[re] - eb;
-        if (ne <= OG_LDS_EDGES)
+        s_wrow[t] = a.win_row[w0 + t];
+        s_wedge[t] = a.win_edge[w0 + t];
+    }
+    __syncthreads();
+    E val[SLOTS];
+    uint32_t nj[RSLOTS], ne_[RSLOTS], nd[RSLOTS];
+    float nr[RSLOTS];
+    auto issue = [&](uint32_t k) {
+        uint32_t rb = s_wrow[k], re = s_wrow[k + 1];
+        uint32_t eb = s_wedge[k], ne = s_wedge[k + 1] - eb;
+        if (ne <= OG_LDS_EDGES) og_window_issue<E>(a.cols, g, eb, ne, t, val);
+#pragma unroll
+        for (uint32_t q = 0; q < RSLOTS; q++)
         {
-            /* Gather the window into LDS: 8 coalesced index loads, then 8
-             * independent gathers per thread in flight (branch-free; lanes
-             * past the end re-read the last edge and do not store). */
-            for (uint32_t k0 = 0; k0 < ne; k0 += 8 * OG_BLOCK)
+            uint32_t v = rb + q * OG_BLOCK + t;
+            if (v < re)
             {
-                uint32_t idx[8];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; u++)
-                    idx[u] = __builtin_nontemporal_load(&a.cols[eb + min(k0 + u * OG_BLOCK + t, ne - 1)]);
-                E val[8];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; u++) val[u] = g[idx[u]];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; u++)
-                    if (k0 + u * OG_BLOCK + t < ne) lds[k0 + u * OG_BLOCK + t] = val[u];
+                nj[q] = __builtin_nontemporal_load(&a.row_ptr[v]);
+                ne_[q] = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
+                nr[q] = __builtin_nontemporal_load(&a.r[v]);
+                nd[q] = __builtin_nontemporal_load(&a.outdeg[v]);
             }
-            __syncthreads();
+        }
+    };
+    issue(0);
+    uint32_t buf = 0;
+    for (uint32_t k = 0; k < nw; k++)
+    {
+        const uint32_t rb = s_wrow[k], re = s_wrow[k + 1];
+        const uint32_t eb = s_wedge[k], ne = s_wedge[k + 1] - eb;
+        const bool in_lds = ne <= OG_LDS_EDGES;
+        if (in_lds)
+        {
+#pragma unroll
+            for (uint32_t u = 0; u < SLOTS; u++)
+                if (u * OG_BLOCK + t < ne) lds[buf][u * OG_BLOCK + t] = val[u];
+        }
+        uint32_t cj[RSLOTS], ce[RSLOTS], cd[RSLOTS];
+        float cr[RSLOTS];
+#pragma unroll
+        for (uint32_t q = 0; q < RSLOTS; q++)
+        {
+            cj[q] = nj[q];
+            ce[q] = ne_[q];
+            cr[q] = nr[q];
+            cd[q] = nd[q];
+        }
+        __syncthreads();
+        if (k + 1 < nw) issue(k + 1);
+        if (in_lds)
+        {
             /* Each thread sums whole rows in CSR order; LDS reads are issued
              * eight at a time ahead of the (sequential) adds. */
-            for (uint32_t v = rb + t; v < re; v += OG_BLOCK)
+            const E *L = lds[buf];
+#pragma unroll
+            for (uint32_t q = 0; q < RSLOTS; q++)
             {
-                uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]) - eb;
-                uint32_t e = __builtin_nontemporal_load(&a.row_ptr[v + 1]) - eb;
+                uint32_t v = rb + q * OG_BLOCK + t;
+                if (v >= re) break;
+                uint32_t j = cj[q] - eb, e = ce[q] - eb;
                 A s = 0;
                 for (; j + 8 <= e; j += 8)
                 {
                     E x[8];
 #pragma unroll
-                    for (uint32_t u = 0; u < 8; u++) x[u] = lds[j + u];
+                    for (uint32_t u = 0; u < 8; u++) x[u] = L[j + u];
 #pragma unroll
                     for (uint32_t u = 0; u < 8; u++) s += (A)x[u];
                 }
-                for (; j < e; j++) s += (A)lds[j];
-                og_epilogue<MODE>(a, v, s, st);
+                for (; j < e; j++) s += (A)L[j];
+                if (a.dd) og_epilogue<MODE>(a, v, s, st);
+                else og_epilogue_pf<MODE>(a, v, s, cr[q], cd[q], st);
             }
-            __syncthreads();
         }
         else
         {
@@ -331,6 +406,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
                 og_epilogue<MODE>(a, v, s, st);
             }
         }
+        buf ^= 1;
     }
     og_stat r = og_block_stat(st);
     if (t == 0)
