@@ -82,9 +82,16 @@ typedef struct
     uint32_t stop_rule;     /* 0: activeVertices == 0 (reference); 1: error_total < epsilon */
     uint32_t init_mode;     /* 0: 0.15 (reference); 1: uniform 1.0 (= 1/N normalized);
                                2: caller's vector (unnormalized, old ids) */
-    uint32_t relabel;       /* 1: degree-ordered rows and columns (default); 0: hubs first only */
-    uint32_t hub_threshold; /* rows with more in-edges are split into chunks; 0: default */
-    uint32_t reserved;
+    uint32_t relabel;       /* 1: degree-ordered rows and columns (default); 0: rows grouped
+                               by in-degree class only, original order within a class */
+    uint32_t hub_threshold; /* rows with more in-edges are split into 512-edge chunks summed
+                               in a fixed tree; rows with at most this many are summed in
+                               the reference's order (bit-exact); 0: default 32;
+                               0xFFFFFFFF: every row in the reference's order */
+    uint32_t ppr;           /* 1: personalized PageRank (variant 0 only): start at the one-hot
+                               e_s, x' = (1-d)[v==s] + d * sum, stop when the L1 change
+                               sum |x'-x| < epsilon, ranks not divided by N */
+    uint32_t ppr_source;    /* s, the caller's vertex id */
 } ogpu_pr_options;
 
 typedef struct
@@ -119,6 +126,39 @@ int ogpu_pr_rows(ogpu_pr *pr, uint32_t *active, double *error_total, uint32_t ma
  * 8E + 20N). */
 double ogpu_pr_bytes_per_iteration(const ogpu_pr *pr);
 void ogpu_pr_free(ogpu_pr *pr);
+
+/* ---- batched personalized PageRank (no counterpart in the reference) ----
+ * k <= 64 restart vertices iterated together as a pull-SpMM over an N x k
+ * fp32 row-major block: per column exactly the personalized option of
+ * ogpu_pr (variant 0, ppr = 1) -- x0 = e_s, x' = (1-d)[v==s] + d * sum of
+ * x/outdeg over in-neighbours, column stops after the first iteration with
+ * L1 change < epsilon -- with rows of at most 512 in-edges summed in CSR
+ * order (bit-exact with the single-source run at hub_threshold 0xFFFFFFFF)
+ * and longer rows summed in 512-edge chunks added in order. */
+typedef struct ogpu_ppr ogpu_ppr;
+
+typedef struct
+{
+    uint32_t iterations;      /* iterations executed (the slowest column's count) */
+    uint32_t converged;       /* columns that reached L1 < epsilon */
+    double seconds_total;     /* init + iterations, host clock */
+    double seconds_plan;
+    double kernel_ms_per_iter;/* mean duration of the SpMM kernel, HIP events */
+} ogpu_ppr_result;
+
+/* sources: k vertex ids of the caller (old ids). */
+int ogpu_ppr_create(ogpu_graph *g, const uint32_t *sources, uint32_t k, uint32_t max_iter,
+                    double epsilon, ogpu_ppr **out);
+int ogpu_ppr_run(ogpu_ppr *p, ogpu_ppr_result *res);
+/* x_out: N x k, row-major, caller's vertex order (not divided by N). */
+int ogpu_ppr_ranks(ogpu_ppr *p, float *x_out);
+/* iters[k]: iterations applied per column; l1_hist[rows x k]: each
+ * iteration's L1 change per column (0 after the column stopped).  Returns
+ * the rows written (<= max_rows) or -1. */
+int ogpu_ppr_columns(ogpu_ppr *p, uint32_t *iters, double *l1_hist, uint32_t max_rows);
+/* Algorithmic bytes of one iteration: E (4 + 4k) + N (8 + 12k). */
+double ogpu_ppr_bytes_per_iteration(const ogpu_ppr *p);
+void ogpu_ppr_free(ogpu_ppr *p);
 
 #ifdef __cplusplus
 }

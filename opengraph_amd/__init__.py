@@ -88,7 +88,8 @@ class PrOptions(C.Structure):
     """ogpu_pr_options (opengraph_amd_ext.h)."""
     _fields_ = [
         ("variant", u32), ("max_iter", u32), ("epsilon", C.c_double), ("stop_rule", u32),
-        ("init_mode", u32), ("relabel", u32), ("hub_threshold", u32), ("reserved", u32),
+        ("init_mode", u32), ("relabel", u32), ("hub_threshold", u32), ("ppr", u32),
+        ("ppr_source", u32),
     ]
 
 
@@ -98,6 +99,14 @@ class PrResult(C.Structure):
         ("iterations", u32), ("rows", u32), ("active", u32), ("pad", u32),
         ("error_total", C.c_double), ("pr_sum", C.c_double), ("seconds_total", C.c_double),
         ("seconds_plan", C.c_double),
+    ]
+
+
+class PprResult(C.Structure):
+    """ogpu_ppr_result (opengraph_amd_ext.h)."""
+    _fields_ = [
+        ("iterations", u32), ("converged", u32), ("seconds_total", C.c_double),
+        ("seconds_plan", C.c_double), ("kernel_ms_per_iter", C.c_double),
     ]
 
 
@@ -165,6 +174,12 @@ SIGNATURES = {
     "ogpu_pr_rows": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
     "ogpu_pr_bytes_per_iteration": (C.c_double, [_VP]),
     "ogpu_pr_free": (None, [_VP]),
+    "ogpu_ppr_create": (C.c_int, [_VP, u32p, u32, u32, C.c_double, C.POINTER(_VP)]),
+    "ogpu_ppr_run": (C.c_int, [_VP, C.POINTER(PprResult)]),
+    "ogpu_ppr_ranks": (C.c_int, [_VP, _VP]),
+    "ogpu_ppr_columns": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
+    "ogpu_ppr_bytes_per_iteration": (C.c_double, [_VP]),
+    "ogpu_ppr_free": (None, [_VP]),
 }
 
 GRAPH_ARRAYS = ["f_src", "f_dst", "f_out_degree", "f_in_degree", "f_edges_idx",
@@ -341,12 +356,14 @@ class PageRank:
     """A prepared PageRank run on a device graph (ogpu_pr)."""
 
     def __init__(self, graph: Graph, variant=0, max_iter=20, epsilon=1e-4, stop_rule=0,
-                 init_mode=0, relabel=1, hub_threshold=0, init=None):
+                 init_mode=0, relabel=1, hub_threshold=0, init=None, ppr_source=None):
         lib = load()
         o = PrOptions()
         lib.ogpu_pr_default_options(C.byref(o))
         o.variant, o.max_iter, o.epsilon = variant, max_iter, epsilon
         o.stop_rule, o.init_mode, o.relabel, o.hub_threshold = stop_rule, init_mode, relabel, hub_threshold
+        if ppr_source is not None:
+            o.ppr, o.ppr_source = 1, ppr_source
         self.graph = graph
         self.h = C.c_void_p()
         check(lib.ogpu_pr_create(graph.h, C.byref(o), C.byref(self.h)))
@@ -391,6 +408,52 @@ class PageRank:
     def free(self):
         if self.h:
             load().ogpu_pr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Ppr:
+    """Batched personalized PageRank: k <= 64 sources in one N x k block (ogpu_ppr)."""
+
+    def __init__(self, graph: Graph, sources, max_iter=100, epsilon=1e-6):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        self.graph, self.k = graph, len(src)
+        self.h = C.c_void_p()
+        check(load().ogpu_ppr_create(graph.h, src.ctypes.data_as(u32p), self.k, max_iter, epsilon,
+                                     C.byref(self.h)))
+
+    def run(self) -> PprResult:
+        r = PprResult()
+        check(load().ogpu_ppr_run(self.h, C.byref(r)))
+        return r
+
+    def ranks(self) -> np.ndarray:
+        """N x k, caller's vertex order."""
+        out = np.empty((self.graph.num_vertices, self.k), np.float32)
+        check(load().ogpu_ppr_ranks(self.h, _ptr(out)))
+        return out
+
+    def columns(self, max_rows=100000):
+        """(iterations per column, L1 history rows x k)."""
+        it = np.zeros(self.k, np.uint32)
+        hist = np.zeros((max_rows, self.k), np.float64)
+        n = load().ogpu_ppr_columns(self.h, it.ctypes.data_as(u32p),
+                                    hist.ctypes.data_as(C.POINTER(C.c_double)), max_rows)
+        if n < 0:
+            check(n)
+        return it, hist[:n]
+
+    def bytes_per_iteration(self):
+        return load().ogpu_ppr_bytes_per_iteration(self.h)
+
+    def free(self):
+        if self.h:
+            load().ogpu_ppr_free(self.h)
             self.h = None
 
     def __del__(self):

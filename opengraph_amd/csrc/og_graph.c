@@ -158,12 +158,12 @@ void og_plan_destroy(og_plan *p)
 {
     if (!p) return;
     og_dfree(p->new2old); og_dfree(p->old2new); og_dfree(p->row_ptr); og_dfree(p->cols);
-    og_dfree(p->outdeg); og_dfree(p->chunk_base); og_dfree(p->win_row); og_dfree(p->win_edge);
+    og_dfree(p->outdeg); og_dfree(p->chunk_base); og_dfree(p->ch_row); og_dfree(p->ch_edge);
     og_dfree(p->push_src); og_dfree(p->push_dst); og_dfree(p->f_row_ptr); og_dfree(p->f_cols);
     free(p);
 }
 
-int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_forward,
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_forward,
                   og_plan **out)
 {
     *out = NULL;
@@ -175,7 +175,7 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_
     p->N = N;
     p->E = E;
     p->relabel = relabel;
-    p->hub_t = hub_t;
+    p->seq_t = seq_t;
     size_t nb = (size_t)N * 4;
     uint32_t *key = og_dalloc(nb), *key2 = og_dalloc(nb), *val2 = og_dalloc(nb);
     uint32_t *indeg_new = og_dalloc(nb + 4), *d_cnt = og_dalloc(16);
@@ -192,13 +192,14 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_
         !p->outdeg || !p->row_ptr || !p->cols)
         goto done;
 
-    /* 1. new order: hubs first, then by out-degree descending (stable) */
-    if (og_launch_plan_keys(N, g->f_out_degree, g->i_out_degree, hub_t, relabel, key, p->new2old, s))
+    /* 1. new order: by row class, then out-degree (in-degree for the rows
+     *    one thread sums) descending, stable */
+    if (og_launch_plan_keys(N, g->f_out_degree, g->i_out_degree, seq_t, relabel, key, p->new2old, s))
         goto done;
     {
         uint32_t *ka = key2, *va = val2, *kb = key, *vb = p->new2old;
         int in_b = 0;
-        if (og_radix_sort_pairs(key, p->new2old, ka, va, kb, vb, N, relabel ? 32 : 1, scratch, s, &in_b))
+        if (og_radix_sort_pairs(key, p->new2old, ka, va, kb, vb, N, relabel ? 32 : 2, scratch, s, &in_b))
             goto done;
         if (!in_b)
         {
@@ -206,47 +207,52 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_
             uint32_t *t = p->new2old; p->new2old = val2; val2 = t;
         }
     }
-    /* 2. inverse permutation, degrees in new order, hub count */
+    /* 2. inverse permutation, degrees in new order, class sizes */
     if (og_launch_fill_u32(d_cnt, 0, 4, s)) goto done;
-    if (og_launch_plan_perm(N, p->new2old, g->f_out_degree, g->i_out_degree, hub_t, p->old2new,
+    if (og_launch_plan_perm(N, p->new2old, g->f_out_degree, g->i_out_degree, seq_t, p->old2new,
                             p->outdeg, indeg_new, d_cnt, s))
         goto done;
     /* 3. row pointers of the relabelled inverse CSR */
     if (og_scan_exclusive(indeg_new, p->row_ptr, N, scratch, s)) goto done;
-    uint32_t H = 0;
-    if (og_hip_check(hipMemcpyAsync(&H, d_cnt, 4, hipMemcpyDeviceToHost, s), "copy hub count") ||
+    uint32_t cls[4] = {0, 0, 0, 0};
+    if (og_hip_check(hipMemcpyAsync(cls, d_cnt, 16, hipMemcpyDeviceToHost, s), "copy class sizes") ||
         og_hip_check(hipStreamSynchronize(s), "sync plan"))
         goto done;
+    const uint32_t H = cls[0], C = H + cls[1];
     p->H = H;
+    p->C = C;
+    p->Z = C + cls[2];
     /* 4. relabelled in-neighbour lists, reference order kept */
     if (E && og_launch_plan_copy(H, N, p->new2old, p->old2new, g->i_edges_idx, g->i_dst, p->row_ptr,
                                  p->cols, s))
         goto done;
-    /* 5. hub chunks */
-    p->chunk_base = og_dalloc((size_t)(H + 1) * 4);
-    if (!p->chunk_base) goto done;
-    if (og_launch_plan_chunks(H, p->row_ptr, OG_TE, key, s)) goto done;
-    if (og_scan_exclusive(key, p->chunk_base, H, scratch, s)) goto done;
-    uint32_t base_edge = 0, n_chunks = 0;
-    if (og_hip_check(hipMemcpyAsync(&base_edge, p->row_ptr + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
-        og_hip_check(hipMemcpyAsync(&n_chunks, p->chunk_base + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
-        og_hip_check(hipStreamSynchronize(s), "sync plan"))
-        goto done;
-    p->n_chunks = n_chunks;
-    /* 6. LDS windows over the non-hub rows */
+    /* 5. chunks of the rows [0, C): key = chunks per row, chunk_first = its prefix */
     {
-        /* measure: edges + OG_ROW_W per row, so a window holds < OG_TE/OG_ROW_W rows */
-        uint64_t m_norm = (uint64_t)(E - base_edge) + (uint64_t)OG_ROW_W * (N - H);
-        uint64_t nw64 = (m_norm + OG_TE - 1) / OG_TE;
-        if (nw64 == 0 && N > H) nw64 = 1;
-        uint32_t nw = (uint32_t)nw64;
-        p->n_windows = nw;
-        p->n_tile_blocks = (nw + OG_WIN_PER_BLOCK - 1) / OG_WIN_PER_BLOCK;
-        p->win_row = og_dalloc((size_t)(nw + 1) * 4);
-        p->win_edge = og_dalloc((size_t)(nw + 1) * 4);
-        if (!p->win_row || !p->win_edge) goto done;
-        if (og_launch_plan_windows(H, N, p->row_ptr, OG_TE, OG_ROW_W, nw, p->win_row, p->win_edge, s))
+        uint32_t *first = og_dalloc((size_t)(C + 1) * 4);
+        if (!first) goto done;
+        uint32_t n_chunks = 0;
+        if (og_launch_plan_chunks(C, p->row_ptr, OG_CH, key, s) ||
+            og_scan_exclusive(key, first, C, scratch, s) ||
+            og_hip_check(hipMemcpyAsync(&n_chunks, first + C, 4, hipMemcpyDeviceToHost, s), "copy") ||
+            og_hip_check(hipStreamSynchronize(s), "sync plan"))
+        {
+            og_dfree(first);
             goto done;
+        }
+        p->n_chunks = n_chunks;
+        p->chunk_base = og_dalloc((size_t)(H + 1) * 4);
+        p->ch_row = og_dalloc((size_t)n_chunks * 4 + 4);
+        p->ch_edge = og_dalloc((size_t)(n_chunks + 1) * 4);
+        int bad = !p->chunk_base || !p->ch_row || !p->ch_edge ||
+                  og_hip_check(hipMemcpyAsync(p->chunk_base, first, (size_t)(H + 1) * 4,
+                                              hipMemcpyDeviceToDevice, s), "copy chunk_base") ||
+                  og_launch_plan_chunk_desc(H, C, p->row_ptr, first, OG_CH, p->ch_row, p->ch_edge, s) ||
+                  og_hip_check(hipStreamSynchronize(s), "sync plan");
+        og_dfree(first);
+        if (bad) goto done;
+        uint32_t waves = (n_chunks + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE;
+        p->n_chunk_blocks = (waves + 3) / 4;
+        p->n_row_blocks = (uint32_t)(((uint64_t)(N - C) + 255) / 256);
     }
     /* 7. forward edges in new ids (push) and the forward CSR in new ids
      *    (data-driven activation of out-neighbours) */

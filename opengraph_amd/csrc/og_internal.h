@@ -81,26 +81,37 @@ void og_graph_destroy(og_graph *g);
 
 /* ---------------- PageRank plan / run (og_pagerank.c, og_pr.hip) ---------------- */
 
-#define OG_TE 2048u          /* window size in edges + OG_ROW_W per row */
-#define OG_ROW_W 4u          /* so a window holds < OG_TE / OG_ROW_W = 512 rows */
-#define OG_WIN_PER_BLOCK 8u  /* windows processed by one tile block */
+/* Pull plan.  Rows (new ids) fall in four classes, in this order:
+ *   [0, H)  more than OG_CH in-edges: several chunks of OG_CH edges, each
+ *           summed by one wavefront; the finalize kernel adds the chunk
+ *           sums of a row in chunk order.
+ *   [H, C)  more than seq_t and at most OG_CH in-edges: one chunk, whose
+ *           wavefront also runs the row's epilogue.
+ *   [C, Z)  1..seq_t in-edges: one thread sums the row in CSR order.
+ *   [Z, N)  no in-edges: epilogue only.
+ * Rows with at most seq_t in-edges therefore add in exactly the reference's
+ * order; chunked rows add in a fixed tree (deterministic). */
+#define OG_CH 512u            /* edges per wavefront chunk: 64 lanes x 8 */
+#define OG_CH_PER_WAVE 4u     /* consecutive chunks one wavefront processes */
+#define OG_SEQ_T 32u          /* default seq_t */
+#define OG_CH_MULTI 0x80000000u /* ch_row flag: the row has several chunks */
 
 typedef struct og_plan
 {
     uint32_t N, E;
     uint32_t relabel;
-    uint32_t hub_t;              /* rows with more in-edges are split into chunks */
+    uint32_t seq_t;              /* rows with more in-edges are chunked */
     uint32_t *new2old, *old2new; /* N each */
     uint32_t *row_ptr;           /* N+1, rows in new order (inverse CSR) */
     uint32_t *cols;              /* E, in-neighbours as new ids, reference order kept */
     uint32_t *outdeg;            /* N, forward out-degree in new order */
-    uint32_t H;                  /* hub rows are new ids [0, H) */
-    uint32_t n_chunks;           /* total hub chunks */
-    uint32_t *chunk_base;        /* H+1 prefix of chunk counts */
-    uint32_t n_windows;          /* windows over rows [H, N) */
-    uint32_t *win_row;           /* n_windows+1 first row of each window */
-    uint32_t *win_edge;          /* n_windows+1 row_ptr[win_row[k]] */
-    uint32_t n_tile_blocks;
+    uint32_t H, C, Z;            /* class boundaries, see above */
+    uint32_t n_chunks;           /* chunks of rows [0, C) */
+    uint32_t *chunk_base;        /* H+1: first chunk of multi-chunk row h */
+    uint32_t *ch_row;            /* n_chunks: row of the chunk | OG_CH_MULTI */
+    uint32_t *ch_edge;           /* n_chunks+1: first edge of the chunk */
+    uint32_t n_chunk_blocks;     /* pull blocks over the chunks */
+    uint32_t n_row_blocks;       /* pull blocks over rows [C, N), one row per thread */
     /* forward side for push: source and destination of each forward edge, new ids */
     uint32_t *push_src, *push_dst;
     /* forward CSR in new ids for the data-driven activation */
@@ -108,7 +119,7 @@ typedef struct og_plan
     uint32_t *f_cols;            /* E */
 } og_plan;
 
-int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t hub_t, int need_push,
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_push,
                   og_plan **out);
 void og_plan_destroy(og_plan *p);
 

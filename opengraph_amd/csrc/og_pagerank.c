@@ -20,7 +20,21 @@
 #include "opengraph_amd_ext.h"
 
 #define OG_CHECK_EVERY 8u
-#define OG_DEFAULT_HUB_T OG_TE
+#define OG_DEFAULT_SEQ_T OG_SEQ_T
+
+/* Blocks of a final reduction over n_parts first-level partials that also
+ * finalizes `rows` long rows (a wavefront each): enough that each block
+ * folds about 256 partials, at most 1024 so that the last block's pass
+ * over the block results stays short. */
+static uint32_t og_final_blocks(uint32_t n_parts, uint32_t rows)
+{
+    uint64_t b = ((uint64_t)n_parts + 255) / 256;
+    uint64_t r = ((uint64_t)rows + 3) / 4;
+    if (r > b) b = r;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return (uint32_t)b;
+}
 
 static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res);
 
@@ -144,11 +158,17 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
         og_set_error("unknown PageRank variant %u", opt->variant);
         return -1;
     }
+    if (opt->ppr && (opt->variant != 0 || opt->ppr_source >= g->N))
+    {
+        og_set_error("personalized PageRank needs variant 0 and a source below %u (got variant %u, "
+                     "source %u)", g->N, opt->variant, opt->ppr_source);
+        return -1;
+    }
     ogpu_pr *p = (ogpu_pr *)calloc(1, sizeof(ogpu_pr));
     if (!p) { og_set_error("out of host memory"); return -1; }
     p->g = g;
     p->opt = *opt;
-    if (p->opt.hub_threshold == 0) p->opt.hub_threshold = OG_DEFAULT_HUB_T;
+    if (p->opt.hub_threshold == 0) p->opt.hub_threshold = OG_DEFAULT_SEQ_T;
     og_variant_mode(opt->variant, &p->mode, &p->push, &p->quant, &p->dd);
     const uint32_t N = g->N, E = g->E;
     double t0 = og_now();
@@ -174,12 +194,10 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     p->gv = (N + 255) / 256;
     if (p->gv > 2048) p->gv = 2048;
     if (p->gv == 0) p->gv = 1;
-    uint32_t f_tiles = (pl->n_tile_blocks + OG_PARTS_PER_FINAL_BLOCK - 1) / OG_PARTS_PER_FINAL_BLOCK;
-    uint32_t f_hubs = (pl->H + 255) / 256;
-    p->n_final_pull = f_tiles > f_hubs ? f_tiles : f_hubs;
-    if (p->n_final_pull == 0) p->n_final_pull = 1;
-    p->n_final_vert = (p->gv + OG_PARTS_PER_FINAL_BLOCK - 1) / OG_PARTS_PER_FINAL_BLOCK;
-    uint32_t parts = pl->n_tile_blocks > p->gv ? pl->n_tile_blocks : p->gv;
+    const uint32_t pull_blocks = pl->n_chunk_blocks + pl->n_row_blocks;
+    p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
+    p->n_final_vert = og_final_blocks(p->gv, 0);
+    uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
     uint32_t fins = p->n_final_pull > p->n_final_vert ? p->n_final_pull : p->n_final_vert;
     p->stats_cap = (opt->max_iter ? opt->max_iter : 1) + 1;
 
@@ -237,10 +255,13 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->N = N;
     a->E = E;
     a->H = pl->H;
+    a->C = pl->C;
     a->n_chunks = pl->n_chunks;
-    a->n_windows = pl->n_windows;
-    a->n_tile_blocks = pl->n_tile_blocks;
-    a->stop_l1 = opt->stop_rule == 1;
+    a->n_chunk_blocks = pl->n_chunk_blocks;
+    a->n_row_blocks = pl->n_row_blocks;
+    a->stop_l1 = opt->stop_rule == 1 || opt->ppr;
+    a->ppr = opt->ppr ? 1 : 0;
+    a->ppr_src = 0;
     a->qmode = p->quant ? (uint32_t)p->mode : 0;
     a->qzero = p->mode == OG_MODE_QPUSH_EPI;
     a->allow_stop = 1;
@@ -249,8 +270,8 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->cols = pl->cols;
     a->outdeg = pl->outdeg;
     a->chunk_base = pl->chunk_base;
-    a->win_row = pl->win_row;
-    a->win_edge = pl->win_edge;
+    a->ch_row = pl->ch_row;
+    a->ch_edge = pl->ch_edge;
     a->new2old = pl->new2old;
     a->old2new = pl->old2new;
     a->push_src = pl->push_src;
@@ -276,6 +297,14 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->f_cols = pl->f_cols;
     a->changed = p->changed;
     a->resid = p->resid;
+    if (opt->ppr &&
+        (og_hip_check(hipMemcpyAsync(&a->ppr_src, pl->old2new + opt->ppr_source, 4,
+                                     hipMemcpyDeviceToHost, og_stream()), "D2H source") ||
+         og_hip_check(hipStreamSynchronize(og_stream()), "sync source")))
+    {
+        ogpu_pr_free(p);
+        return -1;
+    }
     if (og_launch_fill_u32(p->ticket, 0, 1, og_stream())) { ogpu_pr_free(p); return -1; }
     *out = p;
     return 0;
@@ -349,7 +378,7 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     int rc;
     if (!p->push)
     {
-        a.n_parts = p->plan->n_tile_blocks;
+        a.n_parts = p->plan->n_chunk_blocks + p->plan->n_row_blocks;
         a.n_final_blocks = p->n_final_pull;
         rc = og_launch_pull(&a, p->mode, row, s, e0, e1);
     }
@@ -434,8 +463,8 @@ int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
     uint32_t rows = launched, iterations = max_iter;
     for (uint32_t i = 0; i < launched; i++)
     {
-        int stop = p->opt.stop_rule == 1 ? (p->rows_host[i].error_total < p->opt.epsilon)
-                                         : (p->rows_host[i].active == 0);
+        int stop = p->opt.stop_rule == 1 || p->opt.ppr ? (p->rows_host[i].error_total < p->opt.epsilon)
+                                                       : (p->rows_host[i].active == 0);
         if (stop)
         {
             rows = i + 1;
@@ -538,8 +567,8 @@ double ogpu_pr_bytes_per_iteration(const ogpu_pr *p)
 }
 
 /* Data-driven variants (pageRank.c:2405-2896).  Per iteration:
- *   6: contributions -> pull over the worklist (hub chunks + LDS windows,
- *      fused update) -> hub finalize + reduction -> activation of the
+ *   6: contributions -> pull over the worklist (chunks + short rows, fused
+ *      update) -> long-row finalize + reduction -> activation of the
  *      out-neighbours of moved vertices;
  *   7/8: one residual-push kernel over the worklist + reduction;
  * then the worklists swap and the old one is cleared
@@ -585,7 +614,7 @@ static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res)
                 b.c_out = p->c[0];
                 b.g_in = p->c[0];
                 if (og_launch_dd_contrib(&b, s)) return -1;
-                b.n_parts = p->plan->n_tile_blocks;
+                b.n_parts = p->plan->n_chunk_blocks + p->plan->n_row_blocks;
                 b.n_final_blocks = p->n_final_pull;
                 if (og_launch_pull(&b, OG_MODE_F32, it, s, NULL, NULL)) return -1;
                 if (og_launch_dd_activate(&b, s)) return -1;
@@ -674,7 +703,7 @@ static int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g
         {
             og_plan_destroy(g_cache.plan);
             g_cache.plan = NULL;
-            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_HUB_T, 1, &g_cache.plan)) return -1;
+            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_SEQ_T, 1, &g_cache.plan)) return -1;
         }
         *g = g_cache.g;
         *plan = g_cache.plan;
@@ -698,7 +727,7 @@ static int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g
     ogpu_graph *gg = NULL;
     if (ogpu_graph_from_arrays(graph->num_vertices, graph->num_edges, arrays, &gg)) return -1;
     og_plan *pl = NULL;
-    if (og_plan_build((og_graph *)gg, 1, OG_DEFAULT_HUB_T, need_forward, &pl))
+    if (og_plan_build((og_graph *)gg, 1, OG_DEFAULT_SEQ_T, need_forward, &pl))
     {
         ogpu_graph_free(gg);
         return -1;

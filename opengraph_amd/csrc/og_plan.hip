@@ -1,8 +1,8 @@
 /*
  * og_plan.hip -- gfx950 kernels that build the per-graph PageRank plan from
  * the device CSR: the vertex relabelling, the relabelled inverse CSR (rows
- * keep the reference's in-neighbour order), hub chunks, LDS windows, the
- * relabelled forward edge list for push and the forward CSR for the
+ * keep the reference's in-neighbour order), the chunk descriptors of the
+ * long rows, the relabelled forward edge list for push and the forward CSR for the
  * data-driven variants.  Also a general exclusive scan.
  */
 #include <hip/hip_runtime.h>
@@ -114,24 +114,36 @@ extern "C" int og_scan_exclusive(const uint32_t *d_in, uint32_t *d_out, uint32_t
 /* relabelling                                                         */
 /* ------------------------------------------------------------------ */
 
-/* Sort key of old vertex v: hubs (in-degree > hub_t) first; with relabel,
- * then by forward out-degree descending; ties keep id order (stable sort). */
+/* Row class of a vertex with in-degree d: 0 several chunks, 1 one chunk,
+ * 2 summed by one thread, 3 no in-edges. */
+__device__ __forceinline__ uint32_t og_row_class(uint32_t d, uint32_t seq_t)
+{
+    if (d > seq_t) return d > OG_CH ? 0u : 1u;
+    return d ? 2u : 3u;
+}
+
+/* Sort key of old vertex v: its class; with relabel, then (classes 0, 1, 3)
+ * forward out-degree descending, so the contributions gathered most often
+ * share cache lines, or (class 2) in-degree descending, so the rows one
+ * wavefront sums have about the same length.  Ties keep id order (the sort
+ * is stable). */
 __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ outdeg,
-                                    const uint32_t *__restrict__ indeg, uint32_t hub_t,
+                                    const uint32_t *__restrict__ indeg, uint32_t seq_t,
                                     uint32_t relabel, uint32_t *__restrict__ key,
                                     uint32_t *__restrict__ val)
 {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
     {
-        uint32_t hub = indeg[v] > hub_t;
+        uint32_t d = indeg[v];
+        uint32_t cls = og_row_class(d, seq_t);
         uint32_t k;
         if (relabel)
         {
-            uint32_t d = min(outdeg[v], 0x7FFFFFFFu);
-            k = (hub ? 0u : 0x80000000u) | (0x7FFFFFFFu - d);
+            uint32_t w = min(cls == 2 ? d : outdeg[v], 0x3FFFFFFFu);
+            k = (cls << 30) | (0x3FFFFFFFu - w);
         }
         else
-            k = hub ? 0u : 1u;
+            k = cls;
         key[v] = k;
         val[v] = v;
     }
@@ -139,11 +151,11 @@ __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ out
 
 __global__ void og_plan_perm_kernel(uint32_t N, const uint32_t *__restrict__ new2old,
                                     const uint32_t *__restrict__ outdeg_old,
-                                    const uint32_t *__restrict__ indeg_old, uint32_t hub_t,
+                                    const uint32_t *__restrict__ indeg_old, uint32_t seq_t,
                                     uint32_t *__restrict__ old2new, uint32_t *__restrict__ outdeg_new,
-                                    uint32_t *__restrict__ indeg_new, uint32_t *__restrict__ n_hubs)
+                                    uint32_t *__restrict__ indeg_new, uint32_t *__restrict__ class_cnt)
 {
-    uint32_t hubs = 0;
+    uint32_t cnt[4] = {0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
     {
         uint32_t o = new2old[i];
@@ -151,9 +163,10 @@ __global__ void og_plan_perm_kernel(uint32_t N, const uint32_t *__restrict__ new
         outdeg_new[i] = outdeg_old[o];
         uint32_t d = indeg_old[o];
         indeg_new[i] = d;
-        hubs += d > hub_t;
+        cnt[og_row_class(d, seq_t)]++;
     }
-    if (hubs) atomicAdd(n_hubs, hubs);
+    for (int k = 0; k < 4; k++)
+        if (cnt[k]) atomicAdd(&class_cnt[k], cnt[k]);
 }
 
 /* cols[row_ptr[i] + k] = old2new[src_list[idx_old[o] + k]], o = new2old[i]:
@@ -176,7 +189,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_rows_kernel(
     }
 }
 
-/* The same for the hub rows [0, H), one block per row. */
+/* The same for the rows [0, H) (more than one chunk), one block per row. */
 __global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_hubs_kernel(
     const uint32_t *__restrict__ new2old, const uint32_t *__restrict__ old2new,
     const uint32_t *__restrict__ idx_old, const uint32_t *__restrict__ lists,
@@ -189,40 +202,34 @@ __global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_hubs_kernel(
     for (uint32_t k = threadIdx.x; k < n; k += OG_BLOCK) cols[dst + k] = old2new[lists[src + k] & OG_ID_MASK];
 }
 
-__global__ void og_plan_chunk_count_kernel(uint32_t H, const uint32_t *__restrict__ row_ptr,
-                                           uint32_t te, uint32_t *__restrict__ cnt)
+__global__ void og_plan_chunk_count_kernel(uint32_t C, const uint32_t *__restrict__ row_ptr,
+                                           uint32_t ch, uint32_t *__restrict__ cnt)
 {
-    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += gridDim.x * blockDim.x)
-        cnt[h] = (row_ptr[h + 1] - row_ptr[h] + te - 1) / te;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < C; r += gridDim.x * blockDim.x)
+        cnt[r] = (row_ptr[r + 1] - row_ptr[r] + ch - 1) / ch;
 }
 
-/* Windows cut the rows [H, N) by the measure m(r) = row_ptr[r] + rw * r
- * (edges plus rw per row): win_row[k] = first row r with
- * m(r) >= m(H) + k*te (k < n_windows), win_row[n_windows] = N.  A window
- * then holds fewer than te/rw rows and fewer than te + (largest row) edges. */
-__global__ void og_plan_windows_kernel(uint32_t H, uint32_t N, const uint32_t *__restrict__ row_ptr,
-                                       uint32_t te, uint32_t rw, uint32_t n_windows,
-                                       uint32_t *__restrict__ win_row, uint32_t *__restrict__ win_edge)
+/* Chunk k of row r covers edges [row_ptr[r] + k*ch, min(+ch, row_ptr[r+1])):
+ * chunks tile the edges of rows [0, C) in order, so ch_edge[c+1] ends
+ * chunk c.  Rows below H have several chunks and carry OG_CH_MULTI. */
+__global__ void og_plan_chunk_desc_kernel(uint32_t H, uint32_t C, const uint32_t *__restrict__ row_ptr,
+                                          const uint32_t *__restrict__ first, uint32_t ch,
+                                          uint32_t *__restrict__ ch_row, uint32_t *__restrict__ ch_edge)
 {
-    const uint64_t base = (uint64_t)row_ptr[H] + (uint64_t)rw * H;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n_windows; k += gridDim.x * blockDim.x)
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= C; r += gridDim.x * blockDim.x)
     {
-        if (k == n_windows || k == 0)
+        if (r == C)
         {
-            uint32_t r = k == 0 ? H : N;
-            win_row[k] = r;
-            win_edge[k] = row_ptr[r];
+            ch_edge[first[C]] = row_ptr[C];
             continue;
         }
-        uint64_t target = base + (uint64_t)k * te;
-        uint32_t lo = H, hi = N;      /* first r in [lo, hi) with m(r) >= target */
-        while (lo < hi)
+        uint32_t c0 = first[r], n = first[r + 1] - c0, e0 = row_ptr[r];
+        uint32_t tag = r | (r < H ? OG_CH_MULTI : 0u);
+        for (uint32_t k = 0; k < n; k++)
         {
-            uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)row_ptr[mid] + (uint64_t)rw * mid >= target) hi = mid; else lo = mid + 1;
+            ch_row[c0 + k] = tag;
+            ch_edge[c0 + k] = e0 + k * ch;
         }
-        win_row[k] = lo;
-        win_edge[k] = row_ptr[lo];
     }
 }
 
@@ -247,21 +254,21 @@ static unsigned og_grid(uint64_t n, unsigned cap)
 }
 
 extern "C" int og_launch_plan_keys(uint32_t N, const uint32_t *outdeg, const uint32_t *indeg,
-                                   uint32_t hub_t, uint32_t relabel, uint32_t *key, uint32_t *val,
+                                   uint32_t seq_t, uint32_t relabel, uint32_t *key, uint32_t *val,
                                    hipStream_t s)
 {
     hipLaunchKernelGGL(og_plan_keys_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, outdeg,
-                       indeg, hub_t, relabel, key, val);
+                       indeg, seq_t, relabel, key, val);
     return og_hip_check(hipGetLastError(), "og_plan_keys_kernel");
 }
 
 extern "C" int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *outdeg_old,
-                                   const uint32_t *indeg_old, uint32_t hub_t, uint32_t *old2new,
-                                   uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *n_hubs,
+                                   const uint32_t *indeg_old, uint32_t seq_t, uint32_t *old2new,
+                                   uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *class_cnt,
                                    hipStream_t s)
 {
     hipLaunchKernelGGL(og_plan_perm_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, new2old,
-                       outdeg_old, indeg_old, hub_t, old2new, outdeg_new, indeg_new, n_hubs);
+                       outdeg_old, indeg_old, seq_t, old2new, outdeg_new, indeg_new, class_cnt);
     return og_hip_check(hipGetLastError(), "og_plan_perm_kernel");
 }
 
@@ -282,22 +289,22 @@ extern "C" int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2o
     return og_hip_check(hipGetLastError(), "og_plan_copy");
 }
 
-extern "C" int og_launch_plan_chunks(uint32_t H, const uint32_t *row_ptr, uint32_t te, uint32_t *cnt,
+extern "C" int og_launch_plan_chunks(uint32_t C, const uint32_t *row_ptr, uint32_t ch, uint32_t *cnt,
                                      hipStream_t s)
 {
-    if (!H) return 0;
-    hipLaunchKernelGGL(og_plan_chunk_count_kernel, dim3(og_grid(H, 4096)), dim3(OG_BLOCK), 0, s, H,
-                       row_ptr, te, cnt);
+    if (!C) return 0;
+    hipLaunchKernelGGL(og_plan_chunk_count_kernel, dim3(og_grid(C, 16384)), dim3(OG_BLOCK), 0, s, C,
+                       row_ptr, ch, cnt);
     return og_hip_check(hipGetLastError(), "og_plan_chunk_count_kernel");
 }
 
-extern "C" int og_launch_plan_windows(uint32_t H, uint32_t N, const uint32_t *row_ptr, uint32_t te,
-                                      uint32_t rw, uint32_t n_windows, uint32_t *win_row,
-                                      uint32_t *win_edge, hipStream_t s)
+extern "C" int og_launch_plan_chunk_desc(uint32_t H, uint32_t C, const uint32_t *row_ptr,
+                                         const uint32_t *first, uint32_t ch, uint32_t *ch_row,
+                                         uint32_t *ch_edge, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_plan_windows_kernel, dim3(og_grid((uint64_t)n_windows + 1, 16384)),
-                       dim3(OG_BLOCK), 0, s, H, N, row_ptr, te, rw, n_windows, win_row, win_edge);
-    return og_hip_check(hipGetLastError(), "og_plan_windows_kernel");
+    hipLaunchKernelGGL(og_plan_chunk_desc_kernel, dim3(og_grid((uint64_t)C + 1, 16384)), dim3(OG_BLOCK),
+                       0, s, H, C, row_ptr, first, ch, ch_row, ch_edge);
+    return og_hip_check(hipGetLastError(), "og_plan_chunk_desc_kernel");
 }
 
 extern "C" int og_launch_plan_map_pairs(uint32_t E, const uint32_t *a, const uint32_t *b,

@@ -29,8 +29,6 @@ extern "C" {
 /* Iteration index that writes no statistics row (the init-time reduction). */
 #define OG_NO_STATS 0xFFFFFFFFu
 
-/* First-level partials folded by one block of the final reduction. */
-#define OG_PARTS_PER_FINAL_BLOCK 1024u
 
 typedef struct
 {
@@ -42,17 +40,19 @@ typedef struct
 /* Everything a PageRank kernel needs, passed by value at each launch. */
 typedef struct og_pr_dev
 {
-    uint32_t N, E, H;
-    uint32_t n_chunks, n_windows, n_tile_blocks;
+    uint32_t N, E, H, C;     /* row classes of the plan (og_internal.h) */
+    uint32_t n_chunks, n_chunk_blocks, n_row_blocks;
     uint32_t n_parts;        /* valid first-level partials for the next final reduction */
     uint32_t n_final_blocks; /* blocks of the final reduction */
     uint32_t stop_l1;        /* 0: stop when active == 0; 1: when error_total < epsilon */
     uint32_t qmode;          /* 0 or the quantized mode whose parameters the final step computes */
     uint32_t qzero;          /* compute a zero point (quantized push) */
     uint32_t allow_stop;     /* final reduction may raise *done */
+    uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
+    uint32_t ppr_src;        /* new id of the restart vertex */
     double epsilon;
     /* plan */
-    const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *win_row, *win_edge;
+    const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;
     const uint32_t *new2old, *old2new;
     const uint32_t *push_src, *push_dst;
     /* state */
@@ -62,7 +62,7 @@ typedef struct og_pr_dev
     void *next;              /* push accumulators */
     const void *cq_src;      /* quantize: fp32 contributions */
     void *cq_dst;            /* quantize: quantized contributions */
-    void *chunk_sum;         /* hub chunk partial sums (acc type) */
+    void *chunk_sum;         /* chunk sums of the rows [0, H) (acc type) */
     double *part_err;
     uint32_t *part_act;
     float *part_min, *part_max;

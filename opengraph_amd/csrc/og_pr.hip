@@ -3,35 +3,36 @@
  * and of the per-graph plan they run on.
  *
  * Data layout in HBM (all u32 indices, new vertex ids):
- *   row_ptr[N+1], cols[E]  inverse CSR with rows relabelled so that rows with
- *                          more than hub_t in-edges ("hubs") come first and
- *                          the rest follow by forward out-degree, descending
- *                          (so the contributions most often gathered share
- *                          cache lines).  Each row keeps the reference's
- *                          in-neighbour order, so every sum runs in the
- *                          reference's order.
+ *   row_ptr[N+1], cols[E]  inverse CSR with rows relabelled by in-degree
+ *                          class (several chunks, one chunk, short, none),
+ *                          within a class by forward out-degree descending
+ *                          (the contributions gathered most often share
+ *                          cache lines) or, for the short rows, by
+ *                          in-degree descending.  Each row keeps the
+ *                          reference's in-neighbour order.
+ *   ch_row[], ch_edge[]    chunks of OG_CH edges tiling the long rows.
  *   outdeg[N]              forward out-degree in new order.
  *   r[N]                   fp32 ranks (unnormalized, as the reference).
  *   c[2][N]                contributions r/outdeg in the variant's type,
  *                          ping-ponged between iterations; computed in the
  *                          epilogue of the iteration that produced r.
  *
- * One pull iteration is two kernels:
- *   og_pull_kernel<MODE>     blocks [0, n_chunks) sum one TE-edge chunk of a
- *                            hub row each; the remaining blocks each take
- *                            WIN_PER_BLOCK windows of the non-hub rows: the
- *                            window's edges are gathered (coalesced index
- *                            stream, parallel gathers) into LDS, then each
- *                            thread sums whole rows from LDS sequentially in
- *                            CSR order (bit-exact with pageRank.c:998-1011)
- *                            and runs the fused epilogue
- *                            (pageRank.c:1014-1028 + the next iteration's
- *                            contribution, pageRank.c:989-996).
- *   og_pull_finalize<MODE>   sums each hub's chunk partials in chunk order,
- *                            runs its epilogue, and reduces the per-block
- *                            error / active partials in a fixed tree (the
- *                            last block, by ticket, writes the iteration's
- *                            stats and the stop flag).
+ * One pull iteration (a degree-binned gather SpMV plus the update) is two
+ * kernels:
+ *   og_pull_kernel<MODE>     blocks [0, n_chunk_blocks): each wavefront sums
+ *                            OG_CH_PER_WAVE chunks (64 lanes x 8 edges,
+ *                            fixed tree); a row of one chunk gets its fused
+ *                            epilogue there (pageRank.c:1014-1028 plus the
+ *                            next contribution, pageRank.c:989-996), a
+ *                            longer row stores the chunk sum.  The other
+ *                            blocks take the short rows, one per thread,
+ *                            summed in CSR order (bit-exact with
+ *                            pageRank.c:998-1011), and their epilogues.
+ *   og_pull_finalize<MODE>   one wavefront per long row adds its chunk sums
+ *                            and runs its epilogue; then the per-block
+ *                            error / active partials are reduced in a fixed
+ *                            tree (the last block, by ticket, writes the
+ *                            iteration's stats and the stop flag).
  * The fp64 error sum is therefore deterministic; the active count is exact.
  */
 #include <hip/hip_runtime.h>
@@ -41,7 +42,8 @@
 
 #define OG_BLOCK 256
 #define OG_WAVES (OG_BLOCK / 64)
-#define OG_LDS_EDGES (2 * OG_TE)
+#define OG_CH_SLOTS (OG_CH / 64) /* edges per lane of a chunk */
+#define OG_ROW_BATCH 8           /* edges of a short row loaded together */
 
 /* ------------------------------------------------------------------ */
 /* per-mode arithmetic                                                 */
@@ -156,17 +158,25 @@ __device__ __forceinline__ void og_epilogue(const og_pr_dev &a, uint32_t v,
     og_epilogue_pf<MODE>(a, v, sum, prev, deg, st);
 }
 
-/* The same with the old rank and out-degree already loaded (the windowed
- * pull prefetches them with the window's gathers). */
+/* The same with the old rank and out-degree already loaded (the pull loads
+ * them together with the row's first edges). */
 template <int MODE>
 __device__ __forceinline__ void og_epilogue_pf(const og_pr_dev &a, uint32_t v,
                                                typename og_mode<MODE>::acc sum, float prev,
                                                uint32_t deg, og_stat &st)
 {
-    float nr = og_next_rank<MODE>(sum, a.qp);
+    float nr;
+    if constexpr (MODE == OG_MODE_F32)
+    {
+        /* personalized: the restart term (1 - d) only at the source */
+        const float base = !a.ppr || v == a.ppr_src ? OG_BASE_PR : 0.0f;
+        nr = base + OG_DAMP * sum;
+    }
+    else
+        nr = og_next_rank<MODE>(sum, a.qp);
     __builtin_nontemporal_store(nr, &a.r[v]);
     double err = fabs((double)(nr - prev));
-    st.err += err / (double)a.N;
+    st.err += a.ppr ? err : err / (double)a.N;
     st.active += err >= a.epsilon;
     og_store_contrib<MODE>(a.c_out, v, nr, deg);
     if constexpr (MODE >= OG_MODE_Q32)
@@ -240,193 +250,151 @@ template <typename T> __device__ T og_block_sum(T v)
 /* pull iteration                                                      */
 /* ------------------------------------------------------------------ */
 
-/* Issue the index loads, then the gathers, of a window of ne <= LDS_EDGES
- * edges starting at eb: slot u of thread t holds edge u*BLOCK + t.  The
- * slot guards are block-uniform, lanes past the end re-read the last edge,
- * so all loads of a thread are in flight together. */
-template <typename E>
-__device__ __forceinline__ void og_window_issue(const uint32_t *__restrict__ cols,
-                                                const E *__restrict__ g, uint32_t eb, uint32_t ne,
-                                                uint32_t t, E (&val)[OG_LDS_EDGES / OG_BLOCK])
+/* Sum over the wavefront by an xor butterfly: every lane ends with the same
+ * value (each tree node adds the same two operands, in either order). */
+template <typename T> __device__ __forceinline__ T og_wave_allsum(T v)
 {
-    constexpr uint32_t SLOTS = OG_LDS_EDGES / OG_BLOCK;
-    uint32_t idx[SLOTS];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* Chunks [c0, c0 + K) of one wavefront, K = OG_CH_PER_WAVE, all in flight
+ * together: lane l loads edges l, l+64, ... of each chunk (the index loads
+ * coalesce), gathers them and adds them in that order; the wavefront then
+ * adds its 64 lane sums in a fixed tree.  Lanes past a chunk's end re-read
+ * its last edge and add zero, so no branch separates the loads and the
+ * K*OG_CH_SLOTS gathers of a lane overlap.  Lane k then stores chunk k's
+ * sum (row with several chunks) or runs the row's epilogue. */
+template <int MODE>
+__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st)
+{
+    typedef typename og_mode<MODE>::elem E;
+    typedef typename og_mode<MODE>::acc A;
+    constexpr uint32_t K = OG_CH_PER_WAVE, S = OG_CH_SLOTS;
+    const E *__restrict__ g = (const E *)a.g_in;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t c0 = (blockIdx.x * OG_WAVES + wave) * K;
+    if (c0 >= a.n_chunks) return;
+    const uint32_t nk = min(K, a.n_chunks - c0);
+    /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
+     * the end of the last chunk) */
+    const uint32_t my_tag = a.ch_row[c0 + min(lane, nk - 1)];
+    const uint32_t my_edge = a.ch_edge[c0 + min(lane, nk)];
+    uint32_t eb[K], ee[K], row[K];
 #pragma unroll
-    for (uint32_t u = 0; u < SLOTS; u++)
-        if (u * OG_BLOCK < ne) idx[u] = __builtin_nontemporal_load(&cols[eb + min(u * OG_BLOCK + t, ne - 1)]);
+    for (uint32_t k = 0; k < K; k++)
+    {
+        const uint32_t kk = min(k, nk - 1);   /* chunks past nk repeat the last one */
+        row[k] = __builtin_amdgcn_readlane(my_tag, kk) & ~OG_CH_MULTI;
+        eb[k] = __builtin_amdgcn_readlane(my_edge, kk);
+        ee[k] = __builtin_amdgcn_readlane(my_edge, kk + 1);
+    }
+    uint32_t idx[K][S];
 #pragma unroll
-    for (uint32_t u = 0; u < SLOTS; u++)
-        if (u * OG_BLOCK < ne) val[u] = g[idx[u]];
+    for (uint32_t k = 0; k < K; k++)
+#pragma unroll
+        for (uint32_t u = 0; u < S; u++)
+            idx[k][u] = __builtin_nontemporal_load(&a.cols[min(eb[k] + u * 64 + lane, ee[k] - 1)]);
+    /* lane k: old rank and out-degree of chunk k's row, for a one-chunk row */
+    const uint32_t my_row = my_tag & ~OG_CH_MULTI;
+    const float prev = a.r[my_row];
+    const uint32_t deg = a.outdeg[my_row];
+    E val[K][S];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++)
+#pragma unroll
+        for (uint32_t u = 0; u < S; u++) val[k][u] = g[idx[k][u]];
+    A s[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++)
+    {
+        s[k] = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < S; u++) s[k] += eb[k] + u * 64 + lane < ee[k] ? (A)val[k][u] : (A)0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) s[k] = og_wave_allsum(s[k]);
+    A mine = s[0];
+#pragma unroll
+    for (uint32_t k = 1; k < K; k++)
+        if (lane == k) mine = s[k];
+    if (lane < nk)
+    {
+        const uint32_t c = c0 + lane;
+        if (a.dd && !a.wl_cur[my_row])
+        {
+            if (my_tag & OG_CH_MULTI) ((A *)a.chunk_sum)[c] = 0;
+        }
+        else if (my_tag & OG_CH_MULTI) ((A *)a.chunk_sum)[c] = mine;
+        else if (a.dd) og_epilogue<MODE>(a, my_row, mine, st);
+        else og_epilogue_pf<MODE>(a, my_row, mine, prev, deg, st);
+    }
+}
+
+/* Rows [C, N), one per thread.  The row's in-edges are loaded and gathered
+ * OG_ROW_BATCH at a time and added in CSR order (pageRank.c:998-1011), so
+ * the sum is the reference's to the bit.  Rows are sorted by in-degree
+ * here, so the 64 rows of a wavefront take about the same number of
+ * batches.  Lanes past their row's end re-read its last edge and add zero:
+ * x + 0 == x for every partial sum, which starts at +0. */
+template <int MODE>
+__device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st)
+{
+    typedef typename og_mode<MODE>::elem E;
+    typedef typename og_mode<MODE>::acc A;
+    const E *__restrict__ g = (const E *)a.g_in;
+    const uint32_t v = a.C + (blockIdx.x - a.n_chunk_blocks) * OG_BLOCK + threadIdx.x;
+    if (v >= a.N) return;
+    if (a.dd && !a.wl_cur[v]) return;
+    uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]);
+    const uint32_t e = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
+    const float prev = __builtin_nontemporal_load(&a.r[v]);
+    const uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
+    A s = 0;
+    for (; j < e; j += OG_ROW_BATCH)
+    {
+        uint32_t idx[OG_ROW_BATCH];
+#pragma unroll
+        for (uint32_t u = 0; u < OG_ROW_BATCH; u++)
+            idx[u] = __builtin_nontemporal_load(&a.cols[min(j + u, e - 1)]);
+        E val[OG_ROW_BATCH];
+#pragma unroll
+        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) val[u] = g[idx[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) s += j + u < e ? (A)val[u] : (A)0;
+    }
+    if (a.dd) og_epilogue<MODE>(a, v, s, st);
+    else og_epilogue_pf<MODE>(a, v, s, prev, deg, st);
 }
 
 template <int MODE>
 __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t it)
 {
-    typedef typename og_mode<MODE>::elem E;
-    typedef typename og_mode<MODE>::acc A;
     if (*a.done) return;
-    const E *__restrict__ g = (const E *)a.g_in;
-    const uint32_t t = threadIdx.x;
-
-    if (blockIdx.x < a.n_chunks)
-    {
-        /* One TE-edge chunk of a hub row: each thread sums a fixed strided
-         * subset, then a fixed tree over the block. */
-        uint32_t c = blockIdx.x;
-        uint32_t lo = 0, hi = a.H;             /* chunk_base[h] <= c < chunk_base[h+1] */
-        while (hi - lo > 1)
-        {
-            uint32_t mid = (lo + hi) >> 1;
-            if (a.chunk_base[mid] <= c) lo = mid; else hi = mid;
-        }
-        uint32_t h = lo;
-        if (a.dd && !a.wl_cur[h])
-        {
-            if (t == 0) ((A *)a.chunk_sum)[c] = 0;
-            return;
-        }
-        uint32_t eb = a.row_ptr[h] + (c - a.chunk_base[h]) * OG_TE;
-        uint32_t ee = min(eb + OG_TE, a.row_ptr[h + 1]);
-        /* Branch-free: out-of-range lanes re-read the chunk's last edge and
-         * drop it, so all index loads and then all gathers are in flight
-         * together. */
-        A s = 0;
-        uint32_t idx[OG_TE / OG_BLOCK];
-#pragma unroll
-        for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++)
-            idx[u] = __builtin_nontemporal_load(&a.cols[min(eb + u * OG_BLOCK + t, ee - 1)]);
-        E val[OG_TE / OG_BLOCK];
-#pragma unroll
-        for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++) val[u] = g[idx[u]];
-#pragma unroll
-        for (uint32_t u = 0; u < OG_TE / OG_BLOCK; u++)
-            if (eb + u * OG_BLOCK + t < ee) s += (A)val[u];
-        s = og_block_sum<A>(s);
-        if (t == 0) ((A *)a.chunk_sum)[c] = s;
-        return;
-    }
-
-    /* Windows are software-pipelined through two LDS buffers: the index
-     * loads and gathers of window w+1 -- and the row offsets, old ranks and
-     * out-degrees of its (at most 2 per thread) rows -- are issued before
-     * window w is summed, so the sums and epilogues of w run on LDS and
-     * registers while w+1's traffic is in flight. */
-    constexpr uint32_t SLOTS = OG_LDS_EDGES / OG_BLOCK;
-    constexpr uint32_t RSLOTS = OG_TE / OG_ROW_W / OG_BLOCK;
-    __shared__ E lds[2][OG_LDS_EDGES];
-    __shared__ uint32_t s_wrow[OG_WIN_PER_BLOCK + 1], s_wedge[OG_WIN_PER_BLOCK + 1];
     og_stat st;
     og_stat_init(st);
-    const uint32_t tb = blockIdx.x - a.n_chunks;
-    const uint32_t w0 = tb * OG_WIN_PER_BLOCK;
-    const uint32_t nw = min(OG_WIN_PER_BLOCK, a.n_windows - w0);
-    if (t <= nw)
-    {
-        s_wrow[t] = a.win_row[w0 + t];
-        s_wedge[t] = a.win_edge[w0 + t];
-    }
-    __syncthreads();
-    E val[SLOTS];
-    uint32_t nj[RSLOTS], ne_[RSLOTS], nd[RSLOTS];
-    float nr[RSLOTS];
-    auto issue = [&](uint32_t k) {
-        uint32_t rb = s_wrow[k], re = s_wrow[k + 1];
-        uint32_t eb = s_wedge[k], ne = s_wedge[k + 1] - eb;
-        if (ne <= OG_LDS_EDGES) og_window_issue<E>(a.cols, g, eb, ne, t, val);
-#pragma unroll
-        for (uint32_t q = 0; q < RSLOTS; q++)
-        {
-            uint32_t v = rb + q * OG_BLOCK + t;
-            if (v < re)
-            {
-                nj[q] = __builtin_nontemporal_load(&a.row_ptr[v]);
-                ne_[q] = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
-                nr[q] = __builtin_nontemporal_load(&a.r[v]);
-                nd[q] = __builtin_nontemporal_load(&a.outdeg[v]);
-            }
-        }
-    };
-    issue(0);
-    uint32_t buf = 0;
-    for (uint32_t k = 0; k < nw; k++)
-    {
-        const uint32_t rb = s_wrow[k], re = s_wrow[k + 1];
-        const uint32_t eb = s_wedge[k], ne = s_wedge[k + 1] - eb;
-        const bool in_lds = ne <= OG_LDS_EDGES;
-        if (in_lds)
-        {
-#pragma unroll
-            for (uint32_t u = 0; u < SLOTS; u++)
-                if (u * OG_BLOCK + t < ne) lds[buf][u * OG_BLOCK + t] = val[u];
-        }
-        uint32_t cj[RSLOTS], ce[RSLOTS], cd[RSLOTS];
-        float cr[RSLOTS];
-#pragma unroll
-        for (uint32_t q = 0; q < RSLOTS; q++)
-        {
-            cj[q] = nj[q];
-            ce[q] = ne_[q];
-            cr[q] = nr[q];
-            cd[q] = nd[q];
-        }
-        __syncthreads();
-        if (k + 1 < nw) issue(k + 1);
-        if (in_lds)
-        {
-            /* Each thread sums whole rows in CSR order; LDS reads are issued
-             * eight at a time ahead of the (sequential) adds. */
-            const E *L = lds[buf];
-#pragma unroll
-            for (uint32_t q = 0; q < RSLOTS; q++)
-            {
-                uint32_t v = rb + q * OG_BLOCK + t;
-                if (v >= re) break;
-                uint32_t j = cj[q] - eb, e = ce[q] - eb;
-                A s = 0;
-                for (; j + 8 <= e; j += 8)
-                {
-                    E x[8];
-#pragma unroll
-                    for (uint32_t u = 0; u < 8; u++) x[u] = L[j + u];
-#pragma unroll
-                    for (uint32_t u = 0; u < 8; u++) s += (A)x[u];
-                }
-                for (; j < e; j++) s += (A)L[j];
-                if (a.dd) og_epilogue<MODE>(a, v, s, st);
-                else og_epilogue_pf<MODE>(a, v, s, cr[q], cd[q], st);
-            }
-        }
-        else
-        {
-            /* Window larger than LDS (only when hub_t > TE): direct sequential sums. */
-            for (uint32_t v = rb + t; v < re; v += OG_BLOCK)
-            {
-                A s = 0;
-                for (uint32_t j = a.row_ptr[v]; j < a.row_ptr[v + 1]; j++) s += (A)g[a.cols[j]];
-                og_epilogue<MODE>(a, v, s, st);
-            }
-        }
-        buf ^= 1;
-    }
+    if (blockIdx.x < a.n_chunk_blocks) og_pull_chunks<MODE>(a, st);
+    else og_pull_rows<MODE>(a, st);
     og_stat r = og_block_stat(st);
-    if (t == 0)
+    if (threadIdx.x == 0)
     {
-        a.part_err[tb] = r.err;
-        a.part_act[tb] = r.active;
-        a.part_min[tb] = r.cmin;
-        a.part_max[tb] = r.cmax;
+        a.part_err[blockIdx.x] = r.err;
+        a.part_act[blockIdx.x] = r.active;
+        a.part_min[blockIdx.x] = r.cmin;
+        a.part_max[blockIdx.x] = r.cmax;
     }
 }
 
 /* Final reduction shared by pull and push: block b folds a fixed slice of
- * the first-level partials (plus, for pull, the hubs it finalizes); the
- * last block to arrive folds the block results in index order and writes
- * the iteration's statistics, the quantization parameters of the next
- * iteration and the stop flag. */
+ * the first-level partials (plus, for pull, the long rows it finalizes);
+ * the last block to arrive folds the block results in index order and
+ * writes the iteration's statistics, the quantization parameters of the
+ * next iteration and the stop flag. */
 __device__ void og_reduce_tail(const og_pr_dev &a, uint32_t it, og_stat mine)
 {
     const uint32_t t = threadIdx.x;
-    const uint32_t per = OG_PARTS_PER_FINAL_BLOCK;
+    const uint32_t per = (a.n_parts + gridDim.x - 1) / gridDim.x;
     uint32_t pb = blockIdx.x * per, pe = min(pb + per, a.n_parts);
     for (uint32_t i = pb + t; i < pe; i += OG_BLOCK)
     {
@@ -496,12 +464,27 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_finalize_kernel(og_pr_dev a,
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    uint32_t h = blockIdx.x * OG_BLOCK + threadIdx.x;
-    if (h < a.H)
+    /* Rows [0, H): lane l adds chunks l, l+64, ... of the row in order
+     * (four loads in flight; lanes past the end re-read the last chunk and
+     * add zero), then a fixed tree over the wavefront. */
+    const A *__restrict__ cs = (const A *)a.chunk_sum;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t h = blockIdx.x * OG_WAVES + wave; h < a.H; h += gridDim.x * OG_WAVES)
     {
+        if (a.dd && !a.wl_cur[h]) continue;
+        const uint32_t cb = a.chunk_base[h], ce = a.chunk_base[h + 1];
         A s = 0;
-        for (uint32_t c = a.chunk_base[h]; c < a.chunk_base[h + 1]; c++) s += ((const A *)a.chunk_sum)[c];
-        og_epilogue<MODE>(a, h, s, st);
+        for (uint32_t c = cb + lane; c < ce; c += 256)
+        {
+            A x[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) x[u] = cs[min(c + 64 * u, ce - 1)];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) s += c + 64 * u < ce ? x[u] : (A)0;
+        }
+        s = og_wave_sum(s);
+        if (lane == 0) og_epilogue<MODE>(a, h, s, st);
     }
     og_reduce_tail(a, it, st);
 }
@@ -634,7 +617,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_init_kernel(og_pr_dev a, const fl
     og_stat_init(st);
     for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
     {
-        float r = init_old ? init_old[a.new2old[v]] : init_value;
+        float r = init_old ? init_old[a.new2old[v]] : a.ppr ? (v == a.ppr_src ? 1.0f : 0.0f) : init_value;
         a.r[v] = r;
         uint32_t deg = a.outdeg[v];
         og_store_contrib<MODE>(a.c_out, v, r, deg);
@@ -659,7 +642,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_init_kernel(og_pr_dev a, const fl
 __global__ __launch_bounds__(OG_BLOCK) void og_normalize_kernel(og_pr_dev a, float *out_old)
 {
     double s = 0.0;
-    const float fn = (float)a.N;
+    const float fn = a.ppr ? 1.0f : (float)a.N;   /* personalized: not divided by N */
     for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
     {
         float x = a.r[a.old2new[v]] / fn;
@@ -685,7 +668,7 @@ __global__ void og_sum_parts_kernel(const double *p, uint32_t n, double *out)
 template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it, hipStream_t s,
                                                  hipEvent_t ev0, hipEvent_t ev1)
 {
-    uint32_t grid = a->n_chunks + a->n_tile_blocks;
+    uint32_t grid = a->n_chunk_blocks + a->n_row_blocks;
     if (ev0) hipEventRecord(ev0, s);
     if (grid) hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     if (ev1) hipEventRecord(ev1, s);

@@ -621,6 +621,36 @@ uint32_t orc_ppr_fp64(const orc_csr *g, uint32_t s, double tol, uint32_t max_ite
     return it;
 }
 
+uint32_t orc_ppr_f32(const orc_csr *g, uint32_t s, double tol, uint32_t max_iter,
+                     float *x, double *l1_hist)
+{
+    const uint32_t N = g->num_vertices;
+    const float base = 1.0f - 0.85f, damp = 0.85f;
+    float *c = orc_alloc((size_t)N * 4), *nx = orc_alloc((size_t)N * 4);
+    if (!c || !nx) { free(c); free(nx); return 0; }
+    for (uint32_t v = 0; v < N; v++) x[v] = v == s ? 1.0f : 0.0f;
+    uint32_t it;
+    for (it = 0; it < max_iter; it++)
+    {
+        for (uint32_t v = 0; v < N; v++)
+            c[v] = g->f_out_degree[v] ? x[v] / (float)g->f_out_degree[v] : 0.0f;
+        double l1 = 0.0;
+        for (uint32_t v = 0; v < N; v++)
+        {
+            float acc = 0.0f;
+            for (uint32_t j = g->i_edges_idx[v]; j < g->i_edges_idx[v] + g->i_out_degree[v]; j++)
+                acc += c[g->i_dst[j] & ORC_ID_MASK];
+            nx[v] = (v == s ? base : 0.0f) + damp * acc;
+            l1 += fabs((double)(nx[v] - x[v]));
+        }
+        memcpy(x, nx, (size_t)N * 4);
+        if (l1_hist) l1_hist[it] = l1;
+        if (l1 < tol) { it++; break; }
+    }
+    free(c); free(nx);
+    return it;
+}
+
 void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n)
 {
     /* reorder.c:165-221: key FloatToFixed32SORT(rank) = (uint32)(rank*2^20 +

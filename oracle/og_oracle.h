@@ -98,6 +98,15 @@ uint32_t orc_pagerank_fp64(const orc_csr *g, double init, double tol, uint32_t m
 uint32_t orc_ppr_fp64(const orc_csr *g, uint32_t source, double tol, uint32_t max_iter,
                       double *x_out, double *last_l1);
 
+/* The same iteration in the pull kernel's fp32 arithmetic, the sum of each
+ * row in the reference's in-neighbour order (pageRank.c:998-1011 with the
+ * restart term (1.0f - 0.85f) only at s): c = x / (float)outdeg,
+ * x' = ((v == s) ? 1.0f - 0.85f : 0.0f) + 0.85f * sum, L1 = sum over v of
+ * fabs((double)(x' - x)); stops after the first iteration with L1 < tol.
+ * l1_hist[i] (may be NULL) gets iteration i's L1.  Returns iterations. */
+uint32_t orc_ppr_f32(const orc_csr *g, uint32_t source, double tol, uint32_t max_iter,
+                     float *x_out, double *l1_hist);
+
 /* reorder.c:64-221: stable ascending sort of labels by FloatToFixed32SORT(rank). */
 void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n);
 

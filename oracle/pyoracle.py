@@ -60,6 +60,8 @@ def lib():
         L.orc_pagerank_fp64.restype = C.c_uint32
         L.orc_ppr_fp64.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, f64p, f64p]
         L.orc_ppr_fp64.restype = C.c_uint32
+        L.orc_ppr_f32.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, f32p, f64p]
+        L.orc_ppr_f32.restype = C.c_uint32
         L.orc_sort_labels_by_rank.argtypes = [f32p, u32p, C.c_uint32]
         L.orc_stable_sort_pairs.argtypes = [u32p, u32p, C.c_uint32, C.c_int]
         L.orc_stable_sort_pairs.restype = C.c_int
@@ -152,6 +154,15 @@ def ppr_fp64(csr: CSR, source, tol=1e-12, max_iter=1000):
     l1 = C.c_double()
     it = L.orc_ppr_fp64(C.byref(csr.s), source, tol, max_iter, _p(out, f64p), C.byref(l1))
     return out, it, l1.value
+
+
+def ppr_f32(csr: CSR, source, tol=1e-6, max_iter=100):
+    """fp32 personalized PageRank in the pull kernel's arithmetic: (x, iterations, l1 per iteration)."""
+    L = lib()
+    out = np.zeros(csr.num_vertices, np.float32)
+    hist = np.zeros(max(max_iter, 1), np.float64)
+    it = L.orc_ppr_f32(C.byref(csr.s), source, tol, max_iter, _p(out, f32p), _p(hist, f64p))
+    return out, it, hist[:it]
 
 
 def sort_labels_by_rank(ranks, labels):
