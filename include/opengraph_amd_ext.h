@@ -127,6 +127,39 @@ int ogpu_pr_rows(ogpu_pr *pr, uint32_t *active, double *error_total, uint32_t ma
 double ogpu_pr_bytes_per_iteration(const ogpu_pr *pr);
 void ogpu_pr_free(ogpu_pr *pr);
 
+/* ---- autotuner (no counterpart in the reference) ----
+ * Times `iters` plain pull iterations (after two warm-up ones, host clock,
+ * launch gaps included) for each candidate hub_threshold (the row-class
+ * boundary: rows with more in-edges are chunked) and returns the fastest in
+ * *best; ms[i] (may be NULL) gets candidate i's milliseconds per iteration. */
+int ogpu_tune_hub_threshold(ogpu_graph *g, const uint32_t *cands, uint32_t n_cands, uint32_t iters,
+                            uint32_t *best, double *ms);
+
+/* ---- many graphs in one launch (no counterpart in the reference) ----
+ * A plain fp32 pull PageRank (variant 0, reference start and stop rule,
+ * opt->max_iter / epsilon) on each of n graphs.  hub_thresholds[g] (or, if
+ * NULL, opt->hub_threshold) fixes graph g's row-class boundary; 0 means
+ * autotuned (ogpu_tune_hub_threshold over 8..128, once per size class of
+ * graphs with the same ceil(log2 N), on the class's largest graph). */
+typedef struct ogpu_multi ogpu_multi;
+int ogpu_multi_create(ogpu_graph *const *graphs, uint32_t n, const ogpu_pr_options *opt,
+                      const uint32_t *hub_thresholds, ogpu_multi **out);
+/* mode 0: one graph after another, each with its own launches and stop
+ *         checks (what per-graph runs cost);
+ * mode 1: every iteration of all graphs as three persistent launches over
+ *         one work list, max_iter iterations enqueued without host checks
+ *         (finished graphs skip their work, launches after the last stop
+ *         return at once);
+ * mode 2: the same iterations captured once into a hipGraph and launched
+ *         as one graph.
+ * seconds: host clock over init and iterations (results not normalized). */
+int ogpu_multi_run(ogpu_multi *m, int mode, double *seconds);
+/* Graph g's result of the last run, as ogpu_pr_run's (ranks divided by N,
+ * caller's vertex order; ranks may be NULL). */
+int ogpu_multi_result(ogpu_multi *m, uint32_t g, ogpu_pr_result *res, float *ranks);
+uint32_t ogpu_multi_hub_threshold(const ogpu_multi *m, uint32_t g);
+void ogpu_multi_free(ogpu_multi *m);
+
 /* ---- batched personalized PageRank (no counterpart in the reference) ----
  * k <= 64 restart vertices iterated together as a pull-SpMM over an N x k
  * fp32 row-major block: per column exactly the personalized option of

@@ -174,6 +174,12 @@ SIGNATURES = {
     "ogpu_pr_rows": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
     "ogpu_pr_bytes_per_iteration": (C.c_double, [_VP]),
     "ogpu_pr_free": (None, [_VP]),
+    "ogpu_tune_hub_threshold": (C.c_int, [_VP, u32p, u32, u32, u32p, C.POINTER(C.c_double)]),
+    "ogpu_multi_create": (C.c_int, [C.POINTER(_VP), u32, C.POINTER(PrOptions), u32p, C.POINTER(_VP)]),
+    "ogpu_multi_run": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
+    "ogpu_multi_result": (C.c_int, [_VP, u32, C.POINTER(PrResult), _VP]),
+    "ogpu_multi_hub_threshold": (u32, [_VP, u32]),
+    "ogpu_multi_free": (None, [_VP]),
     "ogpu_ppr_create": (C.c_int, [_VP, u32p, u32, u32, C.c_double, C.POINTER(_VP)]),
     "ogpu_ppr_run": (C.c_int, [_VP, C.POINTER(PprResult)]),
     "ogpu_ppr_ranks": (C.c_int, [_VP, _VP]),
@@ -408,6 +414,60 @@ class PageRank:
     def free(self):
         if self.h:
             load().ogpu_pr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def tune_hub_threshold(graph: Graph, cands=(8, 16, 32, 64, 128), iters=5):
+    """(best threshold, ms per iteration of each candidate) (ogpu_tune_hub_threshold)."""
+    c = np.ascontiguousarray(cands, dtype=np.uint32)
+    ms = np.zeros(len(c), np.float64)
+    best = u32()
+    check(load().ogpu_tune_hub_threshold(graph.h, c.ctypes.data_as(u32p), len(c), iters, C.byref(best),
+                                         ms.ctypes.data_as(C.POINTER(C.c_double))))
+    return best.value, ms
+
+
+class Multi:
+    """Plain PageRank on many graphs at once (ogpu_multi): mode 0 one graph
+    after another, 1 all graphs per launch, 2 the same in one hipGraph."""
+
+    def __init__(self, graphs, max_iter=100, epsilon=1e-6, hub_thresholds=None, hub_threshold=0):
+        lib = load()
+        o = PrOptions()
+        lib.ogpu_pr_default_options(C.byref(o))
+        o.max_iter, o.epsilon, o.hub_threshold = max_iter, epsilon, hub_threshold
+        self.graphs = list(graphs)
+        hs = (_VP * len(self.graphs))(*[g.h.value if isinstance(g.h, C.c_void_p) else g.h for g in self.graphs])
+        ht = None
+        if hub_thresholds is not None:
+            self._ht = np.ascontiguousarray(hub_thresholds, dtype=np.uint32)
+            ht = self._ht.ctypes.data_as(u32p)
+        self.h = C.c_void_p()
+        check(lib.ogpu_multi_create(hs, len(self.graphs), C.byref(o), ht, C.byref(self.h)))
+
+    def run(self, mode=2) -> float:
+        s = C.c_double()
+        check(load().ogpu_multi_run(self.h, mode, C.byref(s)))
+        return s.value
+
+    def result(self, g, ranks=True):
+        r = PrResult()
+        out = np.empty(self.graphs[g].num_vertices, np.float32) if ranks else None
+        check(load().ogpu_multi_result(self.h, g, C.byref(r), _ptr(out) if ranks else None))
+        return r, out
+
+    def hub_threshold(self, g):
+        return load().ogpu_multi_hub_threshold(self.h, g)
+
+    def free(self):
+        if self.h:
+            load().ogpu_multi_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -423,15 +423,14 @@ static int og_pr_finish(ogpu_pr *p, double *pr_sum)
     return og_hip_check(hipStreamSynchronize(og_stream()), "sync finish");
 }
 
-int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
+/* Init and iterations until the device stop flag or max_iter; returns the
+ * iterations launched (or -1).  The host reads the flag every
+ * OG_CHECK_EVERY iterations. */
+static int64_t og_pr_solve(ogpu_pr *p)
 {
-    memset(res, 0, sizeof(*res));
-    res->seconds_plan = p->seconds_plan;
-    if (p->dd) return og_dd_run(p, res);
     hipStream_t s = og_stream();
     const uint32_t max_iter = p->opt.max_iter;
     if (og_pr_events(&p->iter_ev, &p->n_iter_ev, max_iter + 1)) return -1;
-    double t0 = og_now();
     if (og_hip_check(hipEventRecord(p->iter_ev[0], s), "hipEventRecord")) return -1;
     if (og_pr_init_state(p)) return -1;
     uint32_t launched = 0;
@@ -449,10 +448,31 @@ int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
             return -1;
         if (*p->done_host) break;
     }
+    return launched;
+}
+
+static int og_pr_collect(ogpu_pr *p, uint32_t launched, double pr_sum, ogpu_pr_result *res);
+
+int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
+{
+    memset(res, 0, sizeof(*res));
+    res->seconds_plan = p->seconds_plan;
+    if (p->dd) return og_dd_run(p, res);
+    double t0 = og_now();
+    int64_t launched = og_pr_solve(p);
+    if (launched < 0) return -1;
     double pr_sum = 0.0;
     if (og_pr_finish(p, &pr_sum)) return -1;
-    double t1 = og_now();
-    /* rows */
+    res->seconds_total = og_now() - t0;
+    return og_pr_collect(p, (uint32_t)launched, pr_sum, res);
+}
+
+/* Statistics rows of the last solve: iterations by the stop rule, the
+ * reference's iteration count, last error / active. */
+static int og_pr_collect(ogpu_pr *p, uint32_t launched, double pr_sum, ogpu_pr_result *res)
+{
+    hipStream_t s = og_stream();
+    const uint32_t max_iter = p->opt.max_iter;
     free(p->rows_host);
     p->rows_host = (og_iter_stat *)calloc(launched ? launched : 1, sizeof(og_iter_stat));
     if (launched &&
@@ -478,7 +498,6 @@ int ogpu_pr_run(ogpu_pr *p, ogpu_pr_result *res)
     res->active = rows ? p->rows_host[rows - 1].active : 0;
     res->error_total = rows ? p->rows_host[rows - 1].error_total : 0.0;
     res->pr_sum = pr_sum;
-    res->seconds_total = t1 - t0;
     return 0;
 }
 
@@ -870,3 +889,304 @@ struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *a, s
 struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 11); }
 struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 12); }
 struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *a, struct GraphCSR *g) { return og_dropin(a, g, 13); }
+
+/* ------------------------------------------------------------------ */
+/* autotuner of the row-class threshold                                */
+/* ------------------------------------------------------------------ */
+
+int ogpu_tune_hub_threshold(ogpu_graph *gg, const uint32_t *cands, uint32_t n_cands, uint32_t iters,
+                            uint32_t *best, double *ms)
+{
+    og_graph *g = (og_graph *)gg;
+    if (!g || !n_cands || !iters) { og_set_error("ogpu_tune_hub_threshold: bad arguments"); return -1; }
+    double best_ms = 0.0;
+    *best = cands[0];
+    for (uint32_t i = 0; i < n_cands; i++)
+    {
+        ogpu_pr_options o;
+        ogpu_pr_default_options(&o);
+        o.max_iter = iters + 2;
+        o.hub_threshold = cands[i] ? cands[i] : OG_DEFAULT_SEQ_T;
+        ogpu_pr *p = NULL;
+        if (og_pr_create_with(g, NULL, &o, &p)) return -1;
+        /* two warm-up iterations, then `iters` timed on the host clock
+         * (pull, finalize and launch gaps: what a run pays) */
+        double t = 0.0;
+        int bad = ogpu_pr_reset(p) || ogpu_pr_enqueue(p, 2, 0) ||
+                  og_hip_check(hipStreamSynchronize(og_stream()), "sync");
+        if (!bad)
+        {
+            double t0 = og_now();
+            bad = ogpu_pr_enqueue(p, iters, 0) || og_hip_check(hipStreamSynchronize(og_stream()), "sync");
+            t = (og_now() - t0) * 1e3 / iters;
+        }
+        ogpu_pr_free(p);
+        if (bad) return -1;
+        if (ms) ms[i] = t;
+        if (i == 0 || t < best_ms)
+        {
+            best_ms = t;
+            *best = cands[i];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* many graphs                                                         */
+/* ------------------------------------------------------------------ */
+
+#define OG_MULTI_TUNE_ITERS 5u
+#define OG_MULTI_WAVES 4u      /* wavefronts per block of the long-row kernel */
+
+struct ogpu_multi
+{
+    uint32_t n;
+    ogpu_pr **pr;
+    uint32_t *hub;              /* chosen threshold per graph */
+    og_multi_dev dev;
+    og_pr_dev *d_devs;
+    og_multi_work *d_work, *d_lwork, *d_iwork;
+    uint32_t *d_wr, *d_lr;
+    double *w_err, *l_err;
+    uint32_t *w_act, *l_act, *d_live, *d_all_done;
+    uint32_t grid, max_iter;
+    int64_t *launched;          /* per graph, of the last run */
+    hipGraphExec_t exec;
+    int have_exec;
+};
+
+void ogpu_multi_free(ogpu_multi *m)
+{
+    if (!m) return;
+    if (m->have_exec) (void)hipGraphExecDestroy(m->exec);
+    for (uint32_t g = 0; m->pr && g < m->n; g++) ogpu_pr_free(m->pr[g]);
+    free(m->pr); free(m->hub); free(m->launched);
+    og_dfree(m->d_devs); og_dfree(m->d_work); og_dfree(m->d_lwork); og_dfree(m->d_iwork);
+    og_dfree(m->d_wr); og_dfree(m->d_lr); og_dfree(m->w_err); og_dfree(m->l_err);
+    og_dfree(m->w_act); og_dfree(m->l_act); og_dfree(m->d_live); og_dfree(m->d_all_done);
+    free(m);
+}
+
+uint32_t ogpu_multi_hub_threshold(const ogpu_multi *m, uint32_t g)
+{
+    return g < m->n ? m->hub[g] : 0;
+}
+
+/* Threshold per graph: given, or tuned once per size class (graphs whose
+ * vertex counts share the same power of two) on the class's largest
+ * graph. */
+static int og_multi_pick(ogpu_multi *m, og_graph *const *graphs, const ogpu_pr_options *opt,
+                         const uint32_t *hub)
+{
+    static const uint32_t cands[] = {8, 16, 32, 64, 128};
+    int32_t cls_best[33];
+    for (int i = 0; i < 33; i++) cls_best[i] = -1;
+    for (uint32_t g = 0; g < m->n; g++)
+    {
+        uint32_t h = hub ? hub[g] : opt->hub_threshold;
+        if (h) { m->hub[g] = h; continue; }
+        uint32_t N = graphs[g]->N, cls = 0;
+        while (cls < 32 && (1u << cls) < N) cls++;
+        if (cls_best[cls] < 0)
+        {
+            uint32_t big = g;
+            for (uint32_t q = 0; q < m->n; q++)
+            {
+                uint32_t Nq = graphs[q]->N, cq = 0;
+                while (cq < 32 && (1u << cq) < Nq) cq++;
+                if (cq == cls && graphs[q]->E > graphs[big]->E) big = q;
+            }
+            uint32_t b = 0;
+            if (ogpu_tune_hub_threshold((ogpu_graph *)graphs[big], cands, 5, OG_MULTI_TUNE_ITERS, &b, NULL))
+                return -1;
+            cls_best[cls] = (int32_t)b;
+        }
+        m->hub[g] = (uint32_t)cls_best[cls];
+    }
+    return 0;
+}
+
+int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_options *opt_in,
+                      const uint32_t *hub, ogpu_multi **out)
+{
+    *out = NULL;
+    og_graph *const *graphs = (og_graph *const *)graphs_in;
+    ogpu_pr_options opt;
+    if (opt_in) opt = *opt_in; else ogpu_pr_default_options(&opt);
+    if (opt.variant != 0 || opt.ppr || opt.init_mode != 0)
+    {
+        og_set_error("ogpu_multi_create: plain pull (variant 0, reference start) only");
+        return -1;
+    }
+    if (og_ctx_ensure()) return -1;
+    ogpu_multi *m = (ogpu_multi *)calloc(1, sizeof(ogpu_multi));
+    if (!m) { og_set_error("out of host memory"); return -1; }
+    m->n = n;
+    m->max_iter = opt.max_iter;
+    m->pr = (ogpu_pr **)calloc(n ? n : 1, sizeof(ogpu_pr *));
+    m->hub = (uint32_t *)calloc(n ? n : 1, 4);
+    m->launched = (int64_t *)calloc(n ? n : 1, 8);
+    if (!m->pr || !m->hub || !m->launched) { og_set_error("out of host memory"); ogpu_multi_free(m); return -1; }
+    if (og_multi_pick(m, graphs, &opt, hub)) { ogpu_multi_free(m); return -1; }
+    /* per-graph runs, device table and work lists */
+    og_pr_dev *devs = (og_pr_dev *)calloc(2 * (size_t)(n ? n : 1), sizeof(og_pr_dev));
+    uint32_t *wr = (uint32_t *)calloc(n + 1, 4), *lr = (uint32_t *)calloc(n + 1, 4);
+    size_t n_work = 0, n_lwork = 0, n_iwork = 0;
+    int rc = -1;
+    og_multi_work *work = NULL, *lwork = NULL, *iwork = NULL;
+    if (!devs || !wr || !lr) { og_set_error("out of host memory"); goto fail; }
+    for (uint32_t g = 0; g < n; g++)
+    {
+        ogpu_pr_options o = opt;
+        o.hub_threshold = m->hub[g];
+        if (og_pr_create_with(graphs[g], NULL, &o, &m->pr[g])) goto fail;
+        ogpu_pr *p = m->pr[g];
+        for (int q = 0; q < 2; q++)
+        {
+            og_pr_dev a = p->dev;
+            a.g_in = p->c[q];
+            a.c_out = p->c[q ^ 1];
+            a.allow_stop = 1;
+            devs[2 * g + q] = a;
+        }
+        wr[g] = (uint32_t)n_work;
+        lr[g] = (uint32_t)n_lwork;
+        n_work += p->plan->n_chunk_blocks + p->plan->n_row_blocks;
+        n_lwork += (p->plan->H + OG_MULTI_WAVES - 1) / OG_MULTI_WAVES;
+        n_iwork += (p->g->N + 255) / 256;
+    }
+    wr[n] = (uint32_t)n_work;
+    lr[n] = (uint32_t)n_lwork;
+    work = (og_multi_work *)calloc(n_work ? n_work : 1, sizeof(og_multi_work));
+    lwork = (og_multi_work *)calloc(n_lwork ? n_lwork : 1, sizeof(og_multi_work));
+    iwork = (og_multi_work *)calloc(n_iwork ? n_iwork : 1, sizeof(og_multi_work));
+    if (!work || !lwork || !iwork) { og_set_error("out of host memory"); goto fail; }
+    {
+        size_t w = 0, l = 0, i = 0;
+        for (uint32_t g = 0; g < n; g++)
+        {
+            og_plan *pl = m->pr[g]->plan;
+            for (uint32_t b = 0; b < pl->n_chunk_blocks; b++) work[w++] = (og_multi_work){g, 0, b, 0};
+            for (uint32_t b = 0; b < pl->n_row_blocks; b++) work[w++] = (og_multi_work){g, 1, b, 0};
+            for (uint32_t b = 0; b < (pl->H + OG_MULTI_WAVES - 1) / OG_MULTI_WAVES; b++)
+                lwork[l++] = (og_multi_work){g, 2, b, 0};
+            for (uint32_t b = 0; b < (m->pr[g]->g->N + 255) / 256; b++) iwork[i++] = (og_multi_work){g, 3, b, 0};
+        }
+    }
+    m->d_devs = og_dalloc(2 * (size_t)(n ? n : 1) * sizeof(og_pr_dev));
+    m->d_work = og_dalloc((n_work ? n_work : 1) * sizeof(og_multi_work));
+    m->d_lwork = og_dalloc((n_lwork ? n_lwork : 1) * sizeof(og_multi_work));
+    m->d_iwork = og_dalloc((n_iwork ? n_iwork : 1) * sizeof(og_multi_work));
+    m->d_wr = og_dalloc((size_t)(n + 1) * 4);
+    m->d_lr = og_dalloc((size_t)(n + 1) * 4);
+    m->w_err = og_dalloc((n_work ? n_work : 1) * 8);
+    m->w_act = og_dalloc((n_work ? n_work : 1) * 4);
+    m->l_err = og_dalloc((n_lwork ? n_lwork : 1) * 8);
+    m->l_act = og_dalloc((n_lwork ? n_lwork : 1) * 4);
+    m->d_live = og_dalloc(4);
+    m->d_all_done = og_dalloc(4);
+    if (!m->d_devs || !m->d_work || !m->d_lwork || !m->d_iwork || !m->d_wr || !m->d_lr || !m->w_err ||
+        !m->w_act || !m->l_err || !m->l_act || !m->d_live || !m->d_all_done)
+        goto fail;
+    {
+        hipStream_t s = og_stream();
+        if (og_hip_check(hipMemcpyAsync(m->d_devs, devs, 2 * (size_t)n * sizeof(og_pr_dev), hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_work, work, n_work * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_lwork, lwork, n_lwork * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_iwork, iwork, n_iwork * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_wr, wr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_lr, lr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipStreamSynchronize(s), "sync"))
+            goto fail;
+    }
+    og_multi_dev *d = &m->dev;
+    d->devs = m->d_devs;
+    d->work = m->d_work;
+    d->lwork = m->d_lwork;
+    d->iwork = m->d_iwork;
+    d->n_work = (uint32_t)n_work;
+    d->n_lwork = (uint32_t)n_lwork;
+    d->n_iwork = (uint32_t)n_iwork;
+    d->n_graphs = n;
+    d->wr = m->d_wr;
+    d->lr = m->d_lr;
+    d->w_err = m->w_err;
+    d->l_err = m->l_err;
+    d->w_act = m->w_act;
+    d->l_act = m->l_act;
+    d->live = m->d_live;
+    d->all_done = m->d_all_done;
+    m->grid = (uint32_t)og_num_cus() * 8;
+    if (m->grid == 0) m->grid = 2048;
+    rc = 0;
+fail:
+    free(devs); free(wr); free(lr); free(work); free(lwork); free(iwork);
+    if (rc) { ogpu_multi_free(m); return -1; }
+    *out = m;
+    return 0;
+}
+
+static int og_multi_enqueue_all(ogpu_multi *m, hipStream_t s)
+{
+    for (uint32_t it = 0; it < m->max_iter; it++)
+        if (og_launch_multi_iter(&m->dev, it, m->grid, s)) return -1;
+    return 0;
+}
+
+int ogpu_multi_run(ogpu_multi *m, int mode, double *seconds)
+{
+    hipStream_t s = og_stream();
+    double t0 = og_now();
+    if (mode == 0)
+    {
+        for (uint32_t g = 0; g < m->n; g++)
+        {
+            m->launched[g] = og_pr_solve(m->pr[g]);
+            if (m->launched[g] < 0) return -1;
+        }
+    }
+    else
+    {
+        if (mode == 2 && !m->have_exec)
+        {
+            /* capture the max_iter iterations once; the init stays outside
+             * so that a replay starts from a fresh state */
+            hipGraph_t graph;
+            if (og_hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+                return -1;
+            int bad = og_multi_enqueue_all(m, s);
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (bad || og_hip_check(e, "hipStreamEndCapture")) return -1;
+            e = hipGraphInstantiate(&m->exec, graph, NULL, NULL, 0);
+            (void)hipGraphDestroy(graph);
+            if (og_hip_check(e, "hipGraphInstantiate")) return -1;
+            m->have_exec = 1;
+            t0 = og_now();
+        }
+        if (og_launch_multi_init(&m->dev, m->grid, s)) return -1;
+        if (mode == 2)
+        {
+            if (og_hip_check(hipGraphLaunch(m->exec, s), "hipGraphLaunch")) return -1;
+        }
+        else if (og_multi_enqueue_all(m, s))
+            return -1;
+        if (og_hip_check(hipStreamSynchronize(s), "sync multi")) return -1;
+        for (uint32_t g = 0; g < m->n; g++) m->launched[g] = m->max_iter;
+    }
+    if (seconds) *seconds = og_now() - t0;
+    return 0;
+}
+
+int ogpu_multi_result(ogpu_multi *m, uint32_t g, ogpu_pr_result *res, float *ranks)
+{
+    if (g >= m->n) { og_set_error("ogpu_multi_result: graph %u of %u", g, m->n); return -1; }
+    memset(res, 0, sizeof(*res));
+    ogpu_pr *p = m->pr[g];
+    double pr_sum = 0.0;
+    if (og_pr_finish(p, &pr_sum)) return -1;
+    uint32_t launched = m->launched[g] > 0 ? (uint32_t)m->launched[g] : 0;
+    if (og_pr_collect(p, launched, pr_sum, res)) return -1;
+    res->seconds_plan = p->seconds_plan;
+    return ranks ? ogpu_pr_ranks(p, ranks) : 0;
+}

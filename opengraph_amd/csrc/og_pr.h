@@ -82,6 +82,34 @@ typedef struct og_pr_dev
 
 int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
+
+/* ---- many graphs in one launch (fp32 pull) ----
+ * Work item: block `first` of graph g's pull (kind 0 chunk block, 1 row
+ * block), of its long-row finalize (kind 2, rows [first*4, first*4+4)) or
+ * of its init (kind 3, vertices [first*256, first*256+256)).  Items are
+ * grouped by graph; the kernels are persistent (block b takes items
+ * b, b + grid, ...) and write one statistics partial per item. */
+typedef struct
+{
+    uint32_t g, kind, first, pad;
+} og_multi_work;
+
+typedef struct og_multi_dev
+{
+    const og_pr_dev *devs;       /* 2 per graph: [2g + parity of the iteration] */
+    const og_multi_work *work;   /* pull items */
+    const og_multi_work *lwork;  /* long-row items */
+    const og_multi_work *iwork;  /* init items */
+    uint32_t n_work, n_lwork, n_iwork, n_graphs;
+    const uint32_t *wr, *lr;     /* n_graphs + 1: item ranges per graph */
+    double *w_err, *l_err;
+    uint32_t *w_act, *l_act;
+    uint32_t *live;              /* graphs not yet stopped */
+    uint32_t *all_done;          /* set when live reaches 0 */
+} og_multi_dev;
+
+int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s);
+int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s);
 int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
 int og_launch_quantize(const og_pr_dev *a, int mode, hipStream_t s);

@@ -266,7 +266,7 @@ template <typename T> __device__ __forceinline__ T og_wave_allsum(T v)
  * K*OG_CH_SLOTS gathers of a lane overlap.  Lane k then stores chunk k's
  * sum (row with several chunks) or runs the row's epilogue. */
 template <int MODE>
-__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st)
+__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
@@ -274,7 +274,7 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st)
     const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t c0 = (blockIdx.x * OG_WAVES + wave) * K;
+    const uint32_t c0 = (cblock * OG_WAVES + wave) * K;
     if (c0 >= a.n_chunks) return;
     const uint32_t nk = min(K, a.n_chunks - c0);
     /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
@@ -339,12 +339,12 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st)
  * batches.  Lanes past their row's end re-read its last edge and add zero:
  * x + 0 == x for every partial sum, which starts at +0. */
 template <int MODE>
-__device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st)
+__device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
     const E *__restrict__ g = (const E *)a.g_in;
-    const uint32_t v = a.C + (blockIdx.x - a.n_chunk_blocks) * OG_BLOCK + threadIdx.x;
+    const uint32_t v = a.C + rblock * OG_BLOCK + threadIdx.x;
     if (v >= a.N) return;
     if (a.dd && !a.wl_cur[v]) return;
     uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]);
@@ -374,8 +374,8 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    if (blockIdx.x < a.n_chunk_blocks) og_pull_chunks<MODE>(a, st);
-    else og_pull_rows<MODE>(a, st);
+    if (blockIdx.x < a.n_chunk_blocks) og_pull_chunks<MODE>(a, st, blockIdx.x);
+    else og_pull_rows<MODE>(a, st, blockIdx.x - a.n_chunk_blocks);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {
@@ -457,35 +457,40 @@ __device__ void og_reduce_tail(const og_pr_dev &a, uint32_t it, og_stat mine)
     }
 }
 
+/* Row h < H of one wavefront: lane l adds chunks l, l+64, ... of the row in
+ * order (four loads in flight; lanes past the end re-read the last chunk
+ * and add zero), then a fixed tree over the wavefront; lane 0 runs the
+ * epilogue. */
+template <int MODE>
+__device__ __forceinline__ void og_finalize_long_row(const og_pr_dev &a, uint32_t h, og_stat &st)
+{
+    typedef typename og_mode<MODE>::acc A;
+    const A *__restrict__ cs = (const A *)a.chunk_sum;
+    const uint32_t lane = threadIdx.x & 63;
+    if (a.dd && !a.wl_cur[h]) return;
+    const uint32_t cb = a.chunk_base[h], ce = a.chunk_base[h + 1];
+    A s = 0;
+    for (uint32_t c = cb + lane; c < ce; c += 256)
+    {
+        A x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) x[u] = cs[min(c + 64 * u, ce - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) s += c + 64 * u < ce ? x[u] : (A)0;
+    }
+    s = og_wave_sum(s);
+    if (lane == 0) og_epilogue<MODE>(a, h, s, st);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(OG_BLOCK) void og_pull_finalize_kernel(og_pr_dev a, uint32_t it)
 {
-    typedef typename og_mode<MODE>::acc A;
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    /* Rows [0, H): lane l adds chunks l, l+64, ... of the row in order
-     * (four loads in flight; lanes past the end re-read the last chunk and
-     * add zero), then a fixed tree over the wavefront. */
-    const A *__restrict__ cs = (const A *)a.chunk_sum;
-    const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (uint32_t h = blockIdx.x * OG_WAVES + wave; h < a.H; h += gridDim.x * OG_WAVES)
-    {
-        if (a.dd && !a.wl_cur[h]) continue;
-        const uint32_t cb = a.chunk_base[h], ce = a.chunk_base[h + 1];
-        A s = 0;
-        for (uint32_t c = cb + lane; c < ce; c += 256)
-        {
-            A x[4];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; u++) x[u] = cs[min(c + 64 * u, ce - 1)];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; u++) s += c + 64 * u < ce ? x[u] : (A)0;
-        }
-        s = og_wave_sum(s);
-        if (lane == 0) og_epilogue<MODE>(a, h, s, st);
-    }
+        og_finalize_long_row<MODE>(a, h, st);
     og_reduce_tail(a, it, st);
 }
 
@@ -897,4 +902,130 @@ extern "C" int og_launch_dd_push(const og_pr_dev *a, uint32_t code, uint32_t it,
     hipLaunchKernelGGL(og_dd_push_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, code);
     hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
     return og_hip_check(hipGetLastError(), "og_dd_push_kernel");
+}
+
+/* ------------------------------------------------------------------ */
+/* many graphs in one launch (fp32 pull)                               */
+/* ------------------------------------------------------------------ */
+
+/* Each graph keeps its own plan and buffers (an og_pr_dev per graph and
+ * iteration parity); the kernels below walk one work list that holds the
+ * blocks of every graph, so an iteration of all graphs is three launches
+ * whatever their number and sizes, and a run is capturable in one hipGraph.
+ * A graph that has stopped skips its items; once every graph has stopped
+ * the kernels return at once. */
+
+__global__ __launch_bounds__(OG_BLOCK) void og_multi_init_kernel(og_multi_dev m)
+{
+    for (uint32_t i = blockIdx.x; i < m.n_iwork; i += gridDim.x)
+    {
+        const og_multi_work w = m.iwork[i];
+        const og_pr_dev &a = m.devs[2 * w.g];
+        const uint32_t v = w.first * OG_BLOCK + threadIdx.x;
+        if (v < a.N)
+        {
+            a.r[v] = OG_BASE_PR;
+            og_store_contrib<OG_MODE_F32>((void *)a.g_in, v, OG_BASE_PR, a.outdeg[v]);
+        }
+        if (w.first == 0 && threadIdx.x == 0) *a.done = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        *m.live = m.n_graphs;
+        *m.all_done = m.n_graphs == 0;
+    }
+}
+
+__global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m, uint32_t parity)
+{
+    if (*m.all_done) return;
+    for (uint32_t i = blockIdx.x; i < m.n_work; i += gridDim.x)
+    {
+        const og_multi_work w = m.work[i];
+        const og_pr_dev &a = m.devs[2 * w.g + parity];
+        if (*a.done) continue;
+        og_stat st;
+        og_stat_init(st);
+        if (w.kind == 0) og_pull_chunks<OG_MODE_F32>(a, st, w.first);
+        else og_pull_rows<OG_MODE_F32>(a, st, w.first);
+        og_stat r = og_block_stat(st);
+        if (threadIdx.x == 0)
+        {
+            m.w_err[i] = r.err;
+            m.w_act[i] = r.active;
+        }
+    }
+}
+
+__global__ __launch_bounds__(OG_BLOCK) void og_multi_long_kernel(og_multi_dev m, uint32_t parity)
+{
+    if (*m.all_done) return;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t i = blockIdx.x; i < m.n_lwork; i += gridDim.x)
+    {
+        const og_multi_work w = m.lwork[i];
+        const og_pr_dev &a = m.devs[2 * w.g + parity];
+        if (*a.done) continue;
+        og_stat st;
+        og_stat_init(st);
+        const uint32_t h = w.first * OG_WAVES + wave;
+        if (h < a.H) og_finalize_long_row<OG_MODE_F32>(a, h, st);
+        og_stat r = og_block_stat(st);
+        if (threadIdx.x == 0)
+        {
+            m.l_err[i] = r.err;
+            m.l_act[i] = r.active;
+        }
+    }
+}
+
+/* Block g: graph g's statistics of iteration `it` from its items' partials
+ * in item order, its stop rule (pageRank.c:1030-1033), and the live count. */
+__global__ __launch_bounds__(OG_BLOCK) void og_multi_reduce_kernel(og_multi_dev m, uint32_t parity,
+                                                                  uint32_t it)
+{
+    if (*m.all_done) return;
+    const uint32_t g = blockIdx.x;
+    const og_pr_dev &a = m.devs[2 * g + parity];
+    if (*a.done) return;
+    og_stat st;
+    og_stat_init(st);
+    for (uint32_t i = m.wr[g] + threadIdx.x; i < m.wr[g + 1]; i += OG_BLOCK)
+    {
+        st.err += m.w_err[i];
+        st.active += m.w_act[i];
+    }
+    for (uint32_t i = m.lr[g] + threadIdx.x; i < m.lr[g + 1]; i += OG_BLOCK)
+    {
+        st.err += m.l_err[i];
+        st.active += m.l_act[i];
+    }
+    og_stat r = og_block_stat(st);
+    if (threadIdx.x == 0)
+    {
+        a.stats[it].error_total = r.err;
+        a.stats[it].active = r.active;
+        const bool stop = a.stop_l1 ? (r.err < a.epsilon) : (r.active == 0);
+        if (stop)
+        {
+            *a.done = 1;
+            if (atomicSub(m.live, 1u) == 1u) *m.all_done = 1;
+        }
+    }
+}
+
+extern "C" int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_multi_init_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m);
+    return og_hip_check(hipGetLastError(), "og_multi_init_kernel");
+}
+
+extern "C" int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s)
+{
+    const uint32_t parity = it & 1;
+    hipLaunchKernelGGL(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
+    if (m->n_lwork)
+        hipLaunchKernelGGL(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
+    hipLaunchKernelGGL(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);
+    return og_hip_check(hipGetLastError(), "og_multi_pull_kernel");
 }
