@@ -160,6 +160,45 @@ int ogpu_multi_result(ogpu_multi *m, uint32_t g, ogpu_pr_result *res, float *ran
 uint32_t ogpu_multi_hub_threshold(const ogpu_multi *m, uint32_t g);
 void ogpu_multi_free(ogpu_multi *m);
 
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A per-vertex MLP on degree features, f = [log1p in-degree, log1p
+ * out-degree, mean log1p in-degree of the in-neighbours, 1/N], layers
+ * 4 -> 64 -> 64 -> 1 (ReLU, softplus output), fp32 parameters packed as
+ * W1[64][4] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 (4545 floats).
+ * It predicts the shape of the converged rank vector; the start vector is
+ * that shape renormalized to sum 1 and scaled to the mass the reference's
+ * iteration keeps at its fixed point (0.15 / (0.15 + 0.85 * share of the
+ * shape on vertices without out-edges)).  The unchanged iteration, to the
+ * same tolerance, then runs from it; uniform start is the fallback. */
+typedef struct ogpu_mlp ogpu_mlp;
+int ogpu_mlp_create(uint64_t seed, ogpu_mlp **out);
+/* get (4545 floats) and/or set the parameters (set resets Adam's state). */
+int ogpu_mlp_params(ogpu_mlp *m, float *get, const float *set);
+/* Adds graph g to the training set: features and the target
+ * N * x / sum(x) of its converged ranks x, from a plain pull on the GPU
+ * (uniform start, stop at L1 change < tol, max_iter); *l1_out (may be
+ * NULL) gets the last L1 change. */
+int ogpu_mlp_add_graph(ogpu_mlp *m, ogpu_graph *g, double tol, uint32_t max_iter, double *l1_out);
+uint32_t ogpu_mlp_data_size(const ogpu_mlp *m);
+/* Adam (lr, 0.9, 0.999, 1e-8) on the mean L1 loss |o - y|, mini-batches
+ * of `batch` vertices in a seeded order per epoch; hand-written forward and
+ * backward kernels, deterministic gradient sums. */
+int ogpu_mlp_train(ogpu_mlp *m, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                   double *last_loss, double *seconds);
+/* Mean loss and gradient (4545) of the given vertices (host arrays, n x 4
+ * features and n targets). */
+int ogpu_mlp_grad(ogpu_mlp *m, const float *feat, const float *y, uint32_t n, float *grad,
+                  double *loss);
+/* o for n feature rows (host arrays); mfma = 1: layer 2 on bf16 MFMA. */
+int ogpu_mlp_forward(ogpu_mlp *m, const float *feat, uint32_t n, int mfma, float *out);
+void ogpu_mlp_free(ogpu_mlp *m);
+/* Features of a fp32 pull run's graph, caller's vertex order (N x 4). */
+int ogpu_pr_features(ogpu_pr *pr, float *feat);
+/* Computes the learned start vector on the device (features, bf16 MFMA
+ * inference, mass scaling) and makes the following runs of pr start from
+ * it; *seconds (may be NULL): host time of this call. */
+int ogpu_pr_set_init_learned(ogpu_pr *pr, ogpu_mlp *m, double *seconds);
+
 /* ---- batched personalized PageRank (no counterpart in the reference) ----
  * k <= 64 restart vertices iterated together as a pull-SpMM over an N x k
  * fp32 row-major block: per column exactly the personalized option of

@@ -174,6 +174,17 @@ SIGNATURES = {
     "ogpu_pr_rows": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
     "ogpu_pr_bytes_per_iteration": (C.c_double, [_VP]),
     "ogpu_pr_free": (None, [_VP]),
+    "ogpu_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(_VP)]),
+    "ogpu_mlp_params": (C.c_int, [_VP, _VP, _VP]),
+    "ogpu_mlp_add_graph": (C.c_int, [_VP, _VP, C.c_double, u32, C.POINTER(C.c_double)]),
+    "ogpu_mlp_data_size": (u32, [_VP]),
+    "ogpu_mlp_train": (C.c_int, [_VP, u32, u32, C.c_float, C.c_uint64, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
+    "ogpu_mlp_grad": (C.c_int, [_VP, _VP, _VP, u32, _VP, C.POINTER(C.c_double)]),
+    "ogpu_mlp_forward": (C.c_int, [_VP, _VP, u32, C.c_int, _VP]),
+    "ogpu_mlp_free": (None, [_VP]),
+    "ogpu_pr_features": (C.c_int, [_VP, _VP]),
+    "ogpu_pr_set_init_learned": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
     "ogpu_tune_hub_threshold": (C.c_int, [_VP, u32p, u32, u32, u32p, C.POINTER(C.c_double)]),
     "ogpu_multi_create": (C.c_int, [C.POINTER(_VP), u32, C.POINTER(PrOptions), u32p, C.POINTER(_VP)]),
     "ogpu_multi_run": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
@@ -411,9 +422,81 @@ class PageRank:
     def bytes_per_iteration(self):
         return load().ogpu_pr_bytes_per_iteration(self.h)
 
+    def features(self) -> np.ndarray:
+        """N x 4 learned-start features, caller's vertex order (variant 0)."""
+        out = np.empty((self.graph.num_vertices, 4), np.float32)
+        check(load().ogpu_pr_features(self.h, _ptr(out)))
+        return out
+
+    def set_init_learned(self, mlp: "Mlp") -> float:
+        """Learned start for the following runs; returns the host seconds it took."""
+        s = C.c_double()
+        check(load().ogpu_pr_set_init_learned(self.h, mlp.h, C.byref(s)))
+        return s.value
+
     def free(self):
         if self.h:
             load().ogpu_pr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+MLP_PARAMS = 4545
+
+
+class Mlp:
+    """The learned warm start's network (ogpu_mlp)."""
+
+    def __init__(self, seed=1):
+        self.h = C.c_void_p()
+        check(load().ogpu_mlp_create(seed, C.byref(self.h)))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        check(load().ogpu_mlp_params(self.h, _ptr(out), None))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == MLP_PARAMS
+        check(load().ogpu_mlp_params(self.h, None, _ptr(a)))
+
+    def add_graph(self, graph: Graph, tol=1e-9, max_iter=500) -> float:
+        l1 = C.c_double()
+        check(load().ogpu_mlp_add_graph(self.h, graph.h, tol, max_iter, C.byref(l1)))
+        return l1.value
+
+    def data_size(self) -> int:
+        return load().ogpu_mlp_data_size(self.h)
+
+    def train(self, epochs=10, batch=65536, lr=3e-3, seed=1):
+        """(last epoch's mean loss, seconds)."""
+        loss, sec = C.c_double(), C.c_double()
+        check(load().ogpu_mlp_train(self.h, epochs, batch, lr, seed, C.byref(loss), C.byref(sec)))
+        return loss.value, sec.value
+
+    def grad(self, feat, y):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        check(load().ogpu_mlp_grad(self.h, _ptr(f), _ptr(yy), len(yy), _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def forward(self, feat, mfma=False):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty(len(f), np.float32)
+        check(load().ogpu_mlp_forward(self.h, _ptr(f), len(f), int(mfma), _ptr(out)))
+        return out
+
+    def free(self):
+        if self.h:
+            load().ogpu_mlp_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,61 @@
+/*
+ * og_learn.h -- kernels of the learned warm start (og_learn.hip), shared
+ * with the host side in og_learn.c.
+ *
+ * The network: per-vertex features f = [log1p in-degree, log1p out-degree,
+ * mean log1p in-degree of the in-neighbours, 1/N], then
+ *   h1 = relu(W1 f + b1)  (64),  h2 = relu(W2 h1 + b2)  (64),
+ *   o  = softplus(W3 . h2 + b3),
+ * fp32 parameters packed as W1[64][4] | b1[64] | W2[64][64] (row = output
+ * unit) | b2[64] | W3[64] | b3  (4545 floats).
+ */
+#ifndef OG_LEARN_H
+#define OG_LEARN_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OG_MLP_H 64
+#define OG_MLP_NP 4545
+#define OG_MLP_W1 0
+#define OG_MLP_B1 256
+#define OG_MLP_W2 320
+#define OG_MLP_B2 4416
+#define OG_MLP_W3 4480
+#define OG_MLP_B3 4544
+#define OG_MLP_TRAIN_BLOCK 64    /* vertices per block of the training kernel */
+
+/* feat[v] (float4) from in-degree (row_ptr), out-degree and the in-neighbour
+ * sum of log1p in-degree (insum); n vertices. */
+int og_launch_feat_li(const uint32_t *row_ptr, uint32_t n, float *li, hipStream_t s);
+int og_launch_feat_finish(const uint32_t *row_ptr, const uint32_t *outdeg, const float *insum,
+                          uint32_t n, float *feat, hipStream_t s);
+/* o[v] = MLP(feat[v]) in fp32 (VALU) or with layer 2 on bf16 MFMA. */
+int og_launch_mlp_infer(const float *params, const float *feat, uint32_t n, int mfma, float *out,
+                        hipStream_t s);
+/* One training step's gradient: vertices idx(base .. base+count) of the
+ * dataset, where idx is the Feistel permutation (seed) of [0, n_data)
+ * (identity if seed == 0).  Block partials then a fixed-order reduction into
+ * grad[4545] and loss_sum[0]. */
+int og_launch_mlp_grad(const float *params, const float *feat, const float *y, uint32_t n_data,
+                       uint64_t seed, uint32_t base, uint32_t count, float *part, float *grad,
+                       double *loss_sum, hipStream_t s);
+size_t og_mlp_part_floats(uint32_t count);
+int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
+                       float b2, float eps, uint32_t step, hipStream_t s);
+/* Sums of o over all vertices and over those without out-edges (fp64,
+ * fixed order), then r0[old] = N * S * o[v] / sum(o) with the fixed-point
+ * mass S = 0.15 / (0.15 + 0.85 * dangling share). */
+int og_launch_warm_scale(const float *o, const uint32_t *outdeg, const uint32_t *new2old, uint32_t n,
+                         double *scratch, float *r0_old, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,485 @@
+/*
+ * og_learn.hip -- gfx950 kernels of the learned warm start: degree
+ * features, the MLP's inference (fp32 VALU, and layer 2 on bf16 MFMA), its
+ * training gradient (forward + backward by hand, deterministic block
+ * partials), Adam, and the scaling of the prediction into a start vector.
+ *
+ * The start vector: o = MLP(features) > 0 is the predicted shape; the
+ * reference's iteration (no dangling redistribution, pageRank.c:989-1028)
+ * has the fixed-point mass identity S = 0.15 + 0.85 (S - D), D the mass on
+ * vertices without out-edges, so a start with shape o / sum(o) gets the
+ * mass S = 0.15 / (0.15 + 0.85 delta), delta = sum over those vertices of
+ * o / sum(o).  Unnormalized start ranks: r0 = N * S * o / sum(o).
+ */
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "og_internal.h"
+#include "og_learn.h"
+
+#define LB 256
+
+typedef short og_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float og_f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float og_softplus(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+__device__ __forceinline__ float og_sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+/* fp32 -> bf16 bits, round to nearest even (finite inputs). */
+__device__ __forceinline__ short og_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+/* ------------------------------------------------------------------ */
+/* features                                                            */
+/* ------------------------------------------------------------------ */
+
+__global__ void og_feat_li_kernel(const uint32_t *__restrict__ row_ptr, uint32_t n, float *__restrict__ li)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        li[v] = log1pf((float)(row_ptr[v + 1] - row_ptr[v]));
+}
+
+__global__ void og_feat_finish_kernel(const uint32_t *__restrict__ row_ptr, const uint32_t *__restrict__ outdeg,
+                                      const float *__restrict__ insum, uint32_t n, float4 *__restrict__ feat)
+{
+    const float inv_n = 1.0f / (float)n;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    {
+        const uint32_t din = row_ptr[v + 1] - row_ptr[v];
+        feat[v] = make_float4(log1pf((float)din), log1pf((float)outdeg[v]),
+                              din ? insum[v] / (float)din : 0.0f, inv_n);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* inference                                                           */
+/* ------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(LB) void og_mlp_infer_f32_kernel(const float *__restrict__ P,
+                                                             const float4 *__restrict__ feat, uint32_t n,
+                                                             float *__restrict__ out)
+{
+    __shared__ float sW[OG_MLP_NP];
+    for (uint32_t i = threadIdx.x; i < OG_MLP_NP; i += LB) sW[i] = P[i];
+    __syncthreads();
+    for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
+    {
+        const float4 f = feat[v];
+        float h1[OG_MLP_H];
+#pragma unroll
+        for (int k = 0; k < OG_MLP_H; k++)
+        {
+            float a = sW[OG_MLP_B1 + k];
+            a = fmaf(sW[OG_MLP_W1 + 4 * k + 0], f.x, a);
+            a = fmaf(sW[OG_MLP_W1 + 4 * k + 1], f.y, a);
+            a = fmaf(sW[OG_MLP_W1 + 4 * k + 2], f.z, a);
+            a = fmaf(sW[OG_MLP_W1 + 4 * k + 3], f.w, a);
+            h1[k] = fmaxf(a, 0.0f);
+        }
+        float z3 = sW[OG_MLP_B3];
+        for (int m = 0; m < OG_MLP_H; m++)
+        {
+            float a = sW[OG_MLP_B2 + m];
+#pragma unroll
+            for (int i = 0; i < OG_MLP_H; i++) a = fmaf(sW[OG_MLP_W2 + OG_MLP_H * m + i], h1[i], a);
+            z3 = fmaf(sW[OG_MLP_W3 + m], fmaxf(a, 0.0f), z3);
+        }
+        out[v] = og_softplus(z3);
+    }
+}
+
+/* Layer 2 on MFMA: a wavefront takes 32 vertices at a time; H1 (32 x 64,
+ * bf16) times W2^T (64 x 64, bf16) accumulates in fp32 through
+ * v_mfma_f32_32x32x8_bf16 (two 32-column output tiles, eight k-steps).
+ * Operand layout of that instruction: lane l holds A[l%32][4(l/32)+j] and
+ * B[4(l/32)+j][l%32] (j < 4); result register r of lane l is
+ * D[8(r/4) + 4(l/32) + r%4][l%32].  Layers 1 and 3 stay on the VALU in
+ * fp32; layer 3 is a butterfly over the 32 lanes of each half. */
+__global__ __launch_bounds__(LB) void og_mlp_infer_mfma_kernel(const float *__restrict__ P,
+                                                              const float4 *__restrict__ feat, uint32_t n,
+                                                              float *__restrict__ out)
+{
+    __shared__ float sW1[4 * OG_MLP_H], sB1[OG_MLP_H];
+    for (uint32_t i = threadIdx.x; i < 4 * OG_MLP_H; i += LB) sW1[i] = P[OG_MLP_W1 + i];
+    for (uint32_t i = threadIdx.x; i < OG_MLP_H; i += LB) sB1[i] = P[OG_MLP_B1 + i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, hs = lane >> 5, col = lane & 31;
+    og_bf16x4 bfr[2][8];
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bfr[tt][s][j] = og_bf16(P[OG_MLP_W2 + OG_MLP_H * (32 * tt + col) + 8 * s + 4 * hs + j]);
+    const float b2c0 = P[OG_MLP_B2 + col], b2c1 = P[OG_MLP_B2 + 32 + col];
+    const float w3c0 = P[OG_MLP_W3 + col], w3c1 = P[OG_MLP_W3 + 32 + col];
+    const float b3 = P[OG_MLP_B3];
+    const uint32_t nw = gridDim.x * (LB / 64);
+    for (uint32_t tile = blockIdx.x * (LB / 64) + (threadIdx.x >> 6); (uint64_t)tile * 32 < n; tile += nw)
+    {
+        const uint32_t vb = tile * 32;
+        const float4 f = feat[min(vb + col, n - 1)];
+        og_bf16x4 afr[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+            {
+                const int k = 8 * s + 4 * hs + j;
+                float a = sB1[k];
+                a = fmaf(sW1[4 * k + 0], f.x, a);
+                a = fmaf(sW1[4 * k + 1], f.y, a);
+                a = fmaf(sW1[4 * k + 2], f.z, a);
+                a = fmaf(sW1[4 * k + 3], f.w, a);
+                afr[s][j] = og_bf16(fmaxf(a, 0.0f));
+            }
+        og_f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+        {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(afr[s], bfr[0][s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(afr[s], bfr[1][s], acc1, 0, 0, 0);
+        }
+        float p[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            p[r] = fmaf(w3c0, fmaxf(acc0[r] + b2c0, 0.0f), w3c1 * fmaxf(acc1[r] + b2c1, 0.0f));
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1)
+#pragma unroll
+            for (int r = 0; r < 16; r++) p[r] += __shfl_xor(p[r], o, 64);
+        float mine = p[0];
+#pragma unroll
+        for (int r = 1; r < 16; r++)
+            if (col == (uint32_t)r) mine = p[r];
+        if (col < 16)
+        {
+            const uint32_t row = 8 * (col >> 2) + 4 * hs + (col & 3);
+            if (vb + row < n) out[vb + row] = og_softplus(mine + b3);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* training                                                            */
+/* ------------------------------------------------------------------ */
+
+#define TB OG_MLP_TRAIN_BLOCK
+#define PART_STRIDE (OG_MLP_NP + 1)   /* gradient partial + loss partial per block */
+
+/* Dataset index of step position x: the Feistel permutation of [0, n)
+ * by cycle walking (identity when p.bits == 0). */
+__device__ __forceinline__ uint32_t og_data_index(const og_perm_t &p, uint32_t x, uint32_t n)
+{
+    if (p.bits == 0) return x;
+    do x = og_perm_apply(&p, x);
+    while (x >= n);
+    return x;
+}
+
+/* One block: TB vertices.  Each thread runs its vertex forward and
+ * backward to dz1, dz2 and g3 = dL/dz3 (L1 loss: sign(o - y) * softplus');
+ * the parameter gradients are then block sums over the vertices in index
+ * order, formed from LDS (dW2 = sum dz2 (x) h1, dW1 = sum dz1 (x) f). */
+__global__ __launch_bounds__(TB) void og_mlp_grad_kernel(const float *__restrict__ P,
+                                                        const float4 *__restrict__ feat,
+                                                        const float *__restrict__ y, uint32_t n_data,
+                                                        og_perm_t perm, uint32_t base, uint32_t count,
+                                                        float *__restrict__ part)
+{
+    __shared__ float sH1[TB][OG_MLP_H + 1];   /* h1, later dz1 */
+    __shared__ float sD2[TB][OG_MLP_H + 1];   /* dz2 */
+    __shared__ float sG[TB][OG_MLP_H + 1];    /* g3 * h2 */
+    __shared__ float sF[TB][4];
+    __shared__ float sg3[TB], sloss[TB];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = blockIdx.x * TB + t;
+    const bool valid = i < count;
+    const uint32_t v = valid ? og_data_index(perm, base + i, n_data) : 0;
+    const float4 f = valid ? feat[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    sF[t][0] = f.x; sF[t][1] = f.y; sF[t][2] = f.z; sF[t][3] = f.w;
+    float z1[OG_MLP_H];
+#pragma unroll
+    for (int k = 0; k < OG_MLP_H; k++)
+    {
+        float a = P[OG_MLP_B1 + k];
+        a = fmaf(P[OG_MLP_W1 + 4 * k + 0], f.x, a);
+        a = fmaf(P[OG_MLP_W1 + 4 * k + 1], f.y, a);
+        a = fmaf(P[OG_MLP_W1 + 4 * k + 2], f.z, a);
+        a = fmaf(P[OG_MLP_W1 + 4 * k + 3], f.w, a);
+        z1[k] = a;
+        sH1[t][k] = fmaxf(a, 0.0f);
+    }
+    float z2[OG_MLP_H];
+    float z3 = P[OG_MLP_B3];
+    for (int m = 0; m < OG_MLP_H; m++)
+    {
+        float a = P[OG_MLP_B2 + m];
+#pragma unroll
+        for (int k = 0; k < OG_MLP_H; k++) a = fmaf(P[OG_MLP_W2 + OG_MLP_H * m + k], fmaxf(z1[k], 0.0f), a);
+        z2[m] = a;
+        z3 = fmaf(P[OG_MLP_W3 + m], fmaxf(a, 0.0f), z3);
+    }
+    const float o = og_softplus(z3);
+    const float d = valid ? o - y[v] : 0.0f;
+    const float g3 = (float)((d > 0.0f) - (d < 0.0f)) * og_sigmoid(z3);
+    sg3[t] = g3;
+    sloss[t] = fabsf(d);
+    float dz1[OG_MLP_H];
+#pragma unroll
+    for (int k = 0; k < OG_MLP_H; k++) dz1[k] = 0.0f;
+    for (int m = 0; m < OG_MLP_H; m++)
+    {
+        const float dm = z2[m] > 0.0f ? g3 * P[OG_MLP_W3 + m] : 0.0f;
+        sD2[t][m] = dm;
+        sG[t][m] = g3 * fmaxf(z2[m], 0.0f);
+#pragma unroll
+        for (int k = 0; k < OG_MLP_H; k++) dz1[k] = fmaf(dm, P[OG_MLP_W2 + OG_MLP_H * m + k], dz1[k]);
+    }
+    __syncthreads();
+    float *out = part + (size_t)blockIdx.x * PART_STRIDE;
+    {
+        /* thread t: row t of dW2, db2[t], dW3[t] */
+        float acc[OG_MLP_H];
+#pragma unroll
+        for (int k = 0; k < OG_MLP_H; k++) acc[k] = 0.0f;
+        float db2 = 0.0f, dw3 = 0.0f;
+        for (uint32_t u = 0; u < TB; u++)
+        {
+            const float dm = sD2[u][t];
+            db2 += dm;
+            dw3 += sG[u][t];
+#pragma unroll
+            for (int k = 0; k < OG_MLP_H; k++) acc[k] = fmaf(dm, sH1[u][k], acc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < OG_MLP_H; k++) out[OG_MLP_W2 + OG_MLP_H * t + k] = acc[k];
+        out[OG_MLP_B2 + t] = db2;
+        out[OG_MLP_W3 + t] = dw3;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < OG_MLP_H; k++) sH1[t][k] = z1[k] > 0.0f ? dz1[k] : 0.0f;
+    __syncthreads();
+    {
+        /* thread t: row t of dW1, db1[t] */
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, db1 = 0.0f;
+        for (uint32_t u = 0; u < TB; u++)
+        {
+            const float dz = sH1[u][t];
+            db1 += dz;
+            a0 = fmaf(dz, sF[u][0], a0);
+            a1 = fmaf(dz, sF[u][1], a1);
+            a2 = fmaf(dz, sF[u][2], a2);
+            a3 = fmaf(dz, sF[u][3], a3);
+        }
+        out[OG_MLP_W1 + 4 * t + 0] = a0;
+        out[OG_MLP_W1 + 4 * t + 1] = a1;
+        out[OG_MLP_W1 + 4 * t + 2] = a2;
+        out[OG_MLP_W1 + 4 * t + 3] = a3;
+        out[OG_MLP_B1 + t] = db1;
+    }
+    if (t == 0)
+    {
+        float s = 0.0f, l = 0.0f;
+        for (uint32_t u = 0; u < TB; u++)
+        {
+            s += sg3[u];
+            l += sloss[u];
+        }
+        out[OG_MLP_B3] = s;
+        out[OG_MLP_NP] = l;
+    }
+}
+
+/* grad[p] = (sum over blocks in order of part[b][p]) / count; loss sum. */
+__global__ __launch_bounds__(LB) void og_mlp_grad_reduce_kernel(const float *__restrict__ part, uint32_t nb,
+                                                                uint32_t count, float *__restrict__ grad,
+                                                                double *__restrict__ loss_sum)
+{
+    const uint32_t p = blockIdx.x * LB + threadIdx.x;
+    if (p > OG_MLP_NP) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nb; b++) s += part[(size_t)b * PART_STRIDE + p];
+    if (p < OG_MLP_NP) grad[p] = (float)(s / (double)count);
+    else *loss_sum = s;
+}
+
+__global__ void og_mlp_adam_kernel(float *__restrict__ P, const float *__restrict__ g, float *__restrict__ m1,
+                                   float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1,
+                                   float c2)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= OG_MLP_NP) return;
+    const float gp = g[p];
+    const float a = b1 * m1[p] + (1.0f - b1) * gp;
+    const float b = b2 * m2[p] + (1.0f - b2) * gp * gp;
+    m1[p] = a;
+    m2[p] = b;
+    P[p] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+/* ------------------------------------------------------------------ */
+/* start vector                                                        */
+/* ------------------------------------------------------------------ */
+
+#define WS_BLOCKS 1024
+
+__global__ __launch_bounds__(LB) void og_warm_sums_kernel(const float *__restrict__ o,
+                                                         const uint32_t *__restrict__ outdeg, uint32_t n,
+                                                         double *__restrict__ part)
+{
+    __shared__ double s1[LB], s2[LB];
+    double a = 0.0, b = 0.0;
+    for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
+    {
+        a += o[v];
+        if (!outdeg[v]) b += o[v];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t w = LB / 2; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w)
+        {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(LB) void og_warm_factor_kernel(double *__restrict__ part, uint32_t n)
+{
+    __shared__ double s1[LB], s2[LB];
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = threadIdx.x; i < WS_BLOCKS; i += LB)
+    {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t w = LB / 2; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w)
+        {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        const double tot = s1[0] > 0.0 ? s1[0] : 1.0;
+        const double delta = s2[0] / tot;
+        const double S = 0.15 / (0.15 + 0.85 * delta);
+        part[2 * WS_BLOCKS] = (double)n * S / tot;     /* r0 = factor * o */
+    }
+}
+
+__global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t *__restrict__ new2old,
+                                     uint32_t n, const double *__restrict__ part, float *__restrict__ r0_old)
+{
+    const float fac = (float)part[2 * WS_BLOCKS];
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        r0_old[new2old ? new2old[v] : v] = fac * o[v];
+}
+
+/* ------------------------------------------------------------------ */
+/* launchers                                                           */
+/* ------------------------------------------------------------------ */
+
+static unsigned og_lgrid(uint64_t n)
+{
+    uint64_t g = (n + LB - 1) / LB;
+    if (g < 1) g = 1;
+    if (g > 16384) g = 16384;
+    return (unsigned)g;
+}
+
+extern "C" int og_launch_feat_li(const uint32_t *row_ptr, uint32_t n, float *li, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_feat_li_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, n, li);
+    return og_hip_check(hipGetLastError(), "og_feat_li_kernel");
+}
+
+extern "C" int og_launch_feat_finish(const uint32_t *row_ptr, const uint32_t *outdeg, const float *insum,
+                                     uint32_t n, float *feat, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_feat_finish_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, outdeg, insum, n,
+                       (float4 *)feat);
+    return og_hip_check(hipGetLastError(), "og_feat_finish_kernel");
+}
+
+extern "C" int og_launch_mlp_infer(const float *params, const float *feat, uint32_t n, int mfma, float *out,
+                                   hipStream_t s)
+{
+    if (!n) return 0;
+    if (mfma)
+    {
+        uint64_t tiles = ((uint64_t)n + 31) / 32;
+        unsigned g = (unsigned)((tiles + 3) / 4 < 8192 ? (tiles + 3) / 4 : 8192);
+        hipLaunchKernelGGL(og_mlp_infer_mfma_kernel, dim3(g), dim3(LB), 0, s, params, (const float4 *)feat, n, out);
+    }
+    else
+        hipLaunchKernelGGL(og_mlp_infer_f32_kernel, dim3(og_lgrid(n) < 4096 ? og_lgrid(n) : 4096), dim3(LB), 0,
+                           s, params, (const float4 *)feat, n, out);
+    return og_hip_check(hipGetLastError(), "og_mlp_infer_kernel");
+}
+
+extern "C" size_t og_mlp_part_floats(uint32_t count)
+{
+    return ((size_t)(count + TB - 1) / TB) * PART_STRIDE;
+}
+
+extern "C" int og_launch_mlp_grad(const float *params, const float *feat, const float *y, uint32_t n_data,
+                                  uint64_t seed, uint32_t base, uint32_t count, float *part, float *grad,
+                                  double *loss_sum, hipStream_t s)
+{
+    if (!count) return 0;
+    og_perm_t perm;
+    memset(&perm, 0, sizeof(perm));
+    if (seed)
+    {
+        uint32_t bits = 0;
+        while (bits < 32 && ((uint64_t)1 << bits) < n_data) bits++;
+        perm = og_perm_make(seed, bits);
+    }
+    const uint32_t nb = (count + TB - 1) / TB;
+    hipLaunchKernelGGL(og_mlp_grad_kernel, dim3(nb), dim3(TB), 0, s, params, (const float4 *)feat, y, n_data,
+                       perm, base, count, part);
+    hipLaunchKernelGGL(og_mlp_grad_reduce_kernel, dim3((OG_MLP_NP + LB) / LB), dim3(LB), 0, s, part, nb, count,
+                       grad, loss_sum);
+    return og_hip_check(hipGetLastError(), "og_mlp_grad_kernel");
+}
+
+extern "C" int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
+                                  float b2, float eps, uint32_t step, hipStream_t s)
+{
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(og_mlp_adam_kernel, dim3((OG_MLP_NP + LB - 1) / LB), dim3(LB), 0, s, params, grad, m1, m2,
+                       lr, b1, b2, eps, c1, c2);
+    return og_hip_check(hipGetLastError(), "og_mlp_adam_kernel");
+}
+
+extern "C" int og_launch_warm_scale(const float *o, const uint32_t *outdeg, const uint32_t *new2old, uint32_t n,
+                                    double *scratch, float *r0_old, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, outdeg, n, scratch);
+    hipLaunchKernelGGL(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, scratch, n);
+    hipLaunchKernelGGL(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, new2old, n, scratch, r0_old);
+    return og_hip_check(hipGetLastError(), "og_warm_scale");
+}

@@ -77,6 +77,9 @@ struct ogpu_pr
     uint8_t *wl[2];
     uint8_t *changed;
     float *resid;
+    /* learned start: features (N x 4) and reduction scratch */
+    float *feat;
+    double *lscratch;
 };
 
 static double og_now(void)
@@ -145,7 +148,39 @@ void ogpu_pr_free(ogpu_pr *p)
     free(p->kev);
     free(p->rows_host);
     og_dfree(p->wl[0]); og_dfree(p->wl[1]); og_dfree(p->changed); og_dfree(p->resid);
+    og_dfree(p->feat); og_dfree(p->lscratch);
     free(p);
+}
+
+/* Buffers the learned start (og_learn.c) borrows from a fp32 pull run: its
+ * kernel arguments and plan, c[1] as scratch, the start-vector buffer and
+ * the feature array (allocated on first use). */
+int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, uint32_t *n_final,
+                      float **c1, float **init_d, float **feat, double **scratch)
+{
+    if (p->mode != OG_MODE_F32 || p->push || p->dd)
+    {
+        og_set_error("learned start: fp32 pull runs (variant 0) only");
+        return -1;
+    }
+    const size_t N = p->g->N;
+    if (!p->init_d && !(p->init_d = og_dalloc(N * 4 + 4))) return -1;
+    if (!p->feat && !(p->feat = og_dalloc(N * 16 + 16))) return -1;
+    if (!p->lscratch && !(p->lscratch = og_dalloc(4096 * 8))) return -1;
+    *dev = &p->dev;
+    *plan = p->plan;
+    *n_final = p->n_final_pull;
+    *c1 = (float *)p->c[1];
+    *init_d = p->init_d;
+    *feat = p->feat;
+    *scratch = p->lscratch;
+    return 0;
+}
+
+/* The next runs start from the vector in init_d. */
+void og_pr_use_init(ogpu_pr *p)
+{
+    p->opt.init_mode = 2;
 }
 
 /* Create a run on graph g.  plan may be supplied (borrowed) or built here. */

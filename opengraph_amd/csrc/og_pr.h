@@ -48,6 +48,7 @@ typedef struct og_pr_dev
     uint32_t qmode;          /* 0 or the quantized mode whose parameters the final step computes */
     uint32_t qzero;          /* compute a zero point (quantized push) */
     uint32_t allow_stop;     /* final reduction may raise *done */
+    uint32_t raw;            /* fp32 pull writes the plain in-neighbour sum to r (features) */
     uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
     uint32_t ppr_src;        /* new id of the restart vertex */
     double epsilon;

@@ -168,6 +168,11 @@ __device__ __forceinline__ void og_epilogue_pf(const og_pr_dev &a, uint32_t v,
     float nr;
     if constexpr (MODE == OG_MODE_F32)
     {
+        if (a.raw)
+        {
+            a.r[v] = sum;            /* sum over in-neighbours of g_in (feature pass) */
+            return;
+        }
         /* personalized: the restart term (1 - d) only at the source */
         const float base = !a.ppr || v == a.ppr_src ? OG_BASE_PR : 0.0f;
         nr = base + OG_DAMP * sum;

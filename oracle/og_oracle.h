@@ -107,6 +107,16 @@ uint32_t orc_ppr_fp64(const orc_csr *g, uint32_t source, double tol, uint32_t ma
 uint32_t orc_ppr_f32(const orc_csr *g, uint32_t source, double tol, uint32_t max_iter,
                      float *x_out, double *l1_hist);
 
+/* Learned warm start (og_oracle_mlp.c, no counterpart in the reference):
+ * features [log1p in-degree, log1p out-degree, mean log1p in-degree of the
+ * in-neighbours, 1/N] per vertex (feat: N x 4); the 4 -> 64 -> 64 -> 1
+ * ReLU MLP with softplus output in double precision; its mean L1 loss
+ * against y and the gradient of that loss (4545 entries). */
+void orc_mlp_features(const orc_csr *g, float *feat);
+void orc_mlp_forward(const float *params, const float *feat, uint32_t n, double *out);
+double orc_mlp_loss(const float *params, const float *feat, const float *y, uint32_t n);
+double orc_mlp_grad(const float *params, const float *feat, const float *y, uint32_t n, double *grad);
+
 /* reorder.c:64-221: stable ascending sort of labels by FloatToFixed32SORT(rank). */
 void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n);
 

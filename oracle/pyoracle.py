@@ -62,6 +62,12 @@ def lib():
         L.orc_ppr_fp64.restype = C.c_uint32
         L.orc_ppr_f32.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, f32p, f64p]
         L.orc_ppr_f32.restype = C.c_uint32
+        L.orc_mlp_features.argtypes = [C.POINTER(OrcCSR), f32p]
+        L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_uint32, f64p]
+        L.orc_mlp_loss.argtypes = [f32p, f32p, f32p, C.c_uint32]
+        L.orc_mlp_loss.restype = C.c_double
+        L.orc_mlp_grad.argtypes = [f32p, f32p, f32p, C.c_uint32, f64p]
+        L.orc_mlp_grad.restype = C.c_double
         L.orc_sort_labels_by_rank.argtypes = [f32p, u32p, C.c_uint32]
         L.orc_stable_sort_pairs.argtypes = [u32p, u32p, C.c_uint32, C.c_int]
         L.orc_stable_sort_pairs.restype = C.c_int
@@ -163,6 +169,40 @@ def ppr_f32(csr: CSR, source, tol=1e-6, max_iter=100):
     hist = np.zeros(max(max_iter, 1), np.float64)
     it = L.orc_ppr_f32(C.byref(csr.s), source, tol, max_iter, _p(out, f32p), _p(hist, f64p))
     return out, it, hist[:it]
+
+
+MLP_PARAMS = 4545
+
+
+def mlp_features(csr: CSR):
+    out = np.zeros((csr.num_vertices, 4), np.float32)
+    lib().orc_mlp_features(C.byref(csr.s), _p(out, f32p))
+    return out
+
+
+def mlp_forward(params, feat):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feat, dtype=np.float32)
+    out = np.zeros(len(f), np.float64)
+    lib().orc_mlp_forward(_p(p, f32p), _p(f, f32p), len(f), _p(out, f64p))
+    return out
+
+
+def mlp_loss(params, feat, y):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feat, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32)
+    return lib().orc_mlp_loss(_p(p, f32p), _p(f, f32p), _p(yy, f32p), len(f))
+
+
+def mlp_grad(params, feat, y):
+    """(loss, gradient[4545]) in double precision."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feat, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = lib().orc_mlp_grad(_p(p, f32p), _p(f, f32p), _p(yy, f32p), len(f), _p(g, f64p))
+    return loss, g
 
 
 def sort_labels_by_rank(ranks, labels):

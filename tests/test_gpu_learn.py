@@ -1,0 +1,161 @@
+"""Learned warm start on the GPU, through the C ABI.
+
+Bars: features within 1e-5 relative of the oracle's double-precision ones
+(fp32 log1p and a sum in another order); fp32 inference within 1e-5 of the
+oracle's double forward; the MFMA inference within 1e-4 of a numpy model of
+the same bf16 roundings (and within bf16 tolerance of fp32); the hand-written
+backward within 1e-3 of the oracle gradient (itself pinned by finite
+differences on the CPU); training lowers the loss; the warm-started run
+stops at the same tolerance on the same fixed point (FP64 oracle L1 <= 1e-5)
+and the trained start is closer to it than the uniform and untrained ones.
+"""
+import numpy as np
+import pytest
+
+import opengraph_amd as og
+from conftest import golden_input
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    og.init(0)
+
+
+@pytest.fixture(scope="module")
+def rmat16(golden):
+    s, d = golden_input("rmat16", golden)
+    g = og.Graph.from_edges(s, d, 0)
+    o = po.build_csr(s, d, 0, 8)
+    yield g, o
+    g.free()
+
+
+def _rand_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(og.MLP_PARAMS, np.float32)
+    p[0:256] = rng.normal(0, np.sqrt(2 / 4), 256)
+    p[256:320] = rng.normal(0, 0.1, 64)
+    p[320:4416] = rng.normal(0, np.sqrt(2 / 64), 4096)
+    p[4416:4480] = rng.normal(0, 0.1, 64)
+    p[4480:4544] = rng.normal(0, np.sqrt(1 / 64), 64)
+    p[4544] = 0.2
+    return p
+
+
+def _bf16(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def test_features_match_oracle(rmat16):
+    g, o = rmat16
+    pr = og.PageRank(g, 0, 10, 1e-6)
+    f = pr.features()
+    pr.free()
+    ref = po.mlp_features(o)
+    np.testing.assert_allclose(f[:, :2], ref[:, :2], rtol=1e-6)
+    np.testing.assert_allclose(f[:, 2], ref[:, 2], rtol=1e-5, atol=1e-6)
+    assert np.all(f[:, 3] == ref[:, 3])
+
+
+def test_forward_fp32_and_mfma(rmat16):
+    _, o = rmat16
+    feat = po.mlp_features(o)[:20000]
+    p = _rand_params(3)
+    m = og.Mlp(1)
+    m.set_params(p)
+    ref = po.mlp_forward(p, feat)
+    out = m.forward(feat, mfma=False)
+    np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
+    # MFMA path: bf16 h1 and W2, fp32 accumulation -- numpy model of exactly that
+    W1, b1 = p[0:256].reshape(64, 4), p[256:320]
+    W2, b2 = p[320:4416].reshape(64, 64), p[4416:4480]
+    W3, b3 = p[4480:4544], p[4544]
+    h1 = np.maximum(feat.astype(np.float64) @ W1.T.astype(np.float64) + b1, 0).astype(np.float32)
+    z2 = _bf16(h1).astype(np.float64) @ _bf16(W2).T.astype(np.float64) + b2
+    z3 = np.maximum(z2, 0) @ W3 + b3
+    model = np.where(z3 > 20, z3, np.log1p(np.exp(np.minimum(z3, 20))))
+    mf = m.forward(feat, mfma=True)
+    m.free()
+    np.testing.assert_allclose(mf, model, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(mf, ref, rtol=5e-2, atol=5e-3)
+
+
+def test_backward_matches_oracle(rmat16):
+    _, o = rmat16
+    rng = np.random.default_rng(5)
+    feat = po.mlp_features(o)[rng.choice(o.num_vertices, 6000, replace=False)]
+    p = _rand_params(4)
+    y = (po.mlp_forward(p, feat) * rng.uniform(0.5, 1.5, len(feat))).astype(np.float32)
+    m = og.Mlp(1)
+    m.set_params(p)
+    loss, g = m.grad(feat, y)
+    m.free()
+    lref, gref = po.mlp_grad(p, feat, y)
+    assert loss == pytest.approx(lref, rel=1e-5)
+    scale = np.abs(gref).max()
+    np.testing.assert_allclose(g, gref, rtol=1e-3, atol=1e-4 * scale)
+
+
+def _graphs(scale, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        a = float(rng.uniform(0.45, 0.65))
+        b = (1 - a - 0.05) / 2
+        ef = int(rng.integers(8, 33))
+        s, d = og.rmat_host(scale, ef << scale, a, b, b, seed * 1000 + i, True)
+        out.append((s, d, og.Graph.from_edges(s, d, 0)))
+    return out
+
+
+def test_training_and_warm_start():
+    train = _graphs(12, 6, 6)
+    m = og.Mlp(7)
+    untrained = og.Mlp(7)
+    for _, _, g in train:
+        l1 = m.add_graph(g, 1e-10, 1000)
+        assert l1 < 1e-10
+    assert m.data_size() == sum(g.num_vertices for _, _, g in train)
+    loss0, _ = m.train(1, 16384, 0.0)
+    loss, sec = m.train(30, 16384, 3e-3)
+    print("loss", loss0, "->", loss, "seconds", sec, flush=True)
+    assert loss < 0.7 * loss0
+    s, d, g = _graphs(14, 1, 101)[0]
+    o = po.build_csr(s, d, 0, 8)
+    x64, _, _ = po.pagerank_fp64(o, 1.0, 1e-14, 3000, 8)
+    N = g.num_vertices
+    res = {}
+    for name, net in (("uniform", None), ("untrained", untrained), ("trained", m)):
+        pr = og.PageRank(g, 0, 500, 1e-6, stop_rule=1, init_mode=1)
+        if net is not None:
+            pr.set_init_learned(net)
+        r = pr.run()
+        x = pr.ranks()
+        pr.free()
+        res[name] = r.rows
+        err = np.abs(x.astype(np.float64) - x64).sum()
+        print(name, "iterations", r.rows, "L1 vs fp64", err, flush=True)
+        assert err <= 1e-5, name
+    # start-vector error against the fixed point
+    starts = {}
+    for name, net in (("untrained", untrained), ("trained", m)):
+        pr = og.PageRank(g, 0, 1, 1e-6, stop_rule=1, init_mode=1)
+        pr.set_init_learned(net)
+        f = pr.features()
+        pr.free()
+        q = net.forward(f, mfma=False).astype(np.float64)
+        xh = q / q.sum()
+        dang = o["f_out_degree"] == 0
+        starts[name] = np.abs(xh * 0.15 / (0.15 + 0.85 * xh[dang].sum()) - x64).sum()
+    starts["uniform"] = np.abs(np.full(N, 1.0 / N) - x64).sum()
+    print("start L1 to the fixed point", starts, "iterations", res, flush=True)
+    assert starts["trained"] < starts["untrained"] and starts["trained"] < starts["uniform"]
+    m.free(); untrained.free()
+    for _, _, g in train:
+        g.free()
+    g.free()

@@ -157,8 +157,11 @@ def test_batched_duplicate_and_dangling_sources(rmat16):
 
 
 def test_batched_iteration_cap(rmat16):
-    g, _ = rmat16
-    p = og.Ppr(g, _sources(g.num_vertices, 4, 9), 5, 1e-12)
+    """Sources with out-edges (a source without any converges in two
+    iterations): all four columns run into the cap."""
+    g, o = rmat16
+    src = np.flatnonzero(o["f_out_degree"] > 0)[[3, 50, 500, 5000]].astype(np.uint32)
+    p = og.Ppr(g, src, 5, 1e-12)
     res = p.run()
     iters, hist = p.columns()
     p.free()
