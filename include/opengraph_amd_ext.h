@@ -82,8 +82,10 @@ typedef struct
     uint32_t stop_rule;     /* 0: activeVertices == 0 (reference); 1: error_total < epsilon */
     uint32_t init_mode;     /* 0: 0.15 (reference); 1: uniform 1.0 (= 1/N normalized);
                                2: caller's vector (unnormalized, old ids) */
-    uint32_t relabel;       /* 1: degree-ordered rows and columns (default); 0: rows grouped
-                               by in-degree class only, original order within a class */
+    uint32_t relabel;       /* 1: degree-ordered rows and columns, rows of more than 512
+                               in-edges with their in-neighbours sorted by source (default);
+                               2: the same without that sort; 0: rows grouped by in-degree
+                               class only, original order within a class */
     uint32_t hub_threshold; /* rows with more in-edges are split into 512-edge chunks summed
                                in a fixed tree; rows with at most this many are summed in
                                the reference's order (bit-exact); 0: default 32;
@@ -92,6 +94,11 @@ typedef struct
                                e_s, x' = (1-d)[v==s] + d * sum, stop when the L1 change
                                sum |x'-x| < epsilon, ranks not divided by N */
     uint32_t ppr_source;    /* s, the caller's vertex id */
+    uint32_t hot_cache;     /* pull (variants 0, 2, 4, 9-13): 0 default (on when the graph
+                               exceeds 4x the copy and its cached sources send >= 1/5 of the
+                               edges), 1 on, 2 off -- persistent blocks
+                               that serve the gathers of the lowest vertex ids (the hottest
+                               after relabelling) from a 64 KB LDS copy; same results */
 } ogpu_pr_options;
 
 typedef struct

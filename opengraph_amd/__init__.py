@@ -89,7 +89,7 @@ class PrOptions(C.Structure):
     _fields_ = [
         ("variant", u32), ("max_iter", u32), ("epsilon", C.c_double), ("stop_rule", u32),
         ("init_mode", u32), ("relabel", u32), ("hub_threshold", u32), ("ppr", u32),
-        ("ppr_source", u32),
+        ("ppr_source", u32), ("hot_cache", u32),
     ]
 
 
@@ -373,7 +373,7 @@ class PageRank:
     """A prepared PageRank run on a device graph (ogpu_pr)."""
 
     def __init__(self, graph: Graph, variant=0, max_iter=20, epsilon=1e-4, stop_rule=0,
-                 init_mode=0, relabel=1, hub_threshold=0, init=None, ppr_source=None):
+                 init_mode=0, relabel=1, hub_threshold=0, init=None, ppr_source=None, hot_cache=0):
         lib = load()
         o = PrOptions()
         lib.ogpu_pr_default_options(C.byref(o))
@@ -381,6 +381,7 @@ class PageRank:
         o.stop_rule, o.init_mode, o.relabel, o.hub_threshold = stop_rule, init_mode, relabel, hub_threshold
         if ppr_source is not None:
             o.ppr, o.ppr_source = 1, ppr_source
+        o.hot_cache = hot_cache
         self.graph = graph
         self.h = C.c_void_p()
         check(lib.ogpu_pr_create(graph.h, C.byref(o), C.byref(self.h)))

@@ -226,6 +226,50 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
     if (E && og_launch_plan_copy(H, N, p->new2old, p->old2new, g->i_edges_idx, g->i_dst, p->row_ptr,
                                  p->cols, s))
         goto done;
+    /* 4b. rows [0, H) (several chunks; their sums are a fixed tree, not the
+     *     CSR order): in-neighbours sorted by source id, so the 64 lanes of
+     *     a chunk gather neighbouring contributions -- with the degree
+     *     order, often the same cache line.  relabel 2 keeps CSR order. */
+    if (relabel == 1 && H)
+    {
+        uint32_t EH = 0;
+        if (og_hip_check(hipMemcpyAsync(&EH, p->row_ptr + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
+            og_hip_check(hipStreamSynchronize(s), "sync plan"))
+            goto done;
+        uint32_t *rid = og_dalloc((size_t)EH * 4 + 4), *ka = og_dalloc((size_t)EH * 4 + 4),
+                 *va = og_dalloc((size_t)EH * 4 + 4), *kb = og_dalloc((size_t)EH * 4 + 4),
+                 *vb = og_dalloc((size_t)EH * 4 + 4);
+        void *rscr = og_dalloc(og_radix_scratch_bytes(EH));
+        int bad = !rid || !ka || !va || !kb || !vb || !rscr;
+        uint32_t nbits = 1, hbits = 1;
+        while (nbits < 32 && (1ull << nbits) < N) nbits++;
+        while (hbits < 32 && (1ull << hbits) < H) hbits++;
+        int in_b = 0;
+        if (!bad)
+        {
+            /* by source (stable), then by row (stable): (row, source) order */
+            bad = og_launch_plan_row_ids(H, p->row_ptr, rid, s) ||
+                  og_radix_sort_pairs(p->cols, rid, ka, va, kb, vb, EH, nbits, rscr, s, &in_b);
+        }
+        if (!bad)
+        {
+            uint32_t *k1 = in_b ? kb : ka, *v1 = in_b ? vb : va;
+            /* second sort reads (row, source) from the first's result; its
+             * outputs alternate (ka|kb not holding that result) and rid/cols */
+            uint32_t *oa_k = in_b ? ka : kb, *oa_v = in_b ? va : vb;
+            int in_b2 = 0;
+            bad = og_radix_sort_pairs(v1, k1, oa_k, oa_v, rid, k1, EH, hbits, rscr, s, &in_b2);
+            if (!bad)
+            {
+                const uint32_t *src_sorted = in_b2 ? k1 : oa_v;
+                bad = og_hip_check(hipMemcpyAsync(p->cols, src_sorted, (size_t)EH * 4, hipMemcpyDeviceToDevice, s),
+                                   "copy sorted rows") ||
+                      og_hip_check(hipStreamSynchronize(s), "sync plan");
+            }
+        }
+        og_dfree(rid); og_dfree(ka); og_dfree(va); og_dfree(kb); og_dfree(vb); og_dfree(rscr);
+        if (bad) goto done;
+    }
     /* 5. chunks of the rows [0, C): key = chunks per row, chunk_first = its prefix */
     {
         uint32_t *first = og_dalloc((size_t)(C + 1) * 4);

@@ -113,7 +113,7 @@ static int og_plan_features(const og_pr_dev *base, const og_plan *pl, uint32_t n
     a.dd = 0;
     a.ppr = 0;
     a.allow_stop = 0;
-    a.n_parts = pl->n_chunk_blocks + pl->n_row_blocks;
+    a.n_parts = og_pull_parts(&a);
     a.n_final_blocks = n_final;
     if (og_launch_fill_u32(a.done, 0, 1, s) || og_launch_feat_li(pl->row_ptr, pl->N, c_scratch, s) ||
         og_launch_pull(&a, 0, OG_NO_STATS, s, NULL, NULL) ||

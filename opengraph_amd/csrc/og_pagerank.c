@@ -229,7 +229,33 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     p->gv = (N + 255) / 256;
     if (p->gv > 2048) p->gv = 2048;
     if (p->gv == 0) p->gv = 1;
-    const uint32_t pull_blocks = pl->n_chunk_blocks + pl->n_row_blocks;
+    /* pull as the persistent kernel with an LDS copy of the hottest
+     * contributions: by default when the graph is several times larger
+     * than that copy */
+    uint32_t hot_grid = 0;
+    if (!p->push && !p->dd && opt->hot_cache != 2)
+    {
+        int on = opt->hot_cache == 1;
+        const uint32_t nh = (uint32_t)(OG_HOT_BYTES / og_elem_bytes(p->mode));
+        if (!on && (uint64_t)N * og_elem_bytes(p->mode) > 4ull * OG_HOT_BYTES && E)
+        {
+            /* worth it when the cached sources send at least a fifth of the edges */
+            uint32_t *deg = (uint32_t *)malloc((size_t)nh * 4);
+            if (!deg) { og_set_error("out of host memory"); ogpu_pr_free(p); return -1; }
+            if (og_hip_check(hipMemcpy(deg, pl->outdeg, (size_t)nh * 4, hipMemcpyDeviceToHost), "D2H"))
+            {
+                free(deg);
+                ogpu_pr_free(p);
+                return -1;
+            }
+            uint64_t hot_edges = 0;
+            for (uint32_t v = 0; v < nh; v++) hot_edges += deg[v];
+            free(deg);
+            on = hot_edges * 5 >= (uint64_t)E;
+        }
+        if (on) hot_grid = (uint32_t)og_num_cus() * (163840u / OG_HOT_BYTES);
+    }
+    const uint32_t pull_blocks = hot_grid ? hot_grid : pl->n_chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv, 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
@@ -294,6 +320,7 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->n_chunks = pl->n_chunks;
     a->n_chunk_blocks = pl->n_chunk_blocks;
     a->n_row_blocks = pl->n_row_blocks;
+    a->hot_grid = hot_grid;
     a->stop_l1 = opt->stop_rule == 1 || opt->ppr;
     a->ppr = opt->ppr ? 1 : 0;
     a->ppr_src = 0;
@@ -413,7 +440,7 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     int rc;
     if (!p->push)
     {
-        a.n_parts = p->plan->n_chunk_blocks + p->plan->n_row_blocks;
+        a.n_parts = og_pull_parts(&a);
         a.n_final_blocks = p->n_final_pull;
         rc = og_launch_pull(&a, p->mode, row, s, e0, e1);
     }
@@ -668,7 +695,7 @@ static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res)
                 b.c_out = p->c[0];
                 b.g_in = p->c[0];
                 if (og_launch_dd_contrib(&b, s)) return -1;
-                b.n_parts = p->plan->n_chunk_blocks + p->plan->n_row_blocks;
+                b.n_parts = og_pull_parts(&b);
                 b.n_final_blocks = p->n_final_pull;
                 if (og_launch_pull(&b, OG_MODE_F32, it, s, NULL, NULL)) return -1;
                 if (og_launch_dd_activate(&b, s)) return -1;

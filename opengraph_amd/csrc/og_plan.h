@@ -29,6 +29,8 @@ int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *out
 int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2old, const uint32_t *old2new,
                         const uint32_t *idx_old, const uint32_t *lists, const uint32_t *row_ptr,
                         uint32_t *cols, hipStream_t s);
+/* rowid[e] = r for every edge e of the rows [0, R). */
+int og_launch_plan_row_ids(uint32_t R, const uint32_t *row_ptr, uint32_t *rowid, hipStream_t s);
 /* cnt[r] = chunks of row r < C (ceil(in-degree / ch)). */
 int og_launch_plan_chunks(uint32_t C, const uint32_t *row_ptr, uint32_t ch, uint32_t *cnt,
                           hipStream_t s);

@@ -202,6 +202,26 @@ __global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_hubs_kernel(
     for (uint32_t k = threadIdx.x; k < n; k += OG_BLOCK) cols[dst + k] = old2new[lists[src + k] & OG_ID_MASK];
 }
 
+/* rowid[e] = r for the edges of rows [0, R), one wavefront per row. */
+__global__ __launch_bounds__(OG_BLOCK) void og_plan_row_ids_kernel(uint32_t R, const uint32_t *__restrict__ row_ptr,
+                                                                  uint32_t *__restrict__ rowid)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t r = wave; r < R; r += nwaves)
+        for (uint32_t e = row_ptr[r] + lane; e < row_ptr[r + 1]; e += 64) rowid[e] = r;
+}
+
+extern "C" int og_launch_plan_row_ids(uint32_t R, const uint32_t *row_ptr, uint32_t *rowid, hipStream_t s)
+{
+    if (!R) return 0;
+    uint64_t waves = R;
+    unsigned grid = (unsigned)((waves + 3) / 4 < 65536 ? (waves + 3) / 4 : 65536);
+    hipLaunchKernelGGL(og_plan_row_ids_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, R, row_ptr, rowid);
+    return og_hip_check(hipGetLastError(), "og_plan_row_ids_kernel");
+}
+
 __global__ void og_plan_chunk_count_kernel(uint32_t C, const uint32_t *__restrict__ row_ptr,
                                            uint32_t ch, uint32_t *__restrict__ cnt)
 {

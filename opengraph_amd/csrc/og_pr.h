@@ -48,6 +48,7 @@ typedef struct og_pr_dev
     uint32_t qmode;          /* 0 or the quantized mode whose parameters the final step computes */
     uint32_t qzero;          /* compute a zero point (quantized push) */
     uint32_t allow_stop;     /* final reduction may raise *done */
+    uint32_t hot_grid;       /* > 0: pull runs as og_pull_hot_kernel on this many blocks */
     uint32_t raw;            /* fp32 pull writes the plain in-neighbour sum to r (features) */
     uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
     uint32_t ppr_src;        /* new id of the restart vertex */
@@ -83,6 +84,16 @@ typedef struct og_pr_dev
 
 int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
+
+/* LDS bytes of contributions a block of the persistent pull keeps (two
+ * blocks per CU fit the 160 KB of LDS). */
+#define OG_HOT_BYTES 65536u
+
+/* First-level statistics partials one pull launch writes. */
+static inline uint32_t og_pull_parts(const og_pr_dev *a)
+{
+    return a->hot_grid ? a->hot_grid : a->n_chunk_blocks + a->n_row_blocks;
+}
 
 /* ---- many graphs in one launch (fp32 pull) ----
  * Work item: block `first` of graph g's pull (kind 0 chunk block, 1 row

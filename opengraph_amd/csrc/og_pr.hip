@@ -270,8 +270,28 @@ template <typename T> __device__ __forceinline__ T og_wave_allsum(T v)
  * its last edge and add zero, so no branch separates the loads and the
  * K*OG_CH_SLOTS gathers of a lane overlap.  Lane k then stores chunk k's
  * sum (row with several chunks) or runs the row's epilogue. */
-template <int MODE>
-__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock)
+/* Gather g[i].  With a hot cache (HOT) the first nh entries come from LDS
+ * and those lanes' global load is pointed at g[0] -- one line, an L1 hit --
+ * so only the other lanes send requests to L2, whose request rate bounds
+ * the scattered gathers.  Branch-free: both loads of every slot stay in
+ * flight together. */
+template <typename E, bool HOT>
+__device__ __forceinline__ E og_gather(const E *__restrict__ g, const E *hot, uint32_t nh, uint32_t i)
+{
+    if constexpr (!HOT) return g[i];
+    else
+    {
+        const bool h = i < nh;
+        const E xg = g[h ? 0u : i];
+        const E xh = hot[h ? i : 0u];
+        return h ? xh : xg;
+    }
+}
+
+template <int MODE, bool HOT = false>
+__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock,
+                                               const typename og_mode<MODE>::elem *hot = nullptr,
+                                               uint32_t nh = 0)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
@@ -309,7 +329,7 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
 #pragma unroll
     for (uint32_t k = 0; k < K; k++)
 #pragma unroll
-        for (uint32_t u = 0; u < S; u++) val[k][u] = g[idx[k][u]];
+        for (uint32_t u = 0; u < S; u++) val[k][u] = og_gather<E, HOT>(g, hot, nh, idx[k][u]);
     A s[K];
 #pragma unroll
     for (uint32_t k = 0; k < K; k++)
@@ -343,8 +363,10 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
  * here, so the 64 rows of a wavefront take about the same number of
  * batches.  Lanes past their row's end re-read its last edge and add zero:
  * x + 0 == x for every partial sum, which starts at +0. */
-template <int MODE>
-__device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock)
+template <int MODE, bool HOT = false>
+__device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock,
+                                             const typename og_mode<MODE>::elem *hot = nullptr,
+                                             uint32_t nh = 0)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
@@ -365,7 +387,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
             idx[u] = __builtin_nontemporal_load(&a.cols[min(j + u, e - 1)]);
         E val[OG_ROW_BATCH];
 #pragma unroll
-        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) val[u] = g[idx[u]];
+        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) val[u] = og_gather<E, HOT>(g, hot, nh, idx[u]);
 #pragma unroll
         for (uint32_t u = 0; u < OG_ROW_BATCH; u++) s += j + u < e ? (A)val[u] : (A)0;
     }
@@ -381,6 +403,40 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     og_stat_init(st);
     if (blockIdx.x < a.n_chunk_blocks) og_pull_chunks<MODE>(a, st, blockIdx.x);
     else og_pull_rows<MODE>(a, st, blockIdx.x - a.n_chunk_blocks);
+    og_stat r = og_block_stat(st);
+    if (threadIdx.x == 0)
+    {
+        a.part_err[blockIdx.x] = r.err;
+        a.part_act[blockIdx.x] = r.active;
+        a.part_min[blockIdx.x] = r.cmin;
+        a.part_max[blockIdx.x] = r.cmax;
+    }
+}
+
+/* The same pull as a persistent kernel whose blocks first copy the
+ * OG_HOT_BYTES of contributions with the smallest ids -- after the degree
+ * relabelling, those of the vertices with the most in- and out-edges -- into
+ * LDS and serve those gathers from there; block b then takes the work
+ * blocks b, b + grid, ... (statistics partial per block). */
+template <int MODE>
+__global__ __launch_bounds__(OG_BLOCK) void og_pull_hot_kernel(og_pr_dev a, uint32_t it)
+{
+    typedef typename og_mode<MODE>::elem E;
+    constexpr uint32_t NH = OG_HOT_BYTES / sizeof(E);
+    __shared__ E hot[NH];
+    if (*a.done) return;
+    const E *__restrict__ g = (const E *)a.g_in;
+    const uint32_t nh = min(NH, a.N);
+    for (uint32_t i = threadIdx.x; i < nh; i += OG_BLOCK) hot[i] = g[i];
+    __syncthreads();
+    og_stat st;
+    og_stat_init(st);
+    const uint32_t items = a.n_chunk_blocks + a.n_row_blocks;
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x)
+    {
+        if (w < a.n_chunk_blocks) og_pull_chunks<MODE, true>(a, st, w, hot, nh);
+        else og_pull_rows<MODE, true>(a, st, w - a.n_chunk_blocks, hot, nh);
+    }
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {
@@ -680,7 +736,10 @@ template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it
 {
     uint32_t grid = a->n_chunk_blocks + a->n_row_blocks;
     if (ev0) hipEventRecord(ev0, s);
-    if (grid) hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
+    if (a->hot_grid)
+        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_BLOCK), 0, s, *a, it);
+    else if (grid)
+        hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     if (ev1) hipEventRecord(ev1, s);
     hipLaunchKernelGGL(og_pull_finalize_kernel<MODE>, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s,
                        *a, it);

@@ -289,3 +289,17 @@ def test_rank_label_post_sort_matches_reference(golden, golden_ranks):
         lib.radixSortEdgesByPageRank(r.ctypes.data_as(C.POINTER(C.c_float)),
                                      lab.ctypes.data_as(C.POINTER(C.c_uint32)), len(r))
         assert sha(lab) == c["labels_sha256"]
+
+
+@pytest.mark.parametrize("code", [0, 2, 4, 11])
+@pytest.mark.parametrize("name", ["enron", "rmat16"])
+def test_hot_cache_same_results(graphs, name, code):
+    """The persistent pull with the LDS copy of the hottest contributions
+    computes the same sums in the same order: identical bits and stats."""
+    g = graphs(name, 0)
+    out = []
+    for hot in (1, 2):
+        res, ranks, act, err = _run(g, code, 1e-7, 30, hot_cache=hot)
+        out.append((ranks, act, res.iterations))
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1], out[1][1]) and out[0][2] == out[1][2]
