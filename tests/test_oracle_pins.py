@@ -76,8 +76,11 @@ def test_oracle_pagerank_matches_reference(golden, csr_cache):
             assert list(r["active"]) == c["active"], c
             assert r["error_total"] == pytest.approx(c["error_total"], rel=1e-9, abs=1e-300)
         else:
+            # fp32 atomic push: the order of the adds, hence the last bits of
+            # the ranks and when the last vertex drops below epsilon, varies
+            # from run to run in the reference as well
             assert c["code"] in (1, 5)
-            assert abs(r["iterations"] - c["iterations"]) <= 2
+            assert abs(r["iterations"] - c["iterations"]) <= max(3, c["iterations"] // 8)
         checked += 1
     assert checked == len(golden["cases"])
 
