@@ -1,26 +1,38 @@
 """Benchmark: PageRank pull iterations on R-MAT scale 24 (one MI355X).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extensions]
 
-Workload: directed R-MAT, scale 24 (2^24 vertices),
-edge factor 16 (2^28 edges), a=0.57 b=0.19 c=0.19, vertex ids permuted,
-seed 1, duplicates and self-loops kept, generated on the GPU; CSR built on
-the GPU (radix sort by source, -o 0); fp32 pull PageRank (-p 0, damping
-0.85).  A step is one full pull iteration (every edge gathered once); the
-stop rule is disabled while timing so every step does the full work.
+Workload: directed R-MAT, scale 24 (2^24 vertices), edge factor 16 (2^28
+edges), a=0.57 b=0.19 c=0.19, vertex ids permuted, seed 1, duplicates and
+self-loops kept, generated on the GPU; CSR built on the GPU (radix sort by
+source, -o 0); fp32 pull PageRank (-p 0, damping 0.85).  A step is one full
+pull iteration (every edge gathered once); the stop rule is disabled while
+timing so every step does the full work.
 
 Printed JSON line: value = GTEPS (edges traversed per second, 1e9) over the
 K timed steps, with inputs resident in HBM; roofline = algorithmic bytes per
 iteration (8E + 20N) over the dominant kernel's average duration, measured
-with HIP events on the library's stream; cpu_baseline = the reference's own
-pageRankPullGraphCSR (compiled from its sources into oracle/_ref) on the same
-CSR on this host, over a bounded number of iterations.
+with HIP events on the library's stream; traffic = the HBM bytes per launch
+that rocprofv3 FETCH_SIZE + WRITE_SIZE counted for that kernel on this same
+command (profiles/pull_rmat24_counters.json, scaled by a plain copy);
+cpu_baseline = the reference's own pageRankPullGraphCSR (compiled from its
+sources into oracle/_ref) on the same CSR on this host, over a bounded number
+of iterations.
+
+"extensions" holds the other measured forms, each also on one GPU: push and
+fixed-point variants with their host runs, the Erdos-Renyi graph of the same
+size, batched personalized PageRank (R-MAT 22, k = 64, against 64
+single-source runs), 256 uneven graphs in one hipGraph launch (against
+per-graph launches) and the learned warm start (time to L1 < 1e-6 against the
+uniform start on R-MAT 24 graphs).  A failure there is reported in the line
+and does not affect the headline number.
 """
 import argparse
 import json
 import os
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -29,32 +41,206 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GTEPS + HBM GB/s per PR iter; time-to-L1<1e-6, learned vs plain, RMAT-24"
 HBM_PEAK_GBS = 8000.0
+COUNTERS = os.path.join(ROOT, "profiles", "pull_rmat24_counters.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, iterations, threads):
-    """Time the reference's compiled pull (or the oracle port) on this host."""
+def _ref_csr(g):
     from oracle import pyoracle as po
-    arrays = g.arrays()
-    csr = po.CSR(arrays, g.num_vertices, g.num_edges)
+    return po, po.CSR(g.arrays(), g.num_vertices, g.num_edges)
+
+
+def cpu_time(g, code, iterations, threads, csr=None):
+    """Seconds per iteration of the reference's compiled PageRank variant
+    (or of the oracle port when the reference could not be built)."""
+    po, c = _ref_csr(g) if csr is None else csr
     kind = "reference" if po.ref_available() else "port"
-    t0 = time.perf_counter()
     if kind == "reference":
-        r = po.ref_pagerank(csr, 0, 0.0, iterations, threads)
-        secs = r["seconds"]
+        r = po.ref_pagerank(c, code, 0.0, iterations, threads)
     else:
-        r = po.pagerank(csr, 0, 0.0, iterations, threads)
-        secs = r["seconds"]
-    wall = time.perf_counter() - t0
-    per_iter = secs / iterations
-    return dict(value=g.num_edges / per_iter / 1e9, unit="GTEPS", cores=threads, kind=kind,
-                sample=f"R-MAT-24 pull, {iterations} full iterations (epsilon 0) of "
-                       f"{'the reference pageRankPullGraphCSR compiled from its sources' if kind == 'reference' else 'the oracle C/OpenMP restatement'}"
-                       f", timed by its own timer; {secs:.2f} s ({wall:.2f} s wall incl. setup)",
-                seconds_per_iteration=per_iter), r["ranks"]
+        r = po.pagerank(c, code, 0.0, iterations, threads)
+    return r["seconds"] / iterations, kind
+
+
+def gpu_iter_ms(g, variant, steps=10, warmup=2, **kw):
+    import opengraph_amd as og
+    pr = og.PageRank(g, variant=variant, max_iter=100, epsilon=1e-6, **kw)
+    pr.reset()
+    pr.enqueue(warmup)
+    og.synchronize()
+    pr.kernel_ms()
+    t0 = time.perf_counter()
+    pr.enqueue(steps, time_kernels=True)
+    og.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    kms, kc = pr.kernel_ms()
+    bpi = pr.bytes_per_iteration()
+    pr.free()
+    return t * 1e3, kms / max(kc, 1), bpi
+
+
+def ext_variants(g, threads, csr, do_cpu):
+    """Pull/push/fixed-point on the headline graph, each against the host run."""
+    out = {}
+    for code, name in ((1, "push_fp32"), (2, "pull_fixed64"), (3, "push_fixed64"), (11, "pull_fixed8")):
+        ms, kms, bpi = gpu_iter_ms(g, code)
+        d = dict(ms_per_iteration=round(ms, 4), gteps=round(g.num_edges / ms / 1e6, 2),
+                 algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
+        if do_cpu and code in (1, 2):
+            s, kind = cpu_time(g, code, 2, threads, csr)
+            d.update(cpu_seconds_per_iteration=round(s, 4), cpu_kind=kind, speedup_vs_cpu=round(s / (ms / 1e3), 1))
+        out[name] = d
+        log(f"  {name}: {d}")
+    return out
+
+
+def ext_er(args):
+    import opengraph_amd as og
+    E = args.edge_factor << args.scale
+    ds, dd = og.er_device(1 << args.scale, E, 2)
+    g = og.Graph.build(ds, dd, E, 0)
+    ds.free(); dd.free()
+    ms, kms, bpi = gpu_iter_ms(g, 0)
+    g.free()
+    return dict(workload=f"Erdos-Renyi N=2^{args.scale} E={E} seed 2, pull fp32",
+                ms_per_iteration=round(ms, 4), gteps=round(E / ms / 1e6, 2),
+                algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
+
+
+def ext_ppr():
+    """Batched personalized PageRank, R-MAT 22 (seed 3), k = 64 one-hot
+    sources (seed 4), tol 1e-6 per column, against 64 single-source runs."""
+    import opengraph_amd as og
+    scale, E = 22, 16 << 22
+    ds, dd = og.rmat_device(scale, E, 0.57, 0.19, 0.19, 3, True)
+    g = og.Graph.build(ds, dd, E, 0)
+    ds.free(); dd.free()
+    src = np.random.default_rng(4).integers(0, g.num_vertices, size=64, dtype=np.uint32)
+    p = og.Ppr(g, src, 100, 1e-6)
+    p.run()                                   # first run pays one-time costs
+    r = p.run()
+    bpi = p.bytes_per_iteration()
+    X = p.ranks()
+    iters, _ = p.columns()
+    p.free()
+    t_single, it_single, maxdiff = 0.0, [], 0.0
+    for j in range(64):
+        pr = og.PageRank(g, 0, 100, 1e-6, ppr_source=int(src[j]))
+        t0 = time.perf_counter()
+        res = pr.run()
+        t_single += time.perf_counter() - t0
+        it_single.append(res.rows)
+        if j < 8:
+            x = pr.ranks()
+            maxdiff = max(maxdiff, float(np.max(np.abs(X[:, j] - x) / np.maximum(np.abs(x), 1e-30))))
+        pr.free()
+    g.free()
+    return dict(workload="R-MAT 22 (seed 3), k=64 one-hot sources (seed 4), tol 1e-6 per column",
+                iterations=int(r.iterations), converged=int(r.converged),
+                batched_seconds=round(r.seconds_total, 4), spmm_kernel_ms_per_iteration=round(r.kernel_ms_per_iter, 4),
+                algorithmic_gb_per_s=round(bpi / (r.kernel_ms_per_iter / 1e3) / 1e9, 1),
+                roofline_frac=round(bpi / (r.kernel_ms_per_iter / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                single_source_seconds_total=round(t_single, 4),
+                single_source_iterations_mean=round(float(np.mean(it_single)), 2),
+                speedup_vs_single_source=round(t_single / r.seconds_total, 2),
+                max_rel_diff_vs_single_first8=maxdiff)
+
+
+def ext_multi():
+    """256 uneven R-MAT graphs (scale U{12..20}, edge factor U{8..32},
+    a U[0.45, 0.65], b = c = (1 - a - 0.05)/2, master seed 5): plain PageRank
+    (tol 1e-6) in one hipGraph launch against per-graph launches."""
+    import opengraph_amd as og
+    rng = np.random.default_rng(5)
+    gs, sumN, sumE = [], 0, 0
+    t0 = time.perf_counter()
+    for i in range(256):
+        sc = int(rng.integers(12, 21)); ef = int(rng.integers(8, 33)); a = float(rng.uniform(0.45, 0.65))
+        b = (1 - a - 0.05) / 2
+        E = ef << sc
+        ds, dd = og.rmat_device(sc, E, a, b, b, 5000 + i, True)
+        gs.append(og.Graph.build(ds, dd, E, 0))
+        ds.free(); dd.free()
+        sumN += gs[-1].num_vertices; sumE += E
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m = og.Multi(gs, 100, 1e-6)               # autotuned row-class thresholds
+    t_setup = time.perf_counter() - t0
+    m.run(2)                                  # capture + first launch
+    t_graph = min(m.run(2) for _ in range(3))
+    t_each = min(m.run(0) for _ in range(2))
+    t_launch = m.run(1)
+    iters = [m.result(i, ranks=False)[0].iterations for i in range(0, 256, 16)]
+    thresholds = sorted(set(m.hub_threshold(i) for i in range(256)))
+    m.free()
+    for g in gs:
+        g.free()
+    return dict(workload="256 R-MAT graphs, scale U{12..20}, ef U{8..32}, a U[0.45,0.65], seed 5; tol 1e-6",
+                sum_vertices=sumN, sum_edges=sumE, build_seconds=round(t_build, 2),
+                setup_and_autotune_seconds=round(t_setup, 2), tuned_thresholds=thresholds,
+                one_hipgraph_seconds=round(t_graph, 4), per_graph_launch_seconds=round(t_each, 4),
+                batched_launch_seconds=round(t_launch, 4), speedup_hipgraph_vs_per_graph=round(t_each / t_graph, 2),
+                sample_iterations=iters)
+
+
+def ext_learned(n_train=200, n_eval=8, epochs=8):
+    """Learned warm start: MLP trained on converged ranks of 200 R-MAT-18
+    graphs (a U[0.45, 0.65], ef U{8..32}, seed 6) by the library's own
+    kernels; evaluated on R-MAT-24 graphs (seeds 100..) by wall time to
+    L1 < 1e-6 against the uniform start (warm-start time included)."""
+    import opengraph_amd as og
+
+    def graph(scale, seed, rng):
+        a = float(rng.uniform(0.45, 0.65)); ef = int(rng.integers(8, 33)); b = (1 - a - 0.05) / 2
+        E = ef << scale
+        ds, dd = og.rmat_device(scale, E, a, b, b, seed, True)
+        g = og.Graph.build(ds, dd, E, 0)
+        ds.free(); dd.free()
+        return g, a, ef
+
+    rng = np.random.default_rng(6)
+    m = og.Mlp(6)
+    t0 = time.perf_counter()
+    worst = 0.0
+    for i in range(n_train):
+        g, _, _ = graph(18, 6000 + i, rng)
+        worst = max(worst, m.add_graph(g, 1e-9, 2000))
+        g.free()
+    t_data = time.perf_counter() - t0
+    loss, t_train = m.train(epochs, 65536, 3e-3, 1)
+    n_vertices = m.data_size()
+    evals = []
+    for k in range(n_eval):
+        g, a, ef = graph(24, 100 + k, np.random.default_rng(100 + k))
+        row = dict(seed=100 + k, a=round(a, 3), edge_factor=ef)
+        for name in ("uniform", "learned"):
+            pr = og.PageRank(g, 0, 1000, 1e-6, stop_rule=1, init_mode=1)
+            og.synchronize()
+            t = time.perf_counter()
+            ws = pr.set_init_learned(m) if name == "learned" else 0.0
+            r = pr.run()
+            t = time.perf_counter() - t
+            row[name] = dict(iterations=int(r.rows), seconds=round(t, 4), warm_start_seconds=round(ws, 4),
+                             last_l1=r.error_total)
+            pr.free()
+        row["time_saved_frac"] = round(1 - row["learned"]["seconds"] / row["uniform"]["seconds"], 4)
+        evals.append(row)
+        log(f"  learned eval {row}")
+        g.free()
+    m.free()
+    tu = sum(e["uniform"]["seconds"] for e in evals)
+    tl = sum(e["learned"]["seconds"] for e in evals)
+    return dict(workload=f"train {n_train} R-MAT-18 (seed 6), eval {n_eval} R-MAT-24 (seeds 100..), L1<1e-6",
+                training_vertices=int(n_vertices), data_seconds=round(t_data, 2),
+                train_epochs=epochs, train_seconds=round(t_train, 2), final_train_loss=round(loss, 5),
+                worst_training_target_l1=worst, eval=evals,
+                total_seconds_uniform=round(tu, 4), total_seconds_learned=round(tl, 4),
+                time_saved_frac=round(1 - tl / tu, 4),
+                iterations_uniform_mean=float(np.mean([e["uniform"]["iterations"] for e in evals])),
+                iterations_learned_mean=float(np.mean([e["learned"]["iterations"] for e in evals])))
 
 
 def main():
@@ -67,6 +253,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-iterations", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--copy-reference", action="store_true",
                     help="after timing, run one 1 GiB device copy (og_copy_kernel): the reference "
                          "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -124,23 +311,64 @@ def main():
         d_src.free()
         d_dst.free()
 
-    # convergence run (reference stop rule) for the record
+    # convergence runs for the record: the reference stop rule, and L1 < 1e-6
+    # from the uniform start (the plain side of the learned comparison)
     res = pr.run()
-    ranks = pr.ranks()
+    pr.free()
+    pr1 = og.PageRank(g, 0, 1000, 1e-6, stop_rule=1, init_mode=1)
+    t0 = time.perf_counter()
+    r1 = pr1.run()
+    t_l1 = time.perf_counter() - t0
+    pr1.free()
     extra = dict(converge_iterations=int(res.iterations), converge_seconds=res.seconds_total,
+                 uniform_to_l1_1e6_iterations=int(r1.rows), uniform_to_l1_1e6_seconds=round(t_l1, 4),
                  plan_seconds=t_plan, csr_build_seconds=t_csr, generate_seconds=t_gen,
                  ms_per_iteration_wall=step_s * 1e3, dominant_kernel_ms=kern_s * 1e3,
                  algorithmic_gb_per_s=achieved)
 
     cpu = None
+    csr = None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         try:
-            cpu, cpu_ranks = cpu_baseline(g, args.cpu_iterations, threads)
-            extra["speedup_vs_cpu_per_iteration"] = cpu["seconds_per_iteration"] / step_s
-            log(f"cpu baseline ({cpu['kind']}, {threads} threads): {cpu['seconds_per_iteration']:.3f} s/iter")
+            csr = _ref_csr(g)
+            s, kind = cpu_time(g, 0, args.cpu_iterations, threads, csr)
+            cpu = dict(value=round(E / s / 1e9, 4), unit="GTEPS", cores=threads, kind=kind,
+                       sample=f"R-MAT-{scale} pull, {args.cpu_iterations} full iterations (epsilon 0) of "
+                              + ("the reference pageRankPullGraphCSR compiled from its sources"
+                                 if kind == "reference" else "the oracle C/OpenMP restatement")
+                              + f", OpenMP on {threads} threads, timed by its own timer: {s:.3f} s/iteration")
+            extra["speedup_vs_cpu_per_iteration"] = round(s / step_s, 1)
+            log(f"cpu baseline ({kind}, {threads} threads): {s:.3f} s/iter")
         except Exception as e:   # the baseline is reported, never required
             log("cpu baseline failed:", e)
+
+    traffic = None
+    try:
+        with open(COUNTERS) as f:
+            cnt = json.load(f)
+        traffic = cnt.get("hbm_bytes_per_launch")
+    except OSError:
+        pass
+
+    extensions = {}
+    if not args.no_extensions:
+        for name, fn in (("variants", lambda: ext_variants(g, threads, csr, csr is not None)),
+                         ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),
+                         ("uneven_batch", ext_multi), ("learned_warm_start", ext_learned)):
+            t0 = time.perf_counter()
+            try:
+                if name == "batched_ppr":
+                    g.free()          # free the headline graph before the big extensions
+                    g = None
+                extensions[name] = fn()
+            except Exception as e:
+                extensions[name] = {"error": f"{type(e).__name__}: {e}"}
+                log(traceback.format_exc())
+            extensions[name]["seconds"] = round(time.perf_counter() - t0, 2)
+            log(f"{name}: {extensions[name]}")
+    if g is not None:
+        g.free()
 
     line = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
@@ -152,10 +380,12 @@ def main():
                    "a": 0.57, "b": 0.19, "c": 0.19, "seed": args.seed, "num_vertices": N, "num_edges": E,
                    "variant": "pull fp32 (-p 0)", "sort": 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_iteration": bytes_iter, "kernel": "og_pull_kernel<0>"},
-        "cpu_baseline": cpu and {k: v for k, v in cpu.items() if k != "seconds_per_iteration"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_iteration": bytes_iter, "kernel": "og_pull_hot_kernel<0>",
+                     "traffic_source": "profiles/pull_rmat24_counters.json" if traffic else None},
+        "cpu_baseline": cpu,
         "extra": extra,
+        "extensions": extensions,
         "device": dev,
     }
     print(json.dumps(line), flush=True)
