@@ -317,7 +317,7 @@ int ogpu_pr_set_init_learned(ogpu_pr *p, ogpu_mlp *m, double *seconds)
     /* features -> o (bf16 MFMA inference, into c1) -> start ranks (old ids) */
     if (og_plan_features(dev, pl, n_final, c1, feat, s) ||
         og_launch_mlp_infer(m->params, feat, pl->N, 1, c1, s) ||
-        og_launch_warm_scale(c1, pl->outdeg, pl->new2old, pl->N, scr, init_d, s) ||
+        og_launch_warm_scale(c1, pl->row_ptr, pl->outdeg, pl->new2old, pl->N, scr, init_d, s) ||
         og_hip_check(hipStreamSynchronize(s), "sync warm start"))
         return -1;
     og_pr_use_init(p);

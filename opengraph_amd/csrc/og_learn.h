@@ -48,11 +48,13 @@ int og_launch_mlp_grad(const float *params, const float *feat, const float *y, u
 size_t og_mlp_part_floats(uint32_t count);
 int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
                        float b2, float eps, uint32_t step, hipStream_t s);
-/* Sums of o over all vertices and over those without out-edges (fp64,
- * fixed order), then r0[old] = N * S * o[v] / sum(o) with the fixed-point
- * mass S = 0.15 / (0.15 + 0.85 * dangling share). */
-int og_launch_warm_scale(const float *o, const uint32_t *outdeg, const uint32_t *new2old, uint32_t n,
-                         double *scratch, float *r0_old, hipStream_t s);
+/* Start ranks in old ids: 1 - d (unnormalized) for vertices without
+ * in-edges, N * S_U * o[v] / sum_U(o) for the others, S_U from the
+ * fixed-point mass identity (fp64 sums in a fixed order; scratch holds
+ * 4 * 1024 + 1 doubles). */
+int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
+                         const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
+                         hipStream_t s);
 
 #ifdef __cplusplus
 }

@@ -4,12 +4,12 @@
  * training gradient (forward + backward by hand, deterministic block
  * partials), Adam, and the scaling of the prediction into a start vector.
  *
- * The start vector: o = MLP(features) > 0 is the predicted shape; the
- * reference's iteration (no dangling redistribution, pageRank.c:989-1028)
- * has the fixed-point mass identity S = 0.15 + 0.85 (S - D), D the mass on
- * vertices without out-edges, so a start with shape o / sum(o) gets the
- * mass S = 0.15 / (0.15 + 0.85 delta), delta = sum over those vertices of
- * o / sum(o).  Unnormalized start ranks: r0 = N * S * o / sum(o).
+ * The start vector: vertices without in-edges get their fixed-point rank
+ * (1 - d) exactly (unnormalized); the others (U) get the predicted shape
+ * o / sum_U(o) scaled to the mass the reference's iteration (no dangling
+ * redistribution, pageRank.c:989-1028) keeps at its fixed point, from the
+ * identity S = 0.15 + 0.85 (S - D), D the mass on vertices without
+ * out-edges (og_warm_factor_kernel).
  */
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -330,72 +330,82 @@ __global__ void og_mlp_adam_kernel(float *__restrict__ P, const float *__restric
 
 #define WS_BLOCKS 1024
 
+/* Block partials of: sum of o over vertices with in-edges (U), the same
+ * over those of them without out-edges, and the counts of vertices without
+ * in-edges (K) and of those without out-edges either. */
 __global__ __launch_bounds__(LB) void og_warm_sums_kernel(const float *__restrict__ o,
+                                                         const uint32_t *__restrict__ row_ptr,
                                                          const uint32_t *__restrict__ outdeg, uint32_t n,
                                                          double *__restrict__ part)
 {
-    __shared__ double s1[LB], s2[LB];
-    double a = 0.0, b = 0.0;
+    __shared__ double sh[4][LB];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
     {
-        a += o[v];
-        if (!outdeg[v]) b += o[v];
+        const bool in = row_ptr[v + 1] != row_ptr[v], dang = outdeg[v] == 0;
+        if (in)
+        {
+            a[0] += o[v];
+            if (dang) a[1] += o[v];
+        }
+        else
+        {
+            a[2] += 1.0;
+            if (dang) a[3] += 1.0;
+        }
     }
-    s1[threadIdx.x] = a;
-    s2[threadIdx.x] = b;
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = a[q];
     __syncthreads();
     for (uint32_t w = LB / 2; w > 0; w >>= 1)
     {
         if (threadIdx.x < w)
-        {
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-            s2[threadIdx.x] += s2[threadIdx.x + w];
-        }
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-    {
-        part[2 * blockIdx.x] = s1[0];
-        part[2 * blockIdx.x + 1] = s2[0];
-    }
+    if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+/* Mass of the rows with in-edges at the fixed point.  Vertices without
+ * in-edges have the fixed-point rank (1-d)/N exactly (normalized), so with
+ * their share f_K and the share f_Kd of those also without out-edges, the
+ * mass identity S = 0.15 + 0.85 (S - D) gives for the rest
+ *   S_U = 0.15 (1 - 0.15 f_K - 0.85 f_Kd) / (0.15 + 0.85 delta_U),
+ * delta_U = share of the predicted shape of U on vertices without out-edges. */
 __global__ __launch_bounds__(LB) void og_warm_factor_kernel(double *__restrict__ part, uint32_t n)
 {
-    __shared__ double s1[LB], s2[LB];
-    double a = 0.0, b = 0.0;
+    __shared__ double sh[4][LB];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t i = threadIdx.x; i < WS_BLOCKS; i += LB)
-    {
-        a += part[2 * i];
-        b += part[2 * i + 1];
-    }
-    s1[threadIdx.x] = a;
-    s2[threadIdx.x] = b;
+        for (int q = 0; q < 4; q++) a[q] += part[4 * i + q];
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = a[q];
     __syncthreads();
     for (uint32_t w = LB / 2; w > 0; w >>= 1)
     {
         if (threadIdx.x < w)
-        {
-            s1[threadIdx.x] += s1[threadIdx.x + w];
-            s2[threadIdx.x] += s2[threadIdx.x + w];
-        }
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
     {
-        const double tot = s1[0] > 0.0 ? s1[0] : 1.0;
-        const double delta = s2[0] / tot;
-        const double S = 0.15 / (0.15 + 0.85 * delta);
-        part[2 * WS_BLOCKS] = (double)n * S / tot;     /* r0 = factor * o */
+        const double sumU = sh[0][0] > 0.0 ? sh[0][0] : 1.0;
+        const double deltaU = sh[1][0] / sumU;
+        const double fK = sh[2][0] / (double)n, fKd = sh[3][0] / (double)n;
+        const double SU = 0.15 * (1.0 - 0.15 * fK - 0.85 * fKd) / (0.15 + 0.85 * deltaU);
+        part[4 * WS_BLOCKS] = (double)n * SU / sumU;     /* r0 = factor * o on U */
     }
 }
 
-__global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t *__restrict__ new2old,
-                                     uint32_t n, const double *__restrict__ part, float *__restrict__ r0_old)
+__global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t *__restrict__ row_ptr,
+                                     const uint32_t *__restrict__ new2old, uint32_t n,
+                                     const double *__restrict__ part, float *__restrict__ r0_old)
 {
-    const float fac = (float)part[2 * WS_BLOCKS];
+    const float fac = (float)part[4 * WS_BLOCKS];
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-        r0_old[new2old ? new2old[v] : v] = fac * o[v];
+    {
+        /* unnormalized: N * (1 - d) / N for vertices without in-edges */
+        const float r = row_ptr[v + 1] != row_ptr[v] ? fac * o[v] : OG_BASE_PR;
+        r0_old[new2old ? new2old[v] : v] = r;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -475,11 +485,13 @@ extern "C" int og_launch_mlp_adam(float *params, const float *grad, float *m1, f
     return og_hip_check(hipGetLastError(), "og_mlp_adam_kernel");
 }
 
-extern "C" int og_launch_warm_scale(const float *o, const uint32_t *outdeg, const uint32_t *new2old, uint32_t n,
-                                    double *scratch, float *r0_old, hipStream_t s)
+extern "C" int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
+                                    const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
+                                    hipStream_t s)
 {
-    hipLaunchKernelGGL(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, outdeg, n, scratch);
+    hipLaunchKernelGGL(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, row_ptr, outdeg, n, scratch);
     hipLaunchKernelGGL(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, scratch, n);
-    hipLaunchKernelGGL(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, new2old, n, scratch, r0_old);
+    hipLaunchKernelGGL(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, row_ptr, new2old, n, scratch,
+                       r0_old);
     return og_hip_check(hipGetLastError(), "og_warm_scale");
 }

@@ -166,7 +166,7 @@ int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, u
     const size_t N = p->g->N;
     if (!p->init_d && !(p->init_d = og_dalloc(N * 4 + 4))) return -1;
     if (!p->feat && !(p->feat = og_dalloc(N * 16 + 16))) return -1;
-    if (!p->lscratch && !(p->lscratch = og_dalloc(4096 * 8))) return -1;
+    if (!p->lscratch && !(p->lscratch = og_dalloc((4 * 1024 + 64) * 8))) return -1;
     *dev = &p->dev;
     *plan = p->plan;
     *n_final = p->n_final_pull;

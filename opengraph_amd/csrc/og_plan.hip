@@ -122,11 +122,11 @@ __device__ __forceinline__ uint32_t og_row_class(uint32_t d, uint32_t seq_t)
     return d ? 2u : 3u;
 }
 
-/* Sort key of old vertex v: its class; with relabel, then (classes 0, 1, 3)
+/* Sort key of old vertex v: its class; with relabel, then (classes 0, 3)
  * forward out-degree descending, so the contributions gathered most often
- * share cache lines, or (class 2) in-degree descending, so the rows one
- * wavefront sums have about the same length.  Ties keep id order (the sort
- * is stable). */
+ * share cache lines, or (classes 1, 2) in-degree descending, so the rows or
+ * chunks one wavefront takes have about the same length.  Ties keep id
+ * order (the sort is stable). */
 __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ outdeg,
                                     const uint32_t *__restrict__ indeg, uint32_t seq_t,
                                     uint32_t relabel, uint32_t *__restrict__ key,
@@ -139,7 +139,7 @@ __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ out
         uint32_t k;
         if (relabel)
         {
-            uint32_t w = min(cls == 2 ? d : outdeg[v], 0x3FFFFFFFu);
+            uint32_t w = min(cls == 1 || cls == 2 ? d : outdeg[v], 0x3FFFFFFFu);
             k = (cls << 30) | (0x3FFFFFFFu - w);
         }
         else

@@ -288,33 +288,19 @@ __device__ __forceinline__ E og_gather(const E *__restrict__ g, const E *hot, ui
     }
 }
 
-template <int MODE, bool HOT = false>
-__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock,
-                                               const typename og_mode<MODE>::elem *hot = nullptr,
-                                               uint32_t nh = 0)
+/* The loads, sums and outputs of a wavefront's chunks with S slots per lane
+ * (S * 64 >= the longest of them). */
+template <int MODE, bool HOT, uint32_t S>
+__device__ __forceinline__ void og_chunk_body(const og_pr_dev &a, og_stat &st, uint32_t c0, uint32_t nk,
+                                              const uint32_t (&eb)[OG_CH_PER_WAVE],
+                                              const uint32_t (&ee)[OG_CH_PER_WAVE], uint32_t my_tag,
+                                              const typename og_mode<MODE>::elem *hot, uint32_t nh)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
-    constexpr uint32_t K = OG_CH_PER_WAVE, S = OG_CH_SLOTS;
+    constexpr uint32_t K = OG_CH_PER_WAVE;
     const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t c0 = (cblock * OG_WAVES + wave) * K;
-    if (c0 >= a.n_chunks) return;
-    const uint32_t nk = min(K, a.n_chunks - c0);
-    /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
-     * the end of the last chunk) */
-    const uint32_t my_tag = a.ch_row[c0 + min(lane, nk - 1)];
-    const uint32_t my_edge = a.ch_edge[c0 + min(lane, nk)];
-    uint32_t eb[K], ee[K], row[K];
-#pragma unroll
-    for (uint32_t k = 0; k < K; k++)
-    {
-        const uint32_t kk = min(k, nk - 1);   /* chunks past nk repeat the last one */
-        row[k] = __builtin_amdgcn_readlane(my_tag, kk) & ~OG_CH_MULTI;
-        eb[k] = __builtin_amdgcn_readlane(my_edge, kk);
-        ee[k] = __builtin_amdgcn_readlane(my_edge, kk + 1);
-    }
     uint32_t idx[K][S];
 #pragma unroll
     for (uint32_t k = 0; k < K; k++)
@@ -355,6 +341,46 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
         else if (a.dd) og_epilogue<MODE>(a, my_row, mine, st);
         else og_epilogue_pf<MODE>(a, my_row, mine, prev, deg, st);
     }
+}
+
+/* Chunks [c0, c0 + K) of one wavefront, K = OG_CH_PER_WAVE, all in flight
+ * together: lane l loads edges l, l+64, ... of each chunk (the index loads
+ * coalesce), gathers them and adds them in that order; the wavefront then
+ * adds its 64 lane sums in a fixed tree.  Lanes past a chunk's end re-read
+ * its last edge and add zero, so no branch separates the loads and the
+ * gathers of a lane overlap; the number of slots per lane follows the
+ * longest of the K chunks (the one-chunk rows are sorted by length, so
+ * neighbouring chunks are about as long).  Lane k then stores chunk k's
+ * sum (row with several chunks) or runs the row's epilogue. */
+template <int MODE, bool HOT = false>
+__device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock,
+                                               const typename og_mode<MODE>::elem *hot = nullptr,
+                                               uint32_t nh = 0)
+{
+    constexpr uint32_t K = OG_CH_PER_WAVE;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t c0 = (cblock * OG_WAVES + wave) * K;
+    if (c0 >= a.n_chunks) return;
+    const uint32_t nk = min(K, a.n_chunks - c0);
+    /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
+     * the end of the last chunk) */
+    const uint32_t my_tag = a.ch_row[c0 + min(lane, nk - 1)];
+    const uint32_t my_edge = a.ch_edge[c0 + min(lane, nk)];
+    uint32_t eb[K], ee[K];
+    uint32_t longest = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++)
+    {
+        const uint32_t kk = min(k, nk - 1);   /* chunks past nk repeat the last one */
+        eb[k] = __builtin_amdgcn_readlane(my_edge, kk);
+        ee[k] = __builtin_amdgcn_readlane(my_edge, kk + 1);
+        longest = max(longest, ee[k] - eb[k]);
+    }
+    if (longest <= 64) og_chunk_body<MODE, HOT, 1>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else if (longest <= 128) og_chunk_body<MODE, HOT, 2>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else if (longest <= 256) og_chunk_body<MODE, HOT, 4>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else og_chunk_body<MODE, HOT, OG_CH_SLOTS>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
 }
 
 /* Rows [C, N), one per thread.  The row's in-edges are loaded and gathered

@@ -149,9 +149,13 @@ def test_training_and_warm_start():
         f = pr.features()
         pr.free()
         q = net.forward(f, mfma=False).astype(np.float64)
-        xh = q / q.sum()
         dang = o["f_out_degree"] == 0
-        starts[name] = np.abs(xh * 0.15 / (0.15 + 0.85 * xh[dang].sum()) - x64).sum()
+        K = o["i_out_degree"] == 0                    # no in-edges: fixed point (1-d)/N
+        qU = np.where(K, 0.0, q)
+        shape = qU / qU.sum()
+        SU = 0.15 * (1 - 0.15 * K.mean() - 0.85 * (K & dang).mean()) / (0.15 + 0.85 * shape[dang].sum())
+        x0 = np.where(K, 0.15 / N, shape * SU)
+        starts[name] = np.abs(x0 - x64).sum()
     starts["uniform"] = np.abs(np.full(N, 1.0 / N) - x64).sum()
     print("start L1 to the fixed point", starts, "iterations", res, flush=True)
     assert starts["trained"] < starts["untrained"] and starts["trained"] < starts["uniform"]
