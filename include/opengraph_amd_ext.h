@@ -172,11 +172,13 @@ void ogpu_multi_free(ogpu_multi *m);
  * out-degree, mean log1p in-degree of the in-neighbours, 1/N], layers
  * 4 -> 64 -> 64 -> 1 (ReLU, softplus output), fp32 parameters packed as
  * W1[64][4] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 (4545 floats).
- * It predicts the shape of the converged rank vector; the start vector is
- * that shape renormalized to sum 1 and scaled to the mass the reference's
- * iteration keeps at its fixed point (0.15 / (0.15 + 0.85 * share of the
- * shape on vertices without out-edges)).  The unchanged iteration, to the
- * same tolerance, then runs from it; uniform start is the fallback. */
+ * It predicts the shape of the converged rank vector.  The start vector:
+ * vertices without in-edges get their fixed-point rank 1 - d (unnormalized)
+ * exactly; the others the predicted shape renormalized to sum 1 and scaled
+ * to the mass the reference's iteration keeps at its fixed point (from
+ * S = 0.15 + 0.85 (S - mass on vertices without out-edges)).  The unchanged
+ * iteration, to the same tolerance, then runs from it; the uniform start is
+ * the fallback. */
 typedef struct ogpu_mlp ogpu_mlp;
 int ogpu_mlp_create(uint64_t seed, ogpu_mlp **out);
 /* get (4545 floats) and/or set the parameters (set resets Adam's state). */
