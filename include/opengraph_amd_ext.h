@@ -88,7 +88,7 @@ typedef struct
                                class only, original order within a class */
     uint32_t hub_threshold; /* rows with more in-edges are split into 512-edge chunks summed
                                in a fixed tree; rows with at most this many are summed in
-                               the reference's order (bit-exact); 0: default 32;
+                               the reference's order (bit-exact); 0: default 16;
                                0xFFFFFFFF: every row in the reference's order */
     uint32_t ppr;           /* 1: personalized PageRank (variant 0 only): start at the one-hot
                                e_s, x' = (1-d)[v==s] + d * sum, stop when the L1 change

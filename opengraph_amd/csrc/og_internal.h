@@ -93,7 +93,7 @@ void og_graph_destroy(og_graph *g);
  * order; chunked rows add in a fixed tree (deterministic). */
 #define OG_CH 512u            /* edges per wavefront chunk: 64 lanes x 8 */
 #define OG_CH_PER_WAVE 4u     /* consecutive chunks one wavefront processes */
-#define OG_SEQ_T 32u          /* default seq_t */
+#define OG_SEQ_T 32u          /* seq_t of the batched-PPR plan */
 #define OG_CH_MULTI 0x80000000u /* ch_row flag: the row has several chunks */
 
 typedef struct og_plan

@@ -20,7 +20,10 @@
 #include "opengraph_amd_ext.h"
 
 #define OG_CHECK_EVERY 8u
-#define OG_DEFAULT_SEQ_T OG_SEQ_T
+/* Rows with more in-edges than this go to the chunk waves: 12..16 is the
+ * fastest on R-MAT 24 (1.38-1.40 ms against 1.47 at 32 and 1.50 at 4) and
+ * on Erdos-Renyi 24 (4.80-4.87 ms against 5.08 at 32). */
+#define OG_DEFAULT_SEQ_T 16u
 
 /* Blocks of a final reduction over n_parts first-level partials that also
  * finalizes `rows` long rows (a wavefront each): enough that each block
@@ -253,7 +256,7 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
             free(deg);
             on = hot_edges * 5 >= (uint64_t)E;
         }
-        if (on) hot_grid = (uint32_t)og_num_cus() * (163840u / OG_HOT_BYTES);
+        if (on) hot_grid = (uint32_t)og_num_cus();
     }
     const uint32_t pull_blocks = hot_grid ? hot_grid : pl->n_chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);

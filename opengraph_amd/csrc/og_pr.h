@@ -85,9 +85,11 @@ typedef struct og_pr_dev
 int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
 
-/* LDS bytes of contributions a block of the persistent pull keeps (two
- * blocks per CU fit the 160 KB of LDS). */
-#define OG_HOT_BYTES 65536u
+/* The persistent pull runs one block of OG_HOT_SUBS * 256 threads per CU
+ * (16 wavefronts, the occupancy its registers allow) and keeps
+ * OG_HOT_BYTES of contributions in LDS -- most of a CU's 160 KB. */
+#define OG_HOT_SUBS 4u
+#define OG_HOT_BYTES 160768u
 
 /* First-level statistics partials one pull launch writes. */
 static inline uint32_t og_pull_parts(const og_pr_dev *a)

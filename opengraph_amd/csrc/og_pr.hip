@@ -202,12 +202,12 @@ template <typename T> __device__ __forceinline__ T og_wave_sum(T v)
     return v;
 }
 
-/* Reduce og_stat over the block; result valid in thread 0. */
-__device__ og_stat og_block_stat(og_stat s)
+/* Reduce og_stat over a block of NW wavefronts; result valid in thread 0. */
+template <int NW = OG_WAVES> __device__ og_stat og_block_stat(og_stat s)
 {
-    __shared__ double sh_err[OG_WAVES];
-    __shared__ uint32_t sh_act[OG_WAVES];
-    __shared__ float sh_min[OG_WAVES], sh_max[OG_WAVES];
+    __shared__ double sh_err[NW];
+    __shared__ uint32_t sh_act[NW];
+    __shared__ float sh_min[NW], sh_max[NW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     s.err = og_wave_sum(s.err);
     s.active = og_wave_sum(s.active);
@@ -227,7 +227,7 @@ __device__ og_stat og_block_stat(og_stat s)
     og_stat r;
     og_stat_init(r);
     if (threadIdx.x == 0)
-        for (int i = 0; i < OG_WAVES; i++)
+        for (int i = 0; i < NW; i++)
         {
             r.err += sh_err[i];
             r.active += sh_act[i];
@@ -359,7 +359,7 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
 {
     constexpr uint32_t K = OG_CH_PER_WAVE;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
     const uint32_t c0 = (cblock * OG_WAVES + wave) * K;
     if (c0 >= a.n_chunks) return;
     const uint32_t nk = min(K, a.n_chunks - c0);
@@ -397,7 +397,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
     const E *__restrict__ g = (const E *)a.g_in;
-    const uint32_t v = a.C + rblock * OG_BLOCK + threadIdx.x;
+    const uint32_t v = a.C + rblock * OG_BLOCK + (threadIdx.x & (OG_BLOCK - 1));
     if (v >= a.N) return;
     if (a.dd && !a.wl_cur[v]) return;
     uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]);
@@ -439,13 +439,15 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     }
 }
 
-/* The same pull as a persistent kernel whose blocks first copy the
- * OG_HOT_BYTES of contributions with the smallest ids -- after the degree
- * relabelling, those of the vertices with the most in- and out-edges -- into
- * LDS and serve those gathers from there; block b then takes the work
- * blocks b, b + grid, ... (statistics partial per block). */
+/* The same pull as a persistent kernel, one block per CU, whose blocks
+ * first copy the OG_HOT_BYTES of contributions with the smallest ids --
+ * after the degree relabelling, those of the vertices with the most in- and
+ * out-edges -- into LDS and serve those gathers from there.  A block is
+ * OG_HOT_SUBS groups of 256 threads; group q of block b takes the work
+ * blocks S*b + q, S*b + q + S*grid, ... (S = OG_HOT_SUBS) with no barrier
+ * between groups; statistics partial per block. */
 template <int MODE>
-__global__ __launch_bounds__(OG_BLOCK) void og_pull_hot_kernel(og_pr_dev a, uint32_t it)
+__global__ __launch_bounds__(OG_HOT_SUBS *OG_BLOCK) void og_pull_hot_kernel(og_pr_dev a, uint32_t it)
 {
     typedef typename og_mode<MODE>::elem E;
     constexpr uint32_t NH = OG_HOT_BYTES / sizeof(E);
@@ -453,17 +455,18 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_hot_kernel(og_pr_dev a, uint
     if (*a.done) return;
     const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t nh = min(NH, a.N);
-    for (uint32_t i = threadIdx.x; i < nh; i += OG_BLOCK) hot[i] = g[i];
+    for (uint32_t i = threadIdx.x; i < nh; i += OG_HOT_SUBS * OG_BLOCK) hot[i] = g[i];
     __syncthreads();
     og_stat st;
     og_stat_init(st);
     const uint32_t items = a.n_chunk_blocks + a.n_row_blocks;
-    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x)
+    const uint32_t q = threadIdx.x / OG_BLOCK;
+    for (uint32_t w = blockIdx.x * OG_HOT_SUBS + q; w < items; w += gridDim.x * OG_HOT_SUBS)
     {
         if (w < a.n_chunk_blocks) og_pull_chunks<MODE, true>(a, st, w, hot, nh);
         else og_pull_rows<MODE, true>(a, st, w - a.n_chunk_blocks, hot, nh);
     }
-    og_stat r = og_block_stat(st);
+    og_stat r = og_block_stat<OG_HOT_SUBS * OG_WAVES>(st);
     if (threadIdx.x == 0)
     {
         a.part_err[blockIdx.x] = r.err;
@@ -763,7 +766,8 @@ template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it
     uint32_t grid = a->n_chunk_blocks + a->n_row_blocks;
     if (ev0) hipEventRecord(ev0, s);
     if (a->hot_grid)
-        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_BLOCK), 0, s, *a, it);
+        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
+                           s, *a, it);
     else if (grid)
         hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     if (ev1) hipEventRecord(ev1, s);
