@@ -320,6 +320,8 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
     a->E = E;
     a->H = pl->H;
     a->C = pl->C;
+    a->Z = pl->Z;
+    a->row_end = N;
     a->n_chunks = pl->n_chunks;
     a->n_chunk_blocks = pl->n_chunk_blocks;
     a->n_row_blocks = pl->n_row_blocks;
@@ -441,6 +443,16 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     a.c_out = p->quant ? p->c[0] : p->c[(it + 1) & 1];
     uint32_t row = it % p->stats_cap;
     int rc;
+    if (!p->push && !p->quant && !p->dd && it >= 2)
+    {
+        /* Rows [Z, N) have no in-edges: from iteration 1 on their rank is
+         * (1-d) + d*0 every time (|delta| = 0) and both contribution buffers
+         * already hold (1-d)/outdeg, so the launch leaves them out and only
+         * their activeVertices count (0 >= epsilon) is added. */
+        a.row_end = a.Z;
+        a.n_row_blocks = (uint32_t)(((uint64_t)(a.Z - a.C) + 255) / 256);
+        a.z_active = 0.0 >= a.epsilon ? a.N - a.Z : 0;
+    }
     if (!p->push)
     {
         a.n_parts = og_pull_parts(&a);

@@ -40,8 +40,10 @@ typedef struct
 /* Everything a PageRank kernel needs, passed by value at each launch. */
 typedef struct og_pr_dev
 {
-    uint32_t N, E, H, C;     /* row classes of the plan (og_internal.h) */
+    uint32_t N, E, H, C, Z;  /* row classes of the plan (og_internal.h) */
     uint32_t n_chunks, n_chunk_blocks, n_row_blocks;
+    uint32_t row_end;        /* pull rows [C, row_end) one per thread: N, or Z when */
+    uint32_t z_active;       /* rows [Z, N) are left out; this is added to the active count */
     uint32_t n_parts;        /* valid first-level partials for the next final reduction */
     uint32_t n_final_blocks; /* blocks of the final reduction */
     uint32_t stop_l1;        /* 0: stop when active == 0; 1: when error_total < epsilon */

@@ -398,7 +398,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
     typedef typename og_mode<MODE>::acc A;
     const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t v = a.C + rblock * OG_BLOCK + (threadIdx.x & (OG_BLOCK - 1));
-    if (v >= a.N) return;
+    if (v >= a.row_end) return;
     if (a.dd && !a.wl_cur[v]) return;
     uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]);
     const uint32_t e = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
@@ -520,6 +520,7 @@ __device__ void og_reduce_tail(const og_pr_dev &a, uint32_t it, og_stat mine)
     og_stat tot = og_block_stat(acc);
     if (t == 0)
     {
+        tot.active += a.z_active;
         if (it != OG_NO_STATS)
         {
             a.stats[it].error_total = tot.err;
