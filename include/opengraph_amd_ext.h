@@ -96,9 +96,17 @@ typedef struct
     uint32_t ppr_source;    /* s, the caller's vertex id */
     uint32_t hot_cache;     /* pull (variants 0, 2, 4, 9-13): 0 default (on when the graph
                                exceeds 4x the copy and its cached sources send >= 1/5 of the
-                               edges), 1 on, 2 off -- persistent blocks
-                               that serve the gathers of the lowest vertex ids (the hottest
-                               after relabelling) from a 64 KB LDS copy; same results */
+                               edges), 1 on, 2 off -- one persistent block per CU that
+                               serves the gathers of the lowest vertex ids (the hottest
+                               after relabelling) from a 157 KB LDS copy; same results.
+                               push (variants 1, 3, 5): the destinations with the lowest
+                               ids add their in-edges in LDS accumulators and flush once
+                               per block; 0 default (on for variant 3 when those
+                               destinations receive >= 1/5 of the edges), 1 on, 2 off.
+                               Variant 3 is exact either way; for 1 and 5 the block
+                               partials are more accurate than the reference's running
+                               fp32 sums and may differ from them by ~1e-5 relative on
+                               rows with thousands of in-edges */
 } ogpu_pr_options;
 
 typedef struct

@@ -258,6 +258,30 @@ static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *
         }
         if (on) hot_grid = (uint32_t)og_num_cus();
     }
+    else if (p->push && !p->dd && opt->hot_cache != 2 && E)
+    {
+        /* push scatter with LDS accumulators for the destinations with the
+         * most in-edges (the smallest ids).  By default for the u64
+         * fixed-point push (exact in any order) when they receive at least a
+         * fifth of the edges; fp32 and quantized pushes only on request
+         * (hot_cache 1): their block partials are more accurate than the
+         * reference's running fp32 sums, which drop the small terms of a
+         * long in-edge list, and can differ from them by a little over
+         * 1e-5 relative on such rows. */
+        int on = opt->hot_cache == 1;
+        const uint32_t nh = (uint32_t)(OG_HOT_BYTES / (p->mode == OG_MODE_FX64 ? 8u : 4u));
+        if (!on && p->mode == OG_MODE_FX64 && N > nh)
+        {
+            uint32_t hot_edges = 0;
+            if (og_hip_check(hipMemcpy(&hot_edges, pl->row_ptr + nh, 4, hipMemcpyDeviceToHost), "D2H"))
+            {
+                ogpu_pr_free(p);
+                return -1;
+            }
+            on = (uint64_t)hot_edges * 5 >= (uint64_t)E;
+        }
+        if (on) hot_grid = (uint32_t)og_num_cus();
+    }
     const uint32_t pull_blocks = hot_grid ? hot_grid : pl->n_chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv, 0);

@@ -617,29 +617,73 @@ __global__ void og_quantize_kernel(og_pr_dev a)
 /* push iteration (pageRank.c:1117-1134, :1785-1799, :2338-2352)      */
 /* ------------------------------------------------------------------ */
 
-template <int PMODE>
-__global__ __launch_bounds__(OG_BLOCK) void og_push_scatter_kernel(og_pr_dev a)
+/* Scatter of the contributions over the forward edges, E atomic adds into
+ * next[].  With HOT, one 1024-thread block per CU keeps the OG_HOT_BYTES of
+ * accumulators of the destinations with the smallest ids -- after the
+ * relabelling, those with the most in-edges -- in LDS, adds their edges
+ * there and flushes each nonzero accumulator with one global atomic at the
+ * end: the hottest destinations then see one global atomic per block
+ * instead of one per edge.  fp32 adds are unordered either way (as the
+ * reference's omp atomic); the u64 fixed-point sum is exact in any order. */
+template <int PMODE, bool HOT>
+__global__ __launch_bounds__(HOT ? OG_HOT_SUBS * OG_BLOCK : OG_BLOCK) void og_push_scatter_kernel(og_pr_dev a)
 {
+    typedef typename std::conditional<PMODE == OG_MODE_FX64, unsigned long long, float>::type T;
+    constexpr uint32_t NH = HOT ? OG_HOT_BYTES / sizeof(T) : 1;
+    __shared__ T acc[NH];
     if (*a.done) return;
-    const uint32_t E = a.E;
-    for (uint32_t e = blockIdx.x * OG_BLOCK + threadIdx.x; e < E; e += gridDim.x * OG_BLOCK)
+    const uint32_t E = a.E, nb = blockDim.x;
+    const uint32_t nh = HOT ? min(NH, a.N) : 0;
+    T *__restrict__ next = (T *)a.next;
+    if constexpr (HOT)
     {
-        uint32_t s = __builtin_nontemporal_load(&a.push_src[e]);
-        uint32_t d = __builtin_nontemporal_load(&a.push_dst[e]);
-        if constexpr (PMODE == OG_MODE_F32)
+        for (uint32_t i = threadIdx.x; i < nh; i += nb) acc[i] = 0;
+        __syncthreads();
+    }
+    /* four edges per thread in flight: loads first (lanes past the end
+     * re-read the last edge), then the adds of the valid ones */
+    constexpr uint32_t B = 4;
+    const uint32_t stride = gridDim.x * nb;
+    for (uint32_t e0 = blockIdx.x * nb + threadIdx.x; e0 < E; e0 += B * stride)
+    {
+        uint32_t s[B], d[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++)
         {
-            unsafeAtomicAdd(&((float *)a.next)[d], ((const float *)a.g_in)[s]);
+            const uint32_t e = min(e0 + u * stride, E - 1);
+            s[u] = __builtin_nontemporal_load(&a.push_src[e]);
+            d[u] = __builtin_nontemporal_load(&a.push_dst[e]);
         }
-        else if constexpr (PMODE == OG_MODE_FX64)
+        T add[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++)
         {
-            atomicAdd((unsigned long long *)&((uint64_t *)a.next)[d],
-                      (unsigned long long)((const uint64_t *)a.g_in)[s]);
+            if constexpr (PMODE == OG_MODE_F32) add[u] = ((const float *)a.g_in)[s[u]];
+            else if constexpr (PMODE == OG_MODE_FX64) add[u] = (T)((const uint64_t *)a.g_in)[s[u]];
+            else
+            {
+                uint32_t qv = ((const uint32_t *)a.g_in)[s[u]];
+                add[u] = (float)(a.qp->scale_d * (double)(uint32_t)(qv - a.qp->zero));
+            }
         }
-        else
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++)
         {
-            uint32_t qv = ((const uint32_t *)a.g_in)[s];
-            double add = a.qp->scale_d * (double)(uint32_t)(qv - a.qp->zero);
-            unsafeAtomicAdd(&((float *)a.next)[d], (float)add);
+            if (e0 + u * stride >= E) break;
+            if (HOT && d[u] < nh) atomicAdd(&acc[d[u]], add[u]);
+            else if constexpr (PMODE == OG_MODE_FX64) atomicAdd(&next[d[u]], add[u]);
+            else unsafeAtomicAdd(&next[d[u]], add[u]);
+        }
+    }
+    if constexpr (HOT)
+    {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nh; i += nb)
+        {
+            const T x = acc[i];
+            if (x == (T)0) continue;   /* contributions are >= 0: nothing to add */
+            if constexpr (PMODE == OG_MODE_FX64) atomicAdd(&next[i], x);
+            else unsafeAtomicAdd(&next[i], x);
         }
     }
 }
@@ -818,19 +862,32 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
     if (ev0) hipEventRecord(ev0, s);
     if (pmode == OG_MODE_F32)
     {
-        hipLaunchKernelGGL(og_push_scatter_kernel<OG_MODE_F32>, dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+        if (a->hot_grid)
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
+                               dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
+        else
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
         if (ev1) hipEventRecord(ev1, s);
         hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_F32>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
     }
     else if (pmode == OG_MODE_FX64)
     {
-        hipLaunchKernelGGL(og_push_scatter_kernel<OG_MODE_FX64>, dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+        if (a->hot_grid)
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
+                               dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
+        else
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
         if (ev1) hipEventRecord(ev1, s);
         hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_FX64>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
     }
     else
     {
-        hipLaunchKernelGGL(og_push_scatter_kernel<OG_MODE_QPUSH_EPI>, dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+        if (a->hot_grid)
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, true>), dim3(a->hot_grid),
+                               dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
+        else
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, false>), dim3(gs), dim3(OG_BLOCK), 0,
+                               s, *a);
         if (ev1) hipEventRecord(ev1, s);
         hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_QPUSH_EPI>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
     }
