@@ -84,13 +84,16 @@ def gpu_iter_ms(g, variant, steps=10, warmup=2, **kw):
 
 def ext_variants(g, threads, csr, do_cpu):
     """Pull/push/fixed-point on the headline graph, each against the host run."""
-    out = {}
-    for code, name in ((1, "push_fp32"), (2, "pull_fixed64"), (3, "push_fixed64"), (11, "pull_fixed8")):
-        ms, kms, bpi = gpu_iter_ms(g, code)
+    out, cpu = {}, {}
+    for code, name, kw in ((1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
+                           (2, "pull_fixed64", {}), (3, "push_fixed64", {}), (11, "pull_fixed8", {})):
+        ms, kms, bpi = gpu_iter_ms(g, code, **kw)
         d = dict(ms_per_iteration=round(ms, 4), gteps=round(g.num_edges / ms / 1e6, 2),
                  algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
         if do_cpu and code in (1, 2):
-            s, kind = cpu_time(g, code, 2, threads, csr)
+            if code not in cpu:
+                cpu[code] = cpu_time(g, code, 2, threads, csr)
+            s, kind = cpu[code]
             d.update(cpu_seconds_per_iteration=round(s, 4), cpu_kind=kind, speedup_vs_cpu=round(s / (ms / 1e3), 1))
         out[name] = d
         log(f"  {name}: {d}")
