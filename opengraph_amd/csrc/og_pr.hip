@@ -1088,7 +1088,11 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_init_kernel(og_multi_dev m)
     }
 }
 
-__global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m, uint32_t parity)
+/* settled != 0 from the third iteration on: row blocks entirely in [Z, N)
+ * (no in-edges; rank and both contribution buffers already final) only
+ * report their activeVertices count (0 >= epsilon). */
+__global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m, uint32_t parity,
+                                                                uint32_t settled)
 {
     if (*m.all_done) return;
     for (uint32_t i = blockIdx.x; i < m.n_work; i += gridDim.x)
@@ -1096,6 +1100,16 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m,
         const og_multi_work w = m.work[i];
         const og_pr_dev &a = m.devs[2 * w.g + parity];
         if (*a.done) continue;
+        const uint32_t v0 = a.C + w.first * OG_BLOCK;
+        if (w.kind == 1 && settled && v0 >= a.Z)
+        {
+            if (threadIdx.x == 0)
+            {
+                m.w_err[i] = 0.0;
+                m.w_act[i] = 0.0 >= a.epsilon ? min((uint32_t)OG_BLOCK, a.N - v0) : 0u;
+            }
+            continue;
+        }
         og_stat st;
         og_stat_init(st);
         if (w.kind == 0) og_pull_chunks<OG_MODE_F32>(a, st, w.first);
@@ -1175,7 +1189,8 @@ extern "C" int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStr
 extern "C" int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s)
 {
     const uint32_t parity = it & 1;
-    hipLaunchKernelGGL(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
+    hipLaunchKernelGGL(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity,
+                       (uint32_t)(it >= 2));
     if (m->n_lwork)
         hipLaunchKernelGGL(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
     hipLaunchKernelGGL(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);

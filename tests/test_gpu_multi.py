@@ -76,6 +76,21 @@ def test_multi_matches_single_graph_runs(batch, mode):
     m.free()
 
 
+def test_multi_epsilon_zero_counts_settled_rows(batch):
+    """epsilon 0: every vertex is active in every iteration (|delta| >= 0),
+    including the rows without in-edges that the batched pull leaves out
+    from the third iteration on; ranks as the single-graph run's."""
+    gs, _ = batch
+    m = og.Multi(gs, 6, 0.0, hub_thresholds=[16] * len(gs))
+    m.run(1)
+    for i, g in enumerate(gs):
+        res, x = m.result(i)
+        r1, x1, act1, _ = _single(g, 16, 6, 0.0)
+        assert res.iterations == r1.iterations and res.active == g.num_vertices == r1.active, i
+        assert np.array_equal(x.view(np.uint32), x1.view(np.uint32)), i
+    m.free()
+
+
 def test_multi_results_match_oracle(batch):
     gs, edges = batch
     m = og.Multi(gs, 40, 1e-6, hub_threshold=0xFFFFFFFF)      # every row in CSR order
