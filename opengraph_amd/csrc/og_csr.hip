@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "og_internal.h"
+#include "og_plan.h"
 
 #define OG_BLOCK 256
 #define RS_TILE 4096u
@@ -174,34 +175,6 @@ __global__ __launch_bounds__(OG_BLOCK) void og_rs_hist_kernel(const uint32_t *__
     counts[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-/* Exclusive scan of `total` u32 counts in place, one block of 1024. */
-__global__ __launch_bounds__(1024) void og_rs_scan_kernel(uint32_t *__restrict__ c, uint32_t total)
-{
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (total + 1023) / 1024;
-    uint32_t b = t * per, e = min(b + per, total);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++) s += c[i];
-    part[t] = s;
-    __syncthreads();
-    /* Hillis-Steele inclusive scan of the 1024 partial sums. */
-    for (uint32_t o = 1; o < 1024; o <<= 1)
-    {
-        uint32_t v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - s;
-    for (uint32_t i = b; i < e; i++)
-    {
-        uint32_t x = c[i];
-        c[i] = run;
-        run += x;
-    }
-}
-
 __device__ __forceinline__ uint64_t og_lanemask_lt()
 {
     uint32_t lane = threadIdx.x & 63;
@@ -330,10 +303,14 @@ __global__ __launch_bounds__(OG_BLOCK) void og_rs_scatter_kernel(
     }
 }
 
+/* Scratch: the (digit, block) counts plus their total, then the multi-block
+ * scan's block sums. */
+#define RS_COUNTS_WORDS (256u * RS_MAX_BLOCKS + 4u)
+
 extern "C" size_t og_radix_scratch_bytes(uint32_t n)
 {
     (void)n;
-    return (size_t)256 * RS_MAX_BLOCKS * sizeof(uint32_t);
+    return (size_t)RS_COUNTS_WORDS * sizeof(uint32_t) + og_scan_scratch_bytes(256u * RS_MAX_BLOCKS);
 }
 
 extern "C" int og_radix_sort_pairs(const uint32_t *k_in, const uint32_t *v_in, uint32_t *k_a,
@@ -361,7 +338,7 @@ extern "C" int og_radix_sort_pairs(const uint32_t *k_in, const uint32_t *v_in, u
         uint32_t *vo = (p & 1) ? v_b : v_a;
         hipLaunchKernelGGL(og_rs_hist_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, n, chunk, p * 8,
                            counts);
-        hipLaunchKernelGGL(og_rs_scan_kernel, dim3(1), dim3(1024), 0, s, counts, 256 * nb);
+        if (og_scan_exclusive(counts, counts, 256 * nb, counts + RS_COUNTS_WORDS, s)) return -1;
         hipLaunchKernelGGL(og_rs_scatter_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, vi, ko, vo, n,
                            chunk, p * 8, counts);
         if (og_hip_check(hipGetLastError(), "og_radix_sort_pairs")) return -1;

@@ -383,40 +383,66 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
     else og_chunk_body<MODE, HOT, OG_CH_SLOTS>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
 }
 
-/* Rows [C, N), one per thread.  The row's in-edges are loaded and gathered
- * OG_ROW_BATCH at a time and added in CSR order (pageRank.c:998-1011), so
- * the sum is the reference's to the bit.  Rows are sorted by in-degree
- * here, so the 64 rows of a wavefront take about the same number of
- * batches.  Lanes past their row's end re-read its last edge and add zero:
+/* In-edges [j, e) of one row added in CSR order, B loaded and gathered at a
+ * time.  Lanes past their row's end re-read its last edge and add zero:
  * x + 0 == x for every partial sum, which starts at +0. */
+template <int MODE, bool HOT, uint32_t B>
+__device__ __forceinline__ typename og_mode<MODE>::acc og_row_sum(const og_pr_dev &a, uint32_t j,
+                                                                  uint32_t e,
+                                                                  const typename og_mode<MODE>::elem *hot,
+                                                                  uint32_t nh)
+{
+    typedef typename og_mode<MODE>::elem E;
+    typedef typename og_mode<MODE>::acc A;
+    const E *__restrict__ g = (const E *)a.g_in;
+    A s = 0;
+    for (; j < e; j += B)
+    {
+        uint32_t idx[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) idx[u] = __builtin_nontemporal_load(&a.cols[min(j + u, e - 1)]);
+        E val[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) val[u] = og_gather<E, HOT>(g, hot, nh, idx[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) s += j + u < e ? (A)val[u] : (A)0;
+    }
+    return s;
+}
+
+/* Rows [C, row_end), one per thread, each summed in CSR order
+ * (pageRank.c:998-1011), so the sum is the reference's to the bit.  The
+ * loads per batch follow the wavefront's longest row (1, 2, 4 or
+ * OG_ROW_BATCH): most short rows of a power-law graph have one to four
+ * in-edges, and rows are sorted by in-degree, so the 64 rows of a
+ * wavefront are about as long. */
 template <int MODE, bool HOT = false>
 __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock,
                                              const typename og_mode<MODE>::elem *hot = nullptr,
                                              uint32_t nh = 0)
 {
-    typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
-    const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t v = a.C + rblock * OG_BLOCK + (threadIdx.x & (OG_BLOCK - 1));
-    if (v >= a.row_end) return;
-    if (a.dd && !a.wl_cur[v]) return;
-    uint32_t j = __builtin_nontemporal_load(&a.row_ptr[v]);
-    const uint32_t e = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
-    const float prev = __builtin_nontemporal_load(&a.r[v]);
-    const uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
-    A s = 0;
-    for (; j < e; j += OG_ROW_BATCH)
+    const bool mine = v < a.row_end && !(a.dd && !a.wl_cur[v]);
+    uint32_t j = 0, e = 0;
+    float prev = 0.0f;
+    uint32_t deg = 0;
+    if (mine)
     {
-        uint32_t idx[OG_ROW_BATCH];
-#pragma unroll
-        for (uint32_t u = 0; u < OG_ROW_BATCH; u++)
-            idx[u] = __builtin_nontemporal_load(&a.cols[min(j + u, e - 1)]);
-        E val[OG_ROW_BATCH];
-#pragma unroll
-        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) val[u] = og_gather<E, HOT>(g, hot, nh, idx[u]);
-#pragma unroll
-        for (uint32_t u = 0; u < OG_ROW_BATCH; u++) s += j + u < e ? (A)val[u] : (A)0;
+        j = __builtin_nontemporal_load(&a.row_ptr[v]);
+        e = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
+        prev = __builtin_nontemporal_load(&a.r[v]);
+        deg = __builtin_nontemporal_load(&a.outdeg[v]);
     }
+    /* longest row of the wavefront (every lane still active here) */
+    uint32_t m = e - j;
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+    if (!mine) return;
+    A s;
+    if (m <= 1) s = og_row_sum<MODE, HOT, 1>(a, j, e, hot, nh);
+    else if (m <= 2) s = og_row_sum<MODE, HOT, 2>(a, j, e, hot, nh);
+    else if (m <= 4) s = og_row_sum<MODE, HOT, 4>(a, j, e, hot, nh);
+    else s = og_row_sum<MODE, HOT, OG_ROW_BATCH>(a, j, e, hot, nh);
     if (a.dd) og_epilogue<MODE>(a, v, s, st);
     else og_epilogue_pf<MODE>(a, v, s, prev, deg, st);
 }
