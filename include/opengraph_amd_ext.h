@@ -245,7 +245,9 @@ int ogpu_ppr_ranks(ogpu_ppr *p, float *x_out);
  * iteration's L1 change per column (0 after the column stopped).  Returns
  * the rows written (<= max_rows) or -1. */
 int ogpu_ppr_columns(ogpu_ppr *p, uint32_t *iters, double *l1_hist, uint32_t max_rows);
-/* Algorithmic bytes of one iteration: E (4 + 4k) + N (8 + 12k). */
+/* Algorithmic bytes of one iteration, E (4 + 4k') + N (8 + 12k') with k'
+ * the column slots the iteration gathers: after a run, the mean over its
+ * executed iterations (slots shrink as columns converge); before, k' = k. */
 double ogpu_ppr_bytes_per_iteration(const ogpu_ppr *p);
 void ogpu_ppr_free(ogpu_ppr *p);
 

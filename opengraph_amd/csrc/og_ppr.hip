@@ -31,9 +31,10 @@
 #define PWAVES (PB / 64)
 #define PG 16            /* gathers in flight per lane */
 
-/* Sum over the in-edges [eb, ee) of c_in[u][col], in CSR order. */
+/* Sum over the in-edges [eb, ee) of c_in[u][col], in CSR order; lanes whose
+ * column is not iterating (on == false) load nothing and return 0. */
 __device__ __forceinline__ float og_ppr_gather(const og_ppr_dev &a, uint32_t eb, uint32_t ee,
-                                               uint32_t lane, uint32_t col)
+                                               uint32_t lane, uint32_t col, bool on)
 {
     float s = 0.0f;
     for (uint32_t base = eb; base < ee; base += 64)
@@ -47,7 +48,7 @@ __device__ __forceinline__ float og_ppr_gather(const og_ppr_dev &a, uint32_t eb,
             for (uint32_t u = 0; u < PG; u++)
             {
                 float t = 0.0f;
-                if (u0 + u < n)
+                if (u0 + u < n && on)
                 {
                     const uint32_t src = __builtin_amdgcn_readlane(my, u0 + u);
                     t = a.c_in[(size_t)src * a.k + col];
@@ -62,24 +63,21 @@ __device__ __forceinline__ float og_ppr_gather(const og_ppr_dev &a, uint32_t eb,
     return s;
 }
 
-/* New x and c of row v in column `col` (lane < k), L1 of active columns. */
+/* New x and c of row v in slot `col` of an iterating column, and its L1.  A
+ * frozen column keeps its x; its c is never gathered again. */
 __device__ __forceinline__ void og_ppr_epilogue(const og_ppr_dev &a, uint32_t v, float s,
                                                 uint32_t col, uint32_t my_src, bool on,
                                                 double &l1)
 {
+    if (!on) return;
     const size_t i = (size_t)v * a.k + col;
     const float xo = a.x[i];
     const float base = v == my_src ? OG_BASE_PR : 0.0f;
     const float xn = base + OG_DAMP * s;
-    float xf = xo;
-    if (on)
-    {
-        a.x[i] = xn;
-        l1 += fabs((double)(xn - xo));
-        xf = xn;
-    }
+    a.x[i] = xn;
+    l1 += fabs((double)(xn - xo));
     const uint32_t deg = a.outdeg[v];
-    a.c_out[i] = deg ? xf / (float)deg : 0.0f;
+    a.c_out[i] = deg ? xn / (float)deg : 0.0f;
 }
 
 /* Per-column sum of a per-lane double over the block, in wave order;
@@ -102,7 +100,7 @@ __global__ __launch_bounds__(PB) void og_ppr_main_kernel(og_ppr_dev a)
     if (*a.done) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t col = min(lane, a.k - 1);
-    const bool live = lane < a.k;
+    const bool live = lane < a.kact;
     const uint64_t act = *a.active;
     const bool on = live && ((act >> lane) & 1u);
     const uint32_t my_src = a.src[col];
@@ -116,7 +114,7 @@ __global__ __launch_bounds__(PB) void og_ppr_main_kernel(og_ppr_dev a)
         {
             const uint32_t tag = a.ch_row[it];
             const uint32_t row = tag & ~OG_CH_MULTI;
-            const float s = og_ppr_gather(a, a.ch_edge[it], a.ch_edge[it + 1], lane, col);
+            const float s = og_ppr_gather(a, a.ch_edge[it], a.ch_edge[it + 1], lane, col, on);
             if (!live) continue;
             if (tag & OG_CH_MULTI) a.chunk_part[(size_t)it * a.k + col] = s;
             else og_ppr_epilogue(a, row, s, col, my_src, on, l1);
@@ -124,7 +122,7 @@ __global__ __launch_bounds__(PB) void og_ppr_main_kernel(og_ppr_dev a)
         else
         {
             const uint32_t v = a.C + (it - a.n_chunks);
-            const float s = og_ppr_gather(a, a.row_ptr[v], a.row_ptr[v + 1], lane, col);
+            const float s = og_ppr_gather(a, a.row_ptr[v], a.row_ptr[v + 1], lane, col, on);
             if (live) og_ppr_epilogue(a, v, s, col, my_src, on, l1);
         }
     }
@@ -141,7 +139,7 @@ __global__ __launch_bounds__(PB) void og_ppr_finalize_kernel(og_ppr_dev a)
     if (*a.done) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t col = min(lane, a.k - 1);
-    const bool live = lane < a.k;
+    const bool live = lane < a.kact;
     const uint64_t act = *a.active;
     const bool on = live && ((act >> lane) & 1u);
     const uint32_t my_src = a.src[col];
@@ -197,11 +195,12 @@ __global__ __launch_bounds__(PB) void og_ppr_finalize_kernel(og_ppr_dev a)
     if (threadIdx.x < a.k)
     {
         const uint32_t j = threadIdx.x;
+        const uint32_t column = a.colmap[j];
         const bool was = (act >> j) & 1u;
-        a.l1[(size_t)a.it * a.k + j] = was ? tot : 0.0;
+        a.l1[(size_t)a.it * a.k + column] = was ? tot : 0.0;
         if (was)
         {
-            a.iters[j] = a.it + 1;
+            a.iters[column] = a.it + 1;
             if (tot < a.epsilon) atomicOr(&s_frozen[j >> 5], 1u << (j & 31));
         }
     }
@@ -237,15 +236,43 @@ __global__ void og_ppr_init_kernel(og_ppr_dev a)
     if (blockIdx.x == 0 && threadIdx.x < a.k) a.iters[threadIdx.x] = 0;
 }
 
-/* out[old][j] = x[old2new[old]][j] (the caller's vertex order). */
+/* out[old][colmap[j]] = x[old2new[old]][j]: the caller's vertex order and
+ * column order. */
 __global__ void og_ppr_out_kernel(og_ppr_dev a, float *out)
 {
     const size_t n = (size_t)a.N * a.k;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     {
         const uint32_t o = (uint32_t)(i / a.k), j = (uint32_t)(i % a.k);
-        out[i] = a.x[(size_t)a.old2new[o] * a.k + j];
+        out[(size_t)o * a.k + a.colmap[j]] = a.x[(size_t)a.old2new[o] * a.k + j];
     }
+}
+
+/* Slot j of every row of x and c takes old slot perm[j]: a wavefront per
+ * row, every lane reads before any writes, so the move is in place. */
+__global__ __launch_bounds__(PB) void og_ppr_permute_kernel(og_ppr_dev a, float *c, og_ppr_perm perm)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = min(lane, a.k - 1);
+    const uint32_t from = perm.p[j];
+    const uint32_t W = gridDim.x * PWAVES;
+    for (uint32_t v = blockIdx.x * PWAVES + (threadIdx.x >> 6); v < a.N; v += W)
+    {
+        const size_t row = (size_t)v * a.k;
+        const float xv = a.x[row + from], cv = c[row + from];
+        __builtin_amdgcn_wave_barrier();
+        if (lane < a.k)
+        {
+            a.x[row + lane] = xv;
+            c[row + lane] = cv;
+        }
+    }
+}
+
+extern "C" int og_launch_ppr_permute(const og_ppr_dev *a, float *c, og_ppr_perm perm, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_ppr_permute_kernel, dim3(4096), dim3(PB), 0, s, *a, c, perm);
+    return og_hip_check(hipGetLastError(), "og_ppr_permute_kernel");
 }
 
 extern "C" int og_launch_ppr_init(const og_ppr_dev *a, hipStream_t s)

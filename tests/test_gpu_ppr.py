@@ -96,7 +96,9 @@ def test_batched_bitexact_vs_single_source_runs(er16, k):
     iters, hist = p.columns()
     p.free()
     assert res.converged == k
-    for j in (range(k) if k <= 3 else range(0, k, 9)):
+    # every column: converged columns move out of the gathered slots during
+    # the run, and each must still match its own single-source run
+    for j in range(k):
         x, r1, err = _single(g, src[j])
         assert iters[j] == r1.rows, j
         assert np.array_equal(X[:, j].view(np.uint32), x.view(np.uint32)), j
