@@ -145,8 +145,15 @@ static void orc_map(const uint32_t *sorted_src, uint32_t E, uint32_t N,
 int orc_build_csr(const uint32_t *src, const uint32_t *dst, uint32_t E,
                   uint32_t sort_mode, int nthreads, orc_csr *g)
 {
+    return orc_build_csr_n(src, dst, E, 0, sort_mode, nthreads, g);
+}
+
+int orc_build_csr_n(const uint32_t *src, const uint32_t *dst, uint32_t E, uint32_t min_vertices,
+                    uint32_t sort_mode, int nthreads, orc_csr *g)
+{
     memset(g, 0, sizeof(*g));
     uint32_t N = orc_num_vertices(src, dst, E);
+    if (N < min_vertices) N = min_vertices;
     g->num_vertices = N;
     g->num_edges = E;
     size_t eb = (size_t)E * 4, nb = (size_t)N * 4;
@@ -664,4 +671,56 @@ void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n)
     }
     orc_stable_sort_pairs(key, labels, n, 1);
     free(key);
+}
+
+int orc_reorder_labels(const uint32_t *src, const uint32_t *dst, uint32_t num_edges, uint32_t N,
+                       uint32_t lmode, uint32_t *labels)
+{
+    if (lmode < 1 || lmode > 5) return -1;
+    uint32_t *deg = orc_alloc((size_t)N * 4), *key = orc_alloc((size_t)N * 4);
+    if (!deg || !key) { free(deg); free(key); return -1; }
+    /* reorder.c:1483-1530: out-degree (1, 4), in-degree (2, 5), both (3) */
+    memset(deg, 0, (size_t)N * 4);
+    for (uint32_t i = 0; i < num_edges; i++)
+    {
+        if (lmode != 2 && lmode != 5) deg[src[i]]++;
+        if (lmode == 2 || lmode == 3 || lmode == 5) deg[dst[i]]++;
+    }
+    if (lmode <= 3)
+    {
+        /* reorder.c:296-358: stable ascending sort of the vertex ids by
+         * degree (radixSortEdgesByDegree), then labels[sorted[i]] = N-1-i:
+         * the largest degree gets label 0, equal degrees in decreasing id
+         * order. */
+        for (uint32_t v = 0; v < N; v++) { key[v] = deg[v]; labels[v] = v; }
+        orc_stable_sort_pairs(key, labels, N, 1);
+        memcpy(key, labels, (size_t)N * 4);
+        for (uint32_t i = 0; i < N; i++) labels[key[i]] = N - 1 - i;
+    }
+    else
+    {
+        /* reorder.c:387-437 thresholds from the integer average degree,
+         * :439-520: bucket = first threshold >= degree; labels run over the
+         * buckets from the last to the first, within a bucket in id order. */
+        const uint32_t nb = 11;
+        uint32_t th[11];
+        const uint32_t avg = num_edges / N;
+        th[0] = avg <= 1 ? 1 : avg / 2;
+        for (uint32_t i = 1; i < nb - 1; i++) th[i] = th[i - 1] * 2;
+        th[nb - 1] = 0xFFFFFFFFu;
+        uint32_t count[11] = {0}, start[11];
+        for (uint32_t v = 0; v < N; v++)
+        {
+            uint32_t b = 0;
+            while (deg[v] > th[b]) b++;
+            key[v] = b;
+            count[b]++;
+        }
+        uint32_t at = 0;
+        for (int b = (int)nb - 1; b >= 0; b--) { start[b] = at; at += count[b]; }
+        for (uint32_t v = 0; v < N; v++) labels[v] = start[key[v]]++;
+    }
+    free(deg);
+    free(key);
+    return 0;
 }

@@ -68,6 +68,11 @@ int orc_stable_sort_pairs(uint32_t *key, uint32_t *val, uint32_t n, int nthreads
  * Allocates every array of *out.  Returns 0, or -1 on allocation failure. */
 int orc_build_csr(const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
                   uint32_t sort_mode, int nthreads, orc_csr *out);
+/* The same with at least min_vertices vertices: a relabelled edge list keeps
+ * the vertex count of the list it came from (reorder.c:1618-1637 changes
+ * only the ids), even when its largest label has no edge. */
+int orc_build_csr_n(const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
+                    uint32_t min_vertices, uint32_t sort_mode, int nthreads, orc_csr *out);
 void orc_free_csr(orc_csr *g);
 
 /* pageRank.c:871-2896: the -p code variants 0..13 on CSR.
@@ -119,6 +124,13 @@ double orc_mlp_grad(const float *params, const float *feat, const float *y, uint
 
 /* reorder.c:64-221: stable ascending sort of labels by FloatToFixed32SORT(rank). */
 void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n);
+
+/* reorder.c:275-572 and :1483-1554: the new label of every vertex for the
+ * reference's -l option lmode (1 out-degree, 2 in-degree, 3 in+out degree:
+ * degree sort; 4 out-degree, 5 in-degree: DBG buckets) of the edge list
+ * (src, dst) with N vertices.  Returns 0, or -1 for another lmode. */
+int orc_reorder_labels(const uint32_t *src, const uint32_t *dst, uint32_t num_edges, uint32_t N,
+                       uint32_t lmode, uint32_t *labels);
 
 #ifdef __cplusplus
 }

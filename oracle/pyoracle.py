@@ -51,6 +51,9 @@ def lib():
         L = C.CDLL(ORACLE_LIB)
         L.orc_build_csr.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(OrcCSR)]
         L.orc_build_csr.restype = C.c_int
+        L.orc_build_csr_n.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                      C.POINTER(OrcCSR)]
+        L.orc_build_csr_n.restype = C.c_int
         L.orc_free_csr.argtypes = [C.POINTER(OrcCSR)]
         L.orc_pagerank.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_double, C.c_uint32, C.c_int,
                                    f32p, C.c_int, f32p, u32p, f64p, C.POINTER(OrcResult)]
@@ -71,6 +74,8 @@ def lib():
         L.orc_sort_labels_by_rank.argtypes = [f32p, u32p, C.c_uint32]
         L.orc_stable_sort_pairs.argtypes = [u32p, u32p, C.c_uint32, C.c_int]
         L.orc_stable_sort_pairs.restype = C.c_int
+        L.orc_reorder_labels.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
+        L.orc_reorder_labels.restype = C.c_int
         _lib = L
     return _lib
 
@@ -107,12 +112,12 @@ class CSR:
         return self.a[k]
 
 
-def build_csr(src, dst, sort_mode=0, nthreads=0) -> CSR:
+def build_csr(src, dst, sort_mode=0, nthreads=0, min_vertices=0) -> CSR:
     L = lib()
     src = np.ascontiguousarray(src, dtype=np.uint32)
     dst = np.ascontiguousarray(dst, dtype=np.uint32)
     g = OrcCSR()
-    rc = L.orc_build_csr(_p(src), _p(dst), len(src), sort_mode, nthreads, C.byref(g))
+    rc = L.orc_build_csr_n(_p(src), _p(dst), len(src), min_vertices, sort_mode, nthreads, C.byref(g))
     if rc:
         raise RuntimeError("orc_build_csr failed")
     N, E = g.num_vertices, g.num_edges
@@ -209,6 +214,16 @@ def sort_labels_by_rank(ranks, labels):
     r = np.ascontiguousarray(ranks, dtype=np.float32)
     lab = np.ascontiguousarray(labels, dtype=np.uint32).copy()
     lib().orc_sort_labels_by_rank(_p(r, f32p), _p(lab), len(lab))
+    return lab
+
+
+def reorder_labels(src, dst, num_vertices, lmode):
+    """New label of every vertex for the reference's -l lmode (1..5)."""
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    lab = np.zeros(num_vertices, np.uint32)
+    if lib().orc_reorder_labels(_p(s), _p(d), len(s), num_vertices, lmode, _p(lab)):
+        raise ValueError(f"lmode {lmode} is not a degree or DBG relabelling")
     return lab
 
 

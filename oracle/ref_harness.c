@@ -11,11 +11,14 @@
  * the reference's (commented-out) rank printing.
  *
  * usage: ref_harness <graph.bin> <sort> <pushpull> <epsilon> <iterations>
- *                    <threads> <out.bin> [dump_csr 0|1]
+ *                    <threads> <out.bin> [dump_csr 0|1] [lmode]
  * out.bin: u32 magic 'OGRF', u32 N, u32 E, u32 iterations, f64 error_total,
  *          f64 time_total, f32 ranks[N], u32 sorted_labels[N]
  *          [then if dump_csr: fwd src,dst[E]; fwd out,in,idx[N];
- *                             inv src,dst[E]; inv out,in,idx[N]]
+ *                             inv src,dst[E]; inv out,in,idx[N]
+ *           and if lmode was given: fwd label_array[N]]
+ * lmode is the reference's -l reordering (reorder.c:1556-1616) applied by
+ * graphCSRPreProcessingStep before the CSR is built.
  * The per-iteration table the reference prints stays on stdout.
  */
 #include <omp.h>
@@ -63,6 +66,8 @@ int main(int argc, char **argv)
     a->iterations = (uint32_t)atoi(argv[5]);
     int threads = atoi(argv[6]);
     int dump_csr = argc > 8 ? atoi(argv[8]) : 0;
+    int with_lmode = argc > 9;
+    if (with_lmode) a->lmode = (uint32_t)atoi(argv[9]);
     a->pre_numThreads = a->algo_numThreads = a->ker_numThreads = threads;
     omp_set_nested(1);
     omp_set_num_threads(threads);
@@ -96,6 +101,7 @@ int main(int argc, char **argv)
         put(f, g->inverse_vertices->out_degree, (size_t)N * 4);
         put(f, g->inverse_vertices->in_degree, (size_t)N * 4);
         put(f, g->inverse_vertices->edges_idx, (size_t)N * 4);
+        if (with_lmode) put(f, fe->label_array, (size_t)N * 4);
     }
     fclose(f);
     free(labels);
