@@ -219,6 +219,32 @@ struct EdgeList *countSortEdgesBySourceAndDestination(struct EdgeList *edgeList)
  * in place, ascending by fixed-point rank, stable. */
 uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t num_vertices);
 
+/* ---- degree relabelling before the CSR build (-l 1..5), on the GPU ----
+ * graphCSRPreProcessingStep applies -l, -L, -O (graphCSR.c:184-203) when
+ * each is 0 or one of these; the other reorderings (HUBSort, HUBCluster,
+ * Corder, labels from a file, the thread-dependent random -l 10) exit. */
+/* reorder.c:1483-1554: adds the out-degree (lmode 1, 4, 6, 8, default), the
+ * in-degree (2, 5, 7, 9) or both (3) of every vertex to degrees. */
+uint32_t *reorderGraphGenerateInOutDegrees(uint32_t *degrees, struct EdgeList *edgeList, uint32_t lmode);
+/* reorder.c:223-271: stable ascending sort of the (degree, label) pairs;
+ * both arrays come back sorted.  Returns labels. */
+uint32_t *radixSortEdgesByDegree(uint32_t *degrees, uint32_t *labels, uint32_t num_vertices);
+/* reorder.c:1618-1637: src and dst of every edge through labels. */
+struct EdgeList *relabelEdgeList(struct EdgeList *edgeList, uint32_t *labels);
+/* reorder.c:296-384: labels by decreasing degree (ties by decreasing id),
+ * relabel, update label_array / inverse_label_array. */
+struct EdgeList *reorderGraphListDegree(struct EdgeList *edgeList, uint32_t *degrees, uint32_t lmode);
+/* reorder.c:275-294 */
+struct EdgeList *reorderGraphProcessDegree(uint32_t sort, struct EdgeList *edgeList, uint32_t lmode);
+/* reorder.c:439-572: DBG -- bucket b = first threshold >= degree; labels
+ * over the buckets from the last to the first, in id order within one. */
+struct EdgeList *reorderGraphListDBG(struct EdgeList *edgeList, uint32_t *degrees, uint32_t *thresholds,
+                                     uint32_t num_buckets, uint32_t lmode);
+/* reorder.c:387-437: 11 thresholds from the edge list's average degree. */
+struct EdgeList *reorderGraphProcessDBG(uint32_t sort, struct EdgeList *edgeList, uint32_t lmode);
+/* reorder.c:1556-1616 for lmode 0..5. */
+struct EdgeList *reorderGraphProcess(struct EdgeList *edgeList, struct Arguments *arguments);
+
 #ifdef __cplusplus
 }
 #endif

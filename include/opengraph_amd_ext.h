@@ -59,6 +59,16 @@ int ogpu_write_bin(const char *path, const uint32_t *src, const uint32_t *dst, u
  * semantics; sort_mode as -o).  Inputs are not modified. */
 int ogpu_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
                      uint32_t sort_mode, ogpu_graph **out);
+/* The reference's degree relabelling before the CSR build (graphCSR.c:184-203
+ * with reorder.c:275-572 and :1618-1637; -l / -L / -O): lmodes[i] in 1 out-,
+ * 2 in-, 3 in+out-degree sort, 4 / 5 DBG by out- / in-degree (0 skips a
+ * level), applied in order; then the CSR of the relabelled edges, keeping
+ * N = max id + 1 of the input.  d_labels_out (N entries, nullable) gets the
+ * new label of every input vertex (the edge list's label_array).  Inputs
+ * are not modified; another lmode is an error. */
+int ogpu_graph_build_reordered(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
+                               uint32_t sort_mode, const uint32_t *lmodes, uint32_t n_lmodes,
+                               uint32_t *d_labels_out, ogpu_graph **out);
 /* Upload host arrays in the order f_src, f_dst, f_out_degree, f_in_degree,
  * f_edges_idx, i_src, i_dst, i_out_degree, i_in_degree, i_edges_idx. */
 int ogpu_graph_from_arrays(uint32_t num_vertices, uint32_t num_edges,

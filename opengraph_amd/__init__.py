@@ -139,6 +139,14 @@ SIGNATURES = {
     "freeEdgeList": (None, [C.POINTER(EdgeList)]),
     "sortRunAlgorithms": (C.POINTER(EdgeList), [C.POINTER(EdgeList), u32]),
     "radixSortEdgesBySource": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
+    "reorderGraphGenerateInOutDegrees": (u32p, [u32p, C.POINTER(EdgeList), u32]),
+    "radixSortEdgesByDegree": (u32p, [u32p, u32p, u32]),
+    "relabelEdgeList": (C.POINTER(EdgeList), [C.POINTER(EdgeList), u32p]),
+    "reorderGraphListDegree": (C.POINTER(EdgeList), [C.POINTER(EdgeList), u32p, u32]),
+    "reorderGraphProcessDegree": (C.POINTER(EdgeList), [u32, C.POINTER(EdgeList), u32]),
+    "reorderGraphListDBG": (C.POINTER(EdgeList), [C.POINTER(EdgeList), u32p, u32p, u32, u32]),
+    "reorderGraphProcessDBG": (C.POINTER(EdgeList), [u32, C.POINTER(EdgeList), u32]),
+    "reorderGraphProcess": (C.POINTER(EdgeList), [C.POINTER(EdgeList), C.POINTER(Arguments)]),
     "radixSortEdgesBySourceAndDestination": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
     "countSortEdgesBySource": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
     "countSortEdgesBySourceAndDestination": (C.POINTER(EdgeList), [C.POINTER(EdgeList)]),
@@ -159,6 +167,7 @@ SIGNATURES = {
     "ogpu_er_host": (C.c_int, [u32, C.c_uint64, C.c_uint64, _VP, _VP]),
     "ogpu_write_bin": (C.c_int, [C.c_char_p, _VP, _VP, C.c_uint64]),
     "ogpu_graph_build": (C.c_int, [_VP, _VP, u32, u32, C.POINTER(_VP)]),
+    "ogpu_graph_build_reordered": (C.c_int, [_VP, _VP, u32, u32, u32p, u32, _VP, C.POINTER(_VP)]),
     "ogpu_graph_from_arrays": (C.c_int, [u32, u32, C.POINTER(_VP), C.POINTER(_VP)]),
     "ogpu_graph_info": (C.c_int, [_VP, u32p, u32p]),
     "ogpu_graph_array": (C.c_int, [_VP, u32, _VP]),
@@ -327,6 +336,23 @@ class Graph:
         h = C.c_void_p()
         check(load().ogpu_graph_build(d_src.ptr, d_dst.ptr, num_edges, sort_mode, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def from_edges_reordered(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0, lmodes=(1,)):
+        """CSR after the reference's -l relabellings (ogpu_graph_build_reordered);
+        returns (graph, labels of the input vertices)."""
+        ds, dd = DeviceArray.from_numpy(src.astype(np.uint32)), DeviceArray.from_numpy(dst.astype(np.uint32))
+        n = int(max(src.max(), dst.max())) + 1 if len(src) else 1
+        dl = DeviceArray(4 * n)
+        lm = np.ascontiguousarray(lmodes, dtype=np.uint32)
+        h = C.c_void_p()
+        try:
+            check(load().ogpu_graph_build_reordered(ds.ptr, dd.ptr, len(src), sort_mode,
+                                                    lm.ctypes.data_as(u32p), len(lm), dl.ptr, C.byref(h)))
+            labels = dl.to_numpy(np.uint32, n)
+        finally:
+            ds.free(); dd.free(); dl.free()
+        return cls(h), labels
 
     @classmethod
     def from_edges(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0):

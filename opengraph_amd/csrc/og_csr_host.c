@@ -425,11 +425,22 @@ int ogpu_graph_to_host(const ogpu_graph *gg, struct GraphCSR **out)
 struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 {
     /* graphCSR.c:164-248 */
-    if (arguments->dflag || arguments->lmode || arguments->lmode_l2 || arguments->lmode_l3 ||
-        arguments->mmode)
+    const uint32_t lm[3] = {arguments->lmode, arguments->lmode_l2, arguments->lmode_l3};
+    int any_lmode = 0;
+    for (int i = 0; i < 3; i++)
     {
-        fprintf(stderr, "opengraph_amd: duplicate removal (-k), reordering (-l/-L/-O) and masking (-M) "
-                        "are not supported by this CSR builder\n");
+        if (lm[i] > 5)
+        {
+            fprintf(stderr, "opengraph_amd: -l/-L/-O %u: only the degree (1, 2, 3) and DBG (4, 5) "
+                            "relabellings are built\n", lm[i]);
+            exit(EXIT_FAILURE);
+        }
+        any_lmode |= lm[i] != 0;
+    }
+    if (arguments->dflag || arguments->mmode)
+    {
+        fprintf(stderr, "opengraph_amd: duplicate removal (-k) and masking (-M) are not supported by "
+                        "this CSR builder\n");
         exit(EXIT_FAILURE);
     }
     if (og_ctx_ensure()) og_die("device");
@@ -462,7 +473,27 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 
     t0 = og_wall();
     og_graph *g = NULL;
-    if (og_graph_build_device(d_src, d_dst, E, arguments->sort, 0, &g)) og_die("graphCSRPreProcessingStep");
+    uint32_t *d_labels = NULL;
+    if (any_lmode)
+    {
+        /* graphCSR.c:184-203: the -l, -L, -O relabellings in turn, then the
+         * CSR of the relabelled list with the input's vertex count */
+        double t1 = og_wall();
+        uint32_t N0 = 0;
+        uint32_t *dmax = og_dalloc(4);
+        if (!dmax || og_launch_max_id(d_src, d_dst, E, dmax, s) ||
+            og_hip_check(hipMemcpyAsync(&N0, dmax, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+            og_hip_check(hipStreamSynchronize(s), "sync"))
+            og_die("graphCSRPreProcessingStep");
+        og_dfree(dmax);
+        N0 = E ? N0 + 1 : 1;
+        if (og_graph_build_reordered_inplace(d_src, d_dst, E, N0, lm, 3, &d_labels))
+            og_die("graphCSRPreProcessingStep");
+        og_print_time("Total Reorder Complete (Seconds)", og_wall() - t1);
+        if (og_graph_build_device_n(d_src, d_dst, E, N0, arguments->sort, &g)) og_die("graphCSRPreProcessingStep");
+    }
+    else if (og_graph_build_device(d_src, d_dst, E, arguments->sort, 0, &g))
+        og_die("graphCSRPreProcessingStep");
     og_dfree(d_src);
     og_dfree(d_dst);
     static const char *msg[4] = {"Radix Sort Edges By Source (Seconds)",
@@ -474,6 +505,20 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     struct GraphCSR *h = NULL;
     if (ogpu_graph_to_host((ogpu_graph *)g, &h)) og_die("graphCSRPreProcessingStep");
     og_graph_destroy(g);
+    if (d_labels)
+    {
+        /* reorder.c:368-373: the edge lists' label arrays carry the new
+         * label of every input vertex (readEdgeListsMem copies them to the
+         * inverse list, edgeList.c:718-727) */
+        struct EdgeList *fe = h->sorted_edges_array, *ie = h->inverse_sorted_edges_array;
+        if (og_hip_check(hipMemcpy(fe->label_array, d_labels, (size_t)h->num_vertices * 4, hipMemcpyDeviceToHost),
+                         "D2H labels"))
+            og_die("graphCSRPreProcessingStep");
+        og_dfree(d_labels);
+        for (uint32_t v = 0; v < h->num_vertices; v++) fe->inverse_label_array[fe->label_array[v]] = v;
+        memcpy(ie->label_array, fe->label_array, (size_t)h->num_vertices * 4);
+        memcpy(ie->inverse_label_array, fe->inverse_label_array, (size_t)h->num_vertices * 4);
+    }
     og_print_time("Mappign Vertices to CSR (Seconds)", og_wall() - t0);
     graphCSRPrint(h);
     return h;
@@ -504,4 +549,209 @@ uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t 
     og_dfree(dr); og_dfree(k0); og_dfree(v0); og_dfree(k1); og_dfree(v1); og_dfree(k2); og_dfree(v2);
     og_dfree(scr);
     return labels;
+}
+
+/* ----------------------- degree relabelling ----------------------- */
+
+/* Upload n u32 to a new device buffer (dies on failure). */
+static uint32_t *og_up(const uint32_t *h, uint32_t n)
+{
+    uint32_t *d = og_dalloc((size_t)(n ? n : 1) * 4);
+    if (!d || (n && og_hip_check(hipMemcpy(d, h, (size_t)n * 4, hipMemcpyHostToDevice), "H2D")))
+        og_die("upload");
+    return d;
+}
+
+static void og_down(uint32_t *h, const uint32_t *d, uint32_t n)
+{
+    if (n && og_hip_check(hipMemcpy(h, d, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H")) og_die("download");
+}
+
+static void og_reorder_banner(const char *what)
+{
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", what);
+    printf(" -----------------------------------------------------\n");
+}
+
+uint32_t *reorderGraphGenerateInOutDegrees(uint32_t *degrees, struct EdgeList *el, uint32_t lmode)
+{
+    /* reorder.c:1483-1554: the counts are added to degrees (the callers
+     * pass it zeroed).  lmode 10 (random keys drawn per OpenMP thread) has
+     * no thread-independent result and is not built. */
+    if (lmode == 10)
+    {
+        fprintf(stderr, "opengraph_amd: -l 10 (random relabelling) is not supported\n");
+        exit(EXIT_FAILURE);
+    }
+    if (og_ctx_ensure()) og_die("reorderGraphGenerateInOutDegrees");
+    const uint32_t N = el->num_vertices, E = el->num_edges;
+    const uint32_t count_src = !(lmode == 2 || lmode == 5 || lmode == 7 || lmode == 9);
+    const uint32_t count_dst = lmode == 2 || lmode == 3 || lmode == 5 || lmode == 7 || lmode == 9;
+    uint32_t *ds = og_up(el->edges_array_src, E), *dd = og_up(el->edges_array_dest, E);
+    uint32_t *deg = og_up(degrees, N);
+    if (og_launch_reorder_degrees(ds, dd, E, count_src, count_dst, deg, og_stream()) ||
+        og_hip_check(hipStreamSynchronize(og_stream()), "sync"))
+        og_die("reorderGraphGenerateInOutDegrees");
+    og_down(degrees, deg, N);
+    og_dfree(ds); og_dfree(dd); og_dfree(deg);
+    return degrees;
+}
+
+uint32_t *radixSortEdgesByDegree(uint32_t *degrees, uint32_t *labels, uint32_t n)
+{
+    /* reorder.c:223-271: stable ascending sort of (degree, label) pairs;
+     * both arrays come back sorted */
+    if (n < 2) return labels;
+    if (og_ctx_ensure()) og_die("radixSortEdgesByDegree");
+    hipStream_t s = og_stream();
+    size_t b = (size_t)n * 4;
+    uint32_t *k0 = og_up(degrees, n), *v0 = og_up(labels, n);
+    uint32_t *k1 = og_dalloc(b), *v1 = og_dalloc(b), *k2 = og_dalloc(b), *v2 = og_dalloc(b);
+    void *scr = og_dalloc(og_radix_scratch_bytes(n));
+    int in_b = 0;
+    if (!k1 || !v1 || !k2 || !v2 || !scr || og_radix_sort_pairs(k0, v0, k1, v1, k2, v2, n, 32, scr, s, &in_b) ||
+        og_hip_check(hipStreamSynchronize(s), "sync"))
+        og_die("radixSortEdgesByDegree");
+    og_down(degrees, in_b ? k2 : k1, n);
+    og_down(labels, in_b ? v2 : v1, n);
+    og_dfree(k0); og_dfree(v0); og_dfree(k1); og_dfree(v1); og_dfree(k2); og_dfree(v2); og_dfree(scr);
+    return labels;
+}
+
+struct EdgeList *relabelEdgeList(struct EdgeList *el, uint32_t *labels)
+{
+    /* reorder.c:1618-1637 */
+    if (og_ctx_ensure()) og_die("relabelEdgeList");
+    const uint32_t E = el->num_edges;
+    uint32_t *ds = og_up(el->edges_array_src, E), *dd = og_up(el->edges_array_dest, E);
+    uint32_t *dl = og_up(labels, el->num_vertices);
+    if (og_launch_relabel(ds, dd, E, dl, og_stream()) || og_hip_check(hipStreamSynchronize(og_stream()), "sync"))
+        og_die("relabelEdgeList");
+    og_down(el->edges_array_src, ds, E);
+    og_down(el->edges_array_dest, dd, E);
+    og_dfree(ds); og_dfree(dd); og_dfree(dl);
+    return el;
+}
+
+/* reorder.c:368-373 / :547-552: carry the new labels into the edge list's
+ * label arrays. */
+static void og_update_label_arrays(struct EdgeList *el, const uint32_t *labels)
+{
+    if (!el->label_array || !el->inverse_label_array) return;
+    for (uint32_t v = 0; v < el->num_vertices; v++)
+    {
+        el->label_array[v] = labels[el->label_array[v]];
+        el->inverse_label_array[el->label_array[v]] = v;
+    }
+}
+
+struct EdgeList *reorderGraphListDegree(struct EdgeList *el, uint32_t *degrees, uint32_t lmode)
+{
+    /* reorder.c:296-384 */
+    static const char *names[4] = {"OUT-DEGREE", "OUT-DEGREE", "IN-DEGREE", "(IN+OUT)-DEGREE"};
+    const uint32_t N = el->num_vertices;
+    og_reorder_banner("Starting Degree Reordering/Relabeling");
+    printf("| %-51s | \n", lmode <= 3 ? names[lmode] : "OUT-DEGREE");
+    double t0 = og_wall();
+    uint32_t *inv = (uint32_t *)og_host_alloc((size_t)(N ? N : 1) * 4);
+    uint32_t *labels = (uint32_t *)og_host_alloc((size_t)(N ? N : 1) * 4);
+    for (uint32_t v = 0; v < N; v++) inv[v] = v;
+    inv = radixSortEdgesByDegree(degrees, inv, N);
+    for (uint32_t v = 0; v < N; v++) labels[inv[v]] = N - 1 - v;
+    og_reorder_banner("Reordering Complete");
+    printf("| %-51f | \n", og_wall() - t0);
+    t0 = og_wall();
+    el = relabelEdgeList(el, labels);
+    og_reorder_banner("Relabeling Complete");
+    printf("| %-51f | \n", og_wall() - t0);
+    printf(" -----------------------------------------------------\n");
+    og_update_label_arrays(el, labels);
+    free(inv);
+    free(labels);
+    return el;
+}
+
+struct EdgeList *reorderGraphProcessDegree(uint32_t sort, struct EdgeList *el, uint32_t lmode)
+{
+    /* reorder.c:275-294 */
+    (void)sort;
+    uint32_t *degrees = (uint32_t *)og_host_alloc((size_t)(el->num_vertices ? el->num_vertices : 1) * 4);
+    memset(degrees, 0, (size_t)el->num_vertices * 4);
+    degrees = reorderGraphGenerateInOutDegrees(degrees, el, lmode);
+    el = reorderGraphListDegree(el, degrees, lmode);
+    free(degrees);
+    return el;
+}
+
+struct EdgeList *reorderGraphListDBG(struct EdgeList *el, uint32_t *degrees, uint32_t *thresholds,
+                                     uint32_t num_buckets, uint32_t lmode)
+{
+    /* reorder.c:439-572: buckets from the last to the first, each in id order */
+    (void)lmode;
+    if (num_buckets < 1 || num_buckets > 16)
+    {
+        fprintf(stderr, "opengraph_amd: DBG with %u buckets (1..16 supported)\n", num_buckets);
+        exit(EXIT_FAILURE);
+    }
+    if (og_ctx_ensure()) og_die("reorderGraphListDBG");
+    const uint32_t N = el->num_vertices;
+    og_dbg_thresholds th;
+    memset(&th, 0, sizeof(th));
+    th.n = num_buckets;
+    memcpy(th.t, thresholds, (size_t)num_buckets * 4);
+    double t0 = og_wall();
+    uint32_t *dd = og_up(degrees, N), *dl = og_dalloc((size_t)(N ? N : 1) * 4);
+    uint32_t *labels = (uint32_t *)og_host_alloc((size_t)(N ? N : 1) * 4);
+    if (!dl || (N && og_reorder_labels_from_degrees(dd, N, 1, th, dl))) og_die("reorderGraphListDBG");
+    og_down(labels, dl, N);
+    og_dfree(dd); og_dfree(dl);
+    og_reorder_banner("Reordering Complete");
+    printf("| %-51f | \n", og_wall() - t0);
+    t0 = og_wall();
+    el = relabelEdgeList(el, labels);
+    og_reorder_banner("Relabeling Complete");
+    printf("| %-51f | \n", og_wall() - t0);
+    printf(" -----------------------------------------------------\n");
+    og_update_label_arrays(el, labels);
+    free(labels);
+    return el;
+}
+
+struct EdgeList *reorderGraphProcessDBG(uint32_t sort, struct EdgeList *el, uint32_t lmode)
+{
+    /* reorder.c:387-437 */
+    (void)sort;
+    og_dbg_thresholds th = og_dbg_default_thresholds(el->avg_degree);
+    printf("| %-51s | \n", lmode == 5 ? "DBG IN-DEGREE" : "DBG OUT-DEGREE");
+    uint32_t *degrees = (uint32_t *)og_host_alloc((size_t)(el->num_vertices ? el->num_vertices : 1) * 4);
+    memset(degrees, 0, (size_t)el->num_vertices * 4);
+    degrees = reorderGraphGenerateInOutDegrees(degrees, el, lmode);
+    el = reorderGraphListDBG(el, degrees, th.t, th.n, lmode);
+    free(degrees);
+    return el;
+}
+
+struct EdgeList *reorderGraphProcess(struct EdgeList *el, struct Arguments *arguments)
+{
+    /* reorder.c:1556-1616 for the degree and DBG relabellings */
+    og_reorder_banner("Reorder Process");
+    double t0 = og_wall();
+    switch (arguments->lmode)
+    {
+    case 0: case 1: case 2: case 3:
+        el = reorderGraphProcessDegree(arguments->sort, el, arguments->lmode);
+        break;
+    case 4: case 5:
+        el = reorderGraphProcessDBG(arguments->sort, el, arguments->lmode);
+        break;
+    default:
+        fprintf(stderr, "opengraph_amd: -l %u: only the degree (1, 2, 3) and DBG (4, 5) relabellings "
+                        "are built\n", arguments->lmode);
+        exit(EXIT_FAILURE);
+    }
+    og_reorder_banner("Total Reorder Complete");
+    printf("| %-51f | \n", og_wall() - t0);
+    printf(" -----------------------------------------------------\n");
+    return el;
 }

@@ -77,6 +77,12 @@ int og_graph_build_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t
                           uint32_t sort_mode, int keep_inputs, og_graph **out)
 {
     (void)keep_inputs;
+    return og_graph_build_device_n(d_src, d_dst, E, 0, sort_mode, out);
+}
+
+int og_graph_build_device_n(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E,
+                            uint32_t min_vertices, uint32_t sort_mode, og_graph **out)
+{
     *out = NULL;
     if (og_ctx_ensure()) return -1;
     if (sort_mode > 3) sort_mode = 0;     /* sortRun.c:60-66: default is radix by source */
@@ -98,6 +104,12 @@ int og_graph_build_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t
         return -1;
     }
     uint32_t N = E ? maxid + 1 : 1;
+    if (N < min_vertices) N = min_vertices;
+    if (N - 1 > OG_ID_MASK)
+    {
+        og_set_error("%u vertices do not fit the 30-bit id space (reorder.h:18-27)", N);
+        return -1;
+    }
     og_graph *g = og_graph_alloc(N, E);
     if (!g) return -1;
     g->sort_mode = sort_mode;
@@ -403,3 +415,148 @@ int ogpu_graph_from_arrays(uint32_t N, uint32_t E, const uint32_t *const host_ar
 }
 
 void ogpu_graph_free(ogpu_graph *g) { og_graph_destroy((og_graph *)g); }
+
+/* ------------------------- degree relabelling ------------------------- */
+
+og_dbg_thresholds og_dbg_default_thresholds(uint32_t avg_degree)
+{
+    /* reorder.c:404-414: 11 buckets, the first at half the average degree
+     * (1 when that is at most 1), doubling, the last unbounded */
+    og_dbg_thresholds th;
+    memset(&th, 0, sizeof(th));
+    th.n = 11;
+    th.t[0] = avg_degree <= 1 ? 1 : avg_degree / 2;
+    for (uint32_t i = 1; i < th.n - 1; i++) th.t[i] = th.t[i - 1] * 2;
+    th.t[th.n - 1] = 0xFFFFFFFFu;
+    return th;
+}
+
+int og_reorder_labels_from_degrees(const uint32_t *d_deg, uint32_t N, uint32_t dbg, og_dbg_thresholds th,
+                                   uint32_t *d_labels)
+{
+    hipStream_t s = og_stream();
+    size_t nb = (size_t)N * 4;
+    uint32_t *key = og_dalloc(nb), *ids = og_dalloc(nb);
+    uint32_t *ka = og_dalloc(nb), *va = og_dalloc(nb), *kb = og_dalloc(nb), *vb = og_dalloc(nb);
+    void *scratch = og_dalloc(og_radix_scratch_bytes(N));
+    int rc = -1, in_b = 0;
+    if (!key || !ids || !ka || !va || !kb || !vb || !scratch) goto done;
+    if (dbg)
+    {
+        /* reorder.c:439-523: buckets from the last to the first, each in id
+         * order: a stable sort of the ids by (n - 1 - bucket) */
+        if (th.n < 1 || th.n > 16)
+        {
+            og_set_error("DBG needs 1..16 buckets (got %u)", th.n);
+            goto done;
+        }
+        if (og_launch_reorder_dbg_keys(d_deg, N, th, key, ids, s) ||
+            og_radix_sort_pairs(key, ids, ka, va, kb, vb, N, 8, scratch, s, &in_b) ||
+            og_launch_reorder_labels(in_b ? vb : va, N, 0, d_labels, s))
+            goto done;
+    }
+    else
+    {
+        /* reorder.c:223-271 and :337-347: stable ascending sort of the ids by
+         * degree, then labels[sorted[i]] = N - 1 - i */
+        if (og_hip_check(hipMemcpyAsync(key, d_deg, nb, hipMemcpyDeviceToDevice, s), "D2D") ||
+            og_launch_iota(ids, N, s) ||
+            og_radix_sort_pairs(key, ids, ka, va, kb, vb, N, 32, scratch, s, &in_b) ||
+            og_launch_reorder_labels(in_b ? vb : va, N, 1, d_labels, s))
+            goto done;
+    }
+    rc = og_hip_check(hipStreamSynchronize(s), "sync relabel");
+done:
+    og_dfree(key); og_dfree(ids); og_dfree(ka); og_dfree(va); og_dfree(kb); og_dfree(vb);
+    og_dfree(scratch);
+    return rc;
+}
+
+int og_reorder_labels_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E, uint32_t N,
+                             uint32_t avg_degree, uint32_t lmode, uint32_t *d_labels)
+{
+    if (lmode < 1 || lmode > 5)
+    {
+        og_set_error("-l %u: only the degree (1, 2, 3) and DBG (4, 5) relabellings are built", lmode);
+        return -1;
+    }
+    if (og_ctx_ensure()) return -1;
+    hipStream_t s = og_stream();
+    uint32_t *deg = og_dalloc((size_t)N * 4);
+    if (!deg) return -1;
+    /* reorder.c:1492-1530: out-degree for 1 and 4, in-degree for 2 and 5, both for 3 */
+    const uint32_t count_src = lmode != 2 && lmode != 5, count_dst = lmode == 2 || lmode == 3 || lmode == 5;
+    int rc = og_launch_fill_u32(deg, 0, N, s) ||
+             og_launch_reorder_degrees(d_src, d_dst, E, count_src, count_dst, deg, s) ||
+             og_reorder_labels_from_degrees(deg, N, lmode >= 4, og_dbg_default_thresholds(avg_degree), d_labels);
+    og_dfree(deg);
+    return rc ? -1 : 0;
+}
+
+int og_graph_build_reordered_inplace(uint32_t *src, uint32_t *dst, uint32_t E, uint32_t N,
+                                     const uint32_t *lmodes, uint32_t n_lmodes, uint32_t **labels_out)
+{
+    *labels_out = NULL;
+    hipStream_t s = og_stream();
+    /* edgeList.c:569: the integer average degree the DBG thresholds use */
+    const uint32_t avg = E / N;
+    size_t nb = (size_t)N * 4;
+    uint32_t *lab = og_dalloc(nb), *acc = og_dalloc(nb);
+    if (!lab || !acc || og_launch_iota(acc, N, s)) goto fail;
+    /* graphCSR.c:184-203: each level relabels the (sorted) list and sorts it
+     * again; the labels and the final stable sort do not depend on the
+     * order the edges had, so the relabelled list in input order, sorted
+     * once at the end, gives the same CSR. */
+    for (uint32_t i = 0; i < n_lmodes; i++)
+    {
+        if (!lmodes[i]) continue;
+        if (og_reorder_labels_device(src, dst, E, N, avg, lmodes[i], lab) ||
+            og_launch_relabel(src, dst, E, lab, s) || og_launch_compose_labels(acc, NULL, N, lab, s))
+            goto fail;
+    }
+    if (og_hip_check(hipStreamSynchronize(s), "sync relabel")) goto fail;
+    og_dfree(lab);
+    *labels_out = acc;
+    return 0;
+fail:
+    og_dfree(lab);
+    og_dfree(acc);
+    return -1;
+}
+
+int ogpu_graph_build_reordered(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
+                               uint32_t sort_mode, const uint32_t *lmodes, uint32_t n_lmodes,
+                               uint32_t *d_labels_out, ogpu_graph **out)
+{
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    hipStream_t s = og_stream();
+    const uint32_t E = num_edges;
+    uint32_t maxid = 0;
+    uint32_t *d_max = og_dalloc(4);
+    if (!d_max) return -1;
+    if (og_launch_max_id(d_src, d_dst, E, d_max, s) ||
+        og_hip_check(hipMemcpyAsync(&maxid, d_max, 4, hipMemcpyDeviceToHost, s), "copy max id") ||
+        og_hip_check(hipStreamSynchronize(s), "sync max id"))
+    {
+        og_dfree(d_max);
+        return -1;
+    }
+    og_dfree(d_max);
+    const uint32_t N = E ? maxid + 1 : 1;
+    size_t eb = (size_t)E * 4;
+    uint32_t *src = og_dalloc(eb ? eb : 4), *dst = og_dalloc(eb ? eb : 4), *acc = NULL;
+    int rc = -1;
+    if (!src || !dst) goto done;
+    if (E && (og_hip_check(hipMemcpyAsync(src, d_src, eb, hipMemcpyDeviceToDevice, s), "D2D") ||
+              og_hip_check(hipMemcpyAsync(dst, d_dst, eb, hipMemcpyDeviceToDevice, s), "D2D")))
+        goto done;
+    if (og_graph_build_reordered_inplace(src, dst, E, N, lmodes, n_lmodes, &acc)) goto done;
+    if (d_labels_out &&
+        og_hip_check(hipMemcpyAsync(d_labels_out, acc, (size_t)N * 4, hipMemcpyDeviceToDevice, s), "D2D"))
+        goto done;
+    rc = og_graph_build_device_n(src, dst, E, N, sort_mode, (og_graph **)out);
+done:
+    og_dfree(src); og_dfree(dst); og_dfree(acc);
+    return rc;
+}

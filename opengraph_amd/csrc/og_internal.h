@@ -77,6 +77,47 @@ typedef struct og_graph
 
 int og_graph_build_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E,
                           uint32_t sort_mode, int keep_inputs, og_graph **out);
+/* The same with N = max(max id + 1, min_vertices): a relabelled edge list
+ * keeps its vertex count even when its last label has no edge. */
+int og_graph_build_device_n(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E,
+                            uint32_t min_vertices, uint32_t sort_mode, og_graph **out);
+
+/* ---------------- degree relabelling (og_reorder.hip, og_graph.c) ---------------- */
+
+/* DBG thresholds (reorder.c:404-414): n buckets, t[n-1] = UINT32_MAX. */
+typedef struct og_dbg_thresholds
+{
+    uint32_t n;
+    uint32_t t[16];
+} og_dbg_thresholds;
+
+int og_launch_reorder_degrees(const uint32_t *src, const uint32_t *dst, uint32_t E, uint32_t count_src,
+                              uint32_t count_dst, uint32_t *deg, hipStream_t s);
+int og_launch_reorder_dbg_keys(const uint32_t *deg, uint32_t N, og_dbg_thresholds th, uint32_t *key,
+                               uint32_t *ids, hipStream_t s);
+int og_launch_iota(uint32_t *ids, uint32_t N, hipStream_t s);
+int og_launch_reorder_labels(const uint32_t *sorted, uint32_t N, uint32_t reverse, uint32_t *labels,
+                             hipStream_t s);
+int og_launch_relabel(uint32_t *src, uint32_t *dst, uint32_t E, const uint32_t *labels, hipStream_t s);
+int og_launch_compose_labels(uint32_t *label_array, uint32_t *inverse, uint32_t N, const uint32_t *labels,
+                             hipStream_t s);
+/* The reference's thresholds from the integer average degree (reorder.c:404-414). */
+og_dbg_thresholds og_dbg_default_thresholds(uint32_t avg_degree);
+/* d_labels[v] = new label of v for -l lmode (1 out-, 2 in-, 3 in+out-degree
+ * sort, 4 / 5 DBG by out- / in-degree; reorder.c:275-572) of the device
+ * edge list (d_src, d_dst) with N vertices; avg_degree sets the DBG
+ * thresholds.  Returns 0, or -1 (error set) for another lmode. */
+int og_reorder_labels_device(const uint32_t *d_src, const uint32_t *d_dst, uint32_t E, uint32_t N,
+                             uint32_t avg_degree, uint32_t lmode, uint32_t *d_labels);
+/* The relabellings lmodes[0..n) (0 skips) applied in turn to the device
+ * edge list in place; *labels_out (device, N entries, the caller frees it
+ * with og_dfree) gets the composed label of every input vertex. */
+int og_graph_build_reordered_inplace(uint32_t *src, uint32_t *dst, uint32_t E, uint32_t N,
+                                     const uint32_t *lmodes, uint32_t n_lmodes, uint32_t **labels_out);
+/* The same from given degrees (reorder.c:296-384 / :439-572): degree sort
+ * (lmode 1..3) or DBG buckets with thresholds th (lmode 4, 5). */
+int og_reorder_labels_from_degrees(const uint32_t *d_deg, uint32_t N, uint32_t dbg, og_dbg_thresholds th,
+                                   uint32_t *d_labels);
 void og_graph_destroy(og_graph *g);
 
 /* ---------------- PageRank plan / run (og_pagerank.c, og_pr.hip) ---------------- */
