@@ -40,9 +40,11 @@ static struct argp_option options[] = {
     {"num-trials", 't', "[1]", 0, "Number of trials."},
     {"tolerance", 'e', "[0.0001]", 0, "Per-vertex tolerance."},
     {"delta", 'b', "[1]", 0, "Unused by PageRank."},
-    {"light-reorder-l1", 'l', "[0]", 0, "Not supported (must be 0)."},
-    {"light-reorder-l2", 'L', "[0]", 0, "Not supported (must be 0)."},
-    {"light-reorder-l3", 'O', "[0]", 0, "Not supported (must be 0)."},
+    {"light-reorder-l1", 'l', "[0]", 0,
+     "Relabel before the CSR build: 1 out-, 2 in-, 3 in+out-degree sort, 4/5 DBG by out-/in-degree "
+     "(other orders are not supported)."},
+    {"light-reorder-l2", 'L', "[0]", 0, "A second relabelling after -l (same values)."},
+    {"light-reorder-l3", 'O', "[0]", 0, "A third relabelling after -L (same values)."},
     {"convert-format", 'c', "[1]", 0, "Unused."},
     {"generate-weights", 'w', 0, 0, "Records carry a weight (12-byte .wbin); weights are skipped."},
     {"symmetrize", 's', 0, 0, "Add the reverse of every edge."},
