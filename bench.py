@@ -100,6 +100,42 @@ def ext_variants(g, threads, csr, do_cpu):
     return out
 
 
+def ext_relabel(g, threads, csr):
+    """DBG relabelling (-l 4) of the headline edge list and the CSR build of
+    the relabelled graph on the GPU, against the plain CSR build, and the
+    reference's own compiled reorderGraphProcess on the host cores (the
+    relabelling alone) with a check that both give the same labels."""
+    import opengraph_amd as og
+    f_src, f_dst = g.array("f_src"), g.array("f_dst")
+    N, E = g.num_vertices, g.num_edges
+    ds, dd = og.DeviceArray.from_numpy(f_src), og.DeviceArray.from_numpy(f_dst)
+    t_plain, t_re, labels = [], [], None
+    for _ in range(3):
+        og.synchronize()
+        t0 = time.perf_counter()
+        h = og.Graph.build(ds, dd, E, 0)
+        og.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        h.free()
+        t0 = time.perf_counter()
+        h, labels = og.Graph.build_reordered(ds, dd, E, N, 0, (4,))
+        og.synchronize()
+        t_re.append(time.perf_counter() - t0)
+        h.free()
+    ds.free(); dd.free()
+    out = dict(workload=f"R-MAT-24 forward edge list, -l 4 (DBG by out-degree), sort 0",
+               gpu_relabel_and_csr_seconds=round(min(t_re), 4), gpu_csr_only_seconds=round(min(t_plain), 4),
+               gpu_relabel_seconds=round(min(t_re) - min(t_plain), 4))
+    if csr is not None:
+        po, _ = csr
+        if po.ref_available():
+            ref_labels, sec = po.ref_reorder(f_src, f_dst, N, 4, threads)
+            out.update(cpu_reference_relabel_seconds=round(sec, 4), cpu_threads=threads,
+                       labels_equal_to_reference=bool(np.array_equal(ref_labels, labels)),
+                       speedup_relabel_vs_cpu=round(sec / max(min(t_re) - min(t_plain), 1e-9), 1))
+    return out
+
+
 def ext_er(args):
     import opengraph_amd as og
     E = args.edge_factor << args.scale
@@ -359,6 +395,7 @@ def main():
     extensions = {}
     if not args.no_extensions:
         for name, fn in (("variants", lambda: ext_variants(g, threads, csr, csr is not None)),
+                         ("relabel_dbg", lambda: ext_relabel(g, threads, csr)),
                          ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),
                          ("uneven_batch", ext_multi), ("learned_warm_start", ext_learned)):
             t0 = time.perf_counter()

@@ -338,21 +338,31 @@ class Graph:
         return cls(h)
 
     @classmethod
-    def from_edges_reordered(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0, lmodes=(1,)):
-        """CSR after the reference's -l relabellings (ogpu_graph_build_reordered);
-        returns (graph, labels of the input vertices)."""
-        ds, dd = DeviceArray.from_numpy(src.astype(np.uint32)), DeviceArray.from_numpy(dst.astype(np.uint32))
-        n = int(max(src.max(), dst.max())) + 1 if len(src) else 1
-        dl = DeviceArray(4 * n)
+    def build_reordered(cls, d_src: DeviceArray, d_dst: DeviceArray, num_edges: int, num_vertices: int,
+                        sort_mode: int = 0, lmodes=(1,)):
+        """CSR after the reference's -l relabellings (ogpu_graph_build_reordered)
+        of device edge arrays whose largest id is num_vertices - 1; returns
+        (graph, labels of the input vertices)."""
+        dl = DeviceArray(4 * max(num_vertices, 1))
         lm = np.ascontiguousarray(lmodes, dtype=np.uint32)
         h = C.c_void_p()
         try:
-            check(load().ogpu_graph_build_reordered(ds.ptr, dd.ptr, len(src), sort_mode,
+            check(load().ogpu_graph_build_reordered(d_src.ptr, d_dst.ptr, num_edges, sort_mode,
                                                     lm.ctypes.data_as(u32p), len(lm), dl.ptr, C.byref(h)))
-            labels = dl.to_numpy(np.uint32, n)
+            labels = dl.to_numpy(np.uint32, num_vertices)
         finally:
-            ds.free(); dd.free(); dl.free()
+            dl.free()
         return cls(h), labels
+
+    @classmethod
+    def from_edges_reordered(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0, lmodes=(1,)):
+        """The same from host edge arrays."""
+        ds, dd = DeviceArray.from_numpy(src.astype(np.uint32)), DeviceArray.from_numpy(dst.astype(np.uint32))
+        n = int(max(src.max(), dst.max())) + 1 if len(src) else 1
+        try:
+            return cls.build_reordered(ds, dd, len(src), n, sort_mode, lmodes)
+        finally:
+            ds.free(); dd.free()
 
     @classmethod
     def from_edges(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0):

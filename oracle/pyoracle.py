@@ -91,6 +91,8 @@ def ref_lib():
         R.ogref_pagerank.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
             C.c_uint32, C.c_double, C.c_uint32, C.c_int, f32p, u32p, f64p, f64p]
         R.ogref_pagerank.restype = C.c_int
+        R.ogref_reorder.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, C.c_int, u32p, f64p]
+        R.ogref_reorder.restype = C.c_int
         _ref = R
     return _ref
 
@@ -242,6 +244,19 @@ def ref_pagerank(csr: CSR, code=0, epsilon=1e-4, iterations=20, threads=1):
     if rc:
         raise RuntimeError("ogref_pagerank failed")
     return dict(ranks=ranks, iterations=it.value, error_total=err.value, seconds=sec.value)
+
+
+def ref_reorder(src, dst, num_vertices, lmode, threads=1):
+    """Run the reference's own compiled reorderGraphProcess (-l lmode) on a
+    copy of the edge list; returns (labels, seconds)."""
+    R = ref_lib()
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    lab = np.zeros(num_vertices, np.uint32)
+    sec = C.c_double()
+    if R.ogref_reorder(num_vertices, len(s), _p(s), _p(d), lmode, threads, _p(lab), C.byref(sec)):
+        raise RuntimeError("ogref_reorder failed")
+    return lab, sec.value
 
 
 def read_bin(path):

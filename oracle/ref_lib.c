@@ -16,10 +16,13 @@
 #include <string.h>
 #include <unistd.h>
 
+#include "edgeList.h"
 #include "graphConfig.h"
 #include "graphCSR.h"
 #include "mt19937.h"
+#include "myMalloc.h"
 #include "pageRank.h"
+#include "reorder.h"
 
 int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
                    uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst,
@@ -72,5 +75,47 @@ int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_i
     *error_out = s->error_total;
     *seconds_out = s->time_total;
     freePageRankStats(s);
+    return 0;
+}
+
+/* The reference's own reorderGraphProcess (reorder.c:1556) with -l lmode on
+ * a copy of the edge list (src, dst) with N vertices: labels_out gets the
+ * edge list's label_array, seconds_out the wall time of the call. */
+int ogref_reorder(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst, uint32_t lmode,
+                  int threads, uint32_t *labels_out, double *seconds_out)
+{
+    struct EdgeList *el = newEdgeList(E);
+    if (!el) return -1;
+    memcpy(el->edges_array_src, src, (size_t)E * 4);
+    memcpy(el->edges_array_dest, dst, (size_t)E * 4);
+    el->num_vertices = N;
+    el->avg_degree = N ? E / N : 0;
+    el->mask_array = (uint32_t *)my_malloc((size_t)N * 4);
+    el->label_array = (uint32_t *)my_malloc((size_t)N * 4);
+    el->inverse_label_array = (uint32_t *)my_malloc((size_t)N * 4);
+    for (uint32_t v = 0; v < N; v++)
+    {
+        el->mask_array[v] = 0;
+        el->label_array[v] = v;
+        el->inverse_label_array[v] = v;
+    }
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.lmode = lmode;
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = threads;
+    omp_set_num_threads(threads);
+    fflush(stdout);
+    int saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+    double t0 = omp_get_wtime();
+    el = reorderGraphProcess(el, &a);
+    double t1 = omp_get_wtime();
+    fflush(stdout);
+    if (saved >= 0) { dup2(saved, 1); close(saved); }
+    if (devnull >= 0) close(devnull);
+    memcpy(labels_out, el->label_array, (size_t)N * 4);
+    *seconds_out = t1 - t0;
+    freeEdgeList(el);
     return 0;
 }
