@@ -415,8 +415,9 @@ __device__ __forceinline__ typename og_mode<MODE>::acc og_row_sum(const og_pr_de
  * loads per batch follow the wavefront's longest row (1, 2, 4 or
  * OG_ROW_BATCH): most short rows of a power-law graph have one to four
  * in-edges, and rows are sorted by in-degree, so the 64 rows of a
- * wavefront are about as long. */
-template <int MODE, bool HOT = false>
+ * wavefront are about as long.  ADAPT = false keeps OG_ROW_BATCH (fewer
+ * registers, for the multi-graph kernel; the sums are the same). */
+template <int MODE, bool HOT = false, bool ADAPT = true>
 __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock,
                                              const typename og_mode<MODE>::elem *hot = nullptr,
                                              uint32_t nh = 0)
@@ -435,8 +436,12 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
         deg = __builtin_nontemporal_load(&a.outdeg[v]);
     }
     /* longest row of the wavefront (every lane still active here) */
-    uint32_t m = e - j;
-    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+    uint32_t m = OG_ROW_BATCH;
+    if constexpr (ADAPT)
+    {
+        m = e - j;
+        for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+    }
     if (!mine) return;
     A s;
     if (m <= 1) s = og_row_sum<MODE, HOT, 1>(a, j, e, hot, nh);
@@ -1139,7 +1144,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m,
         og_stat st;
         og_stat_init(st);
         if (w.kind == 0) og_pull_chunks<OG_MODE_F32>(a, st, w.first);
-        else og_pull_rows<OG_MODE_F32>(a, st, w.first);
+        else og_pull_rows<OG_MODE_F32, false, false>(a, st, w.first);
         og_stat r = og_block_stat(st);
         if (threadIdx.x == 0)
         {
