@@ -214,6 +214,14 @@ struct EdgeList *countSortEdgesBySource(struct EdgeList *edgeList);
 /* countsort.c:262-290 */
 struct EdgeList *countSortEdgesBySourceAndDestination(struct EdgeList *edgeList);
 
+/* graphCSR.c:380-449: writes <fname>.csr (E, N, per vertex the forward and
+ * inverse out-degree, in-degree, edges_idx, per edge the forward and
+ * inverse src, dst; u32, host byte order); exits when it cannot. */
+void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *graphCSR);
+/* graphCSR.c:452-558: a GraphCSR from such a file (label arrays NULL, as
+ * the reference leaves them); exits when the file cannot be read. */
+struct GraphCSR *readFromBinFileGraphCSR(const char *fname);
+
 /* reorder.c:165-221: caller-side post-sort of vertex labels by
  * FloatToFixed32SORT(rank) (graphRun.c:1076).  Returns labels, reordered
  * in place, ascending by fixed-point rank, stable. */

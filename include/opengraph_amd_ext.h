@@ -69,6 +69,14 @@ int ogpu_graph_build(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_
 int ogpu_graph_build_reordered(const uint32_t *d_src, const uint32_t *d_dst, uint32_t num_edges,
                                uint32_t sort_mode, const uint32_t *lmodes, uint32_t n_lmodes,
                                uint32_t *d_labels_out, ogpu_graph **out);
+/* The reference's .csr file (graphCSR.c:380-558, directed, unweighted: E,
+ * N, then per vertex forward out-degree, in-degree, edges_idx and the
+ * inverse ones, then per edge forward src, dst and inverse src, dst, all
+ * u32): ogpu_graph_write_csr writes <fname>.csr (the name the reference
+ * gives it), ogpu_graph_read_csr loads such a file into a device graph
+ * without preprocessing.  The (de)interleaving runs on the GPU. */
+int ogpu_graph_write_csr(const ogpu_graph *g, const char *fname);
+int ogpu_graph_read_csr(const char *path, ogpu_graph **out);
 /* Upload host arrays in the order f_src, f_dst, f_out_degree, f_in_degree,
  * f_edges_idx, i_src, i_dst, i_out_degree, i_in_degree, i_edges_idx. */
 int ogpu_graph_from_arrays(uint32_t num_vertices, uint32_t num_edges,

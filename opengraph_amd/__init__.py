@@ -168,6 +168,10 @@ SIGNATURES = {
     "ogpu_write_bin": (C.c_int, [C.c_char_p, _VP, _VP, C.c_uint64]),
     "ogpu_graph_build": (C.c_int, [_VP, _VP, u32, u32, C.POINTER(_VP)]),
     "ogpu_graph_build_reordered": (C.c_int, [_VP, _VP, u32, u32, u32p, u32, _VP, C.POINTER(_VP)]),
+    "ogpu_graph_write_csr": (C.c_int, [_VP, C.c_char_p]),
+    "ogpu_graph_read_csr": (C.c_int, [C.c_char_p, C.POINTER(_VP)]),
+    "writeToBinFileGraphCSR": (None, [C.c_char_p, C.POINTER(GraphCSR)]),
+    "readFromBinFileGraphCSR": (C.POINTER(GraphCSR), [C.c_char_p]),
     "ogpu_graph_from_arrays": (C.c_int, [u32, u32, C.POINTER(_VP), C.POINTER(_VP)]),
     "ogpu_graph_info": (C.c_int, [_VP, u32p, u32p]),
     "ogpu_graph_array": (C.c_int, [_VP, u32, _VP]),
@@ -363,6 +367,17 @@ class Graph:
             return cls.build_reordered(ds, dd, len(src), n, sort_mode, lmodes)
         finally:
             ds.free(); dd.free()
+
+    @classmethod
+    def read_csr(cls, path: str):
+        """A device graph from a reference .csr file (ogpu_graph_read_csr)."""
+        h = C.c_void_p()
+        check(load().ogpu_graph_read_csr(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def write_csr(self, fname: str):
+        """Write fname + '.csr' in the reference's format (ogpu_graph_write_csr)."""
+        check(load().ogpu_graph_write_csr(self.h, fname.encode()))
 
     @classmethod
     def from_edges(cls, src: np.ndarray, dst: np.ndarray, sort_mode: int = 0):

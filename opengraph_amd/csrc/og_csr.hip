@@ -152,6 +152,77 @@ extern "C" int og_launch_deinterleave(const uint32_t *d_pairs, uint32_t n, uint3
     return og_hip_check(hipGetLastError(), "og_deinterleave_kernel");
 }
 
+/* The .csr file body (graphCSR.c:380-449, directed, unweighted): per
+ * vertex forward out-degree, in-degree, edges_idx, then the inverse ones
+ * (6 words); then per edge forward src, dst, inverse src, dst (4 words).
+ * a[10] = f_src, f_dst, f_out, f_in, f_idx, i_src, i_dst, i_out, i_in, i_idx. */
+struct og_csr_arrays
+{
+    uint32_t *a[10];
+};
+
+__global__ __launch_bounds__(OG_BLOCK) void og_csr_pack_kernel(og_csr_arrays g, uint32_t N, uint32_t E,
+                                                              uint32_t *__restrict__ body)
+{
+    const uint64_t n = N > E ? N : E;
+    for (uint64_t i = blockIdx.x * (uint64_t)OG_BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * OG_BLOCK)
+    {
+        if (i < N)
+        {
+            uint32_t *o = body + 6 * i;
+            o[0] = g.a[2][i]; o[1] = g.a[3][i]; o[2] = g.a[4][i];
+            o[3] = g.a[7][i]; o[4] = g.a[8][i]; o[5] = g.a[9][i];
+        }
+        if (i < E)
+        {
+            uint32_t *o = body + 6 * (uint64_t)N + 4 * i;
+            o[0] = g.a[0][i]; o[1] = g.a[1][i]; o[2] = g.a[5][i]; o[3] = g.a[6][i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(OG_BLOCK) void og_csr_unpack_kernel(const uint32_t *__restrict__ body, uint32_t N,
+                                                                uint32_t E, og_csr_arrays g)
+{
+    const uint64_t n = N > E ? N : E;
+    for (uint64_t i = blockIdx.x * (uint64_t)OG_BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * OG_BLOCK)
+    {
+        if (i < N)
+        {
+            const uint32_t *o = body + 6 * i;
+            g.a[2][i] = o[0]; g.a[3][i] = o[1]; g.a[4][i] = o[2];
+            g.a[7][i] = o[3]; g.a[8][i] = o[4]; g.a[9][i] = o[5];
+        }
+        if (i < E)
+        {
+            const uint32_t *o = body + 6 * (uint64_t)N + 4 * i;
+            g.a[0][i] = o[0]; g.a[1][i] = o[1]; g.a[5][i] = o[2]; g.a[6][i] = o[3];
+        }
+    }
+}
+
+extern "C" int og_launch_csr_pack(uint32_t *const arrays[10], uint32_t N, uint32_t E, uint32_t *body,
+                                  hipStream_t s)
+{
+    og_csr_arrays g;
+    for (int k = 0; k < 10; k++) g.a[k] = arrays[k];
+    const uint64_t n = N > E ? N : E;
+    if (n) hipLaunchKernelGGL(og_csr_pack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
+                              g, N, E, body);
+    return og_hip_check(hipGetLastError(), "og_csr_pack_kernel");
+}
+
+extern "C" int og_launch_csr_unpack(const uint32_t *body, uint32_t N, uint32_t E, uint32_t *const arrays[10],
+                                    hipStream_t s)
+{
+    og_csr_arrays g;
+    for (int k = 0; k < 10; k++) g.a[k] = arrays[k];
+    const uint64_t n = N > E ? N : E;
+    if (n) hipLaunchKernelGGL(og_csr_unpack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
+                              body, N, E, g);
+    return og_hip_check(hipGetLastError(), "og_csr_unpack_kernel");
+}
+
 /* ------------------------------------------------------------------ */
 /* stable LSD radix sort                                               */
 /* ------------------------------------------------------------------ */

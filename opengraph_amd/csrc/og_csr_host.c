@@ -524,6 +524,92 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
     return h;
 }
 
+/* --------------------------- .csr files --------------------------- */
+
+void writeToBinFileGraphCSR(const char *fname, struct GraphCSR *g)
+{
+    /* graphCSR.c:380-449: <fname>.csr, E, N, 6 words per vertex, 4 per edge */
+    const uint32_t N = g->num_vertices, E = g->num_edges;
+    const int inv = g->inverse_vertices != NULL;
+    const size_t per_v = inv ? 6 : 3, per_e = inv ? 4 : 2;
+    const size_t words = 2 + per_v * N + per_e * E;
+    uint32_t *h = (uint32_t *)og_host_alloc(words * 4);
+    h[0] = E;
+    h[1] = N;
+    uint32_t *o = h + 2;
+    for (uint32_t v = 0; v < N; v++, o += per_v)
+    {
+        o[0] = g->vertices->out_degree[v]; o[1] = g->vertices->in_degree[v]; o[2] = g->vertices->edges_idx[v];
+        if (inv)
+        {
+            o[3] = g->inverse_vertices->out_degree[v];
+            o[4] = g->inverse_vertices->in_degree[v];
+            o[5] = g->inverse_vertices->edges_idx[v];
+        }
+    }
+    for (uint32_t e = 0; e < E; e++, o += per_e)
+    {
+        o[0] = g->sorted_edges_array->edges_array_src[e];
+        o[1] = g->sorted_edges_array->edges_array_dest[e];
+        if (inv)
+        {
+            o[2] = g->inverse_sorted_edges_array->edges_array_src[e];
+            o[3] = g->inverse_sorted_edges_array->edges_array_dest[e];
+        }
+    }
+    char *path = (char *)og_host_alloc(strlen(fname) + 5);
+    strcpy(path, fname);
+    strcat(path, ".csr");
+    FILE *f = fopen(path, "wb");
+    if (!f) { fprintf(stderr, "open: %s: %s\n", path, strerror(errno)); exit(1); }
+    size_t wrote = fwrite(h, 4, words, f);
+    if (fclose(f) != 0 || wrote != words) { fprintf(stderr, "write: %s\n", path); exit(1); }
+    free(path);
+    free(h);
+}
+
+struct GraphCSR *readFromBinFileGraphCSR(const char *fname)
+{
+    /* graphCSR.c:452-558 */
+    FILE *f = fopen(fname, "rb");
+    if (!f) { fprintf(stderr, "open: %s: %s\n", fname, strerror(errno)); exit(1); }
+    uint32_t hdr[2] = {0, 0};
+    size_t ret = fread(hdr, 4, 2, f);
+    const uint32_t E = hdr[0], N = hdr[1];
+    struct GraphCSR *g = graphCSRNew(N, E, 1);
+    g->sorted_edges_array = newEdgeList(E);
+    g->sorted_edges_array->num_vertices = N;
+    g->inverse_sorted_edges_array = newEdgeList(E);
+    g->inverse_sorted_edges_array->num_vertices = N;
+    const size_t words = 6 * (size_t)N + 4 * (size_t)E;
+    uint32_t *h = (uint32_t *)og_host_alloc(words * 4 + 4);
+    ret = words ? fread(h, 4, words, f) : ret;
+    fclose(f);
+    if (ret != (words ? words : 2))
+    {
+        fprintf(stderr, "read: %s: truncated .csr file\n", fname);
+        exit(1);
+    }
+    const uint32_t *o = h;
+    for (uint32_t v = 0; v < N; v++, o += 6)
+    {
+        g->vertices->out_degree[v] = o[0]; g->vertices->in_degree[v] = o[1]; g->vertices->edges_idx[v] = o[2];
+        g->inverse_vertices->out_degree[v] = o[3];
+        g->inverse_vertices->in_degree[v] = o[4];
+        g->inverse_vertices->edges_idx[v] = o[5];
+    }
+    for (uint32_t e = 0; e < E; e++, o += 4)
+    {
+        g->sorted_edges_array->edges_array_src[e] = o[0];
+        g->sorted_edges_array->edges_array_dest[e] = o[1];
+        g->inverse_sorted_edges_array->edges_array_src[e] = o[2];
+        g->inverse_sorted_edges_array->edges_array_dest[e] = o[3];
+    }
+    free(h);
+    graphCSRPrint(g);
+    return g;
+}
+
 /* ------------------------ rank post-sort ------------------------ */
 
 uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t n)

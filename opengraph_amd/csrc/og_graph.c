@@ -416,6 +416,88 @@ int ogpu_graph_from_arrays(uint32_t N, uint32_t E, const uint32_t *const host_ar
 
 void ogpu_graph_free(ogpu_graph *g) { og_graph_destroy((og_graph *)g); }
 
+/* ---------------------------- .csr files ---------------------------- */
+
+static void og_graph_array_list(og_graph *g, uint32_t *a[10])
+{
+    a[0] = g->f_src; a[1] = g->f_dst; a[2] = g->f_out_degree; a[3] = g->f_in_degree; a[4] = g->f_edges_idx;
+    a[5] = g->i_src; a[6] = g->i_dst; a[7] = g->i_out_degree; a[8] = g->i_in_degree; a[9] = g->i_edges_idx;
+}
+
+int ogpu_graph_write_csr(const ogpu_graph *gg, const char *fname)
+{
+    /* graphCSR.c:380-449: <fname>.csr = E, N, 6 words per vertex, 4 per
+     * edge; interleaved on the GPU, written with one fwrite */
+    og_graph *g = (og_graph *)gg;
+    if (og_ctx_ensure()) return -1;
+    const uint32_t N = g->N, E = g->E;
+    const size_t words = 2 + 6 * (size_t)N + 4 * (size_t)E;
+    uint32_t *d_body = og_dalloc((words - 2) * 4 + 4);
+    uint32_t *h = (uint32_t *)malloc(words * 4);
+    char *path = (char *)malloc(strlen(fname) + 5);
+    int rc = -1;
+    if (!d_body || !h || !path) { og_set_error("out of memory for the .csr image"); goto done; }
+    uint32_t *arrays[10];
+    og_graph_array_list(g, arrays);
+    h[0] = E;
+    h[1] = N;
+    if (og_launch_csr_pack(arrays, N, E, d_body, og_stream()) ||
+        og_hip_check(hipMemcpy(h + 2, d_body, (words - 2) * 4, hipMemcpyDeviceToHost), "D2H .csr"))
+        goto done;
+    strcpy(path, fname);
+    strcat(path, ".csr");
+    FILE *f = fopen(path, "wb");
+    if (!f) { og_set_error("open: %s", path); goto done; }
+    size_t wrote = fwrite(h, 4, words, f);
+    if (fclose(f) != 0 || wrote != words) { og_set_error("write: %s", path); goto done; }
+    rc = 0;
+done:
+    og_dfree(d_body);
+    free(h);
+    free(path);
+    return rc;
+}
+
+int ogpu_graph_read_csr(const char *path, ogpu_graph **out)
+{
+    /* graphCSR.c:452-558 into a device graph */
+    *out = NULL;
+    if (og_ctx_ensure()) return -1;
+    FILE *f = fopen(path, "rb");
+    if (!f) { og_set_error("open: %s", path); return -1; }
+    uint32_t hdr[2];
+    if (fread(hdr, 4, 2, f) != 2) { fclose(f); og_set_error("%s: no .csr header", path); return -1; }
+    const uint32_t E = hdr[0], N = hdr[1];
+    const size_t words = 6 * (size_t)N + 4 * (size_t)E;
+    uint32_t *h = (uint32_t *)malloc(words * 4 + 4);
+    if (!h) { fclose(f); og_set_error("out of host memory"); return -1; }
+    size_t got = fread(h, 4, words, f);
+    fclose(f);
+    if (got != words)
+    {
+        free(h);
+        og_set_error("%s: %zu of %zu words after the header", path, got, words);
+        return -1;
+    }
+    og_graph *g = og_graph_alloc(N, E);
+    uint32_t *d_body = og_dalloc(words * 4 + 4);
+    int rc = -1;
+    if (!g || !d_body) goto done;
+    uint32_t *arrays[10];
+    og_graph_array_list(g, arrays);
+    if (og_hip_check(hipMemcpy(d_body, h, words * 4, hipMemcpyHostToDevice), "H2D .csr") ||
+        og_launch_csr_unpack(d_body, N, E, arrays, og_stream()) ||
+        og_hip_check(hipStreamSynchronize(og_stream()), "sync .csr"))
+        goto done;
+    rc = 0;
+done:
+    free(h);
+    og_dfree(d_body);
+    if (rc) og_graph_destroy(g);
+    else *out = (ogpu_graph *)g;
+    return rc;
+}
+
 /* ------------------------- degree relabelling ------------------------- */
 
 og_dbg_thresholds og_dbg_default_thresholds(uint32_t avg_degree)

@@ -43,6 +43,12 @@ int og_launch_er(uint32_t *d_src, uint32_t *d_dst, uint64_t num_edges, uint32_t 
 /* d_out[0] = max over src and dst (d_out zeroed here). */
 int og_launch_max_id(const uint32_t *d_src, const uint32_t *d_dst, uint32_t n, uint32_t *d_out,
                      hipStream_t s);
+/* The .csr file body (graphCSR.c:380-449): 6 words per vertex, then 4 per
+ * edge, from / into the arrays f_src, f_dst, f_out_degree, f_in_degree,
+ * f_edges_idx, i_src, i_dst, i_out_degree, i_in_degree, i_edges_idx. */
+int og_launch_csr_pack(uint32_t *const arrays[10], uint32_t N, uint32_t E, uint32_t *body, hipStream_t s);
+int og_launch_csr_unpack(const uint32_t *body, uint32_t N, uint32_t E, uint32_t *const arrays[10],
+                         hipStream_t s);
 /* Split interleaved (src,dst) u32 pairs into two arrays. */
 int og_launch_deinterleave(const uint32_t *d_pairs, uint32_t n, uint32_t stride_words,
                            uint32_t *d_src, uint32_t *d_dst, hipStream_t s);

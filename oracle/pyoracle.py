@@ -93,6 +93,8 @@ def ref_lib():
         R.ogref_pagerank.restype = C.c_int
         R.ogref_reorder.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, C.c_int, u32p, f64p]
         R.ogref_reorder.restype = C.c_int
+        R.ogref_write_csr.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [C.c_char_p]
+        R.ogref_write_csr.restype = C.c_int
         _ref = R
     return _ref
 
@@ -244,6 +246,14 @@ def ref_pagerank(csr: CSR, code=0, epsilon=1e-4, iterations=20, threads=1):
     if rc:
         raise RuntimeError("ogref_pagerank failed")
     return dict(ranks=ranks, iterations=it.value, error_total=err.value, seconds=sec.value)
+
+
+def ref_write_csr(csr: CSR, fname: str):
+    """The reference's own writeToBinFileGraphCSR: writes fname + '.csr'."""
+    arrays = [csr.a[k] for k in ["f_out_degree", "f_in_degree", "f_edges_idx", "f_src", "f_dst",
+                                  "i_out_degree", "i_in_degree", "i_edges_idx", "i_src", "i_dst"]]
+    if ref_lib().ogref_write_csr(csr.num_vertices, csr.num_edges, *[_p(x) for x in arrays], fname.encode()):
+        raise RuntimeError("ogref_write_csr failed")
 
 
 def ref_reorder(src, dst, num_vertices, lmode, threads=1):

@@ -78,6 +78,34 @@ int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_i
     return 0;
 }
 
+/* The reference's own writeToBinFileGraphCSR (graphCSR.c:380-449) on the
+ * given CSR arrays: writes <fname>.csr. */
+int ogref_write_csr(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
+                    uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst, uint32_t *i_out_degree,
+                    uint32_t *i_in_degree, uint32_t *i_edges_idx, uint32_t *i_src, uint32_t *i_dst,
+                    const char *fname)
+{
+    struct Vertex fv = {f_out_degree, f_in_degree, f_edges_idx};
+    struct Vertex iv = {i_out_degree, i_in_degree, i_edges_idx};
+    struct EdgeList fe, ie;
+    memset(&fe, 0, sizeof(fe));
+    memset(&ie, 0, sizeof(ie));
+    fe.num_edges = ie.num_edges = E;
+    fe.num_vertices = ie.num_vertices = N;
+    fe.edges_array_src = f_src; fe.edges_array_dest = f_dst;
+    ie.edges_array_src = i_src; ie.edges_array_dest = i_dst;
+    struct GraphCSR g;
+    memset(&g, 0, sizeof(g));
+    g.num_edges = E;
+    g.num_vertices = N;
+    g.vertices = &fv;
+    g.sorted_edges_array = &fe;
+    g.inverse_vertices = &iv;
+    g.inverse_sorted_edges_array = &ie;
+    writeToBinFileGraphCSR(fname, &g);
+    return 0;
+}
+
 /* The reference's own reorderGraphProcess (reorder.c:1556) with -l lmode on
  * a copy of the edge list (src, dst) with N vertices: labels_out gets the
  * edge list's label_array, seconds_out the wall time of the call. */
