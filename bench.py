@@ -136,6 +136,37 @@ def ext_relabel(g, threads, csr):
     return out
 
 
+def ext_csr_file():
+    """The reference's .csr file of an R-MAT-20 graph (seed 9): write it from
+    the device graph and load it back (GPU interleaving, one file read or
+    write), against building the CSR from the edge list on the GPU."""
+    import tempfile
+    import opengraph_amd as og
+    scale, E = 20, 16 << 20
+    ds, dd = og.rmat_device(scale, E, 0.57, 0.19, 0.19, 9, True)
+    og.synchronize()
+    t0 = time.perf_counter()
+    g = og.Graph.build(ds, dd, E, 0)
+    og.synchronize()
+    t_build = time.perf_counter() - t0
+    ds.free(); dd.free()
+    with tempfile.TemporaryDirectory() as td:
+        base = os.path.join(td, "rmat20")
+        t0 = time.perf_counter()
+        g.write_csr(base)
+        t_write = time.perf_counter() - t0
+        size = os.path.getsize(base + ".csr")
+        t0 = time.perf_counter()
+        h = og.Graph.read_csr(base + ".csr")
+        og.synchronize()
+        t_read = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for a, b in zip(g.arrays().values(), h.arrays().values()))
+    g.free(); h.free()
+    return dict(workload="R-MAT-20 (seed 9), E = 16 Mi, sort 0", file_bytes=size,
+                csr_build_from_edges_seconds=round(t_build, 4), write_seconds=round(t_write, 4),
+                read_to_device_seconds=round(t_read, 4), read_back_identical=same)
+
+
 def ext_er(args):
     import opengraph_amd as og
     E = args.edge_factor << args.scale
@@ -396,6 +427,7 @@ def main():
     if not args.no_extensions:
         for name, fn in (("variants", lambda: ext_variants(g, threads, csr, csr is not None)),
                          ("relabel_dbg", lambda: ext_relabel(g, threads, csr)),
+                         ("csr_file", ext_csr_file),
                          ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),
                          ("uneven_batch", ext_multi), ("learned_warm_start", ext_learned)):
             t0 = time.perf_counter()
