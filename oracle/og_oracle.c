@@ -724,3 +724,78 @@ int orc_reorder_labels(const uint32_t *src, const uint32_t *dst, uint32_t num_ed
     free(key);
     return 0;
 }
+
+int orc_spmv(const orc_csr *g, uint32_t code, uint32_t iterations, float *out)
+{
+    /* SPMV.c:616-1068, unweighted (every weight 0.0001f), run sequentially
+     * (the push variants in source order) */
+    const uint32_t N = g->num_vertices;
+    float *x = orc_alloc((size_t)N * 4);
+    float *y = orc_alloc((size_t)N * 4);
+    uint64_t *acc = orc_alloc((size_t)N * 8), *xq = orc_alloc((size_t)N * 8);
+    if (!x || !y || !acc || !xq) { free(x); free(y); free(acc); free(xq); return -1; }
+    /* SPMV.c:663-669: x = 1/out-degree, 0.001 without out-edges */
+    for (uint32_t v = 0; v < N; v++)
+    {
+        x[v] = g->f_out_degree[v] ? 1.0f / (float)g->f_out_degree[v] : 0.001f;
+        y[v] = 0.0f;
+        acc[v] = 0;
+    }
+    const float w = 0.0001f;
+    if (code == 0)
+    {
+        /* SPMV.c:673-693: y[v] += w * x[src] over the inverse row, every iteration */
+        for (uint32_t it = 0; it < iterations; it++)
+            for (uint32_t v = 0; v < N; v++)
+                for (uint32_t j = g->i_edges_idx[v]; j < g->i_edges_idx[v] + g->i_out_degree[v]; j++)
+                {
+                    const float t = w * x[g->i_dst[j] & ORC_ID_MASK];
+                    y[v] = y[v] + t;
+                }
+    }
+    else if (code == 1)
+    {
+        /* SPMV.c:763-783: y[dst] += w * x[v] over the forward row */
+        for (uint32_t it = 0; it < iterations; it++)
+            for (uint32_t v = 0; v < N; v++)
+                for (uint32_t j = g->f_edges_idx[v]; j < g->f_edges_idx[v] + g->f_out_degree[v]; j++)
+                {
+                    const uint32_t d = g->f_dst[j] & ORC_ID_MASK;
+                    const float t = w * x[v];
+                    y[d] = y[d] + t;
+                }
+    }
+    else if (code == 2)
+    {
+        /* SPMV.c:857-908: Q16.16 inputs, the weight FloatToFixed32(0.0001f) = 7
+         * held in a float, u64 sums of (7 * x) >> 16, Fixed32ToFloat */
+        const float wq = (float)(uint32_t)og_x86_f2i64(w * 65536.0f + 0.5f);
+        for (uint32_t v = 0; v < N; v++) xq[v] = (uint32_t)og_x86_f2i64(x[v] * 65536.0f + 0.5f);
+        const uint64_t wu = (uint64_t)og_x86_f2i64(wq);
+        for (uint32_t it = 0; it < iterations; it++)
+            for (uint32_t v = 0; v < N; v++)
+                for (uint32_t j = g->i_edges_idx[v]; j < g->i_edges_idx[v] + g->i_out_degree[v]; j++)
+                    acc[v] += (wu * xq[g->i_dst[j] & ORC_ID_MASK]) >> 16;
+        for (uint32_t v = 0; v < N; v++) y[v] = (float)acc[v] / 65536.0f;
+    }
+    else if (code == 3)
+    {
+        /* SPMV.c:995-1034: Q32.32 inputs and weight DoubleToFixed64(0.0001f),
+         * u64 sums of the 128-bit product >> 32, Fixed64ToDouble */
+        const uint64_t wu = og_x86_d2u64((double)w * 4294967296.0 + 0.5);
+        for (uint32_t v = 0; v < N; v++) xq[v] = og_x86_d2u64((double)x[v] * 4294967296.0 + 0.5);
+        for (uint32_t it = 0; it < iterations; it++)
+            for (uint32_t v = 0; v < N; v++)
+                for (uint32_t j = g->f_edges_idx[v]; j < g->f_edges_idx[v] + g->f_out_degree[v]; j++)
+                    acc[g->f_dst[j] & ORC_ID_MASK] += (uint64_t)(((unsigned __int128)wu * xq[v]) >> 32);
+        for (uint32_t v = 0; v < N; v++) y[v] = (float)((double)acc[v] / 4294967296.0);
+    }
+    else
+    {
+        free(x); free(y); free(acc); free(xq);
+        return -1;
+    }
+    memcpy(out, y, (size_t)N * 4);
+    free(x); free(y); free(acc); free(xq);
+    return 0;
+}

@@ -132,6 +132,11 @@ void orc_sort_labels_by_rank(const float *ranks, uint32_t *labels, uint32_t n);
 int orc_reorder_labels(const uint32_t *src, const uint32_t *dst, uint32_t num_edges, uint32_t N,
                        uint32_t lmode, uint32_t *labels);
 
+/* SPMV.c:616-1068: the reference's benchmark SpMV on CSR for `iterations`
+ * iterations, -p code 0 pull, 1 push, 2 fixed-point pull, 3 fixed-point
+ * push; out = vector_output.  Returns 0, or -1 for another code. */
+int orc_spmv(const orc_csr *g, uint32_t code, uint32_t iterations, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,8 @@ def lib():
         L.orc_stable_sort_pairs.restype = C.c_int
         L.orc_reorder_labels.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
         L.orc_reorder_labels.restype = C.c_int
+        L.orc_spmv.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_uint32, f32p]
+        L.orc_spmv.restype = C.c_int
         _lib = L
     return _lib
 
@@ -95,6 +97,9 @@ def ref_lib():
         R.ogref_reorder.restype = C.c_int
         R.ogref_write_csr.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [C.c_char_p]
         R.ogref_write_csr.restype = C.c_int
+        R.ogref_spmv.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
+            C.c_uint32, C.c_uint32, C.c_int, f32p, f64p]
+        R.ogref_spmv.restype = C.c_int
         _ref = R
     return _ref
 
@@ -231,6 +236,15 @@ def reorder_labels(src, dst, num_vertices, lmode):
     return lab
 
 
+def spmv(csr: CSR, code=0, iterations=20):
+    """vector_output of the reference's SPMVGraphCSR (-p code 0..3) for
+    `iterations` iterations, run sequentially."""
+    out = np.zeros(csr.num_vertices, np.float32)
+    if lib().orc_spmv(C.byref(csr.s), code, iterations, _p(out, f32p)):
+        raise ValueError(f"SpMV code {code} is not 0..3")
+    return out
+
+
 def ref_pagerank(csr: CSR, code=0, epsilon=1e-4, iterations=20, threads=1):
     """Run the reference's own compiled pageRankGraphCSR on these arrays."""
     R = ref_lib()
@@ -267,6 +281,20 @@ def ref_reorder(src, dst, num_vertices, lmode, threads=1):
     if R.ogref_reorder(num_vertices, len(s), _p(s), _p(d), lmode, threads, _p(lab), C.byref(sec)):
         raise RuntimeError("ogref_reorder failed")
     return lab, sec.value
+
+
+def ref_spmv(csr: CSR, code=0, iterations=20, threads=1):
+    """Run the reference's own compiled SPMVGraphCSR (-p code) on these
+    arrays; returns (vector_output, time_total seconds)."""
+    R = ref_lib()
+    out = np.zeros(csr.num_vertices, np.float32)
+    sec = C.c_double()
+    arrays = [csr.a[k] for k in ["f_out_degree", "f_in_degree", "f_edges_idx", "f_src", "f_dst",
+                                  "i_out_degree", "i_in_degree", "i_edges_idx", "i_src", "i_dst"]]
+    if R.ogref_spmv(csr.num_vertices, csr.num_edges, *[_p(x) for x in arrays], code, iterations,
+                    threads, _p(out, f32p), C.byref(sec)):
+        raise RuntimeError("ogref_spmv failed")
+    return out, sec.value
 
 
 def read_bin(path):

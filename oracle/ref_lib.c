@@ -23,6 +23,7 @@
 #include "myMalloc.h"
 #include "pageRank.h"
 #include "reorder.h"
+#include "SPMV.h"
 
 int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
                    uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst,
@@ -75,6 +76,51 @@ int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_i
     *error_out = s->error_total;
     *seconds_out = s->time_total;
     freePageRankStats(s);
+    return 0;
+}
+
+/* The reference's own SPMVGraphCSR (SPMV.c:588-1068) with -p pushpull for
+ * `iterations` iterations: out gets vector_output, seconds_out time_total. */
+int ogref_spmv(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
+               uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst, uint32_t *i_out_degree,
+               uint32_t *i_in_degree, uint32_t *i_edges_idx, uint32_t *i_src, uint32_t *i_dst,
+               uint32_t pushpull, uint32_t iterations, int threads, float *out, double *seconds_out)
+{
+    struct Vertex fv = {f_out_degree, f_in_degree, f_edges_idx};
+    struct Vertex iv = {i_out_degree, i_in_degree, i_edges_idx};
+    struct EdgeList fe, ie;
+    memset(&fe, 0, sizeof(fe));
+    memset(&ie, 0, sizeof(ie));
+    fe.num_edges = ie.num_edges = E;
+    fe.num_vertices = ie.num_vertices = N;
+    fe.edges_array_src = f_src; fe.edges_array_dest = f_dst;
+    ie.edges_array_src = i_src; ie.edges_array_dest = i_dst;
+    struct GraphCSR g;
+    memset(&g, 0, sizeof(g));
+    g.num_edges = E;
+    g.num_vertices = N;
+    g.vertices = &fv;
+    g.sorted_edges_array = &fe;
+    g.inverse_vertices = &iv;
+    g.inverse_sorted_edges_array = &ie;
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.iterations = iterations;
+    a.pushpull = pushpull;
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = threads;
+    omp_set_num_threads(threads);
+    fflush(stdout);
+    int saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+    struct SPMVStats *s = SPMVGraphCSR(&a, &g);
+    fflush(stdout);
+    if (saved >= 0) { dup2(saved, 1); close(saved); }
+    if (devnull >= 0) close(devnull);
+    if (!s) return -1;
+    memcpy(out, s->vector_output, (size_t)N * sizeof(float));
+    *seconds_out = s->time_total;
+    freeSPMVStats(s);
     return 0;
 }
 
