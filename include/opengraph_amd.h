@@ -164,6 +164,39 @@ struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph);
 /* pageRank.c:167-177 */
 void freePageRankStats(struct PageRankStats *stats);
 
+/* SPMV.h:19-28 */
+struct SPMVStats
+{
+    uint32_t iterations;
+    struct Arguments *arguments;
+    uint32_t num_vertices;
+    float *vector_output;
+    float *vector_input;
+    double time_total;
+};
+
+/* ------------------------------------------------------------------------
+ * SpMV on CSR (the reference's SpMV benchmark), on the GPU: y = 0, then
+ * arguments->iterations passes y += sum over the in-edges of 0.0001f * x,
+ * x[v] = 1.0f / out-degree(v) (0.001f without out-edges), as the reference.
+ * Same graph caching and ownership rules as the PageRank entry points.
+ * ---------------------------------------------------------------------- */
+
+/* SPMV.c:588-614 (decl SPMV.h:51): dispatch on arguments->pushpull. */
+struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:616-713: pull, fp32 (-p 0). */
+struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:715-810: push, fp32 atomics (-p 1). */
+struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:812-948: pull, Q16.16 fixed point (-p 2). */
+struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:950-1068: push, Q32.32 fixed point (-p 3). */
+struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* SPMV.c:45-66: vectors zeroed, iterations 0. */
+struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);
+/* SPMV.c:139-152 */
+void freeSPMVStats(struct SPMVStats *stats);
+
 /* ------------------------------------------------------------------------
  * Edge list -> CSR builder.  Sorting and degree mapping run on the GPU; the
  * host structures produced are laid out and allocated (128-byte aligned

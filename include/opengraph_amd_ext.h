@@ -269,6 +269,44 @@ int ogpu_ppr_columns(ogpu_ppr *p, uint32_t *iters, double *l1_hist, uint32_t max
 double ogpu_ppr_bytes_per_iteration(const ogpu_ppr *p);
 void ogpu_ppr_free(ogpu_ppr *p);
 
+/* ---- SpMV (SPMV.c:588-1068, the reference's SpMV benchmark on CSR) ----
+ * y = 0, then `iterations` passes y += sum over the in-edges (src, v) of
+ * w * x[src], with x[v] = 1.0f / out-degree(v) (0.001f without out-edges)
+ * and w = 0.0001f, in the arithmetic of -p code:
+ *   0 pull, fp32: rows of at most hub_threshold in-edges continue y in the
+ *     reference's CSR order (bit-exact); longer rows add their fixed-tree
+ *     sum to y (hub_threshold 0xFFFFFFFF: every row bit-exact);
+ *   1 push, fp32 atomics over the forward edges (unordered, as the
+ *     reference's omp atomic);
+ *   2 pull, Q16.16 inputs, u64 sums of (7 * x) >> 16 (bit-exact);
+ *   3 push, Q32.32, u64 sums of (w * x) >> 32 (bit-exact): gathered over
+ *     the in-edges by default, scattered with u64 atomics when scatter = 1. */
+typedef struct ogpu_spmv ogpu_spmv;
+typedef struct
+{
+    uint32_t code;           /* the reference's -p, 0..3 */
+    uint32_t hub_threshold;  /* as ogpu_pr_options (0: default 16) */
+    uint32_t hot_cache;      /* as ogpu_pr_options */
+    uint32_t scatter;        /* code 3: 1 scatters with atomics instead of gathering */
+} ogpu_spmv_options;
+void ogpu_spmv_default_options(ogpu_spmv_options *opt);
+int ogpu_spmv_create(ogpu_graph *g, const ogpu_spmv_options *opt, ogpu_spmv **out);
+/* Reset y, run `iterations` passes, map vector_output to the caller's ids
+ * (kept on device); *seconds = device time of all of it. */
+int ogpu_spmv_run(ogpu_spmv *s, uint32_t iterations, double *seconds);
+/* vector_output of the last run, N floats, caller's vertex order. */
+int ogpu_spmv_output(ogpu_spmv *s, float *host_out);
+/* Device time of iteration i of the last run (iteration 0 includes the reset). */
+int ogpu_spmv_iteration_seconds(ogpu_spmv *s, uint32_t i, double *seconds);
+/* Benchmark helpers as ogpu_pr_reset / _enqueue / _kernel_ms: the timed
+ * kernel is the pull gather or the push scatter. */
+int ogpu_spmv_reset(ogpu_spmv *s);
+int ogpu_spmv_enqueue(ogpu_spmv *s, uint32_t n, int time_kernels);
+int ogpu_spmv_kernel_ms(ogpu_spmv *s, double *total_ms, uint32_t *count);
+/* Algorithmic HBM bytes of one pass (pull fp32: 8E + 12N). */
+double ogpu_spmv_bytes_per_iteration(const ogpu_spmv *s);
+void ogpu_spmv_free(ogpu_spmv *s);
+
 #ifdef __cplusplus
 }
 #endif

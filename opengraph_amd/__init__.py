@@ -84,6 +84,20 @@ class PageRankStats(C.Structure):
     ]
 
 
+class SPMVStats(C.Structure):
+    """SPMV.h:19-28."""
+    _fields_ = [
+        ("iterations", u32), ("arguments", C.c_void_p), ("num_vertices", u32),
+        ("vector_output", C.POINTER(C.c_float)), ("vector_input", C.POINTER(C.c_float)),
+        ("time_total", C.c_double),
+    ]
+
+
+class SpmvOptions(C.Structure):
+    """ogpu_spmv_options (opengraph_amd_ext.h)."""
+    _fields_ = [("code", u32), ("hub_threshold", u32), ("hot_cache", u32), ("scatter", u32)]
+
+
 class PrOptions(C.Structure):
     """ogpu_pr_options (opengraph_amd_ext.h)."""
     _fields_ = [
@@ -114,6 +128,7 @@ class PprResult(C.Structure):
 # name -> (restype, argtypes).
 _VP = C.c_void_p
 _PR_FN = (C.POINTER(PageRankStats), [C.POINTER(Arguments), C.POINTER(GraphCSR)])
+_SPMV_FN = (C.POINTER(SPMVStats), [C.POINTER(Arguments), C.POINTER(GraphCSR)])
 SIGNATURES = {
     # opengraph_amd.h
     "pageRankGraphCSR": _PR_FN, "pageRankPullGraphCSR": _PR_FN, "pageRankPushGraphCSR": _PR_FN,
@@ -125,6 +140,10 @@ SIGNATURES = {
     "pageRankDataDrivenPushGraphCSR": _PR_FN, "pageRankDataDrivenPullPushGraphCSR": _PR_FN,
     "newPageRankStatsGraphCSR": (C.POINTER(PageRankStats), [C.POINTER(GraphCSR)]),
     "freePageRankStats": (None, [C.POINTER(PageRankStats)]),
+    "SPMVGraphCSR": _SPMV_FN, "SPMVPullGraphCSR": _SPMV_FN, "SPMVPushGraphCSR": _SPMV_FN,
+    "SPMVPullFixedPointGraphCSR": _SPMV_FN, "SPMVPushFixedPointGraphCSR": _SPMV_FN,
+    "newSPMVStatsGraphCSR": (C.POINTER(SPMVStats), [C.POINTER(GraphCSR)]),
+    "freeSPMVStats": (None, [C.POINTER(SPMVStats)]),
     "graphCSRPreProcessingStep": (C.POINTER(GraphCSR), [C.POINTER(Arguments)]),
     "graphCSRNew": (C.POINTER(GraphCSR), [u32, u32, C.c_uint8]),
     "graphCSRAssignEdgeList": (C.POINTER(GraphCSR), [C.POINTER(GraphCSR), C.POINTER(EdgeList), C.c_uint8]),
@@ -210,6 +229,16 @@ SIGNATURES = {
     "ogpu_ppr_columns": (C.c_int, [_VP, u32p, C.POINTER(C.c_double), u32]),
     "ogpu_ppr_bytes_per_iteration": (C.c_double, [_VP]),
     "ogpu_ppr_free": (None, [_VP]),
+    "ogpu_spmv_default_options": (None, [C.POINTER(SpmvOptions)]),
+    "ogpu_spmv_create": (C.c_int, [_VP, C.POINTER(SpmvOptions), C.POINTER(_VP)]),
+    "ogpu_spmv_run": (C.c_int, [_VP, u32, C.POINTER(C.c_double)]),
+    "ogpu_spmv_output": (C.c_int, [_VP, _VP]),
+    "ogpu_spmv_iteration_seconds": (C.c_int, [_VP, u32, C.POINTER(C.c_double)]),
+    "ogpu_spmv_reset": (C.c_int, [_VP]),
+    "ogpu_spmv_enqueue": (C.c_int, [_VP, u32, C.c_int]),
+    "ogpu_spmv_kernel_ms": (C.c_int, [_VP, C.POINTER(C.c_double), u32p]),
+    "ogpu_spmv_bytes_per_iteration": (C.c_double, [_VP]),
+    "ogpu_spmv_free": (None, [_VP]),
 }
 
 GRAPH_ARRAYS = ["f_src", "f_dst", "f_out_degree", "f_in_degree", "f_edges_idx",
@@ -658,6 +687,54 @@ class Ppr:
             pass
 
 
+class Spmv:
+    """The reference's SpMV benchmark (-p code 0..3) on a device graph (ogpu_spmv)."""
+
+    def __init__(self, graph: Graph, code=0, hub_threshold=0, hot_cache=0, scatter=0):
+        o = SpmvOptions()
+        load().ogpu_spmv_default_options(C.byref(o))
+        o.code, o.hub_threshold, o.hot_cache, o.scatter = code, hub_threshold, hot_cache, scatter
+        self.graph = graph
+        self.h = C.c_void_p()
+        check(load().ogpu_spmv_create(graph.h, C.byref(o), C.byref(self.h)))
+
+    def run(self, iterations=20) -> float:
+        """Device seconds of the whole run."""
+        s = C.c_double()
+        check(load().ogpu_spmv_run(self.h, iterations, C.byref(s)))
+        return s.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty(self.graph.num_vertices, np.float32)
+        check(load().ogpu_spmv_output(self.h, _ptr(out)))
+        return out
+
+    def reset(self):
+        check(load().ogpu_spmv_reset(self.h))
+
+    def enqueue(self, n, time_kernels=False):
+        check(load().ogpu_spmv_enqueue(self.h, n, int(time_kernels)))
+
+    def kernel_ms(self):
+        t, c = C.c_double(), u32()
+        check(load().ogpu_spmv_kernel_ms(self.h, C.byref(t), C.byref(c)))
+        return t.value, c.value
+
+    def bytes_per_iteration(self):
+        return load().ogpu_spmv_bytes_per_iteration(self.h)
+
+    def free(self):
+        if self.h:
+            load().ogpu_spmv_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def synchronize():
     check(load().ogpu_synchronize())
 
@@ -714,4 +791,20 @@ def pagerank_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20, epsilon=1
     ranks = np.ctypeslib.as_array(s.pageRanks, shape=(n,)).copy() if n else np.zeros(0, np.float32)
     out = (ranks, s.iterations, s.error_total, s.time_total)
     lib.freePageRankStats(st)
+    return out
+
+
+def spmv_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20):
+    """Call the drop-in SPMVGraphCSR; returns (vector_output, vector_input, iterations, time)."""
+    lib = load()
+    args = default_arguments(pushpull=pushpull, iterations=iterations)
+    st = lib.SPMVGraphCSR(C.byref(args), C.byref(graph.g))
+    if not st:
+        raise OgpuError("SPMVGraphCSR returned NULL")
+    s = st.contents
+    n = s.num_vertices
+    y = np.ctypeslib.as_array(s.vector_output, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    x = np.ctypeslib.as_array(s.vector_input, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    out = (y, x, s.iterations, s.time_total)
+    lib.freeSPMVStats(st)
     return out

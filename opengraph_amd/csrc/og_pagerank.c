@@ -17,6 +17,7 @@
 
 #include "og_internal.h"
 #include "og_pr.h"
+#include "og_pr_run.h"
 #include "opengraph_amd_ext.h"
 
 #define OG_CHECK_EVERY 8u
@@ -40,50 +41,6 @@ static uint32_t og_final_blocks(uint32_t n_parts, uint32_t rows)
 }
 
 static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res);
-
-struct ogpu_pr
-{
-    og_graph *g;
-    og_plan *plan;
-    int own_plan;
-    ogpu_pr_options opt;
-    int mode, push, quant, dd;
-    og_pr_dev dev;
-    uint32_t gv;              /* grid of the per-vertex kernels */
-    uint32_t n_final_pull, n_final_vert;
-    float *r;
-    void *c[2];
-    void *cq;
-    void *next;
-    void *chunk_sum;
-    float *init_d;
-    float *out_old;
-    double *d_sum;
-    double *part_err, *fin_err;
-    uint32_t *part_act, *fin_act;
-    float *part_min, *part_max, *fin_min, *fin_max;
-    uint32_t *ticket, *done;
-    og_iter_stat *stats;
-    uint32_t stats_cap;
-    og_qparams *qp;
-    uint32_t *done_host;
-    uint32_t it;
-    /* per-iteration end events of a run, kernel timing pairs of enqueue */
-    hipEvent_t *iter_ev;
-    uint32_t n_iter_ev;
-    hipEvent_t *kev;
-    uint32_t n_kev, kev_used;
-    double seconds_plan;
-    uint32_t rows;
-    og_iter_stat *rows_host;
-    /* data-driven state: two byte worklists, changed flags, residuals */
-    uint8_t *wl[2];
-    uint8_t *changed;
-    float *resid;
-    /* learned start: features (N x 4) and reduction scratch */
-    float *feat;
-    double *lscratch;
-};
 
 static double og_now(void)
 {
@@ -187,7 +144,7 @@ void og_pr_use_init(ogpu_pr *p)
 }
 
 /* Create a run on graph g.  plan may be supplied (borrowed) or built here. */
-static int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, ogpu_pr **out)
+int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, ogpu_pr **out)
 {
     *out = NULL;
     if (og_ctx_ensure()) return -1;
@@ -500,7 +457,7 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     return 0;
 }
 
-static int og_pr_events(hipEvent_t **arr, uint32_t *have, uint32_t need)
+int og_pr_events(hipEvent_t **arr, uint32_t *have, uint32_t need)
 {
     if (*have >= need) return 0;
     hipEvent_t *n = (hipEvent_t *)realloc(*arr, need * sizeof(hipEvent_t));
@@ -802,7 +759,7 @@ static void og_fatal(const char *where)
     exit(EXIT_FAILURE);
 }
 
-static int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_plan **plan)
+int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_plan **plan)
 {
     const void *key[12] = {graph,
                            graph->vertices ? graph->vertices->out_degree : NULL,

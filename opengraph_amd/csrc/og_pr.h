@@ -54,6 +54,7 @@ typedef struct og_pr_dev
     uint32_t raw;            /* fp32 pull writes the plain in-neighbour sum to r (features) */
     uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
     uint32_t ppr_src;        /* new id of the restart vertex */
+    uint32_t spmv;           /* pull adds the row sums onto y (SpMV): fp32 y in r, u64 in next */
     double epsilon;
     /* plan */
     const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;
@@ -128,6 +129,15 @@ int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s);
 int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s);
 int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
+/* Only the scatter of a push (next[dst] += g_in[src] over the forward
+ * edges), without the epilogue: SpMV's push accumulates into next. */
+int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t s, hipEvent_t ev0,
+                           hipEvent_t ev1);
+/* SpMV (og_spmv.hip): the per-vertex terms w * x of -p code 0..3 in new ids
+ * into `terms` (fp32, u32 or u64), and vector_output in old ids from r
+ * (codes 0, 1) or the u64 accumulators in next (codes 2, 3). */
+int og_launch_spmv_terms(const og_pr_dev *a, uint32_t code, void *terms, hipStream_t s);
+int og_launch_spmv_out(const og_pr_dev *a, uint32_t code, float *out_old, hipStream_t s);
 int og_launch_quantize(const og_pr_dev *a, int mode, hipStream_t s);
 int og_launch_init(const og_pr_dev *a, int mode, const float *init_old, float init_value,
                    hipStream_t s);

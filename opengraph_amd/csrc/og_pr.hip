@@ -125,12 +125,30 @@ __device__ __forceinline__ void og_epilogue_pf(const og_pr_dev &a, uint32_t v,
                                                typename og_mode<MODE>::acc sum, float prev,
                                                uint32_t deg, og_stat &st);
 
+/* SpMV (SPMV.c:673-693, :857-908): y[v] += the row's sum of the terms
+ * w * x[src].  fp32: y lives in r; `chained` when the sum already started
+ * from y and added the terms in CSR order, as the reference's running
+ * `vector_output[dest] +=` does (bit-exact), else y + the row's sum.
+ * Integer modes: u64 accumulators in next, exact in any order. */
+template <int MODE>
+__device__ __forceinline__ void og_spmv_store(const og_pr_dev &a, uint32_t v,
+                                              typename og_mode<MODE>::acc s, float prev, bool chained)
+{
+    if constexpr (MODE == OG_MODE_F32) a.r[v] = chained ? s : prev + s;
+    else ((uint64_t *)a.next)[v] += (uint64_t)s;
+}
+
 /* pageRank.c:1016-1027: update, |delta| in double, error_total += err/N,
  * activeVertices += err >= epsilon; then the next contribution. */
 template <int MODE>
 __device__ __forceinline__ void og_epilogue(const og_pr_dev &a, uint32_t v,
                                             typename og_mode<MODE>::acc sum, og_stat &st)
 {
+    if (a.spmv)
+    {
+        og_spmv_store<MODE>(a, v, sum, MODE == OG_MODE_F32 ? a.r[v] : 0.0f, false);
+        return;
+    }
     if constexpr (MODE == OG_MODE_F32)
     {
         if (a.dd)
@@ -338,6 +356,7 @@ __device__ __forceinline__ void og_chunk_body(const og_pr_dev &a, og_stat &st, u
             if (my_tag & OG_CH_MULTI) ((A *)a.chunk_sum)[c] = 0;
         }
         else if (my_tag & OG_CH_MULTI) ((A *)a.chunk_sum)[c] = mine;
+        else if (a.spmv) og_spmv_store<MODE>(a, my_row, mine, prev, false);
         else if (a.dd) og_epilogue<MODE>(a, my_row, mine, st);
         else og_epilogue_pf<MODE>(a, my_row, mine, prev, deg, st);
     }
@@ -383,19 +402,20 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
     else og_chunk_body<MODE, HOT, OG_CH_SLOTS>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
 }
 
-/* In-edges [j, e) of one row added in CSR order, B loaded and gathered at a
- * time.  Lanes past their row's end re-read its last edge and add zero:
- * x + 0 == x for every partial sum, which starts at +0. */
+/* In-edges [j, e) of one row added in CSR order onto s, B loaded and
+ * gathered at a time.  Lanes past their row's end re-read its last edge and
+ * add zero: x + 0 == x for every partial sum (PageRank's start at +0; the
+ * SpMV's running y is never -0, its terms being >= 0). */
 template <int MODE, bool HOT, uint32_t B>
 __device__ __forceinline__ typename og_mode<MODE>::acc og_row_sum(const og_pr_dev &a, uint32_t j,
                                                                   uint32_t e,
                                                                   const typename og_mode<MODE>::elem *hot,
-                                                                  uint32_t nh)
+                                                                  uint32_t nh,
+                                                                  typename og_mode<MODE>::acc s)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
     const E *__restrict__ g = (const E *)a.g_in;
-    A s = 0;
     for (; j < e; j += B)
     {
         uint32_t idx[B];
@@ -433,7 +453,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
         j = __builtin_nontemporal_load(&a.row_ptr[v]);
         e = __builtin_nontemporal_load(&a.row_ptr[v + 1]);
         prev = __builtin_nontemporal_load(&a.r[v]);
-        deg = __builtin_nontemporal_load(&a.outdeg[v]);
+        if (!a.spmv) deg = __builtin_nontemporal_load(&a.outdeg[v]);
     }
     /* longest row of the wavefront (every lane still active here) */
     uint32_t m = OG_ROW_BATCH;
@@ -443,12 +463,16 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
         for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
     }
     if (!mine) return;
-    A s;
-    if (m <= 1) s = og_row_sum<MODE, HOT, 1>(a, j, e, hot, nh);
-    else if (m <= 2) s = og_row_sum<MODE, HOT, 2>(a, j, e, hot, nh);
-    else if (m <= 4) s = og_row_sum<MODE, HOT, 4>(a, j, e, hot, nh);
-    else s = og_row_sum<MODE, HOT, OG_ROW_BATCH>(a, j, e, hot, nh);
-    if (a.dd) og_epilogue<MODE>(a, v, s, st);
+    /* SpMV continues the running fp32 y of the row, in the reference's order */
+    A s = 0;
+    if constexpr (MODE == OG_MODE_F32)
+        if (a.spmv) s = prev;
+    if (m <= 1) s = og_row_sum<MODE, HOT, 1>(a, j, e, hot, nh, s);
+    else if (m <= 2) s = og_row_sum<MODE, HOT, 2>(a, j, e, hot, nh, s);
+    else if (m <= 4) s = og_row_sum<MODE, HOT, 4>(a, j, e, hot, nh, s);
+    else s = og_row_sum<MODE, HOT, OG_ROW_BATCH>(a, j, e, hot, nh, s);
+    if (a.spmv) og_spmv_store<MODE>(a, v, s, prev, true);
+    else if (a.dd) og_epilogue<MODE>(a, v, s, st);
     else og_epilogue_pf<MODE>(a, v, s, prev, deg, st);
 }
 
@@ -924,6 +948,38 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
     }
     hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
     return og_hip_check(hipGetLastError(), "og_push");
+}
+
+extern "C" int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t s, hipEvent_t ev0,
+                                      hipEvent_t ev1)
+{
+    unsigned gs = (a->E + OG_BLOCK - 1) / OG_BLOCK;
+    if (gs > 16384) gs = 16384;
+    if (gs == 0) gs = 1;
+    if (ev0) hipEventRecord(ev0, s);
+    if (pmode == OG_MODE_F32)
+    {
+        if (a->hot_grid)
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
+                               dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
+        else
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+    }
+    else if (pmode == OG_MODE_FX64)
+    {
+        if (a->hot_grid)
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
+                               dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
+        else
+            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+    }
+    else
+    {
+        og_set_error("og_launch_push_scatter: bad mode %d", pmode);
+        return -1;
+    }
+    if (ev1) hipEventRecord(ev1, s);
+    return og_hip_check(hipGetLastError(), "og_push_scatter_kernel");
 }
 
 extern "C" int og_launch_init(const og_pr_dev *a, int mode, const float *init_old, float init_value,

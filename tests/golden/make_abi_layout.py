@@ -1,5 +1,6 @@
 """Write tests/golden/abi_layout.json: sizeof / offsetof of the reference's
-boundary structs (Arguments, Vertex, EdgeList, GraphCSR, PageRankStats) as
+boundary structs (Arguments, Vertex, EdgeList, GraphCSR, PageRankStats,
+SPMVStats) as
 the reference's own headers lay them out on x86-64, by compiling a probe
 against /root/reference's include tree.  The ABI test compares
 include/opengraph_amd.h and the ctypes mirror against this fixture.
@@ -26,6 +27,8 @@ FIELDS = {
                  "inverse_vertices", "inverse_sorted_edges_array"],
     "PageRankStats": ["damp", "base_pr", "iterations", "num_vertices", "activeVertices",
                       "realRanks", "pageRanks", "time_total", "error_total"],
+    "SPMVStats": ["iterations", "arguments", "num_vertices", "vector_output", "vector_input",
+                  "time_total"],
 }
 
 
@@ -56,5 +59,5 @@ def layout(includes, incdirs):
 
 if __name__ == "__main__":
     incs = [os.path.join(REF, d) for d in ("structures", "algorithms/openmp", "preprocess", "utils", "config")]
-    json.dump(layout(["graphConfig.h", "graphCSR.h", "pageRank.h"], incs), open(OUT, "w"), indent=1)
+    json.dump(layout(["graphConfig.h", "graphCSR.h", "pageRank.h", "SPMV.h"], incs), open(OUT, "w"), indent=1)
     print(open(OUT).read()[:300])

@@ -38,7 +38,9 @@ def test_headers_declare_the_reference_entry_points():
     for fn in ["pageRankGraphCSR", "pageRankPullGraphCSR", "pageRankPushGraphCSR",
                "pageRankPullFixedPoint64BitGraphCSR", "pageRankPushFixedPointGraphCSR",
                "newPageRankStatsGraphCSR", "freePageRankStats", "graphCSRPreProcessingStep",
-               "sortRunAlgorithms", "mapVerticesWithInOutDegree", "readEdgeListsbin"]:
+               "sortRunAlgorithms", "mapVerticesWithInOutDegree", "readEdgeListsbin", "SPMVGraphCSR",
+               "SPMVPullGraphCSR", "SPMVPushGraphCSR", "SPMVPullFixedPointGraphCSR",
+               "SPMVPushFixedPointGraphCSR", "newSPMVStatsGraphCSR", "freeSPMVStats"]:
         assert fn in names
 
 
@@ -69,7 +71,7 @@ def test_header_struct_layout_matches_reference(ref_layout):
 
 def test_ctypes_struct_layout_matches_reference(ref_layout):
     types = {"Arguments": og.Arguments, "Vertex": og.Vertex, "EdgeList": og.EdgeList,
-             "GraphCSR": og.GraphCSR, "PageRankStats": og.PageRankStats}
+             "GraphCSR": og.GraphCSR, "PageRankStats": og.PageRankStats, "SPMVStats": og.SPMVStats}
     for name, t in types.items():
         want = ref_layout[name]
         assert C.sizeof(t) == want["size"], name
