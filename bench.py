@@ -20,7 +20,9 @@ sources into oracle/_ref) on the same CSR on this host, over a bounded number
 of iterations.
 
 "extensions" holds the other measured forms, each also on one GPU: push and
-fixed-point variants with their host runs, the Erdos-Renyi graph of the same
+fixed-point variants with their host runs, the reference's SpMV benchmark
+(-p 0..3) on the same graph against its host run, the DBG relabelling and the
+.csr file write / load, the Erdos-Renyi graph of the same
 size, batched personalized PageRank (R-MAT 22, k = 64, against 64
 single-source runs), 256 uneven graphs in one hipGraph launch (against
 per-graph launches) and the learned warm start (time to L1 < 1e-6 against the
@@ -97,6 +99,75 @@ def ext_variants(g, threads, csr, do_cpu):
             d.update(cpu_seconds_per_iteration=round(s, 4), cpu_kind=kind, speedup_vs_cpu=round(s / (ms / 1e3), 1))
         out[name] = d
         log(f"  {name}: {d}")
+    return out
+
+
+def spmv_iter_ms(g, code, steps=10, warmup=2, **kw):
+    import opengraph_amd as og
+    s = og.Spmv(g, code, **kw)
+    s.reset()
+    s.enqueue(warmup)
+    og.synchronize()
+    s.kernel_ms()
+    t0 = time.perf_counter()
+    s.enqueue(steps, time_kernels=True)
+    og.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    kms, kc = s.kernel_ms()
+    bpi = s.bytes_per_iteration()
+    s.free()
+    return t * 1e3, kms / max(kc, 1), bpi
+
+
+def ext_spmv(g, threads, csr):
+    """The reference's SpMV benchmark (SPMVGraphCSR -p 0..3) on the headline
+    graph: ms per pass and algorithmic GB/s of each form, the reference's own
+    compiled SpMV on the host cores over 2 passes, and the GPU's 2-pass
+    vector_output compared with the reference's (fixed point: bit-exact;
+    fp32 pull: rows of at most 16 in-edges bit-exact, the rest as the
+    largest relative difference)."""
+    import opengraph_amd as og
+    out = {}
+    forms = ((0, "pull_fp32", {}, 10), (0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}, 3),
+             (1, "push_fp32", {}, 5), (2, "pull_fixed32", {}, 10), (3, "push_fixed64_gathered", {}, 10),
+             (3, "push_fixed64_scattered", {"scatter": 1}, 5))
+    for code, name, kw, steps in forms:
+        ms, kms, bpi = spmv_iter_ms(g, code, steps=steps, **kw)
+        out[name] = dict(ms_per_pass=round(ms, 4), kernel_ms=round(kms, 4), gteps=round(g.num_edges / ms / 1e6, 2),
+                         algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
+        log(f"  spmv {name}: {out[name]}")
+    if csr is not None:
+        po, c = csr
+        if po.ref_available():
+            indeg = g.array("i_out_degree")
+            short = indeg <= 16
+            for code, name in ((0, "pull_fp32"), (2, "pull_fixed32")):
+                ref, sec = po.ref_spmv(c, code, 2, threads)
+                s = og.Spmv(g, code)
+                s.run(2)
+                y = s.output()
+                s.free()
+                d = out[name]
+                d.update(cpu_reference_seconds_per_pass=round(sec / 2, 4), cpu_threads=threads,
+                         speedup_vs_cpu=round(sec / 2 / (d["ms_per_pass"] / 1e3), 1))
+                if code == 2:
+                    d["two_pass_output_bit_exact_vs_reference"] = bool(np.array_equal(y.view(np.uint32),
+                                                                                      ref.view(np.uint32)))
+                else:
+                    rel = np.abs(y.astype(np.float64) - ref) / np.maximum(np.abs(ref.astype(np.float64)), 1e-30)
+                    d["two_pass_short_rows_bit_exact_vs_reference"] = bool(
+                        np.array_equal(y[short].view(np.uint32), ref[short].view(np.uint32)))
+                    d["two_pass_max_rel_diff_long_rows"] = float(rel[~short].max()) if (~short).any() else 0.0
+                    # both against the exact (fp64) two-pass value: the difference is
+                    # the reference's own fp32 running-sum drift on the long rows
+                    od = c["f_out_degree"].astype(np.float64)
+                    x = np.where(od > 0, (1.0 / np.maximum(od, 1)).astype(np.float32), np.float32(0.001))
+                    exact = 2 * 0.0001 * np.bincount(c["f_dst"], weights=x.astype(np.float64)[c["f_src"]],
+                                                     minlength=g.num_vertices)
+                    den = np.maximum(exact, 1e-30)
+                    d["two_pass_max_rel_err_vs_fp64_gpu"] = float((np.abs(y - exact) / den).max())
+                    d["two_pass_max_rel_err_vs_fp64_reference"] = float((np.abs(ref - exact) / den).max())
+                log(f"  spmv {name} vs reference: {d}")
     return out
 
 
@@ -426,6 +497,7 @@ def main():
     extensions = {}
     if not args.no_extensions:
         for name, fn in (("variants", lambda: ext_variants(g, threads, csr, csr is not None)),
+                         ("spmv", lambda: ext_spmv(g, threads, csr)),
                          ("relabel_dbg", lambda: ext_relabel(g, threads, csr)),
                          ("csr_file", ext_csr_file),
                          ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),

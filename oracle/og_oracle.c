@@ -799,3 +799,124 @@ int orc_spmv(const orc_csr *g, uint32_t code, uint32_t iterations, float *out)
     free(x); free(y); free(acc); free(xq);
     return 0;
 }
+
+/* graphStats.c:37-50: the smallest of three, except that a tie between the
+ * first two below the third yields the third (as the reference's min3). */
+static int orc_min3_ref(int a, int b, int c)
+{
+    if (a < b && a < c) return a;
+    if (b < a && b < c) return b;
+    return c;
+}
+
+/* graphStats.c:52-186: the score of the edit-distance matrix of a1 against
+ * a2 (deletion, insertion, substitution cost 1), two rows at a time. */
+static uint32_t orc_levenshtein(const uint32_t *a1, uint32_t n1, const uint32_t *a2, uint32_t n2)
+{
+    if (n1 == 0) return n2;
+    if (n2 == 0) return n1;
+    /* column j of the reference's mat[i][j] kept for all i: prev = j - 1 */
+    uint32_t *prev = orc_alloc((size_t)(n1 + 1) * 4), *cur = orc_alloc((size_t)(n1 + 1) * 4);
+    if (!prev || !cur) { free(prev); free(cur); return 0; }
+    for (uint32_t i = 0; i <= n1; i++) prev[i] = i;
+    for (uint32_t j = 1; j <= n2; j++)
+    {
+        cur[0] = j;
+        for (uint32_t i = 1; i <= n1; i++)
+        {
+            const uint32_t cost = a1[i - 1] == a2[j - 1] ? 0u : 1u;
+            const uint32_t del = prev[i] + 1, ins = cur[i - 1] + 1, sub = prev[i - 1] + cost;
+            cur[i] = (uint32_t)orc_min3_ref((int)del, (int)ins, (int)sub);
+        }
+        uint32_t *t = prev; prev = cur; cur = t;
+    }
+    const uint32_t d = prev[n1];
+    free(prev);
+    free(cur);
+    return d;
+}
+
+/* graphStats.c:427-489: Kendall tau-b by pair counting. */
+static float orc_kendall_small(const float *arr1, const float *arr2, int len)
+{
+    int m1 = 0, m2 = 0, s = 0;
+    for (int i = 0; i < len; i++)
+        for (int j = i + 1; j < len; j++)
+        {
+            if (arr2[i] > arr2[j])
+            {
+                if (arr1[i] > arr1[j]) s++;
+                else if (arr1[i] < arr1[j]) s--;
+                else m1++;
+            }
+            else if (arr2[i] < arr2[j])
+            {
+                if (arr1[i] > arr1[j]) s--;
+                else if (arr1[i] < arr1[j]) s++;
+                else m1++;
+            }
+            else
+            {
+                m2++;
+                if (arr1[i] == arr1[j]) m1++;
+            }
+        }
+    const int nPair = len * (len - 1) / 2;
+    if (m1 < nPair && m2 < nPair) return s / (sqrtf((float)(nPair - m1)) * sqrtf((float)(nPair - m2)));
+    return 0.0f;
+}
+
+int orc_rank_stats_topk(const float *ref_pr, const uint32_t *ref_order, const float *pr, const uint32_t *order,
+                        uint32_t N, uint32_t topk, uint32_t num_vertices, orc_corr_stats *out)
+{
+    /* graphStats.c:746-751 (inverse rankings) and :580-688 */
+    if (num_vertices > N) return -1;
+    if (topk > num_vertices) topk = num_vertices;
+    const uint32_t K = topk, M = num_vertices;
+    uint32_t *inv = orc_alloc((size_t)N * 4), *ref_inv = orc_alloc((size_t)N * 4);
+    uint32_t *ids = orc_alloc((size_t)K * 4 + 4), *ref_ids = orc_alloc((size_t)K * 4 + 4);
+    float *f = orc_alloc((size_t)K * 4 + 4), *ref_f = orc_alloc((size_t)K * 4 + 4);
+    float *re = orc_alloc((size_t)K * 4 + 4), *ref_re = orc_alloc((size_t)K * 4 + 4);
+    int rc = -1;
+    if (!inv || !ref_inv || !ids || !ref_ids || !f || !ref_f || !re || !ref_re) goto done;
+    for (uint32_t v = 0; v < N; v++)
+    {
+        inv[order[v]] = v;
+        ref_inv[ref_order[v]] = v;
+    }
+    for (uint32_t u = 0, v = M - K; v < M; v++, u++)
+    {
+        ids[u] = order[v];
+        f[u] = pr[order[v]];
+        re[u] = (float)order[v] / (float)1.0;
+        ref_ids[u] = ref_order[v];
+        ref_f[u] = ref_pr[order[v]];
+        ref_re[u] = (float)ref_order[v] / (float)1.0;
+    }
+    out->levenshtein_distance = orc_levenshtein(ref_ids, K, ids, K);
+    out->float_Kendall = orc_kendall_small(ref_f, f, (int)K);
+    out->real_Kendall = orc_kendall_small(ref_re, re, (int)K);
+    /* graphStats.c:560-578: intersection_topK(ref order, inverse, M, K) */
+    uint32_t inter = 0, mism = 0;
+    double ef = 0.0, er = 0.0;
+    for (uint32_t v = M - K; v < M; v++)
+        if (inv[ref_order[v]] >= M - K) inter++;
+    for (uint32_t v = M - K; v < M; v++)
+    {
+        const uint32_t w = ref_order[v];
+        /* graphStats.c:539-557 */
+        if ((double)ref_pr[w] > 0.0f) ef += fabs((double)pr[w] - (double)ref_pr[w]) / (double)ref_pr[w];
+        /* graphStats.c:518-536: abs() of the unsigned difference taken as int */
+        er += abs((int)(inv[w] - ref_inv[w])) / ((double)ref_inv[w] + 1);
+        /* graphStats.c:500-516 */
+        if (inv[w] != ref_inv[w]) mism++;
+    }
+    out->intersection = inter;
+    out->mismatch = mism;
+    out->avg_error_float = ef / K;
+    out->avg_error_relative = er / K;
+    rc = 0;
+done:
+    free(inv); free(ref_inv); free(ids); free(ref_ids); free(f); free(ref_f); free(re); free(ref_re);
+    return rc;
+}

@@ -137,6 +137,25 @@ int orc_reorder_labels(const uint32_t *src, const uint32_t *dst, uint32_t num_ed
  * push; out = vector_output.  Returns 0, or -1 for another code. */
 int orc_spmv(const orc_csr *g, uint32_t code, uint32_t iterations, float *out);
 
+/* graphStats.h:28-37 */
+typedef struct
+{
+    uint32_t levenshtein_distance;
+    float float_Kendall;
+    float real_Kendall;
+    uint32_t intersection;
+    uint32_t mismatch;
+    double avg_error_float;
+    double avg_error_relative;
+} orc_corr_stats;
+
+/* graphStats.c:580-688 with the inverse rankings of :746-751: the top-k
+ * comparison of a ranking (pr, order = realRanks ascending by rank) against
+ * the reference ranking (ref_pr, ref_order) over positions
+ * [num_vertices - topk, num_vertices).  Returns 0, -1 on a bad size. */
+int orc_rank_stats_topk(const float *ref_pr, const uint32_t *ref_order, const float *pr, const uint32_t *order,
+                        uint32_t N, uint32_t topk, uint32_t num_vertices, orc_corr_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

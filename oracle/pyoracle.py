@@ -39,6 +39,16 @@ class OrcResult(C.Structure):
                 ("pr_sum", C.c_double), ("seconds_total", C.c_double)]
 
 
+class OrcCorrStats(C.Structure):
+    """orc_corr_stats = graphStats.h:28-37 PageRankCorrelationStats."""
+    _fields_ = [("levenshtein_distance", C.c_uint32), ("float_Kendall", C.c_float),
+                ("real_Kendall", C.c_float), ("intersection", C.c_uint32), ("mismatch", C.c_uint32),
+                ("avg_error_float", C.c_double), ("avg_error_relative", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 _ref = None
 
@@ -77,6 +87,9 @@ def lib():
         L.orc_reorder_labels.argtypes = [u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
         L.orc_reorder_labels.restype = C.c_int
         L.orc_spmv.argtypes = [C.POINTER(OrcCSR), C.c_uint32, C.c_uint32, f32p]
+        L.orc_rank_stats_topk.argtypes = [f32p, u32p, f32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.POINTER(OrcCorrStats)]
+        L.orc_rank_stats_topk.restype = C.c_int
         L.orc_spmv.restype = C.c_int
         _lib = L
     return _lib
@@ -100,6 +113,13 @@ def ref_lib():
         R.ogref_spmv.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
             C.c_uint32, C.c_uint32, C.c_int, f32p, f64p]
         R.ogref_spmv.restype = C.c_int
+        R.ogref_rank_stats_topk.argtypes = [C.c_uint32, f32p, u32p, f32p, u32p, C.c_uint32, C.c_uint32,
+                                            u32p, f32p, f64p]
+        R.ogref_rank_stats_topk.restype = C.c_int
+        R.ogref_collect_stats.argtypes = [C.c_uint32, f32p, u32p, f32p, u32p, C.c_char_p, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,
+                                          C.c_int, C.c_char_p, f64p]
+        R.ogref_collect_stats.restype = C.c_int
         _ref = R
     return _ref
 
@@ -281,6 +301,48 @@ def ref_reorder(src, dst, num_vertices, lmode, threads=1):
     if R.ogref_reorder(num_vertices, len(s), _p(s), _p(d), lmode, threads, _p(lab), C.byref(sec)):
         raise RuntimeError("ogref_reorder failed")
     return lab, sec.value
+
+
+def rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
+    """graphStats.c collectStatsPageRank_topK restated: dict of the seven fields."""
+    rp, ro = np.ascontiguousarray(ref_pr, np.float32), np.ascontiguousarray(ref_order, np.uint32)
+    p, o = np.ascontiguousarray(pr, np.float32), np.ascontiguousarray(order, np.uint32)
+    n = len(rp)
+    out = OrcCorrStats()
+    if lib().orc_rank_stats_topk(_p(rp, f32p), _p(ro), _p(p, f32p), _p(o), n, topk,
+                                 n if num_vertices is None else num_vertices, C.byref(out)):
+        raise ValueError("bad sizes")
+    return out.as_dict()
+
+
+def ref_rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
+    """The reference's own collectStatsPageRank_topK: dict of the seven fields."""
+    rp, ro = np.ascontiguousarray(ref_pr, np.float32), np.ascontiguousarray(ref_order, np.uint32)
+    p, o = np.ascontiguousarray(pr, np.float32), np.ascontiguousarray(order, np.uint32)
+    n = len(rp)
+    u = np.zeros(3, np.uint32)
+    f = np.zeros(2, np.float32)
+    d = np.zeros(2, np.float64)
+    if ref_lib().ogref_rank_stats_topk(n, _p(rp, f32p), _p(ro), _p(p, f32p), _p(o), topk,
+                                       n if num_vertices is None else num_vertices, _p(u), _p(f, f32p),
+                                       _p(d, f64p)):
+        raise RuntimeError("ogref_rank_stats_topk failed")
+    return dict(levenshtein_distance=int(u[0]), float_Kendall=float(f[0]), real_Kendall=float(f[1]),
+                intersection=int(u[1]), mismatch=int(u[2]), avg_error_float=float(d[0]),
+                avg_error_relative=float(d[1]))
+
+
+def ref_collect_stats(ref_pr, ref_order, pr, order, fnameb, bin_size=512, verbosity=0, pushpull=0,
+                      trial=0, time_total=0.0, iterations=0, threads=1, stdout_path=None):
+    """The reference's own collectStatsPageRank: appends to
+    <fnameb>_1_0_<trial>_<pushpull>.stats; returns the call's seconds."""
+    rp, ro = np.ascontiguousarray(ref_pr, np.float32), np.ascontiguousarray(ref_order, np.uint32)
+    p, o = np.ascontiguousarray(pr, np.float32), np.ascontiguousarray(order, np.uint32)
+    sec = C.c_double()
+    ref_lib().ogref_collect_stats(len(rp), _p(rp, f32p), _p(ro), _p(p, f32p), _p(o), fnameb.encode(),
+                                  bin_size, verbosity, pushpull, trial, time_total, iterations, threads,
+                                  stdout_path.encode() if stdout_path else None, C.byref(sec))
+    return sec.value
 
 
 def ref_spmv(csr: CSR, code=0, iterations=20, threads=1):

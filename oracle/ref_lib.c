@@ -19,6 +19,7 @@
 #include "edgeList.h"
 #include "graphConfig.h"
 #include "graphCSR.h"
+#include "graphStats.h"
 #include "mt19937.h"
 #include "myMalloc.h"
 #include "pageRank.h"
@@ -191,5 +192,80 @@ int ogref_reorder(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *d
     memcpy(labels_out, el->label_array, (size_t)N * 4);
     *seconds_out = t1 - t0;
     freeEdgeList(el);
+    return 0;
+}
+
+/* The reference's own collectStatsPageRank_topK (graphStats.c:580-688) on
+ * rankings given as arrays; the inverse rankings are formed as
+ * collectStatsPageRank does (graphStats.c:746-751).  out_u32 gets
+ * levenshtein_distance, intersection, mismatch; out_f32 the float and real
+ * Kendall; out_f64 avg_error_float, avg_error_relative. */
+int ogref_rank_stats_topk(uint32_t N, float *ref_pr, uint32_t *ref_order, float *pr, uint32_t *order,
+                          uint32_t topk, uint32_t num_vertices, uint32_t *out_u32, float *out_f32,
+                          double *out_f64)
+{
+    struct PageRankStats rs, s;
+    memset(&rs, 0, sizeof(rs));
+    memset(&s, 0, sizeof(s));
+    rs.num_vertices = s.num_vertices = N;
+    rs.pageRanks = ref_pr; rs.realRanks = ref_order;
+    s.pageRanks = pr; s.realRanks = order;
+    uint32_t *inv = (uint32_t *)malloc((size_t)N * 4), *ref_inv = (uint32_t *)malloc((size_t)N * 4);
+    if (!inv || !ref_inv) { free(inv); free(ref_inv); return -1; }
+    for (uint32_t v = 0; v < N; v++)
+    {
+        inv[order[v]] = v;
+        ref_inv[ref_order[v]] = v;
+    }
+    struct PageRankCorrelationStats c =
+        collectStatsPageRank_topK(&rs, &s, ref_order, ref_inv, inv, topk, num_vertices, NULL, 0);
+    out_u32[0] = c.levenshtein_distance;
+    out_u32[1] = c.intersection;
+    out_u32[2] = c.mismatch;
+    out_f32[0] = c.float_Kendall;
+    out_f32[1] = c.real_Kendall;
+    out_f64[0] = c.avg_error_float;
+    out_f64[1] = c.avg_error_relative;
+    free(inv);
+    free(ref_inv);
+    return 0;
+}
+
+/* The reference's own collectStatsPageRank (graphStats.c:691-1020): appends
+ * to <fnameb>_1_0_<trial>_<pushpull>.stats; stdout goes to stdout_path
+ * (NULL: discarded).  seconds_out: wall time of the call. */
+int ogref_collect_stats(uint32_t N, float *ref_pr, uint32_t *ref_order, float *pr, uint32_t *order,
+                        char *fnameb, uint32_t bin_size, uint32_t verbosity, uint32_t pushpull,
+                        uint32_t trial, double time_total, uint32_t iterations, int threads,
+                        const char *stdout_path, double *seconds_out)
+{
+    struct PageRankStats rs, s;
+    memset(&rs, 0, sizeof(rs));
+    memset(&s, 0, sizeof(s));
+    rs.num_vertices = s.num_vertices = N;
+    rs.pageRanks = ref_pr; rs.realRanks = ref_order;
+    s.pageRanks = pr; s.realRanks = order;
+    s.time_total = time_total;
+    s.iterations = iterations;
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.fnameb = fnameb;
+    a.binSize = bin_size;
+    a.verbosity = verbosity;
+    a.pushpull = pushpull;
+    a.algorithm = 1;
+    a.datastructure = 0;
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = threads;
+    fflush(stdout);
+    int saved = dup(1);
+    int sink = stdout_path ? open(stdout_path, O_WRONLY | O_CREAT | O_TRUNC, 0644) : open("/dev/null", O_WRONLY);
+    if (saved >= 0 && sink >= 0) dup2(sink, 1);
+    double t0 = omp_get_wtime();
+    collectStatsPageRank(&a, &rs, &s, trial);
+    double t1 = omp_get_wtime();
+    fflush(stdout);
+    if (saved >= 0) { dup2(saved, 1); close(saved); }
+    if (sink >= 0) close(sink);
+    *seconds_out = t1 - t0;
     return 0;
 }
