@@ -21,6 +21,7 @@
 
 #include <stdint.h>
 #include <stddef.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -196,6 +197,41 @@ struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct
 struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph);
 /* SPMV.c:139-152 */
 void freeSPMVStats(struct SPMVStats *stats);
+
+/* graphStats.h:28-37 */
+struct PageRankCorrelationStats
+{
+    uint32_t levenshtein_distance;
+    float float_Kendall;
+    float real_Kendall;
+    uint32_t intersection;
+    uint32_t mismatch;
+    double avg_error_float;
+    double avg_error_relative;
+};
+
+/* ------------------------------------------------------------------------
+ * Rank comparison (-S): a variant's ranking against the pull ranking over
+ * the top-k positions of realRanks (ascending by rank, as
+ * radixSortEdgesByPageRank leaves them).  The statistics are computed on
+ * the GPU; the printed tables and the .stats file are the reference's.
+ * ---------------------------------------------------------------------- */
+
+/* graphStats.c:691-1020 (decl graphStats.h:46): top-k table for 30 .. 10000,
+ * the 1000-wide chunks when arguments->verbosity > 1, and top binSize;
+ * appends to <fnameb>_<algorithm>_<datastructure>_<trial>_<pushpull>.stats. */
+void collectStatsPageRank(struct Arguments *arguments, struct PageRankStats *ref_stats, struct PageRankStats *stats,
+                          uint32_t trial);
+/* graphStats.c:580-688 (decl graphStats.h:47): positions
+ * [num_vertices - topk, num_vertices).  The three rank arrays must be what
+ * collectStatsPageRank passes (ref_stats->realRanks and the inverses of
+ * both realRanks, graphStats.c:746-751): they are formed again on the GPU
+ * from the two realRanks. */
+struct PageRankCorrelationStats collectStatsPageRank_topK(struct PageRankStats *ref_stats, struct PageRankStats *stats,
+                                                          uint32_t *ref_rankedVertices_total,
+                                                          uint32_t *ref_rankedVertices_inverse,
+                                                          uint32_t *rankedVertices_inverse, uint32_t topk,
+                                                          uint32_t num_vertices, FILE *fptr, uint32_t verbose);
 
 /* ------------------------------------------------------------------------
  * Edge list -> CSR builder.  Sorting and degree mapping run on the GPU; the

@@ -307,6 +307,20 @@ int ogpu_spmv_kernel_ms(ogpu_spmv *s, double *total_ms, uint32_t *count);
 double ogpu_spmv_bytes_per_iteration(const ogpu_spmv *s);
 void ogpu_spmv_free(ogpu_spmv *s);
 
+/* ---- rank comparison statistics (graphStats.c:580-688 on the GPU) ----
+ * P problems: problem p compares the two rankings over positions
+ * [num_vertices[p] - k, num_vertices[p]) with k = min(topk[p],
+ * num_vertices[p]) and writes out[p] as collectStatsPageRank_topK would.
+ * ref_pr / pr: N ranks; ref_order / order: realRanks (permutations of
+ * [0, N) ascending by rank).  _device takes device pointers (the rankings
+ * stay resident); the other uploads host arrays first. */
+int ogpu_rank_stats(const float *ref_pr, const uint32_t *ref_order, const float *pr, const uint32_t *order,
+                    uint32_t N, const uint32_t *topk, const uint32_t *num_vertices, uint32_t P,
+                    struct PageRankCorrelationStats *out);
+int ogpu_rank_stats_device(const float *d_ref_pr, const uint32_t *d_ref_order, const float *d_pr,
+                           const uint32_t *d_order, uint32_t N, const uint32_t *topk,
+                           const uint32_t *num_vertices, uint32_t P, struct PageRankCorrelationStats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,16 @@ class SPMVStats(C.Structure):
     ]
 
 
+class PageRankCorrelationStats(C.Structure):
+    """graphStats.h:28-37."""
+    _fields_ = [("levenshtein_distance", u32), ("float_Kendall", C.c_float), ("real_Kendall", C.c_float),
+                ("intersection", u32), ("mismatch", u32), ("avg_error_float", C.c_double),
+                ("avg_error_relative", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SpmvOptions(C.Structure):
     """ogpu_spmv_options (opengraph_amd_ext.h)."""
     _fields_ = [("code", u32), ("hub_threshold", u32), ("hot_cache", u32), ("scatter", u32)]
@@ -144,6 +154,9 @@ SIGNATURES = {
     "SPMVPullFixedPointGraphCSR": _SPMV_FN, "SPMVPushFixedPointGraphCSR": _SPMV_FN,
     "newSPMVStatsGraphCSR": (C.POINTER(SPMVStats), [C.POINTER(GraphCSR)]),
     "freeSPMVStats": (None, [C.POINTER(SPMVStats)]),
+    "collectStatsPageRank": (None, [C.POINTER(Arguments), C.POINTER(PageRankStats), C.POINTER(PageRankStats), u32]),
+    "collectStatsPageRank_topK": (PageRankCorrelationStats, [C.POINTER(PageRankStats), C.POINTER(PageRankStats),
+                                                             u32p, u32p, u32p, u32, u32, C.c_void_p, u32]),
     "graphCSRPreProcessingStep": (C.POINTER(GraphCSR), [C.POINTER(Arguments)]),
     "graphCSRNew": (C.POINTER(GraphCSR), [u32, u32, C.c_uint8]),
     "graphCSRAssignEdgeList": (C.POINTER(GraphCSR), [C.POINTER(GraphCSR), C.POINTER(EdgeList), C.c_uint8]),
@@ -239,6 +252,9 @@ SIGNATURES = {
     "ogpu_spmv_kernel_ms": (C.c_int, [_VP, C.POINTER(C.c_double), u32p]),
     "ogpu_spmv_bytes_per_iteration": (C.c_double, [_VP]),
     "ogpu_spmv_free": (None, [_VP]),
+    "ogpu_rank_stats": (C.c_int, [_VP, _VP, _VP, _VP, u32, u32p, u32p, u32, C.POINTER(PageRankCorrelationStats)]),
+    "ogpu_rank_stats_device": (C.c_int, [_VP, _VP, _VP, _VP, u32, u32p, u32p, u32,
+                                         C.POINTER(PageRankCorrelationStats)]),
 }
 
 GRAPH_ARRAYS = ["f_src", "f_dst", "f_out_degree", "f_in_degree", "f_edges_idx",
@@ -808,3 +824,66 @@ def spmv_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20):
     out = (y, x, s.iterations, s.time_total)
     lib.freeSPMVStats(st)
     return out
+
+
+def rank_stats(ref_pr, ref_order, pr, order, problems):
+    """ogpu_rank_stats: problems = [(topk, num_vertices), ...]; list of dicts
+    with the seven PageRankCorrelationStats fields."""
+    rp, ro = np.ascontiguousarray(ref_pr, np.float32), np.ascontiguousarray(ref_order, np.uint32)
+    p, o = np.ascontiguousarray(pr, np.float32), np.ascontiguousarray(order, np.uint32)
+    ks = np.ascontiguousarray([k for k, _ in problems], np.uint32)
+    ms = np.ascontiguousarray([m for _, m in problems], np.uint32)
+    out = (PageRankCorrelationStats * len(problems))()
+    check(load().ogpu_rank_stats(_ptr(rp), _ptr(ro), _ptr(p), _ptr(o), len(rp), ks.ctypes.data_as(u32p),
+                                 ms.ctypes.data_as(u32p), len(problems), out))
+    return [c.as_dict() for c in out]
+
+
+def rank_stats_device(d_ref_pr, d_ref_order, d_pr, d_order, n, problems):
+    """ogpu_rank_stats_device on DeviceArrays of n entries."""
+    ks = np.ascontiguousarray([k for k, _ in problems], np.uint32)
+    ms = np.ascontiguousarray([m for _, m in problems], np.uint32)
+    out = (PageRankCorrelationStats * len(problems))()
+    check(load().ogpu_rank_stats_device(d_ref_pr.ptr, d_ref_order.ptr, d_pr.ptr, d_order.ptr, n,
+                                        ks.ctypes.data_as(u32p), ms.ctypes.data_as(u32p), len(problems), out))
+    return [c.as_dict() for c in out]
+
+
+def _pr_stats(pr, order):
+    pr = np.ascontiguousarray(pr, np.float32)
+    order = np.ascontiguousarray(order, np.uint32)
+    st = PageRankStats()
+    st.damp, st.base_pr, st.num_vertices = 0.85, 0.15, len(pr)
+    st.pageRanks = pr.ctypes.data_as(C.POINTER(C.c_float))
+    st.realRanks = order.ctypes.data_as(u32p)
+    return st, (pr, order)
+
+
+def collect_stats_pagerank(ref_pr, ref_order, pr, order, fnameb, bin_size=512, verbosity=0, pushpull=0,
+                           trial=0, time_total=0.0, iterations=0, threads=1):
+    """The drop-in collectStatsPageRank: prints its tables and appends to
+    <fnameb>_1_0_<trial>_<pushpull>.stats."""
+    rs, keep1 = _pr_stats(ref_pr, ref_order)
+    st, keep2 = _pr_stats(pr, order)
+    st.time_total, st.iterations = time_total, iterations
+    fb = fnameb.encode()
+    args = default_arguments(binSize=bin_size, verbosity=verbosity, pushpull=pushpull, algorithm=1,
+                             datastructure=0, fnameb=fb)
+    args.pre_numThreads = args.algo_numThreads = args.ker_numThreads = threads
+    load().collectStatsPageRank(C.byref(args), C.byref(rs), C.byref(st), trial)
+    del keep1, keep2, fb
+
+
+def collect_stats_pagerank_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
+    """The drop-in collectStatsPageRank_topK (verbose 0, no file)."""
+    rs, keep1 = _pr_stats(ref_pr, ref_order)
+    st, keep2 = _pr_stats(pr, order)
+    n = len(keep1[0])
+    inv = np.empty(n, np.uint32)
+    inv[keep2[1]] = np.arange(n, dtype=np.uint32)
+    rinv = np.empty(n, np.uint32)
+    rinv[keep1[1]] = np.arange(n, dtype=np.uint32)
+    c = load().collectStatsPageRank_topK(C.byref(rs), C.byref(st), keep1[1].ctypes.data_as(u32p),
+                                         rinv.ctypes.data_as(u32p), inv.ctypes.data_as(u32p), topk,
+                                         n if num_vertices is None else num_vertices, None, 0)
+    return c.as_dict()

@@ -1,6 +1,6 @@
 """Write tests/golden/abi_layout.json: sizeof / offsetof of the reference's
 boundary structs (Arguments, Vertex, EdgeList, GraphCSR, PageRankStats,
-SPMVStats) as
+SPMVStats, PageRankCorrelationStats) as
 the reference's own headers lay them out on x86-64, by compiling a probe
 against /root/reference's include tree.  The ABI test compares
 include/opengraph_amd.h and the ctypes mirror against this fixture.
@@ -29,6 +29,8 @@ FIELDS = {
                       "realRanks", "pageRanks", "time_total", "error_total"],
     "SPMVStats": ["iterations", "arguments", "num_vertices", "vector_output", "vector_input",
                   "time_total"],
+    "PageRankCorrelationStats": ["levenshtein_distance", "float_Kendall", "real_Kendall", "intersection",
+                                 "mismatch", "avg_error_float", "avg_error_relative"],
 }
 
 
@@ -59,5 +61,5 @@ def layout(includes, incdirs):
 
 if __name__ == "__main__":
     incs = [os.path.join(REF, d) for d in ("structures", "algorithms/openmp", "preprocess", "utils", "config")]
-    json.dump(layout(["graphConfig.h", "graphCSR.h", "pageRank.h", "SPMV.h"], incs), open(OUT, "w"), indent=1)
+    json.dump(layout(["graphConfig.h", "graphCSR.h", "pageRank.h", "SPMV.h", "graphStats.h"], incs), open(OUT, "w"), indent=1)
     print(open(OUT).read()[:300])

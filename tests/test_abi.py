@@ -40,7 +40,8 @@ def test_headers_declare_the_reference_entry_points():
                "newPageRankStatsGraphCSR", "freePageRankStats", "graphCSRPreProcessingStep",
                "sortRunAlgorithms", "mapVerticesWithInOutDegree", "readEdgeListsbin", "SPMVGraphCSR",
                "SPMVPullGraphCSR", "SPMVPushGraphCSR", "SPMVPullFixedPointGraphCSR",
-               "SPMVPushFixedPointGraphCSR", "newSPMVStatsGraphCSR", "freeSPMVStats"]:
+               "SPMVPushFixedPointGraphCSR", "newSPMVStatsGraphCSR", "freeSPMVStats",
+               "collectStatsPageRank", "collectStatsPageRank_topK"]:
         assert fn in names
 
 
@@ -71,7 +72,8 @@ def test_header_struct_layout_matches_reference(ref_layout):
 
 def test_ctypes_struct_layout_matches_reference(ref_layout):
     types = {"Arguments": og.Arguments, "Vertex": og.Vertex, "EdgeList": og.EdgeList,
-             "GraphCSR": og.GraphCSR, "PageRankStats": og.PageRankStats, "SPMVStats": og.SPMVStats}
+             "GraphCSR": og.GraphCSR, "PageRankStats": og.PageRankStats, "SPMVStats": og.SPMVStats,
+             "PageRankCorrelationStats": og.PageRankCorrelationStats}
     for name, t in types.items():
         want = ref_layout[name]
         assert C.sizeof(t) == want["size"], name
