@@ -21,8 +21,9 @@ of iterations.
 
 "extensions" holds the other measured forms, each also on one GPU: push and
 fixed-point variants with their host runs, the reference's SpMV benchmark
-(-p 0..3) on the same graph against its host run, the DBG relabelling and the
-.csr file write / load, the Erdos-Renyi graph of the same
+(-p 0..3) on the same graph against its host run, the -S rank comparison of
+two rankings of that graph against the reference's statistics code, the DBG
+relabelling and the .csr file write / load, the Erdos-Renyi graph of the same
 size, batched personalized PageRank (R-MAT 22, k = 64, against 64
 single-source runs), 256 uneven graphs in one hipGraph launch (against
 per-graph launches) and the learned warm start (time to L1 < 1e-6 against the
@@ -168,6 +169,71 @@ def ext_spmv(g, threads, csr):
                     d["two_pass_max_rel_err_vs_fp64_gpu"] = float((np.abs(y - exact) / den).max())
                     d["two_pass_max_rel_err_vs_fp64_reference"] = float((np.abs(ref - exact) / den).max())
                 log(f"  spmv {name} vs reference: {d}")
+    return out
+
+
+def ext_rank_stats(g):
+    """The -S comparison of the fixed-point 8-bit pull ranking (-p 11) with
+    the pull ranking on the headline graph, both ordered by the library's
+    radixSortEdgesByPageRank: every problem collectStatsPageRank computes at
+    verbosity 2 (top-k 30 .. 10000, all 1000-wide chunks, top 512) on the
+    GPU with the rankings resident, against the reference's own compiled
+    collectStatsPageRank_topK on the host for the top-k table and a sample
+    of 40 chunks (field-for-field equal to the GPU's), scaled to all chunks."""
+    import ctypes as C
+    import opengraph_amd as og
+    from oracle import pyoracle as po
+    N = g.num_vertices
+    lib = og.load()
+    rk = {}
+    for code in (0, 11):
+        pr = og.PageRank(g, code, 20, 1e-4)
+        pr.run()
+        r = pr.ranks()
+        pr.free()
+        lab = np.arange(N, dtype=np.uint32)
+        lib.radixSortEdgesByPageRank(r.ctypes.data_as(C.POINTER(C.c_float)), lab.ctypes.data_as(C.POINTER(C.c_uint32)), N)
+        rk[code] = (r, lab)
+    (ref_pr, ref_order), (pr, order) = rk[0], rk[11]
+    tops = [k for k in (30, 100, 300, 1000, 5000, 10000) if k <= N]
+    chunks = [(1000, N - 1000 * x) for x in range((N + 999) // 1000)]
+    probs = [(k, N) for k in tops] + chunks + [(512, N)]
+    arrs = [og.DeviceArray.from_numpy(a) for a in (ref_pr, ref_order, pr, order)]
+    try:
+        times = []
+        for _ in range(3):
+            og.synchronize()
+            t0 = time.perf_counter()
+            got = og.rank_stats_device(*arrs, N, probs)
+            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        og.rank_stats(ref_pr, ref_order, pr, order, [(k, N) for k in tops])
+        t_table_h2d = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        og.rank_stats_device(*arrs, N, [(k, N) for k in tops])
+        t_table = time.perf_counter() - t0
+    finally:
+        for a in arrs:
+            a.free()
+    out = dict(workload=f"R-MAT-24 -p 11 vs -p 0 rankings; {len(probs)} problems (top-k {tops}, "
+                        f"{len(chunks)} chunks of 1000, top 512)",
+               gpu_all_problems_seconds=round(min(times), 4), gpu_topk_table_seconds=round(t_table, 4),
+               gpu_topk_table_with_upload_seconds=round(t_table_h2d, 4),
+               top10000=got[len(tops) - 1] if tops else None)
+    if po.ref_available():
+        ref_tab, t_ref_table = po.ref_rank_stats_batch(ref_pr, ref_order, pr, order, [(k, N) for k in tops])
+        sample = list(range(0, len(chunks), max(1, len(chunks) // 40)))[:40]
+        ref_ch, t_ref_ch = po.ref_rank_stats_batch(ref_pr, ref_order, pr, order, [chunks[x] for x in sample])
+        t_ref_chunk = t_ref_ch / len(sample)
+        same = ref_tab == got[:len(tops)] and ref_ch == [got[len(tops) + x] for x in sample]
+        out.update(cpu_reference_topk_table_seconds=round(t_ref_table, 3),
+                   cpu_reference_seconds_per_chunk=round(t_ref_chunk, 5),
+                   cpu_reference_all_problems_seconds_scaled=round(t_ref_table + t_ref_chunk * len(chunks), 1),
+                   cpu_sample=f"top-k table + {len(sample)} of {len(chunks)} chunks, one thread (the reference's "
+                              f"collectStatsPageRank is serial), collectStatsPageRank_topK calls only",
+                   results_equal_to_reference=bool(same),
+                   speedup_topk_table=round(t_ref_table / t_table, 1),
+                   speedup_all_problems=round((t_ref_table + t_ref_chunk * len(chunks)) / min(times), 1))
     return out
 
 
@@ -498,6 +564,7 @@ def main():
     if not args.no_extensions:
         for name, fn in (("variants", lambda: ext_variants(g, threads, csr, csr is not None)),
                          ("spmv", lambda: ext_spmv(g, threads, csr)),
+                         ("rank_stats", lambda: ext_rank_stats(g)),
                          ("relabel_dbg", lambda: ext_relabel(g, threads, csr)),
                          ("csr_file", ext_csr_file),
                          ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),

@@ -113,9 +113,9 @@ def ref_lib():
         R.ogref_spmv.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
             C.c_uint32, C.c_uint32, C.c_int, f32p, f64p]
         R.ogref_spmv.restype = C.c_int
-        R.ogref_rank_stats_topk.argtypes = [C.c_uint32, f32p, u32p, f32p, u32p, C.c_uint32, C.c_uint32,
-                                            u32p, f32p, f64p]
-        R.ogref_rank_stats_topk.restype = C.c_int
+        R.ogref_rank_stats_batch.argtypes = [C.c_uint32, f32p, u32p, f32p, u32p, C.c_uint32, u32p, u32p,
+                                             u32p, f32p, f64p, f64p]
+        R.ogref_rank_stats_batch.restype = C.c_int
         R.ogref_collect_stats.argtypes = [C.c_uint32, f32p, u32p, f32p, u32p, C.c_char_p, C.c_uint32,
                                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,
                                           C.c_int, C.c_char_p, f64p]
@@ -315,21 +315,32 @@ def rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
     return out.as_dict()
 
 
-def ref_rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
-    """The reference's own collectStatsPageRank_topK: dict of the seven fields."""
+def ref_rank_stats_batch(ref_pr, ref_order, pr, order, problems):
+    """The reference's own collectStatsPageRank_topK for each (topk,
+    num_vertices) in problems, inverse rankings formed once; returns (list of
+    dicts, seconds spent in the _topK calls)."""
     rp, ro = np.ascontiguousarray(ref_pr, np.float32), np.ascontiguousarray(ref_order, np.uint32)
     p, o = np.ascontiguousarray(pr, np.float32), np.ascontiguousarray(order, np.uint32)
-    n = len(rp)
-    u = np.zeros(3, np.uint32)
-    f = np.zeros(2, np.float32)
-    d = np.zeros(2, np.float64)
-    if ref_lib().ogref_rank_stats_topk(n, _p(rp, f32p), _p(ro), _p(p, f32p), _p(o), topk,
-                                       n if num_vertices is None else num_vertices, _p(u), _p(f, f32p),
-                                       _p(d, f64p)):
-        raise RuntimeError("ogref_rank_stats_topk failed")
-    return dict(levenshtein_distance=int(u[0]), float_Kendall=float(f[0]), real_Kendall=float(f[1]),
-                intersection=int(u[1]), mismatch=int(u[2]), avg_error_float=float(d[0]),
-                avg_error_relative=float(d[1]))
+    P = len(problems)
+    ks = np.ascontiguousarray([k for k, _ in problems], np.uint32)
+    ms = np.ascontiguousarray([m for _, m in problems], np.uint32)
+    u = np.zeros(3 * P, np.uint32)
+    f = np.zeros(2 * P, np.float32)
+    d = np.zeros(2 * P, np.float64)
+    sec = C.c_double()
+    if ref_lib().ogref_rank_stats_batch(len(rp), _p(rp, f32p), _p(ro), _p(p, f32p), _p(o), P, _p(ks), _p(ms),
+                                        _p(u), _p(f, f32p), _p(d, f64p), C.byref(sec)):
+        raise RuntimeError("ogref_rank_stats_batch failed")
+    out = [dict(levenshtein_distance=int(u[3 * i]), float_Kendall=float(f[2 * i]),
+                real_Kendall=float(f[2 * i + 1]), intersection=int(u[3 * i + 1]), mismatch=int(u[3 * i + 2]),
+                avg_error_float=float(d[2 * i]), avg_error_relative=float(d[2 * i + 1])) for i in range(P)]
+    return out, sec.value
+
+
+def ref_rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):
+    """The reference's own collectStatsPageRank_topK: dict of the seven fields."""
+    n = len(ref_pr)
+    return ref_rank_stats_batch(ref_pr, ref_order, pr, order, [(topk, n if num_vertices is None else num_vertices)])[0][0]
 
 
 def ref_collect_stats(ref_pr, ref_order, pr, order, fnameb, bin_size=512, verbosity=0, pushpull=0,

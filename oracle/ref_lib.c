@@ -196,13 +196,14 @@ int ogref_reorder(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *d
 }
 
 /* The reference's own collectStatsPageRank_topK (graphStats.c:580-688) on
- * rankings given as arrays; the inverse rankings are formed as
- * collectStatsPageRank does (graphStats.c:746-751).  out_u32 gets
- * levenshtein_distance, intersection, mismatch; out_f32 the float and real
- * Kendall; out_f64 avg_error_float, avg_error_relative. */
-int ogref_rank_stats_topk(uint32_t N, float *ref_pr, uint32_t *ref_order, float *pr, uint32_t *order,
-                          uint32_t topk, uint32_t num_vertices, uint32_t *out_u32, float *out_f32,
-                          double *out_f64)
+ * rankings given as arrays, for P problems (topk[p], num_vertices[p]); the
+ * inverse rankings are formed once, as collectStatsPageRank does
+ * (graphStats.c:746-751).  Per problem: out_u32 gets levenshtein_distance,
+ * intersection, mismatch; out_f32 the float and real Kendall; out_f64
+ * avg_error_float, avg_error_relative.  seconds_out: the _topK calls. */
+int ogref_rank_stats_batch(uint32_t N, float *ref_pr, uint32_t *ref_order, float *pr, uint32_t *order,
+                           uint32_t P, const uint32_t *topk, const uint32_t *num_vertices, uint32_t *out_u32,
+                           float *out_f32, double *out_f64, double *seconds_out)
 {
     struct PageRankStats rs, s;
     memset(&rs, 0, sizeof(rs));
@@ -217,15 +218,20 @@ int ogref_rank_stats_topk(uint32_t N, float *ref_pr, uint32_t *ref_order, float 
         inv[order[v]] = v;
         ref_inv[ref_order[v]] = v;
     }
-    struct PageRankCorrelationStats c =
-        collectStatsPageRank_topK(&rs, &s, ref_order, ref_inv, inv, topk, num_vertices, NULL, 0);
-    out_u32[0] = c.levenshtein_distance;
-    out_u32[1] = c.intersection;
-    out_u32[2] = c.mismatch;
-    out_f32[0] = c.float_Kendall;
-    out_f32[1] = c.real_Kendall;
-    out_f64[0] = c.avg_error_float;
-    out_f64[1] = c.avg_error_relative;
+    double t0 = omp_get_wtime();
+    for (uint32_t p = 0; p < P; p++)
+    {
+        struct PageRankCorrelationStats c =
+            collectStatsPageRank_topK(&rs, &s, ref_order, ref_inv, inv, topk[p], num_vertices[p], NULL, 0);
+        out_u32[3 * p] = c.levenshtein_distance;
+        out_u32[3 * p + 1] = c.intersection;
+        out_u32[3 * p + 2] = c.mismatch;
+        out_f32[2 * p] = c.float_Kendall;
+        out_f32[2 * p + 1] = c.real_Kendall;
+        out_f64[2 * p] = c.avg_error_float;
+        out_f64[2 * p + 1] = c.avg_error_relative;
+    }
+    *seconds_out = omp_get_wtime() - t0;
     free(inv);
     free(ref_inv);
     return 0;
