@@ -1,6 +1,7 @@
 """Benchmark: PageRank pull iterations on R-MAT scale 24 (one MI355X).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extensions]
+                    [--extensions spmv,rank_stats,...]
 
 Workload: directed R-MAT, scale 24 (2^24 vertices), edge factor 16 (2^28
 edges), a=0.57 b=0.19 c=0.19, vertex ids permuted, seed 1, duplicates and
@@ -463,6 +464,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extensions", action="store_true")
+    ap.add_argument("--extensions", default="",
+                    help="comma-separated subset of the extension legs to run (default: all)")
     ap.add_argument("--copy-reference", action="store_true",
                     help="after timing, run one 1 GiB device copy (og_copy_kernel): the reference "
                          "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -569,9 +572,11 @@ def main():
                          ("csr_file", ext_csr_file),
                          ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),
                          ("uneven_batch", ext_multi), ("learned_warm_start", ext_learned)):
+            if args.extensions and name not in args.extensions.split(","):
+                continue
             t0 = time.perf_counter()
             try:
-                if name == "batched_ppr":
+                if name == "batched_ppr" and g is not None:
                     g.free()          # free the headline graph before the big extensions
                     g = None
                 extensions[name] = fn()

@@ -19,19 +19,20 @@
 #include "opengraph_amd.h"
 #include "opengraph_amd_ext.h"
 
-static char doc[] = "OpenGraph PageRank-over-CSR on MI355X (gfx950).";
-static char args_doc[] = "-f <graph file> -a 1 -d 0 -p [variant] ...";
+static char doc[] = "OpenGraph PageRank-over-CSR (and SpMV) on MI355X (gfx950).";
+static char args_doc[] = "-f <graph file> -a 1|5 -d 0 -p [variant] ...";
 
 static struct argp_option options[] = {
     {"graph-file", 'f', "<FILE>", 0, "Edge list in the binary .bin format."},
     {"graph-file-format", 'z', "[1]", 0, "Only [1] binary edge list is supported."},
-    {"algorithm", 'a', "[1]", 0, "Only [1]-Page-rank is supported."},
+    {"algorithm", 'a', "[1]", 0, "[1]-Page-rank [5]-SPMV (the others are not supported)."},
     {"data-structure", 'd', "[0]", 0, "Only [0]-CSR is supported."},
     {"root", 'r', "[0]", 0, "Unused by PageRank."},
     {"direction", 'p', "[0]", 0,
      "[0]-pull [1]-push [2]-pull FP64 [3]-push FP64 [4]-pull Quant32 [5]-push Quant "
      "[6]-DD pull [7]-DD push [8]-DD pull-push [9]-pull FP32 [10]-pull FP16 [11]-pull FP8 "
-     "[12]-pull Quant16 [13]-pull Quant8."},
+     "[12]-pull Quant16 [13]-pull Quant8. SPMV (-a 5): [0]-pull [1]-push [2]-pull fixed "
+     "[3]-push fixed."},
     {"sort", 'o', "[0]", 0, "[0]-radix-src [1]-radix-src-dest [2]-count-src [3]-count-src-dst."},
     {"pre-num-threads", 'n', "[MAX]", 0, "Host threads for preprocessing."},
     {"algo-num-threads", 'N', "[MAX]", 0, "Host threads for the algorithm."},
@@ -49,8 +50,8 @@ static struct argp_option options[] = {
     {"generate-weights", 'w', 0, 0, "Records carry a weight (12-byte .wbin); weights are skipped."},
     {"symmetrize", 's', 0, 0, "Add the reverse of every edge."},
     {"serialize", 'x', 0, 0, "Not supported."},
-    {"stats", 'S', 0, 0, "Not supported."},
-    {"bin-size", 'g', "[512]", 0, "Unused."},
+    {"stats", 'S', 0, 0, "PageRank: compare with a pull run (graphStats.c), write <graph>_1_0_<trial>_0.stats."},
+    {"bin-size", 'g', "[512]", 0, "Top-k of the -S comparison."},
     {"verbosity", 'j', "[0]", 0, "1: append '<threads> <avg seconds>' to <graph>_1_0_<trials>_<p>.perf."},
     {"remove-duplicate", 'k', 0, 0, "Not supported."},
     {"mask-mode", 'M', "[0]", 0, "Not supported (must be 0)."},
@@ -165,10 +166,11 @@ int main(int argc, char **argv)
 
     if (g_generate) return generate(g_generate, a->fnameb);
     if (!a->fnameb) { fprintf(stderr, "missing -f <graph file>\n"); return 2; }
-    if (a->algorithm != 1 || (a->datastructure != 0 && a->datastructure != 4) || a->xflag ||
-        a->Sflag || a->fnameb_format != 1)
+    if ((a->algorithm != 1 && a->algorithm != 5) || (a->datastructure != 0 && a->datastructure != 4) ||
+        a->xflag || a->fnameb_format != 1)
     {
-        fprintf(stderr, "this driver runs PageRank (-a 1) on CSR (-d 0) from binary edge lists only\n");
+        fprintf(stderr, "this driver runs PageRank (-a 1) and SpMV (-a 5) on CSR (-d 0) from binary edge "
+                        "lists only\n");
         return 2;
     }
 
@@ -181,13 +183,36 @@ int main(int argc, char **argv)
     printf(" -----------------------------------------------------\n");
     double time_total = 0.0;
     uint32_t trials = a->trials;
+    /* graphRun.c:393-470 for -a 1 and -a 5 */
     while (trials)
     {
+        if (a->algorithm == 5)
+        {
+            struct SPMVStats *stats = SPMVGraphCSR(a, graph);
+            if (stats)
+            {
+                time_total += stats->time_total;
+                freeSPMVStats(stats);
+            }
+            trials--;
+            continue;
+        }
         struct PageRankStats *stats = pageRankGraphCSR(a, graph);
         if (stats)
         {
             stats->realRanks = radixSortEdgesByPageRank(stats->pageRanks, stats->realRanks, stats->num_vertices);
             time_total += stats->time_total;
+            if (a->Sflag)
+            {
+                /* graphRun.c:414-420: the pull run is the reference ranking, and
+                 * -p stays 0 for the trials after it, as in the reference */
+                a->pushpull = 0;
+                struct PageRankStats *ref_stats = pageRankGraphCSR(a, graph);
+                ref_stats->realRanks =
+                    radixSortEdgesByPageRank(ref_stats->pageRanks, ref_stats->realRanks, ref_stats->num_vertices);
+                collectStatsPageRank(a, ref_stats, stats, trials);
+                freePageRankStats(ref_stats);
+            }
             freePageRankStats(stats);
         }
         trials--;
