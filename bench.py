@@ -89,7 +89,8 @@ def gpu_iter_ms(g, variant, steps=10, warmup=2, **kw):
 def ext_variants(g, threads, csr, do_cpu):
     """Pull/push/fixed-point on the headline graph, each against the host run."""
     out, cpu = {}, {}
-    for code, name, kw in ((1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
+    for code, name, kw in ((0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}),
+                           (1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
                            (2, "pull_fixed64", {}), (3, "push_fixed64", {}), (11, "pull_fixed8", {})):
         ms, kms, bpi = gpu_iter_ms(g, code, **kw)
         d = dict(ms_per_iteration=round(ms, 4), gteps=round(g.num_edges / ms / 1e6, 2),

@@ -109,6 +109,7 @@ void ogpu_pr_free(ogpu_pr *p)
     free(p->rows_host);
     og_dfree(p->wl[0]); og_dfree(p->wl[1]); og_dfree(p->changed); og_dfree(p->resid);
     og_dfree(p->feat); og_dfree(p->lscratch);
+    og_dfree(p->chain_rows);
     free(p);
 }
 
@@ -143,6 +144,46 @@ void og_pr_use_init(ogpu_pr *p)
     p->opt.init_mode = 2;
 }
 
+/* The rows [0, C) of a plan ordered by in-degree, longest first (stable):
+ * the order the every-row-exact pull starts its wavefront chains in, so the
+ * longest one, which bounds the launch, starts at once. */
+static int og_pr_chain_rows(const og_plan *pl, uint32_t **out)
+{
+    *out = NULL;
+    const uint32_t C = pl->C;
+    uint32_t *rp = (uint32_t *)malloc((size_t)(C + 1) * 4), *rows = (uint32_t *)malloc((size_t)(C ? C : 1) * 4);
+    uint32_t *d = og_dalloc((size_t)(C ? C : 1) * 4);
+    if (!rp || !rows || !d)
+    {
+        if (!rp || !rows) og_set_error("out of host memory");
+        free(rp); free(rows); og_dfree(d);
+        return -1;
+    }
+    int rc = -1;
+    if (!og_hip_check(hipMemcpy(rp, pl->row_ptr, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr"))
+    {
+        /* counting sort by length, longest first; ties keep row order */
+        uint32_t maxlen = 0;
+        for (uint32_t r = 0; r < C; r++) if (rp[r + 1] - rp[r] > maxlen) maxlen = rp[r + 1] - rp[r];
+        uint32_t *cnt = (uint32_t *)calloc((size_t)maxlen + 2, 4);
+        if (cnt)
+        {
+            for (uint32_t r = 0; r < C; r++) cnt[maxlen - (rp[r + 1] - rp[r])]++;
+            uint32_t acc = 0;
+            for (uint32_t k = 0; k <= maxlen; k++) { uint32_t t = cnt[k]; cnt[k] = acc; acc += t; }
+            for (uint32_t r = 0; r < C; r++) rows[cnt[maxlen - (rp[r + 1] - rp[r])]++] = r;
+            free(cnt);
+            rc = og_hip_check(hipMemcpy(d, rows, (size_t)C * 4, hipMemcpyHostToDevice), "H2D chain rows");
+        }
+        else og_set_error("out of host memory");
+    }
+    free(rp);
+    free(rows);
+    if (rc) { og_dfree(d); return -1; }
+    *out = d;
+    return 0;
+}
+
 /* Create a run on graph g.  plan may be supplied (borrowed) or built here. */
 int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, ogpu_pr **out)
 {
@@ -166,17 +207,35 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     if (p->opt.hub_threshold == 0) p->opt.hub_threshold = OG_DEFAULT_SEQ_T;
     og_variant_mode(opt->variant, &p->mode, &p->push, &p->quant, &p->dd);
     const uint32_t N = g->N, E = g->E;
+    /* fp32 pull with every row in the reference's order: the rows of more
+     * than OG_DEFAULT_SEQ_T in-edges become wavefront chains over CSR-order
+     * rows (relabel 1's sorted long rows are kept in CSR order instead) */
+    const int chain = opt->variant == 0 && opt->hub_threshold >= 0x80000000u;
     double t0 = og_now();
     if (plan)
+    {
         p->plan = plan;
+        if (chain && plan->relabel == 1 && plan->H)
+        {
+            og_set_error("every-row-exact pull needs a plan that keeps long rows in CSR order");
+            free(p);
+            return -1;
+        }
+    }
     else
     {
-        if (og_plan_build(g, p->opt.relabel, p->opt.hub_threshold, p->push || p->dd, &p->plan))
+        const uint32_t rel = chain && p->opt.relabel == 1 ? 2u : p->opt.relabel;
+        if (og_plan_build(g, rel, chain ? OG_DEFAULT_SEQ_T : p->opt.hub_threshold, p->push || p->dd, &p->plan))
         {
             free(p);
             return -1;
         }
         p->own_plan = 1;
+    }
+    if (chain && og_pr_chain_rows(p->plan, &p->chain_rows))
+    {
+        ogpu_pr_free(p);
+        return -1;
     }
     p->seconds_plan = og_now() - t0;
     og_plan *pl = p->plan;
@@ -239,10 +298,13 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         }
         if (on) hot_grid = (uint32_t)og_num_cus();
     }
-    const uint32_t pull_blocks = hot_grid ? hot_grid : pl->n_chunk_blocks + pl->n_row_blocks;
+    /* chunk-wave blocks, or with chains one block per four rows of [0, C) */
+    const uint32_t chunk_blocks = p->chain_rows ? (pl->C + 3) / 4 : pl->n_chunk_blocks;
+    const uint32_t pull_blocks = hot_grid ? hot_grid : chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv, 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
+    p->parts_cap = parts;
     uint32_t fins = p->n_final_pull > p->n_final_vert ? p->n_final_pull : p->n_final_vert;
     p->stats_cap = (opt->max_iter ? opt->max_iter : 1) + 1;
 
@@ -305,6 +367,14 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     a->row_end = N;
     a->n_chunks = pl->n_chunks;
     a->n_chunk_blocks = pl->n_chunk_blocks;
+    if (p->chain_rows)
+    {
+        /* rows [0, C) are chains, four per 256-thread block; no chunk sums */
+        a->chain_rows = p->chain_rows;
+        a->H = 0;
+        a->n_chunks = 0;
+        a->n_chunk_blocks = chunk_blocks;
+    }
     a->n_row_blocks = pl->n_row_blocks;
     a->hot_grid = hot_grid;
     a->stop_l1 = opt->stop_rule == 1 || opt->ppr;
@@ -438,6 +508,11 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     {
         a.n_parts = og_pull_parts(&a);
         a.n_final_blocks = p->n_final_pull;
+        if (a.n_parts > p->parts_cap)
+        {
+            og_set_error("pull launch with %u partials, %u allocated", a.n_parts, p->parts_cap);
+            return -1;
+        }
         rc = og_launch_pull(&a, p->mode, row, s, e0, e1);
     }
     else
@@ -1097,7 +1172,9 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
     for (uint32_t g = 0; g < n; g++)
     {
         ogpu_pr_options o = opt;
-        o.hub_threshold = m->hub[g];
+        /* every row in CSR order by thread-per-row rows: the multi-graph
+         * kernels have no wavefront-chain items */
+        o.hub_threshold = m->hub[g] >= 0x80000000u ? 0x7FFFFFFFu : m->hub[g];
         if (og_pr_create_with(graphs[g], NULL, &o, &m->pr[g])) goto fail;
         ogpu_pr *p = m->pr[g];
         for (int q = 0; q < 2; q++)

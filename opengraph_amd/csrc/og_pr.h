@@ -55,6 +55,8 @@ typedef struct og_pr_dev
     uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
     uint32_t ppr_src;        /* new id of the restart vertex */
     uint32_t spmv;           /* pull adds the row sums onto y (SpMV): fp32 y in r, u64 in next */
+    const uint32_t *chain_rows; /* fp32 pull, every row exact: rows [0, C) longest first, one
+                                   wavefront each (n_chunk_blocks = ceil(C / 4), H = 0) */
     double epsilon;
     /* plan */
     const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;

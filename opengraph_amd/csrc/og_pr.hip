@@ -402,6 +402,74 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
     else og_chunk_body<MODE, HOT, OG_CH_SLOTS>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
 }
 
+/* Rows [0, C) in the reference's order (fp32 pull with every row exact):
+ * one wavefront per row, the longest rows first (chain_rows).  The lanes
+ * load and gather OG_CHAIN_U x 64 in-edges at a time -- the next group in
+ * flight while the current one is added -- and the wavefront adds the
+ * gathered values one lane after the other onto a running sum held alike
+ * in every lane: the reference's sequential `+=` over the row
+ * (pageRank.c:998-1011, SPMV.c:673-693) to the bit. */
+#define OG_CHAIN_U 4u
+
+template <bool HOT>
+__device__ __forceinline__ void og_chain_load(const og_pr_dev &a, uint32_t base, uint32_t e, const float *hot,
+                                              uint32_t nh, float (&val)[OG_CHAIN_U])
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const float *__restrict__ g = (const float *)a.g_in;
+    uint32_t idx[OG_CHAIN_U];
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+        idx[u] = __builtin_nontemporal_load(&a.cols[min(base + u * 64 + lane, e - 1)]);
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++) val[u] = og_gather<float, HOT>(g, hot, nh, idx[u]);
+}
+
+template <bool HOT>
+__device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, uint32_t cblock,
+                                              const float *hot = nullptr, uint32_t nh = 0)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
+    const uint32_t q = cblock * OG_WAVES + wave;
+    if (q >= a.C) return;
+    const uint32_t v = __builtin_amdgcn_readfirstlane(a.chain_rows[q]);
+    const uint32_t j = __builtin_amdgcn_readfirstlane(a.row_ptr[v]);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1]);
+    const float prev = a.r[v];
+    float y = a.spmv ? prev : 0.0f;
+    float cur[OG_CHAIN_U], nxt[OG_CHAIN_U];
+    og_chain_load<HOT>(a, j, e, hot, nh, cur);
+    for (uint32_t base = j; base < e; base += 64 * OG_CHAIN_U)
+    {
+        if (base + 64 * OG_CHAIN_U < e) og_chain_load<HOT>(a, base + 64 * OG_CHAIN_U, e, hot, nh, nxt);
+#pragma unroll
+        for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+        {
+            const uint32_t s = base + 64 * u;
+            if (s >= e) break;
+            const int x = __float_as_int(cur[u]);
+            if (e - s >= 64)
+            {
+#pragma unroll
+                for (int k = 0; k < 64; k++) y = y + __int_as_float(__builtin_amdgcn_readlane(x, k));
+            }
+            else
+            {
+                const int n = (int)(e - s);
+                for (int k = 0; k < n; k++) y = y + __int_as_float(__builtin_amdgcn_readlane(x, k));
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < OG_CHAIN_U; u++) cur[u] = nxt[u];
+    }
+    if (lane == 0)
+    {
+        if (a.spmv) og_spmv_store<OG_MODE_F32>(a, v, y, prev, true);
+        else og_epilogue_pf<OG_MODE_F32>(a, v, y, prev, a.outdeg[v], st);
+    }
+}
+
 /* In-edges [j, e) of one row added in CSR order onto s, B loaded and
  * gathered at a time.  Lanes past their row's end re-read its last edge and
  * add zero: x + 0 == x for every partial sum (PageRank's start at +0; the
@@ -482,8 +550,9 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    if (blockIdx.x < a.n_chunk_blocks) og_pull_chunks<MODE>(a, st, blockIdx.x);
-    else og_pull_rows<MODE>(a, st, blockIdx.x - a.n_chunk_blocks);
+    if (blockIdx.x >= a.n_chunk_blocks) og_pull_rows<MODE>(a, st, blockIdx.x - a.n_chunk_blocks);
+    else if (MODE == OG_MODE_F32 && a.chain_rows) og_pull_chain<false>(a, st, blockIdx.x);
+    else og_pull_chunks<MODE>(a, st, blockIdx.x);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {
@@ -518,8 +587,9 @@ __global__ __launch_bounds__(OG_HOT_SUBS *OG_BLOCK) void og_pull_hot_kernel(og_p
     const uint32_t q = threadIdx.x / OG_BLOCK;
     for (uint32_t w = blockIdx.x * OG_HOT_SUBS + q; w < items; w += gridDim.x * OG_HOT_SUBS)
     {
-        if (w < a.n_chunk_blocks) og_pull_chunks<MODE, true>(a, st, w, hot, nh);
-        else og_pull_rows<MODE, true>(a, st, w - a.n_chunk_blocks, hot, nh);
+        if (w >= a.n_chunk_blocks) og_pull_rows<MODE, true>(a, st, w - a.n_chunk_blocks, hot, nh);
+        else if (MODE == OG_MODE_F32 && a.chain_rows) og_pull_chain<true>(a, st, w, (const float *)hot, nh);
+        else og_pull_chunks<MODE, true>(a, st, w, hot, nh);
     }
     og_stat r = og_block_stat<OG_HOT_SUBS * OG_WAVES>(st);
     if (threadIdx.x == 0)

@@ -23,6 +23,7 @@ struct ogpu_pr
     int mode, push, quant, dd;
     og_pr_dev dev;
     uint32_t gv;              /* grid of the per-vertex kernels */
+    uint32_t parts_cap;       /* entries of the first-level partial arrays */
     uint32_t n_final_pull, n_final_vert;
     float *r;
     void *c[2];
@@ -56,6 +57,8 @@ struct ogpu_pr
     /* learned start: features (N x 4) and reduction scratch */
     float *feat;
     double *lscratch;
+    /* every-row-exact fp32 pull: rows [0, C) longest first (device) */
+    uint32_t *chain_rows;
 };
 
 /* Create a run of opt->variant on graph g; plan may be supplied (borrowed)

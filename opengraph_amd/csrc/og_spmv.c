@@ -133,6 +133,11 @@ static int og_spmv_launch_iter(ogpu_spmv *s, hipEvent_t e0, hipEvent_t e1)
     a.z_active = 0;
     a.n_parts = og_pull_parts(&a);
     a.n_final_blocks = p->n_final_pull;
+    if (a.n_parts > p->parts_cap)
+    {
+        og_set_error("SpMV pull launch with %u partials, %u allocated", a.n_parts, p->parts_cap);
+        return -1;
+    }
     return og_launch_pull(&a, s->mode, OG_NO_STATS, st, e0, e1);
 }
 
