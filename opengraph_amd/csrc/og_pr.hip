@@ -404,23 +404,29 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
 
 /* Rows [0, C) in the reference's order (fp32 pull with every row exact):
  * one wavefront per row, the longest rows first (chain_rows).  The lanes
- * load and gather OG_CHAIN_U x 64 in-edges at a time -- the next group in
- * flight while the current one is added -- and the wavefront adds the
- * gathered values one lane after the other onto a running sum held alike
- * in every lane: the reference's sequential `+=` over the row
- * (pageRank.c:998-1011, SPMV.c:673-693) to the bit. */
-#define OG_CHAIN_U 4u
+ * load and gather the row's in-edges OG_CHAIN_U x 64 at a time, and the
+ * wavefront adds the gathered values one lane after the other onto a
+ * running sum held alike in every lane: the reference's sequential `+=`
+ * over the row (pageRank.c:998-1011, SPMV.c:673-693) to the bit.  Two
+ * groups are in flight while one is added: the gathers of the next group
+ * and the indices of the one after it. */
+#define OG_CHAIN_U 8u
+#define OG_CHAIN_PRIO_EDGES 8192u
 
-template <bool HOT>
-__device__ __forceinline__ void og_chain_load(const og_pr_dev &a, uint32_t base, uint32_t e, const float *hot,
-                                              uint32_t nh, float (&val)[OG_CHAIN_U])
+__device__ __forceinline__ void og_chain_idx(const og_pr_dev &a, uint32_t base, uint32_t e,
+                                             uint32_t (&idx)[OG_CHAIN_U])
 {
     const uint32_t lane = threadIdx.x & 63;
-    const float *__restrict__ g = (const float *)a.g_in;
-    uint32_t idx[OG_CHAIN_U];
 #pragma unroll
     for (uint32_t u = 0; u < OG_CHAIN_U; u++)
         idx[u] = __builtin_nontemporal_load(&a.cols[min(base + u * 64 + lane, e - 1)]);
+}
+
+template <bool HOT>
+__device__ __forceinline__ void og_chain_gather(const og_pr_dev &a, const uint32_t (&idx)[OG_CHAIN_U],
+                                                const float *hot, uint32_t nh, float (&val)[OG_CHAIN_U])
+{
+    const float *__restrict__ g = (const float *)a.g_in;
 #pragma unroll
     for (uint32_t u = 0; u < OG_CHAIN_U; u++) val[u] = og_gather<float, HOT>(g, hot, nh, idx[u]);
 }
@@ -429,6 +435,7 @@ template <bool HOT>
 __device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, uint32_t cblock,
                                               const float *hot = nullptr, uint32_t nh = 0)
 {
+    constexpr uint32_t G = 64 * OG_CHAIN_U;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
     const uint32_t q = cblock * OG_WAVES + wave;
@@ -438,21 +445,39 @@ __device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, u
     const uint32_t e = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1]);
     const float prev = a.r[v];
     float y = a.spmv ? prev : 0.0f;
-    float cur[OG_CHAIN_U], nxt[OG_CHAIN_U];
-    og_chain_load<HOT>(a, j, e, hot, nh, cur);
-    for (uint32_t base = j; base < e; base += 64 * OG_CHAIN_U)
+    /* a chain thousands of edges long bounds the whole launch: its
+     * wavefront takes the SIMD's issue slots before the others */
+    const bool lead = e - j > OG_CHAIN_PRIO_EDGES;
+    if (lead) __builtin_amdgcn_s_setprio(3);
+    /* groups past the row's end re-read its last edge; they are never added */
+    uint32_t i1[OG_CHAIN_U], i2[OG_CHAIN_U];
+    float g0[OG_CHAIN_U], g1[OG_CHAIN_U];
+    og_chain_idx(a, j, e, i1);
+    og_chain_gather<HOT>(a, i1, hot, nh, g0);
+    og_chain_idx(a, j + G, e, i1);
+    for (uint32_t base = j; base < e; base += G)
     {
-        if (base + 64 * OG_CHAIN_U < e) og_chain_load<HOT>(a, base + 64 * OG_CHAIN_U, e, hot, nh, nxt);
+        og_chain_gather<HOT>(a, i1, hot, nh, g1);     /* group + 1 */
+        og_chain_idx(a, base + 2 * G, e, i2);         /* group + 2 */
 #pragma unroll
         for (uint32_t u = 0; u < OG_CHAIN_U; u++)
         {
             const uint32_t s = base + 64 * u;
             if (s >= e) break;
-            const int x = __float_as_int(cur[u]);
+            const int x = __float_as_int(g0[u]);
             if (e - s >= 64)
             {
+                /* eight lane reads into separate scalars, then their eight
+                 * adds: no read-after-write wait between each pair */
 #pragma unroll
-                for (int k = 0; k < 64; k++) y = y + __int_as_float(__builtin_amdgcn_readlane(x, k));
+                for (int k0 = 0; k0 < 64; k0 += 8)
+                {
+                    int t[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) t[k] = __builtin_amdgcn_readlane(x, k0 + k);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) y = y + __int_as_float(t[k]);
+                }
             }
             else
             {
@@ -461,8 +486,13 @@ __device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, u
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < OG_CHAIN_U; u++) cur[u] = nxt[u];
+        for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+        {
+            g0[u] = g1[u];
+            i1[u] = i2[u];
+        }
     }
+    if (lead) __builtin_amdgcn_s_setprio(0);
     if (lane == 0)
     {
         if (a.spmv) og_spmv_store<OG_MODE_F32>(a, v, y, prev, true);
