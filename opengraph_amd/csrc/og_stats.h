@@ -13,11 +13,11 @@
 extern "C" {
 #endif
 
-/* Edit-distance problems up to these k run with their diagonals in LDS
- * (u16 scores: 3 (k + 1) * 2 + 2 k * 4 bytes, at most 160 KB); larger ones
- * keep u32 diagonals in global memory. */
+/* Edit-distance problems up to OG_LEV_SMALL_K run on one wavefront (16
+ * rows per lane), up to OG_LEV_LDS_K on eight (24 rows per lane); larger
+ * ones walk anti-diagonals kept in global memory. */
 #define OG_LEV_SMALL_K 1024u
-#define OG_LEV_LDS_K 11700u
+#define OG_LEV_LDS_K 12288u
 
 typedef struct
 {

@@ -9,11 +9,12 @@
  *   - per problem the top-k slices (vertex ids, the ranks of the compared
  *     ranking's vertices in both runs, the ids as floats) packed at
  *     off_p = k_0 + ... + k_{p-1};
- *   - the edit distance of the two id lists: one block per problem walks
- *     the anti-diagonals of the (k+1) x (k+1) matrix, three diagonals in
- *     LDS (u16 scores, both lists beside them) or, above OG_LEV_LDS_K, in
- *     global memory; the reference's min3 tie rule is kept (a tie of
- *     deletion and insertion below substitution yields substitution);
+ *   - the edit distance of the two id lists: one block per problem sweeps
+ *     the (k+1) x (k+1) matrix column by column, each thread holding a
+ *     strip of rows in registers, skewed by one step per thread (above
+ *     OG_LEV_LDS_K: anti-diagonals in global memory); the reference's min3
+ *     tie rule is kept (a tie of deletion and insertion below substitution
+ *     yields substitution);
  *   - Kendall pair counts (s, ties in arr1, ties in arr2) for the float and
  *     the id arrays, blocks over groups of rows i, threads over j > i;
  *   - intersection and mismatch counts and the two average errors, whose
@@ -111,26 +112,73 @@ __device__ __forceinline__ void og_lev_diag(uint32_t d, uint32_t K, const uint32
     }
 }
 
-/* Problems with k <= KMAX: diagonals (u16) and both lists in LDS. */
-template <uint32_t KMAX>
-__global__ __launch_bounds__(OG_LEV_BLOCK) void og_lev_lds_kernel(og_stats_dev a, const uint32_t *__restrict__ plist)
+/* Problems with k <= R * 64 * W: one block of W wavefronts; thread t owns
+ * the R rows t*R+1 .. t*R+R of the matrix and keeps their current column
+ * in registers.  Thread t computes column j at step j + t - 1 (a skewed
+ * sweep): the cell above its strip comes from thread t - 1's last row of
+ * the step before, by a lane shift inside a wavefront and through LDS (one
+ * barrier per step) across wavefronts; the compared list sits in LDS, the
+ * strip's own list entries in registers. */
+template <uint32_t R, uint32_t W>
+__global__ __launch_bounds__(64 * W) void og_lev_strip_kernel(og_stats_dev a, const uint32_t *__restrict__ plist)
 {
-    __shared__ uint16_t dg[3][KMAX + 1];
-    __shared__ uint32_t s1[KMAX], s2[KMAX];
+    constexpr uint32_t T = 64 * W;
+    __shared__ uint32_t cols[R * T];
+    __shared__ uint32_t bot[2][W];
     const uint32_t p = plist[blockIdx.x];
     const uint32_t K = a.k[p], o = a.off[p];
-    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x)
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t i = t; i < K; i += T) cols[i] = a.ids[o + i];
+    uint32_t rid[R], col[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++)
     {
-        s1[i] = a.ref_ids[o + i];
-        s2[i] = a.ids[o + i];
+        const uint32_t i = t * R + r + 1;              /* 1-based row */
+        rid[r] = i <= K ? a.ref_ids[o + i - 1] : 0xFFFFFFFFu;
+        col[r] = i;                                    /* column 0: mat[i][0] = i */
     }
+    uint32_t top_prev = t * R;                         /* mat[t*R][0] */
+    uint32_t bottom = col[R - 1];
     __syncthreads();
-    for (uint32_t d = 0; d <= 2 * K; d++)
+    const uint32_t steps = K ? K + T - 1 : 0;
+    for (uint32_t s = 0; s < steps; s++)
     {
-        og_lev_diag<uint16_t>(d, K, s1, s2, dg[d % 3], dg[(d + 2) % 3], dg[(d + 1) % 3]);
-        __syncthreads();
+        uint32_t left = (uint32_t)__shfl_up((int)bottom, 1, 64);
+        if constexpr (W > 1)
+        {
+            if (lane == 63) bot[s & 1][w] = bottom;
+            __syncthreads();
+            if (lane == 0 && w > 0) left = bot[s & 1][w - 1];
+        }
+        const int j = (int)s - (int)t + 1;
+        if (j >= 1 && j <= (int)K)
+        {
+            const uint32_t top = t == 0 ? (uint32_t)j : left;   /* mat[t*R][j] */
+            const uint32_t cj = cols[j - 1];
+            uint32_t up = top, diag = top_prev;
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++)
+            {
+                const uint32_t v = og_min3_ref(up + 1u, col[r] + 1u, diag + (rid[r] == cj ? 0u : 1u));
+                diag = col[r];
+                col[r] = v;
+                up = v;
+            }
+            top_prev = top;
+            bottom = col[R - 1];
+        }
     }
-    if (threadIdx.x == 0) a.lev[p] = K ? dg[(2 * K) % 3][K] : 0u;
+    /* after the sweep every strip holds column K; row K's owner writes it */
+    if (K && t == (K - 1) / R)
+    {
+        const uint32_t rk = (K - 1) % R;
+        uint32_t d = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++)
+            if (r == rk) d = col[r];
+        a.lev[p] = d;
+    }
+    else if (!K && t == 0) a.lev[p] = 0;
 }
 
 /* Larger problems: u32 diagonals in global scratch (3 * (k + 1) entries). */
@@ -277,14 +325,16 @@ extern "C" int og_launch_stats_slices(const og_stats_dev *a, uint32_t total, hip
 
 extern "C" int og_launch_lev_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
 {
-    if (n) hipLaunchKernelGGL(og_lev_lds_kernel<OG_LEV_SMALL_K>, dim3(n), dim3(OG_LEV_BLOCK), 0, s, *a, plist);
-    return og_hip_check(hipGetLastError(), "og_lev_lds_kernel<small>");
+    static_assert(16 * 64 * 1 == OG_LEV_SMALL_K, "small edit-distance strips cover OG_LEV_SMALL_K");
+    if (n) hipLaunchKernelGGL((og_lev_strip_kernel<16, 1>), dim3(n), dim3(64), 0, s, *a, plist);
+    return og_hip_check(hipGetLastError(), "og_lev_strip_kernel<16, 1>");
 }
 
 extern "C" int og_launch_lev_large(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
 {
-    if (n) hipLaunchKernelGGL(og_lev_lds_kernel<OG_LEV_LDS_K>, dim3(n), dim3(OG_LEV_BLOCK), 0, s, *a, plist);
-    return og_hip_check(hipGetLastError(), "og_lev_lds_kernel<large>");
+    static_assert(24 * 64 * 8 == OG_LEV_LDS_K, "large edit-distance strips cover OG_LEV_LDS_K");
+    if (n) hipLaunchKernelGGL((og_lev_strip_kernel<24, 8>), dim3(n), dim3(512), 0, s, *a, plist);
+    return og_hip_check(hipGetLastError(), "og_lev_strip_kernel<24, 8>");
 }
 
 extern "C" int og_launch_lev_global(const og_stats_dev *a, uint32_t p, uint32_t *scratch, hipStream_t s)

@@ -229,6 +229,21 @@ def test_scale18_pull_vs_oracle(rmat18):
     assert res.iterations == 15
 
 
+@pytest.mark.parametrize("hot", [0, 1, 2])
+def test_scale18_every_row_exact_pull_bitexact(rmat18, hot):
+    """Every row in CSR order at scale 18: the rows above 16 in-edges run as
+    wavefront chains (the longest, tens of thousands of edges, with raised
+    issue priority); ranks, iterations and activeVertices equal the oracle's
+    sequential pull, itself byte-identical to the reference on the golden
+    graphs."""
+    g, o = rmat18
+    res, ranks, act, _ = _run(g, 0, 1e-7, 40, hub_threshold=ALL_EXACT, hot_cache=hot)
+    ref = po.pagerank(o, 0, 1e-7, 40, 8)
+    assert int(np.bincount(o["f_dst"]).max()) > 8192
+    assert np.array_equal(ranks.view(np.uint32), ref["ranks"].view(np.uint32)), hot
+    assert res.iterations == ref["iterations"] and list(act) == list(ref["active"])
+
+
 def test_scale18_fixed_point_bitexact_and_pull_equals_push(rmat18):
     g, o = rmat18
     _, a, act_a, _ = _run(g, 2, 1e-7, 60)
