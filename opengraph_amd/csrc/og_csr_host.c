@@ -511,7 +511,7 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
          * label of every input vertex (readEdgeListsMem copies them to the
          * inverse list, edgeList.c:718-727) */
         struct EdgeList *fe = h->sorted_edges_array, *ie = h->inverse_sorted_edges_array;
-        if (og_hip_check(hipMemcpy(fe->label_array, d_labels, (size_t)h->num_vertices * 4, hipMemcpyDeviceToHost),
+        if (og_hip_check(og_copy(fe->label_array, d_labels, (size_t)h->num_vertices * 4, hipMemcpyDeviceToHost),
                          "D2H labels"))
             og_die("graphCSRPreProcessingStep");
         og_dfree(d_labels);
@@ -643,14 +643,14 @@ uint32_t *radixSortEdgesByPageRank(float *pageRanks, uint32_t *labels, uint32_t 
 static uint32_t *og_up(const uint32_t *h, uint32_t n)
 {
     uint32_t *d = og_dalloc((size_t)(n ? n : 1) * 4);
-    if (!d || (n && og_hip_check(hipMemcpy(d, h, (size_t)n * 4, hipMemcpyHostToDevice), "H2D")))
+    if (!d || (n && og_hip_check(og_copy(d, h, (size_t)n * 4, hipMemcpyHostToDevice), "H2D")))
         og_die("upload");
     return d;
 }
 
 static void og_down(uint32_t *h, const uint32_t *d, uint32_t n)
 {
-    if (n && og_hip_check(hipMemcpy(h, d, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H")) og_die("download");
+    if (n && og_hip_check(og_copy(h, d, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H")) og_die("download");
 }
 
 static void og_reorder_banner(const char *what)

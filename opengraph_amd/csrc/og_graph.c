@@ -442,7 +442,7 @@ int ogpu_graph_write_csr(const ogpu_graph *gg, const char *fname)
     h[0] = E;
     h[1] = N;
     if (og_launch_csr_pack(arrays, N, E, d_body, og_stream()) ||
-        og_hip_check(hipMemcpy(h + 2, d_body, (words - 2) * 4, hipMemcpyDeviceToHost), "D2H .csr"))
+        og_hip_check(og_copy(h + 2, d_body, (words - 2) * 4, hipMemcpyDeviceToHost), "D2H .csr"))
         goto done;
     strcpy(path, fname);
     strcat(path, ".csr");
@@ -485,7 +485,7 @@ int ogpu_graph_read_csr(const char *path, ogpu_graph **out)
     if (!g || !d_body) goto done;
     uint32_t *arrays[10];
     og_graph_array_list(g, arrays);
-    if (og_hip_check(hipMemcpy(d_body, h, words * 4, hipMemcpyHostToDevice), "H2D .csr") ||
+    if (og_hip_check(og_copy(d_body, h, words * 4, hipMemcpyHostToDevice), "H2D .csr") ||
         og_launch_csr_unpack(d_body, N, E, arrays, og_stream()) ||
         og_hip_check(hipStreamSynchronize(og_stream()), "sync .csr"))
         goto done;

@@ -29,6 +29,10 @@ void *og_dalloc(size_t bytes);
 void og_dfree(void *p);
 /* Checks a HIP status; on failure records "<what>: <hip error>" and returns -1. */
 int og_hip_check(hipError_t e, const char *what);
+/* A copy ordered after all work already queued on the library stream and
+ * complete on return (hipMemcpy itself runs on the null stream, which the
+ * library's non-blocking stream does not wait for). */
+hipError_t og_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 #define OG_HIP(call) do { if (og_hip_check((call), #call)) return -1; } while (0)
 /* 128-byte aligned host allocation, as the reference's my_malloc
  * (myMalloc.c:24-67); exits on failure like the reference. */

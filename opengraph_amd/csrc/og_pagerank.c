@@ -160,7 +160,7 @@ static int og_pr_chain_rows(const og_plan *pl, uint32_t **out)
         return -1;
     }
     int rc = -1;
-    if (!og_hip_check(hipMemcpy(rp, pl->row_ptr, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr"))
+    if (!og_hip_check(og_copy(rp, pl->row_ptr, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr"))
     {
         /* counting sort by length, longest first; ties keep row order */
         uint32_t maxlen = 0;
@@ -173,7 +173,7 @@ static int og_pr_chain_rows(const og_plan *pl, uint32_t **out)
             for (uint32_t k = 0; k <= maxlen; k++) { uint32_t t = cnt[k]; cnt[k] = acc; acc += t; }
             for (uint32_t r = 0; r < C; r++) rows[cnt[maxlen - (rp[r + 1] - rp[r])]++] = r;
             free(cnt);
-            rc = og_hip_check(hipMemcpy(d, rows, (size_t)C * 4, hipMemcpyHostToDevice), "H2D chain rows");
+            rc = og_hip_check(og_copy(d, rows, (size_t)C * 4, hipMemcpyHostToDevice), "H2D chain rows");
         }
         else og_set_error("out of host memory");
     }
@@ -261,7 +261,7 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
             /* worth it when the cached sources send at least a fifth of the edges */
             uint32_t *deg = (uint32_t *)malloc((size_t)nh * 4);
             if (!deg) { og_set_error("out of host memory"); ogpu_pr_free(p); return -1; }
-            if (og_hip_check(hipMemcpy(deg, pl->outdeg, (size_t)nh * 4, hipMemcpyDeviceToHost), "D2H"))
+            if (og_hip_check(og_copy(deg, pl->outdeg, (size_t)nh * 4, hipMemcpyDeviceToHost), "D2H"))
             {
                 free(deg);
                 ogpu_pr_free(p);
@@ -289,7 +289,7 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         if (!on && p->mode == OG_MODE_FX64 && N > nh)
         {
             uint32_t hot_edges = 0;
-            if (og_hip_check(hipMemcpy(&hot_edges, pl->row_ptr + nh, 4, hipMemcpyDeviceToHost), "D2H"))
+            if (og_hip_check(og_copy(&hot_edges, pl->row_ptr + nh, 4, hipMemcpyDeviceToHost), "D2H"))
             {
                 ogpu_pr_free(p);
                 return -1;

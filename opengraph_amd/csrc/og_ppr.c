@@ -170,8 +170,8 @@ int ogpu_ppr_run(ogpu_ppr *p, ogpu_ppr_result *res)
         src_slot[j] = p->src_col[j];
     }
     a.kact = p->k;
-    if (og_hip_check(hipMemcpy(p->colmap, colmap, (size_t)p->k * 4, hipMemcpyHostToDevice), "H2D") ||
-        og_hip_check(hipMemcpy(p->src, src_slot, (size_t)p->k * 4, hipMemcpyHostToDevice), "H2D"))
+    if (og_hip_check(og_copy(p->colmap, colmap, (size_t)p->k * 4, hipMemcpyHostToDevice), "H2D") ||
+        og_hip_check(og_copy(p->src, src_slot, (size_t)p->k * 4, hipMemcpyHostToDevice), "H2D"))
         return -1;
     double t0 = og_ppr_now();
     if (og_hip_check(hipEventRecord(p->ev[0], s), "hipEventRecord")) return -1;
@@ -235,13 +235,13 @@ int ogpu_ppr_run(ogpu_ppr *p, ogpu_ppr_result *res)
         return -1;
     p->seconds_total = og_ppr_now() - t0;
     uint32_t it_col[OG_PPR_KMAX];
-    if (og_hip_check(hipMemcpy(it_col, p->iters, (size_t)p->k * 4, hipMemcpyDeviceToHost), "D2H iters"))
+    if (og_hip_check(og_copy(it_col, p->iters, (size_t)p->k * 4, hipMemcpyDeviceToHost), "D2H iters"))
         return -1;
     uint32_t executed = 0, converged = 0;
     for (uint32_t j = 0; j < p->k; j++)
         if (it_col[j] > executed) executed = it_col[j];
     uint64_t act = 0;
-    if (og_hip_check(hipMemcpy(&act, p->active, 8, hipMemcpyDeviceToHost), "D2H active")) return -1;
+    if (og_hip_check(og_copy(&act, p->active, 8, hipMemcpyDeviceToHost), "D2H active")) return -1;
     for (uint32_t j = 0; j < p->k; j++) converged += !((act >> j) & 1u);
     p->executed = executed;
     double ms = 0.0;
@@ -279,11 +279,11 @@ int ogpu_ppr_ranks(ogpu_ppr *p, float *x_out)
 
 int ogpu_ppr_columns(ogpu_ppr *p, uint32_t *iters, double *l1_hist, uint32_t max_rows)
 {
-    if (iters && og_hip_check(hipMemcpy(iters, p->iters, (size_t)p->k * 4, hipMemcpyDeviceToHost), "D2H"))
+    if (iters && og_hip_check(og_copy(iters, p->iters, (size_t)p->k * 4, hipMemcpyDeviceToHost), "D2H"))
         return -1;
     uint32_t rows = p->executed < max_rows ? p->executed : max_rows;
     if (l1_hist && rows &&
-        og_hip_check(hipMemcpy(l1_hist, p->l1, (size_t)rows * p->k * 8, hipMemcpyDeviceToHost), "D2H"))
+        og_hip_check(og_copy(l1_hist, p->l1, (size_t)rows * p->k * 8, hipMemcpyDeviceToHost), "D2H"))
         return -1;
     return (int)rows;
 }

@@ -56,6 +56,13 @@ int og_ctx_ensure(void)
 }
 
 hipStream_t og_stream(void) { return g_stream; }
+
+hipError_t og_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, g_stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(g_stream);
+}
 int og_num_cus(void) { return g_cus; }
 
 void *og_dalloc(size_t bytes)
