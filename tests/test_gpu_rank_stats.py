@@ -2,12 +2,14 @@
 
 Bar: every golden case (LAW-enron and R-MAT-16 pull rankings against
 fixed-point 8-bit, 32-bit and 8-bit quantized pull rankings; top-k 30 ..
-10000, 64, 512, 1000-wide chunks) equals the reference's
+10000, 64, 512, 1000-wide chunks; the 13-vertex graphbrew, smaller than
+every table size) equals the reference's
 collectStatsPageRank_topK field for field, as one batch per ranking pair and
 through the drop-in collectStatsPageRank_topK; the drop-in
 collectStatsPageRank writes the reference's .stats file and stdout byte for
 byte at verbosity 0, 1 and 2.  Beyond the fixtures: an edit-distance problem
-above the LDS size (global-memory diagonals), all 1000-wide chunks of a
+above the register-strip sizes (global-memory diagonals) and at both strip
+sizes, all 1000-wide chunks of a
 200k-vertex ranking and rankings with many ties, each against the oracle.
 """
 import ctypes as C
@@ -38,11 +40,11 @@ def _rankings(csr, code):
 @pytest.fixture(scope="module")
 def rank_cases(golden):
     out = {}
-    for gname in ("enron", "rmat16"):
+    for gname in ("graphbrew", "enron", "rmat16"):
         s, d = golden_input(gname, golden)
         csr = po.build_csr(s, d, 0, 4)
         out[gname] = {0: _rankings(csr, 0)}
-        for code in CODES:
+        for code in (CODES if gname != "graphbrew" else (11,)):
             out[gname][code] = _rankings(csr, code)
     return out
 
@@ -87,7 +89,7 @@ def _capture_stdout(fn, path):
     return open(path).read()
 
 
-@pytest.mark.parametrize("gname,code", [("enron", 11), ("rmat16", 4), ("rmat16", 13)])
+@pytest.mark.parametrize("gname,code", [("graphbrew", 11), ("enron", 11), ("rmat16", 4), ("rmat16", 13)])
 def test_dropin_collect_stats_file_and_stdout(rank_cases, tmp_path, gname, code):
     ref_pr, ref_order = rank_cases[gname][0]
     pr, order = rank_cases[gname][code]

@@ -2,7 +2,8 @@
 
 Pins: every golden case (LAW-enron and R-MAT-16 pull rankings against
 fixed-point 8-bit, 32-bit and 8-bit quantized pull rankings; top-k 30 ..
-10000, 64, 512 and 1000-wide chunks at lower num_vertices) equals the
+10000, 64, 512 and 1000-wide chunks at lower num_vertices; the 13-vertex
+graphbrew) equals the
 reference's own collectStatsPageRank_topK field for field
 (tests/golden/make_golden_rank_stats.py), and so does a live comparison on
 random rankings with ties when the reference is built.  Textbook checks:
@@ -28,18 +29,18 @@ def _rankings(csr, code):
 @pytest.fixture(scope="module")
 def rank_cases(golden):
     out = {}
-    for gname in ("enron", "rmat16"):
+    for gname in ("graphbrew", "enron", "rmat16"):
         s, d = golden_input(gname, golden)
         csr = po.build_csr(s, d, 0, 4)
         out[gname] = {0: _rankings(csr, 0)}
-        for code in CODES:
+        for code in (CODES if gname != "graphbrew" else (11,)):
             out[gname][code] = _rankings(csr, code)
     return out
 
 
 def test_golden_cases_match_reference(golden, rank_cases):
     cases = golden["rank_stats"]
-    assert len(cases) == 2 * 3 * 11
+    assert len(cases) == 2 * 3 * 11 + 4
     for key, want in cases.items():
         gname, code, k, m = key.split("/")
         code, k, m = int(code[1:]), int(k[1:]), int(m[1:])
