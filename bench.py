@@ -472,8 +472,8 @@ def main():
                          "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path does not "
-              f"span GPUs; see DESIGN.md for how it would shard).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library runs "
+              f"each graph on a single GPU (sharding by destination rows is not built).", file=sys.stderr)
         sys.exit(2)
 
     import opengraph_amd as og
