@@ -132,7 +132,8 @@ def ext_spmv(g, threads, csr):
     import opengraph_amd as og
     out = {}
     forms = ((0, "pull_fp32", {}, 10), (0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}, 3),
-             (1, "push_fp32", {}, 5), (2, "pull_fixed32", {}, 10), (3, "push_fixed64_gathered", {}, 10),
+             (1, "push_fp32", {}, 5), (1, "push_fp32_plain_atomics", {"hot_cache": 2}, 5),
+             (2, "pull_fixed32", {}, 10), (3, "push_fixed64_gathered", {}, 10),
              (3, "push_fixed64_scattered", {"scatter": 1}, 5))
     for code, name, kw, steps in forms:
         ms, kms, bpi = spmv_iter_ms(g, code, steps=steps, **kw)

@@ -277,7 +277,8 @@ void ogpu_ppr_free(ogpu_ppr *p);
  *     reference's CSR order (bit-exact); longer rows add their fixed-tree
  *     sum to y (hub_threshold 0xFFFFFFFF: every row bit-exact);
  *   1 push, fp32 atomics over the forward edges (unordered, as the
- *     reference's omp atomic);
+ *     reference's omp atomic), the hottest destinations summed per block
+ *     in LDS first;
  *   2 pull, Q16.16 inputs, u64 sums of (7 * x) >> 16 (bit-exact);
  *   3 push, Q32.32, u64 sums of (w * x) >> 32 (bit-exact): gathered over
  *     the in-edges by default, scattered with u64 atomics when scatter = 1. */
@@ -286,7 +287,8 @@ typedef struct
 {
     uint32_t code;           /* the reference's -p, 0..3 */
     uint32_t hub_threshold;  /* as ogpu_pr_options (0: default 16) */
-    uint32_t hot_cache;      /* as ogpu_pr_options */
+    uint32_t hot_cache;      /* as ogpu_pr_options, except that -p 1 keeps LDS accumulators by
+                                default (2 turns them off) */
     uint32_t scatter;        /* code 3: 1 scatters with atomics instead of gathering */
 } ogpu_spmv_options;
 void ogpu_spmv_default_options(ogpu_spmv_options *opt);

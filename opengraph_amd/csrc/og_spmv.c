@@ -78,7 +78,10 @@ int og_spmv_create_with(og_graph *g, og_plan *plan, const ogpu_spmv_options *opt
     po.variant = o.code == 0 ? 0 : o.code == 1 ? 1 : o.code == 2 ? 9 : (s->scatter ? 3 : 2);
     po.max_iter = 1;
     po.hub_threshold = o.hub_threshold;
-    po.hot_cache = o.hot_cache;
+    /* -p 1: the fp32 push adds the hottest destinations in LDS and flushes
+     * once per block by default.  Its sums are unordered in the reference
+     * too (omp atomic); block partials only make them closer to exact. */
+    po.hot_cache = o.code == 1 && o.hot_cache == 0 ? 1 : o.hot_cache;
     if (og_pr_create_with(g, plan, &po, &s->p))
     {
         free(s);
