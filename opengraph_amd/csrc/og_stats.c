@@ -16,6 +16,8 @@
 #include "opengraph_amd.h"
 #include "opengraph_amd_ext.h"
 
+_Static_assert(OG_KT_SMALL_K == OG_LEV_SMALL_K, "the small problems share one list");
+
 /* graphStats.c:472-486 from the pair counts. */
 static float og_kendall_cor(uint32_t K, long long s, long long m1, long long m2)
 {
@@ -47,7 +49,7 @@ int ogpu_rank_stats_device(const float *d_ref_pr, const uint32_t *d_ref_order, c
         k[p] = topk[p] < num_vertices[p] ? topk[p] : num_vertices[p];   /* graphStats.c:588-589 */
         off[p] = (uint32_t)total;
         total += k[p];
-        groups += (k[p] + 7) / 8;
+        if (k[p] > OG_KT_SMALL_K) groups += (k[p] + 7) / 8;
         if (k[p] <= OG_LEV_SMALL_K) n_small++;
         else if (k[p] <= OG_LEV_LDS_K) n_large++;
         else { n_glob++; if (k[p] > kmax) kmax = k[p]; }
@@ -81,7 +83,7 @@ int ogpu_rank_stats_device(const float *d_ref_pr, const uint32_t *d_ref_order, c
         }
         uint64_t gi = 0;
         for (uint32_t p = 0; p < P; p++)
-            for (uint32_t i = 0; i < k[p]; i += 8)
+            for (uint32_t i = 0; k[p] > OG_KT_SMALL_K && i < k[p]; i += 8)
             {
                 grp[2 * gi] = p;
                 grp[2 * gi + 1] = i;
@@ -133,6 +135,7 @@ int ogpu_rank_stats_device(const float *d_ref_pr, const uint32_t *d_ref_order, c
         og_launch_stats_inverse(&a, s) || og_launch_stats_slices(&a, (uint32_t)total, s) ||
         og_launch_lev_small(&a, (const uint32_t *)dmem[5], n_small, s) ||
         og_launch_lev_large(&a, (const uint32_t *)dmem[5] + n_small, n_large, s) ||
+        og_launch_kendall_small(&a, (const uint32_t *)dmem[5], n_small, s) ||
         og_launch_kendall(&a, dmem[6], (uint32_t)groups, s) || og_launch_topk_sums(&a, s))
         goto done;
     for (uint32_t g = 0; g < n_glob; g++)

@@ -39,7 +39,10 @@ int og_launch_stats_slices(const og_stats_dev *a, uint32_t total, hipStream_t s)
 int og_launch_lev_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s);
 int og_launch_lev_large(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s);
 int og_launch_lev_global(const og_stats_dev *a, uint32_t p, uint32_t *scratch, hipStream_t s);
-/* groups: n_groups (problem, first row) pairs of u32 */
+/* Kendall counts: problems with k <= OG_KT_SMALL_K one block each (plist),
+ * larger ones by groups: n_groups (problem, first row) pairs of u32 */
+#define OG_KT_SMALL_K 1024u
+int og_launch_kendall_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s);
 int og_launch_kendall(const og_stats_dev *a, const void *groups, uint32_t n_groups, hipStream_t s);
 int og_launch_topk_sums(const og_stats_dev *a, hipStream_t s);
 

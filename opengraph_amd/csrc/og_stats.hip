@@ -16,7 +16,8 @@
  *     tie rule is kept (a tie of deletion and insertion below substitution
  *     yields substitution);
  *   - Kendall pair counts (s, ties in arr1, ties in arr2) for the float and
- *     the id arrays, blocks over groups of rows i, threads over j > i;
+ *     the id arrays: one block per small problem with its arrays in LDS,
+ *     or blocks over groups of rows i, threads over j > i;
  *   - intersection and mismatch counts and the two average errors, whose
  *     double sums run in position order (as the reference's loops).
  */
@@ -264,6 +265,41 @@ __global__ __launch_bounds__(SB) void og_kendall_kernel(og_stats_dev a, const ui
     og_kt_flush(a.kt + 6 * p + 5, m2r, sh);
 }
 
+/* Problems with k <= OG_KT_SMALL_K: one block each, the four arrays staged
+ * in LDS; thread t takes the rows t, t + 256, ... against every later
+ * position. */
+__global__ __launch_bounds__(SB) void og_kendall_small_kernel(og_stats_dev a, const uint32_t *__restrict__ plist)
+{
+    __shared__ float s1[OG_KT_SMALL_K], s2[OG_KT_SMALL_K], q1[OG_KT_SMALL_K], q2[OG_KT_SMALL_K];
+    __shared__ long long sh[SB / 64];
+    const uint32_t p = plist[blockIdx.x];
+    const uint32_t K = a.k[p], o = a.off[p];
+    for (uint32_t i = threadIdx.x; i < K; i += SB)
+    {
+        s1[i] = a.ref_f[o + i];
+        s2[i] = a.f[o + i];
+        q1[i] = a.ref_re[o + i];
+        q2[i] = a.re[o + i];
+    }
+    __syncthreads();
+    int sf = 0, m1f = 0, m2f = 0, sr = 0, m1r = 0, m2r = 0;
+    for (uint32_t i = threadIdx.x; i < K; i += SB)
+    {
+        const float x1 = s1[i], x2 = s2[i], y1 = q1[i], y2 = q2[i];
+        for (uint32_t j = i + 1; j < K; j++)
+        {
+            og_kt_pair(x1, s1[j], x2, s2[j], sf, m1f, m2f);
+            og_kt_pair(y1, q1[j], y2, q2[j], sr, m1r, m2r);
+        }
+    }
+    og_kt_flush(a.kt + 6 * p + 0, sf, sh);
+    og_kt_flush(a.kt + 6 * p + 1, m1f, sh);
+    og_kt_flush(a.kt + 6 * p + 2, m2f, sh);
+    og_kt_flush(a.kt + 6 * p + 3, sr, sh);
+    og_kt_flush(a.kt + 6 * p + 4, m1r, sh);
+    og_kt_flush(a.kt + 6 * p + 5, m2r, sh);
+}
+
 /* graphStats.c:500-578: per problem, one block.  Counts reduce in any
  * order; the two error sums are formed in position order: the block writes
  * a tile of terms to LDS, thread 0 adds them in sequence. */
@@ -341,6 +377,12 @@ extern "C" int og_launch_lev_global(const og_stats_dev *a, uint32_t p, uint32_t 
 {
     hipLaunchKernelGGL(og_lev_global_kernel, dim3(1), dim3(OG_LEV_BLOCK), 0, s, *a, p, scratch);
     return og_hip_check(hipGetLastError(), "og_lev_global_kernel");
+}
+
+extern "C" int og_launch_kendall_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
+{
+    if (n) hipLaunchKernelGGL(og_kendall_small_kernel, dim3(n), dim3(SB), 0, s, *a, plist);
+    return og_hip_check(hipGetLastError(), "og_kendall_small_kernel");
 }
 
 extern "C" int og_launch_kendall(const og_stats_dev *a, const void *groups, uint32_t n_groups, hipStream_t s)
