@@ -57,15 +57,18 @@ def _ref_csr(g):
     return po, po.CSR(g.arrays(), g.num_vertices, g.num_edges)
 
 
-def cpu_time(g, code, iterations, threads, csr=None):
+def cpu_time(g, code, iterations, threads, csr=None, ranks=False):
     """Seconds per iteration of the reference's compiled PageRank variant
-    (or of the oracle port when the reference could not be built)."""
+    (or of the oracle port when the reference could not be built); with
+    ranks, also its normalized ranks after those iterations."""
     po, c = _ref_csr(g) if csr is None else csr
     kind = "reference" if po.ref_available() else "port"
     if kind == "reference":
         r = po.ref_pagerank(c, code, 0.0, iterations, threads)
     else:
         r = po.pagerank(c, code, 0.0, iterations, threads)
+    if ranks:
+        return r["seconds"] / iterations, kind, r["ranks"]
     return r["seconds"] / iterations, kind
 
 
@@ -90,6 +93,7 @@ def ext_variants(g, threads, csr, do_cpu):
     """Pull/push/fixed-point on the headline graph, each against the host run."""
     out, cpu = {}, {}
     for code, name, kw in ((0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}),
+                           (0, "pull_fp32_exact_above_4096", {"exact_above": 4096}),
                            (1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
                            (2, "pull_fixed64", {}), (3, "push_fixed64", {}), (11, "pull_fixed8", {})):
         ms, kms, bpi = gpu_iter_ms(g, code, **kw)
@@ -546,7 +550,7 @@ def main():
     if not args.no_cpu:
         try:
             csr = _ref_csr(g)
-            s, kind = cpu_time(g, 0, args.cpu_iterations, threads, csr)
+            s, kind, ref_ranks = cpu_time(g, 0, args.cpu_iterations, threads, csr, ranks=True)
             cpu = dict(value=round(E / s / 1e9, 4), unit="GTEPS", cores=threads, kind=kind,
                        sample=f"R-MAT-{scale} pull, {args.cpu_iterations} full iterations (epsilon 0) of "
                               + ("the reference pageRankPullGraphCSR compiled from its sources"
@@ -554,6 +558,25 @@ def main():
                               + f", OpenMP on {threads} threads, timed by its own timer: {s:.3f} s/iteration")
             extra["speedup_vs_cpu_per_iteration"] = round(s / step_s, 1)
             log(f"cpu baseline ({kind}, {threads} threads): {s:.3f} s/iter")
+            # full-size parity on the same iterations: the GPU's ranks against the
+            # reference's, and both against the FP64 oracle of the same map
+            po, c = csr
+            x64, _, _ = po.pagerank_fp64(c, 0.15, 0.0, args.cpu_iterations, threads)
+            par = dict(iterations=args.cpu_iterations,
+                       l1_vs_fp64_reference=float(np.abs(ref_ranks.astype(np.float64) - x64).sum()))
+            for form, kw in (("default", {}), ("exact_above_4096", {"exact_above": 4096}),
+                             ("every_row_exact", {"hub_threshold": 1 << 31})):
+                pr3 = og.PageRank(g, 0, args.cpu_iterations, 0.0, **kw)
+                pr3.run()
+                gpu_ranks = pr3.ranks()
+                pr3.free()
+                rel = np.abs(gpu_ranks.astype(np.float64) - ref_ranks) / np.maximum(ref_ranks.astype(np.float64), 1e-300)
+                par[form] = dict(
+                    max_rel_err_vs_reference=float(rel.max()),
+                    ranks_bit_identical_fraction=float(np.mean(gpu_ranks.view(np.uint32) == ref_ranks.view(np.uint32))),
+                    l1_vs_fp64=float(np.abs(gpu_ranks.astype(np.float64) - x64).sum()))
+            extra["parity_full_size"] = par
+            log(f"full-size parity: {extra['parity_full_size']}")
         except Exception as e:   # the baseline is reported, never required
             log("cpu baseline failed:", e)
 

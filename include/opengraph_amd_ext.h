@@ -125,6 +125,13 @@ typedef struct
                                partials are more accurate than the reference's running
                                fp32 sums and may differ from them by ~1e-5 relative on
                                rows with thousands of in-edges */
+    uint32_t exact_above;   /* fp32 pull (variant 0, not personalized, relabel 1 or 2): the
+                               rows of more than max(exact_above, 512) in-edges are summed
+                               in the reference's order, one wavefront each, instead of in
+                               chunks (bit-exact with the reference's running sums on equal
+                               inputs); 0 off.  Such rows carry the largest fp32 drift of
+                               the reference's sums.  The pull then runs without the LDS
+                               copy of hot_cache */
 } ogpu_pr_options;
 
 typedef struct

@@ -175,9 +175,12 @@ void og_plan_destroy(og_plan *p)
     free(p);
 }
 
-int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_forward,
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, uint32_t chain_t, int need_forward,
                   og_plan **out)
 {
+    /* chains are class-0 rows at the front, which needs the relabelling */
+    if (!relabel) chain_t = 0;
+    if (chain_t && chain_t < OG_CH) chain_t = OG_CH;
     *out = NULL;
     if (og_ctx_ensure()) return -1;
     hipStream_t s = og_stream();
@@ -190,7 +193,7 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
     p->seq_t = seq_t;
     size_t nb = (size_t)N * 4;
     uint32_t *key = og_dalloc(nb), *key2 = og_dalloc(nb), *val2 = og_dalloc(nb);
-    uint32_t *indeg_new = og_dalloc(nb + 4), *d_cnt = og_dalloc(16);
+    uint32_t *indeg_new = og_dalloc(nb + 4), *d_cnt = og_dalloc(32);
     size_t scr = og_radix_scratch_bytes(N);
     size_t scan_scr = og_scan_scratch_bytes(N + 1);
     void *scratch = og_dalloc(scr > scan_scr ? scr : scan_scr);
@@ -206,7 +209,7 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
 
     /* 1. new order: by row class, then out-degree (in-degree for the rows
      *    one thread sums) descending, stable */
-    if (og_launch_plan_keys(N, g->f_out_degree, g->i_out_degree, seq_t, relabel, key, p->new2old, s))
+    if (og_launch_plan_keys(N, g->f_out_degree, g->i_out_degree, seq_t, chain_t, relabel, key, p->new2old, s))
         goto done;
     {
         uint32_t *ka = key2, *va = val2, *kb = key, *vb = p->new2old;
@@ -220,34 +223,40 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
         }
     }
     /* 2. inverse permutation, degrees in new order, class sizes */
-    if (og_launch_fill_u32(d_cnt, 0, 4, s)) goto done;
-    if (og_launch_plan_perm(N, p->new2old, g->f_out_degree, g->i_out_degree, seq_t, p->old2new,
+    if (og_launch_fill_u32(d_cnt, 0, 5, s)) goto done;
+    if (og_launch_plan_perm(N, p->new2old, g->f_out_degree, g->i_out_degree, seq_t, chain_t, p->old2new,
                             p->outdeg, indeg_new, d_cnt, s))
         goto done;
     /* 3. row pointers of the relabelled inverse CSR */
     if (og_scan_exclusive(indeg_new, p->row_ptr, N, scratch, s)) goto done;
-    uint32_t cls[4] = {0, 0, 0, 0};
-    if (og_hip_check(hipMemcpyAsync(cls, d_cnt, 16, hipMemcpyDeviceToHost, s), "copy class sizes") ||
+    uint32_t cls[5] = {0, 0, 0, 0, 0};
+    if (og_hip_check(hipMemcpyAsync(cls, d_cnt, 20, hipMemcpyDeviceToHost, s), "copy class sizes") ||
         og_hip_check(hipStreamSynchronize(s), "sync plan"))
         goto done;
     const uint32_t H = cls[0], C = H + cls[1];
     p->H = H;
     p->C = C;
     p->Z = C + cls[2];
+    p->L = cls[4];
     /* 4. relabelled in-neighbour lists, reference order kept */
     if (E && og_launch_plan_copy(H, N, p->new2old, p->old2new, g->i_edges_idx, g->i_dst, p->row_ptr,
                                  p->cols, s))
         goto done;
-    /* 4b. rows [0, H) (several chunks; their sums are a fixed tree, not the
+    /* 4b. rows [L, H) (several chunks; their sums are a fixed tree, not the
      *     CSR order): in-neighbours sorted by source id, so the 64 lanes of
      *     a chunk gather neighbouring contributions -- with the degree
-     *     order, often the same cache line.  relabel 2 keeps CSR order. */
-    if (relabel == 1 && H)
+     *     order, often the same cache line.  relabel 2 keeps CSR order, and
+     *     so do the chains [0, L).  Edges [EL, EH) are sorted in place. */
+    if (relabel == 1 && H > p->L)
     {
-        uint32_t EH = 0;
+        const uint32_t L = p->L;
+        uint32_t EH = 0, EL = 0;
         if (og_hip_check(hipMemcpyAsync(&EH, p->row_ptr + H, 4, hipMemcpyDeviceToHost, s), "copy") ||
+            og_hip_check(hipMemcpyAsync(&EL, p->row_ptr + L, 4, hipMemcpyDeviceToHost, s), "copy") ||
             og_hip_check(hipStreamSynchronize(s), "sync plan"))
             goto done;
+        uint32_t *cols_h = p->cols + EL;
+        EH -= EL;
         uint32_t *rid = og_dalloc((size_t)EH * 4 + 4), *ka = og_dalloc((size_t)EH * 4 + 4),
                  *va = og_dalloc((size_t)EH * 4 + 4), *kb = og_dalloc((size_t)EH * 4 + 4),
                  *vb = og_dalloc((size_t)EH * 4 + 4);
@@ -255,13 +264,14 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
         int bad = !rid || !ka || !va || !kb || !vb || !rscr;
         uint32_t nbits = 1, hbits = 1;
         while (nbits < 32 && (1ull << nbits) < N) nbits++;
-        while (hbits < 32 && (1ull << hbits) < H) hbits++;
+        while (hbits < 32 && (1ull << hbits) < H - L) hbits++;
         int in_b = 0;
         if (!bad)
         {
-            /* by source (stable), then by row (stable): (row, source) order */
-            bad = og_launch_plan_row_ids(H, p->row_ptr, rid, s) ||
-                  og_radix_sort_pairs(p->cols, rid, ka, va, kb, vb, EH, nbits, rscr, s, &in_b);
+            /* by source (stable), then by row (stable): (row, source) order;
+             * row ids are relative to L, at edge offsets relative to EL */
+            bad = og_launch_plan_row_ids_at(H - L, p->row_ptr + L, EL, rid, s) ||
+                  og_radix_sort_pairs(cols_h, rid, ka, va, kb, vb, EH, nbits, rscr, s, &in_b);
         }
         if (!bad)
         {
@@ -274,7 +284,7 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
             if (!bad)
             {
                 const uint32_t *src_sorted = in_b2 ? k1 : oa_v;
-                bad = og_hip_check(hipMemcpyAsync(p->cols, src_sorted, (size_t)EH * 4, hipMemcpyDeviceToDevice, s),
+                bad = og_hip_check(hipMemcpyAsync(cols_h, src_sorted, (size_t)EH * 4, hipMemcpyDeviceToDevice, s),
                                    "copy sorted rows") ||
                       og_hip_check(hipStreamSynchronize(s), "sync plan");
             }
@@ -296,6 +306,9 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
             goto done;
         }
         p->n_chunks = n_chunks;
+        if (p->L && og_hip_check(hipMemcpyAsync(&p->ch_first, first + p->L, 4, hipMemcpyDeviceToHost, s),
+                                 "copy") == 0)
+            og_hip_check(hipStreamSynchronize(s), "sync plan");
         p->chunk_base = og_dalloc((size_t)(H + 1) * 4);
         p->ch_row = og_dalloc((size_t)n_chunks * 4 + 4);
         p->ch_edge = og_dalloc((size_t)(n_chunks + 1) * 4);
@@ -306,7 +319,7 @@ int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_
                   og_hip_check(hipStreamSynchronize(s), "sync plan");
         og_dfree(first);
         if (bad) goto done;
-        uint32_t waves = (n_chunks + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE;
+        uint32_t waves = (n_chunks - p->ch_first + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE;
         p->n_chunk_blocks = (waves + 3) / 4;
         p->n_row_blocks = (uint32_t)(((uint64_t)(N - C) + 255) / 256);
     }

@@ -141,7 +141,10 @@ void og_graph_destroy(og_graph *g);
  *   [C, Z)  1..seq_t in-edges: one thread sums the row in CSR order.
  *   [Z, N)  no in-edges: epilogue only.
  * Rows with at most seq_t in-edges therefore add in exactly the reference's
- * order; chunked rows add in a fixed tree (deterministic). */
+ * order; chunked rows add in a fixed tree (deterministic).  With chain_t
+ * (>= OG_CH, relabel 1 or 2), the rows of more than chain_t in-edges come
+ * first, [0, L), longest first: the fp32 pull sums each of them in the
+ * reference's order with one wavefront and skips their chunks. */
 #define OG_CH 512u            /* edges per wavefront chunk: 64 lanes x 8 */
 #define OG_CH_PER_WAVE 4u     /* consecutive chunks one wavefront processes */
 #define OG_SEQ_T 32u          /* seq_t of the batched-PPR plan */
@@ -157,11 +160,14 @@ typedef struct og_plan
     uint32_t *cols;              /* E, in-neighbours as new ids, reference order kept */
     uint32_t *outdeg;            /* N, forward out-degree in new order */
     uint32_t H, C, Z;            /* class boundaries, see above */
+    uint32_t L;                  /* rows [0, L) of class 0: more than chain_t in-edges, in-degree
+                                    descending, reference order kept (chain_t 0: L = 0) */
+    uint32_t ch_first;           /* first chunk of row L */
     uint32_t n_chunks;           /* chunks of rows [0, C) */
     uint32_t *chunk_base;        /* H+1: first chunk of multi-chunk row h */
     uint32_t *ch_row;            /* n_chunks: row of the chunk | OG_CH_MULTI */
     uint32_t *ch_edge;           /* n_chunks+1: first edge of the chunk */
-    uint32_t n_chunk_blocks;     /* pull blocks over the chunks */
+    uint32_t n_chunk_blocks;     /* pull blocks over the chunks [ch_first, n_chunks) */
     uint32_t n_row_blocks;       /* pull blocks over rows [C, N), one row per thread */
     /* forward side for push: source and destination of each forward edge, new ids */
     uint32_t *push_src, *push_dst;
@@ -170,7 +176,7 @@ typedef struct og_plan
     uint32_t *f_cols;            /* E */
 } og_plan;
 
-int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, int need_push,
+int og_plan_build(const og_graph *g, uint32_t relabel, uint32_t seq_t, uint32_t chain_t, int need_push,
                   og_plan **out);
 void og_plan_destroy(og_plan *p);
 

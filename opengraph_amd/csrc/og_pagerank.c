@@ -25,6 +25,11 @@
  * fastest on R-MAT 24 (1.38-1.40 ms against 1.47 at 32 and 1.50 at 4) and
  * on Erdos-Renyi 24 (4.80-4.87 ms against 5.08 at 32). */
 #define OG_DEFAULT_SEQ_T 16u
+/* every-row-exact fp32 pull: rows of at most this many in-edges are summed
+ * one per thread, the longer ones as wavefront chains */
+#ifndef OG_EXACT_SEQ_T
+#define OG_EXACT_SEQ_T 16u
+#endif
 
 /* Blocks of a final reduction over n_parts first-level partials that also
  * finalizes `rows` long rows (a wavefront each): enough that each block
@@ -208,9 +213,11 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     og_variant_mode(opt->variant, &p->mode, &p->push, &p->quant, &p->dd);
     const uint32_t N = g->N, E = g->E;
     /* fp32 pull with every row in the reference's order: the rows of more
-     * than OG_DEFAULT_SEQ_T in-edges become wavefront chains over CSR-order
+     * than OG_EXACT_SEQ_T in-edges become wavefront chains over CSR-order
      * rows (relabel 1's sorted long rows are kept in CSR order instead) */
     const int chain = opt->variant == 0 && opt->hub_threshold >= 0x80000000u;
+    /* otherwise, with exact_above, only the longest rows become chains */
+    const uint32_t chain_t = !chain && opt->variant == 0 && !opt->ppr ? opt->exact_above : 0;
     double t0 = og_now();
     if (plan)
     {
@@ -225,7 +232,8 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     else
     {
         const uint32_t rel = chain && p->opt.relabel == 1 ? 2u : p->opt.relabel;
-        if (og_plan_build(g, rel, chain ? OG_DEFAULT_SEQ_T : p->opt.hub_threshold, p->push || p->dd, &p->plan))
+        if (og_plan_build(g, rel, chain ? OG_EXACT_SEQ_T : p->opt.hub_threshold, chain_t, p->push || p->dd,
+                          &p->plan))
         {
             free(p);
             return -1;
@@ -252,7 +260,10 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
      * contributions: by default when the graph is several times larger
      * than that copy */
     uint32_t hot_grid = 0;
-    if (!p->push && !p->dd && opt->hot_cache != 2)
+    /* (pulls with chains run without it: a chain wavefront sharing the
+     * persistent blocks' static split holds back their other work) */
+    const int has_chains = p->chain_rows || (p->plan->L && opt->variant == 0 && !opt->ppr);
+    if (!p->push && !p->dd && opt->hot_cache != 2 && !has_chains)
     {
         int on = opt->hot_cache == 1;
         const uint32_t nh = (uint32_t)(OG_HOT_BYTES / og_elem_bytes(p->mode));
@@ -298,9 +309,24 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         }
         if (on) hot_grid = (uint32_t)og_num_cus();
     }
-    /* chunk-wave blocks, or with chains one block per four rows of [0, C) */
-    const uint32_t chunk_blocks = p->chain_rows ? (pl->C + 3) / 4 : pl->n_chunk_blocks;
-    const uint32_t pull_blocks = hot_grid ? hot_grid : chunk_blocks + pl->n_row_blocks;
+    /* chain blocks (four rows each) and chunk-wave blocks: every row exact,
+     * chains [0, C) and no chunks; fp32 pull on a plan with chains [0, L),
+     * their chunks skipped; otherwise every chunk */
+    uint32_t n_chains = 0, ch_first = 0, chunk_blocks = pl->n_chunk_blocks;
+    if (p->chain_rows)
+    {
+        n_chains = pl->C;
+        chunk_blocks = 0;
+    }
+    else if (pl->L && opt->variant == 0 && !opt->ppr)
+    {
+        n_chains = pl->L;
+        ch_first = pl->ch_first;
+    }
+    else if (pl->L)
+        chunk_blocks = ((pl->n_chunks + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE + 3) / 4;
+    const uint32_t chain_blocks = (n_chains + 3) / 4;
+    const uint32_t pull_blocks = hot_grid ? hot_grid : chain_blocks + chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv, 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
@@ -366,14 +392,16 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     a->Z = pl->Z;
     a->row_end = N;
     a->n_chunks = pl->n_chunks;
-    a->n_chunk_blocks = pl->n_chunk_blocks;
+    a->n_chunk_blocks = chunk_blocks;
+    a->n_chains = n_chains;
+    a->n_chain_blocks = chain_blocks;
+    a->ch_first = ch_first;
     if (p->chain_rows)
     {
         /* rows [0, C) are chains, four per 256-thread block; no chunk sums */
         a->chain_rows = p->chain_rows;
         a->H = 0;
         a->n_chunks = 0;
-        a->n_chunk_blocks = chunk_blocks;
     }
     a->n_row_blocks = pl->n_row_blocks;
     a->hot_grid = hot_grid;
@@ -855,7 +883,7 @@ int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_pl
         {
             og_plan_destroy(g_cache.plan);
             g_cache.plan = NULL;
-            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_SEQ_T, 1, &g_cache.plan)) return -1;
+            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_SEQ_T, 0, 1, &g_cache.plan)) return -1;
         }
         *g = g_cache.g;
         *plan = g_cache.plan;
@@ -879,7 +907,7 @@ int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_pl
     ogpu_graph *gg = NULL;
     if (ogpu_graph_from_arrays(graph->num_vertices, graph->num_edges, arrays, &gg)) return -1;
     og_plan *pl = NULL;
-    if (og_plan_build((og_graph *)gg, 1, OG_DEFAULT_SEQ_T, need_forward, &pl))
+    if (og_plan_build((og_graph *)gg, 1, OG_DEFAULT_SEQ_T, 0, need_forward, &pl))
     {
         ogpu_graph_free(gg);
         return -1;
@@ -1175,6 +1203,7 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
         /* every row in CSR order by thread-per-row rows: the multi-graph
          * kernels have no wavefront-chain items */
         o.hub_threshold = m->hub[g] >= 0x80000000u ? 0x7FFFFFFFu : m->hub[g];
+        o.exact_above = 0;
         if (og_pr_create_with(graphs[g], NULL, &o, &m->pr[g])) goto fail;
         ogpu_pr *p = m->pr[g];
         for (int q = 0; q < 2; q++)

@@ -19,18 +19,20 @@ int og_scan_exclusive(const uint32_t *d_in, uint32_t *d_out, uint32_t n, void *d
                       hipStream_t s);
 /* Row class of an in-degree (0..3, the order of the plan's row ranges). */
 int og_launch_plan_keys(uint32_t N, const uint32_t *outdeg, const uint32_t *indeg, uint32_t seq_t,
-                        uint32_t relabel, uint32_t *key, uint32_t *val, hipStream_t s);
-/* old2new, degrees in new order; class_cnt[k] += rows of class k. */
+                        uint32_t chain_t, uint32_t relabel, uint32_t *key, uint32_t *val, hipStream_t s);
+/* old2new, degrees in new order; class_cnt[k] += rows of class k, and
+ * class_cnt[4] += rows of more than chain_t in-edges (chain_t > 0). */
 int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *outdeg_old,
-                        const uint32_t *indeg_old, uint32_t seq_t, uint32_t *old2new,
+                        const uint32_t *indeg_old, uint32_t seq_t, uint32_t chain_t, uint32_t *old2new,
                         uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *class_cnt,
                         hipStream_t s);
 /* cols of rows [0, N): a block per row below H, a wavefront per row above. */
 int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2old, const uint32_t *old2new,
                         const uint32_t *idx_old, const uint32_t *lists, const uint32_t *row_ptr,
                         uint32_t *cols, hipStream_t s);
-/* rowid[e] = r for every edge e of the rows [0, R). */
-int og_launch_plan_row_ids(uint32_t R, const uint32_t *row_ptr, uint32_t *rowid, hipStream_t s);
+/* rowid[e - base] = r for every edge e of the rows [0, R) of row_ptr. */
+int og_launch_plan_row_ids_at(uint32_t R, const uint32_t *row_ptr, uint32_t base, uint32_t *rowid,
+                              hipStream_t s);
 /* cnt[r] = chunks of row r < C (ceil(in-degree / ch)). */
 int og_launch_plan_chunks(uint32_t C, const uint32_t *row_ptr, uint32_t ch, uint32_t *cnt,
                           hipStream_t s);

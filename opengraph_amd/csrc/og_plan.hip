@@ -125,11 +125,12 @@ __device__ __forceinline__ uint32_t og_row_class(uint32_t d, uint32_t seq_t)
 /* Sort key of old vertex v: its class; with relabel, then (classes 0, 3)
  * forward out-degree descending, so the contributions gathered most often
  * share cache lines, or (classes 1, 2) in-degree descending, so the rows or
- * chunks one wavefront takes have about the same length.  Ties keep id
- * order (the sort is stable). */
+ * chunks one wavefront takes have about the same length.  With chain_t,
+ * the class-0 rows of more than chain_t in-edges go first, in-degree
+ * descending.  Ties keep id order (the sort is stable). */
 __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ outdeg,
                                     const uint32_t *__restrict__ indeg, uint32_t seq_t,
-                                    uint32_t relabel, uint32_t *__restrict__ key,
+                                    uint32_t chain_t, uint32_t relabel, uint32_t *__restrict__ key,
                                     uint32_t *__restrict__ val)
 {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
@@ -137,7 +138,14 @@ __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ out
         uint32_t d = indeg[v];
         uint32_t cls = og_row_class(d, seq_t);
         uint32_t k;
-        if (relabel)
+        if (relabel && chain_t && cls == 0)
+        {
+            /* bit 29 clear: chains by in-degree; set: the rest by out-degree */
+            const bool ch = d > chain_t;
+            const uint32_t w = min(ch ? d : outdeg[v], 0x1FFFFFFFu);
+            k = (ch ? 0u : 0x20000000u) | (0x1FFFFFFFu - w);
+        }
+        else if (relabel)
         {
             uint32_t w = min(cls == 1 || cls == 2 ? d : outdeg[v], 0x3FFFFFFFu);
             k = (cls << 30) | (0x3FFFFFFFu - w);
@@ -152,10 +160,11 @@ __global__ void og_plan_keys_kernel(uint32_t N, const uint32_t *__restrict__ out
 __global__ void og_plan_perm_kernel(uint32_t N, const uint32_t *__restrict__ new2old,
                                     const uint32_t *__restrict__ outdeg_old,
                                     const uint32_t *__restrict__ indeg_old, uint32_t seq_t,
-                                    uint32_t *__restrict__ old2new, uint32_t *__restrict__ outdeg_new,
-                                    uint32_t *__restrict__ indeg_new, uint32_t *__restrict__ class_cnt)
+                                    uint32_t chain_t, uint32_t *__restrict__ old2new,
+                                    uint32_t *__restrict__ outdeg_new, uint32_t *__restrict__ indeg_new,
+                                    uint32_t *__restrict__ class_cnt)
 {
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
     {
         uint32_t o = new2old[i];
@@ -164,8 +173,9 @@ __global__ void og_plan_perm_kernel(uint32_t N, const uint32_t *__restrict__ new
         uint32_t d = indeg_old[o];
         indeg_new[i] = d;
         cnt[og_row_class(d, seq_t)]++;
+        if (chain_t && d > chain_t) cnt[4]++;
     }
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
         if (cnt[k]) atomicAdd(&class_cnt[k], cnt[k]);
 }
 
@@ -202,23 +212,25 @@ __global__ __launch_bounds__(OG_BLOCK) void og_plan_copy_hubs_kernel(
     for (uint32_t k = threadIdx.x; k < n; k += OG_BLOCK) cols[dst + k] = old2new[lists[src + k] & OG_ID_MASK];
 }
 
-/* rowid[e] = r for the edges of rows [0, R), one wavefront per row. */
+/* rowid[e - base] = r for the edges e of rows [0, R) of row_ptr, one
+ * wavefront per row. */
 __global__ __launch_bounds__(OG_BLOCK) void og_plan_row_ids_kernel(uint32_t R, const uint32_t *__restrict__ row_ptr,
-                                                                  uint32_t *__restrict__ rowid)
+                                                                  uint32_t base, uint32_t *__restrict__ rowid)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t r = wave; r < R; r += nwaves)
-        for (uint32_t e = row_ptr[r] + lane; e < row_ptr[r + 1]; e += 64) rowid[e] = r;
+        for (uint32_t e = row_ptr[r] + lane; e < row_ptr[r + 1]; e += 64) rowid[e - base] = r;
 }
 
-extern "C" int og_launch_plan_row_ids(uint32_t R, const uint32_t *row_ptr, uint32_t *rowid, hipStream_t s)
+extern "C" int og_launch_plan_row_ids_at(uint32_t R, const uint32_t *row_ptr, uint32_t base, uint32_t *rowid,
+                                         hipStream_t s)
 {
     if (!R) return 0;
     uint64_t waves = R;
     unsigned grid = (unsigned)((waves + 3) / 4 < 65536 ? (waves + 3) / 4 : 65536);
-    hipLaunchKernelGGL(og_plan_row_ids_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, R, row_ptr, rowid);
+    hipLaunchKernelGGL(og_plan_row_ids_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, R, row_ptr, base, rowid);
     return og_hip_check(hipGetLastError(), "og_plan_row_ids_kernel");
 }
 
@@ -274,21 +286,22 @@ static unsigned og_grid(uint64_t n, unsigned cap)
 }
 
 extern "C" int og_launch_plan_keys(uint32_t N, const uint32_t *outdeg, const uint32_t *indeg,
-                                   uint32_t seq_t, uint32_t relabel, uint32_t *key, uint32_t *val,
-                                   hipStream_t s)
+                                   uint32_t seq_t, uint32_t chain_t, uint32_t relabel, uint32_t *key,
+                                   uint32_t *val, hipStream_t s)
 {
     hipLaunchKernelGGL(og_plan_keys_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, outdeg,
-                       indeg, seq_t, relabel, key, val);
+                       indeg, seq_t, chain_t, relabel, key, val);
     return og_hip_check(hipGetLastError(), "og_plan_keys_kernel");
 }
 
 extern "C" int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *outdeg_old,
-                                   const uint32_t *indeg_old, uint32_t seq_t, uint32_t *old2new,
+                                   const uint32_t *indeg_old, uint32_t seq_t, uint32_t chain_t,
+                                   uint32_t *old2new,
                                    uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *class_cnt,
                                    hipStream_t s)
 {
     hipLaunchKernelGGL(og_plan_perm_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, new2old,
-                       outdeg_old, indeg_old, seq_t, old2new, outdeg_new, indeg_new, class_cnt);
+                       outdeg_old, indeg_old, seq_t, chain_t, old2new, outdeg_new, indeg_new, class_cnt);
     return og_hip_check(hipGetLastError(), "og_plan_perm_kernel");
 }
 

@@ -79,7 +79,7 @@ int ogpu_ppr_create(ogpu_graph *gg, const uint32_t *sources, uint32_t k, uint32_
     p->max_iter = max_iter;
     hipStream_t s = og_stream();
     double t0 = og_ppr_now();
-    if (og_plan_build(g, 1, OG_SEQ_T, 0, &p->plan)) { free(p); return -1; }
+    if (og_plan_build(g, 1, OG_SEQ_T, 0, 0, &p->plan)) { free(p); return -1; }
     p->seconds_plan = og_ppr_now() - t0;
     og_plan *pl = p->plan;
     const uint32_t N = g->N;

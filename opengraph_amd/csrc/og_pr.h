@@ -55,8 +55,13 @@ typedef struct og_pr_dev
     uint32_t ppr;            /* personalized: restart only at ppr_src, error = L1 (no 1/N) */
     uint32_t ppr_src;        /* new id of the restart vertex */
     uint32_t spmv;           /* pull adds the row sums onto y (SpMV): fp32 y in r, u64 in next */
-    const uint32_t *chain_rows; /* fp32 pull, every row exact: rows [0, C) longest first, one
-                                   wavefront each (n_chunk_blocks = ceil(C / 4), H = 0) */
+    /* fp32 pull rows summed in the reference's order by one wavefront each
+     * (chains): n_chains of them, in the first n_chain_blocks pull blocks.
+     * Every row exact: chain_rows lists rows [0, C) longest first (H = 0, no
+     * chunks); otherwise (chain_rows NULL) the chains are rows [0, n_chains)
+     * of the multi-chunk class, whose chunks [0, ch_first) are skipped. */
+    const uint32_t *chain_rows;
+    uint32_t n_chains, n_chain_blocks, ch_first;
     double epsilon;
     /* plan */
     const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;
@@ -99,7 +104,7 @@ int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hip
 /* First-level statistics partials one pull launch writes. */
 static inline uint32_t og_pull_parts(const og_pr_dev *a)
 {
-    return a->hot_grid ? a->hot_grid : a->n_chunk_blocks + a->n_row_blocks;
+    return a->hot_grid ? a->hot_grid : a->n_chain_blocks + a->n_chunk_blocks + a->n_row_blocks;
 }
 
 /* ---- many graphs in one launch (fp32 pull) ----

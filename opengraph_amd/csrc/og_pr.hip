@@ -19,7 +19,11 @@
  *
  * One pull iteration (a degree-binned gather SpMV plus the update) is two
  * kernels:
- *   og_pull_kernel<MODE>     blocks [0, n_chunk_blocks): each wavefront sums
+ *   og_pull_kernel<MODE>     blocks [0, n_chain_blocks) (fp32 only, with
+ *                            chains): one wavefront per chain row adds its
+ *                            in-edges in CSR order, bit-exact with the
+ *                            reference's running sum; the next
+ *                            n_chunk_blocks: each wavefront sums
  *                            OG_CH_PER_WAVE chunks (64 lanes x 8 edges,
  *                            fixed tree); a row of one chunk gets its fused
  *                            epilogue there (pageRank.c:1014-1028 plus the
@@ -379,7 +383,7 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
     constexpr uint32_t K = OG_CH_PER_WAVE;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
-    const uint32_t c0 = (cblock * OG_WAVES + wave) * K;
+    const uint32_t c0 = a.ch_first + (cblock * OG_WAVES + wave) * K;
     if (c0 >= a.n_chunks) return;
     const uint32_t nk = min(K, a.n_chunks - c0);
     /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
@@ -404,13 +408,14 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
 
 /* Rows [0, C) in the reference's order (fp32 pull with every row exact):
  * one wavefront per row, the longest rows first (chain_rows).  The lanes
- * load and gather the row's in-edges OG_CHAIN_U x 64 at a time, and the
- * wavefront adds the gathered values one lane after the other onto a
- * running sum held alike in every lane: the reference's sequential `+=`
- * over the row (pageRank.c:998-1011, SPMV.c:673-693) to the bit.  Two
- * groups are in flight while one is added: the gathers of the next group
- * and the indices of the one after it. */
+ * load and gather the row's in-edges OG_CHAIN_U x 64 at a time and stage
+ * each group in LDS in edge order; lane 0 then adds the group's values one
+ * after the other onto its running sum, four per LDS read: the reference's
+ * sequential `+=` over the row (pageRank.c:998-1011, SPMV.c:673-693) to the
+ * bit.  Two groups are in flight while one is added: the gathers of the
+ * next group and the indices of the one after it. */
 #define OG_CHAIN_U 8u
+#define OG_CHAIN_G (64u * OG_CHAIN_U)
 #define OG_CHAIN_PRIO_EDGES 8192u
 
 __device__ __forceinline__ void og_chain_idx(const og_pr_dev &a, uint32_t base, uint32_t e,
@@ -422,29 +427,120 @@ __device__ __forceinline__ void og_chain_idx(const og_pr_dev &a, uint32_t base, 
         idx[u] = __builtin_nontemporal_load(&a.cols[min(base + u * 64 + lane, e - 1)]);
 }
 
-template <bool HOT>
 __device__ __forceinline__ void og_chain_gather(const og_pr_dev &a, const uint32_t (&idx)[OG_CHAIN_U],
-                                                const float *hot, uint32_t nh, float (&val)[OG_CHAIN_U])
+                                                float (&val)[OG_CHAIN_U])
 {
     const float *__restrict__ g = (const float *)a.g_in;
 #pragma unroll
-    for (uint32_t u = 0; u < OG_CHAIN_U; u++) val[u] = og_gather<float, HOT>(g, hot, nh, idx[u]);
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++) val[u] = g[idx[u]];
 }
 
-template <bool HOT>
-__device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, uint32_t cblock,
-                                              const float *hot = nullptr, uint32_t nh = 0)
+/* the wavefront's LDS writes visible to all its lanes, and its reads done
+ * before the next writes */
+__device__ __forceinline__ void og_wave_sync()
 {
-    constexpr uint32_t G = 64 * OG_CHAIN_U;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Group values g (edges [base, base + OG_CHAIN_G) of a row ending at e)
+ * staged in the wavefront's LDS slice sw and added by lane 0 onto y in
+ * edge order. */
+__device__ __forceinline__ void og_chain_add(float *sw, const float (&g)[OG_CHAIN_U], uint32_t base, uint32_t e,
+                                             uint32_t lane, float &y)
+{
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++) sw[64 * u + lane] = g[u];
+    og_wave_sync();
+    if (lane == 0)
+    {
+        const float4 *s4 = (const float4 *)sw;
+        if (e - base >= OG_CHAIN_G)
+        {
+            /* reads of the next 16 values issued before the adds of these
+             * 16: the LDS latency stays off the chain of adds */
+            float4 c[4], n[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++) c[i] = s4[i];
+#pragma unroll
+            for (uint32_t k = 4; k <= OG_CHAIN_G / 4; k += 4)
+            {
+                if (k < OG_CHAIN_G / 4)
+#pragma unroll
+                    for (uint32_t i = 0; i < 4; i++) n[i] = s4[k + i];
+#pragma unroll
+                for (uint32_t i = 0; i < 4; i++)
+                {
+                    y = y + c[i].x;
+                    y = y + c[i].y;
+                    y = y + c[i].z;
+                    y = y + c[i].w;
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < 4; i++) c[i] = n[i];
+            }
+        }
+        else
+        {
+            const uint32_t n = e - base;
+            for (uint32_t k = 0; k < n; k++) y = y + sw[k];
+        }
+    }
+    og_wave_sync();
+}
+
+/* Exact fast-forward of a chain over one group.  While the running sum y
+ * stays in one binade [2^(ey-1), 2^ey), every representable value is a
+ * multiple of u = 2^(ey-24), so for a term a >= 0 the rounded sum is
+ * fl(y + a) = y + u * m with m the integer nearest a / u -- unless a / u
+ * ends in exactly one half (round-half-even then depends on y).  The
+ * group's sequential sums therefore equal y + u * (m_1 + ... + m_k), an
+ * integer sum in any order, as long as no term is a tie and the last sum
+ * stays below 2^ey.  Those conditions are checked here (y uniform across
+ * the wavefront); when one fails, nothing changes and the caller adds the
+ * group one term at a time. */
+__device__ __forceinline__ bool og_chain_fast(const float (&g)[OG_CHAIN_U], uint32_t base, uint32_t e,
+                                              uint32_t lane, float &y)
+{
+    if (!(y >= 0x1p-100f && y < 0x1p100f)) return false;
+    int ey;
+    frexpf(y, &ey);
+    const int sc = 24 - ey;
+    uint32_t msum = 0;
+    bool bad = false;
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+    {
+        const float t = ldexpf(g[u], sc);          /* a / u, exact */
+        const float fl = floorf(t);
+        const float fr = t - fl;                   /* exact */
+        const bool in = base + 64 * u + lane < e;
+        bad |= in && (!(t >= 0.0f && t < 0x1p24f) || fr == 0.5f);
+        msum += in ? (uint32_t)fl + (fr > 0.5f ? 1u : 0u) : 0u;
+    }
+    bad |= msum >= (1u << 24);
+    if (__any(bad)) return false;
+    const uint32_t M = og_wave_allsum(msum);      /* < 2^30 */
+    const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
+    if (Y + M >= (1u << 24)) return false;
+    y = ldexpf((float)(Y + M), -sc);
+    return true;
+}
+
+__device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, uint32_t cblock)
+{
+    __shared__ __align__(16) float stage[OG_WAVES][OG_CHAIN_G];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
     const uint32_t q = cblock * OG_WAVES + wave;
-    if (q >= a.C) return;
-    const uint32_t v = __builtin_amdgcn_readfirstlane(a.chain_rows[q]);
+    if (q >= a.n_chains) return;
+    const uint32_t v = a.chain_rows ? __builtin_amdgcn_readfirstlane(a.chain_rows[q]) : q;
     const uint32_t j = __builtin_amdgcn_readfirstlane(a.row_ptr[v]);
     const uint32_t e = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1]);
     const float prev = a.r[v];
     float y = a.spmv ? prev : 0.0f;
+    float *sw = stage[wave];
     /* a chain thousands of edges long bounds the whole launch: its
      * wavefront takes the SIMD's issue slots before the others */
     const bool lead = e - j > OG_CHAIN_PRIO_EDGES;
@@ -453,37 +549,16 @@ __device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, u
     uint32_t i1[OG_CHAIN_U], i2[OG_CHAIN_U];
     float g0[OG_CHAIN_U], g1[OG_CHAIN_U];
     og_chain_idx(a, j, e, i1);
-    og_chain_gather<HOT>(a, i1, hot, nh, g0);
-    og_chain_idx(a, j + G, e, i1);
-    for (uint32_t base = j; base < e; base += G)
+    og_chain_gather(a, i1, g0);
+    og_chain_idx(a, j + OG_CHAIN_G, e, i1);
+    for (uint32_t base = j; base < e; base += OG_CHAIN_G)
     {
-        og_chain_gather<HOT>(a, i1, hot, nh, g1);     /* group + 1 */
-        og_chain_idx(a, base + 2 * G, e, i2);         /* group + 2 */
-#pragma unroll
-        for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+        og_chain_gather(a, i1, g1);                     /* group + 1 */
+        og_chain_idx(a, base + 2 * OG_CHAIN_G, e, i2);  /* group + 2 */
+        if (!og_chain_fast(g0, base, e, lane, y))
         {
-            const uint32_t s = base + 64 * u;
-            if (s >= e) break;
-            const int x = __float_as_int(g0[u]);
-            if (e - s >= 64)
-            {
-                /* eight lane reads into separate scalars, then their eight
-                 * adds: no read-after-write wait between each pair */
-#pragma unroll
-                for (int k0 = 0; k0 < 64; k0 += 8)
-                {
-                    int t[8];
-#pragma unroll
-                    for (int k = 0; k < 8; k++) t[k] = __builtin_amdgcn_readlane(x, k0 + k);
-#pragma unroll
-                    for (int k = 0; k < 8; k++) y = y + __int_as_float(t[k]);
-                }
-            }
-            else
-            {
-                const int n = (int)(e - s);
-                for (int k = 0; k < n; k++) y = y + __int_as_float(__builtin_amdgcn_readlane(x, k));
-            }
+            og_chain_add(sw, g0, base, e, lane, y);
+            y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y)));   /* lane 0's sum */
         }
 #pragma unroll
         for (uint32_t u = 0; u < OG_CHAIN_U; u++)
@@ -491,6 +566,100 @@ __device__ __forceinline__ void og_pull_chain(const og_pr_dev &a, og_stat &st, u
             g0[u] = g1[u];
             i1[u] = i2[u];
         }
+    }
+    if (lead) __builtin_amdgcn_s_setprio(0);
+    if (lane == 0)
+    {
+        if (a.spmv) og_spmv_store<OG_MODE_F32>(a, v, y, prev, true);
+        else og_epilogue_pf<OG_MODE_F32>(a, v, y, prev, a.outdeg[v], st);
+    }
+}
+
+/* One group of a lane-read chain: fast-forwarded when exact, else its
+ * values reach the running sum (uniform across the wavefront) through
+ * lane reads -- eight reads into separate scalars, then their eight adds,
+ * so no read-after-write wait separates each pair. */
+__device__ __forceinline__ void og_chain_lanes_add(const float (&g)[OG_CHAIN_U], uint32_t base, uint32_t e,
+                                                   uint32_t lane, float &y)
+{
+    if (og_chain_fast(g, base, e, lane, y)) return;
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++)
+    {
+        const uint32_t s = base + 64 * u;
+        if (s >= e) break;
+        const int x = __float_as_int(g[u]);
+        if (e - s >= 64)
+        {
+#pragma unroll
+            for (int k0 = 0; k0 < 64; k0 += 8)
+            {
+                int t[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) t[k] = __builtin_amdgcn_readlane(x, k0 + k);
+#pragma unroll
+                for (int k = 0; k < 8; k++) y = y + __int_as_float(t[k]);
+            }
+        }
+        else
+        {
+            const int n = (int)(e - s);
+            for (int k = 0; k < n; k++) y = y + __int_as_float(__builtin_amdgcn_readlane(x, k));
+        }
+    }
+}
+
+/* Gathers of one group into gn (indices in idx), then the indices of the
+ * group `ahead` groups later into idx. */
+template <bool HOT>
+__device__ __forceinline__ void og_chain_issue(const og_pr_dev &a, const float *hot, uint32_t nh,
+                                               uint32_t next_base, uint32_t e, uint32_t (&idx)[OG_CHAIN_U],
+                                               float (&gn)[OG_CHAIN_U])
+{
+    const float *__restrict__ g = (const float *)a.g_in;
+#pragma unroll
+    for (uint32_t u = 0; u < OG_CHAIN_U; u++) gn[u] = og_gather<float, HOT>(g, hot, nh, idx[u]);
+    og_chain_idx(a, next_base, e, idx);
+}
+
+/* The chains of a pull that also has chunks (exact_above): the longest rows
+ * only, so the time of the longest bounds the launch.  Most of its groups
+ * take the exact fast-forward, which leaves the gathers as the limit: three
+ * groups rotate through three register sets (no copies, so no step waits
+ * for loads younger than its own group's), two in flight while one is
+ * added. */
+template <bool HOT>
+__device__ __forceinline__ void og_pull_chain_lanes(const og_pr_dev &a, og_stat &st, uint32_t cblock,
+                                                    const float *hot, uint32_t nh)
+{
+    constexpr uint32_t G = OG_CHAIN_G;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
+    const uint32_t q = cblock * OG_WAVES + wave;
+    if (q >= a.n_chains) return;
+    const uint32_t v = a.chain_rows ? __builtin_amdgcn_readfirstlane(a.chain_rows[q]) : q;
+    const uint32_t j = __builtin_amdgcn_readfirstlane(a.row_ptr[v]);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1]);
+    const float prev = a.r[v];
+    float y = a.spmv ? prev : 0.0f;
+    const bool lead = e - j > OG_CHAIN_PRIO_EDGES;
+    if (lead) __builtin_amdgcn_s_setprio(3);
+    /* groups past the row's end re-read its last edge; they are never added */
+    uint32_t idx[OG_CHAIN_U];
+    float g0[OG_CHAIN_U], g1[OG_CHAIN_U], g2[OG_CHAIN_U];
+    og_chain_idx(a, j, e, idx);
+    og_chain_issue<HOT>(a, hot, nh, j + G, e, idx, g0);
+    og_chain_issue<HOT>(a, hot, nh, j + 2 * G, e, idx, g1);
+    for (uint32_t base = j; base < e; base += 3 * G)
+    {
+        og_chain_issue<HOT>(a, hot, nh, base + 3 * G, e, idx, g2);
+        og_chain_lanes_add(g0, base, e, lane, y);
+        if (base + G >= e) break;
+        og_chain_issue<HOT>(a, hot, nh, base + 4 * G, e, idx, g0);
+        og_chain_lanes_add(g1, base + G, e, lane, y);
+        if (base + 2 * G >= e) break;
+        og_chain_issue<HOT>(a, hot, nh, base + 5 * G, e, idx, g1);
+        og_chain_lanes_add(g2, base + 2 * G, e, lane, y);
     }
     if (lead) __builtin_amdgcn_s_setprio(0);
     if (lane == 0)
@@ -580,9 +749,29 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    if (blockIdx.x >= a.n_chunk_blocks) og_pull_rows<MODE>(a, st, blockIdx.x - a.n_chunk_blocks);
-    else if (MODE == OG_MODE_F32 && a.chain_rows) og_pull_chain<false>(a, st, blockIdx.x);
-    else og_pull_chunks<MODE>(a, st, blockIdx.x);
+    const uint32_t nc = a.n_chain_blocks;
+    if (blockIdx.x >= nc + a.n_chunk_blocks) og_pull_rows<MODE>(a, st, blockIdx.x - nc - a.n_chunk_blocks);
+    else if (blockIdx.x >= nc) og_pull_chunks<MODE>(a, st, blockIdx.x - nc);
+    else if constexpr (MODE == OG_MODE_F32) og_pull_chain_lanes<false>(a, st, blockIdx.x, nullptr, 0);
+    og_stat r = og_block_stat(st);
+    if (threadIdx.x == 0)
+    {
+        a.part_err[blockIdx.x] = r.err;
+        a.part_act[blockIdx.x] = r.active;
+        a.part_min[blockIdx.x] = r.cmin;
+        a.part_max[blockIdx.x] = r.cmax;
+    }
+}
+
+/* fp32 pull with every row exact: chains and rows only (no chunk path, so
+ * fewer registers and more wavefronts in flight per SIMD) */
+__global__ __launch_bounds__(OG_BLOCK) void og_pull_exact_kernel(og_pr_dev a, uint32_t it)
+{
+    if (*a.done) return;
+    og_stat st;
+    og_stat_init(st);
+    if (blockIdx.x >= a.n_chain_blocks) og_pull_rows<OG_MODE_F32>(a, st, blockIdx.x - a.n_chain_blocks);
+    else og_pull_chain(a, st, blockIdx.x);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {
@@ -613,12 +802,12 @@ __global__ __launch_bounds__(OG_HOT_SUBS *OG_BLOCK) void og_pull_hot_kernel(og_p
     __syncthreads();
     og_stat st;
     og_stat_init(st);
+    /* (no chains here: a pull with chains runs as og_pull_kernel) */
     const uint32_t items = a.n_chunk_blocks + a.n_row_blocks;
     const uint32_t q = threadIdx.x / OG_BLOCK;
     for (uint32_t w = blockIdx.x * OG_HOT_SUBS + q; w < items; w += gridDim.x * OG_HOT_SUBS)
     {
         if (w >= a.n_chunk_blocks) og_pull_rows<MODE, true>(a, st, w - a.n_chunk_blocks, hot, nh);
-        else if (MODE == OG_MODE_F32 && a.chain_rows) og_pull_chain<true>(a, st, w, (const float *)hot, nh);
         else og_pull_chunks<MODE, true>(a, st, w, hot, nh);
     }
     og_stat r = og_block_stat<OG_HOT_SUBS * OG_WAVES>(st);
@@ -735,7 +924,8 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_finalize_kernel(og_pr_dev a,
     og_stat st;
     og_stat_init(st);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (uint32_t h = blockIdx.x * OG_WAVES + wave; h < a.H; h += gridDim.x * OG_WAVES)
+    /* (rows [0, n_chains) of the multi-chunk class are chains, already done) */
+    for (uint32_t h = a.n_chains + blockIdx.x * OG_WAVES + wave; h < a.H; h += gridDim.x * OG_WAVES)
         og_finalize_long_row<MODE>(a, h, st);
     og_reduce_tail(a, it, st);
 }
@@ -963,11 +1153,13 @@ __global__ void og_sum_parts_kernel(const double *p, uint32_t n, double *out)
 template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it, hipStream_t s,
                                                  hipEvent_t ev0, hipEvent_t ev1)
 {
-    uint32_t grid = a->n_chunk_blocks + a->n_row_blocks;
+    uint32_t grid = a->n_chain_blocks + a->n_chunk_blocks + a->n_row_blocks;
     if (ev0) hipEventRecord(ev0, s);
     if (a->hot_grid)
         hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
+    else if (MODE == OG_MODE_F32 && a->chain_rows && !a->n_chunk_blocks && grid)
+        hipLaunchKernelGGL(og_pull_exact_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     else if (grid)
         hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     if (ev1) hipEventRecord(ev1, s);

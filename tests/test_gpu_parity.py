@@ -244,6 +244,35 @@ def test_scale18_every_row_exact_pull_bitexact(rmat18, hot):
     assert res.iterations == ref["iterations"] and list(act) == list(ref["active"])
 
 
+@pytest.mark.parametrize("hot,relabel", [(0, 1), (2, 1), (2, 2)])
+def test_scale18_exact_above_chains(rmat18, hot, relabel):
+    """exact_above: only the rows of more than 2048 in-edges run as chains
+    (the others as chunks and short rows).  After one iteration, whose
+    inputs are the common start, every chain row equals the oracle's
+    sequential pull to the bit; over 15 iterations every rank stays within
+    1e-5 relative of it, and the chains stay closer to it than the default
+    split's chunk sums.  PageRank variants other than fp32 pull on the same
+    options ignore the option."""
+    g, o = rmat18
+    indeg = np.bincount(o["f_dst"], minlength=o.num_vertices)
+    chains = indeg > 2048
+    assert chains.sum() > 4
+    ref1 = po.pagerank(o, 0, 0.0, 1, 8)["ranks"]
+    _, r1, _, _ = _run(g, 0, 0.0, 1, exact_above=2048, hot_cache=hot, relabel=relabel)
+    assert np.array_equal(r1[chains].view(np.uint32), ref1[chains].view(np.uint32)), (hot, relabel)
+    ref = po.pagerank(o, 0, 0.0, 15, 8)["ranks"]
+    _, r15, _, _ = _run(g, 0, 0.0, 15, exact_above=2048, hot_cache=hot, relabel=relabel)
+    _, d15, _, _ = _run(g, 0, 0.0, 15, hot_cache=hot, relabel=relabel)
+    rel = np.abs(r15.astype(np.float64) - ref) / ref
+    rel_d = np.abs(d15.astype(np.float64) - ref) / ref
+    assert rel.max() <= 1e-5
+    assert rel[chains].max() <= rel_d[chains].max()
+    # fixed-point pull with the same options: chunks for every long row, exact
+    want = po.pagerank(o, 2, 0.0, 6, 8)["ranks"]
+    _, fx, _, _ = _run(g, 2, 0.0, 6, exact_above=2048, hot_cache=hot, relabel=relabel)
+    assert np.array_equal(fx.view(np.uint32), want.view(np.uint32))
+
+
 def test_scale18_fixed_point_bitexact_and_pull_equals_push(rmat18):
     g, o = rmat18
     _, a, act_a, _ = _run(g, 2, 1e-7, 60)
