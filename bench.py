@@ -93,7 +93,7 @@ def ext_variants(g, threads, csr, do_cpu):
     """Pull/push/fixed-point on the headline graph, each against the host run."""
     out, cpu = {}, {}
     for code, name, kw in ((0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}),
-                           (0, "pull_fp32_exact_above_4096", {"exact_above": 4096}),
+                           (0, "pull_fp32_chunk_split_only", {"exact_above": 0}),
                            (1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
                            (2, "pull_fixed64", {}), (3, "push_fixed64", {}), (11, "pull_fixed8", {})):
         ms, kms, bpi = gpu_iter_ms(g, code, **kw)
@@ -564,7 +564,7 @@ def main():
             x64, _, _ = po.pagerank_fp64(c, 0.15, 0.0, args.cpu_iterations, threads)
             par = dict(iterations=args.cpu_iterations,
                        l1_vs_fp64_reference=float(np.abs(ref_ranks.astype(np.float64) - x64).sum()))
-            for form, kw in (("default", {}), ("exact_above_4096", {"exact_above": 4096}),
+            for form, kw in (("default_exact_above_4096", {}), ("chunk_split_only", {"exact_above": 0}),
                              ("every_row_exact", {"hub_threshold": 1 << 31})):
                 pr3 = og.PageRank(g, 0, args.cpu_iterations, 0.0, **kw)
                 pr3.run()
@@ -624,7 +624,7 @@ def main():
                    "variant": "pull fp32 (-p 0)", "sort": 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_iteration": bytes_iter, "kernel": "og_pull_hot_kernel<0>",
+                     "bytes_per_iteration": bytes_iter, "kernel": "og_pull_hot_kernel<0> + og_pull_chains_kernel (concurrent, fork to join)",
                      "traffic_source": "profiles/pull_rmat24_counters.json" if traffic else None},
         "cpu_baseline": cpu,
         "extra": extra,

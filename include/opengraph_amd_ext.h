@@ -129,9 +129,10 @@ typedef struct
                                rows of more than max(exact_above, 512) in-edges are summed
                                in the reference's order, one wavefront each, instead of in
                                chunks (bit-exact with the reference's running sums on equal
-                               inputs); 0 off.  Such rows carry the largest fp32 drift of
-                               the reference's sums.  The pull then runs without the LDS
-                               copy of hot_cache */
+                               inputs); default 4096 (ogpu_pr_default_options), 0 off.
+                               Such rows carry the largest fp32 drift of
+                               the reference's sums.  With hot_cache's persistent kernel
+                               the chains run beside it on a side stream */
 } ogpu_pr_options;
 
 typedef struct

@@ -25,6 +25,9 @@
  * fastest on R-MAT 24 (1.38-1.40 ms against 1.47 at 32 and 1.50 at 4) and
  * on Erdos-Renyi 24 (4.80-4.87 ms against 5.08 at 32). */
 #define OG_DEFAULT_SEQ_T 16u
+/* fp32 pull: rows of more in-edges are summed in the reference's order by
+ * default (exact_above) */
+#define OG_DEFAULT_EXACT_ABOVE 4096u
 /* every-row-exact fp32 pull: rows of at most this many in-edges are summed
  * one per thread, the longer ones as wavefront chains */
 #ifndef OG_EXACT_SEQ_T
@@ -62,6 +65,7 @@ void ogpu_pr_default_options(ogpu_pr_options *o)
     o->epsilon = 0.0001;       /* open-graph.c:280 */
     o->relabel = 1;
     o->hub_threshold = 0;
+    o->exact_above = OG_DEFAULT_EXACT_ABOVE;
 }
 
 static int og_variant_mode(uint32_t v, int *mode, int *push, int *quant, int *dd)
@@ -260,10 +264,9 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
      * contributions: by default when the graph is several times larger
      * than that copy */
     uint32_t hot_grid = 0;
-    /* (pulls with chains run without it: a chain wavefront sharing the
-     * persistent blocks' static split holds back their other work) */
-    const int has_chains = p->chain_rows || (p->plan->L && opt->variant == 0 && !opt->ppr);
-    if (!p->push && !p->dd && opt->hot_cache != 2 && !has_chains)
+    /* (the every-row-exact pull stages its chains in LDS instead; the
+     * chains of exact_above run beside it, in a launch of their own) */
+    if (!p->push && !p->dd && opt->hot_cache != 2 && !p->chain_rows)
     {
         int on = opt->hot_cache == 1;
         const uint32_t nh = (uint32_t)(OG_HOT_BYTES / og_elem_bytes(p->mode));
@@ -325,8 +328,11 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     }
     else if (pl->L)
         chunk_blocks = ((pl->n_chunks + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE + 3) / 4;
-    const uint32_t chain_blocks = (n_chains + 3) / 4;
-    const uint32_t pull_blocks = hot_grid ? hot_grid : chain_blocks + chunk_blocks + pl->n_row_blocks;
+    /* four chains per 256-thread pull block; beside the persistent kernel,
+     * one wavefront per CU taking every grid-th chain */
+    const uint32_t chain_blocks = hot_grid && !p->chain_rows ? (n_chains < hot_grid ? n_chains : hot_grid)
+                                                             : (n_chains + 3) / 4;
+    const uint32_t pull_blocks = hot_grid ? hot_grid + chain_blocks : chain_blocks + chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv, 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
@@ -883,7 +889,7 @@ int og_cache_graph(struct GraphCSR *graph, int need_forward, og_graph **g, og_pl
         {
             og_plan_destroy(g_cache.plan);
             g_cache.plan = NULL;
-            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_SEQ_T, 0, 1, &g_cache.plan)) return -1;
+            if (og_plan_build(g_cache.g, 1, OG_DEFAULT_SEQ_T, OG_DEFAULT_EXACT_ABOVE, 1, &g_cache.plan)) return -1;
         }
         *g = g_cache.g;
         *plan = g_cache.plan;

@@ -56,7 +56,9 @@ typedef struct og_pr_dev
     uint32_t ppr_src;        /* new id of the restart vertex */
     uint32_t spmv;           /* pull adds the row sums onto y (SpMV): fp32 y in r, u64 in next */
     /* fp32 pull rows summed in the reference's order by one wavefront each
-     * (chains): n_chains of them, in the first n_chain_blocks pull blocks.
+     * (chains): n_chains of them, in the first n_chain_blocks pull blocks
+     * (with hot_grid: the n_chain_blocks one-wavefront blocks of
+     * og_pull_chains_kernel, launched beside the persistent kernel).
      * Every row exact: chain_rows lists rows [0, C) longest first (H = 0, no
      * chunks); otherwise (chain_rows NULL) the chains are rows [0, n_chains)
      * of the multi-chunk class, whose chunks [0, ch_first) are skipped. */
@@ -104,7 +106,7 @@ int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s, hip
 /* First-level statistics partials one pull launch writes. */
 static inline uint32_t og_pull_parts(const og_pr_dev *a)
 {
-    return a->hot_grid ? a->hot_grid : a->n_chain_blocks + a->n_chunk_blocks + a->n_row_blocks;
+    return a->hot_grid ? a->hot_grid + a->n_chain_blocks : a->n_chain_blocks + a->n_chunk_blocks + a->n_row_blocks;
 }
 
 /* ---- many graphs in one launch (fp32 pull) ----

@@ -629,13 +629,11 @@ __device__ __forceinline__ void og_chain_issue(const og_pr_dev &a, const float *
  * for loads younger than its own group's), two in flight while one is
  * added. */
 template <bool HOT>
-__device__ __forceinline__ void og_pull_chain_lanes(const og_pr_dev &a, og_stat &st, uint32_t cblock,
+__device__ __forceinline__ void og_pull_chain_lanes(const og_pr_dev &a, og_stat &st, uint32_t q,
                                                     const float *hot, uint32_t nh)
 {
     constexpr uint32_t G = OG_CHAIN_G;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
-    const uint32_t q = cblock * OG_WAVES + wave;
     if (q >= a.n_chains) return;
     const uint32_t v = a.chain_rows ? __builtin_amdgcn_readfirstlane(a.chain_rows[q]) : q;
     const uint32_t j = __builtin_amdgcn_readfirstlane(a.row_ptr[v]);
@@ -752,7 +750,9 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     const uint32_t nc = a.n_chain_blocks;
     if (blockIdx.x >= nc + a.n_chunk_blocks) og_pull_rows<MODE>(a, st, blockIdx.x - nc - a.n_chunk_blocks);
     else if (blockIdx.x >= nc) og_pull_chunks<MODE>(a, st, blockIdx.x - nc);
-    else if constexpr (MODE == OG_MODE_F32) og_pull_chain_lanes<false>(a, st, blockIdx.x, nullptr, 0);
+    else if constexpr (MODE == OG_MODE_F32)
+        og_pull_chain_lanes<false>(a, st, blockIdx.x * OG_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
+                                   nullptr, 0);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {
@@ -760,6 +760,29 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
         a.part_act[blockIdx.x] = r.active;
         a.part_min[blockIdx.x] = r.cmin;
         a.part_max[blockIdx.x] = r.cmax;
+    }
+}
+
+/* The chains of a pull whose other rows run in the persistent kernel: a
+ * launch of its own on a side stream, concurrent with it.  One wavefront
+ * per block, at most one block per CU (grid = n_chain_blocks <= CUs), each
+ * taking chains q = b, b + grid, ...; no LDS and at most 96 registers, so
+ * a wavefront fits on a SIMD beside the persistent block's four.  Its
+ * statistics partials follow the persistent blocks'. */
+__global__ __launch_bounds__(64, 5) void og_pull_chains_kernel(og_pr_dev a, uint32_t it)
+{
+    if (*a.done) return;
+    og_stat st;
+    og_stat_init(st);
+    for (uint32_t q = blockIdx.x; q < a.n_chains; q += gridDim.x) og_pull_chain_lanes<false>(a, st, q, nullptr, 0);
+    og_stat r = og_block_stat<1>(st);
+    if (threadIdx.x == 0)
+    {
+        const uint32_t b = a.hot_grid + blockIdx.x;
+        a.part_err[b] = r.err;
+        a.part_act[b] = r.active;
+        a.part_min[b] = r.cmin;
+        a.part_max[b] = r.cmax;
     }
 }
 
@@ -802,7 +825,7 @@ __global__ __launch_bounds__(OG_HOT_SUBS *OG_BLOCK) void og_pull_hot_kernel(og_p
     __syncthreads();
     og_stat st;
     og_stat_init(st);
-    /* (no chains here: a pull with chains runs as og_pull_kernel) */
+    /* (chains run beside this kernel, in og_pull_chains_kernel) */
     const uint32_t items = a.n_chunk_blocks + a.n_row_blocks;
     const uint32_t q = threadIdx.x / OG_BLOCK;
     for (uint32_t w = blockIdx.x * OG_HOT_SUBS + q; w < items; w += gridDim.x * OG_HOT_SUBS)
@@ -1150,12 +1173,58 @@ __global__ void og_sum_parts_kernel(const double *p, uint32_t n, double *out)
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
 
-template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it, hipStream_t s,
-                                                 hipEvent_t ev0, hipEvent_t ev1)
+/* Side stream (default priority) and fork/join events of the current
+ * device, made on first use and kept until the process ends. */
+typedef struct
+{
+    hipStream_t s;
+    hipEvent_t fork, join;
+} og_side_t;
+
+static og_side_t *og_side()
+{
+    static og_side_t sd[64];
+    static int made[64];
+    int dev = 0;
+    if (og_hip_check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= 64)
+    {
+        og_set_error("device %d out of range for the side stream", dev);
+        return nullptr;
+    }
+    if (!made[dev])
+    {
+        if (og_hip_check(hipStreamCreateWithFlags(&sd[dev].s, hipStreamNonBlocking), "hipStreamCreateWithFlags") ||
+            og_hip_check(hipEventCreateWithFlags(&sd[dev].fork, hipEventDisableTiming), "hipEventCreate") ||
+            og_hip_check(hipEventCreateWithFlags(&sd[dev].join, hipEventDisableTiming), "hipEventCreate"))
+            return nullptr;
+        made[dev] = 1;
+    }
+    return &sd[dev];
+}
+
+template <int MODE> static int og_launch_pull_t(const og_pr_dev *a, uint32_t it, hipStream_t s,
+                                                hipEvent_t ev0, hipEvent_t ev1)
 {
     uint32_t grid = a->n_chain_blocks + a->n_chunk_blocks + a->n_row_blocks;
     if (ev0) hipEventRecord(ev0, s);
-    if (a->hot_grid)
+    if (a->hot_grid && a->n_chain_blocks)
+    {
+        /* fork: chains on the side stream beside the persistent kernel, joined
+         * before the finalize */
+        og_side_t *sd = og_side();
+        if (!sd) return -1;
+        if (og_hip_check(hipEventRecord(sd->fork, s), "hipEventRecord") ||
+            og_hip_check(hipStreamWaitEvent(sd->s, sd->fork, 0), "hipStreamWaitEvent"))
+            return -1;
+        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, sd->s, *a, it);
+        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
+                           s, *a, it);
+        if (og_hip_check(hipEventRecord(sd->join, sd->s), "hipEventRecord") ||
+            og_hip_check(hipStreamWaitEvent(s, sd->join, 0), "hipStreamWaitEvent"))
+            return -1;
+    }
+    else if (a->hot_grid)
         hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
     else if (MODE == OG_MODE_F32 && a->chain_rows && !a->n_chunk_blocks && grid)
@@ -1165,23 +1234,26 @@ template <int MODE> static void og_launch_pull_t(const og_pr_dev *a, uint32_t it
     if (ev1) hipEventRecord(ev1, s);
     hipLaunchKernelGGL(og_pull_finalize_kernel<MODE>, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s,
                        *a, it);
+    return 0;
 }
 
 extern "C" int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1)
 {
+    int rc;
     switch (mode)
     {
-    case OG_MODE_F32: og_launch_pull_t<OG_MODE_F32>(a, it, s, ev0, ev1); break;
-    case OG_MODE_FX64: og_launch_pull_t<OG_MODE_FX64>(a, it, s, ev0, ev1); break;
-    case OG_MODE_FX32: og_launch_pull_t<OG_MODE_FX32>(a, it, s, ev0, ev1); break;
-    case OG_MODE_FX16: og_launch_pull_t<OG_MODE_FX16>(a, it, s, ev0, ev1); break;
-    case OG_MODE_FX8: og_launch_pull_t<OG_MODE_FX8>(a, it, s, ev0, ev1); break;
-    case OG_MODE_Q32: og_launch_pull_t<OG_MODE_Q32>(a, it, s, ev0, ev1); break;
-    case OG_MODE_Q16: og_launch_pull_t<OG_MODE_Q16>(a, it, s, ev0, ev1); break;
-    case OG_MODE_Q8: og_launch_pull_t<OG_MODE_Q8>(a, it, s, ev0, ev1); break;
+    case OG_MODE_F32: rc = og_launch_pull_t<OG_MODE_F32>(a, it, s, ev0, ev1); break;
+    case OG_MODE_FX64: rc = og_launch_pull_t<OG_MODE_FX64>(a, it, s, ev0, ev1); break;
+    case OG_MODE_FX32: rc = og_launch_pull_t<OG_MODE_FX32>(a, it, s, ev0, ev1); break;
+    case OG_MODE_FX16: rc = og_launch_pull_t<OG_MODE_FX16>(a, it, s, ev0, ev1); break;
+    case OG_MODE_FX8: rc = og_launch_pull_t<OG_MODE_FX8>(a, it, s, ev0, ev1); break;
+    case OG_MODE_Q32: rc = og_launch_pull_t<OG_MODE_Q32>(a, it, s, ev0, ev1); break;
+    case OG_MODE_Q16: rc = og_launch_pull_t<OG_MODE_Q16>(a, it, s, ev0, ev1); break;
+    case OG_MODE_Q8: rc = og_launch_pull_t<OG_MODE_Q8>(a, it, s, ev0, ev1); break;
     default: og_set_error("og_launch_pull: bad mode %d", mode); return -1;
     }
+    if (rc) return rc;
     return og_hip_check(hipGetLastError(), "og_pull_kernel");
 }
 
