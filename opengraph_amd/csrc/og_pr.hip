@@ -413,7 +413,8 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
  * after the other onto its running sum, four per LDS read: the reference's
  * sequential `+=` over the row (pageRank.c:998-1011, SPMV.c:673-693) to the
  * bit.  Two groups are in flight while one is added: the gathers of the
- * next group and the indices of the one after it. */
+ * next group and the indices of the one after it.  Groups over which the
+ * sum stays in one binade skip the one-by-one adds (og_chain_fast). */
 #define OG_CHAIN_U 8u
 #define OG_CHAIN_G (64u * OG_CHAIN_U)
 #define OG_CHAIN_PRIO_EDGES 8192u
