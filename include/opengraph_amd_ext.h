@@ -149,8 +149,15 @@ typedef struct
 
 void ogpu_pr_default_options(ogpu_pr_options *opt);
 int ogpu_pr_create(ogpu_graph *g, const ogpu_pr_options *opt, ogpu_pr **out);
-/* Set the start vector for init_mode 2 (host pointer, N floats, old ids). */
+/* The following runs start from this vector (host pointer, N floats, old
+ * ids, unnormalized; init_mode becomes 2).  A vector with a NaN, infinite or
+ * negative entry, or with no positive mass, is rejected: the runs then start
+ * from the uniform vector (init_mode 1) and ogpu_pr_init_fallback returns 1. */
 int ogpu_pr_set_init(ogpu_pr *pr, const float *init);
+/* 1 when the last start vector offered (ogpu_pr_set_init or
+ * ogpu_pr_set_init_learned) was rejected in favour of the uniform start,
+ * 0 when it is used, -1 for a NULL run. */
+int ogpu_pr_init_fallback(const ogpu_pr *pr);
 /* Run to the stop rule or max_iter and normalize; ranks stay on device. */
 int ogpu_pr_run(ogpu_pr *pr, ogpu_pr_result *res);
 /* Benchmark helpers: reset the state, then enqueue n iterations with the
@@ -239,7 +246,10 @@ void ogpu_mlp_free(ogpu_mlp *m);
 int ogpu_pr_features(ogpu_pr *pr, float *feat);
 /* Computes the learned start vector on the device (features, bf16 MFMA
  * inference, mass scaling) and makes the following runs of pr start from
- * it; *seconds (may be NULL): host time of this call. */
+ * it; *seconds (may be NULL): host time of this call.  A prediction with a
+ * NaN, infinite or negative value on a vertex with in-edges, or with no
+ * mass there, is rejected on the device: the runs then start from the
+ * uniform vector (init_mode 1) and ogpu_pr_init_fallback returns 1. */
 int ogpu_pr_set_init_learned(ogpu_pr *pr, ogpu_mlp *m, double *seconds);
 
 /* ---- batched personalized PageRank (no counterpart in the reference) ----

@@ -211,6 +211,7 @@ SIGNATURES = {
     "ogpu_pr_default_options": (None, [C.POINTER(PrOptions)]),
     "ogpu_pr_create": (C.c_int, [_VP, C.POINTER(PrOptions), C.POINTER(_VP)]),
     "ogpu_pr_set_init": (C.c_int, [_VP, _VP]),
+    "ogpu_pr_init_fallback": (C.c_int, [_VP]),
     "ogpu_pr_run": (C.c_int, [_VP, C.POINTER(PrResult)]),
     "ogpu_pr_reset": (C.c_int, [_VP]),
     "ogpu_pr_enqueue": (C.c_int, [_VP, u32, C.c_int]),
@@ -488,8 +489,14 @@ class PageRank:
             self.set_init(init)
 
     def set_init(self, init: np.ndarray):
+        """Start vector of the following runs (unnormalized, caller's ids); a
+        NaN, infinite or negative entry or no mass falls back to uniform."""
         a = np.ascontiguousarray(init, dtype=np.float32)
         check(load().ogpu_pr_set_init(self.h, _ptr(a)))
+
+    def init_fallback(self) -> bool:
+        """True when the last start offered was rejected for the uniform one."""
+        return load().ogpu_pr_init_fallback(self.h) == 1
 
     def run(self) -> PrResult:
         r = PrResult()
@@ -811,6 +818,26 @@ def pagerank_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20, epsilon=1
     out = (ranks, s.iterations, s.error_total, s.time_total)
     lib.freePageRankStats(st)
     return out
+
+
+def pagerank_graph_csr_stats(graph: HostGraphCSR, pushpull=0, iterations=20, epsilon=1e-4):
+    """Call the drop-in pageRankGraphCSR; returns (ranks, fields) with every
+    PageRankStats field (pageRank.h:19-30) and whether realRanks is the
+    identity."""
+    lib = load()
+    args = default_arguments(pushpull=pushpull, iterations=iterations, epsilon=epsilon)
+    st = lib.pageRankGraphCSR(C.byref(args), C.byref(graph.g))
+    if not st:
+        raise OgpuError("pageRankGraphCSR returned NULL")
+    s = st.contents
+    n = s.num_vertices
+    ranks = np.ctypeslib.as_array(s.pageRanks, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    real = np.ctypeslib.as_array(s.realRanks, shape=(n,)) if n else np.zeros(0, np.uint32)
+    fields = dict(damp=s.damp, base_pr=s.base_pr, iterations=s.iterations, num_vertices=n,
+                  activeVertices=s.activeVertices, error_total=s.error_total, time_total=s.time_total,
+                  realRanks_identity=bool(np.array_equal(real, np.arange(n, dtype=np.uint32))))
+    lib.freePageRankStats(st)
+    return ranks, fields
 
 
 def spmv_graph_csr(graph: HostGraphCSR, pushpull=0, iterations=20):

@@ -392,11 +392,101 @@ int ogpu_graph_array(const ogpu_graph *gg, uint32_t which, uint32_t *host_out)
     return og_hip_check(hipStreamSynchronize(og_stream()), "sync");
 }
 
+/* A CSR pair is well formed when, for both the forward and the inverse list
+ * (graphCSR.c:107-161, vertex.c:204-325): the vertex count fits the 30-bit
+ * id space (reorder.h:18-27); every src/dst id is < N; the out-degrees sum
+ * to E and vertex v's edges [edges_idx[v], edges_idx[v] + out_degree[v])
+ * lie inside [0, E) and all have src == v; and each list's in-degree is the
+ * other list's out-degree.  The plan and pull kernels index with these
+ * arrays unchecked, so inputs from files or callers are checked here, on
+ * the host, before anything is uploaded.  Arrays are read with a stride
+ * (words between consecutive entries) so the interleaved .csr image can be
+ * checked in place. */
+typedef struct
+{
+    const uint32_t *a;
+    size_t stride;
+} og_strided;
+
+#define OG_AT(x, i) ((x).a[(size_t)(i) * (x).stride])
+
+static int og_csr_check_list(uint32_t N, uint32_t E, og_strided src, og_strided dst, og_strided deg,
+                             og_strided idx, og_strided other_indeg, const char *what)
+{
+    uint64_t total = 0;
+    int64_t bad_v = -1, bad_e = -1;
+#pragma omp parallel for reduction(+ : total) schedule(static)
+    for (uint32_t v = 0; v < N; v++)
+    {
+        const uint32_t d = OG_AT(deg, v), b = OG_AT(idx, v);
+        total += d;
+        int ok = (uint64_t)b + d <= E && OG_AT(other_indeg, v) == d;
+        for (uint32_t j = b; ok && j < b + d; j++) ok = OG_AT(src, j) == v;
+        if (!ok)
+        {
+#pragma omp critical
+            if (bad_v < 0 || v < bad_v) bad_v = v;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (uint32_t e = 0; e < E; e++)
+        if (OG_AT(src, e) >= N || OG_AT(dst, e) >= N)
+        {
+#pragma omp critical
+            if (bad_e < 0 || e < bad_e) bad_e = e;
+        }
+    if (bad_e >= 0)
+    {
+        og_set_error("%s edge %lld: id (%u, %u) not below N = %u", what, (long long)bad_e,
+                     OG_AT(src, bad_e), OG_AT(dst, bad_e), N);
+        return -1;
+    }
+    if (bad_v >= 0)
+    {
+        og_set_error("%s vertex %lld: degree %u at edges_idx %u is not a run of its edges in [0, %u) "
+                     "(or the other list's in-degree differs)",
+                     what, (long long)bad_v, OG_AT(deg, bad_v), OG_AT(idx, bad_v), E);
+        return -1;
+    }
+    if (total != E)
+    {
+        og_set_error("%s out-degrees sum to %llu, not E = %u", what, (unsigned long long)total, E);
+        return -1;
+    }
+    return 0;
+}
+
+static int og_csr_check(uint32_t N, uint32_t E, const og_strided a[10])
+{
+    if (N == 0 && E != 0) { og_set_error("a CSR with %u edges and no vertices", E); return -1; }
+    if (N && N - 1 > OG_ID_MASK)
+    {
+        og_set_error("%u vertices do not fit the 30-bit id space (reorder.h:18-27)", N);
+        return -1;
+    }
+    return og_csr_check_list(N, E, a[OGPU_F_SRC], a[OGPU_F_DST], a[OGPU_F_OUT_DEGREE], a[OGPU_F_EDGES_IDX],
+                             a[OGPU_I_IN_DEGREE], "forward") ||
+           og_csr_check_list(N, E, a[OGPU_I_SRC], a[OGPU_I_DST], a[OGPU_I_OUT_DEGREE], a[OGPU_I_EDGES_IDX],
+                             a[OGPU_F_IN_DEGREE], "inverse");
+}
+
 int ogpu_graph_from_arrays(uint32_t N, uint32_t E, const uint32_t *const host_arrays[10],
                            ogpu_graph **out)
 {
     *out = NULL;
-    if (og_ctx_ensure()) return -1;
+    og_strided chk[10];
+    for (int i = 0; i < 10; i++)
+    {
+        size_t n = (i == 0 || i == 1 || i == 5 || i == 6) ? E : N;
+        if (n && !host_arrays[i])
+        {
+            og_set_error("graph array %d missing", i);
+            return -1;
+        }
+        chk[i].a = host_arrays[i];
+        chk[i].stride = 1;
+    }
+    if (og_csr_check(N, E, chk) || og_ctx_ensure()) return -1;
     og_graph *g = og_graph_alloc(N, E);
     if (!g) return -1;
     uint32_t *dst[10] = {g->f_src, g->f_dst, g->f_out_degree, g->f_in_degree, g->f_edges_idx,
@@ -473,9 +563,9 @@ done:
 
 int ogpu_graph_read_csr(const char *path, ogpu_graph **out)
 {
-    /* graphCSR.c:452-558 into a device graph */
+    /* graphCSR.c:452-558 into a device graph; the file is checked on the
+     * host before the device is touched */
     *out = NULL;
-    if (og_ctx_ensure()) return -1;
     FILE *f = fopen(path, "rb");
     if (!f) { og_set_error("open: %s", path); return -1; }
     uint32_t hdr[2];
@@ -485,11 +575,31 @@ int ogpu_graph_read_csr(const char *path, ogpu_graph **out)
     uint32_t *h = (uint32_t *)malloc(words * 4 + 4);
     if (!h) { fclose(f); og_set_error("out of host memory"); return -1; }
     size_t got = fread(h, 4, words, f);
+    const int trailing = fgetc(f) != EOF;
     fclose(f);
-    if (got != words)
+    if (got != words || trailing)
     {
         free(h);
-        og_set_error("%s: %zu of %zu words after the header", path, got, words);
+        og_set_error("%s: %s: the file must hold exactly 2 + 6 N + 4 E = %zu words for E = %u, N = %u "
+                     "(directed, unweighted)", path, trailing ? "bytes after the last edge" : "truncated",
+                     words + 2, E, N);
+        return -1;
+    }
+    /* file layout: 6 words per vertex, then 4 per edge */
+    const uint32_t *ev = h + 6 * (size_t)N;
+    const og_strided chk[10] = {{ev + 0, 4}, {ev + 1, 4}, {h + 0, 6}, {h + 1, 6}, {h + 2, 6},
+                                {ev + 2, 4}, {ev + 3, 4}, {h + 3, 6}, {h + 4, 6}, {h + 5, 6}};
+    if (og_csr_check(N, E, chk))
+    {
+        free(h);
+        char msg[512];
+        snprintf(msg, sizeof(msg), "%s: %s", path, ogpu_last_error());
+        og_set_error("%s", msg);
+        return -1;
+    }
+    if (og_ctx_ensure())
+    {
+        free(h);
         return -1;
     }
     og_graph *g = og_graph_alloc(N, E);

@@ -21,8 +21,16 @@ extern "C" {
 void og_set_error(const char *fmt, ...);
 const char *og_get_error(void);
 /* Makes sure the device and stream exist; returns 0 or -1 with the error set. */
+/* The library works on one device per process (ogpu_init), on one stream;
+ * its entry points are not reentrant: callers on several host threads
+ * serialize their calls, as the reference's single caller does. */
 int og_ctx_ensure(void);
 hipStream_t og_stream(void);
+/* the second stream and the fork / join events of the pull (created with
+ * the context) */
+hipStream_t og_side_stream(void);
+hipEvent_t og_side_fork(void);
+hipEvent_t og_side_join(void);
 int og_num_cus(void);
 /* hipMalloc / hipFree with error text; og_dalloc returns NULL on failure. */
 void *og_dalloc(size_t bytes);

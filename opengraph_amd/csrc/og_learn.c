@@ -126,7 +126,7 @@ static int og_plan_features(const og_pr_dev *base, const og_plan *pl, uint32_t n
 
 int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, uint32_t *n_final,
                       float **c1, float **init_d, float **feat, double **scratch);
-void og_pr_use_init(ogpu_pr *p);
+void og_pr_use_init(ogpu_pr *p, int accepted);
 
 int ogpu_mlp_add_graph(ogpu_mlp *m, ogpu_graph *g, double tol, uint32_t max_iter, double *l1_out)
 {
@@ -315,12 +315,16 @@ int ogpu_pr_set_init_learned(ogpu_pr *p, ogpu_mlp *m, double *seconds)
     hipStream_t s = og_stream();
     double t0 = og_learn_now();
     /* features -> o (bf16 MFMA inference, into c1) -> start ranks (old ids) */
+    double ok = 0.0;
     if (og_plan_features(dev, pl, n_final, c1, feat, s) ||
         og_launch_mlp_infer(m->params, feat, pl->N, 1, c1, s) ||
         og_launch_warm_scale(c1, pl->row_ptr, pl->outdeg, pl->new2old, pl->N, scr, init_d, s) ||
+        og_hip_check(hipMemcpyAsync(&ok, scr + OG_WARM_OK, 8, hipMemcpyDeviceToHost, s), "D2H") ||
         og_hip_check(hipStreamSynchronize(s), "sync warm start"))
         return -1;
-    og_pr_use_init(p);
+    /* a rejected prediction (non-finite, negative, no mass) leaves the
+     * uniform start in place */
+    og_pr_use_init(p, ok != 0.0);
     if (seconds) *seconds = og_learn_now() - t0;
     return 0;
 }

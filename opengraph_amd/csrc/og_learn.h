@@ -51,7 +51,12 @@ int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, f
 /* Start ranks in old ids: 1 - d (unnormalized) for vertices without
  * in-edges, N * S_U * o[v] / sum_U(o) for the others, S_U from the
  * fixed-point mass identity (fp64 sums in a fixed order; scratch holds
- * 4 * 1024 + 1 doubles). */
+ * OG_WARM_SCRATCH doubles).  A prediction with a NaN, infinite or negative
+ * value on a vertex with in-edges, or with no mass there, is rejected: the
+ * start written is then the uniform 1.0, and scratch[OG_WARM_OK] is 0
+ * (1 when accepted). */
+#define OG_WARM_SCRATCH (5 * 1024 + 64)
+#define OG_WARM_OK (5 * 1024 + 1)
 int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
                          const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
                          hipStream_t s);

@@ -329,24 +329,32 @@ __global__ void og_mlp_adam_kernel(float *__restrict__ P, const float *__restric
 /* ------------------------------------------------------------------ */
 
 #define WS_BLOCKS 1024
+#define WS_SUMS 5
+static_assert(WS_SUMS * WS_BLOCKS + 1 == OG_WARM_OK && OG_WARM_OK < OG_WARM_SCRATCH, "warm scratch layout");
 
 /* Block partials of: sum of o over vertices with in-edges (U), the same
- * over those of them without out-edges, and the counts of vertices without
- * in-edges (K) and of those without out-edges either. */
+ * over those of them without out-edges, the counts of vertices without
+ * in-edges (K) and of those without out-edges either, and the count of
+ * vertices of U whose prediction is not a finite non-negative number. */
 __global__ __launch_bounds__(LB) void og_warm_sums_kernel(const float *__restrict__ o,
                                                          const uint32_t *__restrict__ row_ptr,
                                                          const uint32_t *__restrict__ outdeg, uint32_t n,
                                                          double *__restrict__ part)
 {
-    __shared__ double sh[4][LB];
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    __shared__ double sh[WS_SUMS][LB];
+    double a[WS_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
     {
         const bool in = row_ptr[v + 1] != row_ptr[v], dang = outdeg[v] == 0;
         if (in)
         {
-            a[0] += o[v];
-            if (dang) a[1] += o[v];
+            const float x = o[v];
+            if (!(x >= 0.0f && x <= 3.402823466e38f)) a[4] += 1.0;   /* NaN, negative or inf */
+            else
+            {
+                a[0] += x;
+                if (dang) a[1] += x;
+            }
         }
         else
         {
@@ -354,15 +362,15 @@ __global__ __launch_bounds__(LB) void og_warm_sums_kernel(const float *__restric
             if (dang) a[3] += 1.0;
         }
     }
-    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = a[q];
+    for (int q = 0; q < WS_SUMS; q++) sh[q][threadIdx.x] = a[q];
     __syncthreads();
     for (uint32_t w = LB / 2; w > 0; w >>= 1)
     {
         if (threadIdx.x < w)
-            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+            for (int q = 0; q < WS_SUMS; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < WS_SUMS) part[WS_SUMS * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 /* Mass of the rows with in-edges at the fixed point.  Vertices without
@@ -370,40 +378,57 @@ __global__ __launch_bounds__(LB) void og_warm_sums_kernel(const float *__restric
  * their share f_K and the share f_Kd of those also without out-edges, the
  * mass identity S = 0.15 + 0.85 (S - D) gives for the rest
  *   S_U = 0.15 (1 - 0.15 f_K - 0.85 f_Kd) / (0.15 + 0.85 delta_U),
- * delta_U = share of the predicted shape of U on vertices without out-edges. */
+ * delta_U = share of the predicted shape of U on vertices without out-edges.
+ * The prediction is accepted only when every value on U is finite and
+ * non-negative and their sum is positive (a start with the fixed point's
+ * mass exists); otherwise part[OG_WARM_OK] = 0 and the start
+ * written is the uniform one. */
 __global__ __launch_bounds__(LB) void og_warm_factor_kernel(double *__restrict__ part, uint32_t n)
 {
-    __shared__ double sh[4][LB];
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    __shared__ double sh[WS_SUMS][LB];
+    double a[WS_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t i = threadIdx.x; i < WS_BLOCKS; i += LB)
-        for (int q = 0; q < 4; q++) a[q] += part[4 * i + q];
-    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = a[q];
+        for (int q = 0; q < WS_SUMS; q++) a[q] += part[WS_SUMS * i + q];
+    for (int q = 0; q < WS_SUMS; q++) sh[q][threadIdx.x] = a[q];
     __syncthreads();
     for (uint32_t w = LB / 2; w > 0; w >>= 1)
     {
         if (threadIdx.x < w)
-            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+            for (int q = 0; q < WS_SUMS; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
     {
-        const double sumU = sh[0][0] > 0.0 ? sh[0][0] : 1.0;
-        const double deltaU = sh[1][0] / sumU;
-        const double fK = sh[2][0] / (double)n, fKd = sh[3][0] / (double)n;
-        const double SU = 0.15 * (1.0 - 0.15 * fK - 0.85 * fKd) / (0.15 + 0.85 * deltaU);
-        part[4 * WS_BLOCKS] = (double)n * SU / sumU;     /* r0 = factor * o on U */
+        const double sumU = sh[0][0];
+        const double nU = (double)n - sh[2][0];
+        bool ok = sh[4][0] == 0.0 && (nU == 0.0 || (sumU > 0.0 && sumU <= 1.7976931348623157e308));
+        double fac = 0.0;
+        if (ok && nU > 0.0)
+        {
+            const double deltaU = sh[1][0] / sumU;
+            const double fK = sh[2][0] / (double)n, fKd = sh[3][0] / (double)n;
+            const double SU = 0.15 * (1.0 - 0.15 * fK - 0.85 * fKd) / (0.15 + 0.85 * deltaU);
+            fac = (double)n * SU / sumU;                 /* r0 = factor * o on U */
+            ok = fac > 0.0 && fac <= 3.402823466e38;
+        }
+        part[WS_SUMS * WS_BLOCKS] = fac;
+        part[OG_WARM_OK] = ok ? 1.0 : 0.0;
     }
 }
 
+/* The start vector in old ids: the scaled prediction, or, if it was
+ * rejected, the uniform start (1.0 unnormalized = 1/N normalized, as
+ * init_mode 1 of the pull). */
 __global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t *__restrict__ row_ptr,
                                      const uint32_t *__restrict__ new2old, uint32_t n,
                                      const double *__restrict__ part, float *__restrict__ r0_old)
 {
-    const float fac = (float)part[4 * WS_BLOCKS];
+    const float fac = (float)part[WS_SUMS * WS_BLOCKS];
+    const bool ok = part[OG_WARM_OK] != 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     {
         /* unnormalized: N * (1 - d) / N for vertices without in-edges */
-        const float r = row_ptr[v + 1] != row_ptr[v] ? fac * o[v] : OG_BASE_PR;
+        const float r = !ok ? 1.0f : row_ptr[v + 1] != row_ptr[v] ? fac * o[v] : OG_BASE_PR;
         r0_old[new2old ? new2old[v] : v] = r;
     }
 }

@@ -16,6 +16,7 @@
 #include <time.h>
 
 #include "og_internal.h"
+#include "og_learn.h"
 #include "og_pr.h"
 #include "og_pr_run.h"
 #include "opengraph_amd_ext.h"
@@ -136,7 +137,7 @@ int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, u
     const size_t N = p->g->N;
     if (!p->init_d && !(p->init_d = og_dalloc(N * 4 + 4))) return -1;
     if (!p->feat && !(p->feat = og_dalloc(N * 16 + 16))) return -1;
-    if (!p->lscratch && !(p->lscratch = og_dalloc((4 * 1024 + 64) * 8))) return -1;
+    if (!p->lscratch && !(p->lscratch = og_dalloc(OG_WARM_SCRATCH * 8))) return -1;
     *dev = &p->dev;
     *plan = p->plan;
     *n_final = p->n_final_pull;
@@ -147,10 +148,17 @@ int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, u
     return 0;
 }
 
-/* The next runs start from the vector in init_d. */
-void og_pr_use_init(ogpu_pr *p)
+/* The next runs start from the vector in init_d (accepted) or, when the
+ * start offered was rejected, from the uniform start. */
+void og_pr_use_init(ogpu_pr *p, int accepted)
 {
-    p->opt.init_mode = 2;
+    p->opt.init_mode = accepted ? 2 : 1;
+    p->init_fallback = !accepted;
+}
+
+int ogpu_pr_init_fallback(const ogpu_pr *p)
+{
+    return p ? (int)p->init_fallback : -1;
 }
 
 /* The rows [0, C) of a plan ordered by in-degree, longest first (stable):
@@ -471,15 +479,33 @@ int ogpu_pr_create(ogpu_graph *g, const ogpu_pr_options *opt, ogpu_pr **out)
 
 int ogpu_pr_set_init(ogpu_pr *p, const float *init)
 {
+    const uint32_t N = p->g->N;
+    /* a start the iteration can use: finite, non-negative, some mass */
+    double sum = 0.0;
+    int ok = 1;
+    for (uint32_t v = 0; v < N && ok; v++)
+    {
+        const float x = init[v];
+        if (!(x >= 0.0f && x <= 3.402823466e38f)) ok = 0;
+        else sum += x;
+    }
+    if (!(sum > 0.0)) ok = 0;
+    if (!ok)
+    {
+        og_pr_use_init(p, 0);
+        return 0;
+    }
     if (!p->init_d)
     {
-        p->init_d = og_dalloc((size_t)p->g->N * 4);
+        p->init_d = og_dalloc((size_t)N * 4);
         if (!p->init_d) return -1;
     }
-    if (og_hip_check(hipMemcpyAsync(p->init_d, init, (size_t)p->g->N * 4, hipMemcpyHostToDevice,
-                                    og_stream()), "H2D init"))
+    if (og_hip_check(hipMemcpyAsync(p->init_d, init, (size_t)N * 4, hipMemcpyHostToDevice, og_stream()),
+                     "H2D init") ||
+        og_hip_check(hipStreamSynchronize(og_stream()), "sync init"))
         return -1;
-    return og_hip_check(hipStreamSynchronize(og_stream()), "sync init");
+    og_pr_use_init(p, 1);
+    return 0;
 }
 
 /* Initial ranks and contributions (and, for the quantized variants, the
@@ -590,6 +616,51 @@ static int og_pr_finish(ogpu_pr *p, double *pr_sum)
     return og_hip_check(hipStreamSynchronize(og_stream()), "sync finish");
 }
 
+static double og_pr_row_seconds(ogpu_pr *p, uint32_t i);
+
+/* The stop rule applied to one statistics row. */
+static int og_pr_row_stops(const ogpu_pr *p, const og_iter_stat *r)
+{
+    if (!p->dd && (p->opt.stop_rule == 1 || p->opt.ppr)) return r->error_total < p->opt.epsilon;
+    return r->active == 0;
+}
+
+/* Drop-in runs: the reference's per-iteration rows (pageRank.c:1031 and
+ * the same line of every variant) for the iterations [printed, upto) that
+ * have completed, up to and including the one that stops the run; the
+ * data-driven variants first print their "Init" row (pageRank.c:2468,
+ * :2636, :2800). */
+static int og_pr_print_rows(ogpu_pr *p, uint32_t upto)
+{
+    if (!p->print_rows || p->print_done || upto <= p->printed) return 0;
+    hipStream_t s = og_stream();
+    const uint32_t from = p->printed, n = upto - from;
+    og_iter_stat *rows = (og_iter_stat *)malloc((size_t)n * sizeof(og_iter_stat));
+    if (!rows) { og_set_error("out of host memory"); return -1; }
+    if (og_hip_check(hipMemcpyAsync(rows, p->stats + from, (size_t)n * sizeof(og_iter_stat),
+                                    hipMemcpyDeviceToHost, s), "D2H stats") ||
+        og_hip_check(hipStreamSynchronize(s), "sync stats"))
+    {
+        free(rows);
+        return -1;
+    }
+    if (p->dd && from == 0)
+    {
+        float ms = 0.0f;
+        if (p->n_iter_ev > 1) (void)hipEventElapsedTime(&ms, p->iter_ev[0], p->iter_ev[1]);
+        printf("| %-10s | %-8u | %-15.13lf | %-9f | \n", "Init", p->g->N, 0.0, ms * 1e-3);
+    }
+    for (uint32_t i = 0; i < n && !p->print_done; i++)
+    {
+        printf("| %-10u | %-8u | %-15.13lf | %-9f | \n", from + i, rows[i].active, rows[i].error_total,
+               og_pr_row_seconds(p, from + i));
+        if (og_pr_row_stops(p, &rows[i])) p->print_done = 1;
+    }
+    p->printed = upto;
+    free(rows);
+    return 0;
+}
+
 /* Init and iterations until the device stop flag or max_iter; returns the
  * iterations launched (or -1).  The host reads the flag every
  * OG_CHECK_EVERY iterations. */
@@ -611,7 +682,7 @@ static int64_t og_pr_solve(ogpu_pr *p)
         }
         launched += n;
         if (og_hip_check(hipMemcpyAsync(p->done_host, p->done, 4, hipMemcpyDeviceToHost, s), "D2H done") ||
-            og_hip_check(hipStreamSynchronize(s), "sync iterations"))
+            og_hip_check(hipStreamSynchronize(s), "sync iterations") || og_pr_print_rows(p, launched))
             return -1;
         if (*p->done_host) break;
     }
@@ -650,9 +721,7 @@ static int og_pr_collect(ogpu_pr *p, uint32_t launched, double pr_sum, ogpu_pr_r
     uint32_t rows = launched, iterations = max_iter;
     for (uint32_t i = 0; i < launched; i++)
     {
-        int stop = p->opt.stop_rule == 1 || p->opt.ppr ? (p->rows_host[i].error_total < p->opt.epsilon)
-                                                       : (p->rows_host[i].active == 0);
-        if (stop)
+        if (og_pr_row_stops(p, &p->rows_host[i]))
         {
             rows = i + 1;
             iterations = i;
@@ -816,7 +885,7 @@ static int og_dd_run(ogpu_pr *p, ogpu_pr_result *res)
         }
         launched += n;
         if (og_hip_check(hipMemcpyAsync(p->done_host, p->done, 4, hipMemcpyDeviceToHost, s), "D2H done") ||
-            og_hip_check(hipStreamSynchronize(s), "sync iterations"))
+            og_hip_check(hipStreamSynchronize(s), "sync iterations") || og_pr_print_rows(p, launched))
             return -1;
         if (*p->done_host) break;
     }
@@ -1009,19 +1078,14 @@ static struct PageRankStats *og_dropin(struct Arguments *arguments, struct Graph
     printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iteration", "Active", "Error", "Time (S)");
     printf(" -----------------------------------------------------\n");
 
+    /* the rows are printed as the iterations complete (read back with each
+     * stop-flag check) */
+    pr->print_rows = 1;
+    pr->print_done = 0;
+    pr->printed = 0;
     ogpu_pr_result res;
     if (ogpu_pr_run(pr, &res)) og_fatal("pagerank run");
     if (ogpu_pr_ranks(pr, stats->pageRanks)) og_fatal("pagerank ranks");
-    if (dd)
-    {
-        /* pageRank.c:2468: the worklist initialisation row */
-        float ms = 0.0f;
-        if (pr->n_iter_ev > 1) (void)hipEventElapsedTime(&ms, pr->iter_ev[0], pr->iter_ev[1]);
-        printf("| %-10s | %-8u | %-15.13lf | %-9f | \n", "Init", graph->num_vertices, 0.0, ms * 1e-3);
-    }
-    for (uint32_t i = 0; i < pr->rows; i++)
-        printf("| %-10u | %-8u | %-15.13lf | %-9f | \n", i, pr->rows_host[i].active,
-               pr->rows_host[i].error_total, og_pr_row_seconds(pr, i));
     stats->iterations = res.iterations;
     stats->activeVertices = res.active;
     stats->error_total = res.error_total;
@@ -1032,6 +1096,7 @@ static struct PageRankStats *og_dropin(struct Arguments *arguments, struct Graph
     printf("| %-10u | %-8lf | %-15.13lf | %-9f | \n", stats->iterations, res.pr_sum, stats->error_total,
            stats->time_total);
     printf(" -----------------------------------------------------\n");
+    fflush(stdout);
     ogpu_pr_free(pr);
     return stats;
 }

@@ -25,6 +25,10 @@ extern "C" {
 #define OG_MODE_Q16 6        /* -p 12 */
 #define OG_MODE_Q8 7         /* -p 13 */
 #define OG_MODE_QPUSH_EPI 8  /* -p 5 */
+/* graphs of at most this many vertices run the data-driven push and
+ * pull-push in vertex order on one thread: the reference's one OpenMP
+ * chunk (schedule(dynamic, 1024), pageRank.c:2645, :2808) */
+#define OG_DD_SERIAL_N 1024u
 
 /* Iteration index that writes no statistics row (the init-time reduction). */
 #define OG_NO_STATS 0xFFFFFFFFu

@@ -1174,36 +1174,6 @@ __global__ void og_sum_parts_kernel(const double *p, uint32_t n, double *out)
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
 
-/* Side stream (default priority) and fork/join events of the current
- * device, made on first use and kept until the process ends. */
-typedef struct
-{
-    hipStream_t s;
-    hipEvent_t fork, join;
-} og_side_t;
-
-static og_side_t *og_side()
-{
-    static og_side_t sd[64];
-    static int made[64];
-    int dev = 0;
-    if (og_hip_check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
-    if (dev < 0 || dev >= 64)
-    {
-        og_set_error("device %d out of range for the side stream", dev);
-        return nullptr;
-    }
-    if (!made[dev])
-    {
-        if (og_hip_check(hipStreamCreateWithFlags(&sd[dev].s, hipStreamNonBlocking), "hipStreamCreateWithFlags") ||
-            og_hip_check(hipEventCreateWithFlags(&sd[dev].fork, hipEventDisableTiming), "hipEventCreate") ||
-            og_hip_check(hipEventCreateWithFlags(&sd[dev].join, hipEventDisableTiming), "hipEventCreate"))
-            return nullptr;
-        made[dev] = 1;
-    }
-    return &sd[dev];
-}
-
 template <int MODE> static int og_launch_pull_t(const og_pr_dev *a, uint32_t it, hipStream_t s,
                                                 hipEvent_t ev0, hipEvent_t ev1)
 {
@@ -1213,16 +1183,15 @@ template <int MODE> static int og_launch_pull_t(const og_pr_dev *a, uint32_t it,
     {
         /* fork: chains on the side stream beside the persistent kernel, joined
          * before the finalize */
-        og_side_t *sd = og_side();
-        if (!sd) return -1;
-        if (og_hip_check(hipEventRecord(sd->fork, s), "hipEventRecord") ||
-            og_hip_check(hipStreamWaitEvent(sd->s, sd->fork, 0), "hipStreamWaitEvent"))
+        hipStream_t side = og_side_stream();
+        if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
+            og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
             return -1;
-        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, sd->s, *a, it);
+        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, side, *a, it);
         hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
-        if (og_hip_check(hipEventRecord(sd->join, sd->s), "hipEventRecord") ||
-            og_hip_check(hipStreamWaitEvent(s, sd->join, 0), "hipStreamWaitEvent"))
+        if (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
+            og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent"))
             return -1;
     }
     else if (a->hot_grid)
@@ -1419,50 +1388,71 @@ __global__ void og_dd_resid_init_kernel(og_pr_dev a)
     }
 }
 
-/* pageRank.c:2645-2685 (push) and :2796-2852 (pull-push): one thread per
- * worklist vertex, residuals moved to out-neighbours with atomic adds.  The
- * reference's reads of neighbours' residuals and ranks race with their
- * updates; here they race the same way, so results are nondeterministic. */
+/* One worklist vertex v of the data-driven push (pageRank.c:2648-2677) or
+ * pull-push (:2810-2850): its new rank, then its residual moved to its
+ * out-neighbours with atomic adds. */
+__device__ __forceinline__ void og_dd_push_vertex(const og_pr_dev &a, uint32_t code, uint32_t v, float fn,
+                                                  og_stat &st)
+{
+    float old = a.r[v];
+    float nr;
+    if (code == 7)
+        nr = a.resid[v] + old;
+    else
+    {
+        float s = 0.0f;
+        for (uint32_t j = a.row_ptr[v]; j < a.row_ptr[v + 1]; j++)
+        {
+            uint32_t u = a.cols[j];
+            s += __hip_atomic_load(&a.r[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / (float)a.outdeg[u];
+        }
+        nr = OG_BASE_PR + (OG_DAMP * s);
+    }
+    st.err += fabs((double)(nr / fn - old / fn));
+    __hip_atomic_store(&a.r[v], nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t deg = a.outdeg[v];
+    float delta = OG_DAMP * (a.resid[v] / (float)deg);
+    for (uint32_t j = a.f_row_ptr[v]; j < a.f_row_ptr[v + 1]; j++)
+    {
+        uint32_t u = a.f_cols[j];
+        float prev = atomicAdd(&a.resid[u], delta);
+        if ((fabs((double)(prev + delta)) >= a.epsilon) && ((double)prev <= a.epsilon))
+        {
+            st.active++;
+            if (code == 8) atomicAdd(&a.resid[u], delta);
+            a.wl_next[u] = 1;
+        }
+    }
+    a.resid[v] = 0.0f;
+}
+
+/* pageRank.c:2645-2685 (push) and :2796-2852 (pull-push).  The reference
+ * runs its vertex loop with schedule(dynamic, 1024): a graph of at most
+ * 1024 vertices is one chunk, taken by one thread in vertex order, so its
+ * result is deterministic; such graphs run here the same way, one thread
+ * over the caller's vertex order (old ids), every read seeing the updates
+ * of the vertices before it.  Larger graphs: one thread per worklist
+ * vertex; the reads of neighbours' residuals and ranks race with their
+ * updates, as the reference's threads do, so results are nondeterministic
+ * (in the reference as well). */
 __global__ __launch_bounds__(OG_BLOCK) void og_dd_push_kernel(og_pr_dev a, uint32_t code)
 {
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
     const float fn = (float)a.N;
-    for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
+    if (a.N <= OG_DD_SERIAL_N)
     {
-        if (!a.wl_cur[v]) continue;
-        float old = a.r[v];
-        float nr;
-        if (code == 7)
-            nr = a.resid[v] + old;
-        else
-        {
-            float s = 0.0f;
-            for (uint32_t j = a.row_ptr[v]; j < a.row_ptr[v + 1]; j++)
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (uint32_t vo = 0; vo < a.N; vo++)
             {
-                uint32_t u = a.cols[j];
-                s += __hip_atomic_load(&a.r[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / (float)a.outdeg[u];
+                const uint32_t v = a.old2new[vo];
+                if (a.wl_cur[v]) og_dd_push_vertex(a, code, v, fn, st);
             }
-            nr = OG_BASE_PR + (OG_DAMP * s);
-        }
-        st.err += fabs((double)(nr / fn - old / fn));
-        __hip_atomic_store(&a.r[v], nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t deg = a.outdeg[v];
-        float delta = OG_DAMP * (a.resid[v] / (float)deg);
-        for (uint32_t j = a.f_row_ptr[v]; j < a.f_row_ptr[v + 1]; j++)
-        {
-            uint32_t u = a.f_cols[j];
-            float prev = atomicAdd(&a.resid[u], delta);
-            if ((fabs((double)(prev + delta)) >= a.epsilon) && ((double)prev <= a.epsilon))
-            {
-                st.active++;
-                if (code == 8) atomicAdd(&a.resid[u], delta);
-                a.wl_next[u] = 1;
-            }
-        }
-        a.resid[v] = 0.0f;
     }
+    else
+        for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
+            if (a.wl_cur[v]) og_dd_push_vertex(a, code, v, fn, st);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {

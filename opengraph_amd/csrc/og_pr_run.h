@@ -59,6 +59,14 @@ struct ogpu_pr
     double *lscratch;
     /* every-row-exact fp32 pull: rows [0, C) longest first (device) */
     uint32_t *chain_rows;
+    /* the last start vector offered (ogpu_pr_set_init, the learned start)
+     * was rejected and the runs start from the uniform vector */
+    uint32_t init_fallback;
+    /* drop-in runs print the reference's per-iteration table rows as the
+     * iterations complete: rows [0, printed) are out, print_done after the
+     * stopping row */
+    int print_rows, print_done;
+    uint32_t printed;
 };
 
 /* Create a run of opt->variant on graph g; plan may be supplied (borrowed)

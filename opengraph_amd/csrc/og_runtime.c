@@ -2,6 +2,7 @@
  * og_runtime.c -- device context, error text and allocation helpers of
  * libopengraph_amd (host C over the HIP runtime C API).
  */
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -15,6 +16,11 @@ static int g_ready = 0;
 static int g_device = 0;
 static int g_cus = 0;
 static hipStream_t g_stream = NULL;
+/* second stream and its fork / join events: the pull's long-row chains run
+ * there beside the persistent kernel (og_pr.hip) */
+static hipStream_t g_side = NULL;
+static hipEvent_t g_fork = NULL, g_join = NULL;
+static pthread_mutex_t g_ctx_lock = PTHREAD_MUTEX_INITIALIZER;
 
 void og_set_error(const char *fmt, ...)
 {
@@ -33,9 +39,21 @@ int og_hip_check(hipError_t e, const char *what)
     return -1;
 }
 
-int og_ctx_ensure(void)
+static void og_ctx_release(void)
 {
-    if (g_ready) return 0;
+    if (g_join) (void)hipEventDestroy(g_join);
+    if (g_fork) (void)hipEventDestroy(g_fork);
+    if (g_side) (void)hipStreamDestroy(g_side);
+    if (g_stream) (void)hipStreamDestroy(g_stream);
+    g_join = g_fork = NULL;
+    g_side = g_stream = NULL;
+}
+
+/* The device, its streams (both non-blocking, default priority) and events,
+ * created together once per process under a lock; on a failure nothing
+ * made so far is kept and the next call starts again. */
+static int og_ctx_create(void)
+{
     int n = 0;
     if (og_hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return -1;
     if (n <= 0)
@@ -48,12 +66,34 @@ int og_ctx_ensure(void)
     hipDeviceProp_t prop;
     if (og_hip_check(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties")) return -1;
     g_cus = prop.multiProcessorCount;
-    /* Default-priority, non-blocking stream for all of the library's work. */
-    if (og_hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate"))
+    if (og_hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate") ||
+        og_hip_check(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking), "hipStreamCreate (side)") ||
+        og_hip_check(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming), "hipEventCreate (fork)") ||
+        og_hip_check(hipEventCreateWithFlags(&g_join, hipEventDisableTiming), "hipEventCreate (join)"))
+    {
+        og_ctx_release();
         return -1;
-    g_ready = 1;
+    }
     return 0;
 }
+
+int og_ctx_ensure(void)
+{
+    if (__atomic_load_n(&g_ready, __ATOMIC_ACQUIRE)) return 0;
+    pthread_mutex_lock(&g_ctx_lock);
+    int rc = 0;
+    if (!g_ready)
+    {
+        rc = og_ctx_create();
+        if (!rc) __atomic_store_n(&g_ready, 1, __ATOMIC_RELEASE);
+    }
+    pthread_mutex_unlock(&g_ctx_lock);
+    return rc;
+}
+
+hipStream_t og_side_stream(void) { return g_side; }
+hipEvent_t og_side_fork(void) { return g_fork; }
+hipEvent_t og_side_join(void) { return g_join; }
 
 hipStream_t og_stream(void) { return g_stream; }
 

@@ -106,6 +106,9 @@ def ref_lib():
         R.ogref_pagerank.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
             C.c_uint32, C.c_double, C.c_uint32, C.c_int, f32p, u32p, f64p, f64p]
         R.ogref_pagerank.restype = C.c_int
+        R.ogref_pagerank_stdout.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
+            C.c_uint32, C.c_double, C.c_uint32, C.c_int, C.c_char_p, f32p, f64p]
+        R.ogref_pagerank_stdout.restype = C.c_int
         R.ogref_reorder.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, C.c_int, u32p, f64p]
         R.ogref_reorder.restype = C.c_int
         R.ogref_write_csr.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [C.c_char_p]
@@ -280,6 +283,23 @@ def ref_pagerank(csr: CSR, code=0, epsilon=1e-4, iterations=20, threads=1):
     if rc:
         raise RuntimeError("ogref_pagerank failed")
     return dict(ranks=ranks, iterations=it.value, error_total=err.value, seconds=sec.value)
+
+
+def ref_pagerank_stdout(csr: CSR, code, epsilon, iterations, threads, stdout_path):
+    """The reference's own pageRankGraphCSR with its printed table written to
+    stdout_path; returns the ranks and the PageRankStats fields."""
+    R = ref_lib()
+    N, E = csr.num_vertices, csr.num_edges
+    arrays = [csr.a[k] for k in ["f_out_degree", "f_in_degree", "f_edges_idx", "f_src", "f_dst",
+                                  "i_out_degree", "i_in_degree", "i_edges_idx", "i_src", "i_dst"]]
+    ranks = np.zeros(N, np.float32)
+    f = np.zeros(7, np.float64)
+    rc = R.ogref_pagerank_stdout(N, E, *[_p(x) for x in arrays], code, epsilon, iterations, threads,
+                                 stdout_path.encode(), _p(ranks, f32p), _p(f, f64p))
+    if rc:
+        raise RuntimeError("ogref_pagerank_stdout failed")
+    return ranks, dict(damp=float(f[0]), base_pr=float(f[1]), iterations=int(f[2]),
+                       num_vertices=int(f[3]), error_total=float(f[4]), realRanks_identity=bool(f[6]))
 
 
 def ref_write_csr(csr: CSR, fname: str):

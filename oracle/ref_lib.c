@@ -80,6 +80,72 @@ int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_i
     return 0;
 }
 
+/* The same run with the reference's stdout (its per-iteration table) sent
+ * to stdout_path, and the returned PageRankStats fields (pageRank.h:19-30)
+ * in fields_out: damp, base_pr, iterations, num_vertices, error_total,
+ * time_total, and 1 if realRanks is the identity (else 0). */
+int ogref_pagerank_stdout(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
+                          uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst,
+                          uint32_t *i_out_degree, uint32_t *i_in_degree, uint32_t *i_edges_idx,
+                          uint32_t *i_src, uint32_t *i_dst, uint32_t pushpull, double epsilon,
+                          uint32_t iterations, int threads, const char *stdout_path, float *ranks_out,
+                          double *fields_out)
+{
+    struct Vertex fv = {f_out_degree, f_in_degree, f_edges_idx};
+    struct Vertex iv = {i_out_degree, i_in_degree, i_edges_idx};
+    struct EdgeList fe, ie;
+    memset(&fe, 0, sizeof(fe));
+    memset(&ie, 0, sizeof(ie));
+    fe.num_edges = ie.num_edges = E;
+    fe.num_vertices = ie.num_vertices = N;
+    fe.avg_degree = ie.avg_degree = N ? E / N : 0;
+    fe.edges_array_src = f_src; fe.edges_array_dest = f_dst;
+    ie.edges_array_src = i_src; ie.edges_array_dest = i_dst;
+    struct GraphCSR g;
+    memset(&g, 0, sizeof(g));
+    g.num_edges = E;
+    g.num_vertices = N;
+    g.avg_degree = N ? E / N : 0;
+    g.vertices = &fv;
+    g.sorted_edges_array = &fe;
+    g.inverse_vertices = &iv;
+    g.inverse_sorted_edges_array = &ie;
+
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.iterations = iterations;
+    a.epsilon = epsilon;
+    a.pushpull = pushpull;
+    a.algorithm = 1;
+    a.pre_numThreads = a.algo_numThreads = a.ker_numThreads = threads;
+    initializeMersenneState(&a.mt19937var, 27491095);
+    omp_set_num_threads(threads);
+
+    fflush(stdout);
+    int saved = dup(1);
+    int out = open(stdout_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (saved < 0 || out < 0) return -1;
+    dup2(out, 1);
+    struct PageRankStats *s = pageRankGraphCSR(&a, &g);
+    fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    close(out);
+    if (!s) return -1;
+    memcpy(ranks_out, s->pageRanks, (size_t)N * sizeof(float));
+    int identity = 1;
+    for (uint32_t v = 0; v < N; v++) identity &= s->realRanks[v] == v;
+    fields_out[0] = s->damp;
+    fields_out[1] = s->base_pr;
+    fields_out[2] = s->iterations;
+    fields_out[3] = s->num_vertices;
+    fields_out[4] = s->error_total;
+    fields_out[5] = s->time_total;
+    fields_out[6] = identity;
+    freePageRankStats(s);
+    return 0;
+}
+
 /* The reference's own SPMVGraphCSR (SPMV.c:588-1068) with -p pushpull for
  * `iterations` iterations: out gets vector_output, seconds_out time_total. */
 int ogref_spmv(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,

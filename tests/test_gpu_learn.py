@@ -163,3 +163,53 @@ def test_training_and_warm_start():
     for _, _, g in train:
         g.free()
     g.free()
+
+
+def test_rejected_start_falls_back_to_uniform():
+    """A learned start with NaN parameters, or whose prediction has no mass
+    (softplus underflow), and a caller's start with a NaN or a negative entry
+    are rejected: the run equals the uniform-start run bit for bit."""
+    s, d, g = _graphs(12, 1, 55)[0]
+    N = g.num_vertices
+    pr = og.PageRank(g, 0, 200, 1e-6, stop_rule=1, init_mode=1)
+    ru = pr.run()
+    xu = pr.ranks()
+    assert not pr.init_fallback()
+    pr.free()
+    nan_net = og.Mlp(3)
+    nan_net.set_params(np.full(nan_net.params().size, np.nan, np.float32))
+    zero_net = og.Mlp(3)
+    p = np.zeros(zero_net.params().size, np.float32)
+    p[-1] = -200.0                     # b3: softplus(-200) == 0 in fp32 on every vertex
+    zero_net.set_params(p)
+    good_net = og.Mlp(3)
+    bad_nan = np.full(N, 1.0, np.float32)
+    bad_nan[N // 2] = np.nan
+    bad_neg = np.full(N, 1.0, np.float32)
+    bad_neg[1] = -0.5
+    cases = [("nan params", nan_net, None), ("zero mass", zero_net, None),
+             ("nan entry", None, bad_nan), ("negative entry", None, bad_neg),
+             ("all zero", None, np.zeros(N, np.float32))]
+    for name, net, vec in cases:
+        pr = og.PageRank(g, 0, 200, 1e-6, stop_rule=1, init_mode=1)
+        if net is not None:
+            pr.set_init_learned(net)
+        else:
+            pr.set_init(vec)
+        assert pr.init_fallback(), name
+        r = pr.run()
+        x = pr.ranks()
+        pr.free()
+        print(name, "iterations", r.rows, "uniform", ru.rows, flush=True)
+        assert r.rows == ru.rows and r.iterations == ru.iterations, name
+        assert np.array_equal(x.view(np.uint32), xu.view(np.uint32)), name
+    # an accepted start is used (and differs from the uniform run)
+    pr = og.PageRank(g, 0, 200, 1e-6, stop_rule=1, init_mode=1)
+    pr.set_init_learned(good_net)
+    assert not pr.init_fallback()
+    pr.run()
+    assert not np.array_equal(pr.ranks().view(np.uint32), xu.view(np.uint32))
+    pr.free()
+    for net in (nan_net, zero_net, good_net):
+        net.free()
+    g.free()
