@@ -609,9 +609,13 @@ uint32_t orc_ppr_fp64(const orc_csr *g, uint32_t s, double tol, uint32_t max_ite
     double l1 = 0.0;
     for (it = 0; it < max_iter; it++)
     {
+#pragma omp parallel for schedule(static)
         for (uint32_t v = 0; v < N; v++)
             c[v] = g->f_out_degree[v] ? x[v] / (double)g->f_out_degree[v] : 0.0;
         l1 = 0.0;
+        /* rows in parallel (each row's sum in CSR order), the L1 change
+         * reduced in any order: an fp64 checker, not a bit-exact one */
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : l1)
         for (uint32_t v = 0; v < N; v++)
         {
             double acc = 0.0;
