@@ -72,6 +72,17 @@ def cpu_time(g, code, iterations, threads, csr=None, ranks=False):
     return r["seconds"] / iterations, kind
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def gpu_iter_ms(g, variant, steps=10, warmup=2, **kw):
     import opengraph_amd as og
     pr = og.PageRank(g, variant=variant, max_iter=100, epsilon=1e-6, **kw)
@@ -92,8 +103,9 @@ def gpu_iter_ms(g, variant, steps=10, warmup=2, **kw):
 def ext_variants(g, threads, csr, do_cpu):
     """Pull/push/fixed-point on the headline graph, each against the host run."""
     out, cpu = {}, {}
-    for code, name, kw in ((0, "pull_fp32_every_row_exact", {"hub_threshold": 1 << 31}),
-                           (0, "pull_fp32_chunk_split_only", {"exact_above": 0}),
+    for code, name, kw in ((0, "pull_fp32_gather_default", {"binned": 2}),
+                           (0, "pull_fp32_gather_every_row_exact", {"hub_threshold": 1 << 31, "binned": 2}),
+                           (0, "pull_fp32_gather_chunk_split_only", {"exact_above": 0, "binned": 2}),
                            (1, "push_fp32", {}), (1, "push_fp32_lds_accumulators", {"hot_cache": 1}),
                            (2, "pull_fixed64", {}), (3, "push_fixed64", {}), (11, "pull_fixed8", {})):
         ms, kms, bpi = gpu_iter_ms(g, code, **kw)
@@ -311,17 +323,85 @@ def ext_csr_file():
                 read_to_device_seconds=round(t_read, 4), read_back_identical=same)
 
 
-def ext_er(args):
+def ext_er(args, threads, do_cpu):
+    """The Erdos-Renyi graph of the headline's size (seed 2): binned pull
+    (the default) and the gather pull, against the reference's own pull on
+    the host cores, with the binned ranks compared to the reference's."""
     import opengraph_amd as og
     E = args.edge_factor << args.scale
     ds, dd = og.er_device(1 << args.scale, E, 2)
     g = og.Graph.build(ds, dd, E, 0)
     ds.free(); dd.free()
-    ms, kms, bpi = gpu_iter_ms(g, 0)
+    out = dict(workload=f"Erdos-Renyi N=2^{args.scale} E={E} seed 2, pull fp32")
+    for name, kw in (("binned", {}), ("gather", {"binned": 2})):
+        ms, kms, bpi = gpu_iter_ms(g, 0, **kw)
+        out[name] = dict(ms_per_iteration=round(ms, 4), gteps=round(E / ms / 1e6, 2),
+                         algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1),
+                         roofline_frac=round(bpi / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
+    if do_cpu:
+        csr = _ref_csr(g)
+        s, kind, ref_ranks = cpu_time(g, 0, args.cpu_iterations, threads, csr, ranks=True)
+        pr = og.PageRank(g, 0, args.cpu_iterations, 0.0)
+        pr.run()
+        x = pr.ranks()
+        pr.free()
+        out.update(cpu_seconds_per_iteration=round(s, 4), cpu_kind=kind, cpu_threads=threads,
+                   speedup_vs_cpu=round(s / (out["binned"]["ms_per_iteration"] / 1e3), 1),
+                   binned_ranks_bit_identical_to_reference=bool(np.array_equal(x.view(np.uint32),
+                                                                              ref_ranks.view(np.uint32))),
+                   parity_iterations=args.cpu_iterations)
     g.free()
-    return dict(workload=f"Erdos-Renyi N=2^{args.scale} E={E} seed 2, pull fp32",
-                ms_per_iteration=round(ms, 4), gteps=round(E / ms / 1e6, 2),
-                algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
+    return out
+
+
+def ext_csr_build(args, threads, do_cpu):
+    """The CSR builder at the headline size: R-MAT-24 (seed 1) edges on the
+    device, forward + inverse CSR by the GPU radix sort (-o 0: by source;
+    -o 1: by destination then source), best of 3, inputs and outputs in HBM;
+    algorithmic bytes 20 B per pair per 8-bit LSD pass (4 B key read for the
+    histogram, 8 B pair read, 8 B pair write) over both lists, plus the
+    degree count and offsets (4E + 8N).  Against the reference's own
+    sortRunAlgorithms + graphCSRAssignEdgeList (forward and inverse) on the
+    host cores, with all ten arrays compared."""
+    import opengraph_amd as og
+    from oracle import pyoracle as po
+    E = args.edge_factor << args.scale
+    ds, dd = og.rmat_device(args.scale, E, 0.57, 0.19, 0.19, args.seed, True)
+    out = dict(workload=f"R-MAT-{args.scale} (seed {args.seed}) edge list in HBM -> forward and inverse CSR")
+    src = dst = None
+    for sort in (0, 1):
+        times = []
+        for _ in range(3):
+            og.synchronize()
+            t0 = time.perf_counter()
+            g = og.Graph.build(ds, dd, E, sort)
+            og.synchronize()
+            times.append(time.perf_counter() - t0)
+            if _ < 2:
+                g.free()
+        N = g.num_vertices
+        key_bits = max(1, int(N - 1).bit_length())
+        passes = 2 * ((key_bits + 7) // 8) * (2 if sort == 1 else 1)
+        bpb = 20 * E * passes + 4 * E + 8 * N
+        t = min(times)
+        d = dict(seconds=round(t, 4), lsd_passes=passes, algorithmic_bytes=bpb,
+                 algorithmic_gb_per_s=round(bpb / t / 1e9, 1), roofline_frac=round(bpb / t / 1e9 / HBM_PEAK_GBS, 4),
+                 medges_per_second=round(E / t / 1e6, 1))
+        if do_cpu and po.ref_available():
+            if src is None:
+                src, dst = ds.to_numpy(np.uint32, E), dd.to_numpy(np.uint32, E)
+            ref, sec = po.ref_csr_build(src, dst, N, sort, threads)
+            same = all(np.array_equal(g.array(k), ref[k]) for k in po.CSR_FIELDS)
+            del ref
+            d.update(cpu_reference_seconds=round(sec, 3), cpu_threads=threads, cpu_kind="reference",
+                     cpu_sample="the reference's sortRunAlgorithms (radix) and graphCSRAssignEdgeList "
+                                "(mapVerticesWithInOutDegree), forward and inverse, full size",
+                     speedup_vs_cpu=round(sec / t, 1), arrays_bit_identical_to_reference=bool(same))
+        g.free()
+        out[f"sort{sort}"] = d
+        log(f"  csr build -o {sort}: {d}")
+    ds.free(); dd.free()
+    return out
 
 
 def ext_ppr():
@@ -552,6 +632,7 @@ def main():
             csr = _ref_csr(g)
             s, kind, ref_ranks = cpu_time(g, 0, args.cpu_iterations, threads, csr, ranks=True)
             cpu = dict(value=round(E / s / 1e9, 4), unit="GTEPS", cores=threads, kind=kind,
+                       cpu_model=_cpu_model(), nproc=os.cpu_count(),
                        sample=f"R-MAT-{scale} pull, {args.cpu_iterations} full iterations (epsilon 0) of "
                               + ("the reference pageRankPullGraphCSR compiled from its sources"
                                  if kind == "reference" else "the oracle C/OpenMP restatement")
@@ -564,8 +645,10 @@ def main():
             x64, _, _ = po.pagerank_fp64(c, 0.15, 0.0, args.cpu_iterations, threads)
             par = dict(iterations=args.cpu_iterations,
                        l1_vs_fp64_reference=float(np.abs(ref_ranks.astype(np.float64) - x64).sum()))
-            for form, kw in (("default_exact_above_4096", {}), ("chunk_split_only", {"exact_above": 0}),
-                             ("every_row_exact", {"hub_threshold": 1 << 31})):
+            for form, kw in (("binned_default", {}),
+                             ("gather_exact_above_4096", {"binned": 2}),
+                             ("gather_chunk_split_only", {"exact_above": 0, "binned": 2}),
+                             ("gather_every_row_exact", {"hub_threshold": 1 << 31, "binned": 2})):
                 pr3 = og.PageRank(g, 0, args.cpu_iterations, 0.0, **kw)
                 pr3.run()
                 gpu_ranks = pr3.ranks()
@@ -595,7 +678,8 @@ def main():
                          ("rank_stats", lambda: ext_rank_stats(g)),
                          ("relabel_dbg", lambda: ext_relabel(g, threads, csr)),
                          ("csr_file", ext_csr_file),
-                         ("erdos_renyi", lambda: ext_er(args)), ("batched_ppr", ext_ppr),
+                         ("csr_build", lambda: ext_csr_build(args, threads, csr is not None)),
+                         ("erdos_renyi", lambda: ext_er(args, threads, csr is not None)), ("batched_ppr", ext_ppr),
                          ("uneven_batch", ext_multi), ("learned_warm_start", ext_learned)):
             if args.extensions and name not in args.extensions.split(","):
                 continue
@@ -624,7 +708,9 @@ def main():
                    "variant": "pull fp32 (-p 0)", "sort": 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_iteration": bytes_iter, "kernel": "og_pull_hot_kernel<0> + og_pull_chains_kernel (concurrent, fork to join)",
+                     "bytes_per_iteration": bytes_iter,
+                     "kernel": "og_bin_gather_kernel + og_bin_sum_kernel + og_bin_long_kernel (one iteration's "
+                               "passes, HIP events from the gather's start to the long-row pass's end)",
                      "traffic_source": "profiles/pull_rmat24_counters.json" if traffic else None},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -132,7 +132,31 @@ typedef struct
                                inputs); default 4096 (ogpu_pr_default_options), 0 off.
                                Such rows carry the largest fp32 drift of
                                the reference's sums.  With hot_cache's persistent kernel
-                               the chains run beside it on a side stream */
+                               the chains run beside it on a side stream.  fp32 push
+                               (variant 1): when not 0, the destinations of more than
+                               1024 in-edges are gathered in CSR order the same way
+                               (the reference's source-ordered adds) and left out of the
+                               atomic scatter */
+    uint32_t binned;        /* fp32 pull (variant 0, not personalized): 0 default (on),
+                               1 on, 2 off -- the binned pull: per iteration a gather pass
+                               writes each edge's contribution, source segment by source
+                               segment from LDS, into an edge array grouped by destination
+                               bins, and a sum pass adds every row from LDS in CSR order
+                               (rows longer than a bin by one block each).  Every row is
+                               the reference's running sum to the bit, and all global
+                               accesses stream.  hub_threshold, exact_above and hot_cache
+                               then apply only to the learned start's feature pass */
+    uint32_t warm_classes;  /* learned start (ogpu_pr_set_init_learned): the predicted
+                               start is corrected over this many in-degree classes (class
+                               0 without in-edges, class c > 0 with floor(log2 in-degree)
+                               = c - 1, the last one open-ended) so that each class's mass
+                               equals that of one application of the iteration to it: one
+                               scale factor per class from the classes' mass balance
+                               s_i m_i = 0.15 n_i + sum_j A_ij s_j (m_i the start's mass
+                               in class i, n_i its vertices, A_ij the mass the iteration
+                               moves from class j to class i), one pass over the edges
+                               and an nc x nc solve.  0: default (8), 1: no correction,
+                               2..16 */
 } ogpu_pr_options;
 
 typedef struct

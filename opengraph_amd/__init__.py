@@ -113,7 +113,8 @@ class PrOptions(C.Structure):
     _fields_ = [
         ("variant", u32), ("max_iter", u32), ("epsilon", C.c_double), ("stop_rule", u32),
         ("init_mode", u32), ("relabel", u32), ("hub_threshold", u32), ("ppr", u32),
-        ("ppr_source", u32), ("hot_cache", u32), ("exact_above", u32),
+        ("ppr_source", u32), ("hot_cache", u32), ("exact_above", u32), ("binned", u32),
+        ("warm_classes", u32),
     ]
 
 
@@ -471,12 +472,16 @@ class PageRank:
 
     def __init__(self, graph: Graph, variant=0, max_iter=20, epsilon=1e-4, stop_rule=0,
                  init_mode=0, relabel=1, hub_threshold=0, init=None, ppr_source=None, hot_cache=0,
-                 exact_above=None):
+                 exact_above=None, binned=None, warm_classes=None):
         lib = load()
         o = PrOptions()
         lib.ogpu_pr_default_options(C.byref(o))
         if exact_above is not None:
             o.exact_above = exact_above
+        if binned is not None:
+            o.binned = binned
+        if warm_classes is not None:
+            o.warm_classes = warm_classes
         o.variant, o.max_iter, o.epsilon = variant, max_iter, epsilon
         o.stop_rule, o.init_mode, o.relabel, o.hub_threshold = stop_rule, init_mode, relabel, hub_threshold
         if ppr_source is not None:

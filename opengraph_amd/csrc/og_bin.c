@@ -1,0 +1,499 @@
+/*
+ * og_bin.c -- one-time build of the binned pull's edge layout (og_bin.h)
+ * from a PageRank plan whose rows keep the reference's in-neighbour order.
+ *
+ * Host: the rows are packed into bins (from the plan's row offsets), the
+ * rows of more than OG_BIN_CAP in-edges are cut into pieces, and, once the
+ * device has sorted the edges into tiles, the tiles are ordered segment by
+ * segment and walked in rounds of 64 entries for the gather pass.
+ * Device: the (bin, segment) keys, their stable radix sort, the entries'
+ * source and slot indices, and the tile boundaries.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "og_bin.h"
+#include "og_internal.h"
+#include "og_plan.h"
+
+#define OG_BIN_ZERO_ROWS 16384u   /* rows of a bin of rows without in-edges */
+
+void og_bin_plan_destroy(og_bin_plan *b)
+{
+    if (!b) return;
+    og_dfree(b->mem_items); og_dfree(b->mem_rounds); og_dfree(b->mem_tile_ent); og_dfree(b->mem_src);
+    og_dfree(b->mem_pos); og_dfree(b->mem_val); og_dfree(b->mem_bins); og_dfree(b->mem_long_row);
+    og_dfree(b->mem_long_off); og_dfree(b->mem_vlong);
+    free(b);
+}
+
+uint32_t og_bin_sum_grid(const og_bin_plan *b)
+{
+    (void)b;
+    uint32_t g = (uint32_t)og_num_cus();
+    return g ? g : 1;
+}
+
+uint32_t og_bin_long_grid(const og_bin_plan *b)
+{
+    uint32_t g = (uint32_t)og_num_cus();
+    return b->d.n_long < g ? b->d.n_long : g;
+}
+
+/* growable host arrays */
+typedef struct
+{
+    void *p;
+    size_t n, cap, elem;
+} og_vec;
+
+static void *og_vec_push(og_vec *v)
+{
+    if (v->n == v->cap)
+    {
+        size_t c = v->cap ? 2 * v->cap : 1024;
+        void *q = realloc(v->p, c * v->elem);
+        if (!q) return NULL;
+        v->p = q;
+        v->cap = c;
+    }
+    return (char *)v->p + (v->n++) * v->elem;
+}
+
+static uint32_t og_round8(uint32_t x) { return (x + 7u) & ~7u; }
+
+static int og_bits(uint64_t x)
+{
+    int b = 0;
+    while (b < 64 && (1ull << b) <= x) b++;
+    return b;
+}
+
+/* Device copy of a host array (NULL and error set on failure). */
+static void *og_bin_upload(const void *h, size_t bytes)
+{
+    void *d = og_dalloc(bytes ? bytes : 16);
+    if (!d) return NULL;
+    if (bytes && og_hip_check(og_copy(d, h, bytes, hipMemcpyHostToDevice), "H2D bin layout"))
+    {
+        og_dfree(d);
+        return NULL;
+    }
+    return d;
+}
+
+/* Host check of every index the binned kernels derive from the layout
+ * (enabled by the environment variable OG_BIN_VERIFY): items within the
+ * rounds, every lane's entry of every round within the entry arrays, each
+ * real entry's source within its segment and slot within its bin, bins and
+ * pieces within the entries, CSR positions and vlong.  Runs before any
+ * binned kernel, so a bad layout is an error, not a stray access. */
+static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t *rp, uint32_t n_tile_ent)
+{
+    const og_bin_dev *d = &b->d;
+    const size_t ne = (size_t)b->n_ent;
+    og_bin_item *items = (og_bin_item *)malloc((size_t)d->n_items * sizeof(og_bin_item) + 16);
+    og_bin_round *rounds = (og_bin_round *)malloc((size_t)b->n_rounds * sizeof(og_bin_round) + 16);
+    uint32_t *tent = (uint32_t *)malloc((size_t)n_tile_ent * 4 + 4);
+    uint16_t *src = (uint16_t *)malloc(ne * 2), *pos = (uint16_t *)malloc(ne * 2);
+    og_bin_desc *bins = (og_bin_desc *)malloc((size_t)d->n_bins * sizeof(og_bin_desc) + 16);
+    uint32_t *lrow = (uint32_t *)malloc((size_t)d->n_long * 4 + 4), *loff = (uint32_t *)malloc((size_t)d->n_long * 4 + 8);
+    uint8_t *seen = (uint8_t *)calloc(ne, 1);
+    int rc = -1;
+#define OG_BIN_FAIL(...) do { og_set_error("binned layout check: " __VA_ARGS__); goto out; } while (0)
+    if (!items || !rounds || !tent || !src || !pos || !bins || !lrow || !loff || !seen)
+        OG_BIN_FAIL("out of host memory");
+    if (og_hip_check(og_copy(items, d->items, (size_t)d->n_items * sizeof(og_bin_item), hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(rounds, d->rounds, (size_t)b->n_rounds * sizeof(og_bin_round), hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(tent, d->tile_ent, (size_t)n_tile_ent * 4, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(src, d->src, ne * 2, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(pos, d->pos, ne * 2, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(bins, d->bins, (size_t)d->n_bins * sizeof(og_bin_desc), hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(lrow, d->long_row, (size_t)d->n_long * 4, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(loff, d->long_off, ((size_t)d->n_long + 1) * 4, hipMemcpyDeviceToHost), "D2H"))
+        goto out;
+    for (uint32_t q = 0; q < d->n_items; q++)
+    {
+        const og_bin_item it = items[q];
+        if (it.seg >= d->n_seg || it.r0 >= it.r1 || it.r1 > b->n_rounds) OG_BIN_FAIL("item %u", q);
+        const uint32_t nv = d->N - it.seg * OG_BIN_SEG < OG_BIN_SEG ? d->N - it.seg * OG_BIN_SEG : OG_BIN_SEG;
+        for (uint32_t r = it.r0; r < it.r1; r++)
+        {
+            const og_bin_round rd = rounds[r];
+            const uint64_t mask = ((uint64_t)rd.mask_hi << 32) | rd.mask_lo;
+            if (mask & 1) OG_BIN_FAIL("round %u: lane 0 starts a tile", r);
+            for (uint32_t k = 0; k < 64; k++)
+            {
+                const uint64_t low = mask & (k == 63 ? ~0ull : ((2ull << k) - 1));
+                const uint32_t c = (uint32_t)__builtin_popcountll(low);
+                const uint64_t t = (uint64_t)rd.first + c;
+                if (t >= n_tile_ent) OG_BIN_FAIL("round %u lane %u: tile %llu of %u", r, k, (unsigned long long)t, n_tile_ent);
+                const uint64_t o = c ? k - (63u - (uint32_t)__builtin_clzll(low)) : (uint64_t)rd.off0 + k;
+                const uint64_t e = (uint64_t)tent[t] + o;
+                if (e >= ne) OG_BIN_FAIL("round %u lane %u: entry %llu of %zu", r, k, (unsigned long long)e, ne);
+                if (e < ne - 64)
+                {
+                    if (src[e] >= nv) OG_BIN_FAIL("entry %llu: source %u past segment size %u", (unsigned long long)e, src[e], nv);
+                    if (seen[e]) OG_BIN_FAIL("entry %llu gathered twice", (unsigned long long)e);
+                    seen[e] = 1;
+                }
+            }
+        }
+    }
+    uint64_t prev_end = 0;
+    for (uint32_t q = 0; q < d->n_bins; q++)
+    {
+        const og_bin_desc bd = bins[q];
+        if (bd.ent0 != prev_end || (bd.ent0 & 7u) || bd.ent1 < bd.ent0 || bd.ent1 - bd.ent0 != ((bd.cnt + 7u) & ~7u) ||
+            bd.ent1 > ne - 64 || bd.cnt > OG_BIN_CAP)
+            OG_BIN_FAIL("bin %u entries [%u, %u) count %u", q, bd.ent0, bd.ent1, bd.cnt);
+        prev_end = bd.ent1;
+        if ((uint64_t)bd.pos0 + bd.cnt > pl->E) OG_BIN_FAIL("bin %u positions", q);
+        if (bd.kind == OG_BIN_PIECE)
+        {
+            if (bd.row0 >= pl->Z || (uint64_t)bd.vbase + bd.cnt > b->n_vlong) OG_BIN_FAIL("piece %u", q);
+        }
+        else
+        {
+            if (bd.row0 > bd.row1 || bd.row1 > d->N) OG_BIN_FAIL("bin %u rows", q);
+            if (bd.cnt && (rp[bd.row0] != bd.pos0 || rp[bd.row1] - rp[bd.row0] != bd.cnt)) OG_BIN_FAIL("bin %u row offsets", q);
+        }
+        for (uint32_t e = bd.ent0; e < bd.ent1; e++)
+        {
+            const int real = e < bd.ent0 + bd.cnt;
+            if (real ? pos[e] >= bd.cnt : pos[e] != OG_BIN_PAD) OG_BIN_FAIL("entry %u: slot %u in bin %u of %u", e, pos[e], q, bd.cnt);
+            if (real && !seen[e]) OG_BIN_FAIL("entry %u never gathered", e);
+        }
+    }
+    for (uint32_t q = 0; q < d->n_long; q++)
+        if (lrow[q] >= pl->Z || loff[q] > loff[q + 1] || loff[q + 1] - loff[q] != rp[lrow[q] + 1] - rp[lrow[q]])
+            OG_BIN_FAIL("long row %u", q);
+    if (d->n_long && loff[d->n_long] != b->n_vlong) OG_BIN_FAIL("vlong size");
+    rc = 0;
+#undef OG_BIN_FAIL
+out:
+    free(items); free(rounds); free(tent); free(src); free(pos); free(bins); free(lrow); free(loff); free(seen);
+    return rc;
+}
+
+int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
+{
+    *out = NULL;
+    const uint32_t N = pl->N, E = pl->E, Z = pl->Z, H = pl->H, C = pl->C;
+    if (pl->relabel == 1 && H)
+    {
+        og_set_error("binned pull: the plan sorts long rows by source (relabel 1); it needs CSR order");
+        return -1;
+    }
+    if (!E || !N)
+    {
+        og_set_error("binned pull: graph without edges");
+        return -1;
+    }
+    og_bin_plan *b = (og_bin_plan *)calloc(1, sizeof(og_bin_plan));
+    uint32_t *rp = (uint32_t *)malloc(((size_t)N + 1) * 4);
+    og_vec bins = {NULL, 0, 0, sizeof(og_bin_desc)};
+    uint32_t *long_rows = NULL, *long_deg = NULL, *long_voff_rowordered = NULL, *long_off = NULL;
+    uint32_t *tseg = NULL, *tent = NULL, *tstart = NULL, *order = NULL, *seg_cnt = NULL;
+    uint32_t *tile_ent2 = NULL;
+    og_vec rounds = {NULL, 0, 0, sizeof(og_bin_round)};
+    og_vec items = {NULL, 0, 0, sizeof(og_bin_item)};
+    uint32_t *key_a = NULL, *val_a = NULL, *key_b = NULL, *val_b = NULL;
+    void *scratch = NULL, *d_tseg = NULL, *d_tent = NULL, *d_tstart = NULL;
+    hipStream_t s = og_stream();
+    int rc = -1;
+    if (!b || !rp)
+    {
+        og_set_error("out of host memory");
+        goto done;
+    }
+    if (og_hip_check(og_copy(rp, pl->row_ptr, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr"))
+        goto done;
+
+    /* ---- bins ---- */
+    {
+        /* rows [0, H2) are added by wavefronts: the plan's class of more
+         * than 512 in-edges and the start of the next class (in-degree
+         * descending) down to OG_BIN_WAVE_ROW */
+        uint32_t H2 = H;
+        while (H2 < C && rp[H2 + 1] - rp[H2] > OG_BIN_WAVE_ROW) H2++;
+        /* long rows: vlong offsets assigned longest first */
+        uint32_t n_long = 0;
+        for (uint32_t r = 0; r < Z; r++)
+            if (rp[r + 1] - rp[r] > OG_BIN_CAP) n_long++;
+        long_rows = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        long_deg = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        long_voff_rowordered = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        long_off = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        if (!long_rows || !long_deg || !long_voff_rowordered || !long_off)
+        {
+            og_set_error("out of host memory");
+            goto done;
+        }
+        uint32_t k = 0;
+        for (uint32_t r = 0; r < Z; r++)
+            if (rp[r + 1] - rp[r] > OG_BIN_CAP) long_rows[k++] = r;
+        /* order[] = long-row ordinals by degree descending (stable) */
+        uint32_t *lord = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        if (!lord) { og_set_error("out of host memory"); goto done; }
+        for (uint32_t i = 0; i < n_long; i++) lord[i] = i;
+        for (uint32_t i = 1; i < n_long; i++)   /* insertion sort: few long rows */
+        {
+            uint32_t x = lord[i], dx = rp[long_rows[x] + 1] - rp[long_rows[x]];
+            int j = (int)i - 1;
+            while (j >= 0 && rp[long_rows[lord[j]] + 1] - rp[long_rows[lord[j]]] < dx)
+            {
+                lord[j + 1] = lord[j];
+                j--;
+            }
+            lord[j + 1] = x;
+        }
+        uint64_t voff = 0;
+        for (uint32_t i = 0; i < n_long; i++)
+        {
+            const uint32_t x = lord[i], r = long_rows[x];
+            long_voff_rowordered[x] = (uint32_t)voff;
+            long_deg[i] = r;                 /* long_deg reused: rows longest first */
+            long_off[i] = (uint32_t)voff;
+            voff += rp[r + 1] - rp[r];
+        }
+        free(lord);
+        long_off[n_long] = (uint32_t)voff;
+        b->n_vlong = voff;
+
+        uint64_t ent = 0;
+        uint32_t li = 0;   /* long rows met so far, row order */
+        uint32_t r = 0;
+        while (r < Z)
+        {
+            const uint32_t deg = rp[r + 1] - rp[r];
+            if (deg > OG_BIN_CAP)
+            {
+                for (uint32_t p0 = 0; p0 < deg; p0 += OG_BIN_CAP)
+                {
+                    const uint32_t cnt = deg - p0 < OG_BIN_CAP ? deg - p0 : OG_BIN_CAP;
+                    og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
+                    if (!d) { og_set_error("out of host memory"); goto done; }
+                    d->row0 = r;
+                    d->row1 = li;
+                    d->ent0 = (uint32_t)ent;
+                    d->ent1 = (uint32_t)(ent + og_round8(cnt));
+                    d->pos0 = rp[r] + p0;
+                    d->kind = OG_BIN_PIECE;
+                    d->vbase = long_voff_rowordered[li] + p0;
+                    d->cnt = cnt;
+                    ent += og_round8(cnt);
+                }
+                li++;
+                r++;
+                continue;
+            }
+            const uint32_t kind = r < H2 ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
+            const uint32_t r0 = r;
+            uint32_t tot = 0;
+            while (r < Z)
+            {
+                const uint32_t dr = rp[r + 1] - rp[r];
+                const uint32_t kr = r < H2 ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
+                if (kr != kind || dr > OG_BIN_CAP || tot + dr > OG_BIN_CAP) break;
+                tot += dr;
+                r++;
+            }
+            og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
+            if (!d) { og_set_error("out of host memory"); goto done; }
+            d->row0 = r0;
+            d->row1 = r;
+            d->ent0 = (uint32_t)ent;
+            d->ent1 = (uint32_t)(ent + og_round8(tot));
+            d->pos0 = rp[r0];
+            d->kind = kind;
+            d->vbase = 0;
+            d->cnt = tot;
+            ent += og_round8(tot);
+        }
+        const uint32_t n_edge_bins = (uint32_t)bins.n;
+        /* rows without in-edges: epilogue-only bins, last */
+        for (uint32_t r0 = Z; r0 < N; r0 += OG_BIN_ZERO_ROWS)
+        {
+            og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
+            if (!d) { og_set_error("out of host memory"); goto done; }
+            d->row0 = r0;
+            d->row1 = N - r0 < OG_BIN_ZERO_ROWS ? N : r0 + OG_BIN_ZERO_ROWS;
+            d->ent0 = d->ent1 = (uint32_t)ent;
+            d->pos0 = rp[r0];
+            d->kind = OG_BIN_THREAD_ROWS;
+            d->vbase = 0;
+            d->cnt = 0;
+        }
+        b->n_zero_bins = (uint32_t)bins.n - n_edge_bins;
+        const uint32_t n_seg = (N + OG_BIN_SEG - 1) / OG_BIN_SEG;
+        if (ent + 64 > 0xFFFFFFFFull || (uint64_t)n_edge_bins * n_seg > 0xFFFFFFFFull || voff > 0xFFFFFFFFull)
+        {
+            og_set_error("binned pull: graph too large for 32-bit entry and tile keys");
+            goto done;
+        }
+        b->n_ent = ent + 64;   /* 64 slack entries for the rounds past a segment's last tile */
+        b->d.N = N;
+        b->d.n_seg = n_seg;
+        b->d.n_bins = (uint32_t)bins.n;
+        b->d.n_bins_live = (uint32_t)bins.n;
+        b->d.n_long = n_long;
+
+        /* ---- device: keys, sort, entries, tiles ---- */
+        b->mem_bins = og_bin_upload(bins.p, bins.n * sizeof(og_bin_desc));
+        b->mem_src = og_dalloc(b->n_ent * 2);
+        b->mem_pos = og_dalloc(b->n_ent * 2);
+        b->mem_val = og_dalloc(b->n_ent * 4);
+        b->mem_long_row = og_bin_upload(long_deg, (size_t)n_long * 4);
+        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
+        b->mem_vlong = og_dalloc(voff * 4);
+        key_a = og_dalloc(((size_t)E + 1) * 4);
+        val_a = og_dalloc(((size_t)E + 1) * 4);
+        key_b = og_dalloc(((size_t)E + 1) * 4);
+        val_b = og_dalloc(((size_t)E + 1) * 4);
+        size_t sb = og_radix_scratch_bytes(E), sc2 = og_scan_scratch_bytes(E);
+        scratch = og_dalloc(sb > sc2 ? sb : sc2);
+        if (!b->mem_bins || !b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off ||
+            !b->mem_vlong || !key_a || !val_a || !key_b || !val_b || !scratch)
+            goto done;
+        const og_bin_desc *d_bins = (const og_bin_desc *)b->mem_bins;
+        if (og_launch_bin_keys(d_bins, n_edge_bins, pl->cols, n_seg, key_b, val_b, s)) goto done;
+        int in_b = 0;
+        const uint32_t kbits = (uint32_t)og_bits((uint64_t)n_edge_bins * n_seg - 1);
+        if (og_radix_sort_pairs(key_b, val_b, key_a, val_a, key_b, val_b, E, kbits ? kbits : 1, scratch, s, &in_b))
+            goto done;
+        uint32_t *K = in_b ? key_b : key_a, *V = in_b ? val_b : val_a;
+        uint32_t *flag = in_b ? key_a : key_b, *fscan = in_b ? val_a : val_b;
+        if (og_launch_bin_fill(K, V, E, d_bins, n_seg, pl->cols, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos,
+                               flag, s) ||
+            og_scan_exclusive(flag, fscan, E, scratch, s) ||
+            og_launch_bin_pads(d_bins, n_edge_bins, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
+            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack"))
+            goto done;
+        uint32_t n_tiles = 0;
+        if (og_hip_check(og_copy(&n_tiles, fscan + E, 4, hipMemcpyDeviceToHost), "D2H tiles")) goto done;
+        b->n_tiles = n_tiles;
+        d_tseg = og_dalloc((size_t)n_tiles * 4);
+        d_tent = og_dalloc((size_t)n_tiles * 4);
+        d_tstart = og_dalloc((size_t)n_tiles * 4);
+        if (!d_tseg || !d_tent || !d_tstart) goto done;
+        if (og_launch_bin_tiles(K, E, d_bins, n_seg, fscan, (uint32_t *)d_tseg, (uint32_t *)d_tent,
+                                (uint32_t *)d_tstart, s))
+            goto done;
+        tseg = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        tent = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        tstart = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        order = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        seg_cnt = (uint32_t *)calloc((size_t)n_seg + 1, 4);
+        tile_ent2 = (uint32_t *)malloc(((size_t)n_tiles + n_seg) * 4 + 4);
+        if (!tseg || !tent || !tstart || !order || !seg_cnt || !tile_ent2)
+        {
+            og_set_error("out of host memory");
+            goto done;
+        }
+        if (og_hip_check(og_copy(tseg, d_tseg, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile seg") ||
+            og_hip_check(og_copy(tent, d_tent, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile ent") ||
+            og_hip_check(og_copy(tstart, d_tstart, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile start"))
+            goto done;
+        og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
+        key_a = val_a = key_b = val_b = NULL;
+        scratch = NULL;
+
+        /* ---- host: tiles segment by segment (bins ascending), rounds, items ---- */
+        /* tile sizes (tiles are in sorted order: each ends where the next starts) */
+        for (uint32_t t = 0; t < n_tiles; t++) tstart[t] = (t + 1 < n_tiles ? tstart[t + 1] : E) - tstart[t];
+        for (uint32_t t = 0; t < n_tiles; t++) seg_cnt[tseg[t] + 1]++;
+        for (uint32_t q = 0; q < n_seg; q++) seg_cnt[q + 1] += seg_cnt[q];
+        {
+            uint32_t *fill = (uint32_t *)malloc(((size_t)n_seg + 1) * 4);
+            if (!fill) { og_set_error("out of host memory"); goto done; }
+            memcpy(fill, seg_cnt, ((size_t)n_seg + 1) * 4);
+            for (uint32_t t = 0; t < n_tiles; t++) order[fill[tseg[t]]++] = t;
+            free(fill);
+        }
+        const uint32_t slack = (uint32_t)ent;
+        uint32_t nt2 = 0;
+        for (uint32_t q = 0; q < n_seg; q++)
+        {
+            const uint32_t t0 = seg_cnt[q], m = seg_cnt[q + 1] - t0;
+            if (!m) continue;
+            const uint32_t base = nt2;
+            uint64_t tot = 0;
+            for (uint32_t i = 0; i < m; i++)
+            {
+                const uint32_t t = order[t0 + i];
+                tile_ent2[nt2++] = tent[t];
+                tot += tstart[t];
+            }
+            tile_ent2[nt2++] = slack;                  /* the segment's end tile */
+            const uint32_t R = (uint32_t)((tot + 63) / 64);
+            const uint32_t r_first = (uint32_t)rounds.n;
+            uint32_t k = 0;       /* tile of lane 0 */
+            uint64_t Fk = 0;      /* its first flat position */
+            for (uint32_t rr = 0; rr < R; rr++)
+            {
+                const uint64_t f0 = 64ull * rr;
+                while (k < m && Fk + tstart[order[t0 + k]] <= f0)
+                {
+                    Fk += tstart[order[t0 + k]];
+                    k++;
+                }
+                og_bin_round *rd = (og_bin_round *)og_vec_push(&rounds);
+                if (!rd) { og_set_error("out of host memory"); goto done; }
+                uint64_t mask = 0;
+                uint32_t kk = k;
+                uint64_t Fkk = Fk;
+                while (kk < m)
+                {
+                    const uint64_t nxt = Fkk + tstart[order[t0 + kk]];
+                    if (nxt >= f0 + 64) break;
+                    mask |= 1ull << (nxt - f0);
+                    Fkk = nxt;
+                    kk++;
+                }
+                rd->first = base + k;
+                rd->off0 = (uint32_t)(f0 - Fk);
+                rd->mask_lo = (uint32_t)mask;
+                rd->mask_hi = (uint32_t)(mask >> 32);
+            }
+            for (uint32_t r0 = r_first; r0 < r_first + R; r0 += OG_BIN_ITEM_ROUNDS)
+            {
+                og_bin_item *it = (og_bin_item *)og_vec_push(&items);
+                if (!it) { og_set_error("out of host memory"); goto done; }
+                it->seg = q;
+                it->r0 = r0;
+                it->r1 = r_first + R - r0 < OG_BIN_ITEM_ROUNDS ? r_first + R : r0 + OG_BIN_ITEM_ROUNDS;
+                it->pad = 0;
+            }
+        }
+        b->n_rounds = (uint32_t)rounds.n;
+        b->mem_tile_ent = og_bin_upload(tile_ent2, (size_t)nt2 * 4);
+        b->mem_rounds = og_bin_upload(rounds.p, rounds.n * sizeof(og_bin_round));
+        b->mem_items = og_bin_upload(items.p, items.n * sizeof(og_bin_item));
+        if (!b->mem_tile_ent || !b->mem_rounds || !b->mem_items) goto done;
+        b->d.n_items = (uint32_t)items.n;
+        b->d.items = (const og_bin_item *)b->mem_items;
+        b->d.rounds = (const og_bin_round *)b->mem_rounds;
+        b->d.tile_ent = (const uint32_t *)b->mem_tile_ent;
+        b->d.src = (const uint16_t *)b->mem_src;
+        b->d.pos = (const uint16_t *)b->mem_pos;
+        b->d.val = (float *)b->mem_val;
+        b->d.bins = d_bins;
+        b->d.long_row = (const uint32_t *)b->mem_long_row;
+        b->d.long_off = (const uint32_t *)b->mem_long_off;
+        b->d.vlong = (float *)b->mem_vlong;
+        if (getenv("OG_BIN_VERIFY") && og_bin_verify(b, pl, rp, nt2)) goto done;
+    }
+    rc = 0;
+done:
+    (void)hipStreamSynchronize(s);
+    og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
+    og_dfree(d_tseg); og_dfree(d_tent); og_dfree(d_tstart);
+    free(rp); free(bins.p); free(long_rows); free(long_deg); free(long_voff_rowordered); free(long_off);
+    free(tseg); free(tent); free(tstart); free(order); free(seg_cnt); free(tile_ent2);
+    free(rounds.p); free(items.p);
+    if (rc) og_bin_plan_destroy(b);
+    else *out = b;
+    return rc;
+}

@@ -1,0 +1,136 @@
+/*
+ * og_bin.h -- the binned fp32 pull (og_bin.hip, og_bin.c): pageRank.c:998-1011
+ * as two streaming passes over an edge layout built once per graph.
+ *
+ * Rows (new ids, the plan's CSR order kept) are packed into bins whose
+ * in-edges fit in one CU's LDS (OG_BIN_CAP values); sources fall in
+ * segments of OG_BIN_SEG consecutive ids (their contributions fit in LDS).
+ * The edges of bin b whose source lies in segment s form tile (b, s).
+ * Entries (one per edge) are stored bin by bin, within a bin tile by tile
+ * in segment order, within a tile in CSR order:
+ *   src[e]  u16  source id minus the segment's first id
+ *   pos[e]  u16  the edge's CSR position minus the bin's first position
+ *               (OG_BIN_PAD: a padding entry; bins are padded to 8 entries)
+ *   val[e]  f32  the gathered contribution, written each iteration
+ * One iteration:
+ *   gather pass  one 1024-thread block per CU: per work item (a segment and
+ *                a run of its tiles), the segment's contributions are copied
+ *                into LDS, then val[e] = lds[src[e]] for the item's entries,
+ *                walked 64 at a time in "rounds" (segment-major order).
+ *   sum pass     one 1024-thread block per CU: per bin, val[e] is written
+ *                to LDS slot pos[e] (the bin's in-edge values in CSR order),
+ *                then every row is added from its slots in order -- the
+ *                reference's running sum -- by a thread (rows of at most
+ *                OG_BIN_WAVE_ROW in-edges) or a wavefront (longer rows, an
+ *                exact fast-forward over binades), and the update runs.
+ *   long rows    rows with more than OG_BIN_CAP in-edges are cut into
+ *                pieces of OG_BIN_CAP; the sum pass writes each piece's
+ *                values in CSR order to vlong, and one block per long row
+ *                adds them in order (exact fast-forward, block-wide).
+ * Every row is therefore summed in the reference's order: the fp32 pull is
+ * bit-exact, and every global access is a streaming one.
+ */
+#ifndef OG_BIN_H
+#define OG_BIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "og_internal.h"
+#include "og_pr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OG_BIN_SEG 39936u      /* sources per segment: 156 KB of fp32 in LDS */
+#define OG_BIN_CAP 39936u      /* in-edge values per bin: 156 KB of LDS */
+#define OG_BIN_PAD 0xFFFFu     /* pos of a padding entry */
+#define OG_BIN_WAVE_ROW 128u   /* rows with more in-edges are added by a wavefront */
+#define OG_BIN_ITEM_ROUNDS 2048u /* rounds (64 entries) per gather work item at most */
+#define OG_BIN_THREADS 1024u
+
+enum { OG_BIN_THREAD_ROWS = 0, OG_BIN_WAVE_ROWS = 1, OG_BIN_PIECE = 2 };
+
+/* One bin: rows [row0, row1) (a piece: the long row row0 and row1 = the
+ * index of that long row), entries [ent0, ent1) (ent1 - ent0 a multiple of
+ * 8), first CSR position pos0, and for a piece the vlong offset of its
+ * first value. */
+typedef struct
+{
+    uint32_t row0, row1, ent0, ent1;
+    uint32_t pos0, kind, vbase, cnt;
+} og_bin_desc;
+
+/* Gather work item: segment seg, rounds [r0, r1). */
+typedef struct
+{
+    uint32_t seg, r0, r1, pad;
+} og_bin_item;
+
+/* Round of 64 entries of one segment: lane 0's entry belongs to tile
+ * `first` at offset off0; mask bit k (k > 0) is set when lane k starts the
+ * next tile. */
+typedef struct
+{
+    uint32_t first, off0, mask_lo, mask_hi;
+} og_bin_round;
+
+/* Everything the binned kernels read, by value. */
+typedef struct og_bin_dev
+{
+    uint32_t N, n_seg, n_items, n_bins, n_bins_live, n_long;
+    const og_bin_item *items;
+    const og_bin_round *rounds;
+    const uint32_t *tile_ent;    /* segment-major tiles (with one end tile per segment) */
+    const uint16_t *src, *pos;
+    float *val;
+    const og_bin_desc *bins;
+    const uint32_t *long_row;    /* n_long: the long rows (new ids), longest first */
+    const uint32_t *long_off;    /* n_long + 1: their values' range in vlong */
+    float *vlong;
+} og_bin_dev;
+
+typedef struct og_bin_plan
+{
+    og_bin_dev d;
+    uint64_t n_ent;              /* entries including padding and the end tiles' slack */
+    uint32_t n_tiles, n_rounds, n_zero_bins;
+    uint64_t n_vlong;
+    /* device allocations */
+    void *mem_items, *mem_rounds, *mem_tile_ent, *mem_src, *mem_pos, *mem_val, *mem_bins;
+    void *mem_long_row, *mem_long_off, *mem_vlong;
+} og_bin_plan;
+
+/* Builds the binned layout of a plan whose rows keep the reference's
+ * in-neighbour order (relabel 2 or 0). */
+int og_bin_plan_build(const og_plan *pl, og_bin_plan **out);
+void og_bin_plan_destroy(og_bin_plan *b);
+/* Blocks of the sum pass and of the long-row pass (statistics partials). */
+uint32_t og_bin_sum_grid(const og_bin_plan *b);
+uint32_t og_bin_long_grid(const og_bin_plan *b);
+
+/* One binned pull iteration: gather, sum, long rows; ev0 / ev1 (may be
+ * NULL) bracket them.  Statistics partials: the sum pass's blocks, then
+ * the long-row blocks; then the final reduction. */
+int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint32_t it, uint32_t sum_grid,
+                       uint32_t long_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+
+/* plan-build kernels */
+int og_launch_bin_keys(const og_bin_desc *bins, uint32_t n_bins, const uint32_t *cols, uint32_t n_seg,
+                       uint32_t *key, uint32_t *val, hipStream_t s);
+int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
+                       uint32_t n_seg, const uint32_t *cols, uint16_t *src, uint16_t *pos, uint32_t *tile_flag,
+                       hipStream_t s);
+int og_launch_bin_tiles(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
+                        const uint32_t *tile_flag_scan, uint32_t *tile_seg, uint32_t *tile_ent,
+                        uint32_t *tile_start, hipStream_t s);
+int og_launch_bin_pads(const og_bin_desc *bins, uint32_t n_bins, uint16_t *src, uint16_t *pos, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

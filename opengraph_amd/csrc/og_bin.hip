@@ -1,0 +1,604 @@
+/*
+ * og_bin.hip -- gfx950 kernels of the binned fp32 pull (layout and passes
+ * described in og_bin.h) and of its one-time layout build.
+ *
+ * Exactness.  The reference adds a row's in-edges one after the other into
+ * an fp32 running sum (pageRank.c:998-1011).  Here every row is added in
+ * that order from LDS: one thread per short row, one wavefront per longer
+ * row and one block per row longer than a bin.  Wavefronts and blocks use an
+ * exact fast-forward: while the running sum y stays in one binade
+ * [2^(e-1), 2^e), every representable value is a multiple of u = 2^(e-24),
+ * so for a term a >= 0 the rounded sum is fl(y + a) = y + u * m with m the
+ * integer nearest a / u, unless a / u ends in exactly one half (then the
+ * tie goes to the even neighbour, which depends on y).  A run of terms is
+ * therefore y + u * (prefix sum of their m), an integer scan, up to the
+ * first term that is a tie, negative or not finite, or whose sum reaches
+ * 2^e; that term is added as an ordinary fp32 add, and the scan restarts
+ * after it in the new binade.  The result equals the sequential fp32 sum
+ * bit for bit.
+ */
+#include <hip/hip_runtime.h>
+
+#include "og_bin.h"
+#include "og_internal.h"
+#include "og_pr.h"
+
+#define BT OG_BIN_THREADS
+#define BW (OG_BIN_THREADS / 64)
+#define GATHER_U 16u  /* rounds of one wavefront in flight */
+
+/* ------------------------------------------------------------------ */
+/* shared pieces                                                       */
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ float og_bin_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ float og_bin_lane(float x, uint32_t k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
+/* inclusive prefix sum over the wavefront */
+__device__ __forceinline__ uint32_t og_bin_wave_scan(uint32_t v)
+{
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+/* The fast-forward quantities of term a for a running sum in the binade
+ * whose unit is 2^-sc: m (valid when !bad) and bad (tie, negative, not
+ * finite, or a term of at least the binade's top). */
+__device__ __forceinline__ void og_bin_term(float a, int sc, bool valid, uint32_t &m, bool &bad)
+{
+    const float t = ldexpf(a, sc);   /* a / u, exact for the ranks' range */
+    const float fl = floorf(t);
+    const float fr = t - fl;         /* exact */
+    bad = valid && (!(t >= 0.0f && t < 0x1p24f) || fr == 0.5f);
+    m = valid && !bad ? (uint32_t)fl + (fr > 0.5f ? 1u : 0u) : 0u;
+}
+
+/* The sequential fp32 sum of x[0..n) (LDS), one wavefront, result uniform. */
+__device__ float og_bin_wave_sum(const float *x, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    float y = 0.0f;
+    uint32_t i = 0;
+    while (i < n)
+    {
+        const uint32_t left = n - i;
+        const bool valid = lane < left;
+        const float a = valid ? x[i + lane] : 0.0f;
+        if (!(y >= 0x1p-100f && y < 0x1p100f))
+        {
+            /* zero, tiny or not finite: one ordinary add */
+            y = og_bin_uniform(y + og_bin_lane(a, 0));
+            i += 1;
+            continue;
+        }
+        int ey;
+        frexpf(y, &ey);
+        const int sc = 24 - ey;
+        uint32_t m;
+        bool bad;
+        og_bin_term(a, sc, valid, m, bad);
+        const uint32_t incl = og_bin_wave_scan(m);   /* < 2^30 */
+        const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
+        const bool fail = valid && (bad || Y + incl >= (1u << 24));
+        const uint64_t fb = __ballot(fail);
+        if (fb == 0)
+        {
+            const uint32_t last = (left < 64 ? left : 64) - 1;
+            y = og_bin_uniform(ldexpf((float)(Y + (uint32_t)__builtin_amdgcn_readlane(incl, last)), -sc));
+            i += last + 1;
+        }
+        else
+        {
+            const uint32_t k = (uint32_t)__builtin_ctzll(fb);
+            const uint32_t pre = k ? (uint32_t)__builtin_amdgcn_readlane(incl, k - 1) : 0u;
+            y = ldexpf((float)(Y + pre), -sc);
+            y = og_bin_uniform(y + og_bin_lane(a, k));
+            i += k + 1;
+        }
+    }
+    return y;
+}
+
+/* ------------------------------------------------------------------ */
+/* gather pass                                                         */
+/* ------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const float *__restrict__ c,
+                                                           const uint32_t *done)
+{
+    __shared__ __align__(16) float seg[OG_BIN_SEG];
+    if (*done) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    /* lane k's mask of the lanes at or below it */
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    for (uint32_t q = blockIdx.x; q < b.n_items; q += gridDim.x)
+    {
+        const og_bin_item it = b.items[q];
+        const uint32_t v0 = it.seg * OG_BIN_SEG;
+        const uint32_t nv = min(OG_BIN_SEG, b.N - v0);
+        __syncthreads();   /* the previous item's reads of seg are done */
+        {
+            const float4 *c4 = (const float4 *)(c + v0);
+            float4 *s4 = (float4 *)seg;
+            for (uint32_t i = threadIdx.x; i < nv / 4; i += BT) s4[i] = c4[i];
+            for (uint32_t i = (nv & ~3u) + threadIdx.x; i < nv; i += BT) seg[i] = c[v0 + i];
+        }
+        __syncthreads();
+        for (uint32_t r = it.r0 + wave * GATHER_U; r < it.r1; r += BW * GATHER_U)
+        {
+            uint32_t ent[GATHER_U];
+#pragma unroll
+            for (uint32_t u = 0; u < GATHER_U; u++)
+            {
+                const uint32_t rr = min(r + u, it.r1 - 1);   /* past the end: repeat the last round */
+                const og_bin_round d = b.rounds[rr];
+                const uint64_t mask = ((uint64_t)d.mask_hi << 32) | d.mask_lo;
+                const uint64_t low = mask & upto;
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(low);
+                const uint32_t t = d.first + cnt;
+                const uint32_t o = cnt ? lane - (63u - (uint32_t)__builtin_clzll(low)) : d.off0 + lane;
+                ent[u] = b.tile_ent[t] + o;
+            }
+            uint32_t sv[GATHER_U];
+#pragma unroll
+            for (uint32_t u = 0; u < GATHER_U; u++) sv[u] = __builtin_nontemporal_load(&b.src[ent[u]]);
+#pragma unroll
+            for (uint32_t u = 0; u < GATHER_U; u++)
+                if (r + u < it.r1) __builtin_nontemporal_store(seg[sv[u]], &b.val[ent[u]]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* sum pass                                                            */
+/* ------------------------------------------------------------------ */
+
+/* running statistics of the updates a thread made */
+struct og_bin_stat
+{
+    double err;
+    uint32_t active;
+};
+
+__device__ __forceinline__ void og_bin_stat_init(og_bin_stat &s)
+{
+    s.err = 0.0;
+    s.active = 0;
+}
+
+/* pageRank.c:1016-1028 and the next contribution (pageRank.c:989-996). */
+__device__ __forceinline__ void og_bin_epilogue(const og_pr_dev &a, uint32_t v, float sum, og_bin_stat &st)
+{
+    const float prev = __builtin_nontemporal_load(&a.r[v]);
+    const uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
+    const float nr = OG_BASE_PR + (OG_DAMP * sum);
+    __builtin_nontemporal_store(nr, &a.r[v]);
+    const double err = fabs((double)(nr - prev));
+    st.err += err / (double)a.N;
+    st.active += err >= a.epsilon;
+    __builtin_nontemporal_store(deg ? nr / (float)deg : 0.0f, &((float *)a.c_out)[v]);
+}
+
+/* og_stat over the block (fixed order); result in thread 0 */
+__device__ og_bin_stat og_bin_block_stat(og_bin_stat s)
+{
+    __shared__ double sh_err[BW];
+    __shared__ uint32_t sh_act[BW];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        s.err += __shfl_down(s.err, o, 64);
+        s.active += __shfl_down(s.active, o, 64);
+    }
+    if (lane == 0)
+    {
+        sh_err[w] = s.err;
+        sh_act[w] = s.active;
+    }
+    __syncthreads();
+    og_bin_stat r;
+    og_bin_stat_init(r);
+    if (threadIdx.x == 0)
+        for (uint32_t i = 0; i < BW; i++)
+        {
+            r.err += sh_err[i];
+            r.active += sh_act[i];
+        }
+    __syncthreads();
+    return r;
+}
+
+/* first-level partial idx of the final reduction (min / max unused: the
+ * neutral values) */
+__device__ __forceinline__ void og_bin_write_part(const og_pr_dev &a, uint32_t idx, const og_bin_stat &r)
+{
+    if (threadIdx.x == 0)
+    {
+        a.part_err[idx] = r.err;
+        a.part_act[idx] = r.active;
+        a.part_min[idx] = 3.402823466e38f;
+        a.part_max[idx] = -3.402823466e38f;
+    }
+}
+
+/* A bin's entries, 8 per thread and sweep: OG_BIN_SWEEPS sweeps cover a
+ * full bin; loaded into registers ahead of time (the next bin's while this
+ * one's rows are added). */
+#define OG_BIN_SWEEPS ((OG_BIN_CAP + 8 * BT - 1) / (8 * BT))
+
+typedef uint32_t og_u32x4 __attribute__((ext_vector_type(4)));
+typedef float og_f32x4 __attribute__((ext_vector_type(4)));
+
+struct og_bin_regs
+{
+    og_u32x4 p[OG_BIN_SWEEPS];
+    og_f32x4 x0[OG_BIN_SWEEPS], x1[OG_BIN_SWEEPS];
+};
+
+__device__ __forceinline__ void og_bin_load(const og_bin_dev &b, const og_bin_desc &d, og_bin_regs &r)
+{
+#pragma unroll
+    for (uint32_t k = 0; k < OG_BIN_SWEEPS; k++)
+    {
+        const uint32_t e = d.ent0 + 8 * (threadIdx.x + k * BT);
+        if (e < d.ent1)
+        {
+            r.p[k] = __builtin_nontemporal_load((const og_u32x4 *)(b.pos + e));
+            r.x0[k] = __builtin_nontemporal_load((const og_f32x4 *)(b.val + e));
+            r.x1[k] = __builtin_nontemporal_load((const og_f32x4 *)(b.val + e + 4));
+        }
+    }
+}
+
+__device__ __forceinline__ void og_bin_put(float *slot, const og_bin_desc &d, const og_bin_regs &r)
+{
+#pragma unroll
+    for (uint32_t k = 0; k < OG_BIN_SWEEPS; k++)
+    {
+        const uint32_t e = d.ent0 + 8 * (threadIdx.x + k * BT);
+        if (e < d.ent1)
+        {
+            const og_u32x4 p = r.p[k];
+            const uint32_t pp[8] = {p.x & 0xFFFFu, p.x >> 16, p.y & 0xFFFFu, p.y >> 16,
+                                    p.z & 0xFFFFu, p.z >> 16, p.w & 0xFFFFu, p.w >> 16};
+            const float xx[8] = {r.x0[k].x, r.x0[k].y, r.x0[k].z, r.x0[k].w,
+                                 r.x1[k].x, r.x1[k].y, r.x1[k].z, r.x1[k].w};
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++)
+                if (pp[t] != OG_BIN_PAD) slot[pp[t]] = xx[t];
+        }
+    }
+}
+
+__global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev a)
+{
+    __shared__ __align__(16) float slot[OG_BIN_CAP];
+    if (*a.done) return;
+    og_bin_stat st;
+    og_bin_stat_init(st);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t q = blockIdx.x;
+    og_bin_desc d;
+    og_bin_regs regs;
+    if (q < b.n_bins_live)
+    {
+        d = b.bins[q];
+        og_bin_load(b, d, regs);
+    }
+    for (; q < b.n_bins_live; q += gridDim.x)
+    {
+        __syncthreads();   /* the previous bin's reads of slot are done */
+        /* this bin's values to their CSR slots (entries padded to 8 per bin) */
+        og_bin_put(slot, d, regs);
+        const og_bin_desc cd = d;
+        /* the next bin's entries load while this one's rows are added */
+        if (q + gridDim.x < b.n_bins_live)
+        {
+            d = b.bins[q + gridDim.x];
+            og_bin_load(b, d, regs);
+        }
+        __syncthreads();
+        if (cd.kind == OG_BIN_THREAD_ROWS)
+        {
+            for (uint32_t v = cd.row0 + threadIdx.x; v < cd.row1; v += BT)
+            {
+                const uint32_t j0 = a.row_ptr[v] - cd.pos0, j1 = a.row_ptr[v + 1] - cd.pos0;
+                float s = 0.0f;
+                uint32_t j = j0;
+                for (; j + 4 <= j1; j += 4)
+                {
+                    const float t0 = slot[j], t1 = slot[j + 1], t2 = slot[j + 2], t3 = slot[j + 3];
+                    s = s + t0;
+                    s = s + t1;
+                    s = s + t2;
+                    s = s + t3;
+                }
+                for (; j < j1; j++) s = s + slot[j];
+                og_bin_epilogue(a, v, s, st);
+            }
+        }
+        else if (cd.kind == OG_BIN_WAVE_ROWS)
+        {
+            for (uint32_t v = cd.row0 + wave; v < cd.row1; v += BW)
+            {
+                const uint32_t j0 = __builtin_amdgcn_readfirstlane(a.row_ptr[v] - cd.pos0);
+                const uint32_t j1 = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1] - cd.pos0);
+                const float y = og_bin_wave_sum(slot + j0, j1 - j0);
+                if (lane == 0) og_bin_epilogue(a, v, y, st);
+            }
+        }
+        else
+        {
+            /* a piece of a long row: its values, in CSR order, to vlong */
+            for (uint32_t k = threadIdx.x; k < cd.cnt; k += BT)
+                __builtin_nontemporal_store(slot[k], &b.vlong[cd.vbase + k]);
+        }
+    }
+    og_bin_stat r = og_bin_block_stat(st);
+    og_bin_write_part(a, blockIdx.x, r);
+}
+
+/* ------------------------------------------------------------------ */
+/* long rows                                                           */
+/* ------------------------------------------------------------------ */
+
+/* One block per long row (longest first, rows q = b, b + grid, ...): the
+ * row's values from vlong in CSR order, staged through LDS in chunks of
+ * OG_BIN_LCH (the next chunk's loads in flight while one is added), added
+ * with the block-wide exact fast-forward, up to 1024 terms per step. */
+#define OG_BIN_LCH 8192u
+#define OG_BIN_LPT (OG_BIN_LCH / BT)   /* values per thread and chunk */
+
+__device__ __forceinline__ void og_bin_long_fetch(const float *x, uint32_t n, uint32_t base, float (&r)[OG_BIN_LPT])
+{
+#pragma unroll
+    for (uint32_t k = 0; k < OG_BIN_LPT; k++)
+    {
+        const uint32_t i = base + k * BT + threadIdx.x;
+        r[k] = i < n ? __builtin_nontemporal_load(&x[i]) : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
+{
+    __shared__ float buf[2][OG_BIN_LCH];
+    __shared__ uint32_t s_wsum[BW];
+    __shared__ uint32_t s_wfail[BW];
+    __shared__ uint32_t s_pre;
+    if (*a.done) return;
+    og_bin_stat st;
+    og_bin_stat_init(st);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t q = blockIdx.x; q < b.n_long; q += gridDim.x)
+    {
+        const uint32_t v = b.long_row[q];
+        const uint32_t off = b.long_off[q], n = b.long_off[q + 1] - off;
+        const float *x = b.vlong + off;
+        float pf[OG_BIN_LPT];
+        __syncthreads();   /* the previous row's reads of buf are done */
+        og_bin_long_fetch(x, n, 0, pf);
+#pragma unroll
+        for (uint32_t k = 0; k < OG_BIN_LPT; k++) buf[0][k * BT + threadIdx.x] = pf[k];
+        og_bin_long_fetch(x, n, OG_BIN_LCH, pf);
+        __syncthreads();
+        float y = 0.0f;     /* identical in every thread */
+        uint32_t cur = 0;
+        for (uint32_t cb = 0; cb < n; cb += OG_BIN_LCH)
+        {
+            const float *xs = buf[cur];
+            const uint32_t clen = n - cb < OG_BIN_LCH ? n - cb : OG_BIN_LCH;
+            uint32_t p = 0;
+            while (p < clen)
+            {
+                const uint32_t left = clen - p;
+                if (!(y >= 0x1p-100f && y < 0x1p100f))
+                {
+                    y = y + xs[p];
+                    p += 1;
+                    continue;
+                }
+                const bool valid = threadIdx.x < left;
+                const float a_t = valid ? xs[p + threadIdx.x] : 0.0f;
+                int ey;
+                frexpf(y, &ey);
+                const int sc = 24 - ey;
+                uint32_t m;
+                bool bad;
+                og_bin_term(a_t, sc, valid, m, bad);
+                const uint32_t incl_w = og_bin_wave_scan(m);   /* < 2^30 */
+                __syncthreads();                               /* previous step's reads of s_* done */
+                if (lane == 63) s_wsum[wave] = min(incl_w, 1u << 25);
+                __syncthreads();
+                uint32_t base = 0;
+                for (uint32_t w = 0; w < wave; w++) base += s_wsum[w];   /* <= 2^29 */
+                const uint32_t Y = (uint32_t)ldexpf(y, sc);
+                const uint32_t incl = base + incl_w;
+                const bool fail = valid && (bad || Y + incl >= (1u << 24));
+                const uint64_t fb = __ballot(fail);
+                if (lane == 0) s_wfail[wave] = fb ? (uint32_t)__builtin_ctzll(fb) : 64u;
+                __syncthreads();
+                uint32_t kf = 0xFFFFFFFFu;
+                for (uint32_t w = 0; w < BW; w++)
+                    if (s_wfail[w] < 64u)
+                    {
+                        kf = w * 64u + s_wfail[w];
+                        break;
+                    }
+                if (kf == 0xFFFFFFFFu)
+                {
+                    const uint32_t cntv = left < BT ? left : BT;
+                    if (threadIdx.x == cntv - 1) s_pre = incl;
+                    __syncthreads();
+                    y = ldexpf((float)(Y + s_pre), -sc);
+                    p += cntv;
+                }
+                else
+                {
+                    if (kf > 0 && threadIdx.x == kf - 1) s_pre = incl;
+                    __syncthreads();
+                    const uint32_t pre = kf ? s_pre : 0u;
+                    y = ldexpf((float)(Y + pre), -sc);
+                    y = y + xs[p + kf];
+                    p += kf + 1;
+                }
+            }
+            /* the next chunk (already loaded) into the other buffer, and the
+             * loads of the one after it */
+            if (cb + OG_BIN_LCH < n)
+            {
+                __syncthreads();
+#pragma unroll
+                for (uint32_t k = 0; k < OG_BIN_LPT; k++) buf[cur ^ 1][k * BT + threadIdx.x] = pf[k];
+                og_bin_long_fetch(x, n, cb + 2 * OG_BIN_LCH, pf);
+                __syncthreads();
+                cur ^= 1;
+            }
+        }
+        if (threadIdx.x == 0) og_bin_epilogue(a, v, y, st);
+    }
+    og_bin_stat r = og_bin_block_stat(st);
+    og_bin_write_part(a, part_base + blockIdx.x, r);
+}
+
+/* ------------------------------------------------------------------ */
+/* launch                                                              */
+/* ------------------------------------------------------------------ */
+
+extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint32_t it, uint32_t sum_grid,
+                                  uint32_t long_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (ev0) hipEventRecord(ev0, s);
+    if (b->n_items)
+        hipLaunchKernelGGL(og_bin_gather_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, (const float *)a->g_in,
+                           (const uint32_t *)a->done);
+    hipLaunchKernelGGL(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
+    if (long_grid)
+        hipLaunchKernelGGL(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, s, *b, *a, sum_grid);
+    if (ev1) hipEventRecord(ev1, s);
+    return og_hip_check(hipGetLastError(), "og_bin_pull");
+}
+
+/* ------------------------------------------------------------------ */
+/* layout build                                                        */
+/* ------------------------------------------------------------------ */
+
+/* key[j] = bin * n_seg + segment of cols[j], val[j] = j for every CSR
+ * position j of the bins (one block per bin). */
+__global__ void og_bin_keys_kernel(const og_bin_desc *__restrict__ bins, const uint32_t *__restrict__ cols,
+                                   uint32_t n_seg, uint32_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    const og_bin_desc d = bins[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < d.cnt; k += blockDim.x)
+    {
+        const uint32_t j = d.pos0 + k;
+        key[j] = blockIdx.x * n_seg + cols[j] / OG_BIN_SEG;
+        val[j] = j;
+    }
+}
+
+/* Entries of the sorted keys: src, pos at entry ent0 + (i - pos0) of bin
+ * key / n_seg; flag the first entry of each tile. */
+__global__ void og_bin_fill_kernel(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val, uint32_t n,
+                                   const og_bin_desc *__restrict__ bins, uint32_t n_seg,
+                                   const uint32_t *__restrict__ cols, uint16_t *__restrict__ src,
+                                   uint16_t *__restrict__ pos, uint32_t *__restrict__ tile_flag)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t k = key[i], j = val[i];
+        const uint32_t bi = k / n_seg, s = k - bi * n_seg;
+        const og_bin_desc d = bins[bi];
+        const uint32_t e = d.ent0 + (i - d.pos0);
+        src[e] = (uint16_t)(cols[j] - s * OG_BIN_SEG);
+        pos[e] = (uint16_t)(j - d.pos0);
+        tile_flag[i] = (i == d.pos0 || key[i - 1] != k) ? 1u : 0u;
+    }
+}
+
+__global__ void og_bin_tiles_kernel(const uint32_t *__restrict__ key, uint32_t n,
+                                    const og_bin_desc *__restrict__ bins, uint32_t n_seg,
+                                    const uint32_t *__restrict__ scan, uint32_t *__restrict__ tile_seg,
+                                    uint32_t *__restrict__ tile_ent, uint32_t *__restrict__ tile_start)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        if (scan[i + 1] == scan[i]) continue;   /* not a tile start */
+        const uint32_t t = scan[i];
+        const uint32_t k = key[i];
+        const uint32_t bi = k / n_seg;
+        const og_bin_desc d = bins[bi];
+        tile_seg[t] = k - bi * n_seg;
+        tile_ent[t] = d.ent0 + (i - d.pos0);
+        tile_start[t] = i;
+    }
+}
+
+__global__ void og_bin_pads_kernel(const og_bin_desc *__restrict__ bins, uint16_t *__restrict__ src,
+                                   uint16_t *__restrict__ pos)
+{
+    const og_bin_desc d = bins[blockIdx.x];
+    for (uint32_t e = d.ent0 + d.cnt + threadIdx.x; e < d.ent1; e += blockDim.x)
+    {
+        src[e] = 0;
+        pos[e] = (uint16_t)OG_BIN_PAD;
+    }
+}
+
+static unsigned og_bin_grid(uint64_t n)
+{
+    uint64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    return g ? (unsigned)g : 1u;
+}
+
+extern "C" int og_launch_bin_keys(const og_bin_desc *bins, uint32_t n_bins, const uint32_t *cols, uint32_t n_seg,
+                                  uint32_t *key, uint32_t *val, hipStream_t s)
+{
+    if (!n_bins) return 0;
+    hipLaunchKernelGGL(og_bin_keys_kernel, dim3(n_bins), dim3(256), 0, s, bins, cols, n_seg, key, val);
+    return og_hip_check(hipGetLastError(), "og_bin_keys_kernel");
+}
+
+extern "C" int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
+                                  uint32_t n_seg, const uint32_t *cols, uint16_t *src, uint16_t *pos,
+                                  uint32_t *tile_flag, hipStream_t s)
+{
+    if (!n) return 0;
+    hipLaunchKernelGGL(og_bin_fill_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, val, n, bins, n_seg, cols,
+                       src, pos, tile_flag);
+    return og_hip_check(hipGetLastError(), "og_bin_fill_kernel");
+}
+
+extern "C" int og_launch_bin_tiles(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
+                                   const uint32_t *tile_flag_scan, uint32_t *tile_seg, uint32_t *tile_ent,
+                                   uint32_t *tile_start, hipStream_t s)
+{
+    if (!n) return 0;
+    hipLaunchKernelGGL(og_bin_tiles_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, n, bins, n_seg,
+                       tile_flag_scan, tile_seg, tile_ent, tile_start);
+    return og_hip_check(hipGetLastError(), "og_bin_tiles_kernel");
+}
+
+extern "C" int og_launch_bin_pads(const og_bin_desc *bins, uint32_t n_bins, uint16_t *src, uint16_t *pos,
+                                  hipStream_t s)
+{
+    if (!n_bins) return 0;
+    hipLaunchKernelGGL(og_bin_pads_kernel, dim3(n_bins), dim3(64), 0, s, bins, src, pos);
+    return og_hip_check(hipGetLastError(), "og_bin_pads_kernel");
+}

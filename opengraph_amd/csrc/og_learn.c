@@ -126,6 +126,7 @@ static int og_plan_features(const og_pr_dev *base, const og_plan *pl, uint32_t n
 
 int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, uint32_t *n_final,
                       float **c1, float **init_d, float **feat, double **scratch);
+int og_pr_learn_graph(ogpu_pr *p, og_graph **g, float **tmp_a, float **tmp_b, uint32_t *classes);
 void og_pr_use_init(ogpu_pr *p, int accepted);
 
 int ogpu_mlp_add_graph(ogpu_mlp *m, ogpu_graph *g, double tol, uint32_t max_iter, double *l1_out)
@@ -304,6 +305,60 @@ out:
     return rc;
 }
 
+/* Scale factors of the coarse correction: the classes' mass balance
+ * s_i m_i = 0.15 n_i + sum_j A_ij s_j over the classes the start has mass
+ * in, A_ij = 0.85 mat_ij / 2^32, by Gaussian elimination with partial
+ * pivoting.  diag(m) - A is column diagonally dominant (the iteration keeps
+ * at most 0.85 of a class's mass), so the solution is positive; anything
+ * else (degenerate input) returns -1 and the start stays uncorrected. */
+static int og_wc_solve(uint32_t nc, const unsigned long long *mat, const double *mass, const double *count,
+                       float *scale)
+{
+    double A[OG_WC_MAX][OG_WC_MAX + 1];
+    uint32_t idx[OG_WC_MAX], k = 0;
+    for (uint32_t i = 0; i < nc; i++)
+    {
+        scale[i] = 1.0f;
+        if (mass[i] > 0.0) idx[k++] = i;
+    }
+    if (!k) return -1;
+    for (uint32_t a = 0; a < k; a++)
+    {
+        for (uint32_t b = 0; b < k; b++)
+            A[a][b] = (a == b ? mass[idx[a]] : 0.0) - 0.85 * (double)mat[idx[a] * nc + idx[b]] / 4294967296.0;
+        A[a][k] = 0.15 * count[idx[a]];
+    }
+    for (uint32_t c = 0; c < k; c++)
+    {
+        uint32_t piv = c;
+        for (uint32_t r = c + 1; r < k; r++)
+            if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (!(fabs(A[piv][c]) > 0.0)) return -1;
+        if (piv != c)
+            for (uint32_t q = 0; q <= k; q++)
+            {
+                double t = A[c][q];
+                A[c][q] = A[piv][q];
+                A[piv][q] = t;
+            }
+        for (uint32_t r = c + 1; r < k; r++)
+        {
+            const double f = A[r][c] / A[c][c];
+            for (uint32_t q = c; q <= k; q++) A[r][q] -= f * A[c][q];
+        }
+    }
+    double x[OG_WC_MAX];
+    for (int r = (int)k - 1; r >= 0; r--)
+    {
+        double t = A[r][k];
+        for (uint32_t q = (uint32_t)r + 1; q < k; q++) t -= A[r][q] * x[q];
+        x[r] = t / A[r][r];
+        if (!(x[r] > 0.0 && x[r] < 1e30)) return -1;
+    }
+    for (uint32_t a = 0; a < k; a++) scale[idx[a]] = (float)x[a];
+    return 0;
+}
+
 int ogpu_pr_set_init_learned(ogpu_pr *p, ogpu_mlp *m, double *seconds)
 {
     const og_pr_dev *dev;
@@ -322,6 +377,34 @@ int ogpu_pr_set_init_learned(ogpu_pr *p, ogpu_mlp *m, double *seconds)
         og_hip_check(hipMemcpyAsync(&ok, scr + OG_WARM_OK, 8, hipMemcpyDeviceToHost, s), "D2H") ||
         og_hip_check(hipStreamSynchronize(s), "sync warm start"))
         return -1;
+    /* coarse correction of an accepted start over in-degree classes */
+    og_graph *g;
+    float *tmp_a, *tmp_b;
+    uint32_t nc;
+    og_pr_learn_graph(p, &g, &tmp_a, &tmp_b, &nc);
+    if (ok != 0.0 && nc >= 2)
+    {
+        unsigned long long mat[OG_WC_MAX * OG_WC_MAX];
+        double mass[OG_WC_MAX], count[OG_WC_MAX];
+        float scale[OG_WC_MAX];
+        uint8_t *cls = (uint8_t *)tmp_a;
+        if (og_launch_wc_sums(init_d, g->f_in_degree, g->f_out_degree, g->N, nc, cls, tmp_b, scr, s) ||
+            og_launch_wc_matrix(g->f_src, g->f_dst, g->E, cls, tmp_b, nc,
+                                (unsigned long long *)(scr + og_wc_matrix()), s) ||
+            og_hip_check(hipMemcpyAsync(mat, scr + og_wc_matrix(), (size_t)nc * nc * 8, hipMemcpyDeviceToHost, s),
+                         "D2H") ||
+            og_hip_check(hipMemcpyAsync(mass, scr + og_wc_mass(), (size_t)nc * 8, hipMemcpyDeviceToHost, s), "D2H") ||
+            og_hip_check(hipMemcpyAsync(count, scr + og_wc_count(), (size_t)nc * 8, hipMemcpyDeviceToHost, s),
+                         "D2H") ||
+            og_hip_check(hipStreamSynchronize(s), "sync warm correction"))
+            return -1;
+        if (!og_wc_solve(nc, mat, mass, count, scale) &&
+            (og_hip_check(hipMemcpyAsync(scr + og_wc_scale(), scale, (size_t)nc * 4, hipMemcpyHostToDevice, s),
+                          "H2D") ||
+             og_launch_wc_apply(init_d, cls, g->N, (const float *)(scr + og_wc_scale()), s) ||
+             og_hip_check(hipStreamSynchronize(s), "sync warm correction")))
+            return -1;
+    }
     /* a rejected prediction (non-finite, negative, no mass) leaves the
      * uniform start in place */
     og_pr_use_init(p, ok != 0.0);

@@ -16,6 +16,8 @@
 
 #include "og_internal.h"
 #include "og_learn.h"
+#include "og_pr.h"
+#include "opengraph_amd_ext.h"
 
 #define LB 256
 
@@ -434,6 +436,109 @@ __global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t
 }
 
 /* ------------------------------------------------------------------ */
+/* coarse correction of the start (in-degree classes)                  */
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ uint32_t og_wc_class(uint32_t indeg, uint32_t nc)
+{
+    if (!indeg) return 0;
+    const uint32_t l = 31u - (uint32_t)__builtin_clz(indeg);
+    return 1u + min(l, nc - 2u);
+}
+
+/* Per vertex: its class and c0 = r0 / out-degree; per block the mass of
+ * r0 and the vertex count of each class (block b's partials at
+ * part[b * 2 * OG_WC_MAX ...]); the last step folds them in block order. */
+__global__ __launch_bounds__(LB) void og_wc_sums_kernel(const float *__restrict__ r0,
+                                                       const uint32_t *__restrict__ indeg,
+                                                       const uint32_t *__restrict__ outdeg, uint32_t n, uint32_t nc,
+                                                       uint8_t *__restrict__ cls, float *__restrict__ c0,
+                                                       double *__restrict__ part)
+{
+    __shared__ double sm[OG_WC_MAX][LB];
+    __shared__ double sn[OG_WC_MAX][LB];
+    for (uint32_t k = 0; k < OG_WC_MAX; k++)
+    {
+        sm[k][threadIdx.x] = 0.0;
+        sn[k][threadIdx.x] = 0.0;
+    }
+    for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
+    {
+        const uint32_t c = og_wc_class(indeg[v], nc);
+        const uint32_t d = outdeg[v];
+        const float x = r0[v];
+        cls[v] = (uint8_t)c;
+        c0[v] = d ? x / (float)d : 0.0f;
+        sm[c][threadIdx.x] += x;
+        sn[c][threadIdx.x] += 1.0;
+    }
+    __syncthreads();
+    for (uint32_t w = LB / 2; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w)
+            for (uint32_t k = 0; k < OG_WC_MAX; k++)
+            {
+                sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+                sn[k][threadIdx.x] += sn[k][threadIdx.x + w];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < OG_WC_MAX)
+    {
+        part[(size_t)blockIdx.x * 2 * OG_WC_MAX + threadIdx.x] = sm[threadIdx.x][0];
+        part[(size_t)blockIdx.x * 2 * OG_WC_MAX + OG_WC_MAX + threadIdx.x] = sn[threadIdx.x][0];
+    }
+}
+
+__global__ void og_wc_fold_kernel(const double *__restrict__ part, double *__restrict__ mass,
+                                  double *__restrict__ count)
+{
+    const uint32_t k = threadIdx.x;
+    if (k >= OG_WC_MAX) return;
+    double m = 0.0, c = 0.0;
+    for (uint32_t b = 0; b < OG_WC_BLOCKS; b++)
+    {
+        m += part[(size_t)b * 2 * OG_WC_MAX + k];
+        c += part[(size_t)b * 2 * OG_WC_MAX + OG_WC_MAX + k];
+    }
+    mass[k] = m;
+    count[k] = c;
+}
+
+/* mat[class(v) * nc + class(u)] += c0[u] over the forward edges (u, v),
+ * in Q32.32: block sums in LDS, then one global add per entry and block. */
+__global__ __launch_bounds__(LB) void og_wc_matrix_kernel(const uint32_t *__restrict__ src,
+                                                         const uint32_t *__restrict__ dst, uint32_t E,
+                                                         const uint8_t *__restrict__ cls,
+                                                         const float *__restrict__ c0, uint32_t nc,
+                                                         unsigned long long *__restrict__ mat)
+{
+    __shared__ unsigned long long sh[OG_WC_MAX * OG_WC_MAX];
+    for (uint32_t k = threadIdx.x; k < OG_WC_MAX * OG_WC_MAX; k += LB) sh[k] = 0ull;
+    __syncthreads();
+    for (uint32_t e = blockIdx.x * LB + threadIdx.x; e < E; e += gridDim.x * LB)
+    {
+        const uint32_t u = src[e], v = dst[e];
+        const float x = c0[u];
+        if (x > 0.0f)
+        {
+            const unsigned long long q = (unsigned long long)((double)x * 4294967296.0 + 0.5);
+            atomicAdd(&sh[cls[v] * nc + cls[u]], q);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nc * nc; k += LB)
+        if (sh[k]) atomicAdd(&mat[k], sh[k]);
+}
+
+__global__ void og_wc_apply_kernel(float *__restrict__ r0, const uint8_t *__restrict__ cls, uint32_t n,
+                                   const float *__restrict__ scale)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        r0[v] *= scale[cls[v]];
+}
+
+/* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
 
@@ -443,6 +548,37 @@ static unsigned og_lgrid(uint64_t n)
     if (g < 1) g = 1;
     if (g > 16384) g = 16384;
     return (unsigned)g;
+}
+
+extern "C" int og_launch_wc_sums(const float *r0, const uint32_t *in_degree, const uint32_t *out_degree,
+                                 uint32_t n, uint32_t nc, uint8_t *cls, float *c0, double *scratch, hipStream_t s)
+{
+    if (nc < 2 || nc > OG_WC_MAX)
+    {
+        og_set_error("coarse start correction: %u classes (2..%u)", nc, OG_WC_MAX);
+        return -1;
+    }
+    hipLaunchKernelGGL(og_wc_sums_kernel, dim3(OG_WC_BLOCKS), dim3(LB), 0, s, r0, in_degree, out_degree, n, nc,
+                       cls, c0, scratch + OG_WC_BASE);
+    hipLaunchKernelGGL(og_wc_fold_kernel, dim3(1), dim3(64), 0, s, scratch + OG_WC_BASE, scratch + og_wc_mass(),
+                       scratch + og_wc_count());
+    return og_hip_check(hipGetLastError(), "og_wc_sums_kernel");
+}
+
+extern "C" int og_launch_wc_matrix(const uint32_t *src, const uint32_t *dst, uint32_t E, const uint8_t *cls,
+                                   const float *c0, uint32_t nc, unsigned long long *mat, hipStream_t s)
+{
+    if (og_launch_fill_u32((uint32_t *)mat, 0, 2ull * nc * nc, s)) return -1;
+    if (!E) return 0;
+    hipLaunchKernelGGL(og_wc_matrix_kernel, dim3(og_lgrid(E) < 2048 ? og_lgrid(E) : 2048), dim3(LB), 0, s, src,
+                       dst, E, cls, c0, nc, mat);
+    return og_hip_check(hipGetLastError(), "og_wc_matrix_kernel");
+}
+
+extern "C" int og_launch_wc_apply(float *r0, const uint8_t *cls, uint32_t n, const float *scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(og_wc_apply_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, r0, cls, n, scale);
+    return og_hip_check(hipGetLastError(), "og_wc_apply_kernel");
 }
 
 extern "C" int og_launch_feat_li(const uint32_t *row_ptr, uint32_t n, float *li, hipStream_t s)

@@ -15,6 +15,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "og_bin.h"
 #include "og_internal.h"
 #include "og_learn.h"
 #include "og_pr.h"
@@ -29,6 +30,9 @@
 /* fp32 pull: rows of more in-edges are summed in the reference's order by
  * default (exact_above) */
 #define OG_DEFAULT_EXACT_ABOVE 4096u
+/* fp32 push: destinations with more in-edges are gathered in CSR order
+ * (when exact_above is not 0) */
+#define OG_PUSH_EXACT_ABOVE 1024u
 /* every-row-exact fp32 pull: rows of at most this many in-edges are summed
  * one per thread, the longer ones as wavefront chains */
 #ifndef OG_EXACT_SEQ_T
@@ -120,6 +124,7 @@ void ogpu_pr_free(ogpu_pr *p)
     og_dfree(p->wl[0]); og_dfree(p->wl[1]); og_dfree(p->changed); og_dfree(p->resid);
     og_dfree(p->feat); og_dfree(p->lscratch);
     og_dfree(p->chain_rows);
+    og_bin_plan_destroy(p->bins);
     free(p);
 }
 
@@ -129,6 +134,11 @@ void ogpu_pr_free(ogpu_pr *p)
 int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, uint32_t *n_final,
                       float **c1, float **init_d, float **feat, double **scratch)
 {
+    if (p->opt.warm_classes > OG_WC_MAX)
+    {
+        og_set_error("learned start: warm_classes %u (0, 1 or 2..%u)", p->opt.warm_classes, OG_WC_MAX);
+        return -1;
+    }
     if (p->mode != OG_MODE_F32 || p->push || p->dd)
     {
         og_set_error("learned start: fp32 pull runs (variant 0) only");
@@ -145,6 +155,17 @@ int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, u
     *init_d = p->init_d;
     *feat = p->feat;
     *scratch = p->lscratch;
+    return 0;
+}
+
+/* The graph, two N-float buffers free until the next run's init (r, c[0])
+ * and the number of correction classes (1: none) of a learned start. */
+int og_pr_learn_graph(ogpu_pr *p, og_graph **g, float **tmp_a, float **tmp_b, uint32_t *classes)
+{
+    *g = p->g;
+    *tmp_a = p->r;
+    *tmp_b = (float *)p->c[0];
+    *classes = p->opt.warm_classes ? p->opt.warm_classes : 8u;
     return 0;
 }
 
@@ -224,12 +245,29 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     if (p->opt.hub_threshold == 0) p->opt.hub_threshold = OG_DEFAULT_SEQ_T;
     og_variant_mode(opt->variant, &p->mode, &p->push, &p->quant, &p->dd);
     const uint32_t N = g->N, E = g->E;
+    /* the binned fp32 pull (og_bin.h) on a plan that keeps every row's
+     * in-neighbours in CSR order; a borrowed plan that sorts long rows by
+     * source rules it out */
+    int binned = opt->variant == 0 && !opt->ppr && opt->binned != 2 && g->E > 0 &&
+                 !(plan && plan->relabel == 1 && plan->H);
+    if (opt->binned == 1 && !binned)
+    {
+        og_set_error("binned pull: fp32 pull (variant 0, not personalized) on a graph with edges and a "
+                     "plan in CSR order only");
+        free(p);
+        return -1;
+    }
     /* fp32 pull with every row in the reference's order: the rows of more
      * than OG_EXACT_SEQ_T in-edges become wavefront chains over CSR-order
      * rows (relabel 1's sorted long rows are kept in CSR order instead) */
-    const int chain = opt->variant == 0 && opt->hub_threshold >= 0x80000000u;
-    /* otherwise, with exact_above, only the longest rows become chains */
-    const uint32_t chain_t = !chain && opt->variant == 0 && !opt->ppr ? opt->exact_above : 0;
+    const int chain = !binned && opt->variant == 0 && opt->hub_threshold >= 0x80000000u;
+    /* otherwise, with exact_above, only the longest rows become chains; the
+     * fp32 push gathers its destinations of more than OG_PUSH_EXACT_ABOVE
+     * in-edges the same way (their atomic sums would stray furthest from the
+     * reference's source-ordered adds) */
+    const uint32_t chain_t = binned ? 0
+                           : !chain && opt->variant == 0 && !opt->ppr ? opt->exact_above
+                           : opt->variant == 1 && opt->exact_above ? OG_PUSH_EXACT_ABOVE : 0;
     double t0 = og_now();
     if (plan)
     {
@@ -243,7 +281,7 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     }
     else
     {
-        const uint32_t rel = chain && p->opt.relabel == 1 ? 2u : p->opt.relabel;
+        const uint32_t rel = (chain || binned) && p->opt.relabel == 1 ? 2u : p->opt.relabel;
         if (og_plan_build(g, rel, chain ? OG_EXACT_SEQ_T : p->opt.hub_threshold, chain_t, p->push || p->dd,
                           &p->plan))
         {
@@ -256,6 +294,21 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     {
         ogpu_pr_free(p);
         return -1;
+    }
+    if (binned && og_bin_plan_build(p->plan, &p->bins))
+    {
+        if (opt->binned == 1)
+        {
+            ogpu_pr_free(p);
+            return -1;
+        }
+        p->bins = NULL;   /* default: the gather pull instead */
+    }
+    if (p->bins)
+    {
+        p->bin_sum_grid = og_bin_sum_grid(p->bins);
+        p->bin_long_grid = og_bin_long_grid(p->bins);
+        p->n_final_bin = og_final_blocks(p->bin_sum_grid + p->bin_long_grid, 0);
     }
     p->seconds_plan = og_now() - t0;
     og_plan *pl = p->plan;
@@ -324,6 +377,7 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
      * chains [0, C) and no chunks; fp32 pull on a plan with chains [0, L),
      * their chunks skipped; otherwise every chunk */
     uint32_t n_chains = 0, ch_first = 0, chunk_blocks = pl->n_chunk_blocks;
+    const int push_chains = opt->variant == 1 && pl->L;
     if (p->chain_rows)
     {
         n_chains = pl->C;
@@ -334,18 +388,25 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         n_chains = pl->L;
         ch_first = pl->ch_first;
     }
+    else if (push_chains)
+        n_chains = pl->L;
     else if (pl->L)
         chunk_blocks = ((pl->n_chunks + OG_CH_PER_WAVE - 1) / OG_CH_PER_WAVE + 3) / 4;
-    /* four chains per 256-thread pull block; beside the persistent kernel,
-     * one wavefront per CU taking every grid-th chain */
-    const uint32_t chain_blocks = hot_grid && !p->chain_rows ? (n_chains < hot_grid ? n_chains : hot_grid)
-                                                             : (n_chains + 3) / 4;
+    /* four chains per 256-thread pull block; beside the persistent kernel
+     * (or the push scatter), one wavefront per CU taking every grid-th chain */
+    const uint32_t cus = (uint32_t)og_num_cus();
+    const uint32_t chain_blocks = push_chains ? (n_chains < cus ? n_chains : cus)
+                                : hot_grid && !p->chain_rows ? (n_chains < hot_grid ? n_chains : hot_grid)
+                                : (n_chains + 3) / 4;
     const uint32_t pull_blocks = hot_grid ? hot_grid + chain_blocks : chain_blocks + chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
-    p->n_final_vert = og_final_blocks(p->gv, 0);
+    p->n_final_vert = og_final_blocks(p->gv + (push_chains ? chain_blocks : 0), 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
+    if (push_chains && parts < p->gv + chain_blocks) parts = p->gv + chain_blocks;
+    if (p->bins && parts < p->bin_sum_grid + p->bin_long_grid) parts = p->bin_sum_grid + p->bin_long_grid;
     p->parts_cap = parts;
     uint32_t fins = p->n_final_pull > p->n_final_vert ? p->n_final_pull : p->n_final_vert;
+    if (fins < p->n_final_bin) fins = p->n_final_bin;
     p->stats_cap = (opt->max_iter ? opt->max_iter : 1) + 1;
 
     size_t eb = og_elem_bytes(p->mode);
@@ -418,6 +479,7 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         a->n_chunks = 0;
     }
     a->n_row_blocks = pl->n_row_blocks;
+    a->push_skip = push_chains ? n_chains : 0;
     a->hot_grid = hot_grid;
     a->stop_l1 = opt->stop_rule == 1 || opt->ppr;
     a->ppr = opt->ppr ? 1 : 0;
@@ -554,6 +616,28 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     a.c_out = p->quant ? p->c[0] : p->c[(it + 1) & 1];
     uint32_t row = it % p->stats_cap;
     int rc;
+    if (p->bins)
+    {
+        /* binned pull: gather, sum (rows without in-edges only in the first
+         * two iterations, as below), long rows, final reduction */
+        og_bin_dev b = p->bins->d;
+        if (it >= 2)
+        {
+            b.n_bins_live = b.n_bins - p->bins->n_zero_bins;
+            a.z_active = 0.0 >= a.epsilon ? a.N - a.Z : 0;
+        }
+        a.n_parts = p->bin_sum_grid + p->bin_long_grid;
+        a.n_final_blocks = p->n_final_bin;
+        if (a.n_parts > p->parts_cap)
+        {
+            og_set_error("binned pull with %u partials, %u allocated", a.n_parts, p->parts_cap);
+            return -1;
+        }
+        if (og_launch_bin_pull(&b, &a, row, p->bin_sum_grid, p->bin_long_grid, s, e0, e1) ||
+            og_launch_final(&a, row, s))
+            return -1;
+        return 0;
+    }
     if (!p->push && !p->quant && !p->dd && it >= 2)
     {
         /* Rows [Z, N) have no in-edges: from iteration 1 on their rank is
@@ -577,7 +661,7 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
     }
     else
     {
-        a.n_parts = p->gv;
+        a.n_parts = p->gv + (a.push_skip ? a.n_chain_blocks : 0);
         a.n_final_blocks = p->n_final_vert;
         rc = og_launch_push(&a, p->mode, row, s, e0, e1);
     }
@@ -1063,7 +1147,9 @@ static struct PageRankStats *og_dropin(struct Arguments *arguments, struct Graph
     og_plan *plan = NULL;
     if (og_cache_graph(graph, push || dd, &g, &plan)) og_fatal("graph upload");
     ogpu_pr *pr = NULL;
-    if (og_pr_create_with(g, plan, &o, &pr)) og_fatal("pagerank setup");
+    /* the fp32 pull runs binned, on a plan of its own that keeps every row
+     * in CSR order (the cached plan sorts long rows for the gather pulls) */
+    if (og_pr_create_with(g, variant == 0 ? NULL : plan, &o, &pr)) og_fatal("pagerank setup");
 
     struct PageRankStats *stats = newPageRankStatsGraphCSR(graph);
     const char *title = og_variant_title(variant);
@@ -1139,6 +1225,7 @@ int ogpu_tune_hub_threshold(ogpu_graph *gg, const uint32_t *cands, uint32_t n_ca
         ogpu_pr_default_options(&o);
         o.max_iter = iters + 2;
         o.hub_threshold = cands[i] ? cands[i] : OG_DEFAULT_SEQ_T;
+        o.binned = 2;   /* the threshold belongs to the gather pull */
         ogpu_pr *p = NULL;
         if (og_pr_create_with(g, NULL, &o, &p)) return -1;
         /* two warm-up iterations, then `iters` timed on the host clock
@@ -1275,6 +1362,7 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
          * kernels have no wavefront-chain items */
         o.hub_threshold = m->hub[g] >= 0x80000000u ? 0x7FFFFFFFu : m->hub[g];
         o.exact_above = 0;
+        o.binned = 2;
         if (og_pr_create_with(graphs[g], NULL, &o, &m->pr[g])) goto fail;
         ogpu_pr *p = m->pr[g];
         for (int q = 0; q < 2; q++)

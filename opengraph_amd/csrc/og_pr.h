@@ -68,6 +68,11 @@ typedef struct og_pr_dev
      * of the multi-chunk class, whose chunks [0, ch_first) are skipped. */
     const uint32_t *chain_rows;
     uint32_t n_chains, n_chain_blocks, ch_first;
+    /* first statistics partial of og_pull_chains_kernel's blocks */
+    uint32_t chain_part_base;
+    /* fp32 push: destinations [0, push_skip) are the chains, summed by
+     * gathering in CSR order; the scatter and its epilogue leave them out */
+    uint32_t push_skip;
     double epsilon;
     /* plan */
     const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;

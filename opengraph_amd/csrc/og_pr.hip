@@ -764,12 +764,13 @@ __global__ __launch_bounds__(OG_BLOCK) void og_pull_kernel(og_pr_dev a, uint32_t
     }
 }
 
-/* The chains of a pull whose other rows run in the persistent kernel: a
- * launch of its own on a side stream, concurrent with it.  One wavefront
- * per block, at most one block per CU (grid = n_chain_blocks <= CUs), each
- * taking chains q = b, b + grid, ...; no LDS and at most 96 registers, so
- * a wavefront fits on a SIMD beside the persistent block's four.  Its
- * statistics partials follow the persistent blocks'. */
+/* The chains of a pull whose other rows run in the persistent kernel (or
+ * of a push whose other destinations are scattered): a launch of its own on
+ * a side stream, concurrent with it.  One wavefront per block, at most one
+ * block per CU (grid = n_chain_blocks <= CUs), each taking chains q = b,
+ * b + grid, ...; no LDS and at most 96 registers, so a wavefront fits on a
+ * SIMD beside the persistent block's four.  Its statistics partials start
+ * at chain_part_base, after the other kernel's. */
 __global__ __launch_bounds__(64, 5) void og_pull_chains_kernel(og_pr_dev a, uint32_t it)
 {
     if (*a.done) return;
@@ -779,7 +780,7 @@ __global__ __launch_bounds__(64, 5) void og_pull_chains_kernel(og_pr_dev a, uint
     og_stat r = og_block_stat<1>(st);
     if (threadIdx.x == 0)
     {
-        const uint32_t b = a.hot_grid + blockIdx.x;
+        const uint32_t b = a.chain_part_base + blockIdx.x;
         a.part_err[b] = r.err;
         a.part_act[b] = r.active;
         a.part_min[b] = r.cmin;
@@ -1039,6 +1040,7 @@ __global__ __launch_bounds__(HOT ? OG_HOT_SUBS * OG_BLOCK : OG_BLOCK) void og_pu
         for (uint32_t u = 0; u < B; u++)
         {
             if (e0 + u * stride >= E) break;
+            if (d[u] < a.push_skip) continue;
             if (HOT && d[u] < nh) atomicAdd(&acc[d[u]], add[u]);
             else if constexpr (PMODE == OG_MODE_FX64) atomicAdd(&next[d[u]], add[u]);
             else unsafeAtomicAdd(&next[d[u]], add[u]);
@@ -1064,7 +1066,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_push_epilogue_kernel(og_pr_dev a)
     og_stat st;
     og_stat_init(st);
     const int EMODE = PMODE == OG_MODE_FX64 ? OG_MODE_FX64 : (PMODE == OG_MODE_F32 ? OG_MODE_F32 : OG_MODE_QPUSH_EPI);
-    for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
+    for (uint32_t v = a.push_skip + blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
     {
         if constexpr (EMODE == OG_MODE_FX64)
         {
@@ -1187,7 +1189,9 @@ template <int MODE> static int og_launch_pull_t(const og_pr_dev *a, uint32_t it,
         if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
             og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
             return -1;
-        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, side, *a, it);
+        og_pr_dev b = *a;
+        b.chain_part_base = a->hot_grid;
+        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, side, b, it);
         hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
         if (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
@@ -1251,13 +1255,29 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
     if (ev0) hipEventRecord(ev0, s);
     if (pmode == OG_MODE_F32)
     {
+        /* the chain destinations gathered on the side stream while the
+         * others are scattered; their partials follow the epilogue's */
+        const uint32_t nc = a->push_skip ? a->n_chain_blocks : 0;
+        hipStream_t side = og_side_stream();
+        if (nc)
+        {
+            og_pr_dev b = *a;
+            b.chain_part_base = a->n_parts - nc;
+            if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
+                og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
+                return -1;
+            hipLaunchKernelGGL(og_pull_chains_kernel, dim3(nc), dim3(64), 0, side, b, it);
+        }
         if (a->hot_grid)
             hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
             hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
         if (ev1) hipEventRecord(ev1, s);
-        hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_F32>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
+        hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_F32>, dim3(a->n_parts - nc), dim3(OG_BLOCK), 0, s, *a);
+        if (nc && (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
+                   og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent")))
+            return -1;
     }
     else if (pmode == OG_MODE_FX64)
     {
@@ -1390,14 +1410,25 @@ __global__ void og_dd_resid_init_kernel(og_pr_dev a)
 
 /* One worklist vertex v of the data-driven push (pageRank.c:2648-2677) or
  * pull-push (:2810-2850): its new rank, then its residual moved to its
- * out-neighbours with atomic adds. */
+ * out-neighbours with atomic adds.
+ *
+ * The reference reads v's residual, pushes it and then stores 0 into it:
+ * whatever another thread adds to it in between is lost.  With a few CPU
+ * threads that window rarely matters; with every worklist vertex in flight
+ * at once it would drop much of the mass.  So the concurrent push (`take`)
+ * takes the residual with one atomic exchange instead: what arrives later
+ * stays for the next iteration, as it does in the reference's
+ * one-thread order.  Its self-loops push nothing, since in that order the
+ * final store drops what a self-loop added.  The serial form (one-chunk
+ * graphs) keeps the reference's read / push / store-0 sequence. */
 __device__ __forceinline__ void og_dd_push_vertex(const og_pr_dev &a, uint32_t code, uint32_t v, float fn,
-                                                  og_stat &st)
+                                                  og_stat &st, bool take = false)
 {
     float old = a.r[v];
     float nr;
+    const float res = take ? atomicExch(&a.resid[v], 0.0f) : a.resid[v];
     if (code == 7)
-        nr = a.resid[v] + old;
+        nr = res + old;
     else
     {
         float s = 0.0f;
@@ -1411,10 +1442,11 @@ __device__ __forceinline__ void og_dd_push_vertex(const og_pr_dev &a, uint32_t c
     st.err += fabs((double)(nr / fn - old / fn));
     __hip_atomic_store(&a.r[v], nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t deg = a.outdeg[v];
-    float delta = OG_DAMP * (a.resid[v] / (float)deg);
+    float delta = OG_DAMP * ((take ? res : a.resid[v]) / (float)deg);
     for (uint32_t j = a.f_row_ptr[v]; j < a.f_row_ptr[v + 1]; j++)
     {
         uint32_t u = a.f_cols[j];
+        if (take && u == v) continue;
         float prev = atomicAdd(&a.resid[u], delta);
         if ((fabs((double)(prev + delta)) >= a.epsilon) && ((double)prev <= a.epsilon))
         {
@@ -1423,7 +1455,7 @@ __device__ __forceinline__ void og_dd_push_vertex(const og_pr_dev &a, uint32_t c
             a.wl_next[u] = 1;
         }
     }
-    a.resid[v] = 0.0f;
+    if (!take) a.resid[v] = 0.0f;
 }
 
 /* pageRank.c:2645-2685 (push) and :2796-2852 (pull-push).  The reference
@@ -1452,7 +1484,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_dd_push_kernel(og_pr_dev a, uint3
     }
     else
         for (uint32_t v = blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
-            if (a.wl_cur[v]) og_dd_push_vertex(a, code, v, fn, st);
+            if (a.wl_cur[v]) og_dd_push_vertex(a, code, v, fn, st, code == 7);
     og_stat r = og_block_stat(st);
     if (threadIdx.x == 0)
     {

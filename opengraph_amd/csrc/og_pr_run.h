@@ -59,6 +59,10 @@ struct ogpu_pr
     double *lscratch;
     /* every-row-exact fp32 pull: rows [0, C) longest first (device) */
     uint32_t *chain_rows;
+    /* binned fp32 pull (og_bin.h): layout, grids of its passes, blocks of
+     * its final reduction */
+    struct og_bin_plan *bins;
+    uint32_t bin_sum_grid, bin_long_grid, n_final_bin;
     /* the last start vector offered (ogpu_pr_set_init, the learned start)
      * was rejected and the runs start from the uniform vector */
     uint32_t init_fallback;

@@ -82,6 +82,10 @@ int og_spmv_create_with(og_graph *g, og_plan *plan, const ogpu_spmv_options *opt
      * once per block by default.  Its sums are unordered in the reference
      * too (omp atomic); block partials only make them closer to exact. */
     po.hot_cache = o.code == 1 && o.hot_cache == 0 ? 1 : o.hot_cache;
+    /* every destination of the -p 1 scatter takes its adds as atomics (the
+     * PageRank push would gather its longest rows instead) */
+    if (o.code == 1) po.exact_above = 0;
+    po.binned = 2;   /* SpMV passes run on the gather pull's kernels */
     if (og_pr_create_with(g, plan, &po, &s->p))
     {
         free(s);

@@ -111,6 +111,9 @@ def ref_lib():
         R.ogref_pagerank_stdout.restype = C.c_int
         R.ogref_reorder.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, C.c_int, u32p, f64p]
         R.ogref_reorder.restype = C.c_int
+        R.ogref_csr_build.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, C.c_uint32, C.c_int,
+                                      C.POINTER(u32p), f64p]
+        R.ogref_csr_build.restype = C.c_int
         R.ogref_write_csr.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [C.c_char_p]
         R.ogref_write_csr.restype = C.c_int
         R.ogref_spmv.argtypes = [C.c_uint32, C.c_uint32] + [u32p] * 10 + [
@@ -321,6 +324,23 @@ def ref_reorder(src, dst, num_vertices, lmode, threads=1):
     if R.ogref_reorder(num_vertices, len(s), _p(s), _p(d), lmode, threads, _p(lab), C.byref(sec)):
         raise RuntimeError("ogref_reorder failed")
     return lab, sec.value
+
+
+def ref_csr_build(src, dst, num_vertices, sort_mode=0, threads=1):
+    """Run the reference's own compiled CSR preprocessing (sortRunAlgorithms,
+    graphCSRAssignEdgeList with mapVerticesWithInOutDegree, the inverse list)
+    on a copy of the edge list; returns (dict of the ten arrays, seconds)."""
+    R = ref_lib()
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    E = len(s)
+    out = {k: np.zeros(E if k in ("f_src", "f_dst", "i_src", "i_dst") else num_vertices, np.uint32)
+           for k in CSR_FIELDS}
+    ptrs = (u32p * 10)(*[_p(out[k]) for k in CSR_FIELDS])
+    sec = C.c_double()
+    if R.ogref_csr_build(num_vertices, E, _p(s), _p(d), sort_mode, threads, ptrs, C.byref(sec)):
+        raise RuntimeError("ogref_csr_build failed")
+    return out, sec.value
 
 
 def rank_stats_topk(ref_pr, ref_order, pr, order, topk, num_vertices=None):

@@ -25,6 +25,7 @@
 #include "pageRank.h"
 #include "reorder.h"
 #include "SPMV.h"
+#include "sortRun.h"
 
 int ogref_pagerank(uint32_t N, uint32_t E, uint32_t *f_out_degree, uint32_t *f_in_degree,
                    uint32_t *f_edges_idx, uint32_t *f_src, uint32_t *f_dst,
@@ -339,5 +340,60 @@ int ogref_collect_stats(uint32_t N, float *ref_pr, uint32_t *ref_order, float *p
     if (saved >= 0) { dup2(saved, 1); close(saved); }
     if (sink >= 0) close(sink);
     *seconds_out = t1 - t0;
+    return 0;
+}
+
+/* The reference's CSR preprocessing (graphCSR.c:164-248, without the file
+ * read, duplicate removal, relabelling and masks) on caller edge arrays:
+ * sortRunAlgorithms (-o sort), graphCSRNew, graphCSRAssignEdgeList (which
+ * runs mapVerticesWithInOutDegree), readEdgeListsMem (the inverse list),
+ * sortRunAlgorithms and graphCSRAssignEdgeList on it.  seconds_out: wall
+ * time of those steps (omp_get_wtime, the edge-list copy-in excluded) on
+ * `threads` OpenMP threads, stdout silenced.  out[10] (caller-owned, E or N
+ * entries) get f_src, f_dst, f_out_degree, f_in_degree, f_edges_idx, i_src,
+ * i_dst, i_out_degree, i_in_degree, i_edges_idx. */
+int ogref_csr_build(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst, uint32_t sort,
+                    int threads, uint32_t *const out[10], double *seconds_out)
+{
+    struct EdgeList *el = newEdgeList(E);
+    if (!el) return -1;
+    memcpy(el->edges_array_src, src, (size_t)E * 4);
+    memcpy(el->edges_array_dest, dst, (size_t)E * 4);
+    el->num_vertices = N;
+    el->avg_degree = N ? E / N : 0;
+    el->mask_array = (uint32_t *)my_malloc((size_t)N * 4);
+    el->label_array = (uint32_t *)my_malloc((size_t)N * 4);
+    el->inverse_label_array = (uint32_t *)my_malloc((size_t)N * 4);
+    for (uint32_t v = 0; v < N; v++)
+    {
+        el->mask_array[v] = 0;
+        el->label_array[v] = v;
+        el->inverse_label_array[v] = v;
+    }
+    omp_set_num_threads(threads);
+    fflush(stdout);
+    int saved = dup(1);
+    int devnull = open("/dev/null", O_WRONLY);
+    if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+    double t0 = omp_get_wtime();
+    el = sortRunAlgorithms(el, sort);
+    struct GraphCSR *g = graphCSRNew(el->num_vertices, el->num_edges, 1);
+    g = graphCSRAssignEdgeList(g, el, 0);
+    struct EdgeList *iel = readEdgeListsMem(el, 1, 0, 0);
+    iel = sortRunAlgorithms(iel, sort);
+    g = graphCSRAssignEdgeList(g, iel, 1);
+    double t1 = omp_get_wtime();
+    fflush(stdout);
+    if (saved >= 0) { dup2(saved, 1); close(saved); }
+    if (devnull >= 0) close(devnull);
+    const uint32_t *arr[10] = {
+        g->sorted_edges_array->edges_array_src, g->sorted_edges_array->edges_array_dest,
+        g->vertices->out_degree, g->vertices->in_degree, g->vertices->edges_idx,
+        g->inverse_sorted_edges_array->edges_array_src, g->inverse_sorted_edges_array->edges_array_dest,
+        g->inverse_vertices->out_degree, g->inverse_vertices->in_degree, g->inverse_vertices->edges_idx};
+    for (int k = 0; k < 10; k++)
+        memcpy(out[k], arr[k], (size_t)(k == 0 || k == 1 || k == 5 || k == 6 ? E : N) * 4);
+    *seconds_out = t1 - t0;
+    graphCSRFree(g);
     return 0;
 }

@@ -47,7 +47,7 @@ def batch():
 
 
 def _single(g, hub, iters, eps):
-    pr = og.PageRank(g, 0, iters, eps, hub_threshold=hub)
+    pr = og.PageRank(g, 0, iters, eps, hub_threshold=hub, binned=2)
     res = pr.run()
     x = pr.ranks()
     act, err = pr.rows()

@@ -154,13 +154,15 @@ def _run(g, code, eps, iters, **kw):
 
 def test_pagerank_bitexact_vs_reference(golden, graphs):
     """Every deterministic variant, every golden case: identical rank bits,
-    iteration count and per-iteration activeVertices."""
+    iteration count and per-iteration activeVertices (fp32 pull: the gather
+    pull with every row in CSR order; the binned pull has its own test)."""
     n = 0
     for c in golden["cases"]:
         if c["code"] not in DETERMINISTIC:
             continue
         g = graphs(c["graph"], c["sort"])
-        res, ranks, act, err = _run(g, c["code"], c["epsilon"], c["max_iter"], hub_threshold=ALL_EXACT)
+        res, ranks, act, err = _run(g, c["code"], c["epsilon"], c["max_iter"], hub_threshold=ALL_EXACT,
+                                    binned=2)
         print(c["graph"], c["sort"], c["code"], c["epsilon"], res.iterations, flush=True)
         assert sha(ranks) == c["ranks_sha256"], c
         assert res.iterations == c["iterations"]
@@ -175,7 +177,8 @@ def test_data_driven_push_bitexact_on_one_chunk_graphs(golden, graphs, code):
     """Data-driven push / pull-push (pageRank.c:2566-2896) on graphbrew: a
     graph of one OpenMP chunk, where the reference is deterministic; same
     rank bits, iteration count, activeVertices and error_total per
-    iteration as its golden outputs, every sort mode."""
+    iteration as its golden outputs, sort modes 0 and 1, epsilon 1e-4 and
+    1e-6 (the four golden cases of each code)."""
     n = 0
     for c in golden["cases"]:
         if c["code"] != code:
@@ -189,7 +192,7 @@ def test_data_driven_push_bitexact_on_one_chunk_graphs(golden, graphs, code):
         assert list(act) == c["active"]
         assert np.allclose(err, c["errors"][:len(err)], rtol=1e-9, atol=1e-12)
         n += 1
-    assert n == 8
+    assert n == 4
 
 
 @pytest.mark.parametrize("code", [7, 8])
@@ -233,14 +236,14 @@ def test_integer_sum_variants_bitexact_with_hub_split(golden, graphs, code):
 
 
 def test_pull_default_plan_within_tolerance(golden, golden_ranks, graphs):
-    """fp32 pull with hub rows summed as chunk trees: max relative error
+    """Gather pull with hub rows summed as chunk trees: max relative error
     <= 1e-5 against the reference's ranks."""
     for key, ref in golden_ranks.items():
         if not key.startswith("ranks/") or "/p0/" not in key:
             continue
         _, gname, sort, p, eps, it = key.split("/")
         res, ranks, act, _ = _run(graphs(gname, int(sort[-1])), 0, float(eps[3:]), int(it[2:]),
-                                  hub_threshold=64)
+                                  hub_threshold=64, binned=2)
         rel = np.abs(ranks - ref) / np.maximum(ref, 1e-30)
         assert rel.max() <= 1e-5, (key, rel.max())
 
@@ -297,7 +300,10 @@ def test_dropin_stats_contract(golden, capfd):
     atomics in no fixed order, as the reference's omp atomic does on many
     threads, and the fp32 pull on LAW-enron sums rows above 16 in-edges as
     fixed trees: there the rows must match in count and labels, active
-    counts within 0.2 % of N, errors within 1e-5 relative."""
+    counts within 0.2 % of N.  Their error_total rows are held to what the
+    1e-5 relative bar on ranks implies: error_total = (1/N) sum |r' - r|, so
+    two runs whose ranks differ by at most 1e-5 relative differ in it by at
+    most 2e-5 times the mean unnormalized rank (the normalized rank sum)."""
     from conftest import GOLDEN, strip_times
     import ctypes
     libc = ctypes.CDLL(None)
@@ -327,13 +333,14 @@ def test_dropin_stats_contract(golden, capfd):
             assert f["error_total"] == pytest.approx(c["error_total"], rel=1e-9, abs=1e-300), key
             assert text == want, key
         else:
-            assert f["error_total"] == pytest.approx(c["error_total"], rel=1e-5, abs=1e-12), key
+            bound = 2e-5 * float(np.sum(ranks, dtype=np.float64))
+            assert abs(f["error_total"] - c["error_total"]) <= bound, key
             got_rows, want_rows = _rows(text), _rows(want)
             assert len(got_rows) == len(want_rows) > 0, key
             for (ga, gn, ge), (wa, wn, we) in zip(got_rows, want_rows):
                 assert ga == wa
                 assert abs(gn - wn) <= max(2, 2e-3 * c["num_vertices"]), (key, ga, gn, wn)
-                assert ge == pytest.approx(we, rel=1e-5, abs=1e-12), (key, ga, ge, we)
+                assert abs(ge - we) <= bound, (key, ga, ge, we)
             strip = [l for l in text.splitlines() if not l.startswith("| ") or "Page Rank" in l]
             wstrip = [l for l in want.splitlines() if not l.startswith("| ") or "Page Rank" in l]
             assert strip == wstrip, key
@@ -352,9 +359,14 @@ def rmat18():
 
 
 def test_scale18_pull_vs_oracle(rmat18):
+    """Default (binned) pull bit-identical to the oracle's sequential pull;
+    the gather pull's default split within 1e-5."""
     g, o = rmat18
-    res, ranks, act, _ = _run(g, 0, 0.0, 15)
     ref = po.pagerank(o, 0, 0.0, 15, 8)
+    res, ranks, act, _ = _run(g, 0, 0.0, 15)
+    assert np.array_equal(ranks.view(np.uint32), ref["ranks"].view(np.uint32))
+    assert res.iterations == 15 and list(act) == list(ref["active"])
+    res, ranks, act, _ = _run(g, 0, 0.0, 15, binned=2)
     rel = np.abs(ranks - ref["ranks"]) / ref["ranks"]
     assert rel.max() <= 1e-5
     assert res.iterations == 15
@@ -368,7 +380,7 @@ def test_scale18_every_row_exact_pull_bitexact(rmat18, hot):
     sequential pull, itself byte-identical to the reference on the golden
     graphs."""
     g, o = rmat18
-    res, ranks, act, _ = _run(g, 0, 1e-7, 40, hub_threshold=ALL_EXACT, hot_cache=hot)
+    res, ranks, act, _ = _run(g, 0, 1e-7, 40, hub_threshold=ALL_EXACT, hot_cache=hot, binned=2)
     ref = po.pagerank(o, 0, 1e-7, 40, 8)
     assert int(np.bincount(o["f_dst"]).max()) > 8192
     assert np.array_equal(ranks.view(np.uint32), ref["ranks"].view(np.uint32)), hot
@@ -389,11 +401,11 @@ def test_scale18_exact_above_chains(rmat18, hot, relabel):
     chains = indeg > 2048
     assert chains.sum() > 4
     ref1 = po.pagerank(o, 0, 0.0, 1, 8)["ranks"]
-    _, r1, _, _ = _run(g, 0, 0.0, 1, exact_above=2048, hot_cache=hot, relabel=relabel)
+    _, r1, _, _ = _run(g, 0, 0.0, 1, exact_above=2048, hot_cache=hot, relabel=relabel, binned=2)
     assert np.array_equal(r1[chains].view(np.uint32), ref1[chains].view(np.uint32)), (hot, relabel)
     ref = po.pagerank(o, 0, 0.0, 15, 8)["ranks"]
-    _, r15, _, _ = _run(g, 0, 0.0, 15, exact_above=2048, hot_cache=hot, relabel=relabel)
-    _, d15, _, _ = _run(g, 0, 0.0, 15, hot_cache=hot, relabel=relabel)
+    _, r15, _, _ = _run(g, 0, 0.0, 15, exact_above=2048, hot_cache=hot, relabel=relabel, binned=2)
+    _, d15, _, _ = _run(g, 0, 0.0, 15, hot_cache=hot, relabel=relabel, binned=2)
     rel = np.abs(r15.astype(np.float64) - ref) / ref
     rel_d = np.abs(d15.astype(np.float64) - ref) / ref
     assert rel.max() <= 1e-5
@@ -474,7 +486,7 @@ def test_hot_cache_same_results(graphs, name, code):
     g = graphs(name, 0)
     out = []
     for hot in (1, 2):
-        res, ranks, act, err = _run(g, code, 1e-7, 30, hot_cache=hot)
+        res, ranks, act, err = _run(g, code, 1e-7, 30, hot_cache=hot, binned=2)
         out.append((ranks, act, res.iterations))
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     assert np.array_equal(out[0][1], out[1][1]) and out[0][2] == out[1][2]
