@@ -61,8 +61,6 @@ static void *og_vec_push(og_vec *v)
     return (char *)v->p + (v->n++) * v->elem;
 }
 
-static uint32_t og_round8(uint32_t x) { return (x + 7u) & ~7u; }
-
 static int og_bits(uint64_t x)
 {
     int b = 0;
@@ -145,8 +143,9 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
     for (uint32_t q = 0; q < d->n_bins; q++)
     {
         const og_bin_desc bd = bins[q];
-        if (bd.ent0 != prev_end || (bd.ent0 & 7u) || bd.ent1 < bd.ent0 || bd.ent1 - bd.ent0 != ((bd.cnt + 7u) & ~7u) ||
-            bd.ent1 > ne - 64 || bd.cnt > OG_BIN_CAP)
+        if (bd.ent0 != prev_end || (bd.ent0 % OG_BIN_TILE_ALIGN) || bd.ent1 < bd.ent0 ||
+            (bd.ent1 - bd.ent0) % OG_BIN_TILE_ALIGN || bd.ent1 - bd.ent0 < bd.cnt || bd.ent1 > ne - 64 ||
+            bd.cnt > OG_BIN_CAP)
             OG_BIN_FAIL("bin %u entries [%u, %u) count %u", q, bd.ent0, bd.ent1, bd.cnt);
         prev_end = bd.ent1;
         if ((uint64_t)bd.pos0 + bd.cnt > pl->E) OG_BIN_FAIL("bin %u positions", q);
@@ -159,12 +158,17 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
             if (bd.row0 > bd.row1 || bd.row1 > d->N) OG_BIN_FAIL("bin %u rows", q);
             if (bd.cnt && (rp[bd.row0] != bd.pos0 || rp[bd.row1] - rp[bd.row0] != bd.cnt)) OG_BIN_FAIL("bin %u row offsets", q);
         }
+        uint32_t real = 0;
         for (uint32_t e = bd.ent0; e < bd.ent1; e++)
         {
-            const int real = e < bd.ent0 + bd.cnt;
-            if (real ? pos[e] >= bd.cnt : pos[e] != OG_BIN_PAD) OG_BIN_FAIL("entry %u: slot %u in bin %u of %u", e, pos[e], q, bd.cnt);
-            if (real && !seen[e]) OG_BIN_FAIL("entry %u never gathered", e);
+            if (pos[e] != OG_BIN_PAD)
+            {
+                if (pos[e] >= bd.cnt) OG_BIN_FAIL("entry %u: slot %u in bin %u of %u", e, pos[e], q, bd.cnt);
+                real++;
+            }
+            if (!seen[e]) OG_BIN_FAIL("entry %u never gathered", e);
         }
+        if (real != bd.cnt) OG_BIN_FAIL("bin %u: %u entries for %u in-edges", q, real, bd.cnt);
     }
     for (uint32_t q = 0; q < d->n_long; q++)
         if (lrow[q] >= pl->Z || loff[q] > loff[q + 1] || loff[q + 1] - loff[q] != rp[lrow[q] + 1] - rp[lrow[q]])
@@ -180,7 +184,7 @@ out:
 int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
 {
     *out = NULL;
-    const uint32_t N = pl->N, E = pl->E, Z = pl->Z, H = pl->H, C = pl->C;
+    const uint32_t N = pl->N, E = pl->E, Z = pl->Z, H = pl->H;
     if (pl->relabel == 1 && H)
     {
         og_set_error("binned pull: the plan sorts long rows by source (relabel 1); it needs CSR order");
@@ -194,13 +198,13 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
     og_bin_plan *b = (og_bin_plan *)calloc(1, sizeof(og_bin_plan));
     uint32_t *rp = (uint32_t *)malloc(((size_t)N + 1) * 4);
     og_vec bins = {NULL, 0, 0, sizeof(og_bin_desc)};
-    uint32_t *long_rows = NULL, *long_deg = NULL, *long_voff_rowordered = NULL, *long_off = NULL;
-    uint32_t *tseg = NULL, *tent = NULL, *tstart = NULL, *order = NULL, *seg_cnt = NULL;
-    uint32_t *tile_ent2 = NULL;
+    uint32_t *long_rows = NULL, *long_list = NULL, *long_voff_rowordered = NULL, *long_off = NULL;
+    uint32_t *tseg = NULL, *tbin = NULL, *tstart = NULL, *tcnt = NULL, *tent = NULL, *order = NULL;
+    uint32_t *seg_cnt = NULL, *tile_ent2 = NULL;
     og_vec rounds = {NULL, 0, 0, sizeof(og_bin_round)};
     og_vec items = {NULL, 0, 0, sizeof(og_bin_item)};
     uint32_t *key_a = NULL, *val_a = NULL, *key_b = NULL, *val_b = NULL;
-    void *scratch = NULL, *d_tseg = NULL, *d_tent = NULL, *d_tstart = NULL;
+    void *scratch = NULL, *d_tseg = NULL, *d_tbin = NULL, *d_tstart = NULL, *d_tent = NULL, *d_tcnt = NULL;
     hipStream_t s = og_stream();
     int rc = -1;
     if (!b || !rp)
@@ -211,22 +215,16 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
     if (og_hip_check(og_copy(rp, pl->row_ptr, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr"))
         goto done;
 
-    /* ---- bins ---- */
     {
-        /* rows [0, H2) are added by wavefronts: the plan's class of more
-         * than 512 in-edges and the start of the next class (in-degree
-         * descending) down to OG_BIN_WAVE_ROW */
-        uint32_t H2 = H;
-        while (H2 < C && rp[H2 + 1] - rp[H2] > OG_BIN_WAVE_ROW) H2++;
-        /* long rows: vlong offsets assigned longest first */
+        /* ---- long rows: vlong offsets assigned longest first ---- */
         uint32_t n_long = 0;
         for (uint32_t r = 0; r < Z; r++)
             if (rp[r + 1] - rp[r] > OG_BIN_CAP) n_long++;
         long_rows = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
-        long_deg = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        long_list = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         long_voff_rowordered = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         long_off = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
-        if (!long_rows || !long_deg || !long_voff_rowordered || !long_off)
+        if (!long_rows || !long_list || !long_voff_rowordered || !long_off)
         {
             og_set_error("out of host memory");
             goto done;
@@ -234,7 +232,6 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         uint32_t k = 0;
         for (uint32_t r = 0; r < Z; r++)
             if (rp[r + 1] - rp[r] > OG_BIN_CAP) long_rows[k++] = r;
-        /* order[] = long-row ordinals by degree descending (stable) */
         uint32_t *lord = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         if (!lord) { og_set_error("out of host memory"); goto done; }
         for (uint32_t i = 0; i < n_long; i++) lord[i] = i;
@@ -254,7 +251,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         {
             const uint32_t x = lord[i], r = long_rows[x];
             long_voff_rowordered[x] = (uint32_t)voff;
-            long_deg[i] = r;                 /* long_deg reused: rows longest first */
+            long_list[i] = r;
             long_off[i] = (uint32_t)voff;
             voff += rp[r + 1] - rp[r];
         }
@@ -262,8 +259,10 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         long_off[n_long] = (uint32_t)voff;
         b->n_vlong = voff;
 
-        uint64_t ent = 0;
-        uint32_t li = 0;   /* long rows met so far, row order */
+        /* ---- bins: consecutive rows of one kind (thread rows, wavefront
+         * rows of more than OG_BIN_WAVE_ROW in-edges) with at most
+         * OG_BIN_CAP in-edges together; pieces of the long rows ---- */
+        uint32_t li = 0;
         uint32_t r = 0;
         while (r < Z)
         {
@@ -275,42 +274,37 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
                     const uint32_t cnt = deg - p0 < OG_BIN_CAP ? deg - p0 : OG_BIN_CAP;
                     og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
                     if (!d) { og_set_error("out of host memory"); goto done; }
+                    memset(d, 0, sizeof(*d));
                     d->row0 = r;
                     d->row1 = li;
-                    d->ent0 = (uint32_t)ent;
-                    d->ent1 = (uint32_t)(ent + og_round8(cnt));
                     d->pos0 = rp[r] + p0;
                     d->kind = OG_BIN_PIECE;
                     d->vbase = long_voff_rowordered[li] + p0;
                     d->cnt = cnt;
-                    ent += og_round8(cnt);
                 }
                 li++;
                 r++;
                 continue;
             }
-            const uint32_t kind = r < H2 ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
+            const uint32_t kind = deg > OG_BIN_WAVE_ROW ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
             const uint32_t r0 = r;
             uint32_t tot = 0;
             while (r < Z)
             {
                 const uint32_t dr = rp[r + 1] - rp[r];
-                const uint32_t kr = r < H2 ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
+                const uint32_t kr = dr > OG_BIN_WAVE_ROW ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
                 if (kr != kind || dr > OG_BIN_CAP || tot + dr > OG_BIN_CAP) break;
                 tot += dr;
                 r++;
             }
             og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
             if (!d) { og_set_error("out of host memory"); goto done; }
+            memset(d, 0, sizeof(*d));
             d->row0 = r0;
             d->row1 = r;
-            d->ent0 = (uint32_t)ent;
-            d->ent1 = (uint32_t)(ent + og_round8(tot));
             d->pos0 = rp[r0];
             d->kind = kind;
-            d->vbase = 0;
             d->cnt = tot;
-            ent += og_round8(tot);
         }
         const uint32_t n_edge_bins = (uint32_t)bins.n;
         /* rows without in-edges: epilogue-only bins, last */
@@ -318,46 +312,37 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         {
             og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
             if (!d) { og_set_error("out of host memory"); goto done; }
+            memset(d, 0, sizeof(*d));
             d->row0 = r0;
             d->row1 = N - r0 < OG_BIN_ZERO_ROWS ? N : r0 + OG_BIN_ZERO_ROWS;
-            d->ent0 = d->ent1 = (uint32_t)ent;
             d->pos0 = rp[r0];
             d->kind = OG_BIN_THREAD_ROWS;
-            d->vbase = 0;
-            d->cnt = 0;
         }
         b->n_zero_bins = (uint32_t)bins.n - n_edge_bins;
         const uint32_t n_seg = (N + OG_BIN_SEG - 1) / OG_BIN_SEG;
-        if (ent + 64 > 0xFFFFFFFFull || (uint64_t)n_edge_bins * n_seg > 0xFFFFFFFFull || voff > 0xFFFFFFFFull)
+        if ((uint64_t)n_edge_bins * n_seg > 0xFFFFFFFFull || voff > 0xFFFFFFFFull)
         {
-            og_set_error("binned pull: graph too large for 32-bit entry and tile keys");
+            og_set_error("binned pull: graph too large for 32-bit tile keys");
             goto done;
         }
-        b->n_ent = ent + 64;   /* 64 slack entries for the rounds past a segment's last tile */
         b->d.N = N;
         b->d.n_seg = n_seg;
         b->d.n_bins = (uint32_t)bins.n;
         b->d.n_bins_live = (uint32_t)bins.n;
         b->d.n_long = n_long;
 
-        /* ---- device: keys, sort, entries, tiles ---- */
+        /* ---- device: (bin, segment) keys of the CSR positions, stable sort,
+         * tile starts ---- */
+        og_bin_desc *hb = (og_bin_desc *)bins.p;
         b->mem_bins = og_bin_upload(bins.p, bins.n * sizeof(og_bin_desc));
-        b->mem_src = og_dalloc(b->n_ent * 2);
-        b->mem_pos = og_dalloc(b->n_ent * 2);
-        b->mem_val = og_dalloc(b->n_ent * 4);
-        b->mem_long_row = og_bin_upload(long_deg, (size_t)n_long * 4);
-        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
-        b->mem_vlong = og_dalloc(voff * 4);
         key_a = og_dalloc(((size_t)E + 1) * 4);
         val_a = og_dalloc(((size_t)E + 1) * 4);
         key_b = og_dalloc(((size_t)E + 1) * 4);
         val_b = og_dalloc(((size_t)E + 1) * 4);
         size_t sb = og_radix_scratch_bytes(E), sc2 = og_scan_scratch_bytes(E);
         scratch = og_dalloc(sb > sc2 ? sb : sc2);
-        if (!b->mem_bins || !b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off ||
-            !b->mem_vlong || !key_a || !val_a || !key_b || !val_b || !scratch)
-            goto done;
-        const og_bin_desc *d_bins = (const og_bin_desc *)b->mem_bins;
+        if (!b->mem_bins || !key_a || !val_a || !key_b || !val_b || !scratch) goto done;
+        og_bin_desc *d_bins = (og_bin_desc *)b->mem_bins;
         if (og_launch_bin_keys(d_bins, n_edge_bins, pl->cols, n_seg, key_b, val_b, s)) goto done;
         int in_b = 0;
         const uint32_t kbits = (uint32_t)og_bits((uint64_t)n_edge_bins * n_seg - 1);
@@ -365,44 +350,83 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             goto done;
         uint32_t *K = in_b ? key_b : key_a, *V = in_b ? val_b : val_a;
         uint32_t *flag = in_b ? key_a : key_b, *fscan = in_b ? val_a : val_b;
-        if (og_launch_bin_fill(K, V, E, d_bins, n_seg, pl->cols, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos,
-                               flag, s) ||
-            og_scan_exclusive(flag, fscan, E, scratch, s) ||
-            og_launch_bin_pads(d_bins, n_edge_bins, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
-            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack"))
+        if (og_launch_bin_flags(K, E, d_bins, n_seg, flag, s) || og_scan_exclusive(flag, fscan, E, scratch, s))
             goto done;
         uint32_t n_tiles = 0;
         if (og_hip_check(og_copy(&n_tiles, fscan + E, 4, hipMemcpyDeviceToHost), "D2H tiles")) goto done;
         b->n_tiles = n_tiles;
         d_tseg = og_dalloc((size_t)n_tiles * 4);
-        d_tent = og_dalloc((size_t)n_tiles * 4);
+        d_tbin = og_dalloc((size_t)n_tiles * 4);
         d_tstart = og_dalloc((size_t)n_tiles * 4);
-        if (!d_tseg || !d_tent || !d_tstart) goto done;
-        if (og_launch_bin_tiles(K, E, d_bins, n_seg, fscan, (uint32_t *)d_tseg, (uint32_t *)d_tent,
-                                (uint32_t *)d_tstart, s))
+        if (!d_tseg || !d_tbin || !d_tstart) goto done;
+        if (og_launch_bin_tiles(K, E, n_seg, fscan, (uint32_t *)d_tseg, (uint32_t *)d_tbin, (uint32_t *)d_tstart, s))
             goto done;
         tseg = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
-        tent = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        tbin = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
         tstart = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        tcnt = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        tent = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
         order = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
         seg_cnt = (uint32_t *)calloc((size_t)n_seg + 1, 4);
         tile_ent2 = (uint32_t *)malloc(((size_t)n_tiles + n_seg) * 4 + 4);
-        if (!tseg || !tent || !tstart || !order || !seg_cnt || !tile_ent2)
+        if (!tseg || !tbin || !tstart || !tcnt || !tent || !order || !seg_cnt || !tile_ent2)
         {
             og_set_error("out of host memory");
             goto done;
         }
         if (og_hip_check(og_copy(tseg, d_tseg, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile seg") ||
-            og_hip_check(og_copy(tent, d_tent, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile ent") ||
+            og_hip_check(og_copy(tbin, d_tbin, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile bin") ||
             og_hip_check(og_copy(tstart, d_tstart, (size_t)n_tiles * 4, hipMemcpyDeviceToHost), "D2H tile start"))
+            goto done;
+
+        /* ---- host: tile sizes, their padded entry ranges, bins' entries ---- */
+        uint64_t ent = 0;
+        uint32_t bprev = 0xFFFFFFFFu;
+        for (uint32_t t = 0; t < n_tiles; t++)
+        {
+            tcnt[t] = (t + 1 < n_tiles ? tstart[t + 1] : E) - tstart[t];
+            if (tbin[t] != bprev)
+            {
+                if (bprev != 0xFFFFFFFFu) hb[bprev].ent1 = (uint32_t)ent;
+                bprev = tbin[t];
+                hb[bprev].ent0 = (uint32_t)ent;
+            }
+            tent[t] = (uint32_t)ent;
+            ent += (tcnt[t] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
+            if (ent + 64 > 0xFFFFFFFFull)
+            {
+                og_set_error("binned pull: graph too large for 32-bit entries");
+                goto done;
+            }
+        }
+        if (bprev != 0xFFFFFFFFu) hb[bprev].ent1 = (uint32_t)ent;
+        for (uint32_t q = n_edge_bins; q < bins.n; q++) hb[q].ent0 = hb[q].ent1 = (uint32_t)ent;
+        b->n_ent = ent + 64;   /* 64 slack entries for the rounds past a segment's last tile */
+        b->mem_src = og_dalloc(b->n_ent * 2);
+        b->mem_pos = og_dalloc(b->n_ent * 2);
+        b->mem_val = og_dalloc(b->n_ent * 4);
+        b->mem_long_row = og_bin_upload(long_list, (size_t)n_long * 4);
+        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
+        b->mem_vlong = og_dalloc(voff * 4);
+        d_tent = og_bin_upload(tent, (size_t)n_tiles * 4);
+        d_tcnt = og_bin_upload(tcnt, (size_t)n_tiles * 4);
+        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off || !b->mem_vlong ||
+            !d_tent || !d_tcnt ||
+            og_hip_check(og_copy(d_bins, hb, bins.n * sizeof(og_bin_desc), hipMemcpyHostToDevice), "H2D bins"))
+            goto done;
+        if (og_launch_bin_fill(K, V, E, d_bins, n_seg, pl->cols, fscan, (const uint32_t *)d_tstart,
+                               (const uint32_t *)d_tent, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
+            og_launch_bin_pads((const uint32_t *)d_tent, (const uint32_t *)d_tcnt, n_tiles, (uint16_t *)b->mem_src,
+                               (uint16_t *)b->mem_pos, s) ||
+            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack") ||
+            og_hip_check(hipStreamSynchronize(s), "sync bin layout"))
             goto done;
         og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
         key_a = val_a = key_b = val_b = NULL;
         scratch = NULL;
 
-        /* ---- host: tiles segment by segment (bins ascending), rounds, items ---- */
-        /* tile sizes (tiles are in sorted order: each ends where the next starts) */
-        for (uint32_t t = 0; t < n_tiles; t++) tstart[t] = (t + 1 < n_tiles ? tstart[t + 1] : E) - tstart[t];
+        /* ---- host: tiles segment by segment (bins ascending), walked in
+         * rounds of 64 entries (padding included), work items ---- */
         for (uint32_t t = 0; t < n_tiles; t++) seg_cnt[tseg[t] + 1]++;
         for (uint32_t q = 0; q < n_seg; q++) seg_cnt[q + 1] += seg_cnt[q];
         {
@@ -424,35 +448,36 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             {
                 const uint32_t t = order[t0 + i];
                 tile_ent2[nt2++] = tent[t];
-                tot += tstart[t];
+                tcnt[t] = (tcnt[t] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
+                tot += tcnt[t];
             }
             tile_ent2[nt2++] = slack;                  /* the segment's end tile */
             const uint32_t R = (uint32_t)((tot + 63) / 64);
             const uint32_t r_first = (uint32_t)rounds.n;
-            uint32_t k = 0;       /* tile of lane 0 */
+            uint32_t kt = 0;      /* tile of lane 0 */
             uint64_t Fk = 0;      /* its first flat position */
             for (uint32_t rr = 0; rr < R; rr++)
             {
                 const uint64_t f0 = 64ull * rr;
-                while (k < m && Fk + tstart[order[t0 + k]] <= f0)
+                while (kt < m && Fk + tcnt[order[t0 + kt]] <= f0)
                 {
-                    Fk += tstart[order[t0 + k]];
-                    k++;
+                    Fk += tcnt[order[t0 + kt]];
+                    kt++;
                 }
                 og_bin_round *rd = (og_bin_round *)og_vec_push(&rounds);
                 if (!rd) { og_set_error("out of host memory"); goto done; }
                 uint64_t mask = 0;
-                uint32_t kk = k;
+                uint32_t kk = kt;
                 uint64_t Fkk = Fk;
                 while (kk < m)
                 {
-                    const uint64_t nxt = Fkk + tstart[order[t0 + kk]];
+                    const uint64_t nxt = Fkk + tcnt[order[t0 + kk]];
                     if (nxt >= f0 + 64) break;
                     mask |= 1ull << (nxt - f0);
                     Fkk = nxt;
                     kk++;
                 }
-                rd->first = base + k;
+                rd->first = base + kt;
                 rd->off0 = (uint32_t)(f0 - Fk);
                 rd->mask_lo = (uint32_t)mask;
                 rd->mask_hi = (uint32_t)(mask >> 32);
@@ -489,9 +514,9 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
 done:
     (void)hipStreamSynchronize(s);
     og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
-    og_dfree(d_tseg); og_dfree(d_tent); og_dfree(d_tstart);
-    free(rp); free(bins.p); free(long_rows); free(long_deg); free(long_voff_rowordered); free(long_off);
-    free(tseg); free(tent); free(tstart); free(order); free(seg_cnt); free(tile_ent2);
+    og_dfree(d_tseg); og_dfree(d_tbin); og_dfree(d_tstart); og_dfree(d_tent); og_dfree(d_tcnt);
+    free(rp); free(bins.p); free(long_rows); free(long_list); free(long_voff_rowordered); free(long_off);
+    free(tseg); free(tbin); free(tstart); free(tcnt); free(tent); free(order); free(seg_cnt); free(tile_ent2);
     free(rounds.p); free(items.p);
     if (rc) og_bin_plan_destroy(b);
     else *out = b;

@@ -10,7 +10,9 @@
  * in segment order, within a tile in CSR order:
  *   src[e]  u16  source id minus the segment's first id
  *   pos[e]  u16  the edge's CSR position minus the bin's first position
- *               (OG_BIN_PAD: a padding entry; bins are padded to 8 entries)
+ *               (OG_BIN_PAD: a padding entry; each tile is padded to
+ *               OG_BIN_TILE_ALIGN entries, so every tile's values fill whole
+ *               64-byte sectors, written by one block)
  *   val[e]  f32  the gathered contribution, written each iteration
  * One iteration:
  *   gather pass  one 1024-thread block per CU: per work item (a segment and
@@ -48,7 +50,8 @@ extern "C" {
 #define OG_BIN_SEG 39936u      /* sources per segment: 156 KB of fp32 in LDS */
 #define OG_BIN_CAP 39936u      /* in-edge values per bin: 156 KB of LDS */
 #define OG_BIN_PAD 0xFFFFu     /* pos of a padding entry */
-#define OG_BIN_WAVE_ROW 128u   /* rows with more in-edges are added by a wavefront */
+#define OG_BIN_WAVE_ROW 4096u  /* rows with more in-edges are added by a wavefront */
+#define OG_BIN_TILE_ALIGN 16u  /* tiles padded to 16 entries: 64-byte value runs */
 #define OG_BIN_ITEM_ROUNDS 2048u /* rounds (64 entries) per gather work item at most */
 #define OG_BIN_THREADS 1024u
 
@@ -121,13 +124,22 @@ int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint32_t it, uin
 /* plan-build kernels */
 int og_launch_bin_keys(const og_bin_desc *bins, uint32_t n_bins, const uint32_t *cols, uint32_t n_seg,
                        uint32_t *key, uint32_t *val, hipStream_t s);
+/* tile_flag[i] = 1 where sorted entry i starts a tile (a bin's first entry
+ * or a change of key) */
+int og_launch_bin_flags(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
+                        uint32_t *tile_flag, hipStream_t s);
+/* per tile t (from the exclusive scan of the flags): segment, bin, first
+ * sorted entry */
+int og_launch_bin_tiles(const uint32_t *key, uint32_t n, uint32_t n_seg, const uint32_t *tile_flag_scan,
+                        uint32_t *tile_seg, uint32_t *tile_bin, uint32_t *tile_start, hipStream_t s);
+/* entry of sorted i: tile_ent[t] + (i - tile_start[t]), t its tile; src and
+ * pos there; then the padding entries of every tile (src 0, pos PAD) */
 int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
-                       uint32_t n_seg, const uint32_t *cols, uint16_t *src, uint16_t *pos, uint32_t *tile_flag,
+                       uint32_t n_seg, const uint32_t *cols, const uint32_t *tile_flag_scan,
+                       const uint32_t *tile_start, const uint32_t *tile_ent, uint16_t *src, uint16_t *pos,
                        hipStream_t s);
-int og_launch_bin_tiles(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
-                        const uint32_t *tile_flag_scan, uint32_t *tile_seg, uint32_t *tile_ent,
-                        uint32_t *tile_start, hipStream_t s);
-int og_launch_bin_pads(const og_bin_desc *bins, uint32_t n_bins, uint16_t *src, uint16_t *pos, hipStream_t s);
+int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_cnt, uint32_t n_tiles, uint16_t *src,
+                       uint16_t *pos, hipStream_t s);
 
 #ifdef __cplusplus
 }

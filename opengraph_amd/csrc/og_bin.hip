@@ -180,17 +180,43 @@ __device__ __forceinline__ void og_bin_stat_init(og_bin_stat &s)
     s.active = 0;
 }
 
-/* pageRank.c:1016-1028 and the next contribution (pageRank.c:989-996). */
-__device__ __forceinline__ void og_bin_epilogue(const og_pr_dev &a, uint32_t v, float sum, og_bin_stat &st)
+/* pageRank.c:1016-1028 and the next contribution (pageRank.c:989-996),
+ * with the old rank and out-degree already loaded. */
+__device__ __forceinline__ void og_bin_epilogue_pf(const og_pr_dev &a, uint32_t v, float sum, float prev,
+                                                   uint32_t deg, og_bin_stat &st)
 {
-    const float prev = __builtin_nontemporal_load(&a.r[v]);
-    const uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
     const float nr = OG_BASE_PR + (OG_DAMP * sum);
     __builtin_nontemporal_store(nr, &a.r[v]);
     const double err = fabs((double)(nr - prev));
     st.err += err / (double)a.N;
     st.active += err >= a.epsilon;
     __builtin_nontemporal_store(deg ? nr / (float)deg : 0.0f, &((float *)a.c_out)[v]);
+}
+
+__device__ __forceinline__ void og_bin_epilogue(const og_pr_dev &a, uint32_t v, float sum, og_bin_stat &st)
+{
+    const float prev = __builtin_nontemporal_load(&a.r[v]);
+    const uint32_t deg = __builtin_nontemporal_load(&a.outdeg[v]);
+    og_bin_epilogue_pf(a, v, sum, prev, deg, st);
+}
+
+/* What a thread row needs besides its values: its slot range (from the
+ * row offsets), old rank and out-degree; loaded one row ahead. */
+struct og_bin_row
+{
+    uint32_t p0, p1, deg;
+    float prev;
+};
+
+__device__ __forceinline__ void og_bin_row_load(const og_pr_dev &a, uint32_t v, uint32_t end, og_bin_row &m)
+{
+    if (v < end)
+    {
+        m.p0 = a.row_ptr[v];
+        m.p1 = a.row_ptr[v + 1];
+        m.prev = __builtin_nontemporal_load(&a.r[v]);
+        m.deg = __builtin_nontemporal_load(&a.outdeg[v]);
+    }
 }
 
 /* og_stat over the block (fixed order); result in thread 0 */
@@ -264,24 +290,33 @@ __device__ __forceinline__ void og_bin_load(const og_bin_dev &b, const og_bin_de
     }
 }
 
-__device__ __forceinline__ void og_bin_put(float *slot, const og_bin_desc &d, const og_bin_regs &r)
+/* 8 entries' values to their slots (padding entries skipped) */
+__device__ __forceinline__ void og_bin_scatter8(float *slot, og_u32x4 p, og_f32x4 x0, og_f32x4 x1)
+{
+    const uint32_t pp[8] = {p.x & 0xFFFFu, p.x >> 16, p.y & 0xFFFFu, p.y >> 16,
+                            p.z & 0xFFFFu, p.z >> 16, p.w & 0xFFFFu, p.w >> 16};
+    const float xx[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (uint32_t t = 0; t < 8; t++)
+        if (pp[t] != OG_BIN_PAD) slot[pp[t]] = xx[t];
+}
+
+/* The bin's entries: the registers' sweeps, then (a bin whose tiles'
+ * padding takes it past OG_BIN_SWEEPS sweeps: up to 15 entries per tile)
+ * the rest straight from memory. */
+__device__ __forceinline__ void og_bin_put(const og_bin_dev &b, float *slot, const og_bin_desc &d,
+                                           const og_bin_regs &r)
 {
 #pragma unroll
     for (uint32_t k = 0; k < OG_BIN_SWEEPS; k++)
     {
         const uint32_t e = d.ent0 + 8 * (threadIdx.x + k * BT);
-        if (e < d.ent1)
-        {
-            const og_u32x4 p = r.p[k];
-            const uint32_t pp[8] = {p.x & 0xFFFFu, p.x >> 16, p.y & 0xFFFFu, p.y >> 16,
-                                    p.z & 0xFFFFu, p.z >> 16, p.w & 0xFFFFu, p.w >> 16};
-            const float xx[8] = {r.x0[k].x, r.x0[k].y, r.x0[k].z, r.x0[k].w,
-                                 r.x1[k].x, r.x1[k].y, r.x1[k].z, r.x1[k].w};
-#pragma unroll
-            for (uint32_t t = 0; t < 8; t++)
-                if (pp[t] != OG_BIN_PAD) slot[pp[t]] = xx[t];
-        }
+        if (e < d.ent1) og_bin_scatter8(slot, r.p[k], r.x0[k], r.x1[k]);
     }
+    for (uint32_t e = d.ent0 + 8 * (threadIdx.x + OG_BIN_SWEEPS * BT); e < d.ent1; e += 8 * BT)
+        og_bin_scatter8(slot, __builtin_nontemporal_load((const og_u32x4 *)(b.pos + e)),
+                        __builtin_nontemporal_load((const og_f32x4 *)(b.val + e)),
+                        __builtin_nontemporal_load((const og_f32x4 *)(b.val + e + 4)));
 }
 
 __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev a)
@@ -295,29 +330,36 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
     uint32_t q = blockIdx.x;
     og_bin_desc d;
     og_bin_regs regs;
+    og_bin_row nxt_row;
     if (q < b.n_bins_live)
     {
         d = b.bins[q];
         og_bin_load(b, d, regs);
+        if (d.kind == OG_BIN_THREAD_ROWS) og_bin_row_load(a, d.row0 + threadIdx.x, d.row1, nxt_row);
     }
     for (; q < b.n_bins_live; q += gridDim.x)
     {
         __syncthreads();   /* the previous bin's reads of slot are done */
         /* this bin's values to their CSR slots (entries padded to 8 per bin) */
-        og_bin_put(slot, d, regs);
+        og_bin_put(b, slot, d, regs);
         const og_bin_desc cd = d;
-        /* the next bin's entries load while this one's rows are added */
+        og_bin_row cur_row = nxt_row;
+        /* the next bin's entries (and its first thread row) load while this
+         * one's rows are added */
         if (q + gridDim.x < b.n_bins_live)
         {
             d = b.bins[q + gridDim.x];
             og_bin_load(b, d, regs);
+            if (d.kind == OG_BIN_THREAD_ROWS) og_bin_row_load(a, d.row0 + threadIdx.x, d.row1, nxt_row);
         }
         __syncthreads();
         if (cd.kind == OG_BIN_THREAD_ROWS)
         {
             for (uint32_t v = cd.row0 + threadIdx.x; v < cd.row1; v += BT)
             {
-                const uint32_t j0 = a.row_ptr[v] - cd.pos0, j1 = a.row_ptr[v + 1] - cd.pos0;
+                const og_bin_row m = cur_row;
+                og_bin_row_load(a, v + BT, cd.row1, cur_row);   /* the thread's next row */
+                const uint32_t j0 = m.p0 - cd.pos0, j1 = m.p1 - cd.pos0;
                 float s = 0.0f;
                 uint32_t j = j0;
                 for (; j + 4 <= j1; j += 4)
@@ -329,7 +371,7 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
                     s = s + t3;
                 }
                 for (; j < j1; j++) s = s + slot[j];
-                og_bin_epilogue(a, v, s, st);
+                og_bin_epilogue_pf(a, v, s, m.prev, m.deg, st);
             }
         }
         else if (cd.kind == OG_BIN_WAVE_ROWS)
@@ -360,9 +402,11 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
 /* One block per long row (longest first, rows q = b, b + grid, ...): the
  * row's values from vlong in CSR order, staged through LDS in chunks of
  * OG_BIN_LCH (the next chunk's loads in flight while one is added), added
- * with the block-wide exact fast-forward, up to 1024 terms per step. */
-#define OG_BIN_LCH 8192u
+ * with the block-wide exact fast-forward, 4 consecutive terms per thread
+ * (up to 4096 terms) per step. */
+#define OG_BIN_LCH 16384u
 #define OG_BIN_LPT (OG_BIN_LCH / BT)   /* values per thread and chunk */
+#define OG_BIN_LTT 4u                  /* terms per thread and step */
 
 __device__ __forceinline__ void og_bin_long_fetch(const float *x, uint32_t n, uint32_t base, float (&r)[OG_BIN_LPT])
 {
@@ -374,17 +418,36 @@ __device__ __forceinline__ void og_bin_long_fetch(const float *x, uint32_t n, ui
     }
 }
 
+/* sum (saturated at 2^25) or minimum over the 16 wavefronts' values, read
+ * by every lane */
+__device__ __forceinline__ uint32_t og_bin_waves_sum(const uint32_t *v)
+{
+    uint32_t x = v[threadIdx.x & (BW - 1)];
+#pragma unroll
+    for (uint32_t o = 1; o < BW; o <<= 1) x = min(x + __shfl_xor(x, o, 64), 1u << 25);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t og_bin_waves_min(const uint32_t *v)
+{
+    uint32_t x = v[threadIdx.x & (BW - 1)];
+#pragma unroll
+    for (uint32_t o = 1; o < BW; o <<= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
 __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
 {
     __shared__ float buf[2][OG_BIN_LCH];
-    __shared__ uint32_t s_wsum[BW];
-    __shared__ uint32_t s_wfail[BW];
+    __shared__ uint32_t s_wsum[2][BW];
+    __shared__ uint32_t s_wfail[2][BW];
     __shared__ uint32_t s_pre;
     if (*a.done) return;
     og_bin_stat st;
     og_bin_stat_init(st);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
+    uint32_t par = 0;    /* which s_wsum / s_wfail set this step uses */
     for (uint32_t q = blockIdx.x; q < b.n_long; q += gridDim.x)
     {
         const uint32_t v = b.long_row[q];
@@ -413,50 +476,77 @@ __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
                     p += 1;
                     continue;
                 }
-                const bool valid = threadIdx.x < left;
-                const float a_t = valid ? xs[p + threadIdx.x] : 0.0f;
                 int ey;
                 frexpf(y, &ey);
                 const int sc = 24 - ey;
-                uint32_t m;
-                bool bad;
-                og_bin_term(a_t, sc, valid, m, bad);
-                const uint32_t incl_w = og_bin_wave_scan(m);   /* < 2^30 */
-                __syncthreads();                               /* previous step's reads of s_* done */
-                if (lane == 63) s_wsum[wave] = min(incl_w, 1u << 25);
+                const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
+                /* this thread's terms p + 4t .. p + 4t + 3 */
+                uint32_t m[OG_BIN_LTT], l[OG_BIN_LTT];
+                bool bad[OG_BIN_LTT], valid[OG_BIN_LTT];
+                uint32_t tot = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < OG_BIN_LTT; k++)
+                {
+                    const uint32_t i = OG_BIN_LTT * threadIdx.x + k;
+                    valid[k] = i < left;
+                    og_bin_term(valid[k] ? xs[p + i] : 0.0f, sc, valid[k], m[k], bad[k]);
+                    tot = min(tot + m[k], 1u << 25);
+                    l[k] = tot;
+                }
+                const uint32_t incl_w = og_bin_wave_scan(tot);          /* <= 2^31 */
+                if (lane == 63) s_wsum[par][wave] = min(incl_w, 1u << 25);
                 __syncthreads();
                 uint32_t base = 0;
-                for (uint32_t w = 0; w < wave; w++) base += s_wsum[w];   /* <= 2^29 */
-                const uint32_t Y = (uint32_t)ldexpf(y, sc);
-                const uint32_t incl = base + incl_w;
-                const bool fail = valid && (bad || Y + incl >= (1u << 24));
-                const uint64_t fb = __ballot(fail);
-                if (lane == 0) s_wfail[wave] = fb ? (uint32_t)__builtin_ctzll(fb) : 64u;
-                __syncthreads();
-                uint32_t kf = 0xFFFFFFFFu;
-                for (uint32_t w = 0; w < BW; w++)
-                    if (s_wfail[w] < 64u)
+                {
+                    /* prefix of the earlier wavefronts' sums */
+                    uint32_t w = s_wsum[par][lane & (BW - 1)];
+                    w = lane < wave ? w : 0u;
+#pragma unroll
+                    for (uint32_t o = 1; o < BW; o <<= 1) w = min(w + __shfl_xor(w, o, 64), 1u << 25);
+                    base = (uint32_t)__builtin_amdgcn_readfirstlane(w);   /* lanes 0..15 hold the sum */
+                }
+                const uint32_t X = min(base + incl_w - tot, 1u << 25);   /* before this thread's terms */
+                uint32_t kfail = OG_BIN_LTT;
+#pragma unroll
+                for (int k = (int)OG_BIN_LTT - 1; k >= 0; k--)
+                    if (valid[k] && (bad[k] || Y + X + l[k] >= (1u << 24))) kfail = (uint32_t)k;
+                const uint64_t fb = __ballot(kfail < OG_BIN_LTT);
+                if (lane == 0)
+                {
+                    uint32_t idx = 0xFFFFFFFFu;
+                    if (fb)
                     {
-                        kf = w * 64u + s_wfail[w];
-                        break;
+                        const uint32_t fl = (uint32_t)__builtin_ctzll(fb);
+                        idx = OG_BIN_LTT * (64u * wave + fl) + (uint32_t)__builtin_amdgcn_readlane(kfail, fl);
                     }
+                    s_wfail[par][wave] = idx;
+                }
+                __syncthreads();
+                const uint32_t kf = og_bin_waves_min(s_wfail[par]);
                 if (kf == 0xFFFFFFFFu)
                 {
-                    const uint32_t cntv = left < BT ? left : BT;
-                    if (threadIdx.x == cntv - 1) s_pre = incl;
-                    __syncthreads();
-                    y = ldexpf((float)(Y + s_pre), -sc);
-                    p += cntv;
+                    const uint32_t total = og_bin_waves_sum(s_wsum[par]);
+                    y = ldexpf((float)(Y + total), -sc);
+                    p += left < OG_BIN_LTT * BT ? left : OG_BIN_LTT * BT;
                 }
                 else
                 {
-                    if (kf > 0 && threadIdx.x == kf - 1) s_pre = incl;
+                    const uint32_t t = kf / OG_BIN_LTT, kk = kf % OG_BIN_LTT;
+                    if (threadIdx.x == t)
+                    {
+                        uint32_t pre = X;
+#pragma unroll
+                        for (uint32_t k = 0; k < OG_BIN_LTT; k++)
+                            if (k + 1 == kk) pre = X + l[k];
+                        s_pre = pre;
+                    }
                     __syncthreads();
-                    const uint32_t pre = kf ? s_pre : 0u;
-                    y = ldexpf((float)(Y + pre), -sc);
+                    y = ldexpf((float)(Y + s_pre), -sc);
                     y = y + xs[p + kf];
                     p += kf + 1;
+                    __syncthreads();   /* s_pre read by all before its next write */
                 }
+                par ^= 1;
             }
             /* the next chunk (already loaded) into the other buffer, and the
              * loads of the one after it */
@@ -485,13 +575,13 @@ extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint3
 {
     if (ev0) hipEventRecord(ev0, s);
     if (b->n_items)
-        hipLaunchKernelGGL(og_bin_gather_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, (const float *)a->g_in,
+        OG_LAUNCH(og_bin_gather_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, (const float *)a->g_in,
                            (const uint32_t *)a->done);
-    hipLaunchKernelGGL(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
+    OG_LAUNCH(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
     if (long_grid)
-        hipLaunchKernelGGL(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, s, *b, *a, sum_grid);
+        OG_LAUNCH(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, s, *b, *a, sum_grid);
     if (ev1) hipEventRecord(ev1, s);
-    return og_hip_check(hipGetLastError(), "og_bin_pull");
+    return og_hip_check(og_launch_status(), "og_bin_pull");
 }
 
 /* ------------------------------------------------------------------ */
@@ -512,29 +602,20 @@ __global__ void og_bin_keys_kernel(const og_bin_desc *__restrict__ bins, const u
     }
 }
 
-/* Entries of the sorted keys: src, pos at entry ent0 + (i - pos0) of bin
- * key / n_seg; flag the first entry of each tile. */
-__global__ void og_bin_fill_kernel(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val, uint32_t n,
-                                   const og_bin_desc *__restrict__ bins, uint32_t n_seg,
-                                   const uint32_t *__restrict__ cols, uint16_t *__restrict__ src,
-                                   uint16_t *__restrict__ pos, uint32_t *__restrict__ tile_flag)
+__global__ void og_bin_flags_kernel(const uint32_t *__restrict__ key, uint32_t n,
+                                    const og_bin_desc *__restrict__ bins, uint32_t n_seg,
+                                    uint32_t *__restrict__ tile_flag)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        const uint32_t k = key[i], j = val[i];
-        const uint32_t bi = k / n_seg, s = k - bi * n_seg;
-        const og_bin_desc d = bins[bi];
-        const uint32_t e = d.ent0 + (i - d.pos0);
-        src[e] = (uint16_t)(cols[j] - s * OG_BIN_SEG);
-        pos[e] = (uint16_t)(j - d.pos0);
-        tile_flag[i] = (i == d.pos0 || key[i - 1] != k) ? 1u : 0u;
+        const uint32_t k = key[i];
+        tile_flag[i] = (i == 0 || key[i - 1] != k || i == bins[k / n_seg].pos0) ? 1u : 0u;
     }
 }
 
-__global__ void og_bin_tiles_kernel(const uint32_t *__restrict__ key, uint32_t n,
-                                    const og_bin_desc *__restrict__ bins, uint32_t n_seg,
+__global__ void og_bin_tiles_kernel(const uint32_t *__restrict__ key, uint32_t n, uint32_t n_seg,
                                     const uint32_t *__restrict__ scan, uint32_t *__restrict__ tile_seg,
-                                    uint32_t *__restrict__ tile_ent, uint32_t *__restrict__ tile_start)
+                                    uint32_t *__restrict__ tile_bin, uint32_t *__restrict__ tile_start)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -542,21 +623,41 @@ __global__ void og_bin_tiles_kernel(const uint32_t *__restrict__ key, uint32_t n
         const uint32_t t = scan[i];
         const uint32_t k = key[i];
         const uint32_t bi = k / n_seg;
-        const og_bin_desc d = bins[bi];
         tile_seg[t] = k - bi * n_seg;
-        tile_ent[t] = d.ent0 + (i - d.pos0);
+        tile_bin[t] = bi;
         tile_start[t] = i;
     }
 }
 
-__global__ void og_bin_pads_kernel(const og_bin_desc *__restrict__ bins, uint16_t *__restrict__ src,
-                                   uint16_t *__restrict__ pos)
+__global__ void og_bin_fill_kernel(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val, uint32_t n,
+                                   const og_bin_desc *__restrict__ bins, uint32_t n_seg,
+                                   const uint32_t *__restrict__ cols, const uint32_t *__restrict__ scan,
+                                   const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ tile_ent,
+                                   uint16_t *__restrict__ src, uint16_t *__restrict__ pos)
 {
-    const og_bin_desc d = bins[blockIdx.x];
-    for (uint32_t e = d.ent0 + d.cnt + threadIdx.x; e < d.ent1; e += blockDim.x)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        src[e] = 0;
-        pos[e] = (uint16_t)OG_BIN_PAD;
+        const uint32_t k = key[i], j = val[i];
+        const uint32_t bi = k / n_seg, s = k - bi * n_seg;
+        const uint32_t t = scan[i + 1] - 1;          /* the tile holding sorted entry i */
+        const uint32_t e = tile_ent[t] + (i - tile_start[t]);
+        src[e] = (uint16_t)(cols[j] - s * OG_BIN_SEG);
+        pos[e] = (uint16_t)(j - bins[bi].pos0);
+    }
+}
+
+__global__ void og_bin_pads_kernel(const uint32_t *__restrict__ tile_ent, const uint32_t *__restrict__ tile_cnt,
+                                   uint32_t n_tiles, uint16_t *__restrict__ src, uint16_t *__restrict__ pos)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles; t += gridDim.x * blockDim.x)
+    {
+        const uint32_t c = tile_cnt[t];
+        const uint32_t pad = (OG_BIN_TILE_ALIGN - c % OG_BIN_TILE_ALIGN) % OG_BIN_TILE_ALIGN;
+        for (uint32_t k = 0; k < pad; k++)
+        {
+            src[tile_ent[t] + c + k] = 0;
+            pos[tile_ent[t] + c + k] = (uint16_t)OG_BIN_PAD;
+        }
     }
 }
 
@@ -571,34 +672,43 @@ extern "C" int og_launch_bin_keys(const og_bin_desc *bins, uint32_t n_bins, cons
                                   uint32_t *key, uint32_t *val, hipStream_t s)
 {
     if (!n_bins) return 0;
-    hipLaunchKernelGGL(og_bin_keys_kernel, dim3(n_bins), dim3(256), 0, s, bins, cols, n_seg, key, val);
-    return og_hip_check(hipGetLastError(), "og_bin_keys_kernel");
+    OG_LAUNCH(og_bin_keys_kernel, dim3(n_bins), dim3(256), 0, s, bins, cols, n_seg, key, val);
+    return og_hip_check(og_launch_status(), "og_bin_keys_kernel");
+}
+
+extern "C" int og_launch_bin_flags(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
+                                   uint32_t *tile_flag, hipStream_t s)
+{
+    if (!n) return 0;
+    OG_LAUNCH(og_bin_flags_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, n, bins, n_seg, tile_flag);
+    return og_hip_check(og_launch_status(), "og_bin_flags_kernel");
+}
+
+extern "C" int og_launch_bin_tiles(const uint32_t *key, uint32_t n, uint32_t n_seg, const uint32_t *tile_flag_scan,
+                                   uint32_t *tile_seg, uint32_t *tile_bin, uint32_t *tile_start, hipStream_t s)
+{
+    if (!n) return 0;
+    OG_LAUNCH(og_bin_tiles_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, n, n_seg, tile_flag_scan,
+                       tile_seg, tile_bin, tile_start);
+    return og_hip_check(og_launch_status(), "og_bin_tiles_kernel");
 }
 
 extern "C" int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
-                                  uint32_t n_seg, const uint32_t *cols, uint16_t *src, uint16_t *pos,
-                                  uint32_t *tile_flag, hipStream_t s)
-{
-    if (!n) return 0;
-    hipLaunchKernelGGL(og_bin_fill_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, val, n, bins, n_seg, cols,
-                       src, pos, tile_flag);
-    return og_hip_check(hipGetLastError(), "og_bin_fill_kernel");
-}
-
-extern "C" int og_launch_bin_tiles(const uint32_t *key, uint32_t n, const og_bin_desc *bins, uint32_t n_seg,
-                                   const uint32_t *tile_flag_scan, uint32_t *tile_seg, uint32_t *tile_ent,
-                                   uint32_t *tile_start, hipStream_t s)
-{
-    if (!n) return 0;
-    hipLaunchKernelGGL(og_bin_tiles_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, n, bins, n_seg,
-                       tile_flag_scan, tile_seg, tile_ent, tile_start);
-    return og_hip_check(hipGetLastError(), "og_bin_tiles_kernel");
-}
-
-extern "C" int og_launch_bin_pads(const og_bin_desc *bins, uint32_t n_bins, uint16_t *src, uint16_t *pos,
+                                  uint32_t n_seg, const uint32_t *cols, const uint32_t *tile_flag_scan,
+                                  const uint32_t *tile_start, const uint32_t *tile_ent, uint16_t *src, uint16_t *pos,
                                   hipStream_t s)
 {
-    if (!n_bins) return 0;
-    hipLaunchKernelGGL(og_bin_pads_kernel, dim3(n_bins), dim3(64), 0, s, bins, src, pos);
-    return og_hip_check(hipGetLastError(), "og_bin_pads_kernel");
+    if (!n) return 0;
+    OG_LAUNCH(og_bin_fill_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, val, n, bins, n_seg, cols,
+                       tile_flag_scan, tile_start, tile_ent, src, pos);
+    return og_hip_check(og_launch_status(), "og_bin_fill_kernel");
+}
+
+extern "C" int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_cnt, uint32_t n_tiles, uint16_t *src,
+                                  uint16_t *pos, hipStream_t s)
+{
+    if (!n_tiles) return 0;
+    OG_LAUNCH(og_bin_pads_kernel, dim3(og_bin_grid(n_tiles)), dim3(256), 0, s, tile_ent, tile_cnt, n_tiles,
+                       src, pos);
+    return og_hip_check(og_launch_status(), "og_bin_pads_kernel");
 }

@@ -67,8 +67,8 @@ extern "C" int og_launch_rmat(uint32_t *d_src, uint32_t *d_dst, uint64_t n, cons
 {
     if (n == 0) return 0;
     unsigned grid = og_grid_for(n, OG_BLOCK, 65536);
-    hipLaunchKernelGGL(og_rmat_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_src, d_dst, n, *g);
-    return og_hip_check(hipGetLastError(), "og_rmat_kernel");
+    OG_LAUNCH(og_rmat_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_src, d_dst, n, *g);
+    return og_hip_check(og_launch_status(), "og_rmat_kernel");
 }
 
 extern "C" int og_launch_er(uint32_t *d_src, uint32_t *d_dst, uint64_t n, uint32_t nv,
@@ -76,8 +76,8 @@ extern "C" int og_launch_er(uint32_t *d_src, uint32_t *d_dst, uint64_t n, uint32
 {
     if (n == 0) return 0;
     unsigned grid = og_grid_for(n, OG_BLOCK, 65536);
-    hipLaunchKernelGGL(og_er_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_src, d_dst, n, nv, key);
-    return og_hip_check(hipGetLastError(), "og_er_kernel");
+    OG_LAUNCH(og_er_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_src, d_dst, n, nv, key);
+    return og_hip_check(og_launch_status(), "og_er_kernel");
 }
 
 /* ------------------------------------------------------------------ */
@@ -117,9 +117,9 @@ __global__ void og_fill_u32_kernel(uint32_t *p, uint32_t v, uint64_t n)
 extern "C" int og_launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s)
 {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(og_fill_u32_kernel, dim3(og_grid_for(n, OG_BLOCK, 16384)), dim3(OG_BLOCK), 0,
+    OG_LAUNCH(og_fill_u32_kernel, dim3(og_grid_for(n, OG_BLOCK, 16384)), dim3(OG_BLOCK), 0,
                        s, p, v, n);
-    return og_hip_check(hipGetLastError(), "og_fill_u32_kernel");
+    return og_hip_check(og_launch_status(), "og_fill_u32_kernel");
 }
 
 extern "C" int og_launch_max_id(const uint32_t *d_src, const uint32_t *d_dst, uint32_t n,
@@ -127,9 +127,9 @@ extern "C" int og_launch_max_id(const uint32_t *d_src, const uint32_t *d_dst, ui
 {
     if (og_launch_fill_u32(d_out, 0, 1, s)) return -1;
     if (n == 0) return 0;
-    hipLaunchKernelGGL(og_max_id_kernel, dim3(og_grid_for(n, OG_BLOCK * 8, 4096)), dim3(OG_BLOCK),
+    OG_LAUNCH(og_max_id_kernel, dim3(og_grid_for(n, OG_BLOCK * 8, 4096)), dim3(OG_BLOCK),
                        0, s, d_src, d_dst, n, d_out);
-    return og_hip_check(hipGetLastError(), "og_max_id_kernel");
+    return og_hip_check(og_launch_status(), "og_max_id_kernel");
 }
 
 __global__ void og_deinterleave_kernel(const uint32_t *__restrict__ pairs, uint32_t n,
@@ -147,9 +147,9 @@ extern "C" int og_launch_deinterleave(const uint32_t *d_pairs, uint32_t n, uint3
                                       uint32_t *d_src, uint32_t *d_dst, hipStream_t s)
 {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(og_deinterleave_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)),
+    OG_LAUNCH(og_deinterleave_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)),
                        dim3(OG_BLOCK), 0, s, d_pairs, n, stride, d_src, d_dst);
-    return og_hip_check(hipGetLastError(), "og_deinterleave_kernel");
+    return og_hip_check(og_launch_status(), "og_deinterleave_kernel");
 }
 
 /* The .csr file body (graphCSR.c:380-449, directed, unweighted): per
@@ -207,9 +207,9 @@ extern "C" int og_launch_csr_pack(uint32_t *const arrays[10], uint32_t N, uint32
     og_csr_arrays g;
     for (int k = 0; k < 10; k++) g.a[k] = arrays[k];
     const uint64_t n = N > E ? N : E;
-    if (n) hipLaunchKernelGGL(og_csr_pack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
+    if (n) OG_LAUNCH(og_csr_pack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
                               g, N, E, body);
-    return og_hip_check(hipGetLastError(), "og_csr_pack_kernel");
+    return og_hip_check(og_launch_status(), "og_csr_pack_kernel");
 }
 
 extern "C" int og_launch_csr_unpack(const uint32_t *body, uint32_t N, uint32_t E, uint32_t *const arrays[10],
@@ -218,9 +218,9 @@ extern "C" int og_launch_csr_unpack(const uint32_t *body, uint32_t N, uint32_t E
     og_csr_arrays g;
     for (int k = 0; k < 10; k++) g.a[k] = arrays[k];
     const uint64_t n = N > E ? N : E;
-    if (n) hipLaunchKernelGGL(og_csr_unpack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
+    if (n) OG_LAUNCH(og_csr_unpack_kernel, dim3(og_grid_for(n, OG_BLOCK, 65536)), dim3(OG_BLOCK), 0, s,
                               body, N, E, g);
-    return og_hip_check(hipGetLastError(), "og_csr_unpack_kernel");
+    return og_hip_check(og_launch_status(), "og_csr_unpack_kernel");
 }
 
 /* ------------------------------------------------------------------ */
@@ -407,12 +407,12 @@ extern "C" int og_radix_sort_pairs(const uint32_t *k_in, const uint32_t *v_in, u
     {
         uint32_t *ko = (p & 1) ? k_b : k_a;
         uint32_t *vo = (p & 1) ? v_b : v_a;
-        hipLaunchKernelGGL(og_rs_hist_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, n, chunk, p * 8,
+        OG_LAUNCH(og_rs_hist_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, n, chunk, p * 8,
                            counts);
         if (og_scan_exclusive(counts, counts, 256 * nb, counts + RS_COUNTS_WORDS, s)) return -1;
-        hipLaunchKernelGGL(og_rs_scatter_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, vi, ko, vo, n,
+        OG_LAUNCH(og_rs_scatter_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, ki, vi, ko, vo, n,
                            chunk, p * 8, counts);
-        if (og_hip_check(hipGetLastError(), "og_radix_sort_pairs")) return -1;
+        if (og_hip_check(og_launch_status(), "og_radix_sort_pairs")) return -1;
         ki = ko;
         vi = vo;
     }
@@ -456,11 +456,11 @@ extern "C" int og_launch_map_vertices(const uint32_t *d_key, uint32_t n_edges, u
     if (og_launch_fill_u32(d_edges_idx, 0, n_vertices, s)) return -1;
     if (n_edges == 0) return 0;
     unsigned grid = og_grid_for(n_edges, OG_BLOCK, 65536);
-    hipLaunchKernelGGL(og_map_start_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_key, n_edges,
+    OG_LAUNCH(og_map_start_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_key, n_edges,
                        d_edges_idx);
-    hipLaunchKernelGGL(og_map_end_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_key, n_edges,
+    OG_LAUNCH(og_map_end_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, d_key, n_edges,
                        d_edges_idx, d_out_degree);
-    return og_hip_check(hipGetLastError(), "og_map_vertices");
+    return og_hip_check(og_launch_status(), "og_map_vertices");
 }
 
 /* ------------------------------------------------------------------ */
@@ -480,9 +480,9 @@ __global__ void og_rank_keys_kernel(const float *__restrict__ r, uint32_t n, uin
 extern "C" int og_launch_rank_keys(const float *d_ranks, uint32_t n, uint32_t *d_keys, hipStream_t s)
 {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(og_rank_keys_kernel, dim3(og_grid_for(n, OG_BLOCK, 16384)), dim3(OG_BLOCK), 0, s,
+    OG_LAUNCH(og_rank_keys_kernel, dim3(og_grid_for(n, OG_BLOCK, 16384)), dim3(OG_BLOCK), 0, s,
                        d_ranks, n, d_keys);
-    return og_hip_check(hipGetLastError(), "og_rank_keys_kernel");
+    return og_hip_check(og_launch_status(), "og_rank_keys_kernel");
 }
 
 /* ------------------------------------------------------------------ */
@@ -500,7 +500,7 @@ extern "C" int og_launch_copy(const void *src, void *dst, uint64_t bytes, hipStr
 {
     uint64_t n16 = bytes / 16;
     if (n16 == 0) return 0;
-    hipLaunchKernelGGL(og_copy_kernel, dim3(og_grid_for(n16, OG_BLOCK * 4, 65536)), dim3(OG_BLOCK), 0, s,
+    OG_LAUNCH(og_copy_kernel, dim3(og_grid_for(n16, OG_BLOCK * 4, 65536)), dim3(OG_BLOCK), 0, s,
                        (const uint4 *)src, (uint4 *)dst, n16);
-    return og_hip_check(hipGetLastError(), "og_copy_kernel");
+    return og_hip_check(og_launch_status(), "og_copy_kernel");
 }

@@ -37,6 +37,20 @@ void *og_dalloc(size_t bytes);
 void og_dfree(void *p);
 /* Checks a HIP status; on failure records "<what>: <hip error>" and returns -1. */
 int og_hip_check(hipError_t e, const char *what);
+/* Kernel launches.  The runtime keeps a failed call's error until
+ * hipGetLastError reads it, so a check after a launch could report an
+ * earlier call's error that its caller already handled.  OG_LAUNCH clears
+ * that first, launches, and keeps the launch's own status: og_launch_status
+ * returns the first failing launch's since its last call (and resets it). */
+void og_launch_note(hipError_t e);
+hipError_t og_launch_status(void);
+#define OG_LAUNCH(...)                     \
+    do                                     \
+    {                                      \
+        (void)hipGetLastError();           \
+        hipLaunchKernelGGL(__VA_ARGS__);   \
+        og_launch_note(hipGetLastError()); \
+    } while (0)
 /* A copy ordered after all work already queued on the library stream and
  * complete on return (hipMemcpy itself runs on the null stream, which the
  * library's non-blocking stream does not wait for). */

@@ -558,11 +558,11 @@ extern "C" int og_launch_wc_sums(const float *r0, const uint32_t *in_degree, con
         og_set_error("coarse start correction: %u classes (2..%u)", nc, OG_WC_MAX);
         return -1;
     }
-    hipLaunchKernelGGL(og_wc_sums_kernel, dim3(OG_WC_BLOCKS), dim3(LB), 0, s, r0, in_degree, out_degree, n, nc,
+    OG_LAUNCH(og_wc_sums_kernel, dim3(OG_WC_BLOCKS), dim3(LB), 0, s, r0, in_degree, out_degree, n, nc,
                        cls, c0, scratch + OG_WC_BASE);
-    hipLaunchKernelGGL(og_wc_fold_kernel, dim3(1), dim3(64), 0, s, scratch + OG_WC_BASE, scratch + og_wc_mass(),
+    OG_LAUNCH(og_wc_fold_kernel, dim3(1), dim3(64), 0, s, scratch + OG_WC_BASE, scratch + og_wc_mass(),
                        scratch + og_wc_count());
-    return og_hip_check(hipGetLastError(), "og_wc_sums_kernel");
+    return og_hip_check(og_launch_status(), "og_wc_sums_kernel");
 }
 
 extern "C" int og_launch_wc_matrix(const uint32_t *src, const uint32_t *dst, uint32_t E, const uint8_t *cls,
@@ -570,29 +570,29 @@ extern "C" int og_launch_wc_matrix(const uint32_t *src, const uint32_t *dst, uin
 {
     if (og_launch_fill_u32((uint32_t *)mat, 0, 2ull * nc * nc, s)) return -1;
     if (!E) return 0;
-    hipLaunchKernelGGL(og_wc_matrix_kernel, dim3(og_lgrid(E) < 2048 ? og_lgrid(E) : 2048), dim3(LB), 0, s, src,
+    OG_LAUNCH(og_wc_matrix_kernel, dim3(og_lgrid(E) < 2048 ? og_lgrid(E) : 2048), dim3(LB), 0, s, src,
                        dst, E, cls, c0, nc, mat);
-    return og_hip_check(hipGetLastError(), "og_wc_matrix_kernel");
+    return og_hip_check(og_launch_status(), "og_wc_matrix_kernel");
 }
 
 extern "C" int og_launch_wc_apply(float *r0, const uint8_t *cls, uint32_t n, const float *scale, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_wc_apply_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, r0, cls, n, scale);
-    return og_hip_check(hipGetLastError(), "og_wc_apply_kernel");
+    OG_LAUNCH(og_wc_apply_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, r0, cls, n, scale);
+    return og_hip_check(og_launch_status(), "og_wc_apply_kernel");
 }
 
 extern "C" int og_launch_feat_li(const uint32_t *row_ptr, uint32_t n, float *li, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_feat_li_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, n, li);
-    return og_hip_check(hipGetLastError(), "og_feat_li_kernel");
+    OG_LAUNCH(og_feat_li_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, n, li);
+    return og_hip_check(og_launch_status(), "og_feat_li_kernel");
 }
 
 extern "C" int og_launch_feat_finish(const uint32_t *row_ptr, const uint32_t *outdeg, const float *insum,
                                      uint32_t n, float *feat, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_feat_finish_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, outdeg, insum, n,
+    OG_LAUNCH(og_feat_finish_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, row_ptr, outdeg, insum, n,
                        (float4 *)feat);
-    return og_hip_check(hipGetLastError(), "og_feat_finish_kernel");
+    return og_hip_check(og_launch_status(), "og_feat_finish_kernel");
 }
 
 extern "C" int og_launch_mlp_infer(const float *params, const float *feat, uint32_t n, int mfma, float *out,
@@ -603,12 +603,12 @@ extern "C" int og_launch_mlp_infer(const float *params, const float *feat, uint3
     {
         uint64_t tiles = ((uint64_t)n + 31) / 32;
         unsigned g = (unsigned)((tiles + 3) / 4 < 8192 ? (tiles + 3) / 4 : 8192);
-        hipLaunchKernelGGL(og_mlp_infer_mfma_kernel, dim3(g), dim3(LB), 0, s, params, (const float4 *)feat, n, out);
+        OG_LAUNCH(og_mlp_infer_mfma_kernel, dim3(g), dim3(LB), 0, s, params, (const float4 *)feat, n, out);
     }
     else
-        hipLaunchKernelGGL(og_mlp_infer_f32_kernel, dim3(og_lgrid(n) < 4096 ? og_lgrid(n) : 4096), dim3(LB), 0,
+        OG_LAUNCH(og_mlp_infer_f32_kernel, dim3(og_lgrid(n) < 4096 ? og_lgrid(n) : 4096), dim3(LB), 0,
                            s, params, (const float4 *)feat, n, out);
-    return og_hip_check(hipGetLastError(), "og_mlp_infer_kernel");
+    return og_hip_check(og_launch_status(), "og_mlp_infer_kernel");
 }
 
 extern "C" size_t og_mlp_part_floats(uint32_t count)
@@ -630,29 +630,29 @@ extern "C" int og_launch_mlp_grad(const float *params, const float *feat, const 
         perm = og_perm_make(seed, bits);
     }
     const uint32_t nb = (count + TB - 1) / TB;
-    hipLaunchKernelGGL(og_mlp_grad_kernel, dim3(nb), dim3(TB), 0, s, params, (const float4 *)feat, y, n_data,
+    OG_LAUNCH(og_mlp_grad_kernel, dim3(nb), dim3(TB), 0, s, params, (const float4 *)feat, y, n_data,
                        perm, base, count, part);
-    hipLaunchKernelGGL(og_mlp_grad_reduce_kernel, dim3((OG_MLP_NP + LB) / LB), dim3(LB), 0, s, part, nb, count,
+    OG_LAUNCH(og_mlp_grad_reduce_kernel, dim3((OG_MLP_NP + LB) / LB), dim3(LB), 0, s, part, nb, count,
                        grad, loss_sum);
-    return og_hip_check(hipGetLastError(), "og_mlp_grad_kernel");
+    return og_hip_check(og_launch_status(), "og_mlp_grad_kernel");
 }
 
 extern "C" int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
                                   float b2, float eps, uint32_t step, hipStream_t s)
 {
     const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-    hipLaunchKernelGGL(og_mlp_adam_kernel, dim3((OG_MLP_NP + LB - 1) / LB), dim3(LB), 0, s, params, grad, m1, m2,
+    OG_LAUNCH(og_mlp_adam_kernel, dim3((OG_MLP_NP + LB - 1) / LB), dim3(LB), 0, s, params, grad, m1, m2,
                        lr, b1, b2, eps, c1, c2);
-    return og_hip_check(hipGetLastError(), "og_mlp_adam_kernel");
+    return og_hip_check(og_launch_status(), "og_mlp_adam_kernel");
 }
 
 extern "C" int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
                                     const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
                                     hipStream_t s)
 {
-    hipLaunchKernelGGL(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, row_ptr, outdeg, n, scratch);
-    hipLaunchKernelGGL(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, scratch, n);
-    hipLaunchKernelGGL(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, row_ptr, new2old, n, scratch,
+    OG_LAUNCH(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, row_ptr, outdeg, n, scratch);
+    OG_LAUNCH(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, scratch, n);
+    OG_LAUNCH(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, row_ptr, new2old, n, scratch,
                        r0_old);
-    return og_hip_check(hipGetLastError(), "og_warm_scale");
+    return og_hip_check(og_launch_status(), "og_warm_scale");
 }

@@ -265,7 +265,9 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
      * fp32 push gathers its destinations of more than OG_PUSH_EXACT_ABOVE
      * in-edges the same way (their atomic sums would stray furthest from the
      * reference's source-ordered adds) */
-    const uint32_t chain_t = binned ? 0
+    /* (binned: every row of more than 512 in-edges counts as a chain, so the
+     * plan orders all rows by in-degree, longest first) */
+    const uint32_t chain_t = binned ? OG_CH
                            : !chain && opt->variant == 0 && !opt->ppr ? opt->exact_above
                            : opt->variant == 1 && opt->exact_above ? OG_PUSH_EXACT_ABOVE : 0;
     double t0 = og_now();
@@ -377,13 +379,14 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
      * chains [0, C) and no chunks; fp32 pull on a plan with chains [0, L),
      * their chunks skipped; otherwise every chunk */
     uint32_t n_chains = 0, ch_first = 0, chunk_blocks = pl->n_chunk_blocks;
-    const int push_chains = opt->variant == 1 && pl->L;
+    /* (a borrowed plan may carry chains the options did not ask for) */
+    const int push_chains = opt->variant == 1 && opt->exact_above && pl->L;
     if (p->chain_rows)
     {
         n_chains = pl->C;
         chunk_blocks = 0;
     }
-    else if (pl->L && opt->variant == 0 && !opt->ppr)
+    else if (pl->L && opt->variant == 0 && !opt->ppr && !binned)
     {
         n_chains = pl->L;
         ch_first = pl->ch_first;

@@ -100,14 +100,14 @@ extern "C" int og_scan_exclusive(const uint32_t *d_in, uint32_t *d_out, uint32_t
     uint32_t *bsum = (uint32_t *)d_scratch;
     if (n == 0)
     {
-        hipLaunchKernelGGL(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, 0u, d_out);
-        return og_hip_check(hipGetLastError(), "og_scan_exclusive");
+        OG_LAUNCH(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, 0u, d_out);
+        return og_hip_check(og_launch_status(), "og_scan_exclusive");
     }
-    hipLaunchKernelGGL(og_scan_tile_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, d_in, d_out, n, bsum);
-    hipLaunchKernelGGL(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, nb, d_out + n);
+    OG_LAUNCH(og_scan_tile_kernel, dim3(nb), dim3(OG_BLOCK), 0, s, d_in, d_out, n, bsum);
+    OG_LAUNCH(og_scan_single_kernel, dim3(1), dim3(1024), 0, s, bsum, nb, d_out + n);
     unsigned g = nb < 65535 ? nb : 65535;
-    hipLaunchKernelGGL(og_scan_add_kernel, dim3(g), dim3(OG_BLOCK), 0, s, d_out, n, bsum);
-    return og_hip_check(hipGetLastError(), "og_scan_exclusive");
+    OG_LAUNCH(og_scan_add_kernel, dim3(g), dim3(OG_BLOCK), 0, s, d_out, n, bsum);
+    return og_hip_check(og_launch_status(), "og_scan_exclusive");
 }
 
 /* ------------------------------------------------------------------ */
@@ -230,8 +230,8 @@ extern "C" int og_launch_plan_row_ids_at(uint32_t R, const uint32_t *row_ptr, ui
     if (!R) return 0;
     uint64_t waves = R;
     unsigned grid = (unsigned)((waves + 3) / 4 < 65536 ? (waves + 3) / 4 : 65536);
-    hipLaunchKernelGGL(og_plan_row_ids_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, R, row_ptr, base, rowid);
-    return og_hip_check(hipGetLastError(), "og_plan_row_ids_kernel");
+    OG_LAUNCH(og_plan_row_ids_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, R, row_ptr, base, rowid);
+    return og_hip_check(og_launch_status(), "og_plan_row_ids_kernel");
 }
 
 __global__ void og_plan_chunk_count_kernel(uint32_t C, const uint32_t *__restrict__ row_ptr,
@@ -289,9 +289,9 @@ extern "C" int og_launch_plan_keys(uint32_t N, const uint32_t *outdeg, const uin
                                    uint32_t seq_t, uint32_t chain_t, uint32_t relabel, uint32_t *key,
                                    uint32_t *val, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_plan_keys_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, outdeg,
+    OG_LAUNCH(og_plan_keys_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, outdeg,
                        indeg, seq_t, chain_t, relabel, key, val);
-    return og_hip_check(hipGetLastError(), "og_plan_keys_kernel");
+    return og_hip_check(og_launch_status(), "og_plan_keys_kernel");
 }
 
 extern "C" int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const uint32_t *outdeg_old,
@@ -300,9 +300,9 @@ extern "C" int og_launch_plan_perm(uint32_t N, const uint32_t *new2old, const ui
                                    uint32_t *outdeg_new, uint32_t *indeg_new, uint32_t *class_cnt,
                                    hipStream_t s)
 {
-    hipLaunchKernelGGL(og_plan_perm_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, new2old,
+    OG_LAUNCH(og_plan_perm_kernel, dim3(og_grid(N, 16384)), dim3(OG_BLOCK), 0, s, N, new2old,
                        outdeg_old, indeg_old, seq_t, chain_t, old2new, outdeg_new, indeg_new, class_cnt);
-    return og_hip_check(hipGetLastError(), "og_plan_perm_kernel");
+    return og_hip_check(og_launch_status(), "og_plan_perm_kernel");
 }
 
 extern "C" int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2old,
@@ -310,34 +310,34 @@ extern "C" int og_launch_plan_copy(uint32_t H, uint32_t N, const uint32_t *new2o
                                    const uint32_t *lists, const uint32_t *row_ptr, uint32_t *cols,
                                    hipStream_t s)
 {
-    if (H) hipLaunchKernelGGL(og_plan_copy_hubs_kernel, dim3(H), dim3(OG_BLOCK), 0, s, new2old, old2new,
+    if (H) OG_LAUNCH(og_plan_copy_hubs_kernel, dim3(H), dim3(OG_BLOCK), 0, s, new2old, old2new,
                               idx_old, lists, row_ptr, cols);
     if (N > H)
     {
         uint64_t waves = N - H;
         unsigned grid = (unsigned)((waves + 3) / 4 < 65536 ? (waves + 3) / 4 : 65536);
-        hipLaunchKernelGGL(og_plan_copy_rows_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, H, N, new2old,
+        OG_LAUNCH(og_plan_copy_rows_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, H, N, new2old,
                            old2new, idx_old, lists, row_ptr, cols);
     }
-    return og_hip_check(hipGetLastError(), "og_plan_copy");
+    return og_hip_check(og_launch_status(), "og_plan_copy");
 }
 
 extern "C" int og_launch_plan_chunks(uint32_t C, const uint32_t *row_ptr, uint32_t ch, uint32_t *cnt,
                                      hipStream_t s)
 {
     if (!C) return 0;
-    hipLaunchKernelGGL(og_plan_chunk_count_kernel, dim3(og_grid(C, 16384)), dim3(OG_BLOCK), 0, s, C,
+    OG_LAUNCH(og_plan_chunk_count_kernel, dim3(og_grid(C, 16384)), dim3(OG_BLOCK), 0, s, C,
                        row_ptr, ch, cnt);
-    return og_hip_check(hipGetLastError(), "og_plan_chunk_count_kernel");
+    return og_hip_check(og_launch_status(), "og_plan_chunk_count_kernel");
 }
 
 extern "C" int og_launch_plan_chunk_desc(uint32_t H, uint32_t C, const uint32_t *row_ptr,
                                          const uint32_t *first, uint32_t ch, uint32_t *ch_row,
                                          uint32_t *ch_edge, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_plan_chunk_desc_kernel, dim3(og_grid((uint64_t)C + 1, 16384)), dim3(OG_BLOCK),
+    OG_LAUNCH(og_plan_chunk_desc_kernel, dim3(og_grid((uint64_t)C + 1, 16384)), dim3(OG_BLOCK),
                        0, s, H, C, row_ptr, first, ch, ch_row, ch_edge);
-    return og_hip_check(hipGetLastError(), "og_plan_chunk_desc_kernel");
+    return og_hip_check(og_launch_status(), "og_plan_chunk_desc_kernel");
 }
 
 extern "C" int og_launch_plan_map_pairs(uint32_t E, const uint32_t *a, const uint32_t *b,
@@ -345,7 +345,7 @@ extern "C" int og_launch_plan_map_pairs(uint32_t E, const uint32_t *a, const uin
                                         hipStream_t s)
 {
     if (!E) return 0;
-    hipLaunchKernelGGL(og_plan_map_pairs_kernel, dim3(og_grid(E, 65536)), dim3(OG_BLOCK), 0, s, E, a, b,
+    OG_LAUNCH(og_plan_map_pairs_kernel, dim3(og_grid(E, 65536)), dim3(OG_BLOCK), 0, s, E, a, b,
                        old2new, oa, ob);
-    return og_hip_check(hipGetLastError(), "og_plan_map_pairs_kernel");
+    return og_hip_check(og_launch_status(), "og_plan_map_pairs_kernel");
 }

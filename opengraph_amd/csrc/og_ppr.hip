@@ -271,27 +271,27 @@ __global__ __launch_bounds__(PB) void og_ppr_permute_kernel(og_ppr_dev a, float 
 
 extern "C" int og_launch_ppr_permute(const og_ppr_dev *a, float *c, og_ppr_perm perm, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_ppr_permute_kernel, dim3(4096), dim3(PB), 0, s, *a, c, perm);
-    return og_hip_check(hipGetLastError(), "og_ppr_permute_kernel");
+    OG_LAUNCH(og_ppr_permute_kernel, dim3(4096), dim3(PB), 0, s, *a, c, perm);
+    return og_hip_check(og_launch_status(), "og_ppr_permute_kernel");
 }
 
 extern "C" int og_launch_ppr_init(const og_ppr_dev *a, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_ppr_init_kernel, dim3(4096), dim3(PB), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_ppr_init_kernel");
+    OG_LAUNCH(og_ppr_init_kernel, dim3(4096), dim3(PB), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_ppr_init_kernel");
 }
 
 extern "C" int og_launch_ppr_iter(const og_ppr_dev *a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (ev0) hipEventRecord(ev0, s);
-    hipLaunchKernelGGL(og_ppr_main_kernel, dim3(OG_PPR_MAIN_BLOCKS), dim3(PB), 0, s, *a);
+    OG_LAUNCH(og_ppr_main_kernel, dim3(OG_PPR_MAIN_BLOCKS), dim3(PB), 0, s, *a);
     if (ev1) hipEventRecord(ev1, s);
-    hipLaunchKernelGGL(og_ppr_finalize_kernel, dim3(OG_PPR_FIN_BLOCKS), dim3(PB), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_ppr_main_kernel");
+    OG_LAUNCH(og_ppr_finalize_kernel, dim3(OG_PPR_FIN_BLOCKS), dim3(PB), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_ppr_main_kernel");
 }
 
 extern "C" int og_launch_ppr_out(const og_ppr_dev *a, float *out_old, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_ppr_out_kernel, dim3(4096), dim3(PB), 0, s, *a, out_old);
-    return og_hip_check(hipGetLastError(), "og_ppr_out_kernel");
+    OG_LAUNCH(og_ppr_out_kernel, dim3(4096), dim3(PB), 0, s, *a, out_old);
+    return og_hip_check(og_launch_status(), "og_ppr_out_kernel");
 }

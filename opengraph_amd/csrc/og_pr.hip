@@ -1191,22 +1191,22 @@ template <int MODE> static int og_launch_pull_t(const og_pr_dev *a, uint32_t it,
             return -1;
         og_pr_dev b = *a;
         b.chain_part_base = a->hot_grid;
-        hipLaunchKernelGGL(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, side, b, it);
-        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
+        OG_LAUNCH(og_pull_chains_kernel, dim3(a->n_chain_blocks), dim3(64), 0, side, b, it);
+        OG_LAUNCH(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
         if (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
             og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent"))
             return -1;
     }
     else if (a->hot_grid)
-        hipLaunchKernelGGL(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
+        OG_LAUNCH(og_pull_hot_kernel<MODE>, dim3(a->hot_grid), dim3(OG_HOT_SUBS * OG_BLOCK), 0,
                            s, *a, it);
     else if (MODE == OG_MODE_F32 && a->chain_rows && !a->n_chunk_blocks && grid)
-        hipLaunchKernelGGL(og_pull_exact_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
+        OG_LAUNCH(og_pull_exact_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     else if (grid)
-        hipLaunchKernelGGL(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
+        OG_LAUNCH(og_pull_kernel<MODE>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, it);
     if (ev1) hipEventRecord(ev1, s);
-    hipLaunchKernelGGL(og_pull_finalize_kernel<MODE>, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s,
+    OG_LAUNCH(og_pull_finalize_kernel<MODE>, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s,
                        *a, it);
     return 0;
 }
@@ -1228,7 +1228,7 @@ extern "C" int og_launch_pull(const og_pr_dev *a, int mode, uint32_t it, hipStre
     default: og_set_error("og_launch_pull: bad mode %d", mode); return -1;
     }
     if (rc) return rc;
-    return og_hip_check(hipGetLastError(), "og_pull_kernel");
+    return og_hip_check(og_launch_status(), "og_pull_kernel");
 }
 
 extern "C" int og_launch_quantize(const og_pr_dev *a, int mode, hipStream_t s)
@@ -1238,12 +1238,12 @@ extern "C" int og_launch_quantize(const og_pr_dev *a, int mode, hipStream_t s)
     if (grid == 0) grid = 1;
     switch (mode)
     {
-    case OG_MODE_Q32: hipLaunchKernelGGL(og_quantize_kernel<OG_MODE_Q32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
-    case OG_MODE_Q16: hipLaunchKernelGGL(og_quantize_kernel<OG_MODE_Q16>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
-    case OG_MODE_Q8: hipLaunchKernelGGL(og_quantize_kernel<OG_MODE_Q8>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
-    default: hipLaunchKernelGGL(og_quantize_kernel<OG_MODE_QPUSH_EPI>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
+    case OG_MODE_Q32: OG_LAUNCH(og_quantize_kernel<OG_MODE_Q32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
+    case OG_MODE_Q16: OG_LAUNCH(og_quantize_kernel<OG_MODE_Q16>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
+    case OG_MODE_Q8: OG_LAUNCH(og_quantize_kernel<OG_MODE_Q8>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
+    default: OG_LAUNCH(og_quantize_kernel<OG_MODE_QPUSH_EPI>, dim3(grid), dim3(OG_BLOCK), 0, s, *a); break;
     }
-    return og_hip_check(hipGetLastError(), "og_quantize_kernel");
+    return og_hip_check(og_launch_status(), "og_quantize_kernel");
 }
 
 extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s,
@@ -1266,15 +1266,15 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
             if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
                 og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
                 return -1;
-            hipLaunchKernelGGL(og_pull_chains_kernel, dim3(nc), dim3(64), 0, side, b, it);
+            OG_LAUNCH(og_pull_chains_kernel, dim3(nc), dim3(64), 0, side, b, it);
         }
         if (a->hot_grid)
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
         if (ev1) hipEventRecord(ev1, s);
-        hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_F32>, dim3(a->n_parts - nc), dim3(OG_BLOCK), 0, s, *a);
+        OG_LAUNCH(og_push_epilogue_kernel<OG_MODE_F32>, dim3(a->n_parts - nc), dim3(OG_BLOCK), 0, s, *a);
         if (nc && (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
                    og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent")))
             return -1;
@@ -1282,26 +1282,26 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
     else if (pmode == OG_MODE_FX64)
     {
         if (a->hot_grid)
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
         if (ev1) hipEventRecord(ev1, s);
-        hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_FX64>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
+        OG_LAUNCH(og_push_epilogue_kernel<OG_MODE_FX64>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
     }
     else
     {
         if (a->hot_grid)
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, true>), dim3(a->hot_grid),
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, false>), dim3(gs), dim3(OG_BLOCK), 0,
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_QPUSH_EPI, false>), dim3(gs), dim3(OG_BLOCK), 0,
                                s, *a);
         if (ev1) hipEventRecord(ev1, s);
-        hipLaunchKernelGGL(og_push_epilogue_kernel<OG_MODE_QPUSH_EPI>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
+        OG_LAUNCH(og_push_epilogue_kernel<OG_MODE_QPUSH_EPI>, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a);
     }
-    hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
-    return og_hip_check(hipGetLastError(), "og_push");
+    OG_LAUNCH(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
+    return og_hip_check(og_launch_status(), "og_push");
 }
 
 extern "C" int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t s, hipEvent_t ev0,
@@ -1314,18 +1314,18 @@ extern "C" int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t
     if (pmode == OG_MODE_F32)
     {
         if (a->hot_grid)
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_F32, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
     }
     else if (pmode == OG_MODE_FX64)
     {
         if (a->hot_grid)
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_FX64, true>), dim3(a->hot_grid),
                                dim3(OG_HOT_SUBS * OG_BLOCK), 0, s, *a);
         else
-            hipLaunchKernelGGL((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
+            OG_LAUNCH((og_push_scatter_kernel<OG_MODE_FX64, false>), dim3(gs), dim3(OG_BLOCK), 0, s, *a);
     }
     else
     {
@@ -1333,7 +1333,7 @@ extern "C" int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t
         return -1;
     }
     if (ev1) hipEventRecord(ev1, s);
-    return og_hip_check(hipGetLastError(), "og_push_scatter_kernel");
+    return og_hip_check(og_launch_status(), "og_push_scatter_kernel");
 }
 
 extern "C" int og_launch_init(const og_pr_dev *a, int mode, const float *init_old, float init_value,
@@ -1342,28 +1342,28 @@ extern "C" int og_launch_init(const og_pr_dev *a, int mode, const float *init_ol
     unsigned grid = a->n_parts;
     switch (mode)
     {
-    case OG_MODE_F32: hipLaunchKernelGGL(og_init_kernel<OG_MODE_F32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
-    case OG_MODE_FX64: hipLaunchKernelGGL(og_init_kernel<OG_MODE_FX64>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
-    case OG_MODE_FX32: hipLaunchKernelGGL(og_init_kernel<OG_MODE_FX32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
-    case OG_MODE_FX16: hipLaunchKernelGGL(og_init_kernel<OG_MODE_FX16>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
-    case OG_MODE_FX8: hipLaunchKernelGGL(og_init_kernel<OG_MODE_FX8>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
-    default: hipLaunchKernelGGL(og_init_kernel<OG_MODE_Q32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    case OG_MODE_F32: OG_LAUNCH(og_init_kernel<OG_MODE_F32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    case OG_MODE_FX64: OG_LAUNCH(og_init_kernel<OG_MODE_FX64>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    case OG_MODE_FX32: OG_LAUNCH(og_init_kernel<OG_MODE_FX32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    case OG_MODE_FX16: OG_LAUNCH(og_init_kernel<OG_MODE_FX16>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    case OG_MODE_FX8: OG_LAUNCH(og_init_kernel<OG_MODE_FX8>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
+    default: OG_LAUNCH(og_init_kernel<OG_MODE_Q32>, dim3(grid), dim3(OG_BLOCK), 0, s, *a, init_old, init_value); break;
     }
-    return og_hip_check(hipGetLastError(), "og_init_kernel");
+    return og_hip_check(og_launch_status(), "og_init_kernel");
 }
 
 extern "C" int og_launch_final(const og_pr_dev *a, uint32_t it, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
-    return og_hip_check(hipGetLastError(), "og_final_kernel");
+    OG_LAUNCH(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
+    return og_hip_check(og_launch_status(), "og_final_kernel");
 }
 
 extern "C" int og_launch_normalize(const og_pr_dev *a, float *d_out_old, double *d_sum,
                                    hipStream_t s)
 {
-    hipLaunchKernelGGL(og_normalize_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, d_out_old);
-    hipLaunchKernelGGL(og_sum_parts_kernel, dim3(1), dim3(OG_BLOCK), 0, s, a->part_err, a->n_parts, d_sum);
-    return og_hip_check(hipGetLastError(), "og_normalize_kernel");
+    OG_LAUNCH(og_normalize_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, d_out_old);
+    OG_LAUNCH(og_sum_parts_kernel, dim3(1), dim3(OG_BLOCK), 0, s, a->part_err, a->n_parts, d_sum);
+    return og_hip_check(og_launch_status(), "og_normalize_kernel");
 }
 
 /* ------------------------------------------------------------------ */
@@ -1504,8 +1504,8 @@ static unsigned og_vgrid(uint64_t n)
 
 extern "C" int og_launch_dd_contrib(const og_pr_dev *a, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_dd_contrib_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_dd_contrib_kernel");
+    OG_LAUNCH(og_dd_contrib_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_dd_contrib_kernel");
 }
 
 extern "C" int og_launch_dd_activate(const og_pr_dev *a, hipStream_t s)
@@ -1513,21 +1513,21 @@ extern "C" int og_launch_dd_activate(const og_pr_dev *a, hipStream_t s)
     if (!a->E) return 0;
     unsigned g = (a->E + OG_BLOCK - 1) / OG_BLOCK;
     if (g > 16384) g = 16384;
-    hipLaunchKernelGGL(og_dd_activate_kernel, dim3(g), dim3(OG_BLOCK), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_dd_activate_kernel");
+    OG_LAUNCH(og_dd_activate_kernel, dim3(g), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_dd_activate_kernel");
 }
 
 extern "C" int og_launch_dd_resid_init(const og_pr_dev *a, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_dd_resid_init_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_dd_resid_init_kernel");
+    OG_LAUNCH(og_dd_resid_init_kernel, dim3(og_vgrid(a->N)), dim3(OG_BLOCK), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_dd_resid_init_kernel");
 }
 
 extern "C" int og_launch_dd_push(const og_pr_dev *a, uint32_t code, uint32_t it, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_dd_push_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, code);
-    hipLaunchKernelGGL(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
-    return og_hip_check(hipGetLastError(), "og_dd_push_kernel");
+    OG_LAUNCH(og_dd_push_kernel, dim3(a->n_parts), dim3(OG_BLOCK), 0, s, *a, code);
+    OG_LAUNCH(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
+    return og_hip_check(og_launch_status(), "og_dd_push_kernel");
 }
 
 /* ------------------------------------------------------------------ */
@@ -1656,17 +1656,17 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_reduce_kernel(og_multi_dev 
 
 extern "C" int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_multi_init_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m);
-    return og_hip_check(hipGetLastError(), "og_multi_init_kernel");
+    OG_LAUNCH(og_multi_init_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m);
+    return og_hip_check(og_launch_status(), "og_multi_init_kernel");
 }
 
 extern "C" int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s)
 {
     const uint32_t parity = it & 1;
-    hipLaunchKernelGGL(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity,
+    OG_LAUNCH(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity,
                        (uint32_t)(it >= 2));
     if (m->n_lwork)
-        hipLaunchKernelGGL(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
-    hipLaunchKernelGGL(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);
-    return og_hip_check(hipGetLastError(), "og_multi_pull_kernel");
+        OG_LAUNCH(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
+    OG_LAUNCH(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);
+    return og_hip_check(og_launch_status(), "og_multi_pull_kernel");
 }

@@ -91,41 +91,41 @@ __global__ __launch_bounds__(RB) void og_compose_labels_kernel(uint32_t *__restr
 extern "C" int og_launch_reorder_degrees(const uint32_t *src, const uint32_t *dst, uint32_t E, uint32_t count_src,
                                          uint32_t count_dst, uint32_t *deg, hipStream_t s)
 {
-    if (E) hipLaunchKernelGGL(og_reorder_degree_kernel, dim3(og_rgrid(E)), dim3(RB), 0, s, src, dst, E, count_src,
+    if (E) OG_LAUNCH(og_reorder_degree_kernel, dim3(og_rgrid(E)), dim3(RB), 0, s, src, dst, E, count_src,
                               count_dst, deg);
-    return og_hip_check(hipGetLastError(), "og_reorder_degree_kernel");
+    return og_hip_check(og_launch_status(), "og_reorder_degree_kernel");
 }
 
 extern "C" int og_launch_reorder_dbg_keys(const uint32_t *deg, uint32_t N, og_dbg_thresholds th, uint32_t *key,
                                           uint32_t *ids, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_reorder_dbg_key_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, deg, N, th, key, ids);
-    return og_hip_check(hipGetLastError(), "og_reorder_dbg_key_kernel");
+    OG_LAUNCH(og_reorder_dbg_key_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, deg, N, th, key, ids);
+    return og_hip_check(og_launch_status(), "og_reorder_dbg_key_kernel");
 }
 
 extern "C" int og_launch_iota(uint32_t *ids, uint32_t N, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_reorder_iota_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, ids, N);
-    return og_hip_check(hipGetLastError(), "og_reorder_iota_kernel");
+    OG_LAUNCH(og_reorder_iota_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, ids, N);
+    return og_hip_check(og_launch_status(), "og_reorder_iota_kernel");
 }
 
 extern "C" int og_launch_reorder_labels(const uint32_t *sorted, uint32_t N, uint32_t reverse, uint32_t *labels,
                                         hipStream_t s)
 {
-    hipLaunchKernelGGL(og_reorder_label_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, sorted, N, reverse, labels);
-    return og_hip_check(hipGetLastError(), "og_reorder_label_kernel");
+    OG_LAUNCH(og_reorder_label_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, sorted, N, reverse, labels);
+    return og_hip_check(og_launch_status(), "og_reorder_label_kernel");
 }
 
 extern "C" int og_launch_relabel(uint32_t *src, uint32_t *dst, uint32_t E, const uint32_t *labels, hipStream_t s)
 {
-    if (E) hipLaunchKernelGGL(og_relabel_kernel, dim3(og_rgrid(E)), dim3(RB), 0, s, src, dst, E, labels);
-    return og_hip_check(hipGetLastError(), "og_relabel_kernel");
+    if (E) OG_LAUNCH(og_relabel_kernel, dim3(og_rgrid(E)), dim3(RB), 0, s, src, dst, E, labels);
+    return og_hip_check(og_launch_status(), "og_relabel_kernel");
 }
 
 extern "C" int og_launch_compose_labels(uint32_t *label_array, uint32_t *inverse, uint32_t N,
                                         const uint32_t *labels, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_compose_labels_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, label_array, inverse, N,
+    OG_LAUNCH(og_compose_labels_kernel, dim3(og_rgrid(N)), dim3(RB), 0, s, label_array, inverse, N,
                        labels);
-    return og_hip_check(hipGetLastError(), "og_compose_labels_kernel");
+    return og_hip_check(og_launch_status(), "og_compose_labels_kernel");
 }

@@ -39,6 +39,20 @@ int og_hip_check(hipError_t e, const char *what)
     return -1;
 }
 
+static __thread hipError_t g_launch_err = hipSuccess;
+
+void og_launch_note(hipError_t e)
+{
+    if (e != hipSuccess && g_launch_err == hipSuccess) g_launch_err = e;
+}
+
+hipError_t og_launch_status(void)
+{
+    hipError_t e = g_launch_err;
+    g_launch_err = hipSuccess;
+    return e;
+}
+
 static void og_ctx_release(void)
 {
     if (g_join) (void)hipEventDestroy(g_join);

@@ -84,15 +84,15 @@ __global__ __launch_bounds__(SB) void og_spmv_out_kernel(og_pr_dev a, uint32_t c
 extern "C" int og_launch_spmv_terms(const og_pr_dev *a, uint32_t code, void *terms, hipStream_t s)
 {
     const unsigned g = og_sgrid(a->N);
-    if (code < 2) hipLaunchKernelGGL(og_spmv_terms_f32_kernel, dim3(g), dim3(SB), 0, s, *a, (float *)terms);
+    if (code < 2) OG_LAUNCH(og_spmv_terms_f32_kernel, dim3(g), dim3(SB), 0, s, *a, (float *)terms);
     else if (code == 2)
-        hipLaunchKernelGGL(og_spmv_terms_fx32_kernel, dim3(g), dim3(SB), 0, s, *a, (uint32_t *)terms);
-    else hipLaunchKernelGGL(og_spmv_terms_fx64_kernel, dim3(g), dim3(SB), 0, s, *a, (uint64_t *)terms);
-    return og_hip_check(hipGetLastError(), "og_spmv_terms");
+        OG_LAUNCH(og_spmv_terms_fx32_kernel, dim3(g), dim3(SB), 0, s, *a, (uint32_t *)terms);
+    else OG_LAUNCH(og_spmv_terms_fx64_kernel, dim3(g), dim3(SB), 0, s, *a, (uint64_t *)terms);
+    return og_hip_check(og_launch_status(), "og_spmv_terms");
 }
 
 extern "C" int og_launch_spmv_out(const og_pr_dev *a, uint32_t code, float *out_old, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_spmv_out_kernel, dim3(og_sgrid(a->N)), dim3(SB), 0, s, *a, code, out_old);
-    return og_hip_check(hipGetLastError(), "og_spmv_out_kernel");
+    OG_LAUNCH(og_spmv_out_kernel, dim3(og_sgrid(a->N)), dim3(SB), 0, s, *a, code, out_old);
+    return og_hip_check(og_launch_status(), "og_spmv_out_kernel");
 }

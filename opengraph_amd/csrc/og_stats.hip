@@ -348,52 +348,52 @@ __global__ __launch_bounds__(SB) void og_topk_sums_kernel(og_stats_dev a)
 
 extern "C" int og_launch_stats_inverse(const og_stats_dev *a, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_stats_inverse_kernel, dim3(og_grid(a->N, SB)), dim3(SB), 0, s, a->order, a->ref_order,
+    OG_LAUNCH(og_stats_inverse_kernel, dim3(og_grid(a->N, SB)), dim3(SB), 0, s, a->order, a->ref_order,
                        a->N, a->inv, a->ref_inv);
-    return og_hip_check(hipGetLastError(), "og_stats_inverse_kernel");
+    return og_hip_check(og_launch_status(), "og_stats_inverse_kernel");
 }
 
 extern "C" int og_launch_stats_slices(const og_stats_dev *a, uint32_t total, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_stats_slices_kernel, dim3(og_grid(total, SB)), dim3(SB), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_stats_slices_kernel");
+    OG_LAUNCH(og_stats_slices_kernel, dim3(og_grid(total, SB)), dim3(SB), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_stats_slices_kernel");
 }
 
 extern "C" int og_launch_lev_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
 {
     static_assert(16 * 64 * 1 == OG_LEV_SMALL_K, "small edit-distance strips cover OG_LEV_SMALL_K");
-    if (n) hipLaunchKernelGGL((og_lev_strip_kernel<16, 1>), dim3(n), dim3(64), 0, s, *a, plist);
-    return og_hip_check(hipGetLastError(), "og_lev_strip_kernel<16, 1>");
+    if (n) OG_LAUNCH((og_lev_strip_kernel<16, 1>), dim3(n), dim3(64), 0, s, *a, plist);
+    return og_hip_check(og_launch_status(), "og_lev_strip_kernel<16, 1>");
 }
 
 extern "C" int og_launch_lev_large(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
 {
     static_assert(24 * 64 * 8 == OG_LEV_LDS_K, "large edit-distance strips cover OG_LEV_LDS_K");
-    if (n) hipLaunchKernelGGL((og_lev_strip_kernel<24, 8>), dim3(n), dim3(512), 0, s, *a, plist);
-    return og_hip_check(hipGetLastError(), "og_lev_strip_kernel<24, 8>");
+    if (n) OG_LAUNCH((og_lev_strip_kernel<24, 8>), dim3(n), dim3(512), 0, s, *a, plist);
+    return og_hip_check(og_launch_status(), "og_lev_strip_kernel<24, 8>");
 }
 
 extern "C" int og_launch_lev_global(const og_stats_dev *a, uint32_t p, uint32_t *scratch, hipStream_t s)
 {
-    hipLaunchKernelGGL(og_lev_global_kernel, dim3(1), dim3(OG_LEV_BLOCK), 0, s, *a, p, scratch);
-    return og_hip_check(hipGetLastError(), "og_lev_global_kernel");
+    OG_LAUNCH(og_lev_global_kernel, dim3(1), dim3(OG_LEV_BLOCK), 0, s, *a, p, scratch);
+    return og_hip_check(og_launch_status(), "og_lev_global_kernel");
 }
 
 extern "C" int og_launch_kendall_small(const og_stats_dev *a, const uint32_t *plist, uint32_t n, hipStream_t s)
 {
-    if (n) hipLaunchKernelGGL(og_kendall_small_kernel, dim3(n), dim3(SB), 0, s, *a, plist);
-    return og_hip_check(hipGetLastError(), "og_kendall_small_kernel");
+    if (n) OG_LAUNCH(og_kendall_small_kernel, dim3(n), dim3(SB), 0, s, *a, plist);
+    return og_hip_check(og_launch_status(), "og_kendall_small_kernel");
 }
 
 extern "C" int og_launch_kendall(const og_stats_dev *a, const void *groups, uint32_t n_groups, hipStream_t s)
 {
     if (n_groups)
-        hipLaunchKernelGGL(og_kendall_kernel, dim3(n_groups), dim3(SB), 0, s, *a, (const uint2 *)groups);
-    return og_hip_check(hipGetLastError(), "og_kendall_kernel");
+        OG_LAUNCH(og_kendall_kernel, dim3(n_groups), dim3(SB), 0, s, *a, (const uint2 *)groups);
+    return og_hip_check(og_launch_status(), "og_kendall_kernel");
 }
 
 extern "C" int og_launch_topk_sums(const og_stats_dev *a, hipStream_t s)
 {
-    if (a->P) hipLaunchKernelGGL(og_topk_sums_kernel, dim3(a->P), dim3(SB), 0, s, *a);
-    return og_hip_check(hipGetLastError(), "og_topk_sums_kernel");
+    if (a->P) OG_LAUNCH(og_topk_sums_kernel, dim3(a->P), dim3(SB), 0, s, *a);
+    return og_hip_check(og_launch_status(), "og_topk_sums_kernel");
 }

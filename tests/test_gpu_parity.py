@@ -296,11 +296,12 @@ def test_dropin_stats_contract(golden, capfd):
     title, epsilon, one row per iteration as it completes, the summary --
     identical text once the time column is blanked.  activeVertices is left
     uninitialized by the reference (newPageRankStatsGraphCSR never sets it);
-    here it is the last iteration's count.  The fp32 push (1, 5) adds with
-    atomics in no fixed order, as the reference's omp atomic does on many
-    threads, and the fp32 pull on LAW-enron sums rows above 16 in-edges as
-    fixed trees: there the rows must match in count and labels, active
-    counts within 0.2 % of N.  Their error_total rows are held to what the
+    here it is the last iteration's count.  The fp32 pull runs binned (every
+    row in CSR order), so it is held to the same text.  The fp32 push (1, 5)
+    adds with atomics in no fixed order, as the reference's omp atomic does
+    on many threads, and the data-driven pull on LAW-enron sums rows above 16
+    in-edges as fixed trees: there the rows must match in count and labels,
+    active counts within 0.2 % of N.  Their error_total rows are held to what the
     1e-5 relative bar on ranks implies: error_total = (1/N) sum |r' - r|, so
     two runs whose ranks differ by at most 1e-5 relative differ in it by at
     most 2e-5 times the mean unnormalized rank (the normalized rank sum)."""
@@ -328,7 +329,7 @@ def test_dropin_stats_contract(golden, capfd):
         assert f["damp"] == c["damp"] and f["base_pr"] == c["base_pr"], key
         assert f["num_vertices"] == c["num_vertices"] and f["realRanks_identity"] and c["realRanks_identity"]
         assert f["iterations"] == c["iterations"], key
-        loose = code in (1, 5) or (name == "enron" and code == 0)
+        loose = code in (1, 5) or (name == "enron" and code == 6)
         if not loose:
             assert f["error_total"] == pytest.approx(c["error_total"], rel=1e-9, abs=1e-300), key
             assert text == want, key
