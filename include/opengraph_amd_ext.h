@@ -263,6 +263,20 @@ int ogpu_mlp_train(ogpu_mlp *m, uint32_t epochs, uint32_t batch, float lr, uint6
  * features and n targets). */
 int ogpu_mlp_grad(ogpu_mlp *m, const float *feat, const float *y, uint32_t n, float *grad,
                   double *loss);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, training loops that forbid
+ * hipMalloc): ogpu_mlp_grad over device-resident rows -- d_feat (n x 4) and
+ * d_y (n) -- writing the mean gradient to d_grad (4545 floats) and the sum
+ * of the n losses |o - y| to *d_loss_sum (one double), all device
+ * pointers.  Its block partials live in a static device array of the
+ * library (4.4 MiB, held by each GPU the library runs on until the process
+ * ends), so the call allocates nothing.  At most OGPU_MLP_GRAD_NOALLOC_MAX
+ * rows per call (more is an error); n = 0 does nothing.  Enqueued on the
+ * library stream (ogpu_synchronize waits for it).  All calls on a GPU share
+ * that array: calls that overlap on one GPU race. */
+#define OGPU_MLP_GRAD_NOALLOC_MAX 16384u
+int ogpu_mlp_grad_noalloc(ogpu_mlp *m, const float *d_feat, const float *d_y, uint32_t n, float *d_grad,
+                          double *d_loss_sum);
 /* o for n feature rows (host arrays); mfma = 1: layer 2 on bf16 MFMA. */
 int ogpu_mlp_forward(ogpu_mlp *m, const float *feat, uint32_t n, int mfma, float *out);
 void ogpu_mlp_free(ogpu_mlp *m);

@@ -228,6 +228,7 @@ SIGNATURES = {
     "ogpu_mlp_train": (C.c_int, [_VP, u32, u32, C.c_float, C.c_uint64, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
     "ogpu_mlp_grad": (C.c_int, [_VP, _VP, _VP, u32, _VP, C.POINTER(C.c_double)]),
+    "ogpu_mlp_grad_noalloc": (C.c_int, [_VP, _VP, _VP, u32, _VP, _VP]),
     "ogpu_mlp_forward": (C.c_int, [_VP, _VP, u32, C.c_int, _VP]),
     "ogpu_mlp_free": (None, [_VP]),
     "ogpu_pr_features": (C.c_int, [_VP, _VP]),

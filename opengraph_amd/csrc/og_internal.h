@@ -26,6 +26,8 @@ const char *og_get_error(void);
  * serialize their calls, as the reference's single caller does. */
 int og_ctx_ensure(void);
 hipStream_t og_stream(void);
+/* the device the context was created on */
+int og_device(void);
 /* the second stream and the fork / join events of the pull (created with
  * the context) */
 hipStream_t og_side_stream(void);

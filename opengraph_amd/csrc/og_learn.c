@@ -262,6 +262,18 @@ out:
     return rc;
 }
 
+int ogpu_mlp_grad_noalloc(ogpu_mlp *m, const float *d_feat, const float *d_y, uint32_t n, float *d_grad,
+                          double *d_loss_sum)
+{
+    if (og_ctx_ensure()) return -1;
+    if (!m || !d_feat || !d_y || !d_grad || !d_loss_sum)
+    {
+        og_set_error("ogpu_mlp_grad_noalloc: NULL argument");
+        return -1;
+    }
+    return og_launch_mlp_grad_noalloc(og_device(), m->params, d_feat, d_y, n, d_grad, d_loss_sum, og_stream());
+}
+
 int ogpu_mlp_forward(ogpu_mlp *m, const float *feat, uint32_t n, int mfma, float *out)
 {
     hipStream_t s = og_stream();

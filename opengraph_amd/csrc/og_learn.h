@@ -48,6 +48,14 @@ int og_launch_mlp_grad(const float *params, const float *feat, const float *y, u
 size_t og_mlp_part_floats(uint32_t count);
 int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
                        float b2, float eps, uint32_t step, hipStream_t s);
+/* The no-allocation gradient: og_learn_noalloc_setup (once per device,
+ * with the device current, at context creation) finds the device address
+ * of the static partial array; og_launch_mlp_grad_noalloc runs
+ * og_launch_mlp_grad over n <= OGPU_MLP_GRAD_NOALLOC_MAX dataset rows
+ * (identity order) with its partials there. */
+int og_learn_noalloc_setup(int device);
+int og_launch_mlp_grad_noalloc(int device, const float *params, const float *feat, const float *y,
+                               uint32_t n, float *grad, double *loss_sum, hipStream_t s);
 /* Start ranks in old ids: 1 - d (unnormalized) for vertices without
  * in-edges, N * S_U * o[v] / sum_U(o) for the others, S_U from the
  * fixed-point mass identity (fp64 sums in a fixed order; scratch holds
@@ -79,7 +87,7 @@ static inline size_t og_wc_count(void) { return og_wc_mass() + OG_WC_MAX; }
 static inline size_t og_wc_matrix(void) { return OG_WC_BASE + OG_WC_BLOCKS * 2 * OG_WC_MAX; }
 static inline size_t og_wc_scale(void) { return og_wc_count() + OG_WC_MAX; }
 int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
-                         const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
+                         const uint32_t *new2old, uint32_t n, double *sums, float *r0_old,
                          hipStream_t s);
 
 #ifdef __cplusplus

@@ -637,6 +637,57 @@ extern "C" int og_launch_mlp_grad(const float *params, const float *feat, const 
     return og_hip_check(og_launch_status(), "og_mlp_grad_kernel");
 }
 
+/* ------------------------------------------------------------------ */
+/* gradient partials without an allocation                             */
+/* ------------------------------------------------------------------ */
+
+/* The block partials of ogpu_mlp_grad_noalloc: enough for
+ * OGPU_MLP_GRAD_NOALLOC_MAX vertices (4.4 MiB), in a static device array
+ * that each GPU the library runs on holds until the process ends.  Host
+ * code never passes `scratch` itself to a kernel (on the host it names a
+ * stand-in, not device memory): og_learn_noalloc_setup looks up its device
+ * address once per device, with that device current, and the launches use
+ * that address. */
+#define OG_NOALLOC_PART_FLOATS \
+    ((((size_t)OGPU_MLP_GRAD_NOALLOC_MAX + TB - 1) / TB) * PART_STRIDE)
+static __device__ __align__(256) float scratch[OG_NOALLOC_PART_FLOATS];
+
+#define OG_NOALLOC_DEVICES 64
+static float *g_scratch_ptr[OG_NOALLOC_DEVICES];
+
+extern "C" int og_learn_noalloc_setup(int device)
+{
+    if (device < 0 || device >= OG_NOALLOC_DEVICES)
+    {
+        og_set_error("gradient scratch: device %d out of range", device);
+        return -1;
+    }
+    if (g_scratch_ptr[device]) return 0;
+    void *p = NULL;
+    if (og_hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return -1;
+    g_scratch_ptr[device] = (float *)p;
+    return 0;
+}
+
+extern "C" int og_launch_mlp_grad_noalloc(int device, const float *params, const float *feat, const float *y,
+                                          uint32_t n, float *grad, double *loss_sum, hipStream_t s)
+{
+    if (n > OGPU_MLP_GRAD_NOALLOC_MAX)
+    {
+        og_set_error("ogpu_mlp_grad_noalloc: %u vertices, at most %u per call", n, OGPU_MLP_GRAD_NOALLOC_MAX);
+        return -1;
+    }
+    float *part = device >= 0 && device < OG_NOALLOC_DEVICES ? g_scratch_ptr[device] : NULL;
+    if (!part)
+    {
+        og_set_error("ogpu_mlp_grad_noalloc: no gradient scratch on device %d (library not initialized)", device);
+        return -1;
+    }
+    /* every partial the reduction reads is written by this call's gradient
+     * kernel first: nothing needs clearing */
+    return og_launch_mlp_grad(params, feat, y, n, 0, 0, n, part, grad, loss_sum, s);
+}
+
 extern "C" int og_launch_mlp_adam(float *params, const float *grad, float *m1, float *m2, float lr, float b1,
                                   float b2, float eps, uint32_t step, hipStream_t s)
 {
@@ -647,12 +698,12 @@ extern "C" int og_launch_mlp_adam(float *params, const float *grad, float *m1, f
 }
 
 extern "C" int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
-                                    const uint32_t *new2old, uint32_t n, double *scratch, float *r0_old,
+                                    const uint32_t *new2old, uint32_t n, double *sums, float *r0_old,
                                     hipStream_t s)
 {
-    OG_LAUNCH(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, row_ptr, outdeg, n, scratch);
-    OG_LAUNCH(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, scratch, n);
-    OG_LAUNCH(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, row_ptr, new2old, n, scratch,
+    OG_LAUNCH(og_warm_sums_kernel, dim3(WS_BLOCKS), dim3(LB), 0, s, o, row_ptr, outdeg, n, sums);
+    OG_LAUNCH(og_warm_factor_kernel, dim3(1), dim3(LB), 0, s, sums, n);
+    OG_LAUNCH(og_warm_write_kernel, dim3(og_lgrid(n)), dim3(LB), 0, s, o, row_ptr, new2old, n, sums,
                        r0_old);
     return og_hip_check(og_launch_status(), "og_warm_scale");
 }

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include "og_internal.h"
+#include "og_learn.h"
 #include "opengraph_amd_ext.h"
 
 static __thread char g_err[512];
@@ -83,13 +84,16 @@ static int og_ctx_create(void)
     if (og_hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate") ||
         og_hip_check(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking), "hipStreamCreate (side)") ||
         og_hip_check(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming), "hipEventCreate (fork)") ||
-        og_hip_check(hipEventCreateWithFlags(&g_join, hipEventDisableTiming), "hipEventCreate (join)"))
+        og_hip_check(hipEventCreateWithFlags(&g_join, hipEventDisableTiming), "hipEventCreate (join)") ||
+        og_learn_noalloc_setup(g_device))
     {
         og_ctx_release();
         return -1;
     }
     return 0;
 }
+
+int og_device(void) { return g_device; }
 
 int og_ctx_ensure(void)
 {
