@@ -5,7 +5,8 @@
  * Host: the rows are packed into bins (from the plan's row offsets), the
  * rows of more than OG_BIN_CAP in-edges are cut into pieces, and, once the
  * device has sorted the edges into tiles, the tiles are ordered segment by
- * segment and walked in rounds of 64 entries for the gather pass.
+ * segment (the gather's source order) and every quad of 16 sources gets
+ * its run of entries.
  * Device: the (bin, segment) keys, their stable radix sort, the entries'
  * source and slot indices, and the tile boundaries.
  */
@@ -17,12 +18,12 @@
 #include "og_internal.h"
 #include "og_plan.h"
 
-#define OG_BIN_ZERO_ROWS 16384u   /* rows of a bin of rows without in-edges */
+#define OG_BIN_ZERO_ROWS (OG_BIN_RPT * OG_BIN_THREADS)   /* rows of a bin of rows without in-edges */
 
 void og_bin_plan_destroy(og_bin_plan *b)
 {
     if (!b) return;
-    og_dfree(b->mem_items); og_dfree(b->mem_rounds); og_dfree(b->mem_tile_ent); og_dfree(b->mem_src);
+    og_dfree(b->mem_items); og_dfree(b->mem_qent); og_dfree(b->mem_src);
     og_dfree(b->mem_pos); og_dfree(b->mem_val); og_dfree(b->mem_bins); og_dfree(b->mem_long_row);
     og_dfree(b->mem_long_off); og_dfree(b->mem_vlong);
     free(b);
@@ -83,60 +84,52 @@ static void *og_bin_upload(const void *h, size_t bytes)
 
 /* Host check of every index the binned kernels derive from the layout
  * (enabled by the environment variable OG_BIN_VERIFY): items within the
- * rounds, every lane's entry of every round within the entry arrays, each
- * real entry's source within its segment and slot within its bin, bins and
- * pieces within the entries, CSR positions and vlong.  Runs before any
- * binned kernel, so a bad layout is an error, not a stray access. */
-static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t *rp, uint32_t n_tile_ent)
+ * sources, every source's entry within the entries, each source within its
+ * segment, each entry gathered once and its source the edge's (the CSR's
+ * column at the entry's slot),
+ * slots within their bins, bins and pieces within the entries, rows per bin,
+ * CSR positions and vlong.  Runs before any binned kernel, so a bad layout
+ * is an error, not a stray access. */
+static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t *rp)
 {
     const og_bin_dev *d = &b->d;
-    const size_t ne = (size_t)b->n_ent;
+    const size_t ne = (size_t)b->n_ent, nq = (ne - 64) / OG_BIN_TILE_ALIGN;
     og_bin_item *items = (og_bin_item *)malloc((size_t)d->n_items * sizeof(og_bin_item) + 16);
-    og_bin_round *rounds = (og_bin_round *)malloc((size_t)b->n_rounds * sizeof(og_bin_round) + 16);
-    uint32_t *tent = (uint32_t *)malloc((size_t)n_tile_ent * 4 + 4);
+    uint32_t *qent = (uint32_t *)malloc(nq * 4 + 4);
     uint16_t *src = (uint16_t *)malloc(ne * 2), *pos = (uint16_t *)malloc(ne * 2);
     og_bin_desc *bins = (og_bin_desc *)malloc((size_t)d->n_bins * sizeof(og_bin_desc) + 16);
     uint32_t *lrow = (uint32_t *)malloc((size_t)d->n_long * 4 + 4), *loff = (uint32_t *)malloc((size_t)d->n_long * 4 + 8);
+    uint32_t *gsrc = (uint32_t *)malloc(ne * 4);   /* source id gathered into each entry */
+    uint32_t *cols = (uint32_t *)malloc((size_t)pl->E * 4 + 4);
     uint8_t *seen = (uint8_t *)calloc(ne, 1);
     int rc = -1;
 #define OG_BIN_FAIL(...) do { og_set_error("binned layout check: " __VA_ARGS__); goto out; } while (0)
-    if (!items || !rounds || !tent || !src || !pos || !bins || !lrow || !loff || !seen)
+    if (!items || !qent || !src || !pos || !bins || !lrow || !loff || !seen || !gsrc || !cols)
         OG_BIN_FAIL("out of host memory");
     if (og_hip_check(og_copy(items, d->items, (size_t)d->n_items * sizeof(og_bin_item), hipMemcpyDeviceToHost), "D2H") ||
-        og_hip_check(og_copy(rounds, d->rounds, (size_t)b->n_rounds * sizeof(og_bin_round), hipMemcpyDeviceToHost), "D2H") ||
-        og_hip_check(og_copy(tent, d->tile_ent, (size_t)n_tile_ent * 4, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(qent, d->qent, nq * 4, hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(src, d->src, ne * 2, hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(pos, d->pos, ne * 2, hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(bins, d->bins, (size_t)d->n_bins * sizeof(og_bin_desc), hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(lrow, d->long_row, (size_t)d->n_long * 4, hipMemcpyDeviceToHost), "D2H") ||
-        og_hip_check(og_copy(loff, d->long_off, ((size_t)d->n_long + 1) * 4, hipMemcpyDeviceToHost), "D2H"))
+        og_hip_check(og_copy(loff, d->long_off, ((size_t)d->n_long + 1) * 4, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(cols, pl->cols, (size_t)pl->E * 4, hipMemcpyDeviceToHost), "D2H"))
         goto out;
     for (uint32_t q = 0; q < d->n_items; q++)
     {
         const og_bin_item it = items[q];
-        if (it.seg >= d->n_seg || it.r0 >= it.r1 || it.r1 > b->n_rounds) OG_BIN_FAIL("item %u", q);
+        if (it.seg >= d->n_seg || it.f0 >= it.f1 || it.f1 > nq * OG_BIN_TILE_ALIGN || it.f0 % OG_BIN_TILE_ALIGN ||
+            it.f1 % OG_BIN_TILE_ALIGN)
+            OG_BIN_FAIL("item %u", q);
         const uint32_t nv = d->N - it.seg * OG_BIN_SEG < OG_BIN_SEG ? d->N - it.seg * OG_BIN_SEG : OG_BIN_SEG;
-        for (uint32_t r = it.r0; r < it.r1; r++)
+        for (uint32_t f = it.f0; f < it.f1; f++)
         {
-            const og_bin_round rd = rounds[r];
-            const uint64_t mask = ((uint64_t)rd.mask_hi << 32) | rd.mask_lo;
-            if (mask & 1) OG_BIN_FAIL("round %u: lane 0 starts a tile", r);
-            for (uint32_t k = 0; k < 64; k++)
-            {
-                const uint64_t low = mask & (k == 63 ? ~0ull : ((2ull << k) - 1));
-                const uint32_t c = (uint32_t)__builtin_popcountll(low);
-                const uint64_t t = (uint64_t)rd.first + c;
-                if (t >= n_tile_ent) OG_BIN_FAIL("round %u lane %u: tile %llu of %u", r, k, (unsigned long long)t, n_tile_ent);
-                const uint64_t o = c ? k - (63u - (uint32_t)__builtin_clzll(low)) : (uint64_t)rd.off0 + k;
-                const uint64_t e = (uint64_t)tent[t] + o;
-                if (e >= ne) OG_BIN_FAIL("round %u lane %u: entry %llu of %zu", r, k, (unsigned long long)e, ne);
-                if (e < ne - 64)
-                {
-                    if (src[e] >= nv) OG_BIN_FAIL("entry %llu: source %u past segment size %u", (unsigned long long)e, src[e], nv);
-                    if (seen[e]) OG_BIN_FAIL("entry %llu gathered twice", (unsigned long long)e);
-                    seen[e] = 1;
-                }
-            }
+            const uint64_t e = (uint64_t)qent[f / OG_BIN_TILE_ALIGN] + f % OG_BIN_TILE_ALIGN;
+            if (e >= ne - 64) OG_BIN_FAIL("source %u: entry %llu of %zu", f, (unsigned long long)e, ne - 64);
+            if (src[f] >= nv) OG_BIN_FAIL("source %u: %u past segment size %u", f, src[f], nv);
+            if (seen[e]) OG_BIN_FAIL("entry %llu gathered twice", (unsigned long long)e);
+            seen[e] = 1;
+            gsrc[e] = it.seg * OG_BIN_SEG + src[f];
         }
     }
     uint64_t prev_end = 0;
@@ -155,7 +148,8 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
         }
         else
         {
-            if (bd.row0 > bd.row1 || bd.row1 > d->N) OG_BIN_FAIL("bin %u rows", q);
+            if (bd.row0 > bd.row1 || bd.row1 > d->N || bd.row1 - bd.row0 > OG_BIN_RPT * OG_BIN_THREADS)
+                OG_BIN_FAIL("bin %u rows", q);
             if (bd.cnt && (rp[bd.row0] != bd.pos0 || rp[bd.row1] - rp[bd.row0] != bd.cnt)) OG_BIN_FAIL("bin %u row offsets", q);
         }
         uint32_t real = 0;
@@ -164,6 +158,8 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
             if (pos[e] != OG_BIN_PAD)
             {
                 if (pos[e] >= bd.cnt) OG_BIN_FAIL("entry %u: slot %u in bin %u of %u", e, pos[e], q, bd.cnt);
+                if (seen[e] && gsrc[e] != cols[bd.pos0 + pos[e]])
+                    OG_BIN_FAIL("entry %u: gathers source %u, the edge's is %u", e, gsrc[e], cols[bd.pos0 + pos[e]]);
                 real++;
             }
             if (!seen[e]) OG_BIN_FAIL("entry %u never gathered", e);
@@ -177,7 +173,8 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
     rc = 0;
 #undef OG_BIN_FAIL
 out:
-    free(items); free(rounds); free(tent); free(src); free(pos); free(bins); free(lrow); free(loff); free(seen);
+    free(items); free(qent); free(src); free(pos); free(bins); free(lrow); free(loff); free(seen);
+    free(gsrc); free(cols);
     return rc;
 }
 
@@ -200,11 +197,11 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
     og_vec bins = {NULL, 0, 0, sizeof(og_bin_desc)};
     uint32_t *long_rows = NULL, *long_list = NULL, *long_voff_rowordered = NULL, *long_off = NULL;
     uint32_t *tseg = NULL, *tbin = NULL, *tstart = NULL, *tcnt = NULL, *tent = NULL, *order = NULL;
-    uint32_t *seg_cnt = NULL, *tile_ent2 = NULL;
-    og_vec rounds = {NULL, 0, 0, sizeof(og_bin_round)};
+    uint32_t *seg_cnt = NULL, *tsrc = NULL, *seg_src = NULL, *qent = NULL;
     og_vec items = {NULL, 0, 0, sizeof(og_bin_item)};
     uint32_t *key_a = NULL, *val_a = NULL, *key_b = NULL, *val_b = NULL;
     void *scratch = NULL, *d_tseg = NULL, *d_tbin = NULL, *d_tstart = NULL, *d_tent = NULL, *d_tcnt = NULL;
+    void *d_tsrc = NULL;
     hipStream_t s = og_stream();
     int rc = -1;
     if (!b || !rp)
@@ -261,7 +258,8 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
 
         /* ---- bins: consecutive rows of one kind (thread rows, wavefront
          * rows of more than OG_BIN_WAVE_ROW in-edges) with at most
-         * OG_BIN_CAP in-edges together; pieces of the long rows ---- */
+         * OG_BIN_CAP in-edges and OG_BIN_RPT * OG_BIN_THREADS rows together;
+         * pieces of the long rows ---- */
         uint32_t li = 0;
         uint32_t r = 0;
         while (r < Z)
@@ -293,7 +291,8 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             {
                 const uint32_t dr = rp[r + 1] - rp[r];
                 const uint32_t kr = dr > OG_BIN_WAVE_ROW ? OG_BIN_WAVE_ROWS : OG_BIN_THREAD_ROWS;
-                if (kr != kind || dr > OG_BIN_CAP || tot + dr > OG_BIN_CAP) break;
+                if (kr != kind || dr > OG_BIN_CAP || tot + dr > OG_BIN_CAP || r - r0 == OG_BIN_RPT * OG_BIN_THREADS)
+                    break;
                 tot += dr;
                 r++;
             }
@@ -368,8 +367,9 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         tent = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
         order = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
         seg_cnt = (uint32_t *)calloc((size_t)n_seg + 1, 4);
-        tile_ent2 = (uint32_t *)malloc(((size_t)n_tiles + n_seg) * 4 + 4);
-        if (!tseg || !tbin || !tstart || !tcnt || !tent || !order || !seg_cnt || !tile_ent2)
+        tsrc = (uint32_t *)malloc((size_t)n_tiles * 4 + 4);
+        seg_src = (uint32_t *)malloc(((size_t)n_seg + 1) * 4);
+        if (!tseg || !tbin || !tstart || !tcnt || !tent || !order || !seg_cnt || !tsrc || !seg_src)
         {
             og_set_error("out of host memory");
             goto done;
@@ -401,32 +401,10 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         }
         if (bprev != 0xFFFFFFFFu) hb[bprev].ent1 = (uint32_t)ent;
         for (uint32_t q = n_edge_bins; q < bins.n; q++) hb[q].ent0 = hb[q].ent1 = (uint32_t)ent;
-        b->n_ent = ent + 64;   /* 64 slack entries for the rounds past a segment's last tile */
-        b->mem_src = og_dalloc(b->n_ent * 2);
-        b->mem_pos = og_dalloc(b->n_ent * 2);
-        b->mem_val = og_dalloc(b->n_ent * 4);
-        b->mem_long_row = og_bin_upload(long_list, (size_t)n_long * 4);
-        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
-        b->mem_vlong = og_dalloc(voff * 4);
-        d_tent = og_bin_upload(tent, (size_t)n_tiles * 4);
-        d_tcnt = og_bin_upload(tcnt, (size_t)n_tiles * 4);
-        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off || !b->mem_vlong ||
-            !d_tent || !d_tcnt ||
-            og_hip_check(og_copy(d_bins, hb, bins.n * sizeof(og_bin_desc), hipMemcpyHostToDevice), "H2D bins"))
-            goto done;
-        if (og_launch_bin_fill(K, V, E, d_bins, n_seg, pl->cols, fscan, (const uint32_t *)d_tstart,
-                               (const uint32_t *)d_tent, (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
-            og_launch_bin_pads((const uint32_t *)d_tent, (const uint32_t *)d_tcnt, n_tiles, (uint16_t *)b->mem_src,
-                               (uint16_t *)b->mem_pos, s) ||
-            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack") ||
-            og_hip_check(hipStreamSynchronize(s), "sync bin layout"))
-            goto done;
-        og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
-        key_a = val_a = key_b = val_b = NULL;
-        scratch = NULL;
-
-        /* ---- host: tiles segment by segment (bins ascending), walked in
-         * rounds of 64 entries (padding included), work items ---- */
+        b->n_ent = ent + 64;   /* 64 spare entries past the last tile */
+        b->d.n_ent = (uint32_t)ent;
+        /* the tiles segment by segment (bins ascending), and each tile's
+         * first entry in that order: where the gather reads its sources */
         for (uint32_t t = 0; t < n_tiles; t++) seg_cnt[tseg[t] + 1]++;
         for (uint32_t q = 0; q < n_seg; q++) seg_cnt[q + 1] += seg_cnt[q];
         {
@@ -436,71 +414,71 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             for (uint32_t t = 0; t < n_tiles; t++) order[fill[tseg[t]]++] = t;
             free(fill);
         }
-        const uint32_t slack = (uint32_t)ent;
-        uint32_t nt2 = 0;
-        for (uint32_t q = 0; q < n_seg; q++)
         {
-            const uint32_t t0 = seg_cnt[q], m = seg_cnt[q + 1] - t0;
-            if (!m) continue;
-            const uint32_t base = nt2;
-            uint64_t tot = 0;
-            for (uint32_t i = 0; i < m; i++)
+            uint64_t f = 0;
+            for (uint32_t q = 0; q < n_seg; q++)
             {
-                const uint32_t t = order[t0 + i];
-                tile_ent2[nt2++] = tent[t];
-                tcnt[t] = (tcnt[t] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
-                tot += tcnt[t];
-            }
-            tile_ent2[nt2++] = slack;                  /* the segment's end tile */
-            const uint32_t R = (uint32_t)((tot + 63) / 64);
-            const uint32_t r_first = (uint32_t)rounds.n;
-            uint32_t kt = 0;      /* tile of lane 0 */
-            uint64_t Fk = 0;      /* its first flat position */
-            for (uint32_t rr = 0; rr < R; rr++)
-            {
-                const uint64_t f0 = 64ull * rr;
-                while (kt < m && Fk + tcnt[order[t0 + kt]] <= f0)
+                seg_src[q] = (uint32_t)f;
+                for (uint32_t i = seg_cnt[q]; i < seg_cnt[q + 1]; i++)
                 {
-                    Fk += tcnt[order[t0 + kt]];
-                    kt++;
+                    tsrc[order[i]] = (uint32_t)f;
+                    f += (tcnt[order[i]] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
                 }
-                og_bin_round *rd = (og_bin_round *)og_vec_push(&rounds);
-                if (!rd) { og_set_error("out of host memory"); goto done; }
-                uint64_t mask = 0;
-                uint32_t kk = kt;
-                uint64_t Fkk = Fk;
-                while (kk < m)
-                {
-                    const uint64_t nxt = Fkk + tcnt[order[t0 + kk]];
-                    if (nxt >= f0 + 64) break;
-                    mask |= 1ull << (nxt - f0);
-                    Fkk = nxt;
-                    kk++;
-                }
-                rd->first = base + kt;
-                rd->off0 = (uint32_t)(f0 - Fk);
-                rd->mask_lo = (uint32_t)mask;
-                rd->mask_hi = (uint32_t)(mask >> 32);
             }
-            for (uint32_t r0 = r_first; r0 < r_first + R; r0 += OG_BIN_ITEM_ROUNDS)
+            seg_src[n_seg] = (uint32_t)f;   /* == ent */
+        }
+        b->mem_src = og_dalloc(b->n_ent * 2);
+        b->mem_pos = og_dalloc(b->n_ent * 2);
+        b->mem_val = og_dalloc(b->n_ent * 4);
+        b->mem_long_row = og_bin_upload(long_list, (size_t)n_long * 4);
+        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
+        b->mem_vlong = og_dalloc(voff * 4);
+        d_tent = og_bin_upload(tent, (size_t)n_tiles * 4);
+        d_tcnt = og_bin_upload(tcnt, (size_t)n_tiles * 4);
+        d_tsrc = og_bin_upload(tsrc, (size_t)n_tiles * 4);
+        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off || !b->mem_vlong ||
+            !d_tent || !d_tcnt || !d_tsrc ||
+            og_hip_check(og_copy(d_bins, hb, bins.n * sizeof(og_bin_desc), hipMemcpyHostToDevice), "H2D bins"))
+            goto done;
+        if (og_launch_bin_fill(K, V, E, d_bins, n_seg, pl->cols, fscan, (const uint32_t *)d_tstart,
+                               (const uint32_t *)d_tent, (const uint32_t *)d_tsrc, (uint16_t *)b->mem_src,
+                               (uint16_t *)b->mem_pos, s) ||
+            og_launch_bin_pads((const uint32_t *)d_tent, (const uint32_t *)d_tsrc, (const uint32_t *)d_tcnt, n_tiles,
+                               (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
+            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack") ||
+            og_hip_check(hipStreamSynchronize(s), "sync bin layout"))
+            goto done;
+        og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
+        key_a = val_a = key_b = val_b = NULL;
+        scratch = NULL;
+
+        /* ---- host: each quad's first entry (a tile's padded sources are
+         * whole quads), and the work items: each segment's sources in runs
+         * of at most OG_BIN_ITEM_SRC ---- */
+        const uint64_t n_quads = ent / OG_BIN_TILE_ALIGN;
+        qent = (uint32_t *)malloc(n_quads * 4 + 4);
+        if (!qent) { og_set_error("out of host memory"); goto done; }
+        for (uint32_t t = 0; t < n_tiles; t++)
+        {
+            const uint32_t P = (tcnt[t] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
+            for (uint32_t k = 0; k < P; k += OG_BIN_TILE_ALIGN) qent[(tsrc[t] + k) / OG_BIN_TILE_ALIGN] = tent[t] + k;
+        }
+        for (uint32_t q = 0; q < n_seg; q++)
+            for (uint32_t f0 = seg_src[q]; f0 < seg_src[q + 1]; f0 += OG_BIN_ITEM_SRC)
             {
                 og_bin_item *it = (og_bin_item *)og_vec_push(&items);
                 if (!it) { og_set_error("out of host memory"); goto done; }
                 it->seg = q;
-                it->r0 = r0;
-                it->r1 = r_first + R - r0 < OG_BIN_ITEM_ROUNDS ? r_first + R : r0 + OG_BIN_ITEM_ROUNDS;
+                it->f0 = f0;
+                it->f1 = seg_src[q + 1] - f0 < OG_BIN_ITEM_SRC ? seg_src[q + 1] : f0 + OG_BIN_ITEM_SRC;
                 it->pad = 0;
             }
-        }
-        b->n_rounds = (uint32_t)rounds.n;
-        b->mem_tile_ent = og_bin_upload(tile_ent2, (size_t)nt2 * 4);
-        b->mem_rounds = og_bin_upload(rounds.p, rounds.n * sizeof(og_bin_round));
+        b->mem_qent = og_bin_upload(qent, n_quads * 4);
         b->mem_items = og_bin_upload(items.p, items.n * sizeof(og_bin_item));
-        if (!b->mem_tile_ent || !b->mem_rounds || !b->mem_items) goto done;
+        if (!b->mem_qent || !b->mem_items) goto done;
         b->d.n_items = (uint32_t)items.n;
         b->d.items = (const og_bin_item *)b->mem_items;
-        b->d.rounds = (const og_bin_round *)b->mem_rounds;
-        b->d.tile_ent = (const uint32_t *)b->mem_tile_ent;
+        b->d.qent = (const uint32_t *)b->mem_qent;
         b->d.src = (const uint16_t *)b->mem_src;
         b->d.pos = (const uint16_t *)b->mem_pos;
         b->d.val = (float *)b->mem_val;
@@ -508,16 +486,16 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         b->d.long_row = (const uint32_t *)b->mem_long_row;
         b->d.long_off = (const uint32_t *)b->mem_long_off;
         b->d.vlong = (float *)b->mem_vlong;
-        if (getenv("OG_BIN_VERIFY") && og_bin_verify(b, pl, rp, nt2)) goto done;
+        if (getenv("OG_BIN_VERIFY") && og_bin_verify(b, pl, rp)) goto done;
     }
     rc = 0;
 done:
     (void)hipStreamSynchronize(s);
     og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
-    og_dfree(d_tseg); og_dfree(d_tbin); og_dfree(d_tstart); og_dfree(d_tent); og_dfree(d_tcnt);
+    og_dfree(d_tseg); og_dfree(d_tbin); og_dfree(d_tstart); og_dfree(d_tent); og_dfree(d_tcnt); og_dfree(d_tsrc);
     free(rp); free(bins.p); free(long_rows); free(long_list); free(long_voff_rowordered); free(long_off);
-    free(tseg); free(tbin); free(tstart); free(tcnt); free(tent); free(order); free(seg_cnt); free(tile_ent2);
-    free(rounds.p); free(items.p);
+    free(tseg); free(tbin); free(tstart); free(tcnt); free(tent); free(order); free(seg_cnt);
+    free(items.p); free(tsrc); free(seg_src); free(qent);
     if (rc) og_bin_plan_destroy(b);
     else *out = b;
     return rc;

@@ -8,23 +8,31 @@
  * The edges of bin b whose source lies in segment s form tile (b, s).
  * Entries (one per edge) are stored bin by bin, within a bin tile by tile
  * in segment order, within a tile in CSR order:
- *   src[e]  u16  source id minus the segment's first id
  *   pos[e]  u16  the edge's CSR position minus the bin's first position
  *               (OG_BIN_PAD: a padding entry; each tile is padded to
  *               OG_BIN_TILE_ALIGN entries, so every tile's values fill whole
  *               64-byte sectors, written by one block)
  *   val[e]  f32  the gathered contribution, written each iteration
+ * The sources are stored in the gather's order instead -- segment by
+ * segment, within a segment tile by tile (bins ascending), each tile padded
+ * the same way -- so every aligned run of 16 of them (a quad) lies in one
+ * tile, and its entries are one run of 16:
+ *   src[f]       u16  source id minus the segment's first id
+ *   qent[f / 16] u32  the entry of the quad's first source
  * One iteration:
  *   gather pass  one 1024-thread block per CU: per work item (a segment and
- *                a run of its tiles), the segment's contributions are copied
- *                into LDS, then val[e] = lds[src[e]] for the item's entries,
- *                walked 64 at a time in "rounds" (segment-major order).
+ *                a run of its sources), the segment's contributions are
+ *                copied into LDS, then val[qent[f / 16] + f % 16] =
+ *                lds[src[f]] for the item's sources, 64 per wavefront and
+ *                step, two independent streaming loads per value.
  *   sum pass     one 1024-thread block per CU: per bin, val[e] is written
  *                to LDS slot pos[e] (the bin's in-edge values in CSR order),
  *                then every row is added from its slots in order -- the
  *                reference's running sum -- by a thread (rows of at most
  *                OG_BIN_WAVE_ROW in-edges) or a wavefront (longer rows, an
  *                exact fast-forward over binades), and the update runs.
+ *                The next bin's entries and row metadata are loaded into
+ *                registers while a bin's rows are added.
  *   long rows    rows with more than OG_BIN_CAP in-edges are cut into
  *                pieces of OG_BIN_CAP; the sum pass writes each piece's
  *                values in CSR order to vlong, and one block per long row
@@ -51,8 +59,9 @@ extern "C" {
 #define OG_BIN_CAP 39936u      /* in-edge values per bin: 156 KB of LDS */
 #define OG_BIN_PAD 0xFFFFu     /* pos of a padding entry */
 #define OG_BIN_WAVE_ROW 4096u  /* rows with more in-edges are added by a wavefront */
+#define OG_BIN_RPT 2u          /* rows per thread and bin: a bin holds at most RPT * THREADS rows */
 #define OG_BIN_TILE_ALIGN 16u  /* tiles padded to 16 entries: 64-byte value runs */
-#define OG_BIN_ITEM_ROUNDS 2048u /* rounds (64 entries) per gather work item at most */
+#define OG_BIN_ITEM_SRC (2048u * 64u) /* sources per gather work item at most */
 #define OG_BIN_THREADS 1024u
 
 enum { OG_BIN_THREAD_ROWS = 0, OG_BIN_WAVE_ROWS = 1, OG_BIN_PIECE = 2 };
@@ -67,27 +76,20 @@ typedef struct
     uint32_t pos0, kind, vbase, cnt;
 } og_bin_desc;
 
-/* Gather work item: segment seg, rounds [r0, r1). */
+/* Gather work item: segment seg, sources [f0, f1) (f0 and f1 multiples
+ * of 16, within the segment's sources). */
 typedef struct
 {
-    uint32_t seg, r0, r1, pad;
+    uint32_t seg, f0, f1, pad;
 } og_bin_item;
-
-/* Round of 64 entries of one segment: lane 0's entry belongs to tile
- * `first` at offset off0; mask bit k (k > 0) is set when lane k starts the
- * next tile. */
-typedef struct
-{
-    uint32_t first, off0, mask_lo, mask_hi;
-} og_bin_round;
 
 /* Everything the binned kernels read, by value. */
 typedef struct og_bin_dev
 {
     uint32_t N, n_seg, n_items, n_bins, n_bins_live, n_long;
+    uint32_t n_ent;              /* entries in tiles; 64 spare ones follow */
     const og_bin_item *items;
-    const og_bin_round *rounds;
-    const uint32_t *tile_ent;    /* segment-major tiles (with one end tile per segment) */
+    const uint32_t *qent;        /* n_ent / 16 */
     const uint16_t *src, *pos;
     float *val;
     const og_bin_desc *bins;
@@ -99,11 +101,11 @@ typedef struct og_bin_dev
 typedef struct og_bin_plan
 {
     og_bin_dev d;
-    uint64_t n_ent;              /* entries including padding and the end tiles' slack */
-    uint32_t n_tiles, n_rounds, n_zero_bins;
+    uint64_t n_ent;              /* entries (and sources) including padding and the spares */
+    uint32_t n_tiles, n_zero_bins;
     uint64_t n_vlong;
     /* device allocations */
-    void *mem_items, *mem_rounds, *mem_tile_ent, *mem_src, *mem_pos, *mem_val, *mem_bins;
+    void *mem_items, *mem_qent, *mem_src, *mem_pos, *mem_val, *mem_bins;
     void *mem_long_row, *mem_long_off, *mem_vlong;
 } og_bin_plan;
 
@@ -132,14 +134,15 @@ int og_launch_bin_flags(const uint32_t *key, uint32_t n, const og_bin_desc *bins
  * sorted entry */
 int og_launch_bin_tiles(const uint32_t *key, uint32_t n, uint32_t n_seg, const uint32_t *tile_flag_scan,
                         uint32_t *tile_seg, uint32_t *tile_bin, uint32_t *tile_start, hipStream_t s);
-/* entry of sorted i: tile_ent[t] + (i - tile_start[t]), t its tile; src and
- * pos there; then the padding entries of every tile (src 0, pos PAD) */
+/* sorted entry i of tile t: pos at tile_ent[t] + (i - tile_start[t]), src
+ * at tile_src[t] + (i - tile_start[t]); then the padding entries of every
+ * tile (src 0, pos PAD) */
 int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
                        uint32_t n_seg, const uint32_t *cols, const uint32_t *tile_flag_scan,
-                       const uint32_t *tile_start, const uint32_t *tile_ent, uint16_t *src, uint16_t *pos,
-                       hipStream_t s);
-int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_cnt, uint32_t n_tiles, uint16_t *src,
-                       uint16_t *pos, hipStream_t s);
+                       const uint32_t *tile_start, const uint32_t *tile_ent, const uint32_t *tile_src,
+                       uint16_t *src, uint16_t *pos, hipStream_t s);
+int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_src, const uint32_t *tile_cnt,
+                       uint32_t n_tiles, uint16_t *src, uint16_t *pos, hipStream_t s);
 
 #ifdef __cplusplus
 }

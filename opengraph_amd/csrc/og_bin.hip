@@ -25,7 +25,7 @@
 
 #define BT OG_BIN_THREADS
 #define BW (OG_BIN_THREADS / 64)
-#define GATHER_U 16u  /* rounds of one wavefront in flight */
+#define GATHER_U 16u  /* values per lane in flight in the gather */
 
 /* ------------------------------------------------------------------ */
 /* shared pieces                                                       */
@@ -41,16 +41,19 @@ __device__ __forceinline__ float og_bin_lane(float x, uint32_t k)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
 }
 
-/* inclusive prefix sum over the wavefront */
+/* Inclusive prefix sum over the wavefront with DPP lane moves: shifts by
+ * 1, 2, 4 and 8 within each row of 16 lanes (lanes shifted in from outside
+ * the row read 0), then row 1 and row 3 add the last lane of the row below
+ * (row_bcast:15), then rows 2 and 3 add lane 31 (row_bcast:31).  Every
+ * disabled lane adds 0. */
 __device__ __forceinline__ uint32_t og_bin_wave_scan(uint32_t v)
 {
-    const uint32_t lane = threadIdx.x & 63;
-#pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1)
-    {
-        const uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);   /* row_shr:1 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);   /* row_shr:2 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);   /* row_shr:4 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);   /* row_shr:8 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  /* row_bcast:15 */
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  /* row_bcast:31 */
     return v;
 }
 
@@ -116,6 +119,21 @@ __device__ float og_bin_wave_sum(const float *x, uint32_t n)
 /* gather pass                                                         */
 /* ------------------------------------------------------------------ */
 
+/* The sources and quad entries of this lane's GATHER_U values from f (64
+ * apart), branch-free (indices clamped to the item's last source) so that
+ * all the loads issue before any is waited for. */
+__device__ __forceinline__ void og_bin_gather_batch(const og_bin_dev &b, uint32_t f, uint32_t f1,
+                                                    uint32_t (&sv)[GATHER_U], uint32_t (&qv)[GATHER_U])
+{
+#pragma unroll
+    for (uint32_t u = 0; u < GATHER_U; u++)
+    {
+        const uint32_t fu = min(f + 64u * u, f1 - 1);
+        sv[u] = __builtin_nontemporal_load(&b.src[fu]);
+        qv[u] = __builtin_nontemporal_load(&b.qent[fu / OG_BIN_TILE_ALIGN]);
+    }
+}
+
 __global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const float *__restrict__ c,
                                                            const uint32_t *done)
 {
@@ -123,8 +141,6 @@ __global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const f
     if (*done) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    /* lane k's mask of the lanes at or below it */
-    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     for (uint32_t q = blockIdx.x; q < b.n_items; q += gridDim.x)
     {
         const og_bin_item it = b.items[q];
@@ -138,27 +154,36 @@ __global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const f
             for (uint32_t i = (nv & ~3u) + threadIdx.x; i < nv; i += BT) seg[i] = c[v0 + i];
         }
         __syncthreads();
-        for (uint32_t r = it.r0 + wave * GATHER_U; r < it.r1; r += BW * GATHER_U)
+        /* wavefront w takes values f = f0 + 64 * (w * U + k * BW * U + u) +
+         * lane; software pipelined: the next batch's loads are issued
+         * before this batch's stores (loads complete in order, and a wait
+         * for this batch's loads must not also wait for stores issued after
+         * them) */
+        uint32_t f = it.f0 + 64u * GATHER_U * wave + lane;
+        uint32_t sv[GATHER_U], qv[GATHER_U];
+        og_bin_gather_batch(b, f, it.f1, sv, qv);
+        for (; f < it.f1; f += 64u * GATHER_U * BW)
         {
-            uint32_t ent[GATHER_U];
+            uint32_t sv2[GATHER_U], qv2[GATHER_U];
+            og_bin_gather_batch(b, f + 64u * GATHER_U * BW, it.f1, sv2, qv2);
+            float x[GATHER_U];   /* every lane's source is a valid one (clamped) */
+#pragma unroll
+            for (uint32_t u = 0; u < GATHER_U; u++) x[u] = seg[sv[u]];
+            /* lanes past the item's end write the spare entries past the
+             * last tile instead: unconditional stores keep the count of
+             * memory operations in flight static */
 #pragma unroll
             for (uint32_t u = 0; u < GATHER_U; u++)
             {
-                const uint32_t rr = min(r + u, it.r1 - 1);   /* past the end: repeat the last round */
-                const og_bin_round d = b.rounds[rr];
-                const uint64_t mask = ((uint64_t)d.mask_hi << 32) | d.mask_lo;
-                const uint64_t low = mask & upto;
-                const uint32_t cnt = (uint32_t)__builtin_popcountll(low);
-                const uint32_t t = d.first + cnt;
-                const uint32_t o = cnt ? lane - (63u - (uint32_t)__builtin_clzll(low)) : d.off0 + lane;
-                ent[u] = b.tile_ent[t] + o;
+                const uint32_t e = f + 64u * u < it.f1 ? qv[u] + (f + 64u * u) % OG_BIN_TILE_ALIGN : b.n_ent + lane;
+                __builtin_nontemporal_store(x[u], &b.val[e]);
             }
-            uint32_t sv[GATHER_U];
-#pragma unroll
-            for (uint32_t u = 0; u < GATHER_U; u++) sv[u] = __builtin_nontemporal_load(&b.src[ent[u]]);
 #pragma unroll
             for (uint32_t u = 0; u < GATHER_U; u++)
-                if (r + u < it.r1) __builtin_nontemporal_store(seg[sv[u]], &b.val[ent[u]]);
+            {
+                sv[u] = sv2[u];
+                qv[u] = qv2[u];
+            }
         }
     }
 }
@@ -200,23 +225,90 @@ __device__ __forceinline__ void og_bin_epilogue(const og_pr_dev &a, uint32_t v, 
     og_bin_epilogue_pf(a, v, sum, prev, deg, st);
 }
 
-/* What a thread row needs besides its values: its slot range (from the
- * row offsets), old rank and out-degree; loaded one row ahead. */
-struct og_bin_row
+/* What a bin's rows need besides their values -- slot range (row
+ * offsets), old rank, out-degree -- for the OG_BIN_RPT rows a thread holds:
+ * row og_bin_meta_row(d, k) for slot k.  Thread rows: thread t holds rows
+ * t, t + BT, ...; wavefront rows: lane l of wavefront w holds the
+ * wavefront's rows l, l + 64, ... (rows w + BW * l + BT * k).  Loaded with
+ * the bin's entries, one bin ahead, so adding a bin's rows issues no
+ * global load (a load there would wait for the next bin's entries too:
+ * loads complete in order). */
+struct og_bin_meta
 {
-    uint32_t p0, p1, deg;
-    float prev;
+    uint32_t p0[OG_BIN_RPT], p1[OG_BIN_RPT], deg[OG_BIN_RPT];
+    float prev[OG_BIN_RPT];
 };
 
-__device__ __forceinline__ void og_bin_row_load(const og_pr_dev &a, uint32_t v, uint32_t end, og_bin_row &m)
+__device__ __forceinline__ uint32_t og_bin_meta_row(const og_bin_desc &d, uint32_t k)
 {
-    if (v < end)
+    const uint32_t t = d.kind == OG_BIN_WAVE_ROWS ? (threadIdx.x >> 6) + BW * (threadIdx.x & 63) : threadIdx.x;
+    return d.row0 + t + BT * k;
+}
+
+__device__ __forceinline__ void og_bin_meta_load(const og_pr_dev &a, const og_bin_desc &d, og_bin_meta &m)
+{
+    if (d.kind == OG_BIN_PIECE) return;
+#pragma unroll
+    for (uint32_t k = 0; k < OG_BIN_RPT; k++)
     {
-        m.p0 = a.row_ptr[v];
-        m.p1 = a.row_ptr[v + 1];
-        m.prev = __builtin_nontemporal_load(&a.r[v]);
-        m.deg = __builtin_nontemporal_load(&a.outdeg[v]);
+        const uint32_t v = og_bin_meta_row(d, k);
+        if (v < d.row1)
+        {
+            m.p0[k] = a.row_ptr[v];
+            m.p1[k] = a.row_ptr[v + 1];
+            m.prev[k] = __builtin_nontemporal_load(&a.r[v]);
+            m.deg[k] = __builtin_nontemporal_load(&a.outdeg[v]);
+        }
     }
+}
+
+/* One thread's sequential fp32 sum of slot[j, j1): the 16-byte groups
+ * wholly inside the range are read 16 values at a time, the next 16 in
+ * flight while one batch is added, so the dependent adds do not wait for
+ * LDS. */
+__device__ __forceinline__ float og_bin_row_sum(const float *slot, uint32_t j, uint32_t j1)
+{
+    constexpr uint32_t QMAX = OG_BIN_CAP / 4 - 1;   /* last float4 of slot */
+    float s = 0.0f;
+    const uint32_t qe = j1 >> 2;                     /* groups [q, qe) lie inside */
+    uint32_t q = (j + 3) >> 2;
+    if (q >= qe)
+    {
+        for (; j < j1; j++) s = s + slot[j];
+        return s;
+    }
+    for (; j < 4 * q; j++) s = s + slot[j];          /* head up to a group boundary */
+    const float4 *s4 = reinterpret_cast<const float4 *>(slot);
+    float4 cur[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) cur[u] = s4[min(q + u, QMAX)];
+    for (; q + 4 <= qe; q += 4)
+    {
+        float4 nxt[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) nxt[u] = s4[min(q + 4 + u, QMAX)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++)
+        {
+            s = s + cur[u].x;
+            s = s + cur[u].y;
+            s = s + cur[u].z;
+            s = s + cur[u].w;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) cur[u] = nxt[u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 3; u++)
+        if (q + u < qe)
+        {
+            s = s + cur[u].x;
+            s = s + cur[u].y;
+            s = s + cur[u].z;
+            s = s + cur[u].w;
+        }
+    for (j = 4 * qe; j < j1; j++) s = s + slot[j];   /* tail */
+    return s;
 }
 
 /* og_stat over the block (fixed order); result in thread 0 */
@@ -330,58 +422,57 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
     uint32_t q = blockIdx.x;
     og_bin_desc d;
     og_bin_regs regs;
-    og_bin_row nxt_row;
+    og_bin_meta meta;
     if (q < b.n_bins_live)
     {
         d = b.bins[q];
         og_bin_load(b, d, regs);
-        if (d.kind == OG_BIN_THREAD_ROWS) og_bin_row_load(a, d.row0 + threadIdx.x, d.row1, nxt_row);
+        og_bin_meta_load(a, d, meta);
     }
     for (; q < b.n_bins_live; q += gridDim.x)
     {
         __syncthreads();   /* the previous bin's reads of slot are done */
-        /* this bin's values to their CSR slots (entries padded to 8 per bin) */
+        /* this bin's values to their CSR slots */
         og_bin_put(b, slot, d, regs);
         const og_bin_desc cd = d;
-        og_bin_row cur_row = nxt_row;
-        /* the next bin's entries (and its first thread row) load while this
-         * one's rows are added */
+        const og_bin_meta cm = meta;
+        /* the next bin's entries and row metadata load while this one's
+         * rows are added */
         if (q + gridDim.x < b.n_bins_live)
         {
             d = b.bins[q + gridDim.x];
             og_bin_load(b, d, regs);
-            if (d.kind == OG_BIN_THREAD_ROWS) og_bin_row_load(a, d.row0 + threadIdx.x, d.row1, nxt_row);
+            og_bin_meta_load(a, d, meta);
         }
         __syncthreads();
         if (cd.kind == OG_BIN_THREAD_ROWS)
         {
-            for (uint32_t v = cd.row0 + threadIdx.x; v < cd.row1; v += BT)
+#pragma unroll
+            for (uint32_t k = 0; k < OG_BIN_RPT; k++)
             {
-                const og_bin_row m = cur_row;
-                og_bin_row_load(a, v + BT, cd.row1, cur_row);   /* the thread's next row */
-                const uint32_t j0 = m.p0 - cd.pos0, j1 = m.p1 - cd.pos0;
-                float s = 0.0f;
-                uint32_t j = j0;
-                for (; j + 4 <= j1; j += 4)
-                {
-                    const float t0 = slot[j], t1 = slot[j + 1], t2 = slot[j + 2], t3 = slot[j + 3];
-                    s = s + t0;
-                    s = s + t1;
-                    s = s + t2;
-                    s = s + t3;
-                }
-                for (; j < j1; j++) s = s + slot[j];
-                og_bin_epilogue_pf(a, v, s, m.prev, m.deg, st);
+                const uint32_t v = cd.row0 + threadIdx.x + BT * k;
+                if (v >= cd.row1) break;
+                const float s = og_bin_row_sum(slot, cm.p0[k] - cd.pos0, cm.p1[k] - cd.pos0);
+                og_bin_epilogue_pf(a, v, s, cm.prev[k], cm.deg[k], st);
             }
         }
         else if (cd.kind == OG_BIN_WAVE_ROWS)
         {
-            for (uint32_t v = cd.row0 + wave; v < cd.row1; v += BW)
+#pragma unroll
+            for (uint32_t k = 0; k < OG_BIN_RPT; k++)
             {
-                const uint32_t j0 = __builtin_amdgcn_readfirstlane(a.row_ptr[v] - cd.pos0);
-                const uint32_t j1 = __builtin_amdgcn_readfirstlane(a.row_ptr[v + 1] - cd.pos0);
-                const float y = og_bin_wave_sum(slot + j0, j1 - j0);
-                if (lane == 0) og_bin_epilogue(a, v, y, st);
+                if (cd.row0 + wave + BT * k >= cd.row1) break;
+                for (uint32_t l = 0; l < 64; l++)
+                {
+                    const uint32_t v = cd.row0 + wave + BW * l + BT * k;
+                    if (v >= cd.row1) break;
+                    const uint32_t j0 = (uint32_t)__builtin_amdgcn_readlane(cm.p0[k], l) - cd.pos0;
+                    const uint32_t j1 = (uint32_t)__builtin_amdgcn_readlane(cm.p1[k], l) - cd.pos0;
+                    const float y = og_bin_wave_sum(slot + j0, j1 - j0);
+                    const float prev = og_bin_lane(cm.prev[k], l);
+                    const uint32_t deg = (uint32_t)__builtin_amdgcn_readlane(cm.deg[k], l);
+                    if (lane == 0) og_bin_epilogue_pf(a, v, y, prev, deg, st);
+                }
             }
         }
         else
@@ -633,21 +724,22 @@ __global__ void og_bin_fill_kernel(const uint32_t *__restrict__ key, const uint3
                                    const og_bin_desc *__restrict__ bins, uint32_t n_seg,
                                    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ scan,
                                    const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ tile_ent,
-                                   uint16_t *__restrict__ src, uint16_t *__restrict__ pos)
+                                   const uint32_t *__restrict__ tile_src, uint16_t *__restrict__ src,
+                                   uint16_t *__restrict__ pos)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         const uint32_t k = key[i], j = val[i];
         const uint32_t bi = k / n_seg, s = k - bi * n_seg;
         const uint32_t t = scan[i + 1] - 1;          /* the tile holding sorted entry i */
-        const uint32_t e = tile_ent[t] + (i - tile_start[t]);
-        src[e] = (uint16_t)(cols[j] - s * OG_BIN_SEG);
-        pos[e] = (uint16_t)(j - bins[bi].pos0);
+        src[tile_src[t] + (i - tile_start[t])] = (uint16_t)(cols[j] - s * OG_BIN_SEG);
+        pos[tile_ent[t] + (i - tile_start[t])] = (uint16_t)(j - bins[bi].pos0);
     }
 }
 
-__global__ void og_bin_pads_kernel(const uint32_t *__restrict__ tile_ent, const uint32_t *__restrict__ tile_cnt,
-                                   uint32_t n_tiles, uint16_t *__restrict__ src, uint16_t *__restrict__ pos)
+__global__ void og_bin_pads_kernel(const uint32_t *__restrict__ tile_ent, const uint32_t *__restrict__ tile_src,
+                                   const uint32_t *__restrict__ tile_cnt, uint32_t n_tiles,
+                                   uint16_t *__restrict__ src, uint16_t *__restrict__ pos)
 {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles; t += gridDim.x * blockDim.x)
     {
@@ -655,7 +747,7 @@ __global__ void og_bin_pads_kernel(const uint32_t *__restrict__ tile_ent, const 
         const uint32_t pad = (OG_BIN_TILE_ALIGN - c % OG_BIN_TILE_ALIGN) % OG_BIN_TILE_ALIGN;
         for (uint32_t k = 0; k < pad; k++)
         {
-            src[tile_ent[t] + c + k] = 0;
+            src[tile_src[t] + c + k] = 0;
             pos[tile_ent[t] + c + k] = (uint16_t)OG_BIN_PAD;
         }
     }
@@ -695,20 +787,20 @@ extern "C" int og_launch_bin_tiles(const uint32_t *key, uint32_t n, uint32_t n_s
 
 extern "C" int og_launch_bin_fill(const uint32_t *key, const uint32_t *val, uint32_t n, const og_bin_desc *bins,
                                   uint32_t n_seg, const uint32_t *cols, const uint32_t *tile_flag_scan,
-                                  const uint32_t *tile_start, const uint32_t *tile_ent, uint16_t *src, uint16_t *pos,
-                                  hipStream_t s)
+                                  const uint32_t *tile_start, const uint32_t *tile_ent, const uint32_t *tile_src,
+                                  uint16_t *src, uint16_t *pos, hipStream_t s)
 {
     if (!n) return 0;
     OG_LAUNCH(og_bin_fill_kernel, dim3(og_bin_grid(n)), dim3(256), 0, s, key, val, n, bins, n_seg, cols,
-                       tile_flag_scan, tile_start, tile_ent, src, pos);
+                       tile_flag_scan, tile_start, tile_ent, tile_src, src, pos);
     return og_hip_check(og_launch_status(), "og_bin_fill_kernel");
 }
 
-extern "C" int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_cnt, uint32_t n_tiles, uint16_t *src,
-                                  uint16_t *pos, hipStream_t s)
+extern "C" int og_launch_bin_pads(const uint32_t *tile_ent, const uint32_t *tile_src, const uint32_t *tile_cnt,
+                                  uint32_t n_tiles, uint16_t *src, uint16_t *pos, hipStream_t s)
 {
     if (!n_tiles) return 0;
-    OG_LAUNCH(og_bin_pads_kernel, dim3(og_bin_grid(n_tiles)), dim3(256), 0, s, tile_ent, tile_cnt, n_tiles,
-                       src, pos);
+    OG_LAUNCH(og_bin_pads_kernel, dim3(og_bin_grid(n_tiles)), dim3(256), 0, s, tile_ent, tile_src, tile_cnt,
+                       n_tiles, src, pos);
     return og_hip_check(og_launch_status(), "og_bin_pads_kernel");
 }
