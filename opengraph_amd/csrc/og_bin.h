@@ -59,7 +59,7 @@ extern "C" {
 #define OG_BIN_CAP 39936u      /* in-edge values per bin: 156 KB of LDS */
 #define OG_BIN_PAD 0xFFFFu     /* pos of a padding entry */
 #define OG_BIN_WAVE_ROW 4096u  /* rows with more in-edges are added by a wavefront */
-#define OG_BIN_RPT 2u          /* rows per thread and bin: a bin holds at most RPT * THREADS rows */
+#define OG_BIN_RPT 1u          /* rows per thread and bin: a bin holds at most RPT * THREADS rows */
 #define OG_BIN_TILE_ALIGN 16u  /* tiles padded to 16 entries: 64-byte value runs */
 #define OG_BIN_ITEM_SRC (2048u * 64u) /* sources per gather work item at most */
 #define OG_BIN_THREADS 1024u

@@ -26,6 +26,7 @@
 #define BT OG_BIN_THREADS
 #define BW (OG_BIN_THREADS / 64)
 #define GATHER_U 16u  /* values per lane in flight in the gather */
+#define OG_BIN_HEAD 128u  /* leading terms of a long sum added one by one */
 
 /* ------------------------------------------------------------------ */
 /* shared pieces                                                       */
@@ -69,47 +70,158 @@ __device__ __forceinline__ void og_bin_term(float a, int sc, bool valid, uint32_
     m = valid && !bad ? (uint32_t)fl + (fr > 0.5f ? 1u : 0u) : 0u;
 }
 
-/* The sequential fp32 sum of x[0..n) (LDS), one wavefront, result uniform. */
-__device__ float og_bin_wave_sum(const float *x, uint32_t n)
+/* One thread's sequential fp32 sum of slot[j, j1): the 16-byte groups
+ * wholly inside the range are read 16 values at a time into two register
+ * sets used in turn, each refilled right after it is added, so the loads
+ * of one are in flight while the other is added. */
+__device__ __forceinline__ void og_bin_add16(float &s, const float4 (&x)[4], uint32_t groups)
+{
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++)
+        if (u < groups)
+        {
+            s = s + x[u].x;
+            s = s + x[u].y;
+            s = s + x[u].z;
+            s = s + x[u].w;
+        }
+}
+
+__device__ __forceinline__ void og_bin_load16(const float4 *s4, uint32_t q, float4 (&x)[4])
+{
+    constexpr uint32_t QMAX = OG_BIN_CAP / 4 - 1;   /* last float4 of slot */
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) x[u] = s4[min(q + u, QMAX)];
+}
+
+__device__ __forceinline__ float og_bin_row_sum(const float *slot, uint32_t j, uint32_t j1)
+{
+    float s = 0.0f;
+    const uint32_t qe = j1 >> 2;                     /* groups [q, qe) lie inside */
+    uint32_t q = (j + 3) >> 2;
+    if (q >= qe)
+    {
+        for (; j < j1; j++) s = s + slot[j];
+        return s;
+    }
+    for (; j < 4 * q; j++) s = s + slot[j];          /* head up to a group boundary */
+    const float4 *s4 = reinterpret_cast<const float4 *>(slot);
+    float4 A[4], B[4];
+    og_bin_load16(s4, q, A);
+    og_bin_load16(s4, q + 4, B);
+    for (; q + 8 <= qe; q += 8)
+    {
+        /* scheduling barriers keep each refill after the adds of its set
+         * and before the adds of the other */
+        og_bin_add16(s, A, 4);
+        __builtin_amdgcn_sched_barrier(0);
+        og_bin_load16(s4, q + 8, A);
+        __builtin_amdgcn_sched_barrier(0);
+        og_bin_add16(s, B, 4);
+        __builtin_amdgcn_sched_barrier(0);
+        og_bin_load16(s4, q + 12, B);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const uint32_t g = qe - q;                       /* 0..7 groups left, in A then B */
+    og_bin_add16(s, A, g < 4 ? g : 4);
+    og_bin_add16(s, B, g > 4 ? g - 4 : 0);
+    for (j = 4 * qe; j < j1; j++) s = s + slot[j];   /* tail */
+    return s;
+}
+
+/* x[i + 4 * lane + k] for k < 4 where inside [0, n), else 0 (n > 0;
+ * branch-free: the index is clamped, the value selected) */
+__device__ __forceinline__ void og_bin_wave_load4(const float *x, uint32_t i, uint32_t n, uint32_t lane,
+                                                  float (&a)[4])
+{
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+    {
+        const uint32_t p = i + 4 * lane + k;
+        const float v = x[min(p, n - 1)];
+        a[k] = p < n ? v : 0.0f;
+    }
+}
+
+/* The sequential fp32 sum of slot[j0, j0 + n) (LDS, n > 0), one
+ * wavefront, result uniform: the first OG_BIN_HEAD terms one after the
+ * other (where the running sum crosses binades every few terms), then 256
+ * terms per step, 4 consecutive ones per lane (a lane's prefix of its own
+ * terms, then a wavefront scan of the lanes' totals); the next 256 are
+ * read while a step is scanned. */
+__device__ float og_bin_wave_sum(const float *slot, uint32_t j0, uint32_t n)
 {
     const uint32_t lane = threadIdx.x & 63;
-    float y = 0.0f;
-    uint32_t i = 0;
+    const uint32_t h = n < OG_BIN_HEAD ? n : OG_BIN_HEAD;
+    float y = og_bin_uniform(og_bin_row_sum(slot, j0, j0 + h));
+    const float *x = slot + j0;
+    uint32_t i = h;
+    float a[4];
+    og_bin_wave_load4(x, i, n, lane, a);
     while (i < n)
     {
         const uint32_t left = n - i;
-        const bool valid = lane < left;
-        const float a = valid ? x[i + lane] : 0.0f;
         if (!(y >= 0x1p-100f && y < 0x1p100f))
         {
             /* zero, tiny or not finite: one ordinary add */
-            y = og_bin_uniform(y + og_bin_lane(a, 0));
+            y = og_bin_uniform(y + x[i]);
             i += 1;
+            og_bin_wave_load4(x, i, n, lane, a);
             continue;
         }
+        float an[4];
+        og_bin_wave_load4(x, i + 256, n, lane, an);   /* used when the whole step is taken */
         int ey;
         frexpf(y, &ey);
         const int sc = 24 - ey;
-        uint32_t m;
-        bool bad;
-        og_bin_term(a, sc, valid, m, bad);
-        const uint32_t incl = og_bin_wave_scan(m);   /* < 2^30 */
         const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
-        const bool fail = valid && (bad || Y + incl >= (1u << 24));
-        const uint64_t fb = __ballot(fail);
+        uint32_t L[4];
+        bool bad[4], valid[4];
+        uint32_t tot = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+        {
+            uint32_t m;
+            valid[k] = 4 * lane + k < left;
+            og_bin_term(a[k], sc, valid[k], m, bad[k]);
+            tot = min(tot + m, 1u << 25);
+            L[k] = tot;
+        }
+        const uint32_t incl = og_bin_wave_scan(tot);   /* <= 2^31 */
+        const uint32_t X = incl - tot;                 /* the earlier lanes' terms */
+        uint32_t kfail = 4;
+#pragma unroll
+        for (int k = 3; k >= 0; k--)
+            if (valid[k] && (bad[k] || Y + X + L[k] >= (1u << 24))) kfail = (uint32_t)k;
+        const uint64_t fb = __ballot(kfail < 4);
         if (fb == 0)
         {
-            const uint32_t last = (left < 64 ? left : 64) - 1;
-            y = og_bin_uniform(ldexpf((float)(Y + (uint32_t)__builtin_amdgcn_readlane(incl, last)), -sc));
-            i += last + 1;
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+            y = og_bin_uniform(ldexpf((float)(Y + total), -sc));
+            i += left < 256 ? left : 256;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) a[k] = an[k];
         }
         else
         {
-            const uint32_t k = (uint32_t)__builtin_ctzll(fb);
-            const uint32_t pre = k ? (uint32_t)__builtin_amdgcn_readlane(incl, k - 1) : 0u;
-            y = ldexpf((float)(Y + pre), -sc);
-            y = og_bin_uniform(y + og_bin_lane(a, k));
-            i += k + 1;
+            /* the first failing term: the prefix before it, then an
+             * ordinary add of it */
+            const uint32_t lf = (uint32_t)__builtin_ctzll(fb);
+            uint32_t pre = X;
+            float af = a[0];
+#pragma unroll
+            for (uint32_t k = 1; k < 4; k++)
+                if (kfail == k)
+                {
+                    pre = X + L[k - 1];
+                    af = a[k];
+                }
+            const uint32_t kf = (uint32_t)__builtin_amdgcn_readlane(kfail, lf);
+            const uint32_t p = (uint32_t)__builtin_amdgcn_readlane(pre, lf);
+            y = ldexpf((float)(Y + p), -sc);
+            y = og_bin_uniform(y + og_bin_lane(af, lf));
+            i += 4 * lf + kf + 1;
+            og_bin_wave_load4(x, i, n, lane, a);
         }
     }
     return y;
@@ -260,55 +372,6 @@ __device__ __forceinline__ void og_bin_meta_load(const og_pr_dev &a, const og_bi
             m.deg[k] = __builtin_nontemporal_load(&a.outdeg[v]);
         }
     }
-}
-
-/* One thread's sequential fp32 sum of slot[j, j1): the 16-byte groups
- * wholly inside the range are read 16 values at a time, the next 16 in
- * flight while one batch is added, so the dependent adds do not wait for
- * LDS. */
-__device__ __forceinline__ float og_bin_row_sum(const float *slot, uint32_t j, uint32_t j1)
-{
-    constexpr uint32_t QMAX = OG_BIN_CAP / 4 - 1;   /* last float4 of slot */
-    float s = 0.0f;
-    const uint32_t qe = j1 >> 2;                     /* groups [q, qe) lie inside */
-    uint32_t q = (j + 3) >> 2;
-    if (q >= qe)
-    {
-        for (; j < j1; j++) s = s + slot[j];
-        return s;
-    }
-    for (; j < 4 * q; j++) s = s + slot[j];          /* head up to a group boundary */
-    const float4 *s4 = reinterpret_cast<const float4 *>(slot);
-    float4 cur[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; u++) cur[u] = s4[min(q + u, QMAX)];
-    for (; q + 4 <= qe; q += 4)
-    {
-        float4 nxt[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) nxt[u] = s4[min(q + 4 + u, QMAX)];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++)
-        {
-            s = s + cur[u].x;
-            s = s + cur[u].y;
-            s = s + cur[u].z;
-            s = s + cur[u].w;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) cur[u] = nxt[u];
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < 3; u++)
-        if (q + u < qe)
-        {
-            s = s + cur[u].x;
-            s = s + cur[u].y;
-            s = s + cur[u].z;
-            s = s + cur[u].w;
-        }
-    for (j = 4 * qe; j < j1; j++) s = s + slot[j];   /* tail */
-    return s;
 }
 
 /* og_stat over the block (fixed order); result in thread 0 */
@@ -468,7 +531,7 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
                     if (v >= cd.row1) break;
                     const uint32_t j0 = (uint32_t)__builtin_amdgcn_readlane(cm.p0[k], l) - cd.pos0;
                     const uint32_t j1 = (uint32_t)__builtin_amdgcn_readlane(cm.p1[k], l) - cd.pos0;
-                    const float y = og_bin_wave_sum(slot + j0, j1 - j0);
+                    const float y = og_bin_wave_sum(slot, j0, j1 - j0);
                     const float prev = og_bin_lane(cm.prev[k], l);
                     const uint32_t deg = (uint32_t)__builtin_amdgcn_readlane(cm.deg[k], l);
                     if (lane == 0) og_bin_epilogue_pf(a, v, y, prev, deg, st);
@@ -511,27 +574,45 @@ __device__ __forceinline__ void og_bin_long_fetch(const float *x, uint32_t n, ui
 
 /* sum (saturated at 2^25) or minimum over the 16 wavefronts' values, read
  * by every lane */
+__device__ __forceinline__ void og_bin_waves_read(const uint32_t *v, uint32_t (&w)[BW])
+{
+    const uint4 *v4 = reinterpret_cast<const uint4 *>(v);   /* 16-byte aligned */
+#pragma unroll
+    for (uint32_t i = 0; i < BW / 4; i++)
+    {
+        const uint4 t = v4[i];
+        w[4 * i] = t.x;
+        w[4 * i + 1] = t.y;
+        w[4 * i + 2] = t.z;
+        w[4 * i + 3] = t.w;
+    }
+}
+
 __device__ __forceinline__ uint32_t og_bin_waves_sum(const uint32_t *v)
 {
-    uint32_t x = v[threadIdx.x & (BW - 1)];
+    uint32_t w[BW];
+    og_bin_waves_read(v, w);
+    uint32_t x = 0;
 #pragma unroll
-    for (uint32_t o = 1; o < BW; o <<= 1) x = min(x + __shfl_xor(x, o, 64), 1u << 25);
+    for (uint32_t i = 0; i < BW; i++) x = min(x + w[i], 1u << 25);
     return x;
 }
 
 __device__ __forceinline__ uint32_t og_bin_waves_min(const uint32_t *v)
 {
-    uint32_t x = v[threadIdx.x & (BW - 1)];
+    uint32_t w[BW];
+    og_bin_waves_read(v, w);
+    uint32_t x = w[0];
 #pragma unroll
-    for (uint32_t o = 1; o < BW; o <<= 1) x = min(x, __shfl_xor(x, o, 64));
+    for (uint32_t i = 1; i < BW; i++) x = min(x, w[i]);
     return x;
 }
 
 __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
 {
     __shared__ float buf[2][OG_BIN_LCH];
-    __shared__ uint32_t s_wsum[2][BW];
-    __shared__ uint32_t s_wfail[2][BW];
+    __shared__ __align__(16) uint32_t s_wsum[2][BW];
+    __shared__ __align__(16) uint32_t s_wfail[2][BW];
     __shared__ uint32_t s_pre;
     if (*a.done) return;
     og_bin_stat st;
@@ -558,6 +639,13 @@ __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
             const float *xs = buf[cur];
             const uint32_t clen = n - cb < OG_BIN_LCH ? n - cb : OG_BIN_LCH;
             uint32_t p = 0;
+            if (cb == 0)
+            {
+                /* the leading terms one after the other (the running sum
+                 * crosses binades every few terms there) */
+                p = clen < OG_BIN_HEAD ? clen : OG_BIN_HEAD;
+                y = og_bin_uniform(og_bin_row_sum(xs, 0, p));
+            }
             while (p < clen)
             {
                 const uint32_t left = clen - p;
@@ -590,11 +678,11 @@ __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
                 uint32_t base = 0;
                 {
                     /* prefix of the earlier wavefronts' sums */
-                    uint32_t w = s_wsum[par][lane & (BW - 1)];
-                    w = lane < wave ? w : 0u;
+                    uint32_t w[BW];
+                    og_bin_waves_read(s_wsum[par], w);
 #pragma unroll
-                    for (uint32_t o = 1; o < BW; o <<= 1) w = min(w + __shfl_xor(w, o, 64), 1u << 25);
-                    base = (uint32_t)__builtin_amdgcn_readfirstlane(w);   /* lanes 0..15 hold the sum */
+                    for (uint32_t k = 0; k < BW; k++)
+                        if (k < wave) base = min(base + w[k], 1u << 25);
                 }
                 const uint32_t X = min(base + incl_w - tot, 1u << 25);   /* before this thread's terms */
                 uint32_t kfail = OG_BIN_LTT;
