@@ -25,21 +25,29 @@ void og_bin_plan_destroy(og_bin_plan *b)
     if (!b) return;
     og_dfree(b->mem_items); og_dfree(b->mem_qent); og_dfree(b->mem_src);
     og_dfree(b->mem_pos); og_dfree(b->mem_val); og_dfree(b->mem_bins); og_dfree(b->mem_long_row);
-    og_dfree(b->mem_long_off); og_dfree(b->mem_vlong);
+    og_dfree(b->mem_long_bin);
     free(b);
+}
+
+/* The long rows get up to an eighth of the CUs, the sum pass the rest: one
+ * block per CU each, side by side.  Workgroups are dealt to the XCDs in
+ * turn, so the sum pass leaves a whole number of CUs per XCD free (the
+ * long-row blocks rounded up to a multiple of OG_BIN_XCDS); otherwise a
+ * long-row block can find its XCD full and wait for a sum block to end. */
+#define OG_BIN_XCDS 8u
+
+uint32_t og_bin_long_grid(const og_bin_plan *b)
+{
+    const uint32_t cus = (uint32_t)og_num_cus();
+    const uint32_t cap = cus / 8 ? cus / 8 : 1;
+    return b->d.n_long < cap ? b->d.n_long : cap;
 }
 
 uint32_t og_bin_sum_grid(const og_bin_plan *b)
 {
-    (void)b;
-    uint32_t g = (uint32_t)og_num_cus();
-    return g ? g : 1;
-}
-
-uint32_t og_bin_long_grid(const og_bin_plan *b)
-{
-    uint32_t g = (uint32_t)og_num_cus();
-    return b->d.n_long < g ? b->d.n_long : g;
+    const uint32_t cus = (uint32_t)og_num_cus();
+    const uint32_t l = (og_bin_long_grid(b) + OG_BIN_XCDS - 1) / OG_BIN_XCDS * OG_BIN_XCDS;
+    return cus > l ? cus - l : 1;
 }
 
 /* growable host arrays */
@@ -88,7 +96,8 @@ static void *og_bin_upload(const void *h, size_t bytes)
  * segment, each entry gathered once and its source the edge's (the CSR's
  * column at the entry's slot),
  * slots within their bins, bins and pieces within the entries, rows per bin,
- * CSR positions and vlong.  Runs before any binned kernel, so a bad layout
+ * CSR positions, the pieces first and each long row's pieces covering its
+ * in-edges in order.  Runs before any binned kernel, so a bad layout
  * is an error, not a stray access. */
 static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t *rp)
 {
@@ -98,13 +107,13 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
     uint32_t *qent = (uint32_t *)malloc(nq * 4 + 4);
     uint16_t *src = (uint16_t *)malloc(ne * 2), *pos = (uint16_t *)malloc(ne * 2);
     og_bin_desc *bins = (og_bin_desc *)malloc((size_t)d->n_bins * sizeof(og_bin_desc) + 16);
-    uint32_t *lrow = (uint32_t *)malloc((size_t)d->n_long * 4 + 4), *loff = (uint32_t *)malloc((size_t)d->n_long * 4 + 8);
+    uint32_t *lrow = (uint32_t *)malloc((size_t)d->n_long * 4 + 4), *lbin = (uint32_t *)malloc((size_t)d->n_long * 4 + 4);
     uint32_t *gsrc = (uint32_t *)malloc(ne * 4);   /* source id gathered into each entry */
     uint32_t *cols = (uint32_t *)malloc((size_t)pl->E * 4 + 4);
     uint8_t *seen = (uint8_t *)calloc(ne, 1);
     int rc = -1;
 #define OG_BIN_FAIL(...) do { og_set_error("binned layout check: " __VA_ARGS__); goto out; } while (0)
-    if (!items || !qent || !src || !pos || !bins || !lrow || !loff || !seen || !gsrc || !cols)
+    if (!items || !qent || !src || !pos || !bins || !lrow || !lbin || !seen || !gsrc || !cols)
         OG_BIN_FAIL("out of host memory");
     if (og_hip_check(og_copy(items, d->items, (size_t)d->n_items * sizeof(og_bin_item), hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(qent, d->qent, nq * 4, hipMemcpyDeviceToHost), "D2H") ||
@@ -112,7 +121,7 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
         og_hip_check(og_copy(pos, d->pos, ne * 2, hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(bins, d->bins, (size_t)d->n_bins * sizeof(og_bin_desc), hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(lrow, d->long_row, (size_t)d->n_long * 4, hipMemcpyDeviceToHost), "D2H") ||
-        og_hip_check(og_copy(loff, d->long_off, ((size_t)d->n_long + 1) * 4, hipMemcpyDeviceToHost), "D2H") ||
+        og_hip_check(og_copy(lbin, d->long_bin, (size_t)d->n_long * 4, hipMemcpyDeviceToHost), "D2H") ||
         og_hip_check(og_copy(cols, pl->cols, (size_t)pl->E * 4, hipMemcpyDeviceToHost), "D2H"))
         goto out;
     for (uint32_t q = 0; q < d->n_items; q++)
@@ -142,9 +151,11 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
             OG_BIN_FAIL("bin %u entries [%u, %u) count %u", q, bd.ent0, bd.ent1, bd.cnt);
         prev_end = bd.ent1;
         if ((uint64_t)bd.pos0 + bd.cnt > pl->E) OG_BIN_FAIL("bin %u positions", q);
+        if ((bd.kind == OG_BIN_PIECE) != (q < d->first_bin)) OG_BIN_FAIL("bin %u: pieces come first", q);
         if (bd.kind == OG_BIN_PIECE)
         {
-            if (bd.row0 >= pl->Z || (uint64_t)bd.vbase + bd.cnt > b->n_vlong) OG_BIN_FAIL("piece %u", q);
+            if (bd.row0 >= pl->Z || bd.pos0 < rp[bd.row0] || (uint64_t)bd.pos0 + bd.cnt > rp[bd.row0 + 1])
+                OG_BIN_FAIL("piece %u", q);
         }
         else
         {
@@ -166,14 +177,22 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
         }
         if (real != bd.cnt) OG_BIN_FAIL("bin %u: %u entries for %u in-edges", q, real, bd.cnt);
     }
+    uint64_t pieces = 0;
     for (uint32_t q = 0; q < d->n_long; q++)
-        if (lrow[q] >= pl->Z || loff[q] > loff[q + 1] || loff[q + 1] - loff[q] != rp[lrow[q] + 1] - rp[lrow[q]])
-            OG_BIN_FAIL("long row %u", q);
-    if (d->n_long && loff[d->n_long] != b->n_vlong) OG_BIN_FAIL("vlong size");
+    {
+        const uint32_t r = lrow[q];
+        if (r >= pl->Z) OG_BIN_FAIL("long row %u", q);
+        const uint32_t n = rp[r + 1] - rp[r];
+        for (uint32_t c0 = 0, k = lbin[q]; c0 < n; c0 += OG_BIN_CAP, k++, pieces++)
+            if (k >= d->first_bin || bins[k].kind != OG_BIN_PIECE || bins[k].row0 != r || bins[k].pos0 != rp[r] + c0 ||
+                bins[k].cnt != (n - c0 < OG_BIN_CAP ? n - c0 : OG_BIN_CAP))
+                OG_BIN_FAIL("long row %u: piece %u", q, k);
+    }
+    if (pieces != d->first_bin) OG_BIN_FAIL("%llu pieces for %u piece bins", (unsigned long long)pieces, d->first_bin);
     rc = 0;
 #undef OG_BIN_FAIL
 out:
-    free(items); free(qent); free(src); free(pos); free(bins); free(lrow); free(loff); free(seen);
+    free(items); free(qent); free(src); free(pos); free(bins); free(lrow); free(lbin); free(seen);
     free(gsrc); free(cols);
     return rc;
 }
@@ -195,7 +214,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
     og_bin_plan *b = (og_bin_plan *)calloc(1, sizeof(og_bin_plan));
     uint32_t *rp = (uint32_t *)malloc(((size_t)N + 1) * 4);
     og_vec bins = {NULL, 0, 0, sizeof(og_bin_desc)};
-    uint32_t *long_rows = NULL, *long_list = NULL, *long_voff_rowordered = NULL, *long_off = NULL;
+    uint32_t *long_rows = NULL, *long_list = NULL, *long_bin_rowordered = NULL, *long_bin = NULL;
     uint32_t *tseg = NULL, *tbin = NULL, *tstart = NULL, *tcnt = NULL, *tent = NULL, *order = NULL;
     uint32_t *seg_cnt = NULL, *tsrc = NULL, *seg_src = NULL, *qent = NULL;
     og_vec items = {NULL, 0, 0, sizeof(og_bin_item)};
@@ -213,22 +232,40 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         goto done;
 
     {
-        /* ---- long rows: vlong offsets assigned longest first ---- */
+        /* ---- long rows, longest first; their pieces first in the bin
+         * list (bins of OG_BIN_CAP in-edges each, in CSR order) ---- */
         uint32_t n_long = 0;
         for (uint32_t r = 0; r < Z; r++)
             if (rp[r + 1] - rp[r] > OG_BIN_CAP) n_long++;
         long_rows = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         long_list = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
-        long_voff_rowordered = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
-        long_off = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
-        if (!long_rows || !long_list || !long_voff_rowordered || !long_off)
+        long_bin_rowordered = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        long_bin = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
+        if (!long_rows || !long_list || !long_bin_rowordered || !long_bin)
         {
             og_set_error("out of host memory");
             goto done;
         }
         uint32_t k = 0;
         for (uint32_t r = 0; r < Z; r++)
-            if (rp[r + 1] - rp[r] > OG_BIN_CAP) long_rows[k++] = r;
+            if (rp[r + 1] - rp[r] > OG_BIN_CAP)
+            {
+                const uint32_t deg = rp[r + 1] - rp[r];
+                long_bin_rowordered[k] = (uint32_t)bins.n;
+                for (uint32_t p0 = 0; p0 < deg; p0 += OG_BIN_CAP)
+                {
+                    og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
+                    if (!d) { og_set_error("out of host memory"); goto done; }
+                    memset(d, 0, sizeof(*d));
+                    d->row0 = r;
+                    d->row1 = k;
+                    d->pos0 = rp[r] + p0;
+                    d->kind = OG_BIN_PIECE;
+                    d->cnt = deg - p0 < OG_BIN_CAP ? deg - p0 : OG_BIN_CAP;
+                }
+                long_rows[k++] = r;
+            }
+        b->d.first_bin = (uint32_t)bins.n;
         uint32_t *lord = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         if (!lord) { og_set_error("out of host memory"); goto done; }
         for (uint32_t i = 0; i < n_long; i++) lord[i] = i;
@@ -243,44 +280,23 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             }
             lord[j + 1] = x;
         }
-        uint64_t voff = 0;
         for (uint32_t i = 0; i < n_long; i++)
         {
-            const uint32_t x = lord[i], r = long_rows[x];
-            long_voff_rowordered[x] = (uint32_t)voff;
-            long_list[i] = r;
-            long_off[i] = (uint32_t)voff;
-            voff += rp[r + 1] - rp[r];
+            long_list[i] = long_rows[lord[i]];
+            long_bin[i] = long_bin_rowordered[lord[i]];
         }
         free(lord);
-        long_off[n_long] = (uint32_t)voff;
-        b->n_vlong = voff;
 
         /* ---- bins: consecutive rows of one kind (thread rows, wavefront
          * rows of more than OG_BIN_WAVE_ROW in-edges) with at most
-         * OG_BIN_CAP in-edges and OG_BIN_RPT * OG_BIN_THREADS rows together;
-         * pieces of the long rows ---- */
-        uint32_t li = 0;
+         * OG_BIN_CAP in-edges and OG_BIN_RPT * OG_BIN_THREADS rows together
+         * (the long rows have their pieces already) ---- */
         uint32_t r = 0;
         while (r < Z)
         {
             const uint32_t deg = rp[r + 1] - rp[r];
             if (deg > OG_BIN_CAP)
             {
-                for (uint32_t p0 = 0; p0 < deg; p0 += OG_BIN_CAP)
-                {
-                    const uint32_t cnt = deg - p0 < OG_BIN_CAP ? deg - p0 : OG_BIN_CAP;
-                    og_bin_desc *d = (og_bin_desc *)og_vec_push(&bins);
-                    if (!d) { og_set_error("out of host memory"); goto done; }
-                    memset(d, 0, sizeof(*d));
-                    d->row0 = r;
-                    d->row1 = li;
-                    d->pos0 = rp[r] + p0;
-                    d->kind = OG_BIN_PIECE;
-                    d->vbase = long_voff_rowordered[li] + p0;
-                    d->cnt = cnt;
-                }
-                li++;
                 r++;
                 continue;
             }
@@ -319,7 +335,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         }
         b->n_zero_bins = (uint32_t)bins.n - n_edge_bins;
         const uint32_t n_seg = (N + OG_BIN_SEG - 1) / OG_BIN_SEG;
-        if ((uint64_t)n_edge_bins * n_seg > 0xFFFFFFFFull || voff > 0xFFFFFFFFull)
+        if ((uint64_t)n_edge_bins * n_seg > 0xFFFFFFFFull)
         {
             og_set_error("binned pull: graph too large for 32-bit tile keys");
             goto done;
@@ -431,12 +447,11 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         b->mem_pos = og_dalloc(b->n_ent * 2);
         b->mem_val = og_dalloc(b->n_ent * 4);
         b->mem_long_row = og_bin_upload(long_list, (size_t)n_long * 4);
-        b->mem_long_off = og_bin_upload(long_off, (size_t)(n_long + 1) * 4);
-        b->mem_vlong = og_dalloc(voff * 4);
+        b->mem_long_bin = og_bin_upload(long_bin, (size_t)n_long * 4);
         d_tent = og_bin_upload(tent, (size_t)n_tiles * 4);
         d_tcnt = og_bin_upload(tcnt, (size_t)n_tiles * 4);
         d_tsrc = og_bin_upload(tsrc, (size_t)n_tiles * 4);
-        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_off || !b->mem_vlong ||
+        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_bin ||
             !d_tent || !d_tcnt || !d_tsrc ||
             og_hip_check(og_copy(d_bins, hb, bins.n * sizeof(og_bin_desc), hipMemcpyHostToDevice), "H2D bins"))
             goto done;
@@ -484,8 +499,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         b->d.val = (float *)b->mem_val;
         b->d.bins = d_bins;
         b->d.long_row = (const uint32_t *)b->mem_long_row;
-        b->d.long_off = (const uint32_t *)b->mem_long_off;
-        b->d.vlong = (float *)b->mem_vlong;
+        b->d.long_bin = (const uint32_t *)b->mem_long_bin;
         if (getenv("OG_BIN_VERIFY") && og_bin_verify(b, pl, rp)) goto done;
     }
     rc = 0;
@@ -493,7 +507,7 @@ done:
     (void)hipStreamSynchronize(s);
     og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);
     og_dfree(d_tseg); og_dfree(d_tbin); og_dfree(d_tstart); og_dfree(d_tent); og_dfree(d_tcnt); og_dfree(d_tsrc);
-    free(rp); free(bins.p); free(long_rows); free(long_list); free(long_voff_rowordered); free(long_off);
+    free(rp); free(bins.p); free(long_rows); free(long_list); free(long_bin_rowordered); free(long_bin);
     free(tseg); free(tbin); free(tstart); free(tcnt); free(tent); free(order); free(seg_cnt);
     free(items.p); free(tsrc); free(seg_src); free(qent);
     if (rc) og_bin_plan_destroy(b);

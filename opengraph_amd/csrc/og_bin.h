@@ -34,9 +34,11 @@
  *                The next bin's entries and row metadata are loaded into
  *                registers while a bin's rows are added.
  *   long rows    rows with more than OG_BIN_CAP in-edges are cut into
- *                pieces of OG_BIN_CAP; the sum pass writes each piece's
- *                values in CSR order to vlong, and one block per long row
- *                adds them in order (exact fast-forward, block-wide).
+ *                pieces of OG_BIN_CAP (bins of their own, first in the bin
+ *                list); one block per long row, on CUs of its own beside
+ *                the sum pass, puts each piece in LDS as the sum pass puts
+ *                a bin and adds them in order (exact fast-forward,
+ *                block-wide).
  * Every row is therefore summed in the reference's order: the fp32 pull is
  * bit-exact, and every global access is a streaming one.
  */
@@ -67,13 +69,12 @@ extern "C" {
 enum { OG_BIN_THREAD_ROWS = 0, OG_BIN_WAVE_ROWS = 1, OG_BIN_PIECE = 2 };
 
 /* One bin: rows [row0, row1) (a piece: the long row row0 and row1 = the
- * index of that long row), entries [ent0, ent1) (ent1 - ent0 a multiple of
- * 8), first CSR position pos0, and for a piece the vlong offset of its
- * first value. */
+ * index of that long row in row order), entries [ent0, ent1) (ent1 - ent0
+ * a multiple of 16), first CSR position pos0, cnt in-edges. */
 typedef struct
 {
     uint32_t row0, row1, ent0, ent1;
-    uint32_t pos0, kind, vbase, cnt;
+    uint32_t pos0, kind, pad, cnt;
 } og_bin_desc;
 
 /* Gather work item: segment seg, sources [f0, f1) (f0 and f1 multiples
@@ -87,6 +88,7 @@ typedef struct
 typedef struct og_bin_dev
 {
     uint32_t N, n_seg, n_items, n_bins, n_bins_live, n_long;
+    uint32_t first_bin;          /* bins [0, first_bin) are the long rows' pieces */
     uint32_t n_ent;              /* entries in tiles; 64 spare ones follow */
     const og_bin_item *items;
     const uint32_t *qent;        /* n_ent / 16 */
@@ -94,8 +96,7 @@ typedef struct og_bin_dev
     float *val;
     const og_bin_desc *bins;
     const uint32_t *long_row;    /* n_long: the long rows (new ids), longest first */
-    const uint32_t *long_off;    /* n_long + 1: their values' range in vlong */
-    float *vlong;
+    const uint32_t *long_bin;    /* n_long: each one's first piece */
 } og_bin_dev;
 
 typedef struct og_bin_plan
@@ -103,23 +104,24 @@ typedef struct og_bin_plan
     og_bin_dev d;
     uint64_t n_ent;              /* entries (and sources) including padding and the spares */
     uint32_t n_tiles, n_zero_bins;
-    uint64_t n_vlong;
     /* device allocations */
     void *mem_items, *mem_qent, *mem_src, *mem_pos, *mem_val, *mem_bins;
-    void *mem_long_row, *mem_long_off, *mem_vlong;
+    void *mem_long_row, *mem_long_bin;
 } og_bin_plan;
 
 /* Builds the binned layout of a plan whose rows keep the reference's
  * in-neighbour order (relabel 2 or 0). */
 int og_bin_plan_build(const og_plan *pl, og_bin_plan **out);
 void og_bin_plan_destroy(og_bin_plan *b);
-/* Blocks of the sum pass and of the long-row pass (statistics partials). */
+/* Blocks of the sum pass and of the long-row pass (statistics partials;
+ * together one per CU: the passes run side by side). */
 uint32_t og_bin_sum_grid(const og_bin_plan *b);
 uint32_t og_bin_long_grid(const og_bin_plan *b);
 
-/* One binned pull iteration: gather, sum, long rows; ev0 / ev1 (may be
- * NULL) bracket them.  Statistics partials: the sum pass's blocks, then
- * the long-row blocks; then the final reduction. */
+/* One binned pull iteration: gather, then the sum pass and (on the side
+ * stream) the long rows; ev0 / ev1 (may be NULL) bracket them.  Statistics
+ * partials: the sum pass's blocks, then the long-row blocks; then the final
+ * reduction. */
 int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint32_t it, uint32_t sum_grid,
                        uint32_t long_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 

@@ -60,14 +60,17 @@ __device__ __forceinline__ uint32_t og_bin_wave_scan(uint32_t v)
 
 /* The fast-forward quantities of term a for a running sum in the binade
  * whose unit is 2^-sc: m (valid when !bad) and bad (tie, negative, not
- * finite, or a term of at least the binade's top). */
+ * finite, or a term of at least the binade's top).  Branch-free: every
+ * lane computes, the result is selected. */
 __device__ __forceinline__ void og_bin_term(float a, int sc, bool valid, uint32_t &m, bool &bad)
 {
     const float t = ldexpf(a, sc);   /* a / u, exact for the ranks' range */
     const float fl = floorf(t);
     const float fr = t - fl;         /* exact */
-    bad = valid && (!(t >= 0.0f && t < 0x1p24f) || fr == 0.5f);
-    m = valid && !bad ? (uint32_t)fl + (fr > 0.5f ? 1u : 0u) : 0u;
+    const bool in = (t >= 0.0f) & (t < 0x1p24f);   /* false for NaN */
+    bad = valid & (!in | (fr == 0.5f));
+    const uint32_t mm = (uint32_t)fl + (fr > 0.5f ? 1u : 0u);
+    m = (valid & !bad) ? mm : 0u;
 }
 
 /* One thread's sequential fp32 sum of slot[j, j1): the 16-byte groups
@@ -192,7 +195,7 @@ __device__ float og_bin_wave_sum(const float *slot, uint32_t j0, uint32_t n)
         uint32_t kfail = 4;
 #pragma unroll
         for (int k = 3; k >= 0; k--)
-            if (valid[k] && (bad[k] || Y + X + L[k] >= (1u << 24))) kfail = (uint32_t)k;
+            kfail = (valid[k] & (bad[k] | (Y + X + L[k] >= (1u << 24)))) ? (uint32_t)k : kfail;
         const uint64_t fb = __ballot(kfail < 4);
         if (fb == 0)
         {
@@ -482,7 +485,8 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
     og_bin_stat_init(st);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint32_t q = blockIdx.x;
+    /* the long rows' pieces (bins [0, first_bin)) are the long-row pass's */
+    uint32_t q = b.first_bin + blockIdx.x;
     og_bin_desc d;
     og_bin_regs regs;
     og_bin_meta meta;
@@ -519,7 +523,7 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
                 og_bin_epilogue_pf(a, v, s, cm.prev[k], cm.deg[k], st);
             }
         }
-        else if (cd.kind == OG_BIN_WAVE_ROWS)
+        else
         {
 #pragma unroll
             for (uint32_t k = 0; k < OG_BIN_RPT; k++)
@@ -538,12 +542,6 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
                 }
             }
         }
-        else
-        {
-            /* a piece of a long row: its values, in CSR order, to vlong */
-            for (uint32_t k = threadIdx.x; k < cd.cnt; k += BT)
-                __builtin_nontemporal_store(slot[k], &b.vlong[cd.vbase + k]);
-        }
     }
     og_bin_stat r = og_bin_block_stat(st);
     og_bin_write_part(a, blockIdx.x, r);
@@ -553,24 +551,12 @@ __global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev 
 /* long rows                                                           */
 /* ------------------------------------------------------------------ */
 
-/* One block per long row (longest first, rows q = b, b + grid, ...): the
- * row's values from vlong in CSR order, staged through LDS in chunks of
- * OG_BIN_LCH (the next chunk's loads in flight while one is added), added
- * with the block-wide exact fast-forward, 4 consecutive terms per thread
- * (up to 4096 terms) per step. */
-#define OG_BIN_LCH 16384u
-#define OG_BIN_LPT (OG_BIN_LCH / BT)   /* values per thread and chunk */
+/* One block per long row (longest first, rows q = b, b + grid, ...),
+ * running beside the sum pass: the row's pieces in order, each scattered
+ * from its entries into LDS in CSR order (as the sum pass does for a bin)
+ * and added with the block-wide exact fast-forward, 4 consecutive terms
+ * per thread (4096 terms) per step. */
 #define OG_BIN_LTT 4u                  /* terms per thread and step */
-
-__device__ __forceinline__ void og_bin_long_fetch(const float *x, uint32_t n, uint32_t base, float (&r)[OG_BIN_LPT])
-{
-#pragma unroll
-    for (uint32_t k = 0; k < OG_BIN_LPT; k++)
-    {
-        const uint32_t i = base + k * BT + threadIdx.x;
-        r[k] = i < n ? __builtin_nontemporal_load(&x[i]) : 0.0f;
-    }
-}
 
 /* sum (saturated at 2^25) or minimum over the 16 wavefronts' values, read
  * by every lane */
@@ -608,136 +594,133 @@ __device__ __forceinline__ uint32_t og_bin_waves_min(const uint32_t *v)
     return x;
 }
 
+/* The block-wide fast-forward of the running sum y (identical in every
+ * thread) over xs[p, clen) (LDS); par alternates the partial arrays from
+ * step to step. */
+__device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, uint32_t clen, float y,
+                                                  uint32_t (&s_wsum)[2][BW], uint32_t (&s_wfail)[2][BW],
+                                                  uint32_t &s_pre, uint32_t &par)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    while (p < clen)
+    {
+        const uint32_t left = clen - p;
+        if (!(y >= 0x1p-100f && y < 0x1p100f))
+        {
+            y = y + xs[p];
+            p += 1;
+            continue;
+        }
+        int ey;
+        frexpf(y, &ey);
+        const int sc = 24 - ey;
+        const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
+        /* this thread's terms p + 4t .. p + 4t + 3 */
+        uint32_t m[OG_BIN_LTT], l[OG_BIN_LTT];
+        bool bad[OG_BIN_LTT], valid[OG_BIN_LTT];
+        uint32_t tot = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < OG_BIN_LTT; k++)
+        {
+            const uint32_t i = OG_BIN_LTT * threadIdx.x + k;
+            valid[k] = i < left;
+            const float v = xs[min(p + i, clen - 1)];
+            og_bin_term(valid[k] ? v : 0.0f, sc, valid[k], m[k], bad[k]);
+            tot = min(tot + m[k], 1u << 25);
+            l[k] = tot;
+        }
+        const uint32_t incl_w = og_bin_wave_scan(tot);          /* <= 2^31 */
+        if (lane == 63) s_wsum[par][wave] = min(incl_w, 1u << 25);
+        __syncthreads();
+        uint32_t base = 0;
+        {
+            /* prefix of the earlier wavefronts' sums */
+            uint32_t w[BW];
+            og_bin_waves_read(s_wsum[par], w);
+#pragma unroll
+            for (uint32_t k = 0; k < BW; k++)
+                if (k < wave) base = min(base + w[k], 1u << 25);
+        }
+        const uint32_t X = min(base + incl_w - tot, 1u << 25);   /* before this thread's terms */
+        uint32_t kfail = OG_BIN_LTT;
+#pragma unroll
+        for (int k = (int)OG_BIN_LTT - 1; k >= 0; k--)
+            kfail = (valid[k] & (bad[k] | (Y + X + l[k] >= (1u << 24)))) ? (uint32_t)k : kfail;
+        const uint64_t fb = __ballot(kfail < OG_BIN_LTT);
+        if (lane == 0)
+        {
+            uint32_t idx = 0xFFFFFFFFu;
+            if (fb)
+            {
+                const uint32_t fl = (uint32_t)__builtin_ctzll(fb);
+                idx = OG_BIN_LTT * (64u * wave + fl) + (uint32_t)__builtin_amdgcn_readlane(kfail, fl);
+            }
+            s_wfail[par][wave] = idx;
+        }
+        __syncthreads();
+        const uint32_t kf = og_bin_waves_min(s_wfail[par]);
+        if (kf == 0xFFFFFFFFu)
+        {
+            const uint32_t total = og_bin_waves_sum(s_wsum[par]);
+            y = ldexpf((float)(Y + total), -sc);
+            p += left < OG_BIN_LTT * BT ? left : OG_BIN_LTT * BT;
+        }
+        else
+        {
+            const uint32_t t = kf / OG_BIN_LTT, kk = kf % OG_BIN_LTT;
+            if (threadIdx.x == t)
+            {
+                uint32_t pre = X;
+#pragma unroll
+                for (uint32_t k = 0; k < OG_BIN_LTT; k++)
+                    if (k + 1 == kk) pre = X + l[k];
+                s_pre = pre;
+            }
+            __syncthreads();
+            y = ldexpf((float)(Y + s_pre), -sc);
+            y = y + xs[p + kf];
+            p += kf + 1;
+            __syncthreads();   /* s_pre read by all before its next write */
+        }
+        par ^= 1;
+    }
+    return y;
+}
+
 __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
 {
-    __shared__ float buf[2][OG_BIN_LCH];
+    __shared__ __align__(16) float buf[OG_BIN_CAP];
     __shared__ __align__(16) uint32_t s_wsum[2][BW];
     __shared__ __align__(16) uint32_t s_wfail[2][BW];
     __shared__ uint32_t s_pre;
     if (*a.done) return;
     og_bin_stat st;
     og_bin_stat_init(st);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
-    uint32_t par = 0;    /* which s_wsum / s_wfail set this step uses */
+    uint32_t par = 0;    /* which s_wsum / s_wfail set a step uses */
     for (uint32_t q = blockIdx.x; q < b.n_long; q += gridDim.x)
     {
         const uint32_t v = b.long_row[q];
-        const uint32_t off = b.long_off[q], n = b.long_off[q + 1] - off;
-        const float *x = b.vlong + off;
-        float pf[OG_BIN_LPT];
-        __syncthreads();   /* the previous row's reads of buf are done */
-        og_bin_long_fetch(x, n, 0, pf);
-#pragma unroll
-        for (uint32_t k = 0; k < OG_BIN_LPT; k++) buf[0][k * BT + threadIdx.x] = pf[k];
-        og_bin_long_fetch(x, n, OG_BIN_LCH, pf);
-        __syncthreads();
+        const uint32_t n = a.row_ptr[v + 1] - a.row_ptr[v];
         float y = 0.0f;     /* identical in every thread */
-        uint32_t cur = 0;
-        for (uint32_t cb = 0; cb < n; cb += OG_BIN_LCH)
+        uint32_t bin = b.long_bin[q];
+        for (uint32_t c0 = 0; c0 < n; c0 += OG_BIN_CAP, bin++)
         {
-            const float *xs = buf[cur];
-            const uint32_t clen = n - cb < OG_BIN_LCH ? n - cb : OG_BIN_LCH;
+            const og_bin_desc d = b.bins[bin];
+            og_bin_regs regs;
+            og_bin_load(b, d, regs);
+            __syncthreads();   /* the previous piece's reads of buf are done */
+            og_bin_put(b, buf, d, regs);
+            __syncthreads();
             uint32_t p = 0;
-            if (cb == 0)
+            if (c0 == 0)
             {
                 /* the leading terms one after the other (the running sum
                  * crosses binades every few terms there) */
-                p = clen < OG_BIN_HEAD ? clen : OG_BIN_HEAD;
-                y = og_bin_uniform(og_bin_row_sum(xs, 0, p));
+                p = d.cnt < OG_BIN_HEAD ? d.cnt : OG_BIN_HEAD;
+                y = og_bin_uniform(og_bin_row_sum(buf, 0, p));
             }
-            while (p < clen)
-            {
-                const uint32_t left = clen - p;
-                if (!(y >= 0x1p-100f && y < 0x1p100f))
-                {
-                    y = y + xs[p];
-                    p += 1;
-                    continue;
-                }
-                int ey;
-                frexpf(y, &ey);
-                const int sc = 24 - ey;
-                const uint32_t Y = (uint32_t)ldexpf(y, sc);   /* in [2^23, 2^24) */
-                /* this thread's terms p + 4t .. p + 4t + 3 */
-                uint32_t m[OG_BIN_LTT], l[OG_BIN_LTT];
-                bool bad[OG_BIN_LTT], valid[OG_BIN_LTT];
-                uint32_t tot = 0;
-#pragma unroll
-                for (uint32_t k = 0; k < OG_BIN_LTT; k++)
-                {
-                    const uint32_t i = OG_BIN_LTT * threadIdx.x + k;
-                    valid[k] = i < left;
-                    og_bin_term(valid[k] ? xs[p + i] : 0.0f, sc, valid[k], m[k], bad[k]);
-                    tot = min(tot + m[k], 1u << 25);
-                    l[k] = tot;
-                }
-                const uint32_t incl_w = og_bin_wave_scan(tot);          /* <= 2^31 */
-                if (lane == 63) s_wsum[par][wave] = min(incl_w, 1u << 25);
-                __syncthreads();
-                uint32_t base = 0;
-                {
-                    /* prefix of the earlier wavefronts' sums */
-                    uint32_t w[BW];
-                    og_bin_waves_read(s_wsum[par], w);
-#pragma unroll
-                    for (uint32_t k = 0; k < BW; k++)
-                        if (k < wave) base = min(base + w[k], 1u << 25);
-                }
-                const uint32_t X = min(base + incl_w - tot, 1u << 25);   /* before this thread's terms */
-                uint32_t kfail = OG_BIN_LTT;
-#pragma unroll
-                for (int k = (int)OG_BIN_LTT - 1; k >= 0; k--)
-                    if (valid[k] && (bad[k] || Y + X + l[k] >= (1u << 24))) kfail = (uint32_t)k;
-                const uint64_t fb = __ballot(kfail < OG_BIN_LTT);
-                if (lane == 0)
-                {
-                    uint32_t idx = 0xFFFFFFFFu;
-                    if (fb)
-                    {
-                        const uint32_t fl = (uint32_t)__builtin_ctzll(fb);
-                        idx = OG_BIN_LTT * (64u * wave + fl) + (uint32_t)__builtin_amdgcn_readlane(kfail, fl);
-                    }
-                    s_wfail[par][wave] = idx;
-                }
-                __syncthreads();
-                const uint32_t kf = og_bin_waves_min(s_wfail[par]);
-                if (kf == 0xFFFFFFFFu)
-                {
-                    const uint32_t total = og_bin_waves_sum(s_wsum[par]);
-                    y = ldexpf((float)(Y + total), -sc);
-                    p += left < OG_BIN_LTT * BT ? left : OG_BIN_LTT * BT;
-                }
-                else
-                {
-                    const uint32_t t = kf / OG_BIN_LTT, kk = kf % OG_BIN_LTT;
-                    if (threadIdx.x == t)
-                    {
-                        uint32_t pre = X;
-#pragma unroll
-                        for (uint32_t k = 0; k < OG_BIN_LTT; k++)
-                            if (k + 1 == kk) pre = X + l[k];
-                        s_pre = pre;
-                    }
-                    __syncthreads();
-                    y = ldexpf((float)(Y + s_pre), -sc);
-                    y = y + xs[p + kf];
-                    p += kf + 1;
-                    __syncthreads();   /* s_pre read by all before its next write */
-                }
-                par ^= 1;
-            }
-            /* the next chunk (already loaded) into the other buffer, and the
-             * loads of the one after it */
-            if (cb + OG_BIN_LCH < n)
-            {
-                __syncthreads();
-#pragma unroll
-                for (uint32_t k = 0; k < OG_BIN_LPT; k++) buf[cur ^ 1][k * BT + threadIdx.x] = pf[k];
-                og_bin_long_fetch(x, n, cb + 2 * OG_BIN_LCH, pf);
-                __syncthreads();
-                cur ^= 1;
-            }
+            y = og_bin_block_sum(buf, p, d.cnt, y, s_wsum, s_wfail, s_pre, par);
         }
         if (threadIdx.x == 0) og_bin_epilogue(a, v, y, st);
     }
@@ -752,14 +735,26 @@ __global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
 extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint32_t it, uint32_t sum_grid,
                                   uint32_t long_grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (ev0) hipEventRecord(ev0, s);
+    (void)it;
+    if (ev0 && og_hip_check(hipEventRecord(ev0, s), "hipEventRecord")) return -1;
     if (b->n_items)
-        OG_LAUNCH(og_bin_gather_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, (const float *)a->g_in,
-                           (const uint32_t *)a->done);
-    OG_LAUNCH(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
+        OG_LAUNCH(og_bin_gather_kernel, dim3(og_num_cus() > 0 ? og_num_cus() : 1), dim3(BT), 0, s, *b,
+                  (const float *)a->g_in,
+                  (const uint32_t *)a->done);
     if (long_grid)
-        OG_LAUNCH(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, s, *b, *a, sum_grid);
-    if (ev1) hipEventRecord(ev1, s);
+    {
+        /* the long rows on the side stream, on CUs of their own, while the
+         * sum pass runs on the others */
+        hipStream_t side = og_side_stream();
+        if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
+            og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
+            return -1;
+        OG_LAUNCH(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, side, *b, *a, sum_grid);
+        if (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord")) return -1;
+    }
+    OG_LAUNCH(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
+    if (long_grid && og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent")) return -1;
+    if (ev1 && og_hip_check(hipEventRecord(ev1, s), "hipEventRecord")) return -1;
     return og_hip_check(og_launch_status(), "og_bin_pull");
 }
 
