@@ -29,25 +29,26 @@ void og_bin_plan_destroy(og_bin_plan *b)
     free(b);
 }
 
-/* The long rows get up to an eighth of the CUs, the sum pass the rest: one
- * block per CU each, side by side.  Workgroups are dealt to the XCDs in
- * turn, so the sum pass leaves a whole number of CUs per XCD free (the
- * long-row blocks rounded up to a multiple of OG_BIN_XCDS); otherwise a
- * long-row block can find its XCD full and wait for a sum block to end. */
+/* Sum and long-row blocks take half a CU's LDS each: two block slots per
+ * CU.  The long rows get up to one slot in eight, the sum pass the rest,
+ * side by side.  Workgroups are dealt to the XCDs in turn, so the sum pass
+ * leaves a whole number of slots per XCD free (the long-row blocks rounded
+ * up to a multiple of OG_BIN_XCDS); otherwise a long-row block can find its
+ * XCD full and wait for a sum block to end. */
 #define OG_BIN_XCDS 8u
 
 uint32_t og_bin_long_grid(const og_bin_plan *b)
 {
     const uint32_t cus = (uint32_t)og_num_cus();
-    const uint32_t cap = cus / 8 ? cus / 8 : 1;
+    const uint32_t cap = cus / 4 ? cus / 4 : 1;
     return b->d.n_long < cap ? b->d.n_long : cap;
 }
 
 uint32_t og_bin_sum_grid(const og_bin_plan *b)
 {
-    const uint32_t cus = (uint32_t)og_num_cus();
+    const uint32_t slots = 2u * (uint32_t)og_num_cus();   /* two blocks per CU */
     const uint32_t l = (og_bin_long_grid(b) + OG_BIN_XCDS - 1) / OG_BIN_XCDS * OG_BIN_XCDS;
-    return cus > l ? cus - l : 1;
+    return slots > l ? slots - l : 1;
 }
 
 /* growable host arrays */

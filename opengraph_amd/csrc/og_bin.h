@@ -58,13 +58,14 @@ extern "C" {
 #endif
 
 #define OG_BIN_SEG 39936u      /* sources per segment: 156 KB of fp32 in LDS */
-#define OG_BIN_CAP 39936u      /* in-edge values per bin: 156 KB of LDS */
+#define OG_BIN_CAP 19968u      /* in-edge values per bin: 78 KB of LDS, two bins per CU */
 #define OG_BIN_PAD 0xFFFFu     /* pos of a padding entry */
 #define OG_BIN_WAVE_ROW 4096u  /* rows with more in-edges are added by a wavefront */
 #define OG_BIN_RPT 1u          /* rows per thread and bin: a bin holds at most RPT * THREADS rows */
 #define OG_BIN_TILE_ALIGN 16u  /* tiles padded to 16 entries: 64-byte value runs */
 #define OG_BIN_ITEM_SRC (2048u * 64u) /* sources per gather work item at most */
-#define OG_BIN_THREADS 1024u
+#define OG_BIN_THREADS 512u    /* sum and long-row blocks (two per CU) */
+#define OG_BIN_GTHREADS 1024u  /* gather blocks (one per CU: a segment fills LDS) */
 
 enum { OG_BIN_THREAD_ROWS = 0, OG_BIN_WAVE_ROWS = 1, OG_BIN_PIECE = 2 };
 

@@ -25,6 +25,8 @@
 
 #define BT OG_BIN_THREADS
 #define BW (OG_BIN_THREADS / 64)
+#define GT OG_BIN_GTHREADS       /* the gather pass's blocks */
+#define GW (OG_BIN_GTHREADS / 64)
 #define GATHER_U 16u  /* values per lane in flight in the gather */
 #define OG_BIN_HEAD 128u  /* leading terms of a long sum added one by one */
 
@@ -249,7 +251,7 @@ __device__ __forceinline__ void og_bin_gather_batch(const og_bin_dev &b, uint32_
     }
 }
 
-__global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const float *__restrict__ c,
+__global__ __launch_bounds__(GT) void og_bin_gather_kernel(og_bin_dev b, const float *__restrict__ c,
                                                            const uint32_t *done)
 {
     __shared__ __align__(16) float seg[OG_BIN_SEG];
@@ -265,11 +267,11 @@ __global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const f
         {
             const float4 *c4 = (const float4 *)(c + v0);
             float4 *s4 = (float4 *)seg;
-            for (uint32_t i = threadIdx.x; i < nv / 4; i += BT) s4[i] = c4[i];
-            for (uint32_t i = (nv & ~3u) + threadIdx.x; i < nv; i += BT) seg[i] = c[v0 + i];
+            for (uint32_t i = threadIdx.x; i < nv / 4; i += GT) s4[i] = c4[i];
+            for (uint32_t i = (nv & ~3u) + threadIdx.x; i < nv; i += GT) seg[i] = c[v0 + i];
         }
         __syncthreads();
-        /* wavefront w takes values f = f0 + 64 * (w * U + k * BW * U + u) +
+        /* wavefront w takes values f = f0 + 64 * (w * U + k * GW * U + u) +
          * lane; software pipelined: the next batch's loads are issued
          * before this batch's stores (loads complete in order, and a wait
          * for this batch's loads must not also wait for stores issued after
@@ -277,10 +279,10 @@ __global__ __launch_bounds__(BT) void og_bin_gather_kernel(og_bin_dev b, const f
         uint32_t f = it.f0 + 64u * GATHER_U * wave + lane;
         uint32_t sv[GATHER_U], qv[GATHER_U];
         og_bin_gather_batch(b, f, it.f1, sv, qv);
-        for (; f < it.f1; f += 64u * GATHER_U * BW)
+        for (; f < it.f1; f += 64u * GATHER_U * GW)
         {
             uint32_t sv2[GATHER_U], qv2[GATHER_U];
-            og_bin_gather_batch(b, f + 64u * GATHER_U * BW, it.f1, sv2, qv2);
+            og_bin_gather_batch(b, f + 64u * GATHER_U * GW, it.f1, sv2, qv2);
             float x[GATHER_U];   /* every lane's source is a valid one (clamped) */
 #pragma unroll
             for (uint32_t u = 0; u < GATHER_U; u++) x[u] = seg[sv[u]];
@@ -477,7 +479,7 @@ __device__ __forceinline__ void og_bin_put(const og_bin_dev &b, float *slot, con
                         __builtin_nontemporal_load((const og_f32x4 *)(b.val + e + 4)));
 }
 
-__global__ __launch_bounds__(BT) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev a)
+__global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_dev a)
 {
     __shared__ __align__(16) float slot[OG_BIN_CAP];
     if (*a.done) return;
@@ -688,7 +690,7 @@ __device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, u
     return y;
 }
 
-__global__ __launch_bounds__(BT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
+__global__ __launch_bounds__(BT, 4) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
 {
     __shared__ __align__(16) float buf[OG_BIN_CAP];
     __shared__ __align__(16) uint32_t s_wsum[2][BW];
@@ -738,7 +740,7 @@ extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint3
     (void)it;
     if (ev0 && og_hip_check(hipEventRecord(ev0, s), "hipEventRecord")) return -1;
     if (b->n_items)
-        OG_LAUNCH(og_bin_gather_kernel, dim3(og_num_cus() > 0 ? og_num_cus() : 1), dim3(BT), 0, s, *b,
+        OG_LAUNCH(og_bin_gather_kernel, dim3(og_num_cus() > 0 ? og_num_cus() : 1), dim3(GT), 0, s, *b,
                   (const float *)a->g_in,
                   (const uint32_t *)a->done);
     if (long_grid)

@@ -24,7 +24,7 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-CAP = 39936   # OG_BIN_CAP: in-edge values per bin
+CAP = 19968   # OG_BIN_CAP: in-edge values per bin
 
 
 @pytest.fixture(scope="module", autouse=True)
