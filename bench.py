@@ -12,10 +12,12 @@ timing so every step does the full work.
 
 Printed JSON line: value = GTEPS (edges traversed per second, 1e9) over the
 K timed steps, with inputs resident in HBM; roofline = algorithmic bytes per
-iteration (8E + 20N) over the dominant kernel's average duration, measured
-with HIP events on the library's stream; traffic = the HBM bytes per launch
-that rocprofv3 FETCH_SIZE + WRITE_SIZE counted for that kernel on this same
-command (profiles/pull_rmat24_counters.json, scaled by a plain copy);
+iteration (8E + 20N) over the iteration's average kernel time, measured with
+HIP events on the library's stream around its three passes (the gather, then
+the sum pass beside the long-row pass on the side stream); traffic = the HBM
+bytes per iteration that rocprofv3 FETCH_SIZE + WRITE_SIZE counted for those
+three kernels on this same command (profiles/pull_rmat24_counters.json,
+scaled by a plain copy);
 cpu_baseline = the reference's own pageRankPullGraphCSR (compiled from its
 sources into oracle/_ref) on the same CSR on this host, over a bounded number
 of iterations.
@@ -709,8 +711,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_iteration": bytes_iter,
-                     "kernel": "og_bin_gather_kernel + og_bin_sum_kernel + og_bin_long_kernel (one iteration's "
-                               "passes, HIP events from the gather's start to the long-row pass's end)",
+                     "kernel": "og_bin_gather_kernel, then og_bin_sum_kernel beside og_bin_long_kernel on the side "
+                               "stream (one iteration's passes; HIP events on the library stream from before the "
+                               "gather to after the join)",
                      "traffic_source": "profiles/pull_rmat24_counters.json" if traffic else None},
         "cpu_baseline": cpu,
         "extra": extra,
