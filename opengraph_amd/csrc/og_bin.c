@@ -25,23 +25,32 @@ void og_bin_plan_destroy(og_bin_plan *b)
     if (!b) return;
     og_dfree(b->mem_items); og_dfree(b->mem_qent); og_dfree(b->mem_src);
     og_dfree(b->mem_pos); og_dfree(b->mem_val); og_dfree(b->mem_bins); og_dfree(b->mem_long_row);
-    og_dfree(b->mem_long_bin);
+    og_dfree(b->mem_long_bin); og_dfree(b->mem_ticket);
     free(b);
 }
 
 /* Sum and long-row blocks take half a CU's LDS each: two block slots per
- * CU.  The long rows get up to one slot in eight, the sum pass the rest,
- * side by side.  Workgroups are dealt to the XCDs in turn, so the sum pass
- * leaves a whole number of slots per XCD free (the long-row blocks rounded
- * up to a multiple of OG_BIN_XCDS); otherwise a long-row block can find its
- * XCD full and wait for a sum block to end. */
+ * CU (a long-row block has twice the threads at half the registers).  The
+ * long rows get up to one slot in eight, the sum pass the rest, side by
+ * side.  Workgroups are dealt to the XCDs in turn, so the sum pass leaves a
+ * whole number of slots per XCD free (the long-row blocks rounded up to a
+ * multiple of OG_BIN_XCDS); otherwise a long-row block can find its XCD
+ * full and wait for a sum block to end. */
 #define OG_BIN_XCDS 8u
 
 uint32_t og_bin_long_grid(const og_bin_plan *b)
 {
+    /* the long rows' share of the slots: their in-edges' share of E, times
+     * 1.6 (an in-edge costs a long-row block about 1.6 times what it costs
+     * the sum pass: R-MAT-24 measurements), in whole XCD rounds, between
+     * OG_BIN_XCDS and a quarter of the CUs */
     const uint32_t cus = (uint32_t)og_num_cus();
     const uint32_t cap = cus / 4 ? cus / 4 : 1;
-    return b->d.n_long < cap ? b->d.n_long : cap;
+    uint64_t l = b->n_edges ? (uint64_t)(2.0 * cus * 1.6 * (double)b->long_terms / (double)b->n_edges) : 0;
+    l = (l + OG_BIN_XCDS - 1) / OG_BIN_XCDS * OG_BIN_XCDS;
+    if (l < OG_BIN_XCDS) l = OG_BIN_XCDS;
+    if (l > cap) l = cap;
+    return b->d.n_long < l ? b->d.n_long : (uint32_t)l;
 }
 
 uint32_t og_bin_sum_grid(const og_bin_plan *b)
@@ -267,6 +276,8 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
                 long_rows[k++] = r;
             }
         b->d.first_bin = (uint32_t)bins.n;
+        b->n_edges = E;
+        for (uint32_t i = 0; i < n_long; i++) b->long_terms += rp[long_rows[i] + 1] - rp[long_rows[i]];
         uint32_t *lord = (uint32_t *)malloc((size_t)(n_long + 1) * 4);
         if (!lord) { og_set_error("out of host memory"); goto done; }
         for (uint32_t i = 0; i < n_long; i++) lord[i] = i;
@@ -449,10 +460,11 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         b->mem_val = og_dalloc(b->n_ent * 4);
         b->mem_long_row = og_bin_upload(long_list, (size_t)n_long * 4);
         b->mem_long_bin = og_bin_upload(long_bin, (size_t)n_long * 4);
+        b->mem_ticket = og_dalloc(256);
         d_tent = og_bin_upload(tent, (size_t)n_tiles * 4);
         d_tcnt = og_bin_upload(tcnt, (size_t)n_tiles * 4);
         d_tsrc = og_bin_upload(tsrc, (size_t)n_tiles * 4);
-        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_bin ||
+        if (!b->mem_src || !b->mem_pos || !b->mem_val || !b->mem_long_row || !b->mem_long_bin || !b->mem_ticket ||
             !d_tent || !d_tcnt || !d_tsrc ||
             og_hip_check(og_copy(d_bins, hb, bins.n * sizeof(og_bin_desc), hipMemcpyHostToDevice), "H2D bins"))
             goto done;
@@ -501,6 +513,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         b->d.bins = d_bins;
         b->d.long_row = (const uint32_t *)b->mem_long_row;
         b->d.long_bin = (const uint32_t *)b->mem_long_bin;
+        b->d.ticket = (uint32_t *)b->mem_ticket;
         if (getenv("OG_BIN_VERIFY") && og_bin_verify(b, pl, rp)) goto done;
     }
     rc = 0;

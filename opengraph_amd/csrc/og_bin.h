@@ -66,6 +66,7 @@ extern "C" {
 #define OG_BIN_ITEM_SRC (2048u * 64u) /* sources per gather work item at most */
 #define OG_BIN_THREADS 512u    /* sum and long-row blocks (two per CU) */
 #define OG_BIN_GTHREADS 1024u  /* gather blocks (one per CU: a segment fills LDS) */
+#define OG_BIN_LTHREADS 1024u  /* long-row blocks (a whole CU each: two sum-block slots) */
 
 enum { OG_BIN_THREAD_ROWS = 0, OG_BIN_WAVE_ROWS = 1, OG_BIN_PIECE = 2 };
 
@@ -98,6 +99,7 @@ typedef struct og_bin_dev
     const og_bin_desc *bins;
     const uint32_t *long_row;    /* n_long: the long rows (new ids), longest first */
     const uint32_t *long_bin;    /* n_long: each one's first piece */
+    uint32_t *ticket;            /* the long-row pass's row counter (zeroed by the gather) */
 } og_bin_dev;
 
 typedef struct og_bin_plan
@@ -105,9 +107,10 @@ typedef struct og_bin_plan
     og_bin_dev d;
     uint64_t n_ent;              /* entries (and sources) including padding and the spares */
     uint32_t n_tiles, n_zero_bins;
+    uint64_t n_edges, long_terms;  /* in-edges, and those of the long rows */
     /* device allocations */
     void *mem_items, *mem_qent, *mem_src, *mem_pos, *mem_val, *mem_bins;
-    void *mem_long_row, *mem_long_bin;
+    void *mem_long_row, *mem_long_bin, *mem_ticket;
 } og_bin_plan;
 
 /* Builds the binned layout of a plan whose rows keep the reference's

@@ -26,6 +26,8 @@
 #define BT OG_BIN_THREADS
 #define BW (OG_BIN_THREADS / 64)
 #define GT OG_BIN_GTHREADS       /* the gather pass's blocks */
+#define LT OG_BIN_LTHREADS       /* the long-row pass's blocks */
+#define LW (OG_BIN_LTHREADS / 64)
 #define GW (OG_BIN_GTHREADS / 64)
 #define GATHER_U 16u  /* values per lane in flight in the gather */
 #define OG_BIN_HEAD 128u  /* leading terms of a long sum added one by one */
@@ -256,6 +258,7 @@ __global__ __launch_bounds__(GT) void og_bin_gather_kernel(og_bin_dev b, const f
 {
     __shared__ __align__(16) float seg[OG_BIN_SEG];
     if (*done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.ticket = 0u;   /* the long-row pass's row counter */
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (uint32_t q = blockIdx.x; q < b.n_items; q += gridDim.x)
@@ -379,11 +382,13 @@ __device__ __forceinline__ void og_bin_meta_load(const og_pr_dev &a, const og_bi
     }
 }
 
-/* og_stat over the block (fixed order); result in thread 0 */
+/* og_stat over the block of NT threads (fixed order); result in thread 0 */
+template <uint32_t NT>
 __device__ og_bin_stat og_bin_block_stat(og_bin_stat s)
 {
-    __shared__ double sh_err[BW];
-    __shared__ uint32_t sh_act[BW];
+    constexpr uint32_t W = NT / 64;
+    __shared__ double sh_err[W];
+    __shared__ uint32_t sh_act[W];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int o = 32; o > 0; o >>= 1)
     {
@@ -399,7 +404,7 @@ __device__ og_bin_stat og_bin_block_stat(og_bin_stat s)
     og_bin_stat r;
     og_bin_stat_init(r);
     if (threadIdx.x == 0)
-        for (uint32_t i = 0; i < BW; i++)
+        for (uint32_t i = 0; i < W; i++)
         {
             r.err += sh_err[i];
             r.active += sh_act[i];
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_d
             }
         }
     }
-    og_bin_stat r = og_bin_block_stat(st);
+    og_bin_stat r = og_bin_block_stat<BT>(st);
     og_bin_write_part(a, blockIdx.x, r);
 }
 
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_d
 /* long rows                                                           */
 /* ------------------------------------------------------------------ */
 
-/* One block per long row (longest first, rows q = b, b + grid, ...),
+/* One block per long row (longest first, taken in turn from a counter),
  * running beside the sum pass: the row's pieces in order, each scattered
  * from its entries into LDS in CSR order (as the sum pass does for a bin)
  * and added with the block-wide exact fast-forward, 4 consecutive terms
@@ -562,11 +567,11 @@ __global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_d
 
 /* sum (saturated at 2^25) or minimum over the 16 wavefronts' values, read
  * by every lane */
-__device__ __forceinline__ void og_bin_waves_read(const uint32_t *v, uint32_t (&w)[BW])
+__device__ __forceinline__ void og_bin_waves_read(const uint32_t *v, uint32_t (&w)[LW])
 {
     const uint4 *v4 = reinterpret_cast<const uint4 *>(v);   /* 16-byte aligned */
 #pragma unroll
-    for (uint32_t i = 0; i < BW / 4; i++)
+    for (uint32_t i = 0; i < LW / 4; i++)
     {
         const uint4 t = v4[i];
         w[4 * i] = t.x;
@@ -578,21 +583,21 @@ __device__ __forceinline__ void og_bin_waves_read(const uint32_t *v, uint32_t (&
 
 __device__ __forceinline__ uint32_t og_bin_waves_sum(const uint32_t *v)
 {
-    uint32_t w[BW];
+    uint32_t w[LW];
     og_bin_waves_read(v, w);
     uint32_t x = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < BW; i++) x = min(x + w[i], 1u << 25);
+    for (uint32_t i = 0; i < LW; i++) x = min(x + w[i], 1u << 25);
     return x;
 }
 
 __device__ __forceinline__ uint32_t og_bin_waves_min(const uint32_t *v)
 {
-    uint32_t w[BW];
+    uint32_t w[LW];
     og_bin_waves_read(v, w);
     uint32_t x = w[0];
 #pragma unroll
-    for (uint32_t i = 1; i < BW; i++) x = min(x, w[i]);
+    for (uint32_t i = 1; i < LW; i++) x = min(x, w[i]);
     return x;
 }
 
@@ -600,7 +605,7 @@ __device__ __forceinline__ uint32_t og_bin_waves_min(const uint32_t *v)
  * thread) over xs[p, clen) (LDS); par alternates the partial arrays from
  * step to step. */
 __device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, uint32_t clen, float y,
-                                                  uint32_t (&s_wsum)[2][BW], uint32_t (&s_wfail)[2][BW],
+                                                  uint32_t (&s_wsum)[2][LW], uint32_t (&s_wfail)[2][LW],
                                                   uint32_t &s_pre, uint32_t &par)
 {
     const uint32_t lane = threadIdx.x & 63;
@@ -638,10 +643,10 @@ __device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, u
         uint32_t base = 0;
         {
             /* prefix of the earlier wavefronts' sums */
-            uint32_t w[BW];
+            uint32_t w[LW];
             og_bin_waves_read(s_wsum[par], w);
 #pragma unroll
-            for (uint32_t k = 0; k < BW; k++)
+            for (uint32_t k = 0; k < LW; k++)
                 if (k < wave) base = min(base + w[k], 1u << 25);
         }
         const uint32_t X = min(base + incl_w - tot, 1u << 25);   /* before this thread's terms */
@@ -666,7 +671,7 @@ __device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, u
         {
             const uint32_t total = og_bin_waves_sum(s_wsum[par]);
             y = ldexpf((float)(Y + total), -sc);
-            p += left < OG_BIN_LTT * BT ? left : OG_BIN_LTT * BT;
+            p += left < OG_BIN_LTT * LT ? left : OG_BIN_LTT * LT;
         }
         else
         {
@@ -690,18 +695,25 @@ __device__ __forceinline__ float og_bin_block_sum(const float *xs, uint32_t p, u
     return y;
 }
 
-__global__ __launch_bounds__(BT, 4) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
+__global__ __launch_bounds__(LT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev a, uint32_t part_base)
 {
     __shared__ __align__(16) float buf[OG_BIN_CAP];
-    __shared__ __align__(16) uint32_t s_wsum[2][BW];
-    __shared__ __align__(16) uint32_t s_wfail[2][BW];
-    __shared__ uint32_t s_pre;
+    __shared__ __align__(16) uint32_t s_wsum[2][LW];
+    __shared__ __align__(16) uint32_t s_wfail[2][LW];
+    __shared__ uint32_t s_pre, s_row;
     if (*a.done) return;
     og_bin_stat st;
     og_bin_stat_init(st);
     uint32_t par = 0;    /* which s_wsum / s_wfail set a step uses */
-    for (uint32_t q = blockIdx.x; q < b.n_long; q += gridDim.x)
+    /* rows taken in turn from a counter the gather pass zeroed (longest
+     * first: each block takes the next when it is done) */
+    for (;;)
     {
+        __syncthreads();   /* the previous row's reads of buf and s_row are done */
+        if (threadIdx.x == 0) s_row = atomicAdd(b.ticket, 1u);
+        __syncthreads();
+        const uint32_t q = s_row;
+        if (q >= b.n_long) break;
         const uint32_t v = b.long_row[q];
         const uint32_t n = a.row_ptr[v + 1] - a.row_ptr[v];
         float y = 0.0f;     /* identical in every thread */
@@ -709,10 +721,22 @@ __global__ __launch_bounds__(BT, 4) void og_bin_long_kernel(og_bin_dev b, og_pr_
         for (uint32_t c0 = 0; c0 < n; c0 += OG_BIN_CAP, bin++)
         {
             const og_bin_desc d = b.bins[bin];
-            og_bin_regs regs;
-            og_bin_load(b, d, regs);
             __syncthreads();   /* the previous piece's reads of buf are done */
-            og_bin_put(b, buf, d, regs);
+            /* the piece's entries to their slots, two sweeps of 8 per thread at a time */
+            for (uint32_t e = d.ent0 + 8 * threadIdx.x; e < d.ent1; e += 16 * LT)
+            {
+                const uint32_t e2 = e + 8 * LT;
+                const bool two = e2 < d.ent1;
+                const uint32_t f = two ? e2 : e;
+                const og_u32x4 p0 = __builtin_nontemporal_load((const og_u32x4 *)(b.pos + e));
+                const og_f32x4 x0 = __builtin_nontemporal_load((const og_f32x4 *)(b.val + e));
+                const og_f32x4 x1 = __builtin_nontemporal_load((const og_f32x4 *)(b.val + e + 4));
+                const og_u32x4 p1 = __builtin_nontemporal_load((const og_u32x4 *)(b.pos + f));
+                const og_f32x4 y0 = __builtin_nontemporal_load((const og_f32x4 *)(b.val + f));
+                const og_f32x4 y1 = __builtin_nontemporal_load((const og_f32x4 *)(b.val + f + 4));
+                og_bin_scatter8(buf, p0, x0, x1);
+                if (two) og_bin_scatter8(buf, p1, y0, y1);
+            }
             __syncthreads();
             uint32_t p = 0;
             if (c0 == 0)
@@ -726,7 +750,7 @@ __global__ __launch_bounds__(BT, 4) void og_bin_long_kernel(og_bin_dev b, og_pr_
         }
         if (threadIdx.x == 0) og_bin_epilogue(a, v, y, st);
     }
-    og_bin_stat r = og_bin_block_stat(st);
+    og_bin_stat r = og_bin_block_stat<LT>(st);
     og_bin_write_part(a, part_base + blockIdx.x, r);
 }
 
@@ -739,6 +763,7 @@ extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint3
 {
     (void)it;
     if (ev0 && og_hip_check(hipEventRecord(ev0, s), "hipEventRecord")) return -1;
+    if (!b->n_items && og_launch_fill_u32(b->ticket, 0u, 1, s)) return -1;
     if (b->n_items)
         OG_LAUNCH(og_bin_gather_kernel, dim3(og_num_cus() > 0 ? og_num_cus() : 1), dim3(GT), 0, s, *b,
                   (const float *)a->g_in,
@@ -751,7 +776,7 @@ extern "C" int og_launch_bin_pull(const og_bin_dev *b, const og_pr_dev *a, uint3
         if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
             og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
             return -1;
-        OG_LAUNCH(og_bin_long_kernel, dim3(long_grid), dim3(BT), 0, side, *b, *a, sum_grid);
+        OG_LAUNCH(og_bin_long_kernel, dim3(long_grid), dim3(LT), 0, side, *b, *a, sum_grid);
         if (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord")) return -1;
     }
     OG_LAUNCH(og_bin_sum_kernel, dim3(sum_grid), dim3(BT), 0, s, *b, *a);
