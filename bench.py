@@ -465,22 +465,36 @@ def ext_multi():
         sumN += gs[-1].num_vertices; sumE += E
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    m = og.Multi(gs, 100, 1e-6)               # autotuned row-class thresholds
+    m = og.Multi(gs, 100, 1e-6)               # autotuned row-class thresholds and tile shapes
     t_setup = time.perf_counter() - t0
     m.run(2)                                  # capture + first launch
     t_graph = min(m.run(2) for _ in range(3))
     t_each = min(m.run(0) for _ in range(2))
     t_launch = m.run(1)
     iters = [m.result(i, ranks=False)[0].iterations for i in range(0, 256, 16)]
-    thresholds = sorted(set(m.hub_threshold(i) for i in range(256)))
+    hubs = [m.hub_threshold(i) for i in range(256)]
+    by_class = {}
+    for i, g in enumerate(gs):
+        cls = max(int(g.num_vertices - 1).bit_length(), 0)   # ceil(log2 N)
+        by_class.setdefault(cls, (hubs[i], m.tile(i)))
+    m.free()
+    # the same batch with the one tile shape all graphs had before tuning
+    m = og.Multi(gs, 100, 1e-6, hub_thresholds=hubs, tiles=[(256, 4, 8)] * 256)
+    m.run(2)
+    t_fixed = min(m.run(2) for _ in range(2))
     m.free()
     for g in gs:
         g.free()
+    tuned = {f"2^{c}": dict(hub_threshold=h, block=t[0], chunks_per_wave=t[1], row_batch=t[2])
+             for c, (h, t) in sorted(by_class.items())}
     return dict(workload="256 R-MAT graphs, scale U{12..20}, ef U{8..32}, a U[0.45,0.65], seed 5; tol 1e-6",
                 sum_vertices=sumN, sum_edges=sumE, build_seconds=round(t_build, 2),
-                setup_and_autotune_seconds=round(t_setup, 2), tuned_thresholds=thresholds,
+                setup_and_autotune_seconds=round(t_setup, 2), tuned_thresholds=sorted(set(hubs)),
+                tuned_per_size_class=tuned,
                 one_hipgraph_seconds=round(t_graph, 4), per_graph_launch_seconds=round(t_each, 4),
                 batched_launch_seconds=round(t_launch, 4), speedup_hipgraph_vs_per_graph=round(t_each / t_graph, 2),
+                fixed_tile_256_4_8_hipgraph_seconds=round(t_fixed, 4),
+                speedup_tuned_tiles_vs_fixed=round(t_fixed / t_graph, 3),
                 sample_iterations=iters)
 
 

@@ -212,10 +212,34 @@ int ogpu_tune_hub_threshold(ogpu_graph *g, const uint32_t *cands, uint32_t n_can
  * opt->max_iter / epsilon) on each of n graphs.  hub_thresholds[g] (or, if
  * NULL, opt->hub_threshold) fixes graph g's row-class boundary; 0 means
  * autotuned (ogpu_tune_hub_threshold over 8..128, once per size class of
- * graphs with the same ceil(log2 N), on the class's largest graph). */
+ * graphs with the same ceil(log2 N), on the class's largest graph).
+ *
+ * Each graph's pull also has a tile shape: threads per block (128, 256 or
+ * 512), 512-edge chunks one wavefront sums together (1, 2 or 4), and edges
+ * a thread loads together on its row (8, or 0: 1, 2, 4 or 8 as the
+ * wavefront's longest row needs).  Every shape gives the same ranks;
+ * ogpu_multi_tile_shapes lists the compiled ones.  tiles[g].block == 0 (or
+ * tiles NULL, as in ogpu_multi_create) means autotuned: once per size class,
+ * on all the class's graphs pulled together, timed with HIP events, by
+ * coordinate descent over block size, then chunks per wavefront, then row
+ * batch from (256, 4, 8).  The graphs of one shape share one pull launch
+ * per iteration. */
 typedef struct ogpu_multi ogpu_multi;
+typedef struct ogpu_multi_tile
+{
+    uint32_t block;            /* threads per block */
+    uint32_t chunks_per_wave;  /* 512-edge chunks a wavefront sums together */
+    uint32_t row_batch;        /* edges a thread loads together on its row; 0 = adaptive */
+    uint32_t pad;
+} ogpu_multi_tile;
 int ogpu_multi_create(ogpu_graph *const *graphs, uint32_t n, const ogpu_pr_options *opt,
                       const uint32_t *hub_thresholds, ogpu_multi **out);
+int ogpu_multi_create_tiled(ogpu_graph *const *graphs, uint32_t n, const ogpu_pr_options *opt,
+                            const uint32_t *hub_thresholds, const ogpu_multi_tile *tiles, ogpu_multi **out);
+/* The compiled tile shapes (up to max into out, which may be NULL); returns their number. */
+uint32_t ogpu_multi_tile_shapes(ogpu_multi_tile *out, uint32_t max);
+/* Graph g's tile shape (given or tuned). */
+int ogpu_multi_tile_of(const ogpu_multi *m, uint32_t g, ogpu_multi_tile *t);
 /* mode 0: one graph after another, each with its own launches and stop
  *         checks (what per-graph runs cost);
  * mode 1: every iteration of all graphs as three persistent launches over

@@ -108,6 +108,14 @@ class SpmvOptions(C.Structure):
     _fields_ = [("code", u32), ("hub_threshold", u32), ("hot_cache", u32), ("scatter", u32)]
 
 
+class MultiTile(C.Structure):
+    """ogpu_multi_tile (opengraph_amd_ext.h): a tile shape of the multi-graph pull."""
+    _fields_ = [("block", u32), ("chunks_per_wave", u32), ("row_batch", u32), ("pad", u32)]
+
+    def shape(self):
+        return (self.block, self.chunks_per_wave, self.row_batch)
+
+
 class PrOptions(C.Structure):
     """ogpu_pr_options (opengraph_amd_ext.h)."""
     _fields_ = [
@@ -235,6 +243,10 @@ SIGNATURES = {
     "ogpu_pr_set_init_learned": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
     "ogpu_tune_hub_threshold": (C.c_int, [_VP, u32p, u32, u32, u32p, C.POINTER(C.c_double)]),
     "ogpu_multi_create": (C.c_int, [C.POINTER(_VP), u32, C.POINTER(PrOptions), u32p, C.POINTER(_VP)]),
+    "ogpu_multi_create_tiled": (C.c_int, [C.POINTER(_VP), u32, C.POINTER(PrOptions), u32p,
+                                          C.POINTER(MultiTile), C.POINTER(_VP)]),
+    "ogpu_multi_tile_shapes": (u32, [C.POINTER(MultiTile), u32]),
+    "ogpu_multi_tile_of": (C.c_int, [_VP, u32, C.POINTER(MultiTile)]),
     "ogpu_multi_run": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "ogpu_multi_result": (C.c_int, [_VP, u32, C.POINTER(PrResult), _VP]),
     "ogpu_multi_hub_threshold": (u32, [_VP, u32]),
@@ -633,7 +645,10 @@ class Multi:
     """Plain PageRank on many graphs at once (ogpu_multi): mode 0 one graph
     after another, 1 all graphs per launch, 2 the same in one hipGraph."""
 
-    def __init__(self, graphs, max_iter=100, epsilon=1e-6, hub_thresholds=None, hub_threshold=0):
+    def __init__(self, graphs, max_iter=100, epsilon=1e-6, hub_thresholds=None, hub_threshold=0,
+                 tiles=None):
+        """tiles: None (every graph's tile shape autotuned), or one
+        (block, chunks_per_wave, row_batch) per graph, (0, 0, 0) to tune it."""
         lib = load()
         o = PrOptions()
         lib.ogpu_pr_default_options(C.byref(o))
@@ -644,8 +659,11 @@ class Multi:
         if hub_thresholds is not None:
             self._ht = np.ascontiguousarray(hub_thresholds, dtype=np.uint32)
             ht = self._ht.ctypes.data_as(u32p)
+        tl = None
+        if tiles is not None:
+            tl = (MultiTile * len(self.graphs))(*[MultiTile(b, k, r, 0) for (b, k, r) in tiles])
         self.h = C.c_void_p()
-        check(lib.ogpu_multi_create(hs, len(self.graphs), C.byref(o), ht, C.byref(self.h)))
+        check(lib.ogpu_multi_create_tiled(hs, len(self.graphs), C.byref(o), ht, tl, C.byref(self.h)))
 
     def run(self, mode=2) -> float:
         s = C.c_double()
@@ -660,6 +678,21 @@ class Multi:
 
     def hub_threshold(self, g):
         return load().ogpu_multi_hub_threshold(self.h, g)
+
+    def tile(self, g):
+        """Graph g's tile shape (block, chunks_per_wave, row_batch)."""
+        t = MultiTile()
+        check(load().ogpu_multi_tile_of(self.h, g, C.byref(t)))
+        return t.shape()
+
+    @staticmethod
+    def tile_shapes():
+        """The compiled tile shapes of the multi-graph pull."""
+        lib = load()
+        n = lib.ogpu_multi_tile_shapes(None, 0)
+        arr = (MultiTile * n)()
+        lib.ogpu_multi_tile_shapes(arr, n)
+        return [t.shape() for t in arr]
 
     def free(self):
         if self.h:

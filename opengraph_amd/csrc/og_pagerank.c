@@ -1260,12 +1260,17 @@ int ogpu_tune_hub_threshold(ogpu_graph *gg, const uint32_t *cands, uint32_t n_ca
 
 #define OG_MULTI_TUNE_ITERS 5u
 #define OG_MULTI_WAVES 4u      /* wavefronts per block of the long-row kernel */
+#define OG_MULTI_TILE_WARM 2u  /* untimed pulls before each tile shape's timed ones */
+#define OG_MULTI_TILE_ITERS 6u /* timed pulls per tile shape */
 
 struct ogpu_multi
 {
     uint32_t n;
     ogpu_pr **pr;
     uint32_t *hub;              /* chosen threshold per graph */
+    uint32_t *tile;             /* chosen tile shape per graph */
+    og_multi_group *groups;     /* pull launches of an iteration, one per tile shape in use */
+    uint32_t n_groups;
     og_multi_dev dev;
     og_pr_dev *d_devs;
     og_multi_work *d_work, *d_lwork, *d_iwork;
@@ -1283,7 +1288,7 @@ void ogpu_multi_free(ogpu_multi *m)
     if (!m) return;
     if (m->have_exec) (void)hipGraphExecDestroy(m->exec);
     for (uint32_t g = 0; m->pr && g < m->n; g++) ogpu_pr_free(m->pr[g]);
-    free(m->pr); free(m->hub); free(m->launched);
+    free(m->pr); free(m->hub); free(m->tile); free(m->groups); free(m->launched);
     og_dfree(m->d_devs); og_dfree(m->d_work); og_dfree(m->d_lwork); og_dfree(m->d_iwork);
     og_dfree(m->d_wr); og_dfree(m->d_lr); og_dfree(m->w_err); og_dfree(m->l_err);
     og_dfree(m->w_act); og_dfree(m->l_act); og_dfree(m->d_live); og_dfree(m->d_all_done);
@@ -1293,6 +1298,46 @@ void ogpu_multi_free(ogpu_multi *m)
 uint32_t ogpu_multi_hub_threshold(const ogpu_multi *m, uint32_t g)
 {
     return g < m->n ? m->hub[g] : 0;
+}
+
+uint32_t ogpu_multi_tile_shapes(ogpu_multi_tile *out, uint32_t max)
+{
+    const uint32_t n = og_multi_tile_count();
+    for (uint32_t i = 0; out && i < n && i < max; i++)
+    {
+        og_multi_tile t;
+        (void)og_multi_tile_shape(i, &t);
+        out[i] = (ogpu_multi_tile){t.block, t.chunks, t.row_batch, 0};
+    }
+    return n;
+}
+
+int ogpu_multi_tile_of(const ogpu_multi *m, uint32_t g, ogpu_multi_tile *t)
+{
+    if (!m || g >= m->n) { og_set_error("ogpu_multi_tile_of: graph %u of %u", g, m ? m->n : 0); return -1; }
+    og_multi_tile s;
+    (void)og_multi_tile_shape(m->tile[g], &s);
+    *t = (ogpu_multi_tile){s.block, s.chunks, s.row_batch, 0};
+    return 0;
+}
+
+/* The compiled shape with these parameters, or -1. */
+static int og_multi_tile_id(uint32_t block, uint32_t chunks, uint32_t row_batch)
+{
+    for (uint32_t i = 0; i < og_multi_tile_count(); i++)
+    {
+        og_multi_tile t;
+        (void)og_multi_tile_shape(i, &t);
+        if (t.block == block && t.chunks == chunks && t.row_batch == row_batch) return (int)i;
+    }
+    return -1;
+}
+
+static uint32_t og_class_of(uint32_t N)
+{
+    uint32_t c = 0;
+    while (c < 32 && (1u << c) < N) c++;
+    return c;
 }
 
 /* Threshold per graph: given, or tuned once per size class (graphs whose
@@ -1308,17 +1353,12 @@ static int og_multi_pick(ogpu_multi *m, og_graph *const *graphs, const ogpu_pr_o
     {
         uint32_t h = hub ? hub[g] : opt->hub_threshold;
         if (h) { m->hub[g] = h; continue; }
-        uint32_t N = graphs[g]->N, cls = 0;
-        while (cls < 32 && (1u << cls) < N) cls++;
+        const uint32_t cls = og_class_of(graphs[g]->N);
         if (cls_best[cls] < 0)
         {
             uint32_t big = g;
             for (uint32_t q = 0; q < m->n; q++)
-            {
-                uint32_t Nq = graphs[q]->N, cq = 0;
-                while (cq < 32 && (1u << cq) < Nq) cq++;
-                if (cq == cls && graphs[q]->E > graphs[big]->E) big = q;
-            }
+                if (og_class_of(graphs[q]->N) == cls && graphs[q]->E > graphs[big]->E) big = q;
             uint32_t b = 0;
             if (ogpu_tune_hub_threshold((ogpu_graph *)graphs[big], cands, 5, OG_MULTI_TUNE_ITERS, &b, NULL))
                 return -1;
@@ -1329,8 +1369,149 @@ static int og_multi_pick(ogpu_multi *m, og_graph *const *graphs, const ogpu_pr_o
     return 0;
 }
 
+/* Graph g's pull items for tile shape t (into out when not NULL): blocks of
+ * t.block / 64 wavefronts of t.chunks chunks each over the chunks
+ * [ch_first, n_chunks), then blocks of t.block rows over [C, N). */
+static size_t og_multi_items(const og_pr_dev *a, uint32_t g, const og_multi_tile *t, og_multi_work *out)
+{
+    const uint32_t nw = t->block / 64;
+    const uint32_t waves = a->n_chunks > a->ch_first ? (a->n_chunks - a->ch_first + t->chunks - 1) / t->chunks : 0;
+    const uint32_t cb = (waves + nw - 1) / nw;
+    const uint32_t rb = a->N > a->C ? (uint32_t)(((uint64_t)(a->N - a->C) + t->block - 1) / t->block) : 0;
+    if (out)
+    {
+        size_t k = 0;
+        for (uint32_t b = 0; b < cb; b++) out[k++] = (og_multi_work){g, 0, b, 0};
+        for (uint32_t b = 0; b < rb; b++) out[k++] = (og_multi_work){g, 1, b, 0};
+    }
+    return (size_t)cb + rb;
+}
+
+/* Blocks of one pull launch: enough for every CU to hold 2048 threads, no
+ * more than there are items. */
+static uint32_t og_multi_grid(uint32_t block, size_t items)
+{
+    uint64_t g = (uint64_t)(og_num_cus() > 0 ? og_num_cus() : 256) * (2048u / block);
+    if (items < g) g = items;
+    return g ? (uint32_t)g : 1u;
+}
+
+/* Milliseconds per pull of the graphs gs[0..ng) with tile shape `tile`:
+ * their items for that shape go to the front of the work list and are
+ * pulled OG_MULTI_TILE_WARM times untimed, then OG_MULTI_TILE_ITERS times
+ * between HIP events.  The graphs must be initialised (a pull iterates
+ * their ranks as a run does; the run initialises them again). */
+static int og_multi_time_tile(ogpu_multi *m, const uint32_t *gs, uint32_t ng, uint32_t tile,
+                              og_multi_work *host, hipEvent_t e0, hipEvent_t e1, float *ms)
+{
+    og_multi_tile t;
+    (void)og_multi_tile_shape(tile, &t);
+    size_t k = 0;
+    for (uint32_t i = 0; i < ng; i++) k += og_multi_items(&m->pr[gs[i]]->dev, gs[i], &t, host + k);
+    hipStream_t s = og_stream();
+    if (og_hip_check(hipMemcpyAsync(m->d_work, host, k * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D"))
+        return -1;
+    const uint32_t grid = og_multi_grid(t.block, k);
+    for (uint32_t it = 0; it < OG_MULTI_TILE_WARM; it++)
+        if (og_launch_multi_pull(&m->dev, tile, 0, (uint32_t)k, 2 + it, grid, s)) return -1;
+    if (og_hip_check(hipEventRecord(e0, s), "hipEventRecord")) return -1;
+    for (uint32_t it = 0; it < OG_MULTI_TILE_ITERS; it++)
+        if (og_launch_multi_pull(&m->dev, tile, 0, (uint32_t)k, 2 + OG_MULTI_TILE_WARM + it, grid, s)) return -1;
+    if (og_hip_check(hipEventRecord(e1, s), "hipEventRecord") ||
+        og_hip_check(hipEventSynchronize(e1), "hipEventSynchronize") ||
+        og_hip_check(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime"))
+        return -1;
+    *ms /= OG_MULTI_TILE_ITERS;
+    return 0;
+}
+
+/* Tile shape per graph: given (tiles[g].block != 0), or tuned once per size
+ * class on all the class's graphs together, by coordinate descent from 256
+ * threads, 4 chunks per wavefront, row batches of 8: the block size first,
+ * then the chunks per wavefront, then the row batch, each the fastest of
+ * its candidates with the others held. */
+static int og_multi_pick_tiles(ogpu_multi *m, const ogpu_multi_tile *tiles, og_multi_work *host)
+{
+    static const uint32_t blocks[] = {128, 256, 512}, chunks[] = {1, 2, 4}, batches[] = {0, 8};
+    uint32_t *gs = (uint32_t *)malloc((m->n ? m->n : 1) * sizeof(uint32_t));
+    int8_t done[33] = {0};
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int rc = -1;
+    if (!gs) { og_set_error("out of host memory"); return -1; }
+    for (uint32_t g = 0; g < m->n; g++)
+    {
+        if (tiles && tiles[g].block)
+        {
+            const int id = og_multi_tile_id(tiles[g].block, tiles[g].chunks_per_wave, tiles[g].row_batch);
+            if (id < 0)
+            {
+                og_set_error("ogpu_multi_create: no compiled tile shape (block %u, chunks %u, row batch %u)",
+                             tiles[g].block, tiles[g].chunks_per_wave, tiles[g].row_batch);
+                goto out;
+            }
+            m->tile[g] = (uint32_t)id;
+        }
+        else
+            m->tile[g] = UINT32_MAX;
+    }
+    for (uint32_t g = 0; g < m->n; g++)
+    {
+        if (m->tile[g] != UINT32_MAX) continue;
+        const uint32_t cls = og_class_of(m->pr[g]->dev.N);
+        if (done[cls]) continue;
+        done[cls] = 1;
+        uint32_t ng = 0;
+        for (uint32_t q = g; q < m->n; q++)
+            if (m->tile[q] == UINT32_MAX && og_class_of(m->pr[q]->dev.N) == cls) gs[ng++] = q;
+        if (!e0)
+        {
+            if (og_hip_check(hipEventCreate(&e0), "hipEventCreate") || og_hip_check(hipEventCreate(&e1), "hipEventCreate") ||
+                og_launch_multi_init(&m->dev, m->grid, og_stream()))
+                goto out;
+        }
+        float t_ms[64];
+        for (int i = 0; i < 64; i++) t_ms[i] = -1.0f;
+        uint32_t blk = 256, ch = 4, rb = 8;
+        for (int step = 0; step < 3; step++)
+        {
+            const uint32_t *c = step == 0 ? blocks : step == 1 ? chunks : batches;
+            const uint32_t nc = step == 2 ? 2 : 3;
+            uint32_t best = step == 0 ? blk : step == 1 ? ch : rb;
+            float best_ms = -1.0f;
+            for (uint32_t i = 0; i < nc; i++)
+            {
+                const int id = og_multi_tile_id(step == 0 ? c[i] : blk, step == 1 ? c[i] : ch, step == 2 ? c[i] : rb);
+                if (id < 0 || id >= 64) continue;
+                if (t_ms[id] < 0.0f && og_multi_time_tile(m, gs, ng, (uint32_t)id, host, e0, e1, &t_ms[id])) goto out;
+                if (best_ms < 0.0f || t_ms[id] < best_ms)
+                {
+                    best_ms = t_ms[id];
+                    best = c[i];
+                }
+            }
+            if (step == 0) blk = best;
+            else if (step == 1) ch = best;
+            else rb = best;
+        }
+        const int id = og_multi_tile_id(blk, ch, rb);
+        for (uint32_t i = 0; i < ng; i++) m->tile[gs[i]] = (uint32_t)id;
+    }
+    rc = 0;
+out:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    free(gs);
+    return rc;
+}
+
 int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_options *opt_in,
                       const uint32_t *hub, ogpu_multi **out)
+{
+    return ogpu_multi_create_tiled(graphs_in, n, opt_in, hub, NULL, out);
+}
+
+int ogpu_multi_create_tiled(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_options *opt_in,
+                            const uint32_t *hub, const ogpu_multi_tile *tiles, ogpu_multi **out)
 {
     *out = NULL;
     og_graph *const *graphs = (og_graph *const *)graphs_in;
@@ -1342,21 +1523,24 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
         return -1;
     }
     if (og_ctx_ensure()) return -1;
+    if (og_multi_tile_count() > 64) { og_set_error("ogpu_multi_create: too many tile shapes"); return -1; }
     ogpu_multi *m = (ogpu_multi *)calloc(1, sizeof(ogpu_multi));
     if (!m) { og_set_error("out of host memory"); return -1; }
     m->n = n;
     m->max_iter = opt.max_iter;
     m->pr = (ogpu_pr **)calloc(n ? n : 1, sizeof(ogpu_pr *));
     m->hub = (uint32_t *)calloc(n ? n : 1, 4);
+    m->tile = (uint32_t *)calloc(n ? n : 1, 4);
     m->launched = (int64_t *)calloc(n ? n : 1, 8);
-    if (!m->pr || !m->hub || !m->launched) { og_set_error("out of host memory"); ogpu_multi_free(m); return -1; }
+    if (!m->pr || !m->hub || !m->tile || !m->launched) { og_set_error("out of host memory"); ogpu_multi_free(m); return -1; }
     if (og_multi_pick(m, graphs, &opt, hub)) { ogpu_multi_free(m); return -1; }
     /* per-graph runs, device table and work lists */
     og_pr_dev *devs = (og_pr_dev *)calloc(2 * (size_t)(n ? n : 1), sizeof(og_pr_dev));
-    uint32_t *wr = (uint32_t *)calloc(n + 1, 4), *lr = (uint32_t *)calloc(n + 1, 4);
-    size_t n_work = 0, n_lwork = 0, n_iwork = 0;
+    uint32_t *wr = (uint32_t *)calloc(2 * (size_t)n + 1, 4), *lr = (uint32_t *)calloc(n + 1, 4);
+    size_t max_work = 0, n_work = 0, n_lwork = 0, n_iwork = 0;
     int rc = -1;
     og_multi_work *work = NULL, *lwork = NULL, *iwork = NULL;
+    og_multi_tile smallest = {128, 1, 0};   /* the shape with the most items */
     if (!devs || !wr || !lr) { og_set_error("out of host memory"); goto fail; }
     for (uint32_t g = 0; g < n; g++)
     {
@@ -1376,38 +1560,35 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
             a.allow_stop = 1;
             devs[2 * g + q] = a;
         }
-        wr[g] = (uint32_t)n_work;
         lr[g] = (uint32_t)n_lwork;
-        n_work += p->plan->n_chunk_blocks + p->plan->n_row_blocks;
+        max_work += og_multi_items(&p->dev, g, &smallest, NULL);
         n_lwork += (p->plan->H + OG_MULTI_WAVES - 1) / OG_MULTI_WAVES;
         n_iwork += (p->g->N + 255) / 256;
     }
-    wr[n] = (uint32_t)n_work;
     lr[n] = (uint32_t)n_lwork;
-    work = (og_multi_work *)calloc(n_work ? n_work : 1, sizeof(og_multi_work));
+    work = (og_multi_work *)calloc(max_work ? max_work : 1, sizeof(og_multi_work));
     lwork = (og_multi_work *)calloc(n_lwork ? n_lwork : 1, sizeof(og_multi_work));
     iwork = (og_multi_work *)calloc(n_iwork ? n_iwork : 1, sizeof(og_multi_work));
-    if (!work || !lwork || !iwork) { og_set_error("out of host memory"); goto fail; }
+    m->groups = (og_multi_group *)calloc(og_multi_tile_count(), sizeof(og_multi_group));
+    if (!work || !lwork || !iwork || !m->groups) { og_set_error("out of host memory"); goto fail; }
     {
-        size_t w = 0, l = 0, i = 0;
+        size_t l = 0, i = 0;
         for (uint32_t g = 0; g < n; g++)
         {
             og_plan *pl = m->pr[g]->plan;
-            for (uint32_t b = 0; b < pl->n_chunk_blocks; b++) work[w++] = (og_multi_work){g, 0, b, 0};
-            for (uint32_t b = 0; b < pl->n_row_blocks; b++) work[w++] = (og_multi_work){g, 1, b, 0};
             for (uint32_t b = 0; b < (pl->H + OG_MULTI_WAVES - 1) / OG_MULTI_WAVES; b++)
                 lwork[l++] = (og_multi_work){g, 2, b, 0};
             for (uint32_t b = 0; b < (m->pr[g]->g->N + 255) / 256; b++) iwork[i++] = (og_multi_work){g, 3, b, 0};
         }
     }
     m->d_devs = og_dalloc(2 * (size_t)(n ? n : 1) * sizeof(og_pr_dev));
-    m->d_work = og_dalloc((n_work ? n_work : 1) * sizeof(og_multi_work));
+    m->d_work = og_dalloc((max_work ? max_work : 1) * sizeof(og_multi_work));
     m->d_lwork = og_dalloc((n_lwork ? n_lwork : 1) * sizeof(og_multi_work));
     m->d_iwork = og_dalloc((n_iwork ? n_iwork : 1) * sizeof(og_multi_work));
-    m->d_wr = og_dalloc((size_t)(n + 1) * 4);
+    m->d_wr = og_dalloc(2 * (size_t)n * 4 + 4);
     m->d_lr = og_dalloc((size_t)(n + 1) * 4);
-    m->w_err = og_dalloc((n_work ? n_work : 1) * 8);
-    m->w_act = og_dalloc((n_work ? n_work : 1) * 4);
+    m->w_err = og_dalloc((max_work ? max_work : 1) * 8);
+    m->w_act = og_dalloc((max_work ? max_work : 1) * 4);
     m->l_err = og_dalloc((n_lwork ? n_lwork : 1) * 8);
     m->l_act = og_dalloc((n_lwork ? n_lwork : 1) * 4);
     m->d_live = og_dalloc(4);
@@ -1418,10 +1599,8 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
     {
         hipStream_t s = og_stream();
         if (og_hip_check(hipMemcpyAsync(m->d_devs, devs, 2 * (size_t)n * sizeof(og_pr_dev), hipMemcpyHostToDevice, s), "H2D") ||
-            og_hip_check(hipMemcpyAsync(m->d_work, work, n_work * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
             og_hip_check(hipMemcpyAsync(m->d_lwork, lwork, n_lwork * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
             og_hip_check(hipMemcpyAsync(m->d_iwork, iwork, n_iwork * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
-            og_hip_check(hipMemcpyAsync(m->d_wr, wr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s), "H2D") ||
             og_hip_check(hipMemcpyAsync(m->d_lr, lr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s), "H2D") ||
             og_hip_check(hipStreamSynchronize(s), "sync"))
             goto fail;
@@ -1431,7 +1610,6 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
     d->work = m->d_work;
     d->lwork = m->d_lwork;
     d->iwork = m->d_iwork;
-    d->n_work = (uint32_t)n_work;
     d->n_lwork = (uint32_t)n_lwork;
     d->n_iwork = (uint32_t)n_iwork;
     d->n_graphs = n;
@@ -1445,6 +1623,32 @@ int ogpu_multi_create(ogpu_graph *const *graphs_in, uint32_t n, const ogpu_pr_op
     d->all_done = m->d_all_done;
     m->grid = (uint32_t)og_num_cus() * 8;
     if (m->grid == 0) m->grid = 2048;
+    if (og_multi_pick_tiles(m, tiles, work)) goto fail;
+    /* the work list: the items of every graph of tile shape 0, then of
+     * shape 1, ...; one pull launch per shape in use */
+    for (uint32_t t = 0; t < og_multi_tile_count(); t++)
+    {
+        og_multi_tile sh;
+        (void)og_multi_tile_shape(t, &sh);
+        const size_t w0 = n_work;
+        for (uint32_t g = 0; g < n; g++)
+        {
+            if (m->tile[g] != t) continue;
+            wr[2 * g] = (uint32_t)n_work;
+            n_work += og_multi_items(&m->pr[g]->dev, g, &sh, work + n_work);
+            wr[2 * g + 1] = (uint32_t)n_work;
+        }
+        if (n_work > w0)
+            m->groups[m->n_groups++] = (og_multi_group){t, (uint32_t)w0, (uint32_t)n_work, og_multi_grid(sh.block, n_work - w0)};
+    }
+    d->n_work = (uint32_t)n_work;
+    {
+        hipStream_t s = og_stream();
+        if (og_hip_check(hipMemcpyAsync(m->d_work, work, (n_work ? n_work : 1) * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipMemcpyAsync(m->d_wr, wr, 2 * (size_t)n * 4 + 4, hipMemcpyHostToDevice, s), "H2D") ||
+            og_hip_check(hipStreamSynchronize(s), "sync"))
+            goto fail;
+    }
     rc = 0;
 fail:
     free(devs); free(wr); free(lr); free(work); free(lwork); free(iwork);
@@ -1456,7 +1660,7 @@ fail:
 static int og_multi_enqueue_all(ogpu_multi *m, hipStream_t s)
 {
     for (uint32_t it = 0; it < m->max_iter; it++)
-        if (og_launch_multi_iter(&m->dev, it, m->grid, s)) return -1;
+        if (og_launch_multi_iter(&m->dev, m->groups, m->n_groups, it, m->grid, s)) return -1;
     return 0;
 }
 

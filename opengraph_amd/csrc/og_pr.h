@@ -136,15 +136,37 @@ typedef struct og_multi_dev
     const og_multi_work *lwork;  /* long-row items */
     const og_multi_work *iwork;  /* init items */
     uint32_t n_work, n_lwork, n_iwork, n_graphs;
-    const uint32_t *wr, *lr;     /* n_graphs + 1: item ranges per graph */
+    const uint32_t *wr;          /* 2 n_graphs: [2g, 2g + 1) = graph g's pull items (those of a
+                                  * tile shape are together, so graphs are not in order) */
+    const uint32_t *lr;          /* n_graphs + 1: long-row item ranges per graph */
     double *w_err, *l_err;
     uint32_t *w_act, *l_act;
     uint32_t *live;              /* graphs not yet stopped */
     uint32_t *all_done;          /* set when live reaches 0 */
 } og_multi_dev;
 
+/* Tile shape of the multi-graph pull: threads per block, chunks per
+ * wavefront, edges per load batch of a thread's row (0: adapted to the
+ * wavefront's longest row). */
+typedef struct og_multi_tile
+{
+    uint32_t block, chunks, row_batch;
+} og_multi_tile;
+/* Pull items [w0, w1) of one tile shape, on grid blocks. */
+typedef struct og_multi_group
+{
+    uint32_t tile, w0, w1, grid;
+} og_multi_group;
+uint32_t og_multi_tile_count(void);
+int og_multi_tile_shape(uint32_t id, og_multi_tile *t);
 int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s);
-int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s);
+/* The pull of iteration `it` over items [w0, w1) with tile shape `tile`. */
+int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, uint32_t w0, uint32_t w1, uint32_t it,
+                         uint32_t grid, hipStream_t s);
+/* One iteration: the pull of every group, the long rows (grid blocks), the
+ * per-graph statistics and stop rule. */
+int og_launch_multi_iter(const og_multi_dev *m, const og_multi_group *groups, uint32_t n_groups,
+                         uint32_t it, uint32_t grid, hipStream_t s);
 int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
 /* Only the scatter of a push (next[dst] += g_in[src] over the forward

@@ -285,13 +285,6 @@ template <typename T> __device__ __forceinline__ T og_wave_allsum(T v)
     return v;
 }
 
-/* Chunks [c0, c0 + K) of one wavefront, K = OG_CH_PER_WAVE, all in flight
- * together: lane l loads edges l, l+64, ... of each chunk (the index loads
- * coalesce), gathers them and adds them in that order; the wavefront then
- * adds its 64 lane sums in a fixed tree.  Lanes past a chunk's end re-read
- * its last edge and add zero, so no branch separates the loads and the
- * K*OG_CH_SLOTS gathers of a lane overlap.  Lane k then stores chunk k's
- * sum (row with several chunks) or runs the row's epilogue. */
 /* Gather g[i].  With a hot cache (HOT) the first nh entries come from LDS
  * and those lanes' global load is pointed at g[0] -- one line, an L1 hit --
  * so only the other lanes send requests to L2, whose request rate bounds
@@ -310,17 +303,15 @@ __device__ __forceinline__ E og_gather(const E *__restrict__ g, const E *hot, ui
     }
 }
 
-/* The loads, sums and outputs of a wavefront's chunks with S slots per lane
- * (S * 64 >= the longest of them). */
-template <int MODE, bool HOT, uint32_t S>
+/* The loads, sums and outputs of a wavefront's K chunks with S slots per
+ * lane (S * 64 >= the longest of them). */
+template <int MODE, bool HOT, uint32_t S, uint32_t K>
 __device__ __forceinline__ void og_chunk_body(const og_pr_dev &a, og_stat &st, uint32_t c0, uint32_t nk,
-                                              const uint32_t (&eb)[OG_CH_PER_WAVE],
-                                              const uint32_t (&ee)[OG_CH_PER_WAVE], uint32_t my_tag,
+                                              const uint32_t (&eb)[K], const uint32_t (&ee)[K], uint32_t my_tag,
                                               const typename og_mode<MODE>::elem *hot, uint32_t nh)
 {
     typedef typename og_mode<MODE>::elem E;
     typedef typename og_mode<MODE>::acc A;
-    constexpr uint32_t K = OG_CH_PER_WAVE;
     const E *__restrict__ g = (const E *)a.g_in;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t idx[K][S];
@@ -366,8 +357,9 @@ __device__ __forceinline__ void og_chunk_body(const og_pr_dev &a, og_stat &st, u
     }
 }
 
-/* Chunks [c0, c0 + K) of one wavefront, K = OG_CH_PER_WAVE, all in flight
- * together: lane l loads edges l, l+64, ... of each chunk (the index loads
+/* Chunks [c0, c0 + K) of one wavefront (K = OG_CH_PER_WAVE but in the
+ * multi-graph kernels, whose tile shape is tuned; a block has NW
+ * wavefronts), all in flight together: lane l loads edges l, l+64, ... of each chunk (the index loads
  * coalesce), gathers them and adds them in that order; the wavefront then
  * adds its 64 lane sums in a fixed tree.  Lanes past a chunk's end re-read
  * its last edge and add zero, so no branch separates the loads and the
@@ -375,15 +367,14 @@ __device__ __forceinline__ void og_chunk_body(const og_pr_dev &a, og_stat &st, u
  * longest of the K chunks (the one-chunk rows are sorted by length, so
  * neighbouring chunks are about as long).  Lane k then stores chunk k's
  * sum (row with several chunks) or runs the row's epilogue. */
-template <int MODE, bool HOT = false>
+template <int MODE, bool HOT = false, uint32_t NW = OG_WAVES, uint32_t K = OG_CH_PER_WAVE>
 __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, uint32_t cblock,
                                                const typename og_mode<MODE>::elem *hot = nullptr,
                                                uint32_t nh = 0)
 {
-    constexpr uint32_t K = OG_CH_PER_WAVE;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (OG_BLOCK - 1)) >> 6);
-    const uint32_t c0 = a.ch_first + (cblock * OG_WAVES + wave) * K;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((threadIdx.x & (NW * 64 - 1)) >> 6);
+    const uint32_t c0 = a.ch_first + (cblock * NW + wave) * K;
     if (c0 >= a.n_chunks) return;
     const uint32_t nk = min(K, a.n_chunks - c0);
     /* descriptors: lane k holds chunk k's row tag and first edge (lane nk
@@ -400,10 +391,10 @@ __device__ __forceinline__ void og_pull_chunks(const og_pr_dev &a, og_stat &st, 
         ee[k] = __builtin_amdgcn_readlane(my_edge, kk + 1);
         longest = max(longest, ee[k] - eb[k]);
     }
-    if (longest <= 64) og_chunk_body<MODE, HOT, 1>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
-    else if (longest <= 128) og_chunk_body<MODE, HOT, 2>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
-    else if (longest <= 256) og_chunk_body<MODE, HOT, 4>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
-    else og_chunk_body<MODE, HOT, OG_CH_SLOTS>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    if (longest <= 64) og_chunk_body<MODE, HOT, 1, K>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else if (longest <= 128) og_chunk_body<MODE, HOT, 2, K>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else if (longest <= 256) og_chunk_body<MODE, HOT, 4, K>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
+    else og_chunk_body<MODE, HOT, OG_CH_SLOTS, K>(a, st, c0, nk, eb, ee, my_tag, hot, nh);
 }
 
 /* Rows [0, C) in the reference's order (fp32 pull with every row exact):
@@ -701,15 +692,16 @@ __device__ __forceinline__ typename og_mode<MODE>::acc og_row_sum(const og_pr_de
  * loads per batch follow the wavefront's longest row (1, 2, 4 or
  * OG_ROW_BATCH): most short rows of a power-law graph have one to four
  * in-edges, and rows are sorted by in-degree, so the 64 rows of a
- * wavefront are about as long.  ADAPT = false keeps OG_ROW_BATCH (fewer
- * registers, for the multi-graph kernel; the sums are the same). */
-template <int MODE, bool HOT = false, bool ADAPT = true>
+ * wavefront are about as long.  ADAPT = false keeps RB (fewer registers,
+ * for the multi-graph kernels; the sums are the same).  A block has BLK
+ * threads, one row each. */
+template <int MODE, bool HOT = false, bool ADAPT = true, uint32_t BLK = OG_BLOCK, uint32_t RB = OG_ROW_BATCH>
 __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, uint32_t rblock,
                                              const typename og_mode<MODE>::elem *hot = nullptr,
                                              uint32_t nh = 0)
 {
     typedef typename og_mode<MODE>::acc A;
-    const uint32_t v = a.C + rblock * OG_BLOCK + (threadIdx.x & (OG_BLOCK - 1));
+    const uint32_t v = a.C + rblock * BLK + (threadIdx.x & (BLK - 1));
     const bool mine = v < a.row_end && !(a.dd && !a.wl_cur[v]);
     uint32_t j = 0, e = 0;
     float prev = 0.0f;
@@ -722,7 +714,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
         if (!a.spmv) deg = __builtin_nontemporal_load(&a.outdeg[v]);
     }
     /* longest row of the wavefront (every lane still active here) */
-    uint32_t m = OG_ROW_BATCH;
+    uint32_t m = RB;
     if constexpr (ADAPT)
     {
         m = e - j;
@@ -736,7 +728,7 @@ __device__ __forceinline__ void og_pull_rows(const og_pr_dev &a, og_stat &st, ui
     if (m <= 1) s = og_row_sum<MODE, HOT, 1>(a, j, e, hot, nh, s);
     else if (m <= 2) s = og_row_sum<MODE, HOT, 2>(a, j, e, hot, nh, s);
     else if (m <= 4) s = og_row_sum<MODE, HOT, 4>(a, j, e, hot, nh, s);
-    else s = og_row_sum<MODE, HOT, OG_ROW_BATCH>(a, j, e, hot, nh, s);
+    else s = og_row_sum<MODE, HOT, ADAPT ? OG_ROW_BATCH : RB>(a, j, e, hot, nh, s);
     if (a.spmv) og_spmv_store<MODE>(a, v, s, prev, true);
     else if (a.dd) og_epilogue<MODE>(a, v, s, st);
     else og_epilogue_pf<MODE>(a, v, s, prev, deg, st);
@@ -1562,33 +1554,39 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_init_kernel(og_multi_dev m)
     }
 }
 
-/* settled != 0 from the third iteration on: row blocks entirely in [Z, N)
- * (no in-edges; rank and both contribution buffers already final) only
- * report their activeVertices count (0 >= epsilon). */
-__global__ __launch_bounds__(OG_BLOCK) void og_multi_pull_kernel(og_multi_dev m, uint32_t parity,
-                                                                uint32_t settled)
+/* Items [w0, w1) of the work list: the graphs whose tile shape is BLK
+ * threads per block, K chunks per wavefront and RB edges per load batch of
+ * a thread's row (0: as many as the wavefront's longest row needs, up to
+ * OG_ROW_BATCH).  The shape changes neither a row's sum nor a chunk's, so
+ * every shape gives the same ranks.  settled != 0 from the third iteration
+ * on: row blocks entirely in [Z, N) (no in-edges; rank and both
+ * contribution buffers already final) only report their activeVertices
+ * count (0 >= epsilon). */
+template <uint32_t BLK, uint32_t K, uint32_t RB>
+__global__ __launch_bounds__(BLK) void og_multi_pull_kernel(og_multi_dev m, uint32_t w0, uint32_t w1,
+                                                           uint32_t parity, uint32_t settled)
 {
     if (*m.all_done) return;
-    for (uint32_t i = blockIdx.x; i < m.n_work; i += gridDim.x)
+    for (uint32_t i = w0 + blockIdx.x; i < w1; i += gridDim.x)
     {
         const og_multi_work w = m.work[i];
         const og_pr_dev &a = m.devs[2 * w.g + parity];
         if (*a.done) continue;
-        const uint32_t v0 = a.C + w.first * OG_BLOCK;
+        const uint32_t v0 = a.C + w.first * BLK;
         if (w.kind == 1 && settled && v0 >= a.Z)
         {
             if (threadIdx.x == 0)
             {
                 m.w_err[i] = 0.0;
-                m.w_act[i] = 0.0 >= a.epsilon ? min((uint32_t)OG_BLOCK, a.N - v0) : 0u;
+                m.w_act[i] = 0.0 >= a.epsilon ? min(BLK, a.N - v0) : 0u;
             }
             continue;
         }
         og_stat st;
         og_stat_init(st);
-        if (w.kind == 0) og_pull_chunks<OG_MODE_F32>(a, st, w.first);
-        else og_pull_rows<OG_MODE_F32, false, false>(a, st, w.first);
-        og_stat r = og_block_stat(st);
+        if (w.kind == 0) og_pull_chunks<OG_MODE_F32, false, BLK / 64, K>(a, st, w.first);
+        else og_pull_rows<OG_MODE_F32, false, RB == 0, BLK, (RB ? RB : OG_ROW_BATCH)>(a, st, w.first);
+        og_stat r = og_block_stat<BLK / 64>(st);
         if (threadIdx.x == 0)
         {
             m.w_err[i] = r.err;
@@ -1630,7 +1628,7 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_reduce_kernel(og_multi_dev 
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    for (uint32_t i = m.wr[g] + threadIdx.x; i < m.wr[g + 1]; i += OG_BLOCK)
+    for (uint32_t i = m.wr[2 * g] + threadIdx.x; i < m.wr[2 * g + 1]; i += OG_BLOCK)
     {
         st.err += m.w_err[i];
         st.active += m.w_act[i];
@@ -1660,11 +1658,59 @@ extern "C" int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStr
     return og_hip_check(og_launch_status(), "og_multi_init_kernel");
 }
 
-extern "C" int og_launch_multi_iter(const og_multi_dev *m, uint32_t it, uint32_t grid, hipStream_t s)
+/* The compiled tile shapes: blocks of 128, 256 or 512 threads, 1, 2 or 4
+ * chunks per wavefront, row batches adapted to the wavefront or fixed at
+ * OG_ROW_BATCH. */
+#define OG_MULTI_TILES(X)                                                                        \
+    X(0, 128, 1, 0) X(1, 128, 2, 0) X(2, 128, 4, 0) X(3, 256, 1, 0) X(4, 256, 2, 0) X(5, 256, 4, 0) \
+    X(6, 512, 1, 0) X(7, 512, 2, 0) X(8, 512, 4, 0) X(9, 128, 1, 8) X(10, 128, 2, 8)              \
+    X(11, 128, 4, 8) X(12, 256, 1, 8) X(13, 256, 2, 8) X(14, 256, 4, 8) X(15, 512, 1, 8)          \
+    X(16, 512, 2, 8) X(17, 512, 4, 8)
+
+extern "C" uint32_t og_multi_tile_count(void)
+{
+#define OG_X(id, blk, k, rb) +1
+    return 0 OG_MULTI_TILES(OG_X);
+#undef OG_X
+}
+
+extern "C" int og_multi_tile_shape(uint32_t id, og_multi_tile *t)
+{
+    switch (id)
+    {
+#define OG_X(i, blk, k, rb) \
+    case i: t->block = blk; t->chunks = k; t->row_batch = rb; return 0;
+        OG_MULTI_TILES(OG_X)
+#undef OG_X
+    default: return -1;
+    }
+}
+
+extern "C" int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, uint32_t w0, uint32_t w1,
+                                    uint32_t it, uint32_t grid, hipStream_t s)
+{
+    const uint32_t parity = it & 1, settled = it >= 2;
+    if (w1 <= w0) return 0;
+    switch (tile)
+    {
+#define OG_X(i, blk, k, rb)                                                                      \
+    case i:                                                                                      \
+        OG_LAUNCH((og_multi_pull_kernel<blk, k, rb>), dim3(grid), dim3(blk), 0, s, *m, w0, w1, parity, \
+                  settled);                                                                      \
+        break;
+        OG_MULTI_TILES(OG_X)
+#undef OG_X
+    default: og_set_error("og_launch_multi_pull: no tile shape %u", tile); return -1;
+    }
+    return og_hip_check(og_launch_status(), "og_multi_pull_kernel");
+}
+
+extern "C" int og_launch_multi_iter(const og_multi_dev *m, const og_multi_group *groups, uint32_t n_groups,
+                                    uint32_t it, uint32_t grid, hipStream_t s)
 {
     const uint32_t parity = it & 1;
-    OG_LAUNCH(og_multi_pull_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity,
-                       (uint32_t)(it >= 2));
+    for (uint32_t q = 0; q < n_groups; q++)
+        if (og_launch_multi_pull(m, groups[q].tile, groups[q].w0, groups[q].w1, it, groups[q].grid, s)) return -1;
     if (m->n_lwork)
         OG_LAUNCH(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
     OG_LAUNCH(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);

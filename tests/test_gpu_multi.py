@@ -117,6 +117,77 @@ def test_autotuned_thresholds_are_candidates(batch):
     m.free()
 
 
+@pytest.fixture(scope="module")
+def singles(batch):
+    """Each graph's single-graph run (threshold 16, 30 iterations)."""
+    gs, _ = batch
+    return [_single(g, 16, 30, 1e-6) for g in gs]
+
+
+def _check_against_singles(m, gs, singles):
+    for i in range(len(gs)):
+        res, x = m.result(i)
+        r1, x1, act1, err1 = singles[i]
+        assert res.iterations == r1.iterations and res.active == r1.active, i
+        assert np.array_equal(x.view(np.uint32), x1.view(np.uint32)), i
+        assert res.error_total == pytest.approx(r1.error_total, rel=1e-12, abs=1e-300), i
+
+
+def test_every_tile_shape_matches_single_graph_runs(batch, singles):
+    """Block size, chunks per wavefront and row batch change neither a row's
+    sum nor a chunk's: every compiled shape gives each graph its
+    single-graph ranks, bit for bit."""
+    gs, _ = batch
+    shapes = og.Multi.tile_shapes()
+    assert len(shapes) == 18 and (256, 4, 8) in shapes
+    for shape in shapes:
+        m = og.Multi(gs, 30, 1e-6, hub_thresholds=[16] * len(gs), tiles=[shape] * len(gs))
+        assert all(m.tile(i) == shape for i in range(len(gs)))
+        m.run(1)
+        print("tile", shape, flush=True)
+        _check_against_singles(m, gs, singles)
+        m.free()
+
+
+def test_mixed_tile_shapes_in_one_hipgraph(batch, singles):
+    """Graphs of different shapes: one pull launch per shape in each
+    iteration, captured in one hipGraph and replayed with eager work
+    between the replays."""
+    gs, _ = batch
+    shapes = og.Multi.tile_shapes()
+    tiles = [shapes[(5 * i) % len(shapes)] for i in range(len(gs))]
+    m = og.Multi(gs, 30, 1e-6, hub_thresholds=[16] * len(gs), tiles=tiles)
+    for rep in range(3):
+        m.run(2)
+        _check_against_singles(m, gs, singles)
+        og.synchronize()
+    m.free()
+
+
+def test_autotuned_tile_shapes(batch, singles):
+    """Tuned per size class (graphs of the same ceil(log2 N)) among the
+    compiled shapes; given shapes are kept; results as single runs."""
+    gs, _ = batch
+    shapes = og.Multi.tile_shapes()
+    tiles = [(0, 0, 0)] * len(gs)
+    tiles[0] = (128, 2, 0)
+    m = og.Multi(gs, 30, 1e-6, hub_thresholds=[16] * len(gs), tiles=tiles)
+    got = [m.tile(i) for i in range(len(gs))]
+    print("tuned", got, flush=True)
+    assert got[0] == (128, 2, 0)
+    assert all(t in shapes for t in got)
+    cls = [int(np.ceil(np.log2(max(g.num_vertices, 1)))) for g in gs]
+    for i in range(1, len(gs)):
+        for j in range(1, len(gs)):
+            if cls[i] == cls[j]:
+                assert got[i] == got[j], (i, j)
+    m.run(2)
+    _check_against_singles(m, gs, singles)
+    m.free()
+    with pytest.raises(RuntimeError):
+        og.Multi(gs[:2], 10, 1e-6, hub_thresholds=[16, 16], tiles=[(192, 4, 8), (256, 4, 8)])
+
+
 def test_tuner_reports_every_candidate(batch):
     gs, _ = batch
     best, ms = og.tune_hub_threshold(gs[0], (4, 32, 256), 3)
