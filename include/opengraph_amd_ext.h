@@ -149,14 +149,15 @@ typedef struct
     uint32_t warm_classes;  /* learned start (ogpu_pr_set_init_learned): the predicted
                                start is corrected over this many in-degree classes (class
                                0 without in-edges, class c > 0 with floor(log2 in-degree)
-                               = c - 1, the last one open-ended) so that each class's mass
-                               equals that of one application of the iteration to it: one
-                               scale factor per class from the classes' mass balance
-                               s_i m_i = 0.15 n_i + sum_j A_ij s_j (m_i the start's mass
-                               in class i, n_i its vertices, A_ij the mass the iteration
-                               moves from class j to class i), one pass over the edges
-                               and an nc x nc solve.  0: default (8), 1: no correction,
-                               2..16 */
+                               = c - 1, the last one open-ended), each split into the
+                               vertices with and without out-edges, so that each class's
+                               mass equals that of one application of the iteration to
+                               it: one scale factor per class from the classes' mass
+                               balance s_i m_i = 0.15 n_i + sum_j A_ij s_j (m_i the start's
+                               mass in class i, n_i its vertices, A_ij the mass the
+                               iteration moves from class j to class i), one pass over the
+                               edges and a (2 nin) x (2 nin) solve.  0: default (16), 1: no
+                               correction, 2..16 */
 } ogpu_pr_options;
 
 typedef struct
@@ -182,6 +183,9 @@ int ogpu_pr_set_init(ogpu_pr *pr, const float *init);
  * ogpu_pr_set_init_learned) was rejected in favour of the uniform start,
  * 0 when it is used, -1 for a NULL run. */
 int ogpu_pr_init_fallback(const ogpu_pr *pr);
+/* The start vector the following runs use (N floats, old ids,
+ * unnormalized): the one set, or the uniform / reference start. */
+int ogpu_pr_get_init(ogpu_pr *pr, float *out);
 /* Run to the stop rule or max_iter and normalize; ranks stay on device. */
 int ogpu_pr_run(ogpu_pr *pr, ogpu_pr_result *res);
 /* Benchmark helpers: reset the state, then enqueue n iterations with the

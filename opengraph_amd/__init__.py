@@ -221,6 +221,7 @@ SIGNATURES = {
     "ogpu_pr_create": (C.c_int, [_VP, C.POINTER(PrOptions), C.POINTER(_VP)]),
     "ogpu_pr_set_init": (C.c_int, [_VP, _VP]),
     "ogpu_pr_init_fallback": (C.c_int, [_VP]),
+    "ogpu_pr_get_init": (C.c_int, [_VP, _VP]),
     "ogpu_pr_run": (C.c_int, [_VP, C.POINTER(PrResult)]),
     "ogpu_pr_reset": (C.c_int, [_VP]),
     "ogpu_pr_enqueue": (C.c_int, [_VP, u32, C.c_int]),
@@ -515,6 +516,12 @@ class PageRank:
     def init_fallback(self) -> bool:
         """True when the last start offered was rejected for the uniform one."""
         return load().ogpu_pr_init_fallback(self.h) == 1
+
+    def get_init(self) -> np.ndarray:
+        """The start vector of the next runs (old ids, unnormalized)."""
+        out = np.empty(self.graph.num_vertices, np.float32)
+        check(load().ogpu_pr_get_init(self.h, _ptr(out)))
+        return out
 
     def run(self) -> PrResult:
         r = PrResult()

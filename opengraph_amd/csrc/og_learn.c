@@ -392,24 +392,30 @@ int ogpu_pr_set_init_learned(ogpu_pr *p, ogpu_mlp *m, double *seconds)
     /* coarse correction of an accepted start over in-degree classes */
     og_graph *g;
     float *tmp_a, *tmp_b;
-    uint32_t nc;
-    og_pr_learn_graph(p, &g, &tmp_a, &tmp_b, &nc);
-    if (ok != 0.0 && nc >= 2)
+    uint32_t nin;
+    og_pr_learn_graph(p, &g, &tmp_a, &tmp_b, &nin);
+    if (ok != 0.0 && nin >= 2)
     {
-        unsigned long long mat[OG_WC_MAX * OG_WC_MAX];
+        const uint32_t nc = 2 * nin;
+        unsigned long long mat[OG_WC_MAX * OG_WC_MAX], qmass[OG_WC_MAX], qcount[OG_WC_MAX];
         double mass[OG_WC_MAX], count[OG_WC_MAX];
         float scale[OG_WC_MAX];
         uint8_t *cls = (uint8_t *)tmp_a;
-        if (og_launch_wc_sums(init_d, g->f_in_degree, g->f_out_degree, g->N, nc, cls, tmp_b, scr, s) ||
+        if (og_launch_wc_sums(init_d, g->f_in_degree, g->f_out_degree, g->N, nin, cls, tmp_b, scr, s) ||
             og_launch_wc_matrix(g->f_src, g->f_dst, g->E, cls, tmp_b, nc,
                                 (unsigned long long *)(scr + og_wc_matrix()), s) ||
             og_hip_check(hipMemcpyAsync(mat, scr + og_wc_matrix(), (size_t)nc * nc * 8, hipMemcpyDeviceToHost, s),
                          "D2H") ||
-            og_hip_check(hipMemcpyAsync(mass, scr + og_wc_mass(), (size_t)nc * 8, hipMemcpyDeviceToHost, s), "D2H") ||
-            og_hip_check(hipMemcpyAsync(count, scr + og_wc_count(), (size_t)nc * 8, hipMemcpyDeviceToHost, s),
+            og_hip_check(hipMemcpyAsync(qmass, scr + og_wc_mass(), (size_t)nc * 8, hipMemcpyDeviceToHost, s), "D2H") ||
+            og_hip_check(hipMemcpyAsync(qcount, scr + og_wc_count(), (size_t)nc * 8, hipMemcpyDeviceToHost, s),
                          "D2H") ||
             og_hip_check(hipStreamSynchronize(s), "sync warm correction"))
             return -1;
+        for (uint32_t i = 0; i < nc; i++)
+        {
+            mass[i] = (double)qmass[i] / 4294967296.0;
+            count[i] = (double)qcount[i];
+        }
         if (!og_wc_solve(nc, mat, mass, count, scale) &&
             (og_hip_check(hipMemcpyAsync(scr + og_wc_scale(), scale, (size_t)nc * 4, hipMemcpyHostToDevice, s),
                           "H2D") ||

@@ -63,28 +63,29 @@ int og_launch_mlp_grad_noalloc(int device, const float *params, const float *fea
  * value on a vertex with in-edges, or with no mass there, is rejected: the
  * start written is then the uniform 1.0, and scratch[OG_WARM_OK] is 0
  * (1 when accepted). */
-#define OG_WC_MAX 16                      /* classes of the coarse correction at most */
-#define OG_WC_BLOCKS 256                  /* blocks of its per-vertex pass */
+#define OG_WC_IN_MAX 16                   /* in-degree classes of the coarse correction at most */
+#define OG_WC_MAX (2 * OG_WC_IN_MAX)      /* its classes at most (each split by out-edges) */
 #define OG_WC_BASE (5 * 1024 + 64)        /* its region of the scratch */
-#define OG_WARM_SCRATCH (OG_WC_BASE + OG_WC_BLOCKS * 2 * OG_WC_MAX + OG_WC_MAX * OG_WC_MAX + 4 * OG_WC_MAX)
+#define OG_WARM_SCRATCH (OG_WC_BASE + OG_WC_MAX * OG_WC_MAX + 3 * OG_WC_MAX)
 #define OG_WARM_OK (5 * 1024 + 1)
-/* The start's coarse correction over nc in-degree classes (class 0: no
- * in-edges; class c > 0: floor(log2 in-degree) = c - 1, the last class
- * open-ended), on the graph's forward edge list in the caller's ids.
+/* The start's coarse correction over nc = 2 nin classes, on the graph's
+ * forward edge list in the caller's ids: vertex v is in class 2 i + o, i
+ * its in-degree class among nin (0: no in-edges; i > 0: floor(log2
+ * in-degree) = i - 1, the last one open-ended), o = 1 when it has
+ * out-edges.  Sums are in Q32.32 (u64, exact in any order).
  * og_launch_wc_sums: cls[v], c0[v] = r0[v] / out-degree, and per class the
- * mass of r0 and the vertex count (fixed-order double sums in
- * scratch[OG_WC_BASE ...]; results at og_wc_mass / og_wc_count).
+ * mass of r0 and the vertex count (u64 at og_wc_mass / og_wc_count).
  * og_launch_wc_matrix: mat[i * nc + j] = sum over edges u -> v with
- * class(v) = i, class(u) = j of c0[u], in Q32.32 (exact in any order).
+ * class(v) = i, class(u) = j of c0[u].
  * og_launch_wc_apply: r0[v] *= s[cls[v]]. */
 int og_launch_wc_sums(const float *r0, const uint32_t *in_degree, const uint32_t *out_degree, uint32_t n,
-                      uint32_t nc, uint8_t *cls, float *c0, double *scratch, hipStream_t s);
+                      uint32_t nin, uint8_t *cls, float *c0, double *scratch, hipStream_t s);
 int og_launch_wc_matrix(const uint32_t *src, const uint32_t *dst, uint32_t E, const uint8_t *cls,
                         const float *c0, uint32_t nc, unsigned long long *mat, hipStream_t s);
 int og_launch_wc_apply(float *r0, const uint8_t *cls, uint32_t n, const float *scale, hipStream_t s);
-static inline size_t og_wc_mass(void) { return OG_WC_BASE + OG_WC_BLOCKS * 2 * OG_WC_MAX + OG_WC_MAX * OG_WC_MAX; }
+static inline size_t og_wc_matrix(void) { return OG_WC_BASE; }
+static inline size_t og_wc_mass(void) { return OG_WC_BASE + OG_WC_MAX * OG_WC_MAX; }
 static inline size_t og_wc_count(void) { return og_wc_mass() + OG_WC_MAX; }
-static inline size_t og_wc_matrix(void) { return OG_WC_BASE + OG_WC_BLOCKS * 2 * OG_WC_MAX; }
 static inline size_t og_wc_scale(void) { return og_wc_count() + OG_WC_MAX; }
 int og_launch_warm_scale(const float *o, const uint32_t *row_ptr, const uint32_t *outdeg,
                          const uint32_t *new2old, uint32_t n, double *sums, float *r0_old,

@@ -439,70 +439,52 @@ __global__ void og_warm_write_kernel(const float *__restrict__ o, const uint32_t
 /* coarse correction of the start (in-degree classes)                  */
 /* ------------------------------------------------------------------ */
 
-__device__ __forceinline__ uint32_t og_wc_class(uint32_t indeg, uint32_t nc)
+__device__ __forceinline__ uint32_t og_wc_class(uint32_t indeg, uint32_t outdeg, uint32_t nin)
 {
-    if (!indeg) return 0;
+    const uint32_t o = outdeg ? 1u : 0u;
+    if (!indeg) return o;
     const uint32_t l = 31u - (uint32_t)__builtin_clz(indeg);
-    return 1u + min(l, nc - 2u);
+    return 2u * (1u + min(l, nin - 2u)) + o;
 }
 
-/* Per vertex: its class and c0 = r0 / out-degree; per block the mass of
- * r0 and the vertex count of each class (block b's partials at
- * part[b * 2 * OG_WC_MAX ...]); the last step folds them in block order. */
+__device__ __forceinline__ unsigned long long og_q32(float x)
+{
+    return (unsigned long long)((double)x * 4294967296.0 + 0.5);
+}
+
+/* Per vertex: its class and c0 = r0 / out-degree; per class the mass of r0
+ * and the vertex count, in Q32.32 and integers (block sums in LDS, then one
+ * global add per class and block). */
 __global__ __launch_bounds__(LB) void og_wc_sums_kernel(const float *__restrict__ r0,
                                                        const uint32_t *__restrict__ indeg,
-                                                       const uint32_t *__restrict__ outdeg, uint32_t n, uint32_t nc,
+                                                       const uint32_t *__restrict__ outdeg, uint32_t n, uint32_t nin,
                                                        uint8_t *__restrict__ cls, float *__restrict__ c0,
-                                                       double *__restrict__ part)
+                                                       unsigned long long *__restrict__ mass,
+                                                       unsigned long long *__restrict__ count)
 {
-    __shared__ double sm[OG_WC_MAX][LB];
-    __shared__ double sn[OG_WC_MAX][LB];
-    for (uint32_t k = 0; k < OG_WC_MAX; k++)
+    __shared__ unsigned long long sm[OG_WC_MAX], sn[OG_WC_MAX];
+    if (threadIdx.x < OG_WC_MAX)
     {
-        sm[k][threadIdx.x] = 0.0;
-        sn[k][threadIdx.x] = 0.0;
+        sm[threadIdx.x] = 0ull;
+        sn[threadIdx.x] = 0ull;
     }
+    __syncthreads();
     for (uint32_t v = blockIdx.x * LB + threadIdx.x; v < n; v += gridDim.x * LB)
     {
-        const uint32_t c = og_wc_class(indeg[v], nc);
         const uint32_t d = outdeg[v];
+        const uint32_t c = og_wc_class(indeg[v], d, nin);
         const float x = r0[v];
         cls[v] = (uint8_t)c;
         c0[v] = d ? x / (float)d : 0.0f;
-        sm[c][threadIdx.x] += x;
-        sn[c][threadIdx.x] += 1.0;
+        atomicAdd(&sm[c], og_q32(x));
+        atomicAdd(&sn[c], 1ull);
     }
     __syncthreads();
-    for (uint32_t w = LB / 2; w > 0; w >>= 1)
+    if (threadIdx.x < 2 * nin)
     {
-        if (threadIdx.x < w)
-            for (uint32_t k = 0; k < OG_WC_MAX; k++)
-            {
-                sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
-                sn[k][threadIdx.x] += sn[k][threadIdx.x + w];
-            }
-        __syncthreads();
+        if (sm[threadIdx.x]) atomicAdd(&mass[threadIdx.x], sm[threadIdx.x]);
+        if (sn[threadIdx.x]) atomicAdd(&count[threadIdx.x], sn[threadIdx.x]);
     }
-    if (threadIdx.x < OG_WC_MAX)
-    {
-        part[(size_t)blockIdx.x * 2 * OG_WC_MAX + threadIdx.x] = sm[threadIdx.x][0];
-        part[(size_t)blockIdx.x * 2 * OG_WC_MAX + OG_WC_MAX + threadIdx.x] = sn[threadIdx.x][0];
-    }
-}
-
-__global__ void og_wc_fold_kernel(const double *__restrict__ part, double *__restrict__ mass,
-                                  double *__restrict__ count)
-{
-    const uint32_t k = threadIdx.x;
-    if (k >= OG_WC_MAX) return;
-    double m = 0.0, c = 0.0;
-    for (uint32_t b = 0; b < OG_WC_BLOCKS; b++)
-    {
-        m += part[(size_t)b * 2 * OG_WC_MAX + k];
-        c += part[(size_t)b * 2 * OG_WC_MAX + OG_WC_MAX + k];
-    }
-    mass[k] = m;
-    count[k] = c;
 }
 
 /* mat[class(v) * nc + class(u)] += c0[u] over the forward edges (u, v),
@@ -520,11 +502,7 @@ __global__ __launch_bounds__(LB) void og_wc_matrix_kernel(const uint32_t *__rest
     {
         const uint32_t u = src[e], v = dst[e];
         const float x = c0[u];
-        if (x > 0.0f)
-        {
-            const unsigned long long q = (unsigned long long)((double)x * 4294967296.0 + 0.5);
-            atomicAdd(&sh[cls[v] * nc + cls[u]], q);
-        }
+        if (x > 0.0f) atomicAdd(&sh[cls[v] * nc + cls[u]], og_q32(x));
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nc * nc; k += LB)
@@ -551,17 +529,19 @@ static unsigned og_lgrid(uint64_t n)
 }
 
 extern "C" int og_launch_wc_sums(const float *r0, const uint32_t *in_degree, const uint32_t *out_degree,
-                                 uint32_t n, uint32_t nc, uint8_t *cls, float *c0, double *scratch, hipStream_t s)
+                                 uint32_t n, uint32_t nin, uint8_t *cls, float *c0, double *scratch, hipStream_t s)
 {
-    if (nc < 2 || nc > OG_WC_MAX)
+    if (nin < 2 || nin > OG_WC_IN_MAX)
     {
-        og_set_error("coarse start correction: %u classes (2..%u)", nc, OG_WC_MAX);
+        og_set_error("coarse start correction: %u in-degree classes (2..%u)", nin, OG_WC_IN_MAX);
         return -1;
     }
-    OG_LAUNCH(og_wc_sums_kernel, dim3(OG_WC_BLOCKS), dim3(LB), 0, s, r0, in_degree, out_degree, n, nc,
-                       cls, c0, scratch + OG_WC_BASE);
-    OG_LAUNCH(og_wc_fold_kernel, dim3(1), dim3(64), 0, s, scratch + OG_WC_BASE, scratch + og_wc_mass(),
-                       scratch + og_wc_count());
+    unsigned long long *mass = (unsigned long long *)(scratch + og_wc_mass());
+    unsigned long long *count = (unsigned long long *)(scratch + og_wc_count());
+    if (og_launch_fill_u32((uint32_t *)mass, 0, 4 * OG_WC_MAX, s)) return -1;   /* mass and count */
+    if (n)
+        OG_LAUNCH(og_wc_sums_kernel, dim3(og_lgrid(n) < 1024 ? og_lgrid(n) : 1024), dim3(LB), 0, s, r0,
+                  in_degree, out_degree, n, nin, cls, c0, mass, count);
     return og_hip_check(og_launch_status(), "og_wc_sums_kernel");
 }
 

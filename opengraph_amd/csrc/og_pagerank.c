@@ -134,9 +134,9 @@ void ogpu_pr_free(ogpu_pr *p)
 int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, uint32_t *n_final,
                       float **c1, float **init_d, float **feat, double **scratch)
 {
-    if (p->opt.warm_classes > OG_WC_MAX)
+    if (p->opt.warm_classes > OG_WC_IN_MAX)
     {
-        og_set_error("learned start: warm_classes %u (0, 1 or 2..%u)", p->opt.warm_classes, OG_WC_MAX);
+        og_set_error("learned start: warm_classes %u (0, 1 or 2..%u)", p->opt.warm_classes, OG_WC_IN_MAX);
         return -1;
     }
     if (p->mode != OG_MODE_F32 || p->push || p->dd)
@@ -159,13 +159,14 @@ int og_pr_learn_parts(ogpu_pr *p, const og_pr_dev **dev, const og_plan **plan, u
 }
 
 /* The graph, two N-float buffers free until the next run's init (r, c[0])
- * and the number of correction classes (1: none) of a learned start. */
+ * and the number of in-degree classes of a learned start's correction
+ * (1: none). */
 int og_pr_learn_graph(ogpu_pr *p, og_graph **g, float **tmp_a, float **tmp_b, uint32_t *classes)
 {
     *g = p->g;
     *tmp_a = p->r;
     *tmp_b = (float *)p->c[0];
-    *classes = p->opt.warm_classes ? p->opt.warm_classes : 8u;
+    *classes = p->opt.warm_classes ? p->opt.warm_classes : OG_WC_IN_MAX;
     return 0;
 }
 
@@ -180,6 +181,16 @@ void og_pr_use_init(ogpu_pr *p, int accepted)
 int ogpu_pr_init_fallback(const ogpu_pr *p)
 {
     return p ? (int)p->init_fallback : -1;
+}
+
+int ogpu_pr_get_init(ogpu_pr *p, float *out)
+{
+    const uint32_t N = p->g->N;
+    if (p->opt.init_mode == 2 && p->init_d)
+        return og_hip_check(og_copy(out, p->init_d, (size_t)N * 4, hipMemcpyDeviceToHost), "D2H start");
+    const float x = p->opt.init_mode == 1 ? 1.0f : OG_BASE_PR;
+    for (uint32_t v = 0; v < N; v++) out[v] = x;
+    return 0;
 }
 
 /* The rows [0, C) of a plan ordered by in-degree, longest first (stable):
