@@ -524,15 +524,22 @@ def ext_learned(n_train=200, n_eval=8, epochs=8):
     t_data = time.perf_counter() - t0
     loss, t_train = m.train(epochs, 65536, 3e-3, 1)
     n_vertices = m.data_size()
+    # the method without the trained network: a constant prediction (zero
+    # weights, softplus(1) everywhere) through the same scaling and class
+    # correction
+    const = og.Mlp(6)
+    pc = np.zeros(const.params().size, np.float32)
+    pc[-1] = 1.0
+    const.set_params(pc)
     evals = []
     for k in range(n_eval):
         g, a, ef = graph(24, 100 + k, np.random.default_rng(100 + k))
         row = dict(seed=100 + k, a=round(a, 3), edge_factor=ef)
-        for name in ("uniform", "learned"):
+        for name, net in (("uniform", None), ("constant_corrected", const), ("learned", m)):
             pr = og.PageRank(g, 0, 1000, 1e-6, stop_rule=1, init_mode=1)
             og.synchronize()
             t = time.perf_counter()
-            ws = pr.set_init_learned(m) if name == "learned" else 0.0
+            ws = pr.set_init_learned(net) if net is not None else 0.0
             r = pr.run()
             t = time.perf_counter() - t
             row[name] = dict(iterations=int(r.rows), seconds=round(t, 4), warm_start_seconds=round(ws, 4),
@@ -543,8 +550,10 @@ def ext_learned(n_train=200, n_eval=8, epochs=8):
         log(f"  learned eval {row}")
         g.free()
     m.free()
+    const.free()
     tu = sum(e["uniform"]["seconds"] for e in evals)
     tl = sum(e["learned"]["seconds"] for e in evals)
+    tc = sum(e["constant_corrected"]["seconds"] for e in evals)
     return dict(workload=f"train {n_train} R-MAT-18 (seed 6), eval {n_eval} R-MAT-24 (seeds 100..), L1<1e-6",
                 training_vertices=int(n_vertices), data_seconds=round(t_data, 2),
                 train_epochs=epochs, train_seconds=round(t_train, 2), final_train_loss=round(loss, 5),
@@ -552,7 +561,11 @@ def ext_learned(n_train=200, n_eval=8, epochs=8):
                 total_seconds_uniform=round(tu, 4), total_seconds_learned=round(tl, 4),
                 time_saved_frac=round(1 - tl / tu, 4),
                 iterations_uniform_mean=float(np.mean([e["uniform"]["iterations"] for e in evals])),
-                iterations_learned_mean=float(np.mean([e["learned"]["iterations"] for e in evals])))
+                iterations_learned_mean=float(np.mean([e["learned"]["iterations"] for e in evals])),
+                total_seconds_constant_corrected=round(tc, 4),
+                iterations_constant_corrected_mean=float(np.mean([e["constant_corrected"]["iterations"]
+                                                                   for e in evals])),
+                time_saved_frac_vs_constant_corrected=round(1 - tl / tc, 4))
 
 
 def main():

@@ -488,7 +488,10 @@ __global__ __launch_bounds__(LB) void og_wc_sums_kernel(const float *__restrict_
 }
 
 /* mat[class(v) * nc + class(u)] += c0[u] over the forward edges (u, v),
- * in Q32.32: block sums in LDS, then one global add per entry and block. */
+ * in Q32.32: block sums in LDS, then one global add per entry and block.
+ * Each thread takes WC_U edges LB apart per step, all their loads (the
+ * edge pair, then c0 and both classes) issued before the adds. */
+#define WC_U 8
 __global__ __launch_bounds__(LB) void og_wc_matrix_kernel(const uint32_t *__restrict__ src,
                                                          const uint32_t *__restrict__ dst, uint32_t E,
                                                          const uint8_t *__restrict__ cls,
@@ -498,11 +501,29 @@ __global__ __launch_bounds__(LB) void og_wc_matrix_kernel(const uint32_t *__rest
     __shared__ unsigned long long sh[OG_WC_MAX * OG_WC_MAX];
     for (uint32_t k = threadIdx.x; k < OG_WC_MAX * OG_WC_MAX; k += LB) sh[k] = 0ull;
     __syncthreads();
-    for (uint32_t e = blockIdx.x * LB + threadIdx.x; e < E; e += gridDim.x * LB)
+    const uint64_t step = (uint64_t)gridDim.x * LB * WC_U;
+    for (uint64_t base = (uint64_t)blockIdx.x * LB * WC_U + threadIdx.x; base < E; base += step)
     {
-        const uint32_t u = src[e], v = dst[e];
-        const float x = c0[u];
-        if (x > 0.0f) atomicAdd(&sh[cls[v] * nc + cls[u]], og_q32(x));
+        uint32_t u[WC_U], v[WC_U];
+#pragma unroll
+        for (uint32_t k = 0; k < WC_U; k++)
+        {
+            const uint64_t e = min(base + (uint64_t)k * LB, (uint64_t)E - 1);   /* past the end: re-read, not added */
+            u[k] = __builtin_nontemporal_load(&src[e]);
+            v[k] = __builtin_nontemporal_load(&dst[e]);
+        }
+        float x[WC_U];
+        uint32_t cu[WC_U], cv[WC_U];
+#pragma unroll
+        for (uint32_t k = 0; k < WC_U; k++)
+        {
+            x[k] = c0[u[k]];
+            cu[k] = cls[u[k]];
+            cv[k] = cls[v[k]];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < WC_U; k++)
+            if (base + (uint64_t)k * LB < E && x[k] > 0.0f) atomicAdd(&sh[cv[k] * nc + cu[k]], og_q32(x[k]));
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nc * nc; k += LB)
@@ -550,7 +571,8 @@ extern "C" int og_launch_wc_matrix(const uint32_t *src, const uint32_t *dst, uin
 {
     if (og_launch_fill_u32((uint32_t *)mat, 0, 2ull * nc * nc, s)) return -1;
     if (!E) return 0;
-    OG_LAUNCH(og_wc_matrix_kernel, dim3(og_lgrid(E) < 2048 ? og_lgrid(E) : 2048), dim3(LB), 0, s, src,
+    const uint64_t steps = ((uint64_t)E + LB * WC_U - 1) / (LB * WC_U);
+    OG_LAUNCH(og_wc_matrix_kernel, dim3(steps < 2048 ? (unsigned)steps : 2048u), dim3(LB), 0, s, src,
                        dst, E, cls, c0, nc, mat);
     return og_hip_check(og_launch_status(), "og_wc_matrix_kernel");
 }
