@@ -25,7 +25,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libopengraph_amd.so")
+# OGPU_LIB names another build of the same library (the sanitized host build)
+LIB_PATH = os.environ.get("OGPU_LIB") or os.path.join(_HERE, "libopengraph_amd.so")
 
 u32 = C.c_uint32
 u32p = C.POINTER(C.c_uint32)

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(_HERE, "libogoracle.so")
+# OG_ORACLE_LIB names another build of the same restatement (the sanitized one)
+ORACLE_LIB = os.environ.get("OG_ORACLE_LIB") or os.path.join(_HERE, "libogoracle.so")
 REF_LIB = os.path.join(_HERE, "_ref", "libogref.so")
 REF_HARNESS = os.path.join(_HERE, "_ref", "ref_harness")
 
