@@ -117,14 +117,18 @@ typedef struct
                                edges), 1 on, 2 off -- one persistent block per CU that
                                serves the gathers of the lowest vertex ids (the hottest
                                after relabelling) from a 157 KB LDS copy; same results.
-                               push (variants 1, 3, 5): the destinations with the lowest
-                               ids add their in-edges in LDS accumulators and flush once
-                               per block; 0 default (on for variant 3 when those
-                               destinations receive >= 1/5 of the edges), 1 on, 2 off.
-                               Variant 3 is exact either way; for 1 and 5 the block
-                               partials are more accurate than the reference's running
-                               fp32 sums and may differ from them by ~1e-5 relative on
-                               rows with thousands of in-edges */
+                               push (variants 1, 3, 5): 0 default: partitioned -- the
+                               destinations in ranges of at most 16384 rows, one block
+                               per range adds each in-edge's contribution to its
+                               destination's accumulator with an LDS atomic (one running
+                               sum per destination, as the reference's atomics) and runs
+                               the update; 1: atomic scatter over the forward edges with
+                               LDS accumulators for the destinations with the lowest ids,
+                               flushed once per block (their block partials are more
+                               accurate than the reference's running fp32 sums and may
+                               differ from them by ~1e-5 relative on rows with thousands
+                               of in-edges); 2: plain atomic scatter.  Variant 3 is exact
+                               in every form */
     uint32_t exact_above;   /* fp32 pull (variant 0, not personalized, relabel 1 or 2): the
                                rows of more than max(exact_above, 512) in-edges are summed
                                in the reference's order, one wavefront each, instead of in
@@ -136,7 +140,7 @@ typedef struct
                                (variant 1): when not 0, the destinations of more than
                                1024 in-edges are gathered in CSR order the same way
                                (the reference's source-ordered adds) and left out of the
-                               atomic scatter */
+                               atomic adds */
     uint32_t binned;        /* fp32 pull (variant 0, not personalized): 0 default (on),
                                1 on, 2 off -- the binned pull: per iteration a gather pass
                                writes each edge's contribution, source segment by source

@@ -125,6 +125,7 @@ void ogpu_pr_free(ogpu_pr *p)
     og_dfree(p->feat); og_dfree(p->lscratch);
     og_dfree(p->chain_rows);
     og_bin_plan_destroy(p->bins);
+    og_dfree(p->pp_v0); og_dfree(p->pp_dloc);
     free(p);
 }
 
@@ -231,6 +232,65 @@ static int og_pr_chain_rows(const og_plan *pl, uint32_t **out)
     if (rc) { og_dfree(d); return -1; }
     *out = d;
     return 0;
+}
+
+/* Destination ranges of the partitioned push: consecutive rows of the plan,
+ * at most OG_PP_ROWS (its LDS accumulators) and about E / (8 CUs) in-edges
+ * each (a single row may have more), with a boundary at row first (the
+ * chains before it are gathered instead); each inverse-CSR edge's offset in
+ * its range on the device. */
+static int og_push_parts(ogpu_pr *p, uint32_t first)
+{
+    const og_plan *pl = p->plan;
+    const uint32_t N = pl->N, E = pl->E;
+    uint32_t *rp = (uint32_t *)malloc(((size_t)N + 1) * 4);
+    uint32_t *v0 = (uint32_t *)malloc(((size_t)N + 2) * 4);   /* at most N ranges */
+    if (!rp || !v0)
+    {
+        free(rp); free(v0);
+        og_set_error("out of host memory");
+        return -1;
+    }
+    int rc = -1;
+    if (og_hip_check(og_copy(rp, pl->row_ptr, ((size_t)N + 1) * 4, hipMemcpyDeviceToHost), "D2H row_ptr")) goto out;
+    {
+        const uint32_t cus = og_num_cus() > 0 ? (uint32_t)og_num_cus() : 256u;
+        uint64_t cap = (uint64_t)E / (8ull * cus);
+        if (cap < 4096) cap = 4096;
+        uint32_t n = 0, v = 0, pf = 0;
+        v0[n++] = 0;
+        while (v < N)
+        {
+            const uint32_t start = v;
+            uint64_t e = 0;
+            if (v == first) pf = n - 1;
+            while (v < N && v - start < OG_PP_ROWS)
+            {
+                if (v == first && v != start) break;
+                const uint32_t d = rp[v + 1] - rp[v];
+                if (v != start && e + d > cap) break;
+                e += d;
+                v++;
+            }
+            v0[n++] = v;
+        }
+        if (first >= N) pf = n - 1;   /* every row a chain: nothing pushed */
+        p->n_pp = n - 1;
+        p->pp_first = pf;
+        p->pp_v0 = og_dalloc((size_t)n * 4);
+        p->pp_dloc = og_dalloc((size_t)E * 2 + 2);
+        if (!p->pp_v0 || !p->pp_dloc) goto out;
+        hipStream_t s = og_stream();
+        if (og_hip_check(hipMemcpyAsync(p->pp_v0, v0, (size_t)n * 4, hipMemcpyHostToDevice, s), "H2D") ||
+            og_launch_push_dloc(pl->row_ptr, p->pp_v0, p->n_pp, p->pp_dloc, s) ||
+            og_hip_check(hipStreamSynchronize(s), "sync push ranges"))
+            goto out;
+    }
+    rc = 0;
+out:
+    free(rp);
+    free(v0);
+    return rc;
 }
 
 /* Create a run on graph g.  plan may be supplied (borrowed) or built here. */
@@ -362,29 +422,15 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
         }
         if (on) hot_grid = (uint32_t)og_num_cus();
     }
-    else if (p->push && !p->dd && opt->hot_cache != 2 && E)
+    else if (p->push && !p->dd && opt->hot_cache == 1 && E)
     {
-        /* push scatter with LDS accumulators for the destinations with the
-         * most in-edges (the smallest ids).  By default for the u64
-         * fixed-point push (exact in any order) when they receive at least a
-         * fifth of the edges; fp32 and quantized pushes only on request
-         * (hot_cache 1): their block partials are more accurate than the
+        /* the scatter over the forward edges with LDS accumulators for the
+         * destinations with the most in-edges (the smallest ids), on
+         * request: their block partials are more accurate than the
          * reference's running fp32 sums, which drop the small terms of a
          * long in-edge list, and can differ from them by a little over
-         * 1e-5 relative on such rows. */
-        int on = opt->hot_cache == 1;
-        const uint32_t nh = (uint32_t)(OG_HOT_BYTES / (p->mode == OG_MODE_FX64 ? 8u : 4u));
-        if (!on && p->mode == OG_MODE_FX64 && N > nh)
-        {
-            uint32_t hot_edges = 0;
-            if (og_hip_check(og_copy(&hot_edges, pl->row_ptr + nh, 4, hipMemcpyDeviceToHost), "D2H"))
-            {
-                ogpu_pr_free(p);
-                return -1;
-            }
-            on = (uint64_t)hot_edges * 5 >= (uint64_t)E;
-        }
-        if (on) hot_grid = (uint32_t)og_num_cus();
+         * 1e-5 relative on such rows (exact for the u64 push) */
+        hot_grid = (uint32_t)og_num_cus();
     }
     /* chain blocks (four rows each) and chunk-wave blocks: every row exact,
      * chains [0, C) and no chunks; fp32 pull on a plan with chains [0, L),
@@ -409,18 +455,31 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     /* four chains per 256-thread pull block; beside the persistent kernel
      * (or the push scatter), one wavefront per CU taking every grid-th chain */
     const uint32_t cus = (uint32_t)og_num_cus();
-    const uint32_t chain_blocks = push_chains ? (n_chains < cus ? n_chains : cus)
+    /* (the partitioned push leaves its range blocks room for eight chain
+     * wavefronts per CU beside them; the scatter, one) */
+    const uint32_t push_chain_cap = opt->hot_cache == 0 ? 8 * cus : cus;
+    const uint32_t chain_blocks = push_chains ? (n_chains < push_chain_cap ? n_chains : push_chain_cap)
                                 : hot_grid && !p->chain_rows ? (n_chains < hot_grid ? n_chains : hot_grid)
                                 : (n_chains + 3) / 4;
     const uint32_t pull_blocks = hot_grid ? hot_grid + chain_blocks : chain_blocks + chunk_blocks + pl->n_row_blocks;
     p->n_final_pull = og_final_blocks(pull_blocks, pl->H);
     p->n_final_vert = og_final_blocks(p->gv + (push_chains ? chain_blocks : 0), 0);
+    /* the default push (hot_cache 0): partitioned by destination range */
+    if (p->push && !p->dd && opt->hot_cache == 0 && og_push_parts(p, push_chains ? n_chains : 0))
+    {
+        ogpu_pr_free(p);
+        return -1;
+    }
+    const uint32_t pp_blocks = p->pp_v0 ? p->n_pp - p->pp_first + (push_chains ? chain_blocks : 0) : 0;
+    p->n_final_pp = og_final_blocks(pp_blocks, 0);
     uint32_t parts = pull_blocks > p->gv ? pull_blocks : p->gv;
+    if (parts < pp_blocks) parts = pp_blocks;
     if (push_chains && parts < p->gv + chain_blocks) parts = p->gv + chain_blocks;
     if (p->bins && parts < p->bin_sum_grid + p->bin_long_grid) parts = p->bin_sum_grid + p->bin_long_grid;
     p->parts_cap = parts;
     uint32_t fins = p->n_final_pull > p->n_final_vert ? p->n_final_pull : p->n_final_vert;
     if (fins < p->n_final_bin) fins = p->n_final_bin;
+    if (fins < p->n_final_pp) fins = p->n_final_pp;
     p->stats_cap = (opt->max_iter ? opt->max_iter : 1) + 1;
 
     size_t eb = og_elem_bytes(p->mode);
@@ -512,6 +571,10 @@ int og_pr_create_with(og_graph *g, og_plan *plan, const ogpu_pr_options *opt, og
     a->old2new = pl->old2new;
     a->push_src = pl->push_src;
     a->push_dst = pl->push_dst;
+    a->pp_v0 = p->pp_v0;
+    a->pp_dloc = p->pp_dloc;
+    a->n_pp = p->n_pp;
+    a->pp_first = p->pp_first;
     a->r = p->r;
     a->next = p->next;
     a->chunk_sum = p->chunk_sum;
@@ -672,6 +735,12 @@ static int og_pr_launch_iter(ogpu_pr *p, uint32_t it, int allow_stop, hipEvent_t
             return -1;
         }
         rc = og_launch_pull(&a, p->mode, row, s, e0, e1);
+    }
+    else if (a.pp_v0)
+    {
+        a.n_parts = a.n_pp - a.pp_first + (p->mode == OG_MODE_F32 && a.push_skip ? a.n_chain_blocks : 0);
+        a.n_final_blocks = p->n_final_pp;
+        rc = og_launch_push(&a, p->mode, row, s, e0, e1);
     }
     else
     {

@@ -73,6 +73,13 @@ typedef struct og_pr_dev
     /* fp32 push: destinations [0, push_skip) are the chains, summed by
      * gathering in CSR order; the scatter and its epilogue leave them out */
     uint32_t push_skip;
+    /* partitioned push (pp_v0 set): destination ranges [pp_v0[p],
+     * pp_v0[p + 1]) of at most OG_PP_ROWS rows, ranges [pp_first, n_pp)
+     * pushed (pp_v0[pp_first] = push_skip); pp_dloc[e] = the destination of
+     * inverse-CSR edge e minus its range's first row */
+    const uint32_t *pp_v0;
+    const uint16_t *pp_dloc;
+    uint32_t n_pp, pp_first;
     double epsilon;
     /* plan */
     const uint32_t *row_ptr, *cols, *outdeg, *chunk_base, *ch_row, *ch_edge;
@@ -169,6 +176,10 @@ int og_launch_multi_iter(const og_multi_dev *m, const og_multi_group *groups, ui
                          uint32_t it, uint32_t grid, hipStream_t s);
 int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStream_t s, hipEvent_t ev0,
                    hipEvent_t ev1);
+/* Destination rows per range of the partitioned push (its LDS accumulators). */
+#define OG_PP_ROWS 16384u
+int og_launch_push_dloc(const uint32_t *row_ptr, const uint32_t *pp_v0, uint32_t n_pp, uint16_t *dloc,
+                        hipStream_t s);
 /* Only the scatter of a push (next[dst] += g_in[src] over the forward
  * edges), without the epilogue: SpMV's push accumulates into next. */
 int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t s, hipEvent_t ev0,

@@ -1051,41 +1051,49 @@ __global__ __launch_bounds__(HOT ? OG_HOT_SUBS * OG_BLOCK : OG_BLOCK) void og_pu
     }
 }
 
+/* The update of vertex v from its push sum s (pageRank.c:1117-1134 and
+ * the next contribution): u64 fixed point, fp32, or the quantized push's
+ * fp32 update with the fp32 contribution and its min / max. */
+template <int PMODE, typename T>
+__device__ __forceinline__ void og_push_vertex(const og_pr_dev &a, uint32_t v, T s, og_stat &st)
+{
+    if constexpr (PMODE == OG_MODE_FX64) og_epilogue<OG_MODE_FX64>(a, v, (uint64_t)s, st);
+    else if constexpr (PMODE == OG_MODE_F32) og_epilogue<OG_MODE_F32>(a, v, (float)s, st);
+    else
+    {
+        float prev = a.r[v];
+        float nr = OG_BASE_PR + OG_DAMP * (float)s;
+        a.r[v] = nr;
+        double err = fabs((double)(nr - prev));
+        st.err += err / (double)a.N;
+        st.active += err >= a.epsilon;
+        uint32_t deg = a.outdeg[v];
+        float c = deg ? nr / (float)deg : 0.0f;
+        ((float *)a.c_out)[v] = c;
+        st.cmin = fminf(st.cmin, c);
+        st.cmax = fmaxf(st.cmax, c);
+    }
+}
+
 template <int PMODE>
 __global__ __launch_bounds__(OG_BLOCK) void og_push_epilogue_kernel(og_pr_dev a)
 {
     if (*a.done) return;
     og_stat st;
     og_stat_init(st);
-    const int EMODE = PMODE == OG_MODE_FX64 ? OG_MODE_FX64 : (PMODE == OG_MODE_F32 ? OG_MODE_F32 : OG_MODE_QPUSH_EPI);
     for (uint32_t v = a.push_skip + blockIdx.x * OG_BLOCK + threadIdx.x; v < a.N; v += gridDim.x * OG_BLOCK)
     {
-        if constexpr (EMODE == OG_MODE_FX64)
+        if constexpr (PMODE == OG_MODE_FX64)
         {
             uint64_t s = ((uint64_t *)a.next)[v];
             ((uint64_t *)a.next)[v] = 0;
-            og_epilogue<OG_MODE_FX64>(a, v, s, st);
+            og_push_vertex<PMODE>(a, v, s, st);
         }
         else
         {
             float s = ((float *)a.next)[v];
             ((float *)a.next)[v] = 0.0f;
-            if constexpr (EMODE == OG_MODE_F32) og_epilogue<OG_MODE_F32>(a, v, s, st);
-            else
-            {
-                /* quantized push: fp32 update, fp32 contribution + min/max */
-                float prev = a.r[v];
-                float nr = OG_BASE_PR + OG_DAMP * s;
-                a.r[v] = nr;
-                double err = fabs((double)(nr - prev));
-                st.err += err / (double)a.N;
-                st.active += err >= a.epsilon;
-                uint32_t deg = a.outdeg[v];
-                float c = deg ? nr / (float)deg : 0.0f;
-                ((float *)a.c_out)[v] = c;
-                st.cmin = fminf(st.cmin, c);
-                st.cmax = fmaxf(st.cmax, c);
-            }
+            og_push_vertex<PMODE>(a, v, s, st);
         }
     }
     og_stat r = og_block_stat(st);
@@ -1096,6 +1104,99 @@ __global__ __launch_bounds__(OG_BLOCK) void og_push_epilogue_kernel(og_pr_dev a)
         a.part_min[blockIdx.x] = r.cmin;
         a.part_max[blockIdx.x] = r.cmax;
     }
+}
+
+/* The partitioned push: block b takes destination range p = pp_first + b,
+ * rows [pp_v0[p], pp_v0[p + 1]) of the plan, with one accumulator per
+ * destination in LDS.  Its in-edges -- the inverse CSR's edges of those
+ * rows, so a contiguous range -- are visited OG_PP_B per thread at a time,
+ * and each adds its source's contribution to its destination's accumulator
+ * with an LDS atomic (pp_dloc[e]: the destination's offset in the range):
+ * the reference's atomic scatter (pageRank.c:1091-1112), with the atomics
+ * kept in LDS.  Each destination still has one running sum to which its
+ * terms are added one at a time, in the order the atomics land, as the
+ * reference's OpenMP atomics do; the u64 fixed-point sums are exact in any
+ * order.  The block then runs the update of its rows (as
+ * og_push_epilogue_kernel) and writes its statistics partial. */
+#define OG_PP_THREADS 512u
+#define OG_PP_B 4u
+template <int PMODE>
+__global__ __launch_bounds__(OG_PP_THREADS) void og_push_part_kernel(og_pr_dev a)
+{
+    typedef typename std::conditional<PMODE == OG_MODE_FX64, unsigned long long, float>::type T;
+    __shared__ T acc[OG_PP_ROWS];
+    if (*a.done) return;
+    const uint32_t p = a.pp_first + blockIdx.x;
+    const uint32_t v0 = a.pp_v0[p], nv = a.pp_v0[p + 1] - v0;
+    const uint32_t e0 = a.row_ptr[v0], e1 = a.row_ptr[v0 + nv];
+    for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) acc[i] = 0;
+    __syncthreads();
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += OG_PP_B * OG_PP_THREADS)
+    {
+        uint32_t src[OG_PP_B], dl[OG_PP_B];
+#pragma unroll
+        for (uint32_t k = 0; k < OG_PP_B; k++)
+        {
+            const uint32_t ek = min(e + k * OG_PP_THREADS, e1 - 1);   /* past the end: re-read, not added */
+            src[k] = __builtin_nontemporal_load(&a.cols[ek]);
+            dl[k] = __builtin_nontemporal_load(&a.pp_dloc[ek]);
+        }
+        T add[OG_PP_B];
+#pragma unroll
+        for (uint32_t k = 0; k < OG_PP_B; k++)
+        {
+            if constexpr (PMODE == OG_MODE_F32) add[k] = ((const float *)a.g_in)[src[k]];
+            else if constexpr (PMODE == OG_MODE_FX64) add[k] = (T)((const uint64_t *)a.g_in)[src[k]];
+            else
+            {
+                const uint32_t qv = ((const uint32_t *)a.g_in)[src[k]];
+                add[k] = (float)(a.qp->scale_d * (double)(uint32_t)(qv - a.qp->zero));
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < OG_PP_B; k++)
+            if (e + k * OG_PP_THREADS < e1) atomicAdd(&acc[dl[k]], add[k]);
+    }
+    __syncthreads();
+    og_stat st;
+    og_stat_init(st);
+    for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) og_push_vertex<PMODE>(a, v0 + i, acc[i], st);
+    og_stat r = og_block_stat<OG_PP_THREADS / 64>(st);
+    if (threadIdx.x == 0)
+    {
+        a.part_err[blockIdx.x] = r.err;
+        a.part_act[blockIdx.x] = r.active;
+        a.part_min[blockIdx.x] = r.cmin;
+        a.part_max[blockIdx.x] = r.cmax;
+    }
+}
+
+/* pp_dloc[e] = (row of edge e) - pp_v0[p] for the edges of range p (one
+ * block per range; the row by binary search over its row pointers). */
+__global__ __launch_bounds__(OG_BLOCK) void og_push_dloc_kernel(const uint32_t *__restrict__ row_ptr,
+                                                               const uint32_t *__restrict__ pp_v0,
+                                                               uint16_t *__restrict__ dloc)
+{
+    const uint32_t v0 = pp_v0[blockIdx.x], v1 = pp_v0[blockIdx.x + 1];
+    const uint32_t e0 = row_ptr[v0], e1 = row_ptr[v1];
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += OG_BLOCK)
+    {
+        uint32_t lo = v0, hi = v1;   /* row_ptr[lo] <= e < row_ptr[hi] */
+        while (hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) / 2;
+            if (row_ptr[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        dloc[e] = (uint16_t)(lo - v0);
+    }
+}
+
+extern "C" int og_launch_push_dloc(const uint32_t *row_ptr, const uint32_t *pp_v0, uint32_t n_pp, uint16_t *dloc,
+                                   hipStream_t s)
+{
+    if (n_pp) OG_LAUNCH(og_push_dloc_kernel, dim3(n_pp), dim3(OG_BLOCK), 0, s, row_ptr, pp_v0, dloc);
+    return og_hip_check(og_launch_status(), "og_push_dloc_kernel");
 }
 
 __global__ __launch_bounds__(OG_BLOCK) void og_final_kernel(og_pr_dev a, uint32_t it)
@@ -1245,6 +1346,38 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
     if (gs > 16384) gs = 16384;
     if (gs == 0) gs = 1;
     if (ev0) hipEventRecord(ev0, s);
+    if (a->pp_v0)
+    {
+        /* partitioned push; fp32 with chains: the chain destinations gathered
+         * on the side stream, their partials after the ranges' */
+        const uint32_t nr = a->n_pp - a->pp_first;
+        const uint32_t nc = pmode == OG_MODE_F32 && a->push_skip ? a->n_chain_blocks : 0;
+        hipStream_t side = og_side_stream();
+        if (nc)
+        {
+            og_pr_dev b = *a;
+            b.chain_part_base = nr;
+            if (og_hip_check(hipEventRecord(og_side_fork(), s), "hipEventRecord") ||
+                og_hip_check(hipStreamWaitEvent(side, og_side_fork(), 0), "hipStreamWaitEvent"))
+                return -1;
+            OG_LAUNCH(og_pull_chains_kernel, dim3(nc), dim3(64), 0, side, b, it);
+        }
+        if (nr)
+        {
+            if (pmode == OG_MODE_F32)
+                OG_LAUNCH(og_push_part_kernel<OG_MODE_F32>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+            else if (pmode == OG_MODE_FX64)
+                OG_LAUNCH(og_push_part_kernel<OG_MODE_FX64>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+            else
+                OG_LAUNCH(og_push_part_kernel<OG_MODE_QPUSH_EPI>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+        }
+        if (nc && (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
+                   og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent")))
+            return -1;
+        if (ev1) hipEventRecord(ev1, s);
+        OG_LAUNCH(og_final_kernel, dim3(a->n_final_blocks), dim3(OG_BLOCK), 0, s, *a, it);
+        return og_hip_check(og_launch_status(), "og_push_part_kernel");
+    }
     if (pmode == OG_MODE_F32)
     {
         /* the chain destinations gathered on the side stream while the

@@ -63,6 +63,12 @@ struct ogpu_pr
      * its final reduction */
     struct og_bin_plan *bins;
     uint32_t bin_sum_grid, bin_long_grid, n_final_bin;
+    /* partitioned push (og_push_part_kernel): destination ranges (device
+     * n_pp + 1 row bounds), each inverse-CSR edge's offset in its range,
+     * the first range pushed, blocks of its final reduction */
+    uint32_t *pp_v0;
+    uint16_t *pp_dloc;
+    uint32_t n_pp, pp_first, n_final_pp;
     /* the last start vector offered (ogpu_pr_set_init, the learned start)
      * was rejected and the runs start from the uniform vector */
     uint32_t init_fallback;
