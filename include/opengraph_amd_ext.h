@@ -375,8 +375,10 @@ typedef struct
 {
     uint32_t code;           /* the reference's -p, 0..3 */
     uint32_t hub_threshold;  /* as ogpu_pr_options (0: default 16) */
-    uint32_t hot_cache;      /* as ogpu_pr_options, except that -p 1 keeps LDS accumulators by
-                                default (2 turns them off) */
+    uint32_t hot_cache;      /* as ogpu_pr_options: the pushes (-p 1, -p 3 scattered) are
+                                partitioned by destination range by default (0), 1 scatters
+                                with LDS accumulators for the hottest destinations, 2 with
+                                plain atomics */
     uint32_t scatter;        /* code 3: 1 scatters with atomics instead of gathering */
 } ogpu_spmv_options;
 void ogpu_spmv_default_options(ogpu_spmv_options *opt);

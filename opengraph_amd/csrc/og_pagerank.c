@@ -978,9 +978,13 @@ double ogpu_pr_bytes_per_iteration(const ogpu_pr *p)
     double N = p->g->N, E = p->g->E;
     switch (p->opt.variant)
     {
-    case 1: return 12.0 * E + 20.0 * N;           /* index + atomic read-modify-write */
+    /* the partitioned pushes: source id, u16 destination offset and the
+     * gathered contribution per edge; rank read and written, out-degree and
+     * the new contribution per vertex.  The scatters: index pair + atomic
+     * read-modify-write per edge. */
+    case 1: return p->pp_v0 ? 10.0 * E + 16.0 * N : 12.0 * E + 20.0 * N;
     case 2: return 12.0 * E + 24.0 * N;           /* 8-byte contributions */
-    case 3: return 20.0 * E + 24.0 * N;
+    case 3: return p->pp_v0 ? 14.0 * E + 20.0 * N : 20.0 * E + 24.0 * N;
     case 9: case 4: return 8.0 * E + 20.0 * N;
     case 10: case 12: return 6.0 * E + 18.0 * N;
     case 11: case 13: return 5.0 * E + 17.0 * N;

@@ -1117,10 +1117,13 @@ __global__ __launch_bounds__(OG_BLOCK) void og_push_epilogue_kernel(og_pr_dev a)
  * terms are added one at a time, in the order the atomics land, as the
  * reference's OpenMP atomics do; the u64 fixed-point sums are exact in any
  * order.  The block then runs the update of its rows (as
- * og_push_epilogue_kernel) and writes its statistics partial. */
+ * og_push_epilogue_kernel) and writes its statistics partial.  SPMV: one
+ * SpMV push pass (SPMV.c): the accumulators start from the running output
+ * next[v] (fp32 y, or the u64 sums), the terms are added onto it as the
+ * reference's atomics add them onto y, and it is stored back; no update. */
 #define OG_PP_THREADS 512u
 #define OG_PP_B 4u
-template <int PMODE>
+template <int PMODE, bool SPMV>
 __global__ __launch_bounds__(OG_PP_THREADS) void og_push_part_kernel(og_pr_dev a)
 {
     typedef typename std::conditional<PMODE == OG_MODE_FX64, unsigned long long, float>::type T;
@@ -1129,7 +1132,7 @@ __global__ __launch_bounds__(OG_PP_THREADS) void og_push_part_kernel(og_pr_dev a
     const uint32_t p = a.pp_first + blockIdx.x;
     const uint32_t v0 = a.pp_v0[p], nv = a.pp_v0[p + 1] - v0;
     const uint32_t e0 = a.row_ptr[v0], e1 = a.row_ptr[v0 + nv];
-    for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) acc[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) acc[i] = SPMV ? ((const T *)a.next)[v0 + i] : (T)0;
     __syncthreads();
     for (uint32_t e = e0 + threadIdx.x; e < e1; e += OG_PP_B * OG_PP_THREADS)
     {
@@ -1158,6 +1161,11 @@ __global__ __launch_bounds__(OG_PP_THREADS) void og_push_part_kernel(og_pr_dev a
             if (e + k * OG_PP_THREADS < e1) atomicAdd(&acc[dl[k]], add[k]);
     }
     __syncthreads();
+    if constexpr (SPMV)
+    {
+        for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) ((T *)a.next)[v0 + i] = acc[i];
+        return;
+    }
     og_stat st;
     og_stat_init(st);
     for (uint32_t i = threadIdx.x; i < nv; i += OG_PP_THREADS) og_push_vertex<PMODE>(a, v0 + i, acc[i], st);
@@ -1365,11 +1373,11 @@ extern "C" int og_launch_push(const og_pr_dev *a, int pmode, uint32_t it, hipStr
         if (nr)
         {
             if (pmode == OG_MODE_F32)
-                OG_LAUNCH(og_push_part_kernel<OG_MODE_F32>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+                OG_LAUNCH((og_push_part_kernel<OG_MODE_F32, false>), dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
             else if (pmode == OG_MODE_FX64)
-                OG_LAUNCH(og_push_part_kernel<OG_MODE_FX64>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+                OG_LAUNCH((og_push_part_kernel<OG_MODE_FX64, false>), dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
             else
-                OG_LAUNCH(og_push_part_kernel<OG_MODE_QPUSH_EPI>, dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+                OG_LAUNCH((og_push_part_kernel<OG_MODE_QPUSH_EPI, false>), dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
         }
         if (nc && (og_hip_check(hipEventRecord(og_side_join(), side), "hipEventRecord") ||
                    og_hip_check(hipStreamWaitEvent(s, og_side_join(), 0), "hipStreamWaitEvent")))
@@ -1436,7 +1444,16 @@ extern "C" int og_launch_push_scatter(const og_pr_dev *a, int pmode, hipStream_t
     if (gs > 16384) gs = 16384;
     if (gs == 0) gs = 1;
     if (ev0) hipEventRecord(ev0, s);
-    if (pmode == OG_MODE_F32)
+    if (a->pp_v0 && (pmode == OG_MODE_F32 || pmode == OG_MODE_FX64))
+    {
+        /* partitioned: every destination range, onto the running output */
+        const uint32_t nr = a->n_pp - a->pp_first;
+        if (nr && pmode == OG_MODE_F32)
+            OG_LAUNCH((og_push_part_kernel<OG_MODE_F32, true>), dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+        else if (nr)
+            OG_LAUNCH((og_push_part_kernel<OG_MODE_FX64, true>), dim3(nr), dim3(OG_PP_THREADS), 0, s, *a);
+    }
+    else if (pmode == OG_MODE_F32)
     {
         if (a->hot_grid)
             OG_LAUNCH((og_push_scatter_kernel<OG_MODE_F32, true>), dim3(a->hot_grid),

@@ -78,10 +78,10 @@ int og_spmv_create_with(og_graph *g, og_plan *plan, const ogpu_spmv_options *opt
     po.variant = o.code == 0 ? 0 : o.code == 1 ? 1 : o.code == 2 ? 9 : (s->scatter ? 3 : 2);
     po.max_iter = 1;
     po.hub_threshold = o.hub_threshold;
-    /* -p 1: the fp32 push adds the hottest destinations in LDS and flushes
-     * once per block by default.  Its sums are unordered in the reference
-     * too (omp atomic); block partials only make them closer to exact. */
-    po.hot_cache = o.code == 1 && o.hot_cache == 0 ? 1 : o.hot_cache;
+    /* the pushes (-p 1, -p 3 scattered) are partitioned by destination
+     * range by default (hot_cache 0): LDS atomics onto each destination's
+     * running y, as the reference's atomics (its sums are unordered too) */
+    po.hot_cache = o.hot_cache;
     /* every destination of the -p 1 scatter takes its adds as atomics (the
      * PageRank push would gather its longest rows instead) */
     if (o.code == 1) po.exact_above = 0;
@@ -218,11 +218,15 @@ double ogpu_spmv_bytes_per_iteration(const ogpu_spmv *s)
     const double N = s->p->g->N, E = s->p->g->E;
     switch (s->opt.code)
     {
-    case 1: return 16.0 * E + 4.0 * N;   /* src, dst, fp32 atomic read + write per edge; terms */
+    /* partitioned pushes: source id, u16 destination offset, gathered term
+     * per edge; y read and written per vertex.  Scatters: src, dst and the
+     * atomic read + write per edge. */
+    case 1: return s->p->pp_v0 ? 10.0 * E + 8.0 * N : 16.0 * E + 4.0 * N;
     case 2: return 8.0 * E + 20.0 * N;   /* index + u32 term per edge; row_ptr, u64 y read + write */
     case 3:
-        return s->scatter ? 24.0 * E + 8.0 * N   /* src, dst, u64 atomic read + write per edge */
-                          : 12.0 * E + 20.0 * N; /* index + u64 term per edge */
+        return !s->scatter ? 12.0 * E + 20.0 * N   /* index + u64 term per edge */
+               : s->p->pp_v0 ? 14.0 * E + 16.0 * N
+                             : 24.0 * E + 8.0 * N;
     default: return 8.0 * E + 12.0 * N;  /* index + fp32 term per edge; row_ptr, y read + write */
     }
 }
