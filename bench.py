@@ -113,7 +113,7 @@ def ext_variants(g, threads, csr, do_cpu):
         ms, kms, bpi = gpu_iter_ms(g, code, **kw)
         d = dict(ms_per_iteration=round(ms, 4), gteps=round(g.num_edges / ms / 1e6, 2),
                  algorithmic_gb_per_s=round(bpi / (kms / 1e3) / 1e9, 1))
-        if do_cpu and code in (1, 2):
+        if do_cpu and code in (1, 2, 3):
             if code not in cpu:
                 cpu[code] = cpu_time(g, code, 2, threads, csr)
             s, kind = cpu[code]
