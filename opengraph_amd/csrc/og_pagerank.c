@@ -1472,12 +1472,38 @@ static size_t og_multi_items(const og_pr_dev *a, uint32_t g, const og_multi_tile
 }
 
 /* Blocks of one pull launch: enough for every CU to hold 2048 threads, no
- * more than there are items. */
+ * more than there are items, a multiple of the 8 XCDs. */
 static uint32_t og_multi_grid(uint32_t block, size_t items)
 {
     uint64_t g = (uint64_t)(og_num_cus() > 0 ? og_num_cus() : 256) * (2048u / block);
     if (items < g) g = items;
-    return g ? (uint32_t)g : 1u;
+    g = (g + OG_XCDS - 1) / OG_XCDS * OG_XCDS;
+    return g ? (uint32_t)g : OG_XCDS;
+}
+
+/* The items [w0, w0 + sum cnt) of graphs gs[0..ng) (cnt[i] items each, in
+ * that order) cut into 8 contiguous ranges of about equal work, one per
+ * XCD: a graph's work is its edges and vertices, spread evenly over its
+ * items.  Graphs stay whole except at the cuts, so each XCD's L2 holds the
+ * contributions of its own graphs. */
+static void og_multi_split(const ogpu_multi *m, const uint32_t *gs, const size_t *cnt, uint32_t ng, uint32_t w0,
+                           og_multi_xr *xr)
+{
+    double tot = 0.0;
+    for (uint32_t i = 0; i < ng; i++) tot += (double)m->pr[gs[i]]->dev.E + m->pr[gs[i]]->dev.N;
+    uint32_t x = 0, pos = w0;
+    double acc = 0.0;
+    xr->o[0] = w0;
+    for (uint32_t i = 0; i < ng; i++)
+    {
+        const double w = cnt[i] ? ((double)m->pr[gs[i]]->dev.E + m->pr[gs[i]]->dev.N) / (double)cnt[i] : 0.0;
+        for (size_t j = 0; j < cnt[i]; j++, pos++)
+        {
+            while (x + 1 < OG_XCDS && acc >= tot * (double)(x + 1) / OG_XCDS) xr->o[++x] = pos;
+            acc += w;
+        }
+    }
+    while (x < OG_XCDS) xr->o[++x] = pos;
 }
 
 /* Milliseconds per pull of the graphs gs[0..ng) with tile shape `tile`:
@@ -1486,21 +1512,27 @@ static uint32_t og_multi_grid(uint32_t block, size_t items)
  * between HIP events.  The graphs must be initialised (a pull iterates
  * their ranks as a run does; the run initialises them again). */
 static int og_multi_time_tile(ogpu_multi *m, const uint32_t *gs, uint32_t ng, uint32_t tile,
-                              og_multi_work *host, hipEvent_t e0, hipEvent_t e1, float *ms)
+                              og_multi_work *host, size_t *cnt, hipEvent_t e0, hipEvent_t e1, float *ms)
 {
     og_multi_tile t;
     (void)og_multi_tile_shape(tile, &t);
     size_t k = 0;
-    for (uint32_t i = 0; i < ng; i++) k += og_multi_items(&m->pr[gs[i]]->dev, gs[i], &t, host + k);
+    for (uint32_t i = 0; i < ng; i++)
+    {
+        cnt[i] = og_multi_items(&m->pr[gs[i]]->dev, gs[i], &t, host + k);
+        k += cnt[i];
+    }
+    og_multi_xr xr;
+    og_multi_split(m, gs, cnt, ng, 0, &xr);
     hipStream_t s = og_stream();
     if (og_hip_check(hipMemcpyAsync(m->d_work, host, k * sizeof(og_multi_work), hipMemcpyHostToDevice, s), "H2D"))
         return -1;
     const uint32_t grid = og_multi_grid(t.block, k);
     for (uint32_t it = 0; it < OG_MULTI_TILE_WARM; it++)
-        if (og_launch_multi_pull(&m->dev, tile, 0, (uint32_t)k, 2 + it, grid, s)) return -1;
+        if (og_launch_multi_pull(&m->dev, tile, &xr, 2 + it, grid, s)) return -1;
     if (og_hip_check(hipEventRecord(e0, s), "hipEventRecord")) return -1;
     for (uint32_t it = 0; it < OG_MULTI_TILE_ITERS; it++)
-        if (og_launch_multi_pull(&m->dev, tile, 0, (uint32_t)k, 2 + OG_MULTI_TILE_WARM + it, grid, s)) return -1;
+        if (og_launch_multi_pull(&m->dev, tile, &xr, 2 + OG_MULTI_TILE_WARM + it, grid, s)) return -1;
     if (og_hip_check(hipEventRecord(e1, s), "hipEventRecord") ||
         og_hip_check(hipEventSynchronize(e1), "hipEventSynchronize") ||
         og_hip_check(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime"))
@@ -1518,10 +1550,11 @@ static int og_multi_pick_tiles(ogpu_multi *m, const ogpu_multi_tile *tiles, og_m
 {
     static const uint32_t blocks[] = {128, 256, 512}, chunks[] = {1, 2, 4}, batches[] = {0, 8};
     uint32_t *gs = (uint32_t *)malloc((m->n ? m->n : 1) * sizeof(uint32_t));
+    size_t *cnt = (size_t *)malloc((m->n ? m->n : 1) * sizeof(size_t));
     int8_t done[33] = {0};
     hipEvent_t e0 = NULL, e1 = NULL;
     int rc = -1;
-    if (!gs) { og_set_error("out of host memory"); return -1; }
+    if (!gs || !cnt) { free(gs); free(cnt); og_set_error("out of host memory"); return -1; }
     for (uint32_t g = 0; g < m->n; g++)
     {
         if (tiles && tiles[g].block)
@@ -1566,7 +1599,7 @@ static int og_multi_pick_tiles(ogpu_multi *m, const ogpu_multi_tile *tiles, og_m
             {
                 const int id = og_multi_tile_id(step == 0 ? c[i] : blk, step == 1 ? c[i] : ch, step == 2 ? c[i] : rb);
                 if (id < 0 || id >= 64) continue;
-                if (t_ms[id] < 0.0f && og_multi_time_tile(m, gs, ng, (uint32_t)id, host, e0, e1, &t_ms[id])) goto out;
+                if (t_ms[id] < 0.0f && og_multi_time_tile(m, gs, ng, (uint32_t)id, host, cnt, e0, e1, &t_ms[id])) goto out;
                 if (best_ms < 0.0f || t_ms[id] < best_ms)
                 {
                     best_ms = t_ms[id];
@@ -1585,6 +1618,7 @@ out:
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     free(gs);
+    free(cnt);
     return rc;
 }
 
@@ -1709,21 +1743,37 @@ int ogpu_multi_create_tiled(ogpu_graph *const *graphs_in, uint32_t n, const ogpu
     if (m->grid == 0) m->grid = 2048;
     if (og_multi_pick_tiles(m, tiles, work)) goto fail;
     /* the work list: the items of every graph of tile shape 0, then of
-     * shape 1, ...; one pull launch per shape in use */
-    for (uint32_t t = 0; t < og_multi_tile_count(); t++)
+     * shape 1, ...; one pull launch per shape in use, its items cut into
+     * one range per XCD */
     {
-        og_multi_tile sh;
-        (void)og_multi_tile_shape(t, &sh);
-        const size_t w0 = n_work;
-        for (uint32_t g = 0; g < n; g++)
+        uint32_t *gs = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+        size_t *cnt = (size_t *)malloc((n ? n : 1) * sizeof(size_t));
+        if (!gs || !cnt) { free(gs); free(cnt); og_set_error("out of host memory"); goto fail; }
+        for (uint32_t t = 0; t < og_multi_tile_count(); t++)
         {
-            if (m->tile[g] != t) continue;
-            wr[2 * g] = (uint32_t)n_work;
-            n_work += og_multi_items(&m->pr[g]->dev, g, &sh, work + n_work);
-            wr[2 * g + 1] = (uint32_t)n_work;
+            og_multi_tile sh;
+            (void)og_multi_tile_shape(t, &sh);
+            const size_t w0 = n_work;
+            uint32_t ng = 0;
+            for (uint32_t g = 0; g < n; g++)
+            {
+                if (m->tile[g] != t) continue;
+                wr[2 * g] = (uint32_t)n_work;
+                cnt[ng] = og_multi_items(&m->pr[g]->dev, g, &sh, work + n_work);
+                n_work += cnt[ng];
+                gs[ng++] = g;
+                wr[2 * g + 1] = (uint32_t)n_work;
+            }
+            if (n_work > w0)
+            {
+                og_multi_group *q = &m->groups[m->n_groups++];
+                q->tile = t;
+                q->grid = og_multi_grid(sh.block, n_work - w0);
+                og_multi_split(m, gs, cnt, ng, (uint32_t)w0, &q->xr);
+            }
         }
-        if (n_work > w0)
-            m->groups[m->n_groups++] = (og_multi_group){t, (uint32_t)w0, (uint32_t)n_work, og_multi_grid(sh.block, n_work - w0)};
+        free(gs);
+        free(cnt);
     }
     d->n_work = (uint32_t)n_work;
     {

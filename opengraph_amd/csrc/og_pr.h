@@ -159,16 +159,24 @@ typedef struct og_multi_tile
 {
     uint32_t block, chunks, row_batch;
 } og_multi_tile;
-/* Pull items [w0, w1) of one tile shape, on grid blocks. */
+/* The items of one pull launch, split by XCD: blocks b with b % 8 == x
+ * (the XCD the hardware places them on) take items [o[x], o[x + 1]). */
+#define OG_XCDS 8u
+typedef struct og_multi_xr
+{
+    uint32_t o[OG_XCDS + 1];
+} og_multi_xr;
+/* Pull items xr of one tile shape, on grid blocks (a multiple of 8). */
 typedef struct og_multi_group
 {
-    uint32_t tile, w0, w1, grid;
+    uint32_t tile, grid;
+    og_multi_xr xr;
 } og_multi_group;
 uint32_t og_multi_tile_count(void);
 int og_multi_tile_shape(uint32_t id, og_multi_tile *t);
 int og_launch_multi_init(const og_multi_dev *m, uint32_t grid, hipStream_t s);
-/* The pull of iteration `it` over items [w0, w1) with tile shape `tile`. */
-int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, uint32_t w0, uint32_t w1, uint32_t it,
+/* The pull of iteration `it` over the items xr with tile shape `tile`. */
+int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, const og_multi_xr *xr, uint32_t it,
                          uint32_t grid, hipStream_t s);
 /* One iteration: the pull of every group, the long rows (grid blocks), the
  * per-graph statistics and stop rule. */

@@ -1704,20 +1704,23 @@ __global__ __launch_bounds__(OG_BLOCK) void og_multi_init_kernel(og_multi_dev m)
     }
 }
 
-/* Items [w0, w1) of the work list: the graphs whose tile shape is BLK
+/* The items xr of the work list: the graphs whose tile shape is BLK
  * threads per block, K chunks per wavefront and RB edges per load batch of
  * a thread's row (0: as many as the wavefront's longest row needs, up to
  * OG_ROW_BATCH).  The shape changes neither a row's sum nor a chunk's, so
  * every shape gives the same ranks.  settled != 0 from the third iteration
  * on: row blocks entirely in [Z, N) (no in-edges; rank and both
  * contribution buffers already final) only report their activeVertices
- * count (0 >= epsilon). */
+ * count (0 >= epsilon).  Block b runs on XCD b % 8 and takes that XCD's
+ * items (contiguous ranges of whole graphs, so a graph's contributions are
+ * gathered through one XCD's L2). */
 template <uint32_t BLK, uint32_t K, uint32_t RB>
-__global__ __launch_bounds__(BLK) void og_multi_pull_kernel(og_multi_dev m, uint32_t w0, uint32_t w1,
+__global__ __launch_bounds__(BLK) void og_multi_pull_kernel(og_multi_dev m, og_multi_xr xr,
                                                            uint32_t parity, uint32_t settled)
 {
     if (*m.all_done) return;
-    for (uint32_t i = w0 + blockIdx.x; i < w1; i += gridDim.x)
+    const uint32_t x = blockIdx.x % OG_XCDS, nb = gridDim.x / OG_XCDS;
+    for (uint32_t i = xr.o[x] + blockIdx.x / OG_XCDS; i < xr.o[x + 1]; i += nb)
     {
         const og_multi_work w = m.work[i];
         const og_pr_dev &a = m.devs[2 * w.g + parity];
@@ -1836,16 +1839,21 @@ extern "C" int og_multi_tile_shape(uint32_t id, og_multi_tile *t)
     }
 }
 
-extern "C" int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, uint32_t w0, uint32_t w1,
+extern "C" int og_launch_multi_pull(const og_multi_dev *m, uint32_t tile, const og_multi_xr *xr,
                                     uint32_t it, uint32_t grid, hipStream_t s)
 {
     const uint32_t parity = it & 1, settled = it >= 2;
-    if (w1 <= w0) return 0;
+    if (xr->o[OG_XCDS] <= xr->o[0]) return 0;
+    if (grid < OG_XCDS || grid % OG_XCDS)
+    {
+        og_set_error("og_launch_multi_pull: grid %u is not a positive multiple of %u", grid, OG_XCDS);
+        return -1;
+    }
     switch (tile)
     {
 #define OG_X(i, blk, k, rb)                                                                      \
     case i:                                                                                      \
-        OG_LAUNCH((og_multi_pull_kernel<blk, k, rb>), dim3(grid), dim3(blk), 0, s, *m, w0, w1, parity, \
+        OG_LAUNCH((og_multi_pull_kernel<blk, k, rb>), dim3(grid), dim3(blk), 0, s, *m, *xr, parity, \
                   settled);                                                                      \
         break;
         OG_MULTI_TILES(OG_X)
@@ -1860,7 +1868,7 @@ extern "C" int og_launch_multi_iter(const og_multi_dev *m, const og_multi_group 
 {
     const uint32_t parity = it & 1;
     for (uint32_t q = 0; q < n_groups; q++)
-        if (og_launch_multi_pull(m, groups[q].tile, groups[q].w0, groups[q].w1, it, groups[q].grid, s)) return -1;
+        if (og_launch_multi_pull(m, groups[q].tile, &groups[q].xr, it, groups[q].grid, s)) return -1;
     if (m->n_lwork)
         OG_LAUNCH(og_multi_long_kernel, dim3(grid), dim3(OG_BLOCK), 0, s, *m, parity);
     OG_LAUNCH(og_multi_reduce_kernel, dim3(m->n_graphs), dim3(OG_BLOCK), 0, s, *m, parity, it);
