@@ -496,29 +496,38 @@ def test_hot_cache_same_results(graphs, name, code):
 @pytest.mark.parametrize("code", [1, 3, 5])
 @pytest.mark.parametrize("name", ["enron", "rmat16"])
 def test_push_lds_accumulators(golden, graphs, name, code):
-    """Push with LDS accumulators for the hottest destinations against the
+    """The push forms against each other: the partitioned push (default),
+    the scatter with LDS accumulators for the hottest destinations and the
     plain atomic scatter.  The u64 fixed-point push (3) is order-free:
-    bit-identical ranks, iterations and activeVertices.  The fp32 push (1)
-    adds block partials instead of one running fp32 sum per destination,
-    which loses the small terms of long in-edge lists less: after the same
-    20 iterations (epsilon 0) it is within 1e-5 relative of the fp64
-    iterate and no farther from it than the plain scatter (up to 5 %, or
-    2e-6 where both are at rounding level).  The quantized
-    push (5) carries its quantization error in both: the two agree within
-    1e-4 relative."""
+    bit-identical ranks, iterations and activeVertices in all three.  The
+    fp32 scatter with accumulators (1) adds block partials instead of one
+    running fp32 sum per destination, which loses the small terms of long
+    in-edge lists less: after the same 20 iterations (epsilon 0) it is
+    within 1e-5 relative of the fp64 iterate and no farther from it than
+    the plain scatter (up to 5 %, or 2e-6 where both are at rounding level);
+    the partitioned push, one running sum per destination as the plain
+    scatter, is within 1e-5 of it too.  The quantized push (5) carries its
+    quantization error in every form: they agree within 1e-4 relative."""
     g = graphs(name, 0)
     if code == 3:
-        out = [_run(g, code, 1e-7, 30, hot_cache=hot) for hot in (1, 2)]
-        assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
-        assert np.array_equal(out[0][2], out[1][2]) and out[0][0].iterations == out[1][0].iterations
+        # the partitioned push (default, 0) and both scatters: the same bits
+        out = [_run(g, code, 1e-7, 30, hot_cache=hot) for hot in (1, 2, 0)]
+        for o in out[1:]:
+            assert np.array_equal(out[0][1].view(np.uint32), o[1].view(np.uint32))
+            assert np.array_equal(out[0][2], o[2]) and out[0][0].iterations == o[0].iterations
         return
-    x = [_run(g, code, 0.0, 20, hot_cache=hot)[1].astype(np.float64) for hot in (1, 2)]
+    x = [_run(g, code, 0.0, 20, hot_cache=hot)[1].astype(np.float64) for hot in (1, 2, 0)]
     if code == 5:
         np.testing.assert_allclose(x[0], x[1], rtol=1e-4, atol=0)
+        np.testing.assert_allclose(x[2], x[1], rtol=1e-4, atol=0)
         return
     s, d = golden_input(name, golden)
     x64, it, _ = po.pagerank_fp64(po.build_csr(s, d, 0, 8), 0.15, 0.0, 20, 8)
     assert it == 20
     rel = [np.max(np.abs(v - x64) / x64) for v in x]
-    print(name, "max relative error vs fp64 iterate: lds", rel[0], "plain", rel[1], flush=True)
+    print(name, "max relative error vs fp64 iterate: lds", rel[0], "plain", rel[1], "partitioned", rel[2],
+          flush=True)
     assert rel[0] <= 1e-5 and rel[0] <= max(1.05 * rel[1], 2e-6)
+    # the partitioned push keeps one running fp32 sum per destination, like
+    # the plain scatter: its drift is of the same size
+    assert rel[2] <= 1e-5
