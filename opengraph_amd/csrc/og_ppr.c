@@ -127,6 +127,7 @@ int ogpu_ppr_create(ogpu_graph *gg, const uint32_t *sources, uint32_t k, uint32_
     og_ppr_dev *a = &p->dev;
     memset(a, 0, sizeof(*a));
     a->N = N;
+    a->E = pl->E;
     a->k = k;
     a->H = pl->H;
     a->C = pl->C;

@@ -24,7 +24,7 @@ extern "C" {
  * gathered row only touches its first kact floats. */
 typedef struct og_ppr_dev
 {
-    uint32_t N, k, H, C, n_chunks;
+    uint32_t N, E, k, H, C, n_chunks;
     uint32_t it;                 /* iteration being computed */
     uint32_t max_iter;
     uint32_t kact;               /* slots [0, kact) may hold active columns */

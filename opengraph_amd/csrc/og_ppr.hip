@@ -11,10 +11,11 @@
  * below epsilon.
  *
  * Layout: x, c[2] are N x k fp32, row-major (row v = k consecutive floats,
- * 256 B for k = 64), in the plan's vertex order.  Lane j of a wavefront
- * owns column j, so every gathered row c[u][0..k) is one coalesced 256 B
- * read and a row's sum runs over its in-edges in CSR order per column --
- * for rows of at most OG_CH in-edges bit-exactly the single-source
+ * 256 B for k = 64), in the plan's vertex order.  An iteration is one pass
+ * per 32 column slots in use; in a pass 32 lanes own the 32 slots of a work
+ * item, so every gathered piece c[u][32 cb .. 32 cb + 32) is one coalesced
+ * 128 B read, and a row's sum runs over its in-edges in CSR order per column
+ * -- for rows of at most OG_CH in-edges bit-exactly the single-source
  * sequential sum.  Rows with more in-edges are cut into OG_CH-edge chunks
  * (the plan's chunks) whose sums the finalize kernel adds in chunk order.
  *
@@ -30,38 +31,6 @@
 #define PB 256           /* block */
 #define PWAVES (PB / 64)
 #define PG 16            /* gathers in flight per lane */
-
-/* Sum over the in-edges [eb, ee) of c_in[u][col], in CSR order; lanes whose
- * column is not iterating (on == false) load nothing and return 0. */
-__device__ __forceinline__ float og_ppr_gather(const og_ppr_dev &a, uint32_t eb, uint32_t ee,
-                                               uint32_t lane, uint32_t col, bool on)
-{
-    float s = 0.0f;
-    for (uint32_t base = eb; base < ee; base += 64)
-    {
-        const uint32_t n = min(64u, ee - base);
-        const uint32_t my = a.cols[base + min(lane, n - 1)];
-        for (uint32_t u0 = 0; u0 < n; u0 += PG)
-        {
-            float v[PG];
-#pragma unroll
-            for (uint32_t u = 0; u < PG; u++)
-            {
-                float t = 0.0f;
-                if (u0 + u < n && on)
-                {
-                    const uint32_t src = __builtin_amdgcn_readlane(my, u0 + u);
-                    t = a.c_in[(size_t)src * a.k + col];
-                }
-                v[u] = t;
-            }
-            /* adding +0 for the slots past n leaves every partial sum unchanged */
-#pragma unroll
-            for (uint32_t u = 0; u < PG; u++) s += v[u];
-        }
-    }
-    return s;
-}
 
 /* New x and c of row v in slot `col` of an iterating column, and its L1.  A
  * frozen column keeps its x; its c is never gathered again. */
@@ -95,39 +64,92 @@ __device__ __forceinline__ double og_ppr_block_col(double v)
     return r;
 }
 
-__global__ __launch_bounds__(PB) void og_ppr_main_kernel(og_ppr_dev a)
+/* One pass of an iteration over column slots [32 cb, 32 cb + 32): a
+ * wavefront takes PRW = 2 consecutive work items at a time, 32 lanes each,
+ * lane 32 r + j owning slot 32 cb + j of item r.  Every gathered piece
+ * c[u][32 cb .. 32 cb + 32) is one whole 128-byte line, and a pass's
+ * working set is N x 128 B of contributions instead of N x 256 B (16
+ * columns per pass read half lines and took 3.8 ms per iteration at
+ * R-MAT-22, k = 64; 32 columns 3.0 ms; the whole 256-byte row 4.6 ms).
+ * Each lane adds its column's terms over the item's in-edges in CSR order
+ * (+0 past the end), as before, so the results do not depend on the
+ * blocking. */
+#define PCB 32u          /* column slots per pass: a 128-byte line of a row */
+#define PRW (64u / PCB)  /* work items per wavefront */
+__global__ __launch_bounds__(PB) void og_ppr_main_kernel(og_ppr_dev a, uint32_t cb)
 {
     if (*a.done) return;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t col = min(lane, a.k - 1);
-    const bool live = lane < a.kact;
+    const uint32_t rr = lane / PCB, jj = lane % PCB;
+    const uint32_t slot = cb * PCB + jj;
+    const uint32_t col = min(slot, a.k - 1);
+    const bool live = slot < a.kact;
     const uint64_t act = *a.active;
-    const bool on = live && ((act >> lane) & 1u);
+    const bool on = live && ((act >> slot) & 1u);
     const uint32_t my_src = a.src[col];
     const uint32_t wave = blockIdx.x * PWAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t W = gridDim.x * PWAVES;
     const uint32_t items = a.n_chunks + (a.N - a.C);
     double l1 = 0.0;
-    for (uint32_t it = wave; it < items; it += W)
+    for (uint32_t it0 = wave * PRW; it0 < items; it0 += W * PRW)
     {
-        if (it < a.n_chunks)
+        const uint32_t it = it0 + rr;
+        const bool have = it < items;
+        uint32_t eb = 0, ee = 0, row = 0, tag = 0;
+        if (have)
         {
-            const uint32_t tag = a.ch_row[it];
-            const uint32_t row = tag & ~OG_CH_MULTI;
-            const float s = og_ppr_gather(a, a.ch_edge[it], a.ch_edge[it + 1], lane, col, on);
-            if (!live) continue;
-            if (tag & OG_CH_MULTI) a.chunk_part[(size_t)it * a.k + col] = s;
-            else og_ppr_epilogue(a, row, s, col, my_src, on, l1);
+            if (it < a.n_chunks)
+            {
+                tag = a.ch_row[it];
+                row = tag & ~OG_CH_MULTI;
+                eb = a.ch_edge[it];
+                ee = a.ch_edge[it + 1];
+            }
+            else
+            {
+                row = a.C + (it - a.n_chunks);
+                eb = a.row_ptr[row];
+                ee = a.row_ptr[row + 1];
+            }
         }
-        else
-        {
-            const uint32_t v = a.C + (it - a.n_chunks);
-            const float s = og_ppr_gather(a, a.row_ptr[v], a.row_ptr[v + 1], lane, col, on);
-            if (live) og_ppr_epilogue(a, v, s, col, my_src, on, l1);
-        }
+        const uint32_t len = ee - eb;
+        uint32_t maxlen = len;   /* the wavefront's longest item */
+        for (int o = PCB; o < 64; o <<= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, o, 64));
+        float s = 0.0f;
+        if (on)
+            for (uint32_t b0 = 0; b0 < maxlen; b0 += PG)
+            {
+                /* branch-free: slots past the item's end re-read an edge in
+                 * bounds and add +0 */
+                float v[PG];
+#pragma unroll
+                for (uint32_t u = 0; u < PG; u++)
+                {
+                    const uint32_t e = min(eb + min(b0 + u, len - 1u), a.E - 1u);
+                    const uint32_t src = a.cols[e];
+                    const float t = a.c_in[(size_t)src * a.k + slot];
+                    v[u] = b0 + u < len ? t : 0.0f;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < PG; u++) s += v[u];
+            }
+        if (!have || !live) continue;
+        if (it < a.n_chunks && (tag & OG_CH_MULTI)) a.chunk_part[(size_t)it * a.k + slot] = s;
+        else og_ppr_epilogue(a, row, s, slot, my_src, on, l1);
     }
-    const double b = og_ppr_block_col(l1);
-    if (threadIdx.x < a.k) a.part_l1[(size_t)blockIdx.x * a.k + threadIdx.x] = b;
+    /* per slot: the PRW lane groups of each wavefront, then the wavefronts,
+     * in a fixed order */
+    __shared__ double sh[PWAVES][64];
+    const uint32_t w = threadIdx.x >> 6;
+    sh[w][lane] = l1;
+    __syncthreads();
+    if (threadIdx.x < PCB && cb * PCB + threadIdx.x < a.k)
+    {
+        double r = 0.0;
+        for (uint32_t i = 0; i < PWAVES; i++)
+            for (uint32_t g = 0; g < PRW; g++) r += sh[i][g * PCB + threadIdx.x];
+        a.part_l1[(size_t)blockIdx.x * a.k + cb * PCB + threadIdx.x] = r;
+    }
 }
 
 /* Rows [0, H): add the chunk sums in chunk order, epilogue.  Then the
@@ -284,7 +306,9 @@ extern "C" int og_launch_ppr_init(const og_ppr_dev *a, hipStream_t s)
 extern "C" int og_launch_ppr_iter(const og_ppr_dev *a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (ev0) hipEventRecord(ev0, s);
-    OG_LAUNCH(og_ppr_main_kernel, dim3(OG_PPR_MAIN_BLOCKS), dim3(PB), 0, s, *a);
+    /* one pass per 32 column slots in use */
+    for (uint32_t cb = 0; cb * PCB < a->kact; cb++)
+        OG_LAUNCH(og_ppr_main_kernel, dim3(OG_PPR_MAIN_BLOCKS), dim3(PB), 0, s, *a, cb);
     if (ev1) hipEventRecord(ev1, s);
     OG_LAUNCH(og_ppr_finalize_kernel, dim3(OG_PPR_FIN_BLOCKS), dim3(PB), 0, s, *a);
     return og_hip_check(og_launch_status(), "og_ppr_main_kernel");
