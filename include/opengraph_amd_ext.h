@@ -361,9 +361,10 @@ void ogpu_ppr_free(ogpu_ppr *p);
  * y = 0, then `iterations` passes y += sum over the in-edges (src, v) of
  * w * x[src], with x[v] = 1.0f / out-degree(v) (0.001f without out-edges)
  * and w = 0.0001f, in the arithmetic of -p code:
- *   0 pull, fp32: rows of at most hub_threshold in-edges continue y in the
- *     reference's CSR order (bit-exact); longer rows add their fixed-tree
- *     sum to y (hub_threshold 0xFFFFFFFF: every row bit-exact);
+ *   0 pull, fp32: by default the binned pull, every row continuing y in the
+ *     reference's CSR order (bit-exact); with a hub_threshold, the gather
+ *     pull: rows of at most hub_threshold in-edges bit-exact, longer rows
+ *     add their fixed-tree sum to y (0xFFFFFFFF: every row bit-exact);
  *   1 push, fp32 atomics over the forward edges (unordered, as the
  *     reference's omp atomic), the hottest destinations summed per block
  *     in LDS first;
@@ -374,7 +375,9 @@ typedef struct ogpu_spmv ogpu_spmv;
 typedef struct
 {
     uint32_t code;           /* the reference's -p, 0..3 */
-    uint32_t hub_threshold;  /* as ogpu_pr_options (0: default 16) */
+    uint32_t hub_threshold;  /* 0 (default): -p 0 runs on the binned pull, every row's running
+                                y in the reference's order; otherwise -p 0 runs on the gather
+                                pull with this row-class threshold, as ogpu_pr_options */
     uint32_t hot_cache;      /* as ogpu_pr_options: the pushes (-p 1, -p 3 scattered) are
                                 partitioned by destination range by default (0), 1 scatters
                                 with LDS accumulators for the hottest destinations, 2 with

@@ -101,9 +101,9 @@ __device__ __forceinline__ void og_bin_load16(const float4 *s4, uint32_t q, floa
     for (uint32_t u = 0; u < 4; u++) x[u] = s4[min(q + u, QMAX)];
 }
 
-__device__ __forceinline__ float og_bin_row_sum(const float *slot, uint32_t j, uint32_t j1)
+__device__ __forceinline__ float og_bin_row_sum(const float *slot, uint32_t j, uint32_t j1, float s0)
 {
-    float s = 0.0f;
+    float s = s0;
     const uint32_t qe = j1 >> 2;                     /* groups [q, qe) lie inside */
     uint32_t q = (j + 3) >> 2;
     if (q >= qe)
@@ -156,11 +156,11 @@ __device__ __forceinline__ void og_bin_wave_load4(const float *x, uint32_t i, ui
  * terms per step, 4 consecutive ones per lane (a lane's prefix of its own
  * terms, then a wavefront scan of the lanes' totals); the next 256 are
  * read while a step is scanned. */
-__device__ float og_bin_wave_sum(const float *slot, uint32_t j0, uint32_t n)
+__device__ float og_bin_wave_sum(const float *slot, uint32_t j0, uint32_t n, float y0)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t h = n < OG_BIN_HEAD ? n : OG_BIN_HEAD;
-    float y = og_bin_uniform(og_bin_row_sum(slot, j0, j0 + h));
+    float y = og_bin_uniform(og_bin_row_sum(slot, j0, j0 + h, y0));
     const float *x = slot + j0;
     uint32_t i = h;
     float a[4];
@@ -330,6 +330,12 @@ __device__ __forceinline__ void og_bin_stat_init(og_bin_stat &s)
 __device__ __forceinline__ void og_bin_epilogue_pf(const og_pr_dev &a, uint32_t v, float sum, float prev,
                                                    uint32_t deg, og_bin_stat &st)
 {
+    if (a.spmv)
+    {
+        /* SpMV pass: the running y, started from the old one, is the sum */
+        __builtin_nontemporal_store(sum, &a.r[v]);
+        return;
+    }
     const float nr = OG_BASE_PR + (OG_DAMP * sum);
     __builtin_nontemporal_store(nr, &a.r[v]);
     const double err = fabs((double)(nr - prev));
@@ -526,7 +532,9 @@ __global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_d
             {
                 const uint32_t v = cd.row0 + threadIdx.x + BT * k;
                 if (v >= cd.row1) break;
-                const float s = og_bin_row_sum(slot, cm.p0[k] - cd.pos0, cm.p1[k] - cd.pos0);
+                /* (an SpMV pass continues the row's running y) */
+                const float s = og_bin_row_sum(slot, cm.p0[k] - cd.pos0, cm.p1[k] - cd.pos0,
+                                               a.spmv ? cm.prev[k] : 0.0f);
                 og_bin_epilogue_pf(a, v, s, cm.prev[k], cm.deg[k], st);
             }
         }
@@ -542,8 +550,8 @@ __global__ __launch_bounds__(BT, 4) void og_bin_sum_kernel(og_bin_dev b, og_pr_d
                     if (v >= cd.row1) break;
                     const uint32_t j0 = (uint32_t)__builtin_amdgcn_readlane(cm.p0[k], l) - cd.pos0;
                     const uint32_t j1 = (uint32_t)__builtin_amdgcn_readlane(cm.p1[k], l) - cd.pos0;
-                    const float y = og_bin_wave_sum(slot, j0, j1 - j0);
                     const float prev = og_bin_lane(cm.prev[k], l);
+                    const float y = og_bin_wave_sum(slot, j0, j1 - j0, a.spmv ? prev : 0.0f);
                     const uint32_t deg = (uint32_t)__builtin_amdgcn_readlane(cm.deg[k], l);
                     if (lane == 0) og_bin_epilogue_pf(a, v, y, prev, deg, st);
                 }
@@ -716,7 +724,7 @@ __global__ __launch_bounds__(LT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
         if (q >= b.n_long) break;
         const uint32_t v = b.long_row[q];
         const uint32_t n = a.row_ptr[v + 1] - a.row_ptr[v];
-        float y = 0.0f;     /* identical in every thread */
+        float y = a.spmv ? a.r[v] : 0.0f;     /* identical in every thread */
         uint32_t bin = b.long_bin[q];
         for (uint32_t c0 = 0; c0 < n; c0 += OG_BIN_CAP, bin++)
         {
@@ -744,7 +752,7 @@ __global__ __launch_bounds__(LT) void og_bin_long_kernel(og_bin_dev b, og_pr_dev
                 /* the leading terms one after the other (the running sum
                  * crosses binades every few terms there) */
                 p = d.cnt < OG_BIN_HEAD ? d.cnt : OG_BIN_HEAD;
-                y = og_bin_uniform(og_bin_row_sum(buf, 0, p));
+                y = og_bin_uniform(og_bin_row_sum(buf, 0, p, y));
             }
             y = og_bin_block_sum(buf, p, d.cnt, y, s_wsum, s_wfail, s_pre, par);
         }

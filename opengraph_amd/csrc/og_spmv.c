@@ -17,6 +17,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "og_bin.h"
 #include "og_internal.h"
 #include "og_pr.h"
 #include "og_pr_run.h"
@@ -85,7 +86,10 @@ int og_spmv_create_with(og_graph *g, og_plan *plan, const ogpu_spmv_options *opt
     /* every destination of the -p 1 scatter takes its adds as atomics (the
      * PageRank push would gather its longest rows instead) */
     if (o.code == 1) po.exact_above = 0;
-    po.binned = 2;   /* SpMV passes run on the gather pull's kernels */
+    /* -p 0 with the default row split runs on the binned pull (every row's
+     * running y in the reference's order); a given hub_threshold, and the
+     * other codes, on the gather pull's kernels */
+    po.binned = o.code == 0 && o.hub_threshold == 0 ? 0 : 2;
     if (og_pr_create_with(g, plan, &po, &s->p))
     {
         free(s);
@@ -134,6 +138,20 @@ static int og_spmv_launch_iter(ogpu_spmv *s, hipEvent_t e0, hipEvent_t e1)
     }
     a.spmv = 1;
     a.next = s->acc;
+    if (p->bins)
+    {
+        /* binned: gather of the terms, then the sum and long-row passes
+         * adding onto each row's y; rows without in-edges keep theirs */
+        og_bin_dev b = p->bins->d;
+        b.n_bins_live = b.n_bins - p->bins->n_zero_bins;
+        a.n_parts = p->bin_sum_grid + p->bin_long_grid;
+        if (a.n_parts > p->parts_cap)
+        {
+            og_set_error("SpMV binned pass with %u partials, %u allocated", a.n_parts, p->parts_cap);
+            return -1;
+        }
+        return og_launch_bin_pull(&b, &a, 0, p->bin_sum_grid, p->bin_long_grid, st, e0, e1);
+    }
     /* rows [Z, N) have no in-edges: their y stays 0 */
     a.row_end = a.Z;
     a.n_row_blocks = (uint32_t)(((uint64_t)(a.Z - a.C) + 255) / 256);
@@ -286,7 +304,9 @@ static struct SPMVStats *og_spmv_dropin(struct Arguments *arguments, struct Grap
     ogpu_spmv_default_options(&o);
     o.code = code;
     ogpu_spmv *s = NULL;
-    if (og_spmv_create_with(g, plan, &o, &s)) og_spmv_fatal("spmv setup");
+    /* -p 0 runs binned, on a plan of its own that keeps every row in CSR
+     * order (the cached plan sorts long rows for the gather pulls) */
+    if (og_spmv_create_with(g, code == 0 ? NULL : plan, &o, &s)) og_spmv_fatal("spmv setup");
     struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
     stats->arguments = arguments;
     /* SPMV.c:663-669: the input vector the passes use */

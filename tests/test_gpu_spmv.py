@@ -4,11 +4,12 @@ Bar: bit-exact against the compiled reference's vector_output for every
 golden case (graphbrew, LAW-enron, R-MAT-16; sort modes 0 and 1; 20
 iterations; tests/golden/make_golden_spmv.py) for fp32 pull with every row
 in the reference's order, fixed-point pull, and fixed-point push both
-gathered and scattered.  With the default row split the fp32 pull's rows of
-at most 16 in-edges stay bit-exact and the longer ones lie within the fp32
-summation bound of the exact value, as the reference's own running sums do;
-fp32 push (unordered atomics, as the reference's omp atomic) is held to the
-same bound.  Larger graphs (R-MAT-20) against the oracle, itself pinned on
+gathered and scattered.  The default fp32 pull (binned) is bit-exact on
+every row; with a given row split (the gather pull) rows of at most 16
+in-edges stay bit-exact and the longer ones lie within the fp32 summation
+bound of the exact value, as the reference's own running sums do; fp32 push
+(unordered atomics, as the reference's omp atomic) is held to the same
+bound.  Larger graphs (R-MAT-20) against the oracle, itself pinned on
 those fixtures; the drop-in SPMVGraphCSR on host GraphCSR structs.
 """
 import hashlib
@@ -92,8 +93,11 @@ def test_default_rows_and_push_within_bound(golden, gname):
     short = np.bincount(csr["f_dst"], minlength=csr.num_vertices) <= 16
     g = og.Graph.from_edges(s, d, 0)
     try:
-        for hot in (0, 1):
-            y = _run(g, 0, 20, hot_cache=hot)
+        # the default (binned) pull: every row's running y in the reference's order
+        y = _run(g, 0, 20)
+        assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+        for hot in (0, 1):     # the gather pull with its row split
+            y = _run(g, 0, 20, hub_threshold=16, hot_cache=hot)
             assert np.array_equal(y[short].view(np.uint32), ref[short].view(np.uint32)), hot
             assert np.all(np.abs(y.astype(np.float64) - want) <= bound), hot
         for hot in (0, 2):          # LDS accumulators (default for -p 1) and plain atomics
@@ -157,7 +161,9 @@ def test_scale20_fp32(rmat20):
     ref = po.spmv(csr, 0, 8)
     want, bound = _exact_and_bound(csr, 8)
     short = np.bincount(csr["f_dst"], minlength=csr.num_vertices) <= 16
-    y = _run(g, 0, 8)
+    y = _run(g, 0, 8)                          # binned: bit-exact on every row
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+    y = _run(g, 0, 8, hub_threshold=16)        # gather pull with its row split
     assert np.array_equal(y[short].view(np.uint32), ref[short].view(np.uint32))
     assert np.all(np.abs(y.astype(np.float64) - want) <= bound)
     y = _run(g, 0, 8, hub_threshold=ALL_EXACT)
@@ -178,11 +184,8 @@ def test_dropin_spmv_graph_csr(golden):
             assert it == 20 and t > 0.0
             assert np.array_equal(x.view(np.uint32), x_want.astype(np.float32).view(np.uint32))
             want = po.spmv(csr, code if code <= 3 else 0, 20)       # SPMV.c:609-611: default pull
-            short = np.bincount(csr["f_dst"], minlength=csr.num_vertices) <= 16
-            if code in (2, 3):
-                assert np.array_equal(y.view(np.uint32), want.view(np.uint32)), (gname, code)
-            else:
-                assert np.array_equal(y[short].view(np.uint32), want[short].view(np.uint32)), (gname, code)
+            # fixed point, and the fp32 pull (binned by default): every row bit-exact
+            assert np.array_equal(y.view(np.uint32), want.view(np.uint32)), (gname, code)
         y, _, _, _ = og.spmv_graph_csr(host, 1, 20)
         want, bound = _exact_and_bound(csr, 20)
         assert np.all(np.abs(y.astype(np.float64) - want) <= bound)
