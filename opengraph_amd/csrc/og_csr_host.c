@@ -520,6 +520,7 @@ struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
         memcpy(ie->inverse_label_array, fe->inverse_label_array, (size_t)h->num_vertices * 4);
     }
     og_print_time("Mappign Vertices to CSR (Seconds)", og_wall() - t0);
+    og_trace_entry("graphCSRPreProcessingStep", h->num_vertices, h->num_edges);
     graphCSRPrint(h);
     return h;
 }

@@ -34,6 +34,7 @@ hipStream_t og_side_stream(void);
 hipEvent_t og_side_fork(void);
 hipEvent_t og_side_join(void);
 int og_num_cus(void);
+void og_trace_entry(const char *fn, uint32_t n_vertices, uint32_t n_edges);
 /* hipMalloc / hipFree with error text; og_dalloc returns NULL on failure. */
 void *og_dalloc(size_t bytes);
 void og_dfree(void *p);

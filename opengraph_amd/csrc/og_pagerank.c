@@ -1233,6 +1233,7 @@ static struct PageRankStats *og_dropin(struct Arguments *arguments, struct Graph
     og_graph *g = NULL;
     og_plan *plan = NULL;
     if (og_cache_graph(graph, push || dd, &g, &plan)) og_fatal("graph upload");
+    og_trace_entry("pageRankGraphCSR", g->N, g->E);
     ogpu_pr *pr = NULL;
     /* the fp32 pull runs binned, on a plan of its own that keeps every row
      * in CSR order (the cached plan sorts long rows for the gather pulls) */

@@ -123,6 +123,16 @@ hipError_t og_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
 }
 int og_num_cus(void) { return g_cus; }
 
+/* With OGPU_TRACE set (any value), each drop-in entry point names itself,
+ * the device and the graph on stderr: a host program linked against the
+ * library can show which definition it called.  Output only. */
+void og_trace_entry(const char *fn, uint32_t n_vertices, uint32_t n_edges)
+{
+    if (!getenv("OGPU_TRACE")) return;
+    fprintf(stderr, "opengraph_amd: %s on HIP device %d (%d CUs), %u vertices, %u edges\n", fn, g_device,
+            g_cus, n_vertices, n_edges);
+}
+
 void *og_dalloc(size_t bytes)
 {
     void *p = NULL;

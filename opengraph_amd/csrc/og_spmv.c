@@ -4,13 +4,14 @@
  *
  * y = 0; `iterations` times y += A^T (w x) with the fixed x and w of the
  * reference (og_spmv.hip).  The passes are PageRank's kernels with the SpMV
- * store: pull (-p 0, 2, and 3 by default) is the degree-binned gather over
- * the in-edges, push (-p 1, and 3 on request) the scatter with atomics
- * over the forward edges.  fp32 pull rows of at most hub_threshold
- * in-edges continue y in the reference's order (bit-exact); longer rows add
- * their fixed-tree sum to y.  The integer sums of -p 2 and 3 are exact in
- * any order, so -p 3 (a push in the reference) gathers by default: the same
- * bits without atomics.
+ * store.  -p 0 runs on the binned pull by default: every row continues y
+ * in the reference's order (bit-exact); with a hub_threshold it runs on the
+ * degree-binned gather pull, where rows of at most hub_threshold in-edges
+ * continue y in order and longer rows add their fixed-tree sum.  -p 2 (and
+ * 3 by default) gather over the in-edges, -p 1 (and 3 on request) push over
+ * the forward edges, partitioned by destination range.  The integer sums of
+ * -p 2 and 3 are exact in any order, so -p 3 (a push in the reference)
+ * gathers by default: the same bits without atomics.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -300,6 +301,7 @@ static struct SPMVStats *og_spmv_dropin(struct Arguments *arguments, struct Grap
     og_graph *g = NULL;
     og_plan *plan = NULL;
     if (og_cache_graph(graph, code == 1, &g, &plan)) og_spmv_fatal("graph upload");
+    og_trace_entry("SPMVGraphCSR", g->N, g->E);
     ogpu_spmv_options o;
     ogpu_spmv_default_options(&o);
     o.code = code;

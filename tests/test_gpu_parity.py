@@ -261,7 +261,8 @@ def test_push_within_tolerance(golden_ranks, graphs, code):
 
 def test_dropin_entry_point_matches_reference(golden, graphs):
     """pageRankGraphCSR called with the reference's structs (the path the
-    reference's driver takes, graphRun.c:1035-1040)."""
+    reference's driver takes, graphRun.c:1035-1040): identical rank bits
+    (the fp32 pull runs binned, every row in CSR order)."""
     for c in golden["cases"]:
         if c["graph"] != "enron" or c["sort"] != 0 or c["code"] not in (0, 2, 9):
             continue
@@ -270,11 +271,7 @@ def test_dropin_entry_point_matches_reference(golden, graphs):
         host = og.HostGraphCSR(o.a)
         ranks, iters, err, t = og.pagerank_graph_csr(host, c["code"], c["max_iter"], c["epsilon"])
         assert iters == c["iterations"]
-        if c["code"] != 0:
-            assert sha(ranks) == c["ranks_sha256"]
-        else:
-            ref = po.pagerank(o, 0, c["epsilon"], c["max_iter"], 4)["ranks"]
-            assert np.max(np.abs(ranks - ref) / ref) <= 1e-5
+        assert sha(ranks) == c["ranks_sha256"], c
 
 
 def _rows(text):
