@@ -112,7 +112,7 @@ static void *og_bin_upload(const void *h, size_t bytes)
 static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t *rp)
 {
     const og_bin_dev *d = &b->d;
-    const size_t ne = (size_t)b->n_ent, nq = (ne - 64) / OG_BIN_TILE_ALIGN;
+    const size_t ne = (size_t)b->n_ent, nq = (ne - OG_BIN_SPARE) / OG_BIN_TILE_ALIGN;
     og_bin_item *items = (og_bin_item *)malloc((size_t)d->n_items * sizeof(og_bin_item) + 16);
     uint32_t *qent = (uint32_t *)malloc(nq * 4 + 4);
     uint16_t *src = (uint16_t *)malloc(ne * 2), *pos = (uint16_t *)malloc(ne * 2);
@@ -144,7 +144,8 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
         for (uint32_t f = it.f0; f < it.f1; f++)
         {
             const uint64_t e = (uint64_t)qent[f / OG_BIN_TILE_ALIGN] + f % OG_BIN_TILE_ALIGN;
-            if (e >= ne - 64) OG_BIN_FAIL("source %u: entry %llu of %zu", f, (unsigned long long)e, ne - 64);
+            if (e >= ne - OG_BIN_SPARE)
+                OG_BIN_FAIL("source %u: entry %llu of %zu", f, (unsigned long long)e, ne - OG_BIN_SPARE);
             if (src[f] >= nv) OG_BIN_FAIL("source %u: %u past segment size %u", f, src[f], nv);
             if (seen[e]) OG_BIN_FAIL("entry %llu gathered twice", (unsigned long long)e);
             seen[e] = 1;
@@ -156,7 +157,7 @@ static int og_bin_verify(const og_bin_plan *b, const og_plan *pl, const uint32_t
     {
         const og_bin_desc bd = bins[q];
         if (bd.ent0 != prev_end || (bd.ent0 % OG_BIN_TILE_ALIGN) || bd.ent1 < bd.ent0 ||
-            (bd.ent1 - bd.ent0) % OG_BIN_TILE_ALIGN || bd.ent1 - bd.ent0 < bd.cnt || bd.ent1 > ne - 64 ||
+            (bd.ent1 - bd.ent0) % OG_BIN_TILE_ALIGN || bd.ent1 - bd.ent0 < bd.cnt || bd.ent1 > ne - OG_BIN_SPARE ||
             bd.cnt > OG_BIN_CAP)
             OG_BIN_FAIL("bin %u entries [%u, %u) count %u", q, bd.ent0, bd.ent1, bd.cnt);
         prev_end = bd.ent1;
@@ -421,7 +422,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
             }
             tent[t] = (uint32_t)ent;
             ent += (tcnt[t] + OG_BIN_TILE_ALIGN - 1) / OG_BIN_TILE_ALIGN * OG_BIN_TILE_ALIGN;
-            if (ent + 64 > 0xFFFFFFFFull)
+            if (ent + OG_BIN_SPARE > 0xFFFFFFFFull)
             {
                 og_set_error("binned pull: graph too large for 32-bit entries");
                 goto done;
@@ -429,7 +430,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
         }
         if (bprev != 0xFFFFFFFFu) hb[bprev].ent1 = (uint32_t)ent;
         for (uint32_t q = n_edge_bins; q < bins.n; q++) hb[q].ent0 = hb[q].ent1 = (uint32_t)ent;
-        b->n_ent = ent + 64;   /* 64 spare entries past the last tile */
+        b->n_ent = ent + OG_BIN_SPARE;   /* spare entries past the last tile */
         b->d.n_ent = (uint32_t)ent;
         /* the tiles segment by segment (bins ascending), and each tile's
          * first entry in that order: where the gather reads its sources */
@@ -473,7 +474,7 @@ int og_bin_plan_build(const og_plan *pl, og_bin_plan **out)
                                (uint16_t *)b->mem_pos, s) ||
             og_launch_bin_pads((const uint32_t *)d_tent, (const uint32_t *)d_tsrc, (const uint32_t *)d_tcnt, n_tiles,
                                (uint16_t *)b->mem_src, (uint16_t *)b->mem_pos, s) ||
-            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, 64 * 2, s), "memset slack") ||
+            og_hip_check(hipMemsetAsync((uint16_t *)b->mem_src + ent, 0, OG_BIN_SPARE * 2, s), "memset slack") ||
             og_hip_check(hipStreamSynchronize(s), "sync bin layout"))
             goto done;
         og_dfree(key_a); og_dfree(val_a); og_dfree(key_b); og_dfree(val_b); og_dfree(scratch);

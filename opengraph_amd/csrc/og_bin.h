@@ -23,8 +23,9 @@
  *   gather pass  one 1024-thread block per CU: per work item (a segment and
  *                a run of its sources), the segment's contributions are
  *                copied into LDS, then val[qent[f / 16] + f % 16] =
- *                lds[src[f]] for the item's sources, 64 per wavefront and
- *                step, two independent streaming loads per value.
+ *                lds[src[f]] for the item's sources, 4 consecutive ones
+ *                per lane (one 8-byte source load, one quad entry, one
+ *                16-byte value store), 256 per wavefront and step.
  *   sum pass     one 1024-thread block per CU: per bin, val[e] is written
  *                to LDS slot pos[e] (the bin's in-edge values in CSR order),
  *                then every row is added from its slots in order -- the
@@ -64,6 +65,7 @@ extern "C" {
 #define OG_BIN_RPT 1u          /* rows per thread and bin: a bin holds at most RPT * THREADS rows */
 #define OG_BIN_TILE_ALIGN 16u  /* tiles padded to 16 entries: 64-byte value runs */
 #define OG_BIN_ITEM_SRC (2048u * 64u) /* sources per gather work item at most */
+#define OG_BIN_SPARE 256u      /* spare entries past the last tile (the gather's idle stores) */
 #define OG_BIN_THREADS 512u    /* sum and long-row blocks (two per CU) */
 #define OG_BIN_GTHREADS 1024u  /* gather blocks (one per CU: a segment fills LDS) */
 #define OG_BIN_LTHREADS 1024u  /* long-row blocks (a whole CU each: two sum-block slots) */
@@ -91,7 +93,7 @@ typedef struct og_bin_dev
 {
     uint32_t N, n_seg, n_items, n_bins, n_bins_live, n_long;
     uint32_t first_bin;          /* bins [0, first_bin) are the long rows' pieces */
-    uint32_t n_ent;              /* entries in tiles; 64 spare ones follow */
+    uint32_t n_ent;              /* entries in tiles; OG_BIN_SPARE spare ones follow */
     const og_bin_item *items;
     const uint32_t *qent;        /* n_ent / 16 */
     const uint16_t *src, *pos;

@@ -29,7 +29,7 @@
 #define LT OG_BIN_LTHREADS       /* the long-row pass's blocks */
 #define LW (OG_BIN_LTHREADS / 64)
 #define GW (OG_BIN_GTHREADS / 64)
-#define GATHER_U 16u  /* values per lane in flight in the gather */
+#define GATHER_Q 8u   /* runs of 4 consecutive sources per lane in flight in the gather */
 #define OG_BIN_HEAD 128u  /* leading terms of a long sum added one by one */
 
 /* ------------------------------------------------------------------ */
@@ -238,17 +238,21 @@ __device__ float og_bin_wave_sum(const float *slot, uint32_t j0, uint32_t n, flo
 /* gather pass                                                         */
 /* ------------------------------------------------------------------ */
 
-/* The sources and quad entries of this lane's GATHER_U values from f (64
- * apart), branch-free (indices clamped to the item's last source) so that
- * all the loads issue before any is waited for. */
+typedef uint32_t og_u32x2 __attribute__((ext_vector_type(2)));
+typedef float og_f32x4g __attribute__((ext_vector_type(4)));
+
+/* The sources (4 u16 in one 8-byte load) and quad entry of this lane's
+ * GATHER_Q runs of 4 sources from f (256 apart; f a multiple of 4, so a run
+ * lies in one quad), branch-free (indices clamped to the item's last run)
+ * so that all the loads issue before any is waited for. */
 __device__ __forceinline__ void og_bin_gather_batch(const og_bin_dev &b, uint32_t f, uint32_t f1,
-                                                    uint32_t (&sv)[GATHER_U], uint32_t (&qv)[GATHER_U])
+                                                    og_u32x2 (&sv)[GATHER_Q], uint32_t (&qv)[GATHER_Q])
 {
 #pragma unroll
-    for (uint32_t u = 0; u < GATHER_U; u++)
+    for (uint32_t u = 0; u < GATHER_Q; u++)
     {
-        const uint32_t fu = min(f + 64u * u, f1 - 1);
-        sv[u] = __builtin_nontemporal_load(&b.src[fu]);
+        const uint32_t fu = min(f + 256u * u, f1 - 4);
+        sv[u] = __builtin_nontemporal_load((const og_u32x2 *)(b.src + fu));
         qv[u] = __builtin_nontemporal_load(&b.qent[fu / OG_BIN_TILE_ALIGN]);
     }
 }
@@ -274,32 +278,44 @@ __global__ __launch_bounds__(GT) void og_bin_gather_kernel(og_bin_dev b, const f
             for (uint32_t i = (nv & ~3u) + threadIdx.x; i < nv; i += GT) seg[i] = c[v0 + i];
         }
         __syncthreads();
-        /* wavefront w takes values f = f0 + 64 * (w * U + k * GW * U + u) +
-         * lane; software pipelined: the next batch's loads are issued
-         * before this batch's stores (loads complete in order, and a wait
-         * for this batch's loads must not also wait for stores issued after
-         * them) */
-        uint32_t f = it.f0 + 64u * GATHER_U * wave + lane;
-        uint32_t sv[GATHER_U], qv[GATHER_U];
+        /* wavefront w takes the runs of 4 sources f = f0 + 256 * (w * Q +
+         * k * GW * Q + u) + 4 * lane (f0 and f1 are multiples of 16, so a
+         * run lies in one quad and is wholly inside or past the item); each
+         * run's 4 values go to 4 consecutive entries, one 16-byte store
+         * (a quad's entry is a multiple of 16).  Software pipelined: the
+         * next batch's loads are issued before this batch's stores (loads
+         * complete in order, and a wait for this batch's loads must not
+         * also wait for stores issued after them) */
+        uint32_t f = it.f0 + 256u * GATHER_Q * wave + 4u * lane;
+        og_u32x2 sv[GATHER_Q];
+        uint32_t qv[GATHER_Q];
         og_bin_gather_batch(b, f, it.f1, sv, qv);
-        for (; f < it.f1; f += 64u * GATHER_U * GW)
+        for (; f < it.f1; f += 256u * GATHER_Q * GW)
         {
-            uint32_t sv2[GATHER_U], qv2[GATHER_U];
-            og_bin_gather_batch(b, f + 64u * GATHER_U * GW, it.f1, sv2, qv2);
-            float x[GATHER_U];   /* every lane's source is a valid one (clamped) */
+            og_u32x2 sv2[GATHER_Q];
+            uint32_t qv2[GATHER_Q];
+            og_bin_gather_batch(b, f + 256u * GATHER_Q * GW, it.f1, sv2, qv2);
+            og_f32x4g x[GATHER_Q];   /* every lane's sources are valid ones (clamped) */
 #pragma unroll
-            for (uint32_t u = 0; u < GATHER_U; u++) x[u] = seg[sv[u]];
-            /* lanes past the item's end write the spare entries past the
-             * last tile instead: unconditional stores keep the count of
-             * memory operations in flight static */
-#pragma unroll
-            for (uint32_t u = 0; u < GATHER_U; u++)
+            for (uint32_t u = 0; u < GATHER_Q; u++)
             {
-                const uint32_t e = f + 64u * u < it.f1 ? qv[u] + (f + 64u * u) % OG_BIN_TILE_ALIGN : b.n_ent + lane;
-                __builtin_nontemporal_store(x[u], &b.val[e]);
+                x[u].x = seg[sv[u].x & 0xFFFFu];
+                x[u].y = seg[sv[u].x >> 16];
+                x[u].z = seg[sv[u].y & 0xFFFFu];
+                x[u].w = seg[sv[u].y >> 16];
+            }
+            /* lanes past the item's end write the OG_BIN_SPARE spare
+             * entries past the last tile instead: unconditional stores keep
+             * the count of memory operations in flight static */
+#pragma unroll
+            for (uint32_t u = 0; u < GATHER_Q; u++)
+            {
+                const uint32_t fu = f + 256u * u;
+                const uint32_t e = fu < it.f1 ? qv[u] + fu % OG_BIN_TILE_ALIGN : b.n_ent + 4u * lane;
+                __builtin_nontemporal_store(x[u], (og_f32x4g *)(b.val + e));
             }
 #pragma unroll
-            for (uint32_t u = 0; u < GATHER_U; u++)
+            for (uint32_t u = 0; u < GATHER_Q; u++)
             {
                 sv[u] = sv2[u];
                 qv[u] = qv2[u];
